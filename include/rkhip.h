@@ -1,0 +1,161 @@
+/*
+ * rkhip.h -- C ABI of librkhip.so, the MI355X (gfx950) batched Levenberg-Marquardt library.
+ *
+ * Plain C types only (pointers + sizes); host arrays are float32 / int32, row-major, and are
+ * copied during the call (borrowed only for its duration).  Every call returns an int status
+ * (RK_OK = 0); rk_last_error() gives the calling thread's last message.  Calls on one context
+ * are serialised by an internal mutex; contexts are independent.  Each context owns one HIP
+ * stream (default priority) on its device.
+ *
+ * The Python package robokit_amd binds these with ctypes; the entry points replace the
+ * reference's Warp launches on the LM path (citations are files of the reference repository).
+ */
+#ifndef RKHIP_H
+#define RKHIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RK_OK 0
+#define RK_EINVAL 1     /* bad argument / shape mismatch  (reference raises ValueError) */
+#define RK_EHIP 2       /* HIP runtime error */
+#define RK_ECAPACITY 3  /* batch larger than capacity (optimizer.py:321-322 ValueError) */
+#define RK_ENOMEM 4     /* device memory budget exceeded */
+#define RK_EBUSY 5      /* no free serving slot / request not finished */
+
+typedef struct rk_context rk_context;
+typedef struct rk_batch rk_batch;
+typedef struct rk_group rk_group;
+typedef struct rk_server rk_server;
+
+/* thread-local message of the last failing call */
+const char* rk_last_error(void);
+int32_t rk_abi_version(void);
+
+/* one context per device ordinal; owns a stream and the uploaded robots */
+int rk_context_create(int32_t device, rk_context** out);
+int rk_context_destroy(rk_context* ctx);
+int rk_context_sync(rk_context* ctx);
+/* bytes of device memory currently allocated by the context (batches, groups, servers, robots) */
+int rk_context_mem(rk_context* ctx, int64_t* bytes);
+
+/*
+ * Robot topology (robo/robot_spec.py:45-138 fields).  Copied to a resident device model.
+ * joint_type FIXED 0 / PRISMATIC 1 / REVOLUTE 2; parent_T [nj][7] xyz-wxyz;
+ * limits [na][2] float32-narrowed (robo/robot_spec_tensors.py:17-31).
+ * Limits: nj <= 64, nl <= 64, na <= 32.
+ */
+typedef struct {
+    int32_t nj, nl, na;
+    const int32_t* joint_type;
+    const float* joint_axis;
+    const float* parent_T;
+    const int32_t* parent_joint;
+    const int32_t* act_idx;
+    const int32_t* mimic_idx;
+    const float* mimic_mul;
+    const float* mimic_off;
+    const int32_t* link_parent_joint;
+    const float* limits;
+} rk_robot_desc;
+
+/* replaces Robot.load -> RobotSpec.get_tensors (robo/robot_spec.py:1219-1250): resident device copy */
+int rk_robot_upload(rk_context* ctx, const rk_robot_desc* desc, int32_t* robot_id);
+
+/* replaces Robot.forward_kinematics (robo/robot.py:131-271): q [B][na] -> T_link [B][nl][7] */
+int rk_fk(rk_context* ctx, int32_t robot, int32_t B, const float* q, float* T_link);
+
+/* replaces RobotState.get_link_jacobian (robo/robot_state.py:192-224 -> robot_kernels.py:667-737):
+ * frame 0 spatial / 1 body; J [B][6][na] */
+int rk_link_jacobian(rk_context* ctx, int32_t robot, int32_t B, const float* q, int32_t link, int32_t frame,
+                     float* J);
+
+/* device SE(3) ops for tests (lie/se3.py:38-290 host API): op 0 compose(a,b) 1 inverse(a) 2 log(a)
+ * 3 jlog(a) [36] 4 adjoint(a) [36] 5 apply(a, b[3]) ; a,b [n][7]; out [n][w] */
+int rk_se3_op(rk_context* ctx, int32_t op, int32_t n, const float* a, const float* b, float* out);
+
+/*
+ * Residual-term structure of a batch (uniform over its problems), in this row order:
+ *   FrameTask log mode   (terms/dense/frame_task.py:505-621)      6 rows per pose target
+ *   PositionTask         (terms/dense/position_task.py:188-272)   3 rows per position target
+ *   PositionLimit "abs"  (terms/dense/position_limit.py:508-573)  na rows
+ *   SmoothnessTask       (terms/dense/smoothness_task.py:350-422) na rows
+ * Limits: n_pose <= 4, n_pos <= 12.
+ */
+typedef struct {
+    int32_t robot;
+    int32_t n_pose;
+    const int32_t* pose_links;
+    const float* pose_weights; /* [n_pose][6] */
+    int32_t n_pos;
+    const int32_t* pos_links;
+    const float* pos_weights;  /* [n_pos][3] */
+    int32_t limit;
+    const float* limit_weights; /* [na] */
+    int32_t smooth;
+    const float* smooth_weights; /* [na] */
+} rk_terms_desc;
+
+/* LMOptimizerConfig fields (opt/lm_optimizer.py:502-523) */
+typedef struct {
+    float lm_lambda, lambda_factor, lambda_min, lambda_max, rho_min, gain_ratio_epsilon;
+} rk_lm_params;
+
+/*
+ * A batch: device-resident structure-of-arrays buffers for up to `capacity` problems of one
+ * term structure (LMOptimizer._build, opt/lm_optimizer.py:544-635: buffers sized once, smaller
+ * batches are prefixes, larger ones fail with RK_ECAPACITY).  trace_iters > 0 keeps a per-
+ * iteration cost / accept trace of that many iterations.
+ */
+int rk_batch_create(rk_context* ctx, const rk_terms_desc* terms, int32_t capacity, int32_t trace_iters,
+                    rk_batch** out);
+int rk_batch_destroy(rk_batch* b);
+/* host arrays: q [B][na], pose_targets [B][n_pose][7], pos_targets [B][n_pos][3], q_prev [B][na]
+ * (NULL = leave unchanged); sets the live size to B and resets damping to `lm_lambda` at solve */
+int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targets, const float* pos_targets,
+                 const float* q_prev);
+
+#define RK_MODE_FUSED 0 /* one launch: whole solve per problem in LDS/registers */
+#define RK_MODE_PLAIN 1 /* reference decomposition: per-term kernels, J in HBM, host loop */
+#define RK_FROM_SEED 0x100 /* flag: start from the seeds of the last rk_batch_set (kept on the device) */
+#define RK_USE_GRAPH 0x1   /* rk_group_solve flag: capture the launch sequence in a hipGraph and replay it */
+
+/*
+ * Optimizer.solve with a fixed iteration count (opt/optimizer.py:216-240, LM step
+ * lm_optimizer.py:834-1005).  threads_per_problem (fused mode): 0 = library default, else one
+ * of 4, 8, 16 (robots with nj <= 16, na <= 8, n_pose <= 2, n_pos <= 2) or 16, 32, 64.
+ * kernel_ms (nullable) = HIP-event time of the solve's kernels on the context stream.
+ */
+int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mode, int32_t threads_per_problem,
+                   float* kernel_ms);
+/* outputs (nullable): q [B][na], cost [B], lambda [B], cost_trace [B][trace_iters+1], accept_trace [B][trace_iters] */
+int rk_batch_get(rk_batch* b, float* q, float* cost, float* lambda, float* cost_trace, int32_t* accept_trace);
+/* iterates of the last traced solve: q_trace [B][trace_iters][na] = q before step i,
+ * lam_trace [B][trace_iters] = damping step i used (the state an iteration_callback sees,
+ * opt/optimizer.py:230-236) */
+int rk_batch_get_iterates(rk_batch* b, float* q_trace, float* lam_trace);
+/* _ik_metrics_kernel (helpers/ik/ik.py:59-82) over the batch's pose targets at its current q */
+int rk_batch_metrics(rk_batch* b, float pos_thr, float ang_thr, float* pos_err, float* ang_err, int32_t* success);
+
+/*
+ * Uneven batch: several batches (different robots / sizes / terms) solved by ONE persistent
+ * kernel that pulls problem tiles from a device work queue; each batch runs with its own
+ * threads-per-problem.  No reference counterpart (the reference batches one robot per optimizer).
+ */
+int rk_group_create(rk_context* ctx, int32_t nbatches, rk_batch* const* batches, rk_group** out);
+int rk_group_destroy(rk_group* g);
+/* tpp [nbatches] (0 = default), workgroups per CU of the persistent grid (0 = occupancy limit),
+ * iterations per launch (0 = all in one launch) */
+int rk_group_set_tuning(rk_group* g, const int32_t* tpp, int32_t wg_per_cu, int32_t iters_per_launch);
+/* time every legal threads-per-problem for each batch on a short run and keep the fastest;
+ * chosen_tpp [nbatches] (nullable) receives the choice */
+int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t* chosen_tpp);
+/* solve all batches; flags: RK_USE_GRAPH (capture once per shape, then replay), RK_FROM_SEED */
+int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t flags, float* kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RKHIP_H */

@@ -1,0 +1,275 @@
+"""ctypes wrapper over the CPU oracle (liboracle_f32.so / liboracle_f64.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, as the checker.  The product package (robokit_amd) never imports it.
+
+The C source (rk_oracle.c) restates the reference's Warp-CPU LM path; see its header for
+the file:line each function follows.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+from pathlib import Path
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_LIBS = {}
+
+FRAME, POSITION, LIMIT, SMOOTH = 0, 1, 2, 3
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def _lib(dtype: str):
+    if dtype not in _LIBS:
+        path = HERE / f"liboracle_{dtype}.so"
+        if not path.exists():
+            build()
+        _LIBS[dtype] = C.CDLL(str(path))
+    return _LIBS[dtype]
+
+
+class _Robot(C.Structure):
+    _fields_ = [
+        ("nj", C.c_int), ("nl", C.c_int), ("na", C.c_int),
+        ("joint_type", C.c_void_p), ("joint_axis", C.c_void_p), ("parent_T", C.c_void_p),
+        ("parent_joint", C.c_void_p), ("act_idx", C.c_void_p), ("mimic_idx", C.c_void_p),
+        ("mimic_mul", C.c_void_p), ("mimic_off", C.c_void_p), ("link_parent_joint", C.c_void_p),
+        ("limits", C.c_void_p),
+    ]
+
+
+class _Term(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("nframes", C.c_int), ("links", C.c_void_p), ("targets", C.c_void_p),
+        ("weights", C.c_void_p), ("q_prev", C.c_void_p),
+    ]
+
+
+class _LMParams(C.Structure):
+    _fields_ = [(n, C.c_float) for n in
+                ("lm_lambda", "lambda_factor", "lambda_min", "lambda_max", "rho_min", "gain_ratio_epsilon")]
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class OracleRobot:
+    """Holds the float32/int32 arrays the C struct points at (keeps them alive)."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self.arrays = dict(
+            joint_type=np.ascontiguousarray(spec.joint_types, np.int32),
+            joint_axis=np.ascontiguousarray(spec.joint_axes, np.float32),
+            parent_T=np.ascontiguousarray(spec.parent_joint_transforms, np.float32),
+            parent_joint=np.ascontiguousarray(spec.parent_joint_indices, np.int32),
+            act_idx=np.ascontiguousarray(spec.actuated_joint_indices, np.int32),
+            mimic_idx=np.ascontiguousarray(spec.mimic_actuated_joint_indices, np.int32),
+            mimic_mul=np.ascontiguousarray(spec.mimic_multipliers, np.float32),
+            mimic_off=np.ascontiguousarray(spec.mimic_offsets, np.float32),
+            link_parent_joint=np.ascontiguousarray(spec.link_parent_joint_indices, np.int32),
+            limits=np.ascontiguousarray(spec.actuated_joint_limits_f32, np.float32),
+        )
+        self.struct = _Robot(spec.num_joints, spec.num_links, spec.num_actuated_joints,
+                             *[_ptr(self.arrays[k]) for k in ("joint_type", "joint_axis", "parent_T", "parent_joint",
+                                                              "act_idx", "mimic_idx", "mimic_mul", "mimic_off",
+                                                              "link_parent_joint", "limits")])
+
+
+@dataclass
+class Term:
+    kind: int
+    links: Sequence[int] = ()
+    targets: Optional[np.ndarray] = None  # [B, k, 7]
+    weights: Optional[np.ndarray] = None
+    q_prev: Optional[np.ndarray] = None  # [B, na]
+
+
+def _terms(terms: Sequence[Term], keep: list):
+    arr = (_Term * max(1, len(terms)))()
+    for i, t in enumerate(terms):
+        links = np.ascontiguousarray(t.links, np.int32)
+        tg = None if t.targets is None else np.ascontiguousarray(t.targets, np.float32)
+        w = np.ascontiguousarray(t.weights, np.float32)
+        qp = None if t.q_prev is None else np.ascontiguousarray(t.q_prev, np.float32)
+        keep.extend([links, tg, w, qp])
+        arr[i] = _Term(t.kind, len(links), _ptr(links), _ptr(tg), _ptr(w), _ptr(qp))
+    return arr
+
+
+def _real(dtype: str):
+    return np.float32 if dtype == "f32" else np.float64
+
+
+def fk(spec, q: np.ndarray, base: Optional[np.ndarray] = None, dtype: str = "f32"):
+    lib, rt = _lib(dtype), _real(dtype)
+    rb = OracleRobot(spec)
+    q = np.ascontiguousarray(np.atleast_2d(q), rt)
+    B = q.shape[0]
+    bs = None if base is None else np.ascontiguousarray(np.broadcast_to(base, (B, 7)), rt)
+    Tl = np.zeros((B, spec.num_links, 7), rt)
+    Tj = np.zeros((B, spec.num_joints, 7), rt)
+    lib.ork_fk(C.byref(rb.struct), B, _ptr(q), _ptr(bs), _ptr(Tl), _ptr(Tj))
+    return Tl, Tj
+
+
+def link_jacobian(spec, q: np.ndarray, link: int, frame: str, base: Optional[np.ndarray] = None, dtype: str = "f32"):
+    lib, rt = _lib(dtype), _real(dtype)
+    rb = OracleRobot(spec)
+    q = np.ascontiguousarray(np.atleast_2d(q), rt)
+    B = q.shape[0]
+    bs = None if base is None else np.ascontiguousarray(np.broadcast_to(base, (B, 7)), rt)
+    cols = spec.num_actuated_joints + (6 if base is not None else 0)
+    J = np.zeros((B, 6, cols), rt)
+    lib.ork_link_jacobian(C.byref(rb.struct), B, _ptr(q), _ptr(bs), int(link), 0 if frame == "spatial" else 1, _ptr(J))
+    return J
+
+
+def se3(op: str, *args, eps: float = 1e-4, dtype: str = "f32"):
+    lib, rt = _lib(dtype), _real(dtype)
+    a = [np.ascontiguousarray(np.atleast_2d(x), rt) for x in args]
+    n = a[0].shape[0]
+    out_w = {"compose": 7, "inverse": 7, "apply": 3, "exp": 7, "log": 6, "adjoint": 36, "jlog": 36}[op]
+    out = np.zeros((n, out_w), rt)
+    fn = getattr(lib, f"ork_se3_{op}")
+    if op in ("exp", "log", "jlog"):
+        cast = C.c_float if dtype == "f32" else C.c_double
+        fn(n, _ptr(a[0]), cast(eps), _ptr(out))
+    else:
+        fn(n, *[_ptr(x) for x in a], _ptr(out))
+    return out.reshape((n, 6, 6)) if out_w == 36 else out
+
+
+def residual_jacobian(spec, terms: Sequence[Term], b: int, q: np.ndarray, dtype: str = "f32"):
+    """Stacked residual r[R], Jacobian J[R, na], H = sum J^T J, g = sum J^T r, cost at q for problem b."""
+    lib, rt = _lib(dtype), _real(dtype)
+    rb = OracleRobot(spec)
+    keep: list = []
+    ta = _terms(terms, keep)
+    na = spec.num_actuated_joints
+    R = sum(6 * len(t.links) if t.kind == FRAME else 3 * len(t.links) if t.kind == POSITION else na for t in terms)
+    q = np.ascontiguousarray(q, rt)
+    r = np.zeros(R, rt)
+    J = np.zeros((R, na), rt)
+    H = np.zeros((na, na), rt)
+    g = np.zeros(na, rt)
+    cost = (C.c_float if dtype == "f32" else C.c_double)()
+    lib.ork_residual_jacobian(C.byref(rb.struct), len(terms), ta, int(b), _ptr(q), None, _ptr(r), _ptr(J), _ptr(H),
+                              _ptr(g), C.byref(cost))
+    return r, J, H, g, float(cost.value)
+
+
+def damped_solve(H: np.ndarray, g: np.ndarray, lam: float, dtype: str = "f32"):
+    lib, rt = _lib(dtype), _real(dtype)
+    D = H.shape[0]
+    H = np.ascontiguousarray(H, rt)
+    g = np.ascontiguousarray(g, rt)
+    delta = np.zeros(D, rt)
+    pred = (C.c_float if dtype == "f32" else C.c_double)()
+    cast = C.c_float if dtype == "f32" else C.c_double
+    ok = lib.ork_damped_solve(D, _ptr(H), _ptr(g), cast(lam), _ptr(delta), C.byref(pred))
+    return delta, float(pred.value), bool(ok)
+
+
+@dataclass
+class LMParams:
+    """LMOptimizerConfig fields (opt/lm_optimizer.py:502-523)."""
+
+    lm_lambda: float = 1e-3
+    lambda_factor: float = 2.0
+    lambda_min: float = 1e-5
+    lambda_max: float = 1e10
+    rho_min: float = 1e-3
+    gain_ratio_epsilon: float = 1e-8
+
+    def struct(self):
+        return _LMParams(self.lm_lambda, self.lambda_factor, self.lambda_min, self.lambda_max, self.rho_min,
+                         self.gain_ratio_epsilon)
+
+
+def accept(cost, cost_prop, pred, lam, params: LMParams, dtype: str = "f32"):
+    lib, rt = _lib(dtype), _real(dtype)
+    cost = np.array(cost, rt, ndmin=1)
+    cp = np.array(cost_prop, rt, ndmin=1)
+    pr = np.array(pred, rt, ndmin=1)
+    lam = np.array(lam, rt, ndmin=1)
+    acc = np.zeros(cost.shape[0], np.int32)
+    p = params.struct()
+    lib.ork_accept(cost.shape[0], _ptr(cost), _ptr(cp), _ptr(pr), C.byref(p), _ptr(lam), _ptr(acc))
+    return cost, lam, acc
+
+
+def lm_solve(spec, terms: Sequence[Term], q_seed: np.ndarray, iters: int, params: LMParams, dtype: str = "f32",
+             traces: bool = True):
+    """Fixed-iteration LM over a batch; returns dict(q, cost, cost_trace[B, iters+1], accept[B, iters], lam[B, iters])."""
+    lib, rt = _lib(dtype), _real(dtype)
+    rb = OracleRobot(spec)
+    keep: list = []
+    ta = _terms(terms, keep)
+    q = np.ascontiguousarray(q_seed, rt).copy()
+    B = q.shape[0]
+    cost = np.zeros(B, rt)
+    ct = np.zeros((B, iters + 1), rt) if traces else None
+    at = np.zeros((B, iters), np.int32) if traces else None
+    lt = np.zeros((B, iters), rt) if traces else None
+    p = params.struct()
+    lib.ork_lm_solve(C.byref(rb.struct), len(terms), ta, C.byref(p), B, int(iters), _ptr(q), None, _ptr(cost), _ptr(ct),
+                     _ptr(at), _ptr(lt))
+    return dict(q=q, cost=cost, cost_trace=ct, accept=at, lam=lt)
+
+
+def lm_step(spec, terms: Sequence[Term], q: np.ndarray, lam: np.ndarray, params: LMParams, dtype: str = "f32",
+            rows: Optional[np.ndarray] = None):
+    """One LM step per problem from iterate q[B, na] with damping lam[B].  ``rows`` maps each
+    entry to the problem index its targets belong to (default: entry i is problem i)."""
+    lib, rt = _lib(dtype), _real(dtype)
+    rb = OracleRobot(spec)
+    q = np.ascontiguousarray(q, rt)
+    B = q.shape[0]
+    if rows is not None:
+        terms = [Term(t.kind, t.links, None if t.targets is None else np.asarray(t.targets)[rows], t.weights,
+                      None if t.q_prev is None else np.asarray(t.q_prev)[rows]) for t in terms]
+    keep: list = []
+    ta = _terms(terms, keep)
+    lam = np.ascontiguousarray(lam, rt)
+    out = {k: np.zeros(B, rt) for k in ("cost", "cost_prop", "pred", "lam_new")}
+    acc = np.zeros(B, np.int32)
+    qn = np.zeros_like(q)
+    p = params.struct()
+    lib.ork_lm_step(C.byref(rb.struct), len(terms), ta, C.byref(p), B, _ptr(q), _ptr(lam), _ptr(out["cost"]),
+                    _ptr(out["cost_prop"]), _ptr(out["pred"]), _ptr(acc), _ptr(qn), _ptr(out["lam_new"]))
+    out.update(acc=acc, q_new=qn)
+    return out
+
+
+def ik_metrics(spec, q: np.ndarray, links: Sequence[int], targets: np.ndarray, pos_thr: float = 1e-3,
+               ang_thr: float = 1e-2, dtype: str = "f32"):
+    lib, rt = _lib(dtype), _real(dtype)
+    rb = OracleRobot(spec)
+    q = np.ascontiguousarray(q, rt)
+    B = q.shape[0]
+    links_a = np.ascontiguousarray(links, np.int32)
+    tg = np.ascontiguousarray(targets, np.float32)
+    pe = np.zeros((B, len(links)), rt)
+    ae = np.zeros((B, len(links)), rt)
+    ok = np.zeros(B, np.int32)
+    lib.ork_ik_metrics(C.byref(rb.struct), B, _ptr(q), None, len(links), _ptr(links_a), _ptr(tg), C.c_float(pos_thr),
+                       C.c_float(ang_thr), _ptr(pe), _ptr(ae), _ptr(ok))
+    return pe, ae, ok.astype(bool)
+
+
+def ik_terms(links: Sequence[int], targets: np.ndarray, na: int, limit_weight: float = 1.0,
+             pos_w: float = 1.0, ori_w: float = 1.0) -> List[Term]:
+    """FrameTask(log, w_pos, w_ori) over all target links + PositionLimit(abs, weight)."""
+    k = len(links)
+    w = np.tile(np.array([pos_w] * 3 + [ori_w] * 3, np.float32), k)
+    return [Term(FRAME, links, targets, w), Term(LIMIT, (), None, np.full(na, limit_weight, np.float32))]

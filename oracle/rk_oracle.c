@@ -1,0 +1,904 @@
+/*
+ * rk_oracle.c -- TEST INFRASTRUCTURE ONLY (the CPU oracle / checker).
+ *
+ * A plain-C restatement of RoboKit's Warp-CPU Levenberg-Marquardt path, used by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg to check and
+ * time the HIP product path.  Nothing in robokit_amd/ may link or call this.
+ *
+ * Built twice from this one source (see oracle/Makefile):
+ *   liboracle_f32.so  REAL=float   -- follows the reference's FP32 arithmetic
+ *   liboracle_f64.so  REAL=double  -- the FP64 twin, measures intrinsic sensitivity
+ *
+ * Every function cites the reference file:line whose algorithm it restates
+ * (paths relative to the reference repository root).  Operation order follows
+ * those formulas; Warp's own tile_matmul / tile_cholesky rounding order is not
+ * available (Warp is absent), so sums here run in ascending index order.
+ *
+ * Layout conventions (same as the reference): pose = [x,y,z,qw,qx,qy,qz],
+ * twist = [v; w], joint types FIXED 0 / PRISMATIC 1 / REVOLUTE 2.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef REAL
+#define REAL float
+#endif
+typedef REAL real;
+
+#if REAL_IS_DOUBLE
+#define RSIN sin
+#define RCOS cos
+#define RSQRT sqrt
+#define RATAN2 atan2
+#define RASIN asin
+#define RFABS fabs
+#else
+#define RSIN sinf
+#define RCOS cosf
+#define RSQRT sqrtf
+#define RATAN2 atan2f
+#define RASIN asinf
+#define RFABS fabsf
+#endif
+
+#define EXPORT __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------- */
+/* quaternion helpers: xform/warp/rotation_conversions.py                     */
+/* ------------------------------------------------------------------------- */
+
+/* quaternion_multiply_func + standardize_quaternion_func, rotation_conversions.py:259-279, 249-256 */
+static void quat_mul(const real *a, const real *b, real *out) {
+    real w1 = a[0], x1 = a[1], y1 = a[2], z1 = a[3];
+    real w2 = b[0], x2 = b[1], y2 = b[2], z2 = b[3];
+    real w = w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2;
+    real x = w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2;
+    real y = w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2;
+    real z = w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2;
+    if (w < (real)0) { w = -w; x = -x; y = -y; z = -z; }
+    out[0] = w; out[1] = x; out[2] = y; out[3] = z;
+}
+
+static void cross3(const real *a, const real *b, real *o) {
+    real x = a[1] * b[2] - a[2] * b[1];
+    real y = a[2] * b[0] - a[0] * b[2];
+    real z = a[0] * b[1] - a[1] * b[0];
+    o[0] = x; o[1] = y; o[2] = z;
+}
+
+/* quaternion_apply_func, rotation_conversions.py:193-200: v + 2(w (u x v) + u x (u x v)) */
+static void quat_apply(const real *q, const real *v, real *out) {
+    real w = q[0];
+    real u[3] = {q[1], q[2], q[3]};
+    real uv[3], uuv[3];
+    cross3(u, v, uv);
+    cross3(u, uv, uuv);
+    for (int i = 0; i < 3; ++i) out[i] = v[i] + (real)2 * (w * uv[i] + uuv[i]);
+}
+
+/* quaternion_to_matrix_func, rotation_conversions.py:174-187 (row-major 3x3) */
+static void quat_to_mat(const real *q, real *R) {
+    real w = q[0], x = q[1], y = q[2], z = q[3];
+    real two_s = (real)2 / (w * w + x * x + y * y + z * z);
+    R[0] = (real)1 - two_s * (y * y + z * z);
+    R[1] = two_s * (x * y - z * w);
+    R[2] = two_s * (x * z + y * w);
+    R[3] = two_s * (x * y + z * w);
+    R[4] = (real)1 - two_s * (x * x + z * z);
+    R[5] = two_s * (y * z - x * w);
+    R[6] = two_s * (x * z - y * w);
+    R[7] = two_s * (y * z + x * w);
+    R[8] = (real)1 - two_s * (x * x + y * y);
+}
+
+/* quaternion_to_axis_angle_func, rotation_conversions.py:226-246 */
+static void quat_to_axis_angle(const real *q, real *aa) {
+    real w = q[0];
+    real norm = RSQRT(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    real half_angle = RATAN2(norm, w);
+    real angle = (real)2 * half_angle;
+    real s;
+    if (RFABS(angle) < (real)1e-6) {
+        s = (real)0.5 - (angle * angle) / (real)48;
+    } else {
+        s = RSIN(half_angle) / angle;
+    }
+    aa[0] = q[1] / s; aa[1] = q[2] / s; aa[2] = q[3] / s;
+}
+
+/* ------------------------------------------------------------------------- */
+/* SE(3): lie/se3_kernels.py, lie/so3_kernels.py                             */
+/* ------------------------------------------------------------------------- */
+
+/* se3_compose_func, se3_kernels.py:87-112 */
+static void se3_compose(const real *a, const real *b, real *out) {
+    real q[4], t[3];
+    quat_mul(a + 3, b + 3, q);
+    quat_apply(a + 3, b, t);
+    out[0] = a[0] + t[0]; out[1] = a[1] + t[1]; out[2] = a[2] + t[2];
+    out[3] = q[0]; out[4] = q[1]; out[5] = q[2]; out[6] = q[3];
+}
+
+/* se3_inverse_func, se3_kernels.py:69-78 */
+static void se3_inverse(const real *a, real *out) {
+    real qi[4] = {a[3], -a[4], -a[5], -a[6]};
+    real t[3];
+    quat_apply(qi, a, t);
+    out[0] = -t[0]; out[1] = -t[1]; out[2] = -t[2];
+    out[3] = qi[0]; out[4] = qi[1]; out[5] = qi[2]; out[6] = qi[3];
+}
+
+static void hat(const real *v, real *K) {
+    K[0] = 0; K[1] = -v[2]; K[2] = v[1];
+    K[3] = v[2]; K[4] = 0; K[5] = -v[0];
+    K[6] = -v[1]; K[7] = v[0]; K[8] = 0;
+}
+
+static void mat3_mul(const real *A, const real *B, real *C) {
+    real T[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            real s = A[i * 3 + 0] * B[0 * 3 + j];
+            s += A[i * 3 + 1] * B[1 * 3 + j];
+            s += A[i * 3 + 2] * B[2 * 3 + j];
+            T[i * 3 + j] = s;
+        }
+    memcpy(C, T, sizeof(T));
+}
+
+static void mat3_vec(const real *A, const real *v, real *o) {
+    real t[3];
+    for (int i = 0; i < 3; ++i) t[i] = A[i * 3 + 0] * v[0] + A[i * 3 + 1] * v[1] + A[i * 3 + 2] * v[2];
+    o[0] = t[0]; o[1] = t[1]; o[2] = t[2];
+}
+
+/* se3_exp_map_func, se3_kernels.py:125-167 (eps-regularised) */
+static void se3_exp(const real *xi, real eps, real *out) {
+    const real *w = xi + 3;
+    real angle_sq = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    real angle_sq_reg = angle_sq + eps * eps;
+    real angle_reg = RSQRT(angle_sq_reg);
+    real half = angle_reg * (real)0.5;
+    real sh = RSIN(half), ch = RCOS(half);
+    real k = sh / angle_reg;
+    real q[4] = {ch, k * w[0], k * w[1], k * w[2]};
+    real qn = RSQRT(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int i = 0; i < 4; ++i) q[i] = q[i] / qn;
+    real sa = RSIN(angle_reg), ca = RCOS(angle_reg);
+    real A = sa / angle_reg;
+    real c1 = ((real)1 - ca) / angle_sq_reg;
+    real c2 = ((real)1 - A) / angle_sq_reg;
+    real K[9], K2[9], V[9];
+    hat(w, K);
+    mat3_mul(K, K, K2);
+    for (int i = 0; i < 9; ++i) V[i] = ((i % 4) == 0 ? (real)1 : (real)0) + c1 * K[i] + c2 * K2[i];
+    real t[3];
+    mat3_vec(V, xi, t);
+    out[0] = t[0]; out[1] = t[1]; out[2] = t[2];
+    out[3] = q[0]; out[4] = q[1]; out[5] = q[2]; out[6] = q[3];
+}
+
+/* se3_V_matrix_inverse_func, se3_kernels.py:230-282 */
+static void se3_V_inv(const real *w, real eps, real *Vi) {
+    real angle = RSQRT(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    real K[9], K2[9];
+    hat(w, K);
+    mat3_mul(K, K, K2);
+    real c2;
+    if (angle < eps) {
+        c2 = (real)1 / (real)12;
+    } else {
+        real sa = RSIN(angle), ca = RCOS(angle);
+        real omc = (real)1 - ca;
+        if (RFABS(omc) < eps)
+            c2 = (real)1 / (real)12;
+        else
+            c2 = ((real)1 / (angle * angle)) * ((real)1 - (angle * sa) / ((real)2 * omc));
+    }
+    for (int i = 0; i < 9; ++i) Vi[i] = ((i % 4) == 0 ? (real)1 : (real)0) - (real)0.5 * K[i] + c2 * K2[i];
+}
+
+/* se3_log_map_func, se3_kernels.py:285-309 */
+static void se3_log(const real *T, real eps, real *out) {
+    real w[3], Vi[9], v[3];
+    quat_to_axis_angle(T + 3, w);
+    se3_V_inv(w, eps, Vi);
+    mat3_vec(Vi, T, v);
+    out[0] = v[0]; out[1] = v[1]; out[2] = v[2];
+    out[3] = w[0]; out[4] = w[1]; out[5] = w[2];
+}
+
+/* se3_adjoint_func, se3_kernels.py:323-346: [[R, t^R],[0,R]] row-major 6x6 */
+static void se3_adjoint(const real *T, real *Ad) {
+    real R[9], tx[9], tR[9];
+    quat_to_mat(T + 3, R);
+    hat(T, tx);
+    mat3_mul(tx, R, tR);
+    memset(Ad, 0, 36 * sizeof(real));
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            Ad[i * 6 + j] = R[i * 3 + j];
+            Ad[i * 6 + 3 + j] = tR[i * 3 + j];
+            Ad[(i + 3) * 6 + 3 + j] = R[i * 3 + j];
+        }
+}
+
+/* so3_jac_left_inv_func, so3_kernels.py:13-40 */
+static void so3_jl_inv(const real *th, real eps, real *out) {
+    real t2 = th[0] * th[0] + th[1] * th[1] + th[2] * th[2];
+    int taylor = t2 < eps;
+    real t2s = taylor ? (real)1 : t2;
+    real ts = RSQRT(t2s);
+    real hts = ts / (real)2;
+    real K[9], K2[9];
+    hat(th, K);
+    mat3_mul(K, K, K2);
+    if (taylor) {
+        for (int i = 0; i < 9; ++i)
+            out[i] = ((i % 4) == 0 ? (real)1 : (real)0) - (real)0.5 * K[i] + t2s * K2[i] / (real)12;
+    } else {
+        real ch = RCOS(hts), sh = RSIN(hts);
+        real coeff = ((real)1 - ts * ch / ((real)2 * sh)) / t2s;
+        for (int i = 0; i < 9; ++i) out[i] = ((i % 4) == 0 ? (real)1 : (real)0) - (real)0.5 * K[i] + coeff * K2[i];
+    }
+}
+
+/* se3_jlog_func, se3_kernels.py:358-428 -> row-major 6x6 */
+static void se3_jlog(const real *T, real eps, real *J) {
+    const real *p = T;
+    real w[3];
+    quat_to_axis_angle(T + 3, w);
+    real t2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    real th = RSQRT(t2);
+    int taylor = t2 < eps;
+    real jl[9], jlT[9];
+    so3_jl_inv(w, eps, jl);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) jlT[i * 3 + j] = jl[j * 3 + i];
+    real th_safe = taylor ? (real)1 : th;
+    real th_inv = (real)1 / th_safe;
+    real th2_inv = th_inv * th_inv;
+    real st = RSIN(th), ct = RCOS(th);
+    real omc = (real)1 - ct;
+    real inv_2_2ct = taylor ? (real)0.5 : (real)1 / ((real)2 * omc);
+    real beta = th2_inv - st * th_inv * inv_2_2ct;
+    real bdot = (real)-2 * th2_inv * th2_inv + ((real)1 + st * th_inv) * th2_inv * inv_2_2ct;
+    real wTp = w[0] * p[0] + w[1] * p[1] + w[2] * p[2];
+    real v3[3];
+    for (int i = 0; i < 3; ++i) v3[i] = bdot * wTp * w[i] - (t2 * bdot + (real)2 * beta) * p[i];
+    real C[9], sk[9], B[9];
+    hat(p, sk);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            C[i * 3 + j] = v3[i] * w[j] + beta * (w[i] * p[j]) + ((i == j) ? (wTp * beta) : (real)0);
+    for (int i = 0; i < 9; ++i) C[i] = C[i] + (real)0.5 * sk[i];
+    mat3_mul(C, jlT, B);
+    if (taylor)
+        for (int i = 0; i < 9; ++i) B[i] = (real)0.5 * sk[i];
+    memset(J, 0, 36 * sizeof(real));
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            J[i * 6 + j] = jlT[i * 3 + j];
+            J[i * 6 + 3 + j] = B[i * 3 + j];
+            J[(i + 3) * 6 + 3 + j] = jlT[i * 3 + j];
+        }
+}
+
+static void mat6_vec(const real *M, const real *v, real *o) {
+    real t[6];
+    for (int i = 0; i < 6; ++i) {
+        real s = M[i * 6 + 0] * v[0];
+        for (int k = 1; k < 6; ++k) s += M[i * 6 + k] * v[k];
+        t[i] = s;
+    }
+    memcpy(o, t, sizeof(t));
+}
+
+/* ------------------------------------------------------------------------- */
+/* exported SE(3) ops (batched, for tests against the reference's constants)  */
+/* ------------------------------------------------------------------------- */
+EXPORT void ork_se3_compose(int n, const real *a, const real *b, real *out) {
+    for (int i = 0; i < n; ++i) se3_compose(a + 7 * i, b + 7 * i, out + 7 * i);
+}
+EXPORT void ork_se3_inverse(int n, const real *a, real *out) {
+    for (int i = 0; i < n; ++i) se3_inverse(a + 7 * i, out + 7 * i);
+}
+EXPORT void ork_se3_apply(int n, const real *T, const real *p, real *out) {
+    for (int i = 0; i < n; ++i) {
+        real r[3];
+        quat_apply(T + 7 * i + 3, p + 3 * i, r);
+        for (int k = 0; k < 3; ++k) out[3 * i + k] = r[k] + T[7 * i + k];
+    }
+}
+EXPORT void ork_se3_exp(int n, const real *xi, real eps, real *out) {
+    for (int i = 0; i < n; ++i) se3_exp(xi + 6 * i, eps, out + 7 * i);
+}
+EXPORT void ork_se3_log(int n, const real *T, real eps, real *out) {
+    for (int i = 0; i < n; ++i) se3_log(T + 7 * i, eps, out + 6 * i);
+}
+EXPORT void ork_se3_adjoint(int n, const real *T, real *out) {
+    for (int i = 0; i < n; ++i) se3_adjoint(T + 7 * i, out + 36 * i);
+}
+EXPORT void ork_se3_jlog(int n, const real *T, real eps, real *out) {
+    for (int i = 0; i < n; ++i) se3_jlog(T + 7 * i, eps, out + 36 * i);
+}
+
+/* ------------------------------------------------------------------------- */
+/* robot model (robo/robot_spec.py:45-138, 1043-1142)                         */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int nj, nl, na;
+    const int32_t *joint_type;        /* [nj] */
+    const float *joint_axis;          /* [nj*3] */
+    const float *parent_T;            /* [nj*7] parent_joint_transforms */
+    const int32_t *parent_joint;      /* [nj] */
+    const int32_t *act_idx;           /* [nj] actuated_joint_indices */
+    const int32_t *mimic_idx;         /* [nj] mimic_actuated_joint_indices */
+    const float *mimic_mul;           /* [nj] */
+    const float *mimic_off;           /* [nj] */
+    const int32_t *link_parent_joint; /* [nl] */
+    const float *limits;              /* [na*2] float32-narrowed lo,hi */
+} ork_robot;
+
+/* joint_twists, robot_spec.py:320-331 */
+static void joint_twist(const ork_robot *rb, int j, real *tw) {
+    for (int k = 0; k < 6; ++k) tw[k] = 0;
+    const float *ax = rb->joint_axis + 3 * j;
+    if (rb->joint_type[j] == 2) { tw[3] = ax[0]; tw[4] = ax[1]; tw[5] = ax[2]; }
+    else if (rb->joint_type[j] == 1) { tw[0] = ax[0]; tw[1] = ax[1]; tw[2] = ax[2]; }
+}
+
+/* joints_to_actuated_mapping, robot_spec.py:1120-1132 */
+static real j2a(const ork_robot *rb, int j, int a) {
+    if (rb->act_idx[j] >= 0) return rb->act_idx[j] == a ? (real)1 : (real)0;
+    if (rb->mimic_idx[j] >= 0) return rb->mimic_idx[j] == a ? (real)rb->mimic_mul[j] : (real)0;
+    return 0;
+}
+
+/* link_ancestor_joints_mask, robot_spec.py:1134-1143 */
+static int ancestor(const ork_robot *rb, int link, int joint) {
+    int j = rb->link_parent_joint[link];
+    while (j != -1) {
+        if (j == joint) return 1;
+        j = rb->parent_joint[j];
+    }
+    return 0;
+}
+
+/* _compute_joint_local_transform_func, robo/robot_kernels.py:264-297 */
+static void joint_local(const ork_robot *rb, const real *q, int j, real *X) {
+    real pos;
+    if (rb->mimic_idx[j] == -1) {
+        pos = rb->act_idx[j] == -1 ? (real)0 : q[rb->act_idx[j]];
+    } else {
+        pos = q[rb->mimic_idx[j]] * (real)rb->mimic_mul[j] + (real)rb->mimic_off[j];
+    }
+    real Xj[7] = {0, 0, 0, 1, 0, 0, 0};
+    const float *ax = rb->joint_axis + 3 * j;
+    if (rb->joint_type[j] == 2) {
+        real half = pos * (real)0.5;
+        real c = RCOS(half), s = RSIN(half);
+        Xj[3] = c; Xj[4] = (real)ax[0] * s; Xj[5] = (real)ax[1] * s; Xj[6] = (real)ax[2] * s;
+    } else if (rb->joint_type[j] == 1) {
+        Xj[0] = (real)ax[0] * pos; Xj[1] = (real)ax[1] * pos; Xj[2] = (real)ax[2] * pos;
+    }
+    real P[7];
+    for (int k = 0; k < 7; ++k) P[k] = (real)rb->parent_T[7 * j + k];
+    se3_compose(P, Xj, X);
+}
+
+/*
+ * Two-pass FK: compute_forward_kinematics_local_kernel (robot_kernels.py:300-324)
+ * then compute_forward_kinematics_accum_kernel (robot_kernels.py:327-377): each
+ * link composes its parent chain leaf-to-root, then T_world_base on the left.
+ * With S != NULL also the motion-subspace column S_world[:, j] = Ad(T_wj) twist_j
+ * (robot_kernels.py:365-368), layout S[6*nj] row-major (row r, joint j) = S[r*nj+j].
+ */
+static void fk(const ork_robot *rb, const real *q, const real *base, real *Xl, real *Tj, real *Tl, real *S) {
+    for (int j = 0; j < rb->nj; ++j) joint_local(rb, q, j, Xl + 7 * j);
+    for (int l = 0; l < rb->nl; ++l) {
+        int j = rb->link_parent_joint[l];
+        real Xw[7];
+        for (int k = 0; k < 7; ++k) Xw[k] = base[k];
+        if (j != -1) {
+            for (int k = 0; k < 7; ++k) Xw[k] = Xl[7 * j + k];
+            int p = rb->parent_joint[j];
+            while (p != -1) {
+                real t[7];
+                se3_compose(Xl + 7 * p, Xw, t);
+                memcpy(Xw, t, sizeof(t));
+                p = rb->parent_joint[p];
+            }
+            real t[7];
+            se3_compose(base, Xw, t);
+            memcpy(Xw, t, sizeof(t));
+        }
+        memcpy(Tl + 7 * l, Xw, sizeof(Xw));
+        if (j != -1) {
+            memcpy(Tj + 7 * j, Xw, sizeof(Xw));
+            if (S) {
+                real Ad[36], tw[6], col[6];
+                se3_adjoint(Xw, Ad);
+                joint_twist(rb, j, tw);
+                mat6_vec(Ad, tw, col);
+                for (int r = 0; r < 6; ++r) S[r * rb->nj + j] = col[r];
+            }
+        }
+    }
+}
+
+static const real IDENT7[7] = {0, 0, 0, 1, 0, 0, 0};
+
+/* Robot.forward_kinematics (robo/robot.py:131-271), CPU two-pass path. base may be NULL. */
+EXPORT void ork_fk(const ork_robot *rb, int B, const real *q, const real *base, real *T_link, real *T_joint) {
+    real *Xl = (real *)malloc(sizeof(real) * 7 * (rb->nj + 1));
+    real *Tj = (real *)malloc(sizeof(real) * 7 * (rb->nj + 1));
+    for (int b = 0; b < B; ++b) {
+        const real *bs = base ? base + 7 * b : IDENT7;
+        fk(rb, q + (size_t)b * rb->na, bs, Xl, Tj, T_link + (size_t)b * 7 * rb->nl, NULL);
+        if (T_joint) memcpy(T_joint + (size_t)b * 7 * rb->nj, Tj, sizeof(real) * 7 * rb->nj);
+    }
+    free(Xl);
+    free(Tj);
+}
+
+/*
+ * RobotState.get_link_jacobian -> compute_link_jacobian_kernel, robot_kernels.py:667-737.
+ * frame 0 = spatial, 1 = body.  Output J[6 x (6*fb + na)] row-major per batch element;
+ * with base != NULL the floating-base columns come first.
+ */
+EXPORT void ork_link_jacobian(const ork_robot *rb, int B, const real *q, const real *base, int link, int frame,
+                              real *J) {
+    int nj = rb->nj, na = rb->na;
+    int fb = base ? 6 : 0;
+    int cols = fb + na;
+    real *Xl = (real *)malloc(sizeof(real) * 7 * (nj + 1));
+    real *Tj = (real *)malloc(sizeof(real) * 7 * (nj + 1));
+    real *Tl = (real *)malloc(sizeof(real) * 7 * (rb->nl + 1));
+    real *S = (real *)calloc(6 * (nj + 1), sizeof(real));
+    for (int b = 0; b < B; ++b) {
+        const real *bs = base ? base + 7 * b : IDENT7;
+        memset(S, 0, sizeof(real) * 6 * nj);
+        fk(rb, q + (size_t)b * na, bs, Xl, Tj, Tl, S);
+        real *Jb = J + (size_t)b * 6 * cols;
+        const real *Tf = Tl + 7 * link;
+        for (int c = 0; c < cols; ++c) {
+            real col[6];
+            if (c < fb) {
+                real u[6] = {0, 0, 0, 0, 0, 0};
+                u[c] = 1;
+                real Ad[36];
+                if (frame == 0) {
+                    se3_adjoint(bs, Ad);
+                } else {
+                    real Tfw[7], Tfb[7];
+                    se3_inverse(Tf, Tfw);
+                    se3_compose(Tfw, bs, Tfb);
+                    se3_adjoint(Tfb, Ad);
+                }
+                mat6_vec(Ad, u, col);
+            } else {
+                int a = c - fb;
+                real sc[6] = {0, 0, 0, 0, 0, 0};
+                for (int j = 0; j < nj; ++j) {
+                    if (!ancestor(rb, link, j)) continue;
+                    real w = j2a(rb, j, a);
+                    if (w != (real)0)
+                        for (int r = 0; r < 6; ++r) sc[r] += S[r * nj + j] * w;
+                }
+                if (frame == 1) {
+                    real Tfw[7], Ad[36];
+                    se3_inverse(Tf, Tfw);
+                    se3_adjoint(Tfw, Ad);
+                    mat6_vec(Ad, sc, col);
+                } else {
+                    memcpy(col, sc, sizeof(col));
+                }
+            }
+            for (int r = 0; r < 6; ++r) Jb[r * cols + c] = col[r];
+        }
+    }
+    free(Xl);
+    free(Tj);
+    free(Tl);
+    free(S);
+}
+
+/* ------------------------------------------------------------------------- */
+/* residual terms                                                            */
+/* ------------------------------------------------------------------------- */
+enum { ORK_FRAME = 0, ORK_POSITION = 1, ORK_LIMIT = 2, ORK_SMOOTH = 3 };
+
+typedef struct {
+    int kind;
+    int nframes;            /* FRAME / POSITION: number of targets */
+    const int32_t *links;   /* [nframes] link indices */
+    const float *targets;   /* [B][nframes][7] world targets (one per problem) */
+    const float *weights;   /* FRAME: 6*nframes; POSITION: 3*nframes; LIMIT/SMOOTH: na */
+    const float *q_prev;    /* SMOOTH: [B][na] */
+} ork_term;
+
+static int term_rows(const ork_robot *rb, const ork_term *t) {
+    switch (t->kind) {
+    case ORK_FRAME: return 6 * t->nframes;
+    case ORK_POSITION: return 3 * t->nframes;
+    default: return rb->na;
+    }
+}
+
+/* spatial column of link `link` for actuated dof a: sum over ancestor joints of S_j * map[j,a]
+ * (frame_task.py:431-437, position_task.py:245-251) */
+static void spatial_column(const ork_robot *rb, const real *S, int link, int a, real *sc) {
+    for (int r = 0; r < 6; ++r) sc[r] = 0;
+    for (int j = 0; j < rb->nj; ++j) {
+        if (!ancestor(rb, link, j)) continue;
+        real w = j2a(rb, j, a);
+        if (w != (real)0)
+            for (int r = 0; r < 6; ++r) sc[r] += S[r * rb->nj + j] * w;
+    }
+}
+
+/*
+ * Weighted residual of one term for one problem b (row-major r[R]); with J != NULL also its
+ * weighted Jacobian J[R x na] (caller zeroes it).  Follows:
+ *   FRAME    compute_(multi_)frame_task_weighted_residual_kernel frame_task.py:505-554,
+ *            compute_(multi_)frame_task_weighted_jacobian_kernel frame_task.py:391-456, 557-621
+ *   POSITION position_task.py:188-212, 215-272
+ *   LIMIT    compute_position_limit_weighted_residual/jacobian_kernel position_limit.py:508-573 ("abs")
+ *   SMOOTH   compute_smoothness_task_weighted_residual/jacobian_kernel smoothness_task.py:350-365, 404-422
+ */
+static void term_eval(const ork_robot *rb, const ork_term *t, int b, const real *q, const real *Tl, const real *S,
+                      real *r, real *J) {
+    int na = rb->na;
+    if (t->kind == ORK_FRAME) {
+        for (int f = 0; f < t->nframes; ++f) {
+            const real *Tf = Tl + 7 * t->links[f];
+            real tgt[7];
+            for (int k = 0; k < 7; ++k) tgt[k] = (real)t->targets[((size_t)b * t->nframes + f) * 7 + k];
+            const float *w = t->weights + 6 * f;
+            real Tfw[7], Tft[7], e[6];
+            se3_inverse(Tf, Tfw);
+            se3_compose(Tfw, tgt, Tft);
+            se3_log(Tft, (real)1e-4, e);
+            for (int i = 0; i < 6; ++i) r[6 * f + i] = (real)w[i] * e[i];
+            if (J) {
+                real Ttw[7], Ttf[7], jl[36], Ad[36];
+                se3_inverse(tgt, Ttw);
+                se3_compose(Ttw, Tf, Ttf);
+                se3_jlog(Ttf, (real)1e-4, jl);
+                se3_adjoint(Tfw, Ad);
+                for (int a = 0; a < na; ++a) {
+                    real sc[6], bc[6], jc[6];
+                    spatial_column(rb, S, t->links[f], a, sc);
+                    mat6_vec(Ad, sc, bc);
+                    mat6_vec(jl, bc, jc);
+                    for (int i = 0; i < 6; ++i) J[(6 * f + i) * na + a] = -(real)w[i] * jc[i];
+                }
+            }
+        }
+    } else if (t->kind == ORK_POSITION) {
+        for (int f = 0; f < t->nframes; ++f) {
+            const real *Tf = Tl + 7 * t->links[f];
+            const float *tg = t->targets + ((size_t)b * t->nframes + f) * 7;
+            const float *w = t->weights + 3 * f;
+            for (int i = 0; i < 3; ++i) r[3 * f + i] = (real)w[i] * ((real)tg[i] - Tf[i]);
+            if (J) {
+                for (int a = 0; a < na; ++a) {
+                    real sc[6], wxp[3];
+                    spatial_column(rb, S, t->links[f], a, sc);
+                    cross3(sc + 3, Tf, wxp);
+                    for (int i = 0; i < 3; ++i) J[(3 * f + i) * na + a] = -(real)w[i] * (sc[i] + wxp[i]);
+                }
+            }
+        }
+    } else if (t->kind == ORK_LIMIT) {
+        for (int d = 0; d < na; ++d) {
+            real lo = (real)rb->limits[2 * d], hi = (real)rb->limits[2 * d + 1];
+            real x = q[d];
+            real eu = x - hi > (real)0 ? x - hi : (real)0;
+            real el = lo - x > (real)0 ? lo - x : (real)0;
+            r[d] = (real)t->weights[d] * (eu + el);
+            if (J) {
+                real sign = 0;
+                if (x > hi) sign = 1;
+                else if (x < lo) sign = -1;
+                J[d * na + d] = (real)t->weights[d] * sign;
+            }
+        }
+    } else if (t->kind == ORK_SMOOTH) {
+        const float *qp = t->q_prev + (size_t)b * na;
+        for (int d = 0; d < na; ++d) {
+            r[d] = (real)t->weights[d] * (q[d] - (real)qp[d]);
+            if (J) J[d * na + d] = (real)t->weights[d];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* LM: opt/lm_optimizer.py generic (CPU) path                                */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    float lm_lambda, lambda_factor, lambda_min, lambda_max, rho_min, gain_ratio_epsilon;
+} ork_lm_params;
+
+/*
+ * Dense damped Cholesky solve, the algorithm of _cholesky_kernel dense branch
+ * (lm_optimizer.py:56-79): A = H + lam I; L L^T = A; delta = -A^{-1} g;
+ * pred = 0.5 * sum(delta * (lam*delta - g)).  Returns 0 on a non-positive pivot
+ * (delta/pred then hold NaN, which the accept test rejects, lm_optimizer.py:352-362).
+ */
+static int damped_cholesky(int D, const real *H, const real *g, real lam, real *L, real *delta, real *pred) {
+    for (int i = 0; i < D * D; ++i) L[i] = H[i];
+    for (int i = 0; i < D; ++i) L[i * D + i] = L[i * D + i] + lam;
+    int ok = 1;
+    for (int j = 0; j < D; ++j) {
+        real s = L[j * D + j];
+        for (int k = 0; k < j; ++k) s -= L[j * D + k] * L[j * D + k];
+        if (!(s > (real)0)) ok = 0;
+        real d = RSQRT(s);
+        L[j * D + j] = d;
+        for (int i = j + 1; i < D; ++i) {
+            real v = L[i * D + j];
+            for (int k = 0; k < j; ++k) v -= L[i * D + k] * L[j * D + k];
+            L[i * D + j] = v / d;
+        }
+    }
+    real y[256];
+    for (int i = 0; i < D; ++i) {
+        real v = -g[i];
+        for (int k = 0; k < i; ++k) v -= L[i * D + k] * y[k];
+        y[i] = v / L[i * D + i];
+    }
+    for (int i = D - 1; i >= 0; --i) {
+        real v = y[i];
+        for (int k = i + 1; k < D; ++k) v -= L[k * D + i] * delta[k];
+        delta[i] = v / L[i * D + i];
+    }
+    real p = 0;
+    for (int i = 0; i < D; ++i) p += delta[i] * (lam * delta[i] - g[i]);
+    *pred = (real)0.5 * p;
+    return ok;
+}
+
+/* lm_accept_reject_cost_kernel, lm_optimizer.py:337-362 */
+EXPORT void ork_accept(int n, real *cost, const real *cost_prop, const real *pred, const ork_lm_params *p, real *lam,
+                       int32_t *acc) {
+    for (int b = 0; b < n; ++b) {
+        int accepted = pred[b] > (real)0;
+        if (accepted) {
+            real rho = (cost[b] - cost_prop[b]) / (pred[b] + (real)p->gain_ratio_epsilon);
+            accepted = rho >= (real)p->rho_min;
+        }
+        if (accepted) {
+            acc[b] = 1;
+            real l = lam[b] / (real)p->lambda_factor;
+            lam[b] = l > (real)p->lambda_min ? l : (real)p->lambda_min;
+            cost[b] = cost_prop[b];
+        } else {
+            acc[b] = 0;
+            real l = lam[b] * (real)p->lambda_factor;
+            if (l < (real)p->lambda_min) l = (real)p->lambda_min;
+            if (l > (real)p->lambda_max) l = (real)p->lambda_max;
+            lam[b] = l;
+        }
+    }
+}
+
+/* damped solve only (exported for the one-step delta test, tests/opt/test_optimizer.py:366-429) */
+EXPORT int ork_damped_solve(int D, const real *H, const real *g, real lam, real *delta, real *pred) {
+    real *L = (real *)malloc(sizeof(real) * D * D);
+    int ok = damped_cholesky(D, H, g, lam, L, delta, pred);
+    free(L);
+    return ok;
+}
+
+/*
+ * Residual + Jacobian + normal equations of all terms at q for problem b:
+ * the per-term ResidualTask.accumulate_normal_equations (terms/task.py:176-232) with
+ * _accum_kernel (lm_optimizer.py:228-257): H += J^T J, g += J^T r, cost += 0.5 r^T r.
+ * Also returns the stacked residual r_all (rows in term order) when non-NULL.
+ */
+typedef struct {
+    real *Xl, *Tj, *Tl, *S, *r, *J, *H, *g, *L, *delta, *q_prop;
+    int R;
+} ork_ws;
+
+static void ws_alloc(ork_ws *w, const ork_robot *rb, int nterms, const ork_term *terms) {
+    int R = 0;
+    for (int t = 0; t < nterms; ++t) R += term_rows(rb, &terms[t]);
+    w->R = R;
+    int nj = rb->nj, na = rb->na;
+    w->Xl = (real *)calloc(7 * (nj + 1), sizeof(real));
+    w->Tj = (real *)calloc(7 * (nj + 1), sizeof(real));
+    w->Tl = (real *)calloc(7 * (rb->nl + 1), sizeof(real));
+    w->S = (real *)calloc(6 * (nj + 1), sizeof(real));
+    w->r = (real *)calloc(R + 1, sizeof(real));
+    w->J = (real *)calloc((size_t)(R + 1) * na, sizeof(real));
+    w->H = (real *)calloc(na * na, sizeof(real));
+    w->g = (real *)calloc(na, sizeof(real));
+    w->L = (real *)calloc(na * na, sizeof(real));
+    w->delta = (real *)calloc(na, sizeof(real));
+    w->q_prop = (real *)calloc(na, sizeof(real));
+}
+
+static void ws_free(ork_ws *w) {
+    free(w->Xl); free(w->Tj); free(w->Tl); free(w->S); free(w->r); free(w->J);
+    free(w->H); free(w->g); free(w->L); free(w->delta); free(w->q_prop);
+}
+
+static real normal_equations(const ork_robot *rb, int nterms, const ork_term *terms, int b, const real *q,
+                             const real *base, ork_ws *w) {
+    int na = rb->na;
+    memset(w->S, 0, sizeof(real) * 6 * rb->nj);
+    fk(rb, q, base, w->Xl, w->Tj, w->Tl, w->S);
+    memset(w->H, 0, sizeof(real) * na * na);
+    memset(w->g, 0, sizeof(real) * na);
+    real cost = 0;
+    int row = 0;
+    for (int t = 0; t < nterms; ++t) {
+        int Rt = term_rows(rb, &terms[t]);
+        real *r = w->r + row;
+        real *J = w->J + (size_t)row * na;
+        memset(J, 0, sizeof(real) * Rt * na);
+        term_eval(rb, &terms[t], b, q, w->Tl, w->S, r, J);
+        for (int i = 0; i < na; ++i)
+            for (int j = 0; j < na; ++j) {
+                real s = 0;
+                for (int k = 0; k < Rt; ++k) s += J[k * na + i] * J[k * na + j];
+                w->H[i * na + j] = w->H[i * na + j] + s;
+            }
+        for (int i = 0; i < na; ++i) {
+            real s = 0;
+            for (int k = 0; k < Rt; ++k) s += J[k * na + i] * r[k];
+            w->g[i] = w->g[i] + s;
+        }
+        real rr = 0;
+        for (int k = 0; k < Rt; ++k) rr += r[k] * r[k];
+        cost = cost + (real)0.5 * rr;
+        row += Rt;
+    }
+    return cost;
+}
+
+/* cost-only pass: per-term residual then _strided_cost_kernel add (optimizer.py:80-105) */
+static real cost_only(const ork_robot *rb, int nterms, const ork_term *terms, int b, const real *q, const real *base,
+                      ork_ws *w) {
+    fk(rb, q, base, w->Xl, w->Tj, w->Tl, NULL);
+    real cost = 0;
+    for (int t = 0; t < nterms; ++t) {
+        int Rt = term_rows(rb, &terms[t]);
+        term_eval(rb, &terms[t], b, q, w->Tl, NULL, w->r, NULL);
+        real s = 0;
+        for (int k = 0; k < Rt; ++k) s += w->r[k] * w->r[k];
+        cost = cost + (real)0.5 * s;
+    }
+    return cost;
+}
+
+/* stacked residual and Jacobian at q (exported for finite-difference and one-step tests) */
+EXPORT int ork_residual_jacobian(const ork_robot *rb, int nterms, const ork_term *terms, int b, const real *q,
+                                 const real *base, real *r_out, real *J_out, real *H_out, real *g_out,
+                                 real *cost_out) {
+    ork_ws w;
+    ws_alloc(&w, rb, nterms, terms);
+    const real *bs = base ? base : IDENT7;
+    real c = normal_equations(rb, nterms, terms, b, q, bs, &w);
+    if (r_out) memcpy(r_out, w.r, sizeof(real) * w.R);
+    if (J_out) memcpy(J_out, w.J, sizeof(real) * w.R * rb->na);
+    if (H_out) memcpy(H_out, w.H, sizeof(real) * rb->na * rb->na);
+    if (g_out) memcpy(g_out, w.g, sizeof(real) * rb->na);
+    if (cost_out) *cost_out = c;
+    int R = w.R;
+    ws_free(&w);
+    return R;
+}
+
+/*
+ * Optimizer.solve (opt/optimizer.py:216-240) with LMOptimizer's generic step
+ * (_propose_body generic branch lm_optimizer.py:912-962, _accept_body :964-982),
+ * fixed iteration count (early_stopping_interval = 0).  Problems are independent,
+ * so the batch runs problem by problem.
+ *
+ *   q        [B][na] in: seeds, out: solution
+ *   cost_trace [B][iters+1] (nullable): cost at the seed, then after every step
+ *   accept_trace [B][iters] (nullable), lambda_trace [B][iters] (nullable)
+ */
+EXPORT void ork_lm_solve(const ork_robot *rb, int nterms, const ork_term *terms, const ork_lm_params *p, int B,
+                         int iters, real *q, const real *base, real *cost_out, real *cost_trace,
+                         int32_t *accept_trace, real *lambda_trace) {
+    int na = rb->na;
+    ork_ws w;
+    ws_alloc(&w, rb, nterms, terms);
+    for (int b = 0; b < B; ++b) {
+        real *qb = q + (size_t)b * na;
+        const real *bs = base ? base + 7 * b : IDENT7;
+        real lam = (real)p->lm_lambda;
+        real cost = 0;
+        if (cost_trace) cost_trace[(size_t)b * (iters + 1)] = cost_only(rb, nterms, terms, b, qb, bs, &w);
+        if (iters == 0) cost = cost_only(rb, nterms, terms, b, qb, bs, &w);
+        for (int it = 0; it < iters; ++it) {
+            cost = normal_equations(rb, nterms, terms, b, qb, bs, &w);
+            real pred;
+            damped_cholesky(na, w.H, w.g, lam, w.L, w.delta, &pred);
+            /* integrate_joint_positions_kernel, robot_kernels.py:562-590 (mask 1, no clamp, no decay) */
+            for (int d = 0; d < na; ++d) w.q_prop[d] = qb[d] + (real)1 * (w.delta[d] - (real)0 * qb[d]);
+            real cp = cost_only(rb, nterms, terms, b, w.q_prop, bs, &w);
+            int32_t acc;
+            ork_accept(1, &cost, &cp, &pred, p, &lam, &acc);
+            if (acc) memcpy(qb, w.q_prop, sizeof(real) * na);
+            if (cost_trace) cost_trace[(size_t)b * (iters + 1) + it + 1] = cost;
+            if (accept_trace) accept_trace[(size_t)b * iters + it] = acc;
+            if (lambda_trace) lambda_trace[(size_t)b * iters + it] = lam;
+        }
+        if (cost_out) cost_out[b] = cost;
+    }
+    ws_free(&w);
+}
+
+/*
+ * One LM step (the body of ork_lm_solve's loop) from given iterates: problem b starts at
+ * q[b] with damping lam[b].  Outputs the cost at q, the proposal's cost, the predicted
+ * reduction, the accept flag, the next iterate and damping.  Used to check each GPU step
+ * from the GPU's own iterate ("shadowing"), which is insensitive to the rounding-level forks
+ * a free-running comparison of two LM trajectories suffers.
+ */
+EXPORT void ork_lm_step(const ork_robot *rb, int nterms, const ork_term *terms, const ork_lm_params *p, int B,
+                        const real *q, const real *lam, real *cost_cur, real *cost_prop, real *pred_out,
+                        int32_t *acc_out, real *q_new, real *lam_new) {
+    int na = rb->na;
+    ork_ws w;
+    ws_alloc(&w, rb, nterms, terms);
+    for (int b = 0; b < B; ++b) {
+        const real *qb = q + (size_t)b * na;
+        real l = lam[b];
+        real cost = normal_equations(rb, nterms, terms, b, qb, IDENT7, &w);
+        cost_cur[b] = cost;
+        real pred;
+        damped_cholesky(na, w.H, w.g, l, w.L, w.delta, &pred);
+        for (int d = 0; d < na; ++d) w.q_prop[d] = qb[d] + (real)1 * (w.delta[d] - (real)0 * qb[d]);
+        real cp = cost_only(rb, nterms, terms, b, w.q_prop, IDENT7, &w);
+        cost_prop[b] = cp;
+        pred_out[b] = pred;
+        int32_t acc;
+        ork_accept(1, &cost, &cp, &pred, p, &l, &acc);
+        acc_out[b] = acc;
+        lam_new[b] = l;
+        for (int d = 0; d < na; ++d) q_new[(size_t)b * na + d] = acc ? w.q_prop[d] : qb[d];
+    }
+    ws_free(&w);
+}
+
+/*
+ * _ik_metrics_kernel, helpers/ik/ik.py:59-82: per target position error |t(T_link^-1 T_target)|,
+ * angle 2 asin(min(|q_vec|, 1)); success iff every target is strictly under both thresholds.
+ */
+EXPORT void ork_ik_metrics(const ork_robot *rb, int B, const real *q, const real *base, int ntargets,
+                           const int32_t *links, const float *targets, float pos_thr, float ang_thr, real *pos_err,
+                           real *ang_err, int32_t *success) {
+    real *Tl = (real *)malloc(sizeof(real) * 7 * (rb->nl + 1));
+    real *Xl = (real *)malloc(sizeof(real) * 7 * (rb->nj + 1));
+    real *Tj = (real *)malloc(sizeof(real) * 7 * (rb->nj + 1));
+    for (int b = 0; b < B; ++b) {
+        const real *bs = base ? base + 7 * b : IDENT7;
+        fk(rb, q + (size_t)b * rb->na, bs, Xl, Tj, Tl, NULL);
+        int reached = 1;
+        for (int i = 0; i < ntargets; ++i) {
+            real inv[7], e[7], tg[7];
+            for (int k = 0; k < 7; ++k) tg[k] = (real)targets[((size_t)b * ntargets + i) * 7 + k];
+            se3_inverse(Tl + 7 * links[i], inv);
+            se3_compose(inv, tg, e);
+            real pe = RSQRT(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+            real v = RSQRT(e[4] * e[4] + e[5] * e[5] + e[6] * e[6]);
+            real ae = (real)2 * RASIN(v < (real)1 ? v : (real)1);
+            pos_err[(size_t)b * ntargets + i] = pe;
+            ang_err[(size_t)b * ntargets + i] = ae;
+            if (pe >= (real)pos_thr || ae >= (real)ang_thr) reached = 0;
+        }
+        success[b] = reached;
+    }
+    free(Tl);
+    free(Xl);
+    free(Tj);
+}

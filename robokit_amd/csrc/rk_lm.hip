@@ -1,0 +1,1898 @@
+// rk_lm.hip -- batched Levenberg-Marquardt for MI355X (gfx950): kernels + C ABI (include/rkhip.h).
+//
+// Two execution forms of the same per-problem math:
+//   * fused:  one kernel runs the whole fixed-iteration solve of a problem inside one wavefront;
+//             G lanes (4..64) own a problem, its FK poses, motion subspace, Jacobian, normal
+//             matrix and Cholesky factor live in LDS; nothing but inputs/outputs touches HBM.
+//             The persistent variant pulls tiles of problems of several batches (different
+//             robots / sizes) from a device work queue.
+//   * plain:  the reference's GPU decomposition (lm_optimizer.py:834-982 fast path): per-term
+//             residual / Jacobian kernels writing J to HBM, a solve kernel, integrate, FK at the
+//             proposal, per-term residuals, accept -- a host loop of ~13 launches per iteration.
+//
+// Determinism: every reduction (each H entry, each g entry, the cost, each Cholesky entry,
+// the triangular solves) is summed by ONE lane in a fixed row order, so a problem's result does
+// not depend on G, on its slot in the wave, or on what else is in the batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rkhip.h"
+#include "rk_device.h"
+
+#define RK_MAXT 4   // pose targets per problem
+#define RK_MAXQ 12  // position targets per problem
+#define RK_MAX_ROBOTS 64
+
+namespace rk {
+
+constexpr int WAVE = 64;
+
+struct Terms {
+  int robot, n_pose, n_pos, limit, smooth;
+  int R, Rf, Rq;
+  int pose_link[RK_MAXT];
+  float pose_w[RK_MAXT][6];
+  int pos_link[RK_MAXQ];
+  float pos_w[RK_MAXQ][3];
+  float limit_w[RK_NA_MAX];
+  float smooth_w[RK_NA_MAX];
+};
+
+struct LMParams {
+  float lm_lambda, lambda_factor, lambda_min, lambda_max, rho_min, gain_ratio_epsilon;
+};
+
+// Structure-of-arrays problem buffers of one batch: element (k, problem s) at [k * stride + s].
+struct Problems {
+  const Robot* robot;
+  const Terms* terms;
+  float* q;               // [na][stride]
+  const float* q_seed;    // [na][stride] start iterate for solves that restart from the seeds, or null
+  const float* pose_tgt;  // [n_pose*7][stride]
+  const float* pos_tgt;   // [n_pos*3][stride]
+  const float* q_prev;    // [na][stride]
+  float* cost;            // [stride]
+  float* lam;             // [stride]
+  int* nacc;              // [stride]
+  float* trace;           // [(trace_iters+1)][stride] or null
+  int* acc_trace;         // [trace_iters][stride] or null
+  float* q_trace;         // [trace_iters][na][stride] iterate q_i before step i, or null
+  float* lam_trace;       // [trace_iters][stride] damping used by step i, or null
+  int trace_iters;
+  int n, stride;
+};
+
+template <int NJ, int NA, int NP, int NQ>
+struct Cfg {
+  static constexpr int nj = NJ, na = NA, np = NP, nq = NQ;
+  static constexpr int rp = 6 * NP + 3 * NQ;
+  static constexpr int rmax = rp + 2 * NA;
+};
+using CfgS = Cfg<16, 8, 2, 2>;    // serial arms: nj <= 16, na <= 8, <= 2 pose / 2 position targets
+using CfgL = Cfg<32, 32, 4, 12>;  // trees: nj <= 32, na <= 32, <= 4 pose / 12 position targets
+
+template <class C>
+struct alignas(16) Shared {
+  float Tw[2][C::nj][8];  // world joint poses: [cur] at q, [1-cur] at the proposal
+  float S[C::nj][6];      // motion-subspace columns at q
+  float J[C::rp][C::na];  // pose + position rows of the weighted Jacobian
+  float r[C::rmax];       // weighted residual rows (all terms)
+  float jl[C::na], js[C::na];  // diagonal Jacobian of limit / smoothness rows
+  float H[C::na][C::na + 1];   // J^T J (lower), then its damped Cholesky factor in place
+  float g[C::na], dq[C::na], q[C::na], qp[C::na], qprev[C::na];
+  float tg[C::np][36];  // per pose target: R(T_f^-1) 9, t(T_f^-1) 3, jlog A 9, B 9, weights 6
+  float tp[C::np][8];   // pose targets
+  float tq[C::nq][4];   // position targets
+  float cost, pred, lam, pad0;
+  int cur, acc, nacc, pad1;
+};
+
+__device__ __forceinline__ void tri_index(int k, int& i, int& j) {
+  int r = (int)((sqrtf(8.f * (float)k + 1.f) - 1.f) * 0.5f);
+  while ((r + 1) * (r + 2) / 2 <= k) ++r;
+  while (r * (r + 1) / 2 > k) --r;
+  i = r;
+  j = k - r * (r + 1) / 2;
+}
+
+// ---------------------------------------------------------------------------------------------
+// per-problem building blocks shared by the fused and plain forms
+// ---------------------------------------------------------------------------------------------
+
+// Pose residual of target t (+ the per-target Jacobian blocks when `tg` is non-null).
+// r = w * log(T_f^-1 T_tgt) (frame_task.py:505-554); the Jacobian uses jlog(T_tgt^-1 T_f) and
+// Ad(T_f^-1) (frame_task.py:391-456, 557-621).
+__device__ __forceinline__ void pose_target(const Pose& Tf, const Pose& Tt, const float* w, float* r6, float* tg) {
+  Pose Tfi = inverse(Tf);
+  Pose E = compose(Tfi, Tt);
+  float e[6];
+  se3_log(E, e);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) r6[i] = w[i] * e[i];
+  if (tg) {
+    Pose F = compose(inverse(Tt), Tf);
+    Mat3 A, B;
+    se3_jlog_blocks(F, A, B);
+    Mat3 R = quat_to_mat(Tfi.q);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) tg[i] = R.m[i];
+    tg[9] = Tfi.t[0]; tg[10] = Tfi.t[1]; tg[11] = Tfi.t[2];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) tg[12 + i] = A.m[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) tg[21 + i] = B.m[i];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) tg[30 + i] = w[i];
+  }
+}
+
+// spatial column of link `link` for actuated dof a: sum of S_j * map[j][a] over ancestor joints
+// in ascending j (frame_task.py:431-437).  S_at(j, row) reads the subspace.
+template <class SAt>
+__device__ __forceinline__ void spatial_column(const Robot* rb, int link, int a, SAt S_at, float* sc) {
+#pragma unroll
+  for (int r = 0; r < 6; ++r) sc[r] = 0.f;
+  unsigned long long anc = rb->anc[link];
+  for (int c = rb->col_start[a]; c < rb->col_start[a + 1]; ++c) {
+    int j = rb->col_joint[c];
+    if ((anc >> j) & 1ull) {
+      float w = rb->col_w[c];
+#pragma unroll
+      for (int r = 0; r < 6; ++r) sc[r] += S_at(j, r) * w;
+    }
+  }
+}
+
+// pose-row Jacobian column: -w * jlog * Ad(T_f^-1) * sc, with the structured blocks in tg
+__device__ __forceinline__ void pose_jac_column(const float* tg, const float* sc, float* out6) {
+  const float* R = tg;
+  const float* t = tg + 9;
+  const float* A = tg + 12;
+  const float* B = tg + 21;
+  const float* w = tg + 30;
+  float Rv[3], Rw[3], txRw[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Rv[i] = R[i * 3 + 0] * sc[0] + R[i * 3 + 1] * sc[1] + R[i * 3 + 2] * sc[2];
+    Rw[i] = R[i * 3 + 0] * sc[3] + R[i * 3 + 1] * sc[4] + R[i * 3 + 2] * sc[5];
+  }
+  cross3(t, Rw, txRw);
+  float bv[3] = {Rv[0] + txRw[0], Rv[1] + txRw[1], Rv[2] + txRw[2]};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float top = A[i * 3 + 0] * bv[0] + A[i * 3 + 1] * bv[1] + A[i * 3 + 2] * bv[2] + B[i * 3 + 0] * Rw[0] +
+                B[i * 3 + 1] * Rw[1] + B[i * 3 + 2] * Rw[2];
+    float bot = A[i * 3 + 0] * Rw[0] + A[i * 3 + 1] * Rw[1] + A[i * 3 + 2] * Rw[2];
+    out6[i] = -w[i] * top;
+    out6[3 + i] = -w[3 + i] * bot;
+  }
+}
+
+// position-row Jacobian column: -w * (v + w x p)  (position_task.py:253-262)
+__device__ __forceinline__ void pos_jac_column(const float* sc, const float* p, const float* w, float* out3) {
+  float wxp[3];
+  cross3(sc + 3, p, wxp);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) out3[i] = -w[i] * (sc[i] + wxp[i]);
+}
+
+__device__ __forceinline__ float limit_row(float x, float lo, float hi, float w, float* jdiag) {
+  float eu = fmaxf(x - hi, 0.f);
+  float el = fmaxf(lo - x, 0.f);
+  if (jdiag) *jdiag = w * (x > hi ? 1.f : (x < lo ? -1.f : 0.f));
+  return w * (eu + el);
+}
+
+// cost = sum over terms (in row order) of 0.5 * sum of squares of its rows
+__device__ __forceinline__ float term_cost(const Terms* tm, int na, const float* r) {
+  float cost = 0.f;
+  int off = 0;
+  const int sizes[4] = {tm->Rf, tm->Rq, tm->limit ? na : 0, tm->smooth ? na : 0};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    if (sizes[t] == 0) continue;
+    float s = 0.f;
+    for (int k = 0; k < sizes[t]; ++k) s += r[off + k] * r[off + k];
+    cost = cost + 0.5f * s;
+    off += sizes[t];
+  }
+  return cost;
+}
+
+// LM accept / damping update (lm_accept_reject_cost_kernel, lm_optimizer.py:337-362)
+__device__ __forceinline__ bool lm_accept(const LMParams& lp, float& cost, float cost_prop, float pred, float& lam) {
+  bool acc = pred > 0.f;
+  if (acc) {
+    float rho = (cost - cost_prop) / (pred + lp.gain_ratio_epsilon);
+    acc = rho >= lp.rho_min;
+  }
+  if (acc) {
+    lam = fmaxf(lam / lp.lambda_factor, lp.lambda_min);
+    cost = cost_prop;
+  } else {
+    lam = fminf(fmaxf(lam * lp.lambda_factor, lp.lambda_min), lp.lambda_max);
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------------------------
+// fused form: G lanes per problem, everything in LDS
+// ---------------------------------------------------------------------------------------------
+
+// FK level by level (joints of one depth in parallel): T_j = T_parent o (P_j o M(q_j)).
+template <int G, class C>
+__device__ __forceinline__ void fk_levels(const Robot* rb, const float* q, float (*Tw)[8], int lane) {
+  const int nlev = rb->nlevels;
+  for (int lv = 0; lv < nlev; ++lv) {
+    const int b = rb->level_start[lv], e = rb->level_start[lv + 1];
+    for (int k = b + lane; k < e; k += G) {
+      int j = rb->order[k];
+      Pose X = joint_local(rb, j, q);
+      int p = rb->parent[j];
+      if (p >= 0) {
+        Pose Tp;
+        load_pose(Tw[p], Tp);
+        X = compose(Tp, X);
+      }
+      store_pose(Tw[j], X);
+    }
+    __syncthreads();
+  }
+}
+
+template <class C>
+__device__ __forceinline__ Pose link_pose(const Robot* rb, float (*T)[8], int link) {
+  int j = rb->link_joint[link];
+  Pose P;
+  if (j >= 0) {
+    load_pose(T[j], P);
+  } else {
+    P.t[0] = P.t[1] = P.t[2] = 0.f;
+    P.q[0] = 1.f; P.q[1] = P.q[2] = P.q[3] = 0.f;
+  }
+  return P;
+}
+
+// residual rows of all terms at (T, q); with_jac also fills tg, jl, js.  No trailing sync.
+template <int G, class C>
+__device__ __forceinline__ void residual_rows(const Robot* rb, const Terms* tm, Shared<C>* sh, float (*T)[8],
+                                              const float* q, int lane, bool with_jac) {
+  const int na = rb->na;
+  for (int t = lane; t < tm->n_pose; t += G) {
+    Pose Tf = link_pose<C>(rb, T, tm->pose_link[t]);
+    Pose Tt;
+    load_pose(sh->tp[t], Tt);
+    pose_target(Tf, Tt, tm->pose_w[t], sh->r + 6 * t, with_jac ? sh->tg[t] : nullptr);
+  }
+  for (int u = lane; u < tm->n_pos; u += G) {
+    Pose Tf = link_pose<C>(rb, T, tm->pos_link[u]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) sh->r[tm->Rf + 3 * u + i] = tm->pos_w[u][i] * (sh->tq[u][i] - Tf.t[i]);
+  }
+  const int off_l = tm->Rf + tm->Rq;
+  if (tm->limit)
+    for (int d = lane; d < na; d += G)
+      sh->r[off_l + d] = limit_row(q[d], rb->lo[d], rb->hi[d], tm->limit_w[d], with_jac ? &sh->jl[d] : nullptr);
+  const int off_s = off_l + (tm->limit ? na : 0);
+  if (tm->smooth)
+    for (int d = lane; d < na; d += G) {
+      sh->r[off_s + d] = tm->smooth_w[d] * (q[d] - sh->qprev[d]);
+      if (with_jac) sh->js[d] = tm->smooth_w[d];
+    }
+}
+
+// H = J^T J (lower), g = J^T r, cost (each entry summed by one lane, rows in order)
+template <int G, class C>
+__device__ __forceinline__ void normal_equations(const Terms* tm, Shared<C>* sh, int na, int lane) {
+  const int ntri = na * (na + 1) / 2;
+  const int Rf = tm->Rf, Rq = tm->Rq;
+  const int off_l = Rf + Rq, off_s = off_l + (tm->limit ? na : 0);
+  for (int k = lane; k <= ntri + na; k += G) {
+    if (k < ntri) {
+      int i, j;
+      tri_index(k, i, j);
+      float s = 0.f;
+      for (int r = 0; r < Rf; ++r) s += sh->J[r][i] * sh->J[r][j];
+      float h = s;
+      if (Rq) {
+        float s2 = 0.f;
+        for (int r = Rf; r < Rf + Rq; ++r) s2 += sh->J[r][i] * sh->J[r][j];
+        h = h + s2;
+      }
+      if (i == j) {
+        if (tm->limit) h = h + sh->jl[i] * sh->jl[i];
+        if (tm->smooth) h = h + sh->js[i] * sh->js[i];
+      }
+      sh->H[i][j] = h;
+    } else if (k < ntri + na) {
+      int i = k - ntri;
+      float s = 0.f;
+      for (int r = 0; r < Rf; ++r) s += sh->J[r][i] * sh->r[r];
+      float gi = s;
+      if (Rq) {
+        float s2 = 0.f;
+        for (int r = Rf; r < Rf + Rq; ++r) s2 += sh->J[r][i] * sh->r[r];
+        gi = gi + s2;
+      }
+      if (tm->limit) gi = gi + sh->jl[i] * sh->r[off_l + i];
+      if (tm->smooth) gi = gi + sh->js[i] * sh->r[off_s + i];
+      sh->g[i] = gi;
+    } else {
+      sh->cost = term_cost(tm, na, sh->r);
+    }
+  }
+}
+
+// damped Cholesky (H + lam I) L L^T, delta = -(H + lam I)^-1 g, pred = 0.5 delta.(lam delta - g)
+// (_cholesky_kernel dense branch, lm_optimizer.py:56-79).  Ends with a sync.
+template <int G, class C>
+__device__ __forceinline__ void chol_solve(Shared<C>* sh, int na, int lane) {
+  if (na <= 12) {
+    if (lane == 0) {
+      const float lam = sh->lam;
+      for (int j = 0; j < na; ++j) {
+        float s = sh->H[j][j] + lam;
+        for (int k = 0; k < j; ++k) s -= sh->H[j][k] * sh->H[j][k];
+        float d = sqrtf(s);
+        sh->H[j][j] = d;
+        for (int i = j + 1; i < na; ++i) {
+          float v = sh->H[i][j];
+          for (int k = 0; k < j; ++k) v -= sh->H[i][k] * sh->H[j][k];
+          sh->H[i][j] = v / d;
+        }
+      }
+    }
+  } else {
+    for (int d = lane; d < na; d += G) sh->H[d][d] = sh->H[d][d] + sh->lam;
+    __syncthreads();
+    for (int k = 0; k < na; ++k) {
+      if (lane == 0) sh->H[k][k] = sqrtf(sh->H[k][k]);
+      __syncthreads();
+      for (int i = k + 1 + lane; i < na; i += G) sh->H[i][k] = sh->H[i][k] / sh->H[k][k];
+      __syncthreads();
+      const int m = na - k - 1;
+      const int nt = m * (m + 1) / 2;
+      for (int t = lane; t < nt; t += G) {
+        int ii, jj;
+        tri_index(t, ii, jj);
+        const int i = k + 1 + ii, j = k + 1 + jj;
+        sh->H[i][j] -= sh->H[i][k] * sh->H[j][k];
+      }
+      __syncthreads();
+    }
+  }
+  if (lane == 0) {
+    const float lam = sh->lam;
+    for (int i = 0; i < na; ++i) {
+      float v = -sh->g[i];
+      for (int k = 0; k < i; ++k) v -= sh->H[i][k] * sh->dq[k];
+      sh->dq[i] = v / sh->H[i][i];
+    }
+    for (int i = na - 1; i >= 0; --i) {
+      float v = sh->dq[i];
+      for (int k = i + 1; k < na; ++k) v -= sh->H[k][i] * sh->dq[k];
+      sh->dq[i] = v / sh->H[i][i];
+    }
+    float p = 0.f;
+    for (int i = 0; i < na; ++i) p += sh->dq[i] * (lam * sh->dq[i] - sh->g[i]);
+    sh->pred = 0.5f * p;
+  }
+  __syncthreads();
+}
+
+// Jacobian columns of the pose + position rows at the current point (after residual_rows)
+template <int G, class C>
+__device__ __forceinline__ void jacobian_columns(const Robot* rb, const Terms* tm, Shared<C>* sh, float (*T)[8],
+                                                 int lane) {
+  const int na = rb->na;
+  const int nitems = (tm->n_pose + tm->n_pos) * na;
+  auto S_at = [&](int j, int r) { return sh->S[j][r]; };
+  for (int k = lane; k < nitems; k += G) {
+    const int t = k / na, a = k - t * na;
+    float sc[6];
+    if (t < tm->n_pose) {
+      spatial_column(rb, tm->pose_link[t], a, S_at, sc);
+      float c6[6];
+      pose_jac_column(sh->tg[t], sc, c6);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) sh->J[6 * t + i][a] = c6[i];
+    } else {
+      const int u = t - tm->n_pose;
+      const int link = tm->pos_link[u];
+      spatial_column(rb, link, a, S_at, sc);
+      Pose Tf = link_pose<C>(rb, T, link);
+      float c3[3];
+      pos_jac_column(sc, Tf.t, tm->pos_w[u], c3);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) sh->J[tm->Rf + 3 * u + i][a] = c3[i];
+    }
+  }
+}
+
+// The whole fixed-iteration solve of the problems first .. first + 64/G - 1 of P.
+template <int G, class C>
+__device__ __forceinline__ void solve_tile(const Problems& P, int first, const LMParams& lp, int it0, int iters,
+                                           char* lds) {
+  constexpr int PPW = WAVE / G;
+  const int lane = threadIdx.x & (G - 1);
+  const int slot = threadIdx.x / G;
+  Shared<C>* sh = reinterpret_cast<Shared<C>*>(lds) + slot;
+  const Robot* rb = P.robot;
+  const Terms* tm = P.terms;
+  const int nj = rb->nj, na = rb->na;
+  const int idx = first + slot;
+  const bool valid = idx < P.n;
+  const int s = valid ? idx : P.n - 1;
+  const int S = P.stride;
+  (void)PPW;
+
+  for (int d = lane; d < na; d += G) {
+    sh->q[d] = (it0 == 0 && P.q_seed) ? P.q_seed[d * S + s] : P.q[d * S + s];
+    if (tm->smooth) sh->qprev[d] = P.q_prev[d * S + s];
+  }
+  for (int k = lane; k < tm->n_pose * 7; k += G) sh->tp[k / 7][k % 7] = P.pose_tgt[k * S + s];
+  for (int k = lane; k < tm->n_pos * 3; k += G) sh->tq[k / 3][k % 3] = P.pos_tgt[k * S + s];
+  if (lane == 0) {
+    sh->lam = it0 == 0 ? lp.lm_lambda : P.lam[s];
+    sh->nacc = it0 == 0 ? 0 : P.nacc[s];
+    sh->cur = 0;
+  }
+  __syncthreads();
+  fk_levels<G, C>(rb, sh->q, sh->Tw[0], lane);
+
+  for (int it = 0; it < iters; ++it) {
+    const int cur = sh->cur;
+    float(*T)[8] = sh->Tw[cur];
+    if (valid && P.q_trace && it0 + it < P.trace_iters) {
+      const int gi = it0 + it;
+      for (int d = lane; d < na; d += G) P.q_trace[(gi * na + d) * S + s] = sh->q[d];
+      if (lane == 0) P.lam_trace[gi * S + s] = sh->lam;
+    }
+    for (int j = lane; j < nj; j += G) {
+      Pose Tj;
+      load_pose(T[j], Tj);
+      subspace_column(rb, j, Tj, sh->S[j]);
+    }
+    residual_rows<G, C>(rb, tm, sh, T, sh->q, lane, true);
+    __syncthreads();
+    jacobian_columns<G, C>(rb, tm, sh, T, lane);
+    __syncthreads();
+    normal_equations<G, C>(tm, sh, na, lane);
+    __syncthreads();
+    chol_solve<G, C>(sh, na, lane);
+    for (int d = lane; d < na; d += G) sh->qp[d] = sh->q[d] + sh->dq[d];
+    __syncthreads();
+    fk_levels<G, C>(rb, sh->qp, sh->Tw[1 - cur], lane);
+    residual_rows<G, C>(rb, tm, sh, sh->Tw[1 - cur], sh->qp, lane, false);
+    __syncthreads();
+    if (lane == 0) {
+      const float cp = term_cost(tm, na, sh->r);
+      float cost = sh->cost, lam = sh->lam;
+      const int gi = it0 + it;
+      if (valid && P.trace && gi == 0) P.trace[s] = cost;
+      const bool acc = lm_accept(lp, cost, cp, sh->pred, lam);
+      sh->cost = cost;
+      sh->lam = lam;
+      sh->acc = acc;
+      if (acc) {
+        sh->cur = 1 - cur;
+        sh->nacc += 1;
+      }
+      if (valid && P.trace && gi < P.trace_iters) {
+        P.trace[(gi + 1) * S + s] = cost;
+        P.acc_trace[gi * S + s] = acc;
+      }
+    }
+    __syncthreads();
+    if (sh->acc)
+      for (int d = lane; d < na; d += G) sh->q[d] = sh->qp[d];
+    __syncthreads();
+  }
+  if (valid) {
+    for (int d = lane; d < na; d += G) P.q[d * S + s] = sh->q[d];
+    if (lane == 0) {
+      if (iters > 0) P.cost[s] = sh->cost;
+      P.lam[s] = sh->lam;
+      P.nacc[s] = sh->nacc;
+    }
+  }
+  __syncthreads();
+}
+
+template <int G, class C>
+__global__ __launch_bounds__(64) void k_fused(Problems P, LMParams lp, int it0, int iters) {
+  extern __shared__ __align__(16) char lds[];
+  solve_tile<G, C>(P, blockIdx.x * (WAVE / G), lp, it0, iters, lds);
+}
+
+// variants of (threads per problem, LDS configuration)
+enum Variant { V_S4 = 0, V_S8, V_S16, V_L16, V_L32, V_L64, NVARIANT };
+__host__ __device__ constexpr int variant_G(int v) {
+  return v == V_S4 ? 4 : v == V_S8 ? 8 : v == V_S16 ? 16 : v == V_L16 ? 16 : v == V_L32 ? 32 : 64;
+}
+__host__ __device__ constexpr bool variant_small(int v) { return v <= V_S16; }
+inline size_t variant_lds(int v) {
+  return variant_small(v) ? (WAVE / variant_G(v)) * sizeof(Shared<CfgS>)
+                          : (WAVE / variant_G(v)) * sizeof(Shared<CfgL>);
+}
+
+struct Bucket {
+  Problems P;
+  int variant;
+  int tile_base;  // first global tile of this bucket
+  int tiles;
+  int pad;
+};
+
+// persistent uneven-batch kernel: each workgroup (one wavefront) pulls tiles from the queue
+__global__ __launch_bounds__(64) void k_persistent(const Bucket* __restrict__ buckets, int nb, int total_tiles,
+                                                   int* counter, LMParams lp, int it0, int iters) {
+  extern __shared__ __align__(16) char lds[];
+  __shared__ int tile_s;
+  for (;;) {
+    if (threadIdx.x == 0) tile_s = atomicAdd(counter, 1);
+    __syncthreads();
+    const int tile = tile_s;
+    __syncthreads();
+    if (tile >= total_tiles) break;
+    int b = 0;
+    while (b + 1 < nb && tile >= buckets[b + 1].tile_base) ++b;
+    const Bucket& bk = buckets[b];
+    const int local = tile - bk.tile_base;
+    switch (bk.variant) {
+      case V_S4: solve_tile<4, CfgS>(bk.P, local * 16, lp, it0, iters, lds); break;
+      case V_S8: solve_tile<8, CfgS>(bk.P, local * 8, lp, it0, iters, lds); break;
+      case V_S16: solve_tile<16, CfgS>(bk.P, local * 4, lp, it0, iters, lds); break;
+      case V_L16: solve_tile<16, CfgL>(bk.P, local * 4, lp, it0, iters, lds); break;
+      case V_L32: solve_tile<32, CfgL>(bk.P, local * 2, lp, it0, iters, lds); break;
+      default: solve_tile<64, CfgL>(bk.P, local * 1, lp, it0, iters, lds); break;
+    }
+  }
+}
+
+__global__ void k_reset_counter(int* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// plain form: the reference's decomposition (one thread per element, J in HBM)
+// ---------------------------------------------------------------------------------------------
+struct PlainBufs {
+  float* Tj;    // [nj*7][S] world joint poses
+  float* Sw;    // [nj*6][S] motion subspace
+  float* J;     // [R*na][S] weighted Jacobian
+  float* r;     // [R][S] residual at q
+  float* rp;    // [R][S] residual at the proposal
+  float* qp;    // [na][S]
+  float* dq;    // [na][S]
+  float* pred;  // [S]
+  int* acc;     // [S]
+};
+
+__device__ __forceinline__ Pose plain_pose(const float* Tj, int j, int S, int b) {
+  Pose P;
+  if (j < 0) {
+    P.t[0] = P.t[1] = P.t[2] = 0.f;
+    P.q[0] = 1.f; P.q[1] = P.q[2] = P.q[3] = 0.f;
+    return P;
+  }
+  P.t[0] = Tj[(j * 7 + 0) * S + b]; P.t[1] = Tj[(j * 7 + 1) * S + b]; P.t[2] = Tj[(j * 7 + 2) * S + b];
+  P.q[0] = Tj[(j * 7 + 3) * S + b]; P.q[1] = Tj[(j * 7 + 4) * S + b];
+  P.q[2] = Tj[(j * 7 + 5) * S + b]; P.q[3] = Tj[(j * 7 + 6) * S + b];
+  return P;
+}
+
+// FK (+ motion subspace) one thread per problem, joints in depth order
+__global__ void k_plain_fk(Problems P, const float* __restrict__ qsrc, float* Tj, float* Sw) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.n) return;
+  const Robot* rb = P.robot;
+  const int S = P.stride, na = rb->na;
+  float q[RK_NA_MAX];
+  for (int d = 0; d < na; ++d) q[d] = qsrc[d * S + b];
+  for (int k = 0; k < rb->nj; ++k) {
+    const int j = rb->order[k];
+    Pose X = joint_local(rb, j, q);
+    const int p = rb->parent[j];
+    if (p >= 0) X = compose(plain_pose(Tj, p, S, b), X);
+    const float v[7] = {X.t[0], X.t[1], X.t[2], X.q[0], X.q[1], X.q[2], X.q[3]};
+    for (int c = 0; c < 7; ++c) Tj[(j * 7 + c) * S + b] = v[c];
+    if (Sw) {
+      float s6[6];
+      subspace_column(rb, j, X, s6);
+      for (int r = 0; r < 6; ++r) Sw[(j * 6 + r) * S + b] = s6[r];
+    }
+  }
+}
+
+__global__ void k_fill(float* p, long n, float v) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// pose residual rows, thread per (problem, target)
+__global__ void k_plain_pose_res(Problems P, const float* Tj, float* r) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.y;
+  if (b >= P.n) return;
+  const Terms* tm = P.terms;
+  const int S = P.stride;
+  Pose Tf = plain_pose(Tj, P.robot->link_joint[tm->pose_link[t]], S, b);
+  Pose Tt;
+  float tv[7];
+  for (int c = 0; c < 7; ++c) tv[c] = P.pose_tgt[(t * 7 + c) * S + b];
+  load_pose(tv, Tt);
+  float r6[6];
+  pose_target(Tf, Tt, tm->pose_w[t], r6, nullptr);
+  for (int i = 0; i < 6; ++i) r[(6 * t + i) * S + b] = r6[i];
+}
+
+// pose Jacobian, thread per (problem, actuated dof), loops targets (frame_task.py:557-621)
+__global__ void k_plain_pose_jac(Problems P, const float* Tj, const float* Sw, float* J) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y;
+  if (b >= P.n) return;
+  const Robot* rb = P.robot;
+  const Terms* tm = P.terms;
+  const int S = P.stride, na = rb->na;
+  auto S_at = [&](int j, int r) { return Sw[(j * 6 + r) * S + b]; };
+  for (int t = 0; t < tm->n_pose; ++t) {
+    Pose Tf = plain_pose(Tj, rb->link_joint[tm->pose_link[t]], S, b);
+    float tv[7];
+    for (int c = 0; c < 7; ++c) tv[c] = P.pose_tgt[(t * 7 + c) * S + b];
+    Pose Tt;
+    load_pose(tv, Tt);
+    float r6[6], tg[36];
+    pose_target(Tf, Tt, tm->pose_w[t], r6, tg);
+    float sc[6], c6[6];
+    spatial_column(rb, tm->pose_link[t], a, S_at, sc);
+    pose_jac_column(tg, sc, c6);
+    for (int i = 0; i < 6; ++i) J[((6 * t + i) * na + a) * S + b] = c6[i];
+  }
+}
+
+__global__ void k_plain_pos_res(Problems P, const float* Tj, float* r) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int u = blockIdx.y;
+  if (b >= P.n) return;
+  const Terms* tm = P.terms;
+  const int S = P.stride;
+  Pose Tf = plain_pose(Tj, P.robot->link_joint[tm->pos_link[u]], S, b);
+  for (int i = 0; i < 3; ++i)
+    r[(tm->Rf + 3 * u + i) * S + b] = tm->pos_w[u][i] * (P.pos_tgt[(u * 3 + i) * S + b] - Tf.t[i]);
+}
+
+__global__ void k_plain_pos_jac(Problems P, const float* Tj, const float* Sw, float* J) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y;
+  if (b >= P.n) return;
+  const Robot* rb = P.robot;
+  const Terms* tm = P.terms;
+  const int S = P.stride, na = rb->na;
+  auto S_at = [&](int j, int r) { return Sw[(j * 6 + r) * S + b]; };
+  for (int u = 0; u < tm->n_pos; ++u) {
+    Pose Tf = plain_pose(Tj, rb->link_joint[tm->pos_link[u]], S, b);
+    float sc[6], c3[3];
+    spatial_column(rb, tm->pos_link[u], a, S_at, sc);
+    pos_jac_column(sc, Tf.t, tm->pos_w[u], c3);
+    for (int i = 0; i < 3; ++i) J[((tm->Rf + 3 * u + i) * na + a) * S + b] = c3[i];
+  }
+}
+
+// limit + smoothness rows (and their diagonal Jacobian), thread per (problem, dof)
+__global__ void k_plain_diag_rows(Problems P, const float* qsrc, float* r, float* J) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = blockIdx.y;
+  if (b >= P.n) return;
+  const Robot* rb = P.robot;
+  const Terms* tm = P.terms;
+  const int S = P.stride, na = rb->na;
+  const float x = qsrc[d * S + b];
+  int off = tm->Rf + tm->Rq;
+  if (tm->limit) {
+    float jd;
+    r[(off + d) * S + b] = limit_row(x, rb->lo[d], rb->hi[d], tm->limit_w[d], &jd);
+    if (J) J[((off + d) * na + d) * S + b] = jd;
+    off += na;
+  }
+  if (tm->smooth) {
+    r[(off + d) * S + b] = tm->smooth_w[d] * (x - P.q_prev[d * S + b]);
+    if (J) J[((off + d) * na + d) * S + b] = tm->smooth_w[d];
+  }
+}
+
+// the solve kernel of the plain form (the reference's _solve_kernel, lm_optimizer.py:145-189):
+// J, r from HBM -> H, g, cost -> damped Cholesky -> delta, pred.  One problem per G lanes.
+template <int G, class C>
+__global__ __launch_bounds__(64) void k_plain_solve(Problems P, PlainBufs B) {
+  extern __shared__ __align__(16) char lds[];
+  const int lane = threadIdx.x & (G - 1);
+  const int slot = threadIdx.x / G;
+  Shared<C>* sh = reinterpret_cast<Shared<C>*>(lds) + slot;
+  const Robot* rb = P.robot;
+  const Terms* tm = P.terms;
+  const int na = rb->na, S = P.stride;
+  const int idx = blockIdx.x * (WAVE / G) + slot;
+  const bool valid = idx < P.n;
+  const int s = valid ? idx : P.n - 1;
+  const int Rp = tm->Rf + tm->Rq;
+  const int R = tm->R;
+  for (int k = lane; k < Rp * na; k += G) sh->J[k / na][k % na] = B.J[k * S + s];
+  for (int k = lane; k < R; k += G) sh->r[k] = B.r[k * S + s];
+  const int off_l = Rp, off_s = Rp + (tm->limit ? na : 0);
+  for (int d = lane; d < na; d += G) {
+    if (tm->limit) sh->jl[d] = B.J[((off_l + d) * na + d) * S + s];
+    if (tm->smooth) sh->js[d] = B.J[((off_s + d) * na + d) * S + s];
+  }
+  if (lane == 0) sh->lam = P.lam[s];
+  __syncthreads();
+  normal_equations<G, C>(tm, sh, na, lane);
+  __syncthreads();
+  chol_solve<G, C>(sh, na, lane);
+  if (valid) {
+    for (int d = lane; d < na; d += G) B.dq[d * S + s] = sh->dq[d];
+    if (lane == 0) {
+      B.pred[s] = sh->pred;
+      P.cost[s] = sh->cost;
+    }
+  }
+}
+
+__global__ void k_plain_trace_state(Problems P, int it) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.n || it >= P.trace_iters) return;
+  const int S = P.stride, na = P.robot->na;
+  for (int d = 0; d < na; ++d) P.q_trace[(it * na + d) * S + b] = P.q[d * S + b];
+  P.lam_trace[it * S + b] = P.lam[b];
+}
+
+__global__ void k_plain_integrate(Problems P, PlainBufs B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = blockIdx.y;
+  if (b >= P.n) return;
+  const int S = P.stride;
+  B.qp[d * S + b] = P.q[d * S + b] + B.dq[d * S + b];
+}
+
+__global__ void k_plain_accept(Problems P, PlainBufs B, LMParams lp, int it) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.n) return;
+  const int S = P.stride, na = P.robot->na;
+  const Terms* tm = P.terms;
+  // proposed cost: per-term sums of the proposal's rows (strided layout)
+  float cp = 0.f;
+  int off = 0;
+  const int sizes[4] = {tm->Rf, tm->Rq, tm->limit ? na : 0, tm->smooth ? na : 0};
+  for (int t = 0; t < 4; ++t) {
+    if (!sizes[t]) continue;
+    float s = 0.f;
+    for (int k = 0; k < sizes[t]; ++k) {
+      const float v = B.rp[(off + k) * S + b];
+      s += v * v;
+    }
+    cp = cp + 0.5f * s;
+    off += sizes[t];
+  }
+  float cost = P.cost[b], lam = P.lam[b];
+  if (P.trace && it == 0) P.trace[b] = cost;
+  const bool acc = lm_accept(lp, cost, cp, B.pred[b], lam);
+  P.cost[b] = cost;
+  P.lam[b] = lam;
+  B.acc[b] = acc;
+  if (acc) P.nacc[b] += 1;
+  if (P.trace && it < P.trace_iters) {
+    P.trace[(it + 1) * S + b] = cost;
+    P.acc_trace[it * S + b] = acc;
+  }
+}
+
+__global__ void k_plain_accept_state(Problems P, PlainBufs B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = blockIdx.y;
+  if (b >= P.n) return;
+  const int S = P.stride;
+  if (B.acc[b]) P.q[d * S + b] = B.qp[d * S + b];
+}
+
+__global__ void k_init_lm(Problems P, float lam0) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.n) return;
+  P.lam[b] = lam0;
+  P.nacc[b] = 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// utility kernels: FK / link Jacobian / SE(3) ops / IK metrics
+// ---------------------------------------------------------------------------------------------
+__device__ void fk_thread(const Robot* rb, const float* q, Pose* Tw) {
+  for (int k = 0; k < rb->nj; ++k) {
+    const int j = rb->order[k];
+    Pose X = joint_local(rb, j, q);
+    const int p = rb->parent[j];
+    if (p >= 0) X = compose(Tw[p], X);
+    Tw[j] = X;
+  }
+}
+
+__global__ void k_fk_host_layout(const Robot* rb, int B, const float* q, float* T_link) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  Pose Tw[RK_NJ_MAX];
+  fk_thread(rb, q + (size_t)b * rb->na, Tw);
+  for (int l = 0; l < rb->nl; ++l) {
+    const int j = rb->link_joint[l];
+    Pose P;
+    if (j >= 0) P = Tw[j];
+    else {
+      P.t[0] = P.t[1] = P.t[2] = 0.f;
+      P.q[0] = 1.f; P.q[1] = P.q[2] = P.q[3] = 0.f;
+    }
+    store_pose(T_link + ((size_t)b * rb->nl + l) * 7, P);
+  }
+}
+
+__global__ void k_link_jacobian(const Robot* rb, int B, const float* q, int link, int frame, float* J) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  Pose Tw[RK_NJ_MAX];
+  fk_thread(rb, q + (size_t)b * rb->na, Tw);
+  const int na = rb->na;
+  const int jl = rb->link_joint[link];
+  Pose Tf;
+  if (jl >= 0) Tf = Tw[jl];
+  else {
+    Tf.t[0] = Tf.t[1] = Tf.t[2] = 0.f;
+    Tf.q[0] = 1.f; Tf.q[1] = Tf.q[2] = Tf.q[3] = 0.f;
+  }
+  Pose Tfi = inverse(Tf);
+  Mat3 R = quat_to_mat(Tfi.q);
+  auto S_at = [&](int j, int r) {
+    float s6[6];
+    subspace_column(rb, j, Tw[j], s6);
+    return s6[r];
+  };
+  for (int a = 0; a < na; ++a) {
+    float sc[6], col[6];
+    spatial_column(rb, link, a, S_at, sc);
+    if (frame == 1) {
+      float Rv[3], Rw[3], txRw[3];
+      mat3_vec(R, sc, Rv);
+      mat3_vec(R, sc + 3, Rw);
+      cross3(Tfi.t, Rw, txRw);
+      for (int i = 0; i < 3; ++i) {
+        col[i] = Rv[i] + txRw[i];
+        col[3 + i] = Rw[i];
+      }
+    } else {
+      for (int i = 0; i < 6; ++i) col[i] = sc[i];
+    }
+    for (int i = 0; i < 6; ++i) J[((size_t)b * 6 + i) * na + a] = col[i];
+  }
+}
+
+__global__ void k_se3_op(int op, int n, const float* a, const float* bb, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Pose A, Bp;
+  load_pose(a + 7 * i, A);
+  if (op == 0) {
+    load_pose(bb + 7 * i, Bp);
+    store_pose(out + 7 * i, compose(A, Bp));
+  } else if (op == 1) {
+    store_pose(out + 7 * i, inverse(A));
+  } else if (op == 2) {
+    se3_log(A, out + 6 * i);
+  } else if (op == 3) {
+    Mat3 MA, MB;
+    se3_jlog_blocks(A, MA, MB);
+    float* o = out + 36 * i;
+    for (int k = 0; k < 36; ++k) o[k] = 0.f;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        o[r * 6 + c] = MA.m[r * 3 + c];
+        o[r * 6 + 3 + c] = MB.m[r * 3 + c];
+        o[(r + 3) * 6 + 3 + c] = MA.m[r * 3 + c];
+      }
+  } else if (op == 4) {
+    Mat3 R = quat_to_mat(A.q);
+    float* o = out + 36 * i;
+    for (int k = 0; k < 36; ++k) o[k] = 0.f;
+    for (int c = 0; c < 3; ++c) {
+      float col[3] = {R.m[c], R.m[3 + c], R.m[6 + c]};
+      float tx[3];
+      cross3(A.t, col, tx);
+      for (int r = 0; r < 3; ++r) {
+        o[r * 6 + c] = R.m[r * 3 + c];
+        o[r * 6 + 3 + c] = tx[r];
+        o[(r + 3) * 6 + 3 + c] = R.m[r * 3 + c];
+      }
+    }
+  } else if (op == 5) {
+    float p[3];
+    quat_apply(A.q, bb + 3 * i, p);
+    for (int k = 0; k < 3; ++k) out[3 * i + k] = p[k] + A.t[k];
+  }
+}
+
+// _ik_metrics_kernel (helpers/ik/ik.py:59-82)
+__global__ void k_ik_metrics(Problems P, float pos_thr, float ang_thr, float* pos_err, float* ang_err, int* success) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.n) return;
+  const Robot* rb = P.robot;
+  const Terms* tm = P.terms;
+  const int S = P.stride;
+  float q[RK_NA_MAX];
+  for (int d = 0; d < rb->na; ++d) q[d] = P.q[d * S + b];
+  Pose Tw[RK_NJ_MAX];
+  fk_thread(rb, q, Tw);
+  int reached = 1;
+  for (int t = 0; t < tm->n_pose; ++t) {
+    const int j = rb->link_joint[tm->pose_link[t]];
+    Pose Tf;
+    if (j >= 0) Tf = Tw[j];
+    else {
+      Tf.t[0] = Tf.t[1] = Tf.t[2] = 0.f;
+      Tf.q[0] = 1.f; Tf.q[1] = Tf.q[2] = Tf.q[3] = 0.f;
+    }
+    float tv[7];
+    for (int c = 0; c < 7; ++c) tv[c] = P.pose_tgt[(t * 7 + c) * S + b];
+    Pose Tt;
+    load_pose(tv, Tt);
+    Pose E = compose(inverse(Tf), Tt);
+    const float pe = sqrtf(E.t[0] * E.t[0] + E.t[1] * E.t[1] + E.t[2] * E.t[2]);
+    const float v = sqrtf(E.q[1] * E.q[1] + E.q[2] * E.q[2] + E.q[3] * E.q[3]);
+    const float ae = 2.f * asinf(fminf(v, 1.f));
+    pos_err[(size_t)b * tm->n_pose + t] = pe;
+    ang_err[(size_t)b * tm->n_pose + t] = ae;
+    if (pe >= pos_thr || ae >= ang_thr) reached = 0;
+  }
+  success[b] = reached;
+}
+
+}  // namespace rk
+
+// =============================================================================================
+// host side
+// =============================================================================================
+using namespace rk;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(RK_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+struct rk_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::vector<Robot> robots;
+  Robot* robots_dev = nullptr;
+  int cu_count = 0;
+  int64_t mem_bytes = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct rk_batch {
+  rk_context* ctx = nullptr;
+  Terms th{};
+  Terms* td = nullptr;
+  int capacity = 0, n = 0, trace_iters = 0, na = 0;
+  float *q = nullptr, *q_seed = nullptr, *pose_tgt = nullptr, *pos_tgt = nullptr, *q_prev = nullptr;
+  float *cost = nullptr, *lam = nullptr, *trace = nullptr;
+  int *nacc = nullptr, *acc_trace = nullptr;
+  float *q_trace = nullptr, *lam_trace = nullptr;
+  PlainBufs pb{};
+  bool plain_alloc = false;
+  int64_t bytes = 0;
+};
+
+struct rk_group {
+  rk_context* ctx = nullptr;
+  std::vector<rk_batch*> batches;
+  std::vector<int> tpp;  // chosen threads per problem, 0 = default
+  int wg_per_cu = 0, iters_per_launch = 0;
+  Bucket* buckets_dev = nullptr;
+  int* counter = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_iters = -1;
+  std::vector<float> graph_key;
+  int64_t bytes = 0;
+};
+
+static int dmalloc(rk_context* ctx, void** p, size_t bytes, int64_t* acct) {
+  if (bytes == 0) bytes = 4;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(RK_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  ctx->mem_bytes += (int64_t)bytes;
+  if (acct) *acct += (int64_t)bytes;
+  return RK_OK;
+}
+
+static void dfree(rk_context* ctx, void* p, size_t bytes) {
+  if (!p) return;
+  hipFree(p);
+  ctx->mem_bytes -= (int64_t)std::max<size_t>(bytes, 4);
+}
+
+static Problems make_problems(rk_batch* b, bool from_seed = false) {
+  Problems P;
+  P.robot = b->ctx->robots_dev + b->th.robot;
+  P.terms = b->td;
+  P.q = b->q;
+  P.q_seed = from_seed ? b->q_seed : nullptr;
+  P.pose_tgt = b->pose_tgt;
+  P.pos_tgt = b->pos_tgt;
+  P.q_prev = b->q_prev;
+  P.cost = b->cost;
+  P.lam = b->lam;
+  P.nacc = b->nacc;
+  P.trace = b->trace;
+  P.acc_trace = b->acc_trace;
+  P.q_trace = b->q_trace;
+  P.lam_trace = b->lam_trace;
+  P.trace_iters = b->trace_iters;
+  P.n = b->n;
+  P.stride = b->capacity;
+  return P;
+}
+
+static bool fits_small(const rk_batch* b, const Robot& r) {
+  return r.nj <= CfgS::nj && r.na <= CfgS::na && b->th.n_pose <= CfgS::np && b->th.n_pos <= CfgS::nq;
+}
+
+static int pick_variant(const rk_batch* b, int G, int* v) {
+  const Robot& r = b->ctx->robots[b->th.robot];
+  const bool small = fits_small(b, r);
+  if (G == 0) G = small ? 8 : 64;
+  if (small && (G == 4 || G == 8 || G == 16)) {
+    *v = G == 4 ? V_S4 : G == 8 ? V_S8 : V_S16;
+    return RK_OK;
+  }
+  if (G == 16 || G == 32 || G == 64) {
+    *v = G == 16 ? V_L16 : G == 32 ? V_L32 : V_L64;
+    return RK_OK;
+  }
+  return fail(RK_EINVAL, "threads_per_problem must be 4, 8, 16 (small robots) or 16, 32, 64");
+}
+
+static void legal_tpps(const rk_batch* b, std::vector<int>& out) {
+  if (fits_small(b, b->ctx->robots[b->th.robot])) out = {4, 8, 16, 32, 64};
+  else out = {16, 32, 64};
+}
+
+template <int G, class C>
+static int launch_fused(rk_context* ctx, const Problems& P, const LMParams& lp, int it0, int iters) {
+  const size_t lds = (WAVE / G) * sizeof(Shared<C>);
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIP_TRY(hipFuncSetAttribute((const void*)k_fused<G, C>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set = true;
+  }
+  const int grid = (P.n + (WAVE / G) - 1) / (WAVE / G);
+  hipLaunchKernelGGL((k_fused<G, C>), dim3(grid), dim3(WAVE), lds, ctx->stream, P, lp, it0, iters);
+  HIP_TRY(hipGetLastError());
+  return RK_OK;
+}
+
+static int launch_variant(rk_context* ctx, int v, const Problems& P, const LMParams& lp, int it0, int iters) {
+  switch (v) {
+    case V_S4: return launch_fused<4, CfgS>(ctx, P, lp, it0, iters);
+    case V_S8: return launch_fused<8, CfgS>(ctx, P, lp, it0, iters);
+    case V_S16: return launch_fused<16, CfgS>(ctx, P, lp, it0, iters);
+    case V_L16: return launch_fused<16, CfgL>(ctx, P, lp, it0, iters);
+    case V_L32: return launch_fused<32, CfgL>(ctx, P, lp, it0, iters);
+    default: return launch_fused<64, CfgL>(ctx, P, lp, it0, iters);
+  }
+}
+
+static LMParams to_lp(const rk_lm_params* p) {
+  LMParams lp;
+  lp.lm_lambda = p->lm_lambda;
+  lp.lambda_factor = p->lambda_factor;
+  lp.lambda_min = p->lambda_min;
+  lp.lambda_max = p->lambda_max;
+  lp.rho_min = p->rho_min;
+  lp.gain_ratio_epsilon = p->gain_ratio_epsilon;
+  return lp;
+}
+
+static int alloc_plain(rk_batch* b) {
+  if (b->plain_alloc) return RK_OK;
+  rk_context* ctx = b->ctx;
+  const Robot& r = ctx->robots[b->th.robot];
+  const size_t S = b->capacity;
+  int rc;
+  if ((rc = dmalloc(ctx, (void**)&b->pb.Tj, sizeof(float) * r.nj * 7 * S, &b->bytes))) return rc;
+  if ((rc = dmalloc(ctx, (void**)&b->pb.Sw, sizeof(float) * r.nj * 6 * S, &b->bytes))) return rc;
+  if ((rc = dmalloc(ctx, (void**)&b->pb.J, sizeof(float) * b->th.R * r.na * S, &b->bytes))) return rc;
+  if ((rc = dmalloc(ctx, (void**)&b->pb.r, sizeof(float) * b->th.R * S, &b->bytes))) return rc;
+  if ((rc = dmalloc(ctx, (void**)&b->pb.rp, sizeof(float) * b->th.R * S, &b->bytes))) return rc;
+  if ((rc = dmalloc(ctx, (void**)&b->pb.qp, sizeof(float) * r.na * S, &b->bytes))) return rc;
+  if ((rc = dmalloc(ctx, (void**)&b->pb.dq, sizeof(float) * r.na * S, &b->bytes))) return rc;
+  if ((rc = dmalloc(ctx, (void**)&b->pb.pred, sizeof(float) * S, &b->bytes))) return rc;
+  if ((rc = dmalloc(ctx, (void**)&b->pb.acc, sizeof(int) * S, &b->bytes))) return rc;
+  b->plain_alloc = true;
+  return RK_OK;
+}
+
+template <int G, class C>
+static int launch_plain_solve(rk_context* ctx, const Problems& P, const PlainBufs& B) {
+  const size_t lds = (WAVE / G) * sizeof(Shared<C>);
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIP_TRY(hipFuncSetAttribute((const void*)k_plain_solve<G, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+    attr_set = true;
+  }
+  const int grid = (P.n + (WAVE / G) - 1) / (WAVE / G);
+  hipLaunchKernelGGL((k_plain_solve<G, C>), dim3(grid), dim3(WAVE), lds, ctx->stream, P, B);
+  return RK_OK;
+}
+
+// the reference's GPU decomposition, one host-loop iteration = 13 launches for IK terms
+static int solve_plain(rk_batch* b, int iters, const LMParams& lp, bool from_seed) {
+  int rc;
+  if ((rc = alloc_plain(b))) return rc;
+  rk_context* ctx = b->ctx;
+  if (from_seed)
+    HIP_TRY(hipMemcpyAsync(b->q, b->q_seed, sizeof(float) * b->na * b->capacity, hipMemcpyDeviceToDevice,
+                           ctx->stream));
+  Problems P = make_problems(b);
+  const Robot& rob = ctx->robots[b->th.robot];
+  const int na = rob.na, S = b->capacity;
+  const dim3 blk(256);
+  const dim3 g1((b->n + 255) / 256);
+  const bool small = fits_small(b, rob);
+  hipLaunchKernelGGL(k_init_lm, g1, blk, 0, ctx->stream, P, lp.lm_lambda);
+  for (int it = 0; it < iters; ++it) {
+    if (P.q_trace) hipLaunchKernelGGL(k_plain_trace_state, g1, blk, 0, ctx->stream, P, it);
+    hipLaunchKernelGGL(k_plain_fk, g1, blk, 0, ctx->stream, P, (const float*)b->q, b->pb.Tj, b->pb.Sw);
+    const long nJ = (long)b->th.R * na * S;
+    hipLaunchKernelGGL(k_fill, dim3((nJ + 255) / 256), blk, 0, ctx->stream, b->pb.J, nJ, 0.f);
+    if (b->th.n_pose) {
+      hipLaunchKernelGGL(k_plain_pose_res, dim3(g1.x, b->th.n_pose), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
+                         b->pb.r);
+      hipLaunchKernelGGL(k_plain_pose_jac, dim3(g1.x, na), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
+                         (const float*)b->pb.Sw, b->pb.J);
+    }
+    if (b->th.n_pos) {
+      hipLaunchKernelGGL(k_plain_pos_res, dim3(g1.x, b->th.n_pos), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
+                         b->pb.r);
+      hipLaunchKernelGGL(k_plain_pos_jac, dim3(g1.x, na), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
+                         (const float*)b->pb.Sw, b->pb.J);
+    }
+    if (b->th.limit || b->th.smooth)
+      hipLaunchKernelGGL(k_plain_diag_rows, dim3(g1.x, na), blk, 0, ctx->stream, P, (const float*)b->q, b->pb.r,
+                         b->pb.J);
+    if (small) {
+      if ((rc = launch_plain_solve<8, CfgS>(ctx, P, b->pb))) return rc;
+    } else {
+      if ((rc = launch_plain_solve<64, CfgL>(ctx, P, b->pb))) return rc;
+    }
+    hipLaunchKernelGGL(k_plain_integrate, dim3(g1.x, na), blk, 0, ctx->stream, P, b->pb);
+    hipLaunchKernelGGL(k_plain_fk, g1, blk, 0, ctx->stream, P, (const float*)b->pb.qp, b->pb.Tj, (float*)nullptr);
+    if (b->th.n_pose)
+      hipLaunchKernelGGL(k_plain_pose_res, dim3(g1.x, b->th.n_pose), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
+                         b->pb.rp);
+    if (b->th.n_pos)
+      hipLaunchKernelGGL(k_plain_pos_res, dim3(g1.x, b->th.n_pos), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
+                         b->pb.rp);
+    if (b->th.limit || b->th.smooth)
+      hipLaunchKernelGGL(k_plain_diag_rows, dim3(g1.x, na), blk, 0, ctx->stream, P, (const float*)b->pb.qp,
+                         b->pb.rp, (float*)nullptr);
+    hipLaunchKernelGGL(k_plain_accept, g1, blk, 0, ctx->stream, P, b->pb, lp, it);
+    hipLaunchKernelGGL(k_plain_accept_state, dim3(g1.x, na), blk, 0, ctx->stream, P, b->pb);
+  }
+  HIP_TRY(hipGetLastError());
+  return RK_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------------
+extern "C" {
+
+const char* rk_last_error(void) { return g_err.c_str(); }
+int32_t rk_abi_version(void) { return 1; }
+
+int rk_context_create(int32_t device, rk_context** out) {
+  if (!out) return fail(RK_EINVAL, "out is null");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(RK_EINVAL, "device ordinal out of range");
+  HIP_TRY(hipSetDevice(device));
+  rk_context* c = new rk_context();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(RK_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  c->cu_count = prop.multiProcessorCount;
+  HIP_TRY(hipEventCreate(&c->ev0));
+  HIP_TRY(hipEventCreate(&c->ev1));
+  int rc = dmalloc(c, (void**)&c->robots_dev, sizeof(Robot) * RK_MAX_ROBOTS, nullptr);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return RK_OK;
+}
+
+int rk_context_destroy(rk_context* ctx) {
+  if (!ctx) return RK_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  hipFree(ctx->robots_dev);
+  hipEventDestroy(ctx->ev0);
+  hipEventDestroy(ctx->ev1);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return RK_OK;
+}
+
+int rk_context_sync(rk_context* ctx) {
+  if (!ctx) return fail(RK_EINVAL, "null context");
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return RK_OK;
+}
+
+int rk_context_mem(rk_context* ctx, int64_t* bytes) {
+  if (!ctx || !bytes) return fail(RK_EINVAL, "null argument");
+  *bytes = ctx->mem_bytes;
+  return RK_OK;
+}
+
+int rk_robot_upload(rk_context* ctx, const rk_robot_desc* d, int32_t* robot_id) {
+  if (!ctx || !d || !robot_id) return fail(RK_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if ((int)ctx->robots.size() >= RK_MAX_ROBOTS) return fail(RK_ENOMEM, "too many robots on this context");
+  if (d->nj < 1 || d->nj > RK_NJ_MAX || d->nl < 1 || d->nl > RK_NL_MAX || d->na < 1 || d->na > RK_NA_MAX)
+    return fail(RK_EINVAL, "robot size out of range (nj, nl <= 64, 1 <= na <= 32)");
+  Robot r;
+  memset(&r, 0, sizeof(r));
+  r.nj = d->nj;
+  r.nl = d->nl;
+  r.na = d->na;
+  for (int j = 0; j < d->nj; ++j) {
+    r.joint_type[j] = d->joint_type[j];
+    r.parent[j] = d->parent_joint[j];
+    r.act[j] = d->act_idx[j];
+    r.mimic[j] = d->mimic_idx[j];
+    r.mul[j] = d->mimic_mul[j];
+    r.off[j] = d->mimic_off[j];
+    for (int k = 0; k < 3; ++k) r.axis[j][k] = d->joint_axis[3 * j + k];
+    for (int k = 0; k < 7; ++k) r.pT[j][k] = d->parent_T[7 * j + k];
+    if (r.parent[j] < -1 || r.parent[j] >= d->nj || r.act[j] >= d->na || r.mimic[j] >= d->na ||
+        r.joint_type[j] < 0 || r.joint_type[j] > 2)
+      return fail(RK_EINVAL, "invalid joint index or type");
+  }
+  for (int l = 0; l < d->nl; ++l) {
+    r.link_joint[l] = d->link_parent_joint[l];
+    if (r.link_joint[l] < -1 || r.link_joint[l] >= d->nj) return fail(RK_EINVAL, "invalid link parent joint");
+  }
+  for (int a = 0; a < d->na; ++a) {
+    r.lo[a] = d->limits[2 * a];
+    r.hi[a] = d->limits[2 * a + 1];
+  }
+  // depth order
+  std::vector<int> depth(d->nj, -1);
+  for (int pass = 0; pass < d->nj + 1; ++pass) {
+    bool changed = false;
+    for (int j = 0; j < d->nj; ++j) {
+      if (depth[j] >= 0) continue;
+      const int p = r.parent[j];
+      if (p < 0) depth[j] = 0, changed = true;
+      else if (depth[p] >= 0) depth[j] = depth[p] + 1, changed = true;
+    }
+    if (!changed) break;
+  }
+  int maxd = 0;
+  for (int j = 0; j < d->nj; ++j) {
+    if (depth[j] < 0) return fail(RK_EINVAL, "circular joint parents");
+    maxd = std::max(maxd, depth[j]);
+  }
+  int k = 0;
+  r.nlevels = maxd + 1;
+  for (int lv = 0; lv <= maxd; ++lv) {
+    r.level_start[lv] = k;
+    for (int j = 0; j < d->nj; ++j)
+      if (depth[j] == lv) r.order[k++] = j;
+  }
+  r.level_start[maxd + 1] = k;
+  for (int l = 0; l < d->nl; ++l) {
+    unsigned long long m = 0;
+    for (int j = r.link_joint[l]; j != -1; j = r.parent[j]) m |= 1ull << j;
+    r.anc[l] = m;
+  }
+  int c = 0;
+  for (int a = 0; a < d->na; ++a) {
+    r.col_start[a] = c;
+    for (int j = 0; j < d->nj; ++j) {
+      float w = 0.f;
+      if (r.act[j] >= 0) w = r.act[j] == a ? 1.f : 0.f;
+      else if (r.mimic[j] >= 0) w = r.mimic[j] == a ? r.mul[j] : 0.f;
+      if (w != 0.f) {
+        if (c >= RK_NJ_MAX) return fail(RK_EINVAL, "too many joint-to-dof entries");
+        r.col_joint[c] = j;
+        r.col_w[c] = w;
+        ++c;
+      }
+    }
+  }
+  r.col_start[d->na] = c;
+  const int id = (int)ctx->robots.size();
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipMemcpyAsync(ctx->robots_dev + id, &r, sizeof(Robot), hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->robots.push_back(r);
+  *robot_id = id;
+  return RK_OK;
+}
+
+static int check_robot(rk_context* ctx, int32_t robot) {
+  if (!ctx) return fail(RK_EINVAL, "null context");
+  if (robot < 0 || robot >= (int)ctx->robots.size()) return fail(RK_EINVAL, "unknown robot id");
+  return RK_OK;
+}
+
+int rk_fk(rk_context* ctx, int32_t robot, int32_t B, const float* q, float* T_link) {
+  int rc;
+  if ((rc = check_robot(ctx, robot))) return rc;
+  if (B <= 0 || !q || !T_link) return fail(RK_EINVAL, "bad batch or null array");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  const Robot& r = ctx->robots[robot];
+  float *dq = nullptr, *dT = nullptr;
+  HIP_TRY(hipMalloc(&dq, sizeof(float) * B * r.na));
+  HIP_TRY(hipMalloc(&dT, sizeof(float) * B * r.nl * 7));
+  hipMemcpyAsync(dq, q, sizeof(float) * B * r.na, hipMemcpyHostToDevice, ctx->stream);
+  hipLaunchKernelGGL(k_fk_host_layout, dim3((B + 127) / 128), dim3(128), 0, ctx->stream, ctx->robots_dev + robot, B,
+                     (const float*)dq, dT);
+  hipMemcpyAsync(T_link, dT, sizeof(float) * B * r.nl * 7, hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  hipFree(dq);
+  hipFree(dT);
+  if (e != hipSuccess) return fail(RK_EHIP, std::string("rk_fk: ") + hipGetErrorString(e));
+  return RK_OK;
+}
+
+int rk_link_jacobian(rk_context* ctx, int32_t robot, int32_t B, const float* q, int32_t link, int32_t frame,
+                     float* J) {
+  int rc;
+  if ((rc = check_robot(ctx, robot))) return rc;
+  const Robot& r = ctx->robots[robot];
+  if (B <= 0 || !q || !J || link < 0 || link >= r.nl || (frame != 0 && frame != 1))
+    return fail(RK_EINVAL, "bad argument to rk_link_jacobian");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  float *dq = nullptr, *dJ = nullptr;
+  HIP_TRY(hipMalloc(&dq, sizeof(float) * B * r.na));
+  HIP_TRY(hipMalloc(&dJ, sizeof(float) * B * 6 * r.na));
+  hipMemcpyAsync(dq, q, sizeof(float) * B * r.na, hipMemcpyHostToDevice, ctx->stream);
+  hipLaunchKernelGGL(k_link_jacobian, dim3((B + 127) / 128), dim3(128), 0, ctx->stream, ctx->robots_dev + robot, B,
+                     (const float*)dq, link, frame, dJ);
+  hipMemcpyAsync(J, dJ, sizeof(float) * B * 6 * r.na, hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  hipFree(dq);
+  hipFree(dJ);
+  if (e != hipSuccess) return fail(RK_EHIP, std::string("rk_link_jacobian: ") + hipGetErrorString(e));
+  return RK_OK;
+}
+
+int rk_se3_op(rk_context* ctx, int32_t op, int32_t n, const float* a, const float* b, float* out) {
+  if (!ctx || n <= 0 || !a || !out || op < 0 || op > 5 || ((op == 0 || op == 5) && !b))
+    return fail(RK_EINVAL, "bad argument to rk_se3_op");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int ow = op == 0 || op == 1 ? 7 : op == 2 ? 6 : op == 5 ? 3 : 36;
+  const int bw = op == 5 ? 3 : 7;
+  float *da = nullptr, *db = nullptr, *dout = nullptr;
+  HIP_TRY(hipMalloc(&da, sizeof(float) * n * 7));
+  HIP_TRY(hipMalloc(&db, sizeof(float) * n * 7));
+  HIP_TRY(hipMalloc(&dout, sizeof(float) * n * ow));
+  hipMemcpyAsync(da, a, sizeof(float) * n * 7, hipMemcpyHostToDevice, ctx->stream);
+  if (b) hipMemcpyAsync(db, b, sizeof(float) * n * bw, hipMemcpyHostToDevice, ctx->stream);
+  hipLaunchKernelGGL(k_se3_op, dim3((n + 127) / 128), dim3(128), 0, ctx->stream, op, n, (const float*)da,
+                     (const float*)db, dout);
+  hipMemcpyAsync(out, dout, sizeof(float) * n * ow, hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  hipFree(da);
+  hipFree(db);
+  hipFree(dout);
+  if (e != hipSuccess) return fail(RK_EHIP, std::string("rk_se3_op: ") + hipGetErrorString(e));
+  return RK_OK;
+}
+
+int rk_batch_create(rk_context* ctx, const rk_terms_desc* t, int32_t capacity, int32_t trace_iters, rk_batch** out) {
+  int rc;
+  if (!t || !out) return fail(RK_EINVAL, "null argument");
+  if ((rc = check_robot(ctx, t->robot))) return rc;
+  if (capacity <= 0) return fail(RK_EINVAL, "capacity must be positive");
+  if (trace_iters < 0) return fail(RK_EINVAL, "trace_iters must be >= 0");
+  const Robot& r = ctx->robots[t->robot];
+  if (t->n_pose < 0 || t->n_pose > RK_MAXT || t->n_pos < 0 || t->n_pos > RK_MAXQ)
+    return fail(RK_EINVAL, "at most 4 pose targets and 12 position targets per problem");
+  if (t->n_pose + t->n_pos + (t->limit ? 1 : 0) + (t->smooth ? 1 : 0) == 0)
+    return fail(RK_EINVAL, "a batch needs at least one residual term");
+  if (r.nj > CfgL::nj) return fail(RK_EINVAL, "the LM kernels take robots with nj <= 32");
+  Terms th;
+  memset(&th, 0, sizeof(th));
+  th.robot = t->robot;
+  th.n_pose = t->n_pose;
+  th.n_pos = t->n_pos;
+  th.limit = t->limit ? 1 : 0;
+  th.smooth = t->smooth ? 1 : 0;
+  for (int i = 0; i < t->n_pose; ++i) {
+    th.pose_link[i] = t->pose_links[i];
+    if (th.pose_link[i] < 0 || th.pose_link[i] >= r.nl) return fail(RK_EINVAL, "pose link out of range");
+    for (int k = 0; k < 6; ++k) th.pose_w[i][k] = t->pose_weights[6 * i + k];
+  }
+  for (int i = 0; i < t->n_pos; ++i) {
+    th.pos_link[i] = t->pos_links[i];
+    if (th.pos_link[i] < 0 || th.pos_link[i] >= r.nl) return fail(RK_EINVAL, "position link out of range");
+    for (int k = 0; k < 3; ++k) th.pos_w[i][k] = t->pos_weights[3 * i + k];
+  }
+  for (int d = 0; d < r.na; ++d) {
+    th.limit_w[d] = th.limit ? t->limit_weights[d] : 0.f;
+    th.smooth_w[d] = th.smooth ? t->smooth_weights[d] : 0.f;
+  }
+  th.Rf = 6 * th.n_pose;
+  th.Rq = 3 * th.n_pos;
+  th.R = th.Rf + th.Rq + (th.limit ? r.na : 0) + (th.smooth ? r.na : 0);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  rk_batch* b = new rk_batch();
+  b->ctx = ctx;
+  b->th = th;
+  b->capacity = capacity;
+  b->trace_iters = trace_iters;
+  b->na = r.na;
+  const size_t S = capacity;
+  auto A = [&](void** p, size_t bytes) { return dmalloc(ctx, p, bytes, &b->bytes); };
+  if ((rc = A((void**)&b->td, sizeof(Terms))) || (rc = A((void**)&b->q, sizeof(float) * r.na * S)) ||
+      (rc = A((void**)&b->q_seed, sizeof(float) * r.na * S)) ||
+      (rc = A((void**)&b->pose_tgt, sizeof(float) * std::max(1, th.n_pose * 7) * S)) ||
+      (rc = A((void**)&b->pos_tgt, sizeof(float) * std::max(1, th.n_pos * 3) * S)) ||
+      (rc = A((void**)&b->q_prev, sizeof(float) * r.na * S)) || (rc = A((void**)&b->cost, sizeof(float) * S)) ||
+      (rc = A((void**)&b->lam, sizeof(float) * S)) || (rc = A((void**)&b->nacc, sizeof(int) * S))) {
+    rk_batch_destroy(b);
+    return rc;
+  }
+  if (trace_iters > 0) {
+    if ((rc = A((void**)&b->trace, sizeof(float) * (trace_iters + 1) * S)) ||
+        (rc = A((void**)&b->acc_trace, sizeof(int) * trace_iters * S)) ||
+        (rc = A((void**)&b->q_trace, sizeof(float) * trace_iters * r.na * S)) ||
+        (rc = A((void**)&b->lam_trace, sizeof(float) * trace_iters * S))) {
+      rk_batch_destroy(b);
+      return rc;
+    }
+  }
+  hipMemcpyAsync(b->td, &th, sizeof(Terms), hipMemcpyHostToDevice, ctx->stream);
+  hipMemsetAsync(b->q_prev, 0, sizeof(float) * r.na * S, ctx->stream);
+  hipMemsetAsync(b->cost, 0, sizeof(float) * S, ctx->stream);
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  *out = b;
+  return RK_OK;
+}
+
+int rk_batch_destroy(rk_batch* b) {
+  if (!b) return RK_OK;
+  rk_context* ctx = b->ctx;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  void* ptrs[] = {b->td, b->q, b->q_seed, b->pose_tgt, b->pos_tgt, b->q_prev, b->cost, b->lam, b->nacc, b->trace,
+                  b->acc_trace, b->q_trace, b->lam_trace, b->pb.Tj, b->pb.Sw, b->pb.J, b->pb.r, b->pb.rp, b->pb.qp, b->pb.dq, b->pb.pred,
+                  b->pb.acc};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  ctx->mem_bytes -= b->bytes;
+  delete b;
+  return RK_OK;
+}
+
+// host row-major [B][k] -> device SoA [k][stride]
+static int upload_soa(rk_context* ctx, float* dst, const float* src, int B, int k, int stride) {
+  std::vector<float> tmp((size_t)k * B);
+  for (int i = 0; i < B; ++i)
+    for (int c = 0; c < k; ++c) tmp[(size_t)c * B + i] = src[(size_t)i * k + c];
+  for (int c = 0; c < k; ++c)
+    HIP_TRY(hipMemcpyAsync(dst + (size_t)c * stride, tmp.data() + (size_t)c * B, sizeof(float) * B,
+                           hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return RK_OK;
+}
+
+}  // extern "C"
+
+template <class T>
+static int download_soa(rk_context* ctx, T* dst, const T* src, int B, int k, int stride) {
+  std::vector<T> tmp((size_t)k * B);
+  for (int c = 0; c < k; ++c)
+    HIP_TRY(hipMemcpyAsync(tmp.data() + (size_t)c * B, src + (size_t)c * stride, sizeof(T) * B,
+                           hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < B; ++i)
+    for (int c = 0; c < k; ++c) dst[(size_t)i * k + c] = tmp[(size_t)c * B + i];
+  return RK_OK;
+}
+
+extern "C" {
+
+int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targets, const float* pos_targets,
+                 const float* q_prev) {
+  if (!b) return fail(RK_EINVAL, "null batch");
+  if (B > b->capacity)
+    return fail(RK_ECAPACITY, "batch size " + std::to_string(B) + " exceeds the " + std::to_string(b->capacity) +
+                                  " this batch was built for");
+  if (B <= 0) return fail(RK_EINVAL, "batch size must be positive");
+  rk_context* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc;
+  b->n = B;
+  if (q) {
+    if ((rc = upload_soa(ctx, b->q, q, B, b->na, b->capacity))) return rc;
+    HIP_TRY(hipMemcpyAsync(b->q_seed, b->q, sizeof(float) * b->na * b->capacity, hipMemcpyDeviceToDevice,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  if (pose_targets && b->th.n_pose &&
+      (rc = upload_soa(ctx, b->pose_tgt, pose_targets, B, b->th.n_pose * 7, b->capacity)))
+    return rc;
+  if (pos_targets && b->th.n_pos && (rc = upload_soa(ctx, b->pos_tgt, pos_targets, B, b->th.n_pos * 3, b->capacity)))
+    return rc;
+  if (q_prev && (rc = upload_soa(ctx, b->q_prev, q_prev, B, b->na, b->capacity))) return rc;
+  return RK_OK;
+}
+
+int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mode, int32_t tpp, float* kernel_ms) {
+  if (!b || !p) return fail(RK_EINVAL, "null argument");
+  if (b->n <= 0) return fail(RK_EINVAL, "batch has no problems (call rk_batch_set)");
+  if (iters < 0) return fail(RK_EINVAL, "iters must be >= 0");
+  if (b->trace && iters > b->trace_iters) return fail(RK_EINVAL, "iters exceeds the batch's trace length");
+  rk_context* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  const LMParams lp = to_lp(p);
+  int rc = RK_OK;
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  const bool from_seed = (mode & RK_FROM_SEED) != 0;
+  mode &= ~RK_FROM_SEED;
+  if (mode == RK_MODE_PLAIN) {
+    rc = solve_plain(b, iters, lp, from_seed);
+  } else if (mode == RK_MODE_FUSED) {
+    int v;
+    if ((rc = pick_variant(b, tpp, &v))) return rc;
+    rc = launch_variant(ctx, v, make_problems(b, from_seed), lp, 0, iters);
+  } else {
+    return fail(RK_EINVAL, "mode must be RK_MODE_FUSED or RK_MODE_PLAIN");
+  }
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  HIP_TRY(hipEventSynchronize(ctx->ev1));
+  if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, ctx->ev0, ctx->ev1));
+  return RK_OK;
+}
+
+int rk_batch_get(rk_batch* b, float* q, float* cost, float* lambda, float* cost_trace, int32_t* accept_trace) {
+  if (!b) return fail(RK_EINVAL, "null batch");
+  rk_context* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int B = b->n;
+  int rc;
+  if (q && (rc = download_soa(ctx, q, b->q, B, b->na, b->capacity))) return rc;
+  if (cost) {
+    HIP_TRY(hipMemcpyAsync(cost, b->cost, sizeof(float) * B, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  if (lambda) {
+    HIP_TRY(hipMemcpyAsync(lambda, b->lam, sizeof(float) * B, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (cost_trace) {
+    if (!b->trace) return fail(RK_EINVAL, "batch was created without a trace");
+    if ((rc = download_soa(ctx, cost_trace, b->trace, B, b->trace_iters + 1, b->capacity))) return rc;
+  }
+  if (accept_trace) {
+    if (!b->acc_trace) return fail(RK_EINVAL, "batch was created without a trace");
+    if ((rc = download_soa<int>(ctx, accept_trace, b->acc_trace, B, b->trace_iters, b->capacity)))
+      return rc;
+  }
+  return RK_OK;
+}
+
+int rk_batch_get_iterates(rk_batch* b, float* q_trace, float* lam_trace) {
+  if (!b || !q_trace || !lam_trace) return fail(RK_EINVAL, "null argument");
+  if (!b->q_trace) return fail(RK_EINVAL, "batch was created without a trace");
+  rk_context* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = download_soa<float>(ctx, q_trace, b->q_trace, b->n, b->trace_iters * b->na, b->capacity))) return rc;
+  if ((rc = download_soa<float>(ctx, lam_trace, b->lam_trace, b->n, b->trace_iters, b->capacity))) return rc;
+  return RK_OK;
+}
+
+int rk_batch_metrics(rk_batch* b, float pos_thr, float ang_thr, float* pos_err, float* ang_err, int32_t* success) {
+  if (!b || !pos_err || !ang_err || !success) return fail(RK_EINVAL, "null argument");
+  if (b->th.n_pose == 0) return fail(RK_EINVAL, "metrics need pose targets");
+  rk_context* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int B = b->n, k = b->th.n_pose;
+  float *dpe = nullptr, *dae = nullptr;
+  int* dok = nullptr;
+  HIP_TRY(hipMalloc(&dpe, sizeof(float) * B * k));
+  HIP_TRY(hipMalloc(&dae, sizeof(float) * B * k));
+  HIP_TRY(hipMalloc(&dok, sizeof(int) * B));
+  hipLaunchKernelGGL(k_ik_metrics, dim3((B + 127) / 128), dim3(128), 0, ctx->stream, make_problems(b), pos_thr,
+                     ang_thr, dpe, dae, dok);
+  hipMemcpyAsync(pos_err, dpe, sizeof(float) * B * k, hipMemcpyDeviceToHost, ctx->stream);
+  hipMemcpyAsync(ang_err, dae, sizeof(float) * B * k, hipMemcpyDeviceToHost, ctx->stream);
+  hipMemcpyAsync(success, dok, sizeof(int) * B, hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  hipFree(dpe);
+  hipFree(dae);
+  hipFree(dok);
+  if (e != hipSuccess) return fail(RK_EHIP, std::string("rk_batch_metrics: ") + hipGetErrorString(e));
+  return RK_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// uneven batch (persistent kernel)
+// ---------------------------------------------------------------------------------------------
+int rk_group_create(rk_context* ctx, int32_t nb, rk_batch* const* batches, rk_group** out) {
+  if (!ctx || nb <= 0 || !batches || !out) return fail(RK_EINVAL, "bad argument to rk_group_create");
+  for (int i = 0; i < nb; ++i)
+    if (!batches[i] || batches[i]->ctx != ctx) return fail(RK_EINVAL, "batch of another context or null");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  rk_group* g = new rk_group();
+  g->ctx = ctx;
+  g->batches.assign(batches, batches + nb);
+  g->tpp.assign(nb, 0);
+  int rc;
+  if ((rc = dmalloc(ctx, (void**)&g->buckets_dev, sizeof(Bucket) * nb, &g->bytes)) ||
+      (rc = dmalloc(ctx, (void**)&g->counter, sizeof(int) * 4, &g->bytes))) {
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return RK_OK;
+}
+
+int rk_group_destroy(rk_group* g) {
+  if (!g) return RK_OK;
+  rk_context* ctx = g->ctx;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  if (g->graph) hipGraphExecDestroy(g->graph);
+  hipFree(g->buckets_dev);
+  hipFree(g->counter);
+  ctx->mem_bytes -= g->bytes;
+  delete g;
+  return RK_OK;
+}
+
+int rk_group_set_tuning(rk_group* g, const int32_t* tpp, int32_t wg_per_cu, int32_t iters_per_launch) {
+  if (!g) return fail(RK_EINVAL, "null group");
+  if (tpp)
+    for (size_t i = 0; i < g->batches.size(); ++i) {
+      int v;
+      int rc = pick_variant(g->batches[i], tpp[i], &v);
+      if (rc) return rc;
+      g->tpp[i] = tpp[i];
+    }
+  if (wg_per_cu < 0 || iters_per_launch < 0) return fail(RK_EINVAL, "negative tuning value");
+  g->wg_per_cu = wg_per_cu;
+  g->iters_per_launch = iters_per_launch;
+  if (g->graph) {
+    hipGraphExecDestroy(g->graph);
+    g->graph = nullptr;
+  }
+  return RK_OK;
+}
+
+struct GroupLaunch {
+  int nb = 0, total = 0, grid = 0;
+  size_t lds = 0;
+};
+
+// upload the bucket table (outside any capture) and size the persistent grid
+static int group_prepare(rk_group* g, GroupLaunch* gl, bool from_seed) {
+  rk_context* ctx = g->ctx;
+  std::vector<Bucket> bk(g->batches.size());
+  size_t lds = 0;
+  for (size_t i = 0; i < g->batches.size(); ++i) {
+    int v;
+    int rc = pick_variant(g->batches[i], g->tpp[i], &v);
+    if (rc) return rc;
+    bk[i].variant = v;
+    bk[i].P = make_problems(g->batches[i], from_seed);
+    const int ppw = WAVE / variant_G(v);
+    bk[i].tiles = (g->batches[i]->n + ppw - 1) / ppw;
+    lds = std::max(lds, variant_lds(v));
+  }
+  // big problems first (longest-processing-time order)
+  std::vector<int> order(bk.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return g->batches[a]->na * g->batches[a]->th.R > g->batches[b]->na * g->batches[b]->th.R;
+  });
+  std::vector<Bucket> sorted(bk.size());
+  int base = 0;
+  for (size_t i = 0; i < order.size(); ++i) {
+    sorted[i] = bk[order[i]];
+    sorted[i].tile_base = base;
+    base += sorted[i].tiles;
+  }
+  HIP_TRY(hipMemcpyAsync(g->buckets_dev, sorted.data(), sizeof(Bucket) * sorted.size(), hipMemcpyHostToDevice,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  static size_t attr_lds = 0;
+  if (lds > attr_lds) {
+    HIP_TRY(hipFuncSetAttribute((const void*)k_persistent, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_lds = lds;
+  }
+  int per_cu = g->wg_per_cu;
+  if (per_cu <= 0) {
+    int occ = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_persistent, WAVE, lds));
+    per_cu = std::max(1, occ);
+  }
+  gl->nb = (int)sorted.size();
+  gl->total = base;
+  gl->lds = lds;
+  gl->grid = std::max(1, std::min(base, per_cu * ctx->cu_count));
+  return RK_OK;
+}
+
+// enqueue the solve's launches on the context stream (capturable: no allocation, copy or memset)
+static int group_launch(rk_group* g, const GroupLaunch& gl, int iters, const LMParams& lp) {
+  rk_context* ctx = g->ctx;
+  const int chunk = g->iters_per_launch > 0 ? g->iters_per_launch : std::max(1, iters);
+  for (int it0 = 0; it0 < std::max(1, iters); it0 += chunk) {
+    const int n_it = std::max(0, std::min(chunk, iters - it0));
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, ctx->stream, g->counter);
+    hipLaunchKernelGGL(k_persistent, dim3(gl.grid), dim3(WAVE), gl.lds, ctx->stream, (const Bucket*)g->buckets_dev,
+                       gl.nb, gl.total, g->counter, lp, it0, n_it);
+    if (iters == 0) break;
+  }
+  HIP_TRY(hipGetLastError());
+  return RK_OK;
+}
+
+int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t flags, float* kernel_ms) {
+  const bool use_graph = (flags & RK_USE_GRAPH) != 0;
+  const bool from_seed = (flags & RK_FROM_SEED) != 0;
+  if (!g || !p) return fail(RK_EINVAL, "null argument");
+  if (iters < 0) return fail(RK_EINVAL, "iters must be >= 0");
+  for (rk_batch* b : g->batches) {
+    if (b->n <= 0) return fail(RK_EINVAL, "a batch of the group has no problems");
+    if (b->trace && iters > b->trace_iters) return fail(RK_EINVAL, "iters exceeds a batch's trace length");
+  }
+  rk_context* ctx = g->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  const LMParams lp = to_lp(p);
+  std::vector<float> key = {(float)iters, lp.lm_lambda, lp.lambda_factor, lp.lambda_min, lp.lambda_max, lp.rho_min,
+                            lp.gain_ratio_epsilon};
+  for (rk_batch* b : g->batches) key.push_back((float)b->n);
+  key.push_back(from_seed ? 1.f : 0.f);
+  int rc;
+  if (use_graph) {
+    if (!g->graph || g->graph_key != key) {
+      if (g->graph) {
+        hipGraphExecDestroy(g->graph);
+        g->graph = nullptr;
+      }
+      GroupLaunch gl;
+      if ((rc = group_prepare(g, &gl, from_seed))) return rc;
+      hipGraph_t graph;
+      HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      rc = group_launch(g, gl, iters, lp);
+      hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+      if (rc) return rc;
+      if (e != hipSuccess) return fail(RK_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&g->graph, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (e != hipSuccess) return fail(RK_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      g->graph_key = key;
+    }
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(hipGraphLaunch(g->graph, ctx->stream));
+  } else {
+    GroupLaunch gl;
+    if ((rc = group_prepare(g, &gl, from_seed))) return rc;
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    if ((rc = group_launch(g, gl, iters, lp))) return rc;
+  }
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  HIP_TRY(hipEventSynchronize(ctx->ev1));
+  if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, ctx->ev0, ctx->ev1));
+  return RK_OK;
+}
+
+int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t* chosen) {
+  if (!g || !p || iters <= 0) return fail(RK_EINVAL, "bad argument to rk_group_autotune");
+  rk_context* ctx = g->ctx;
+  const LMParams lp = to_lp(p);
+  // each batch alone, each legal G; state is restored after every trial
+  for (size_t i = 0; i < g->batches.size(); ++i) {
+    rk_batch* b = g->batches[i];
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t S = b->capacity;
+    float* q_save = nullptr;
+    int rc;
+    if ((rc = dmalloc(ctx, (void**)&q_save, sizeof(float) * b->na * S, nullptr))) return rc;
+    HIP_TRY(hipMemcpyAsync(q_save, b->q, sizeof(float) * b->na * S, hipMemcpyDeviceToDevice, ctx->stream));
+    float* tr = b->trace;
+    int* at = b->acc_trace;
+    float* qt = b->q_trace;
+    float* lt = b->lam_trace;
+    b->trace = nullptr;
+    b->acc_trace = nullptr;
+    b->q_trace = nullptr;
+    b->lam_trace = nullptr;
+    std::vector<int> cands;
+    legal_tpps(b, cands);
+    float best = 1e30f;
+    int best_g = cands[0];
+    for (int G : cands) {
+      int v;
+      if (pick_variant(b, G, &v)) continue;
+      float ms_sum = 0.f;
+      for (int rep = 0; rep < 2; ++rep) {
+        HIP_TRY(hipMemcpyAsync(b->q, q_save, sizeof(float) * b->na * S, hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+        if ((rc = launch_variant(ctx, v, make_problems(b), lp, 0, iters))) return rc;
+        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        if (rep == 1) ms_sum += ms;
+      }
+      if (ms_sum < best) {
+        best = ms_sum;
+        best_g = G;
+      }
+    }
+    HIP_TRY(hipMemcpyAsync(b->q, q_save, sizeof(float) * b->na * S, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    dfree(ctx, q_save, sizeof(float) * b->na * S);
+    b->trace = tr;
+    b->acc_trace = at;
+    b->q_trace = qt;
+    b->lam_trace = lt;
+    g->tpp[i] = best_g;
+    if (chosen) chosen[i] = best_g;
+  }
+  if (g->graph) {
+    hipGraphExecDestroy(g->graph);
+    g->graph = nullptr;
+  }
+  return RK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,321 @@
+"""ctypes binding of librkhip.so (include/rkhip.h).
+
+There is no fallback: if the library is missing or the device is unavailable, the calls
+raise.  The library is built in-tree by ``robokit_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "librkhip.so"
+
+RK_OK, RK_EINVAL, RK_EHIP, RK_ECAPACITY, RK_ENOMEM, RK_EBUSY = range(6)
+MODE_FUSED, MODE_PLAIN = 0, 1
+FROM_SEED, USE_GRAPH = 0x100, 0x1
+SE3_OPS = {"compose": (0, 7), "inverse": (1, 7), "log": (2, 6), "jlog": (3, 36), "adjoint": (4, 36), "apply": (5, 3)}
+
+
+class RKError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[rk {code}] {msg}")
+        self.code = code
+
+
+class RobotDesc(C.Structure):
+    _fields_ = [
+        ("nj", C.c_int32), ("nl", C.c_int32), ("na", C.c_int32),
+        ("joint_type", C.c_void_p), ("joint_axis", C.c_void_p), ("parent_T", C.c_void_p),
+        ("parent_joint", C.c_void_p), ("act_idx", C.c_void_p), ("mimic_idx", C.c_void_p),
+        ("mimic_mul", C.c_void_p), ("mimic_off", C.c_void_p), ("link_parent_joint", C.c_void_p),
+        ("limits", C.c_void_p),
+    ]
+
+
+class TermsDesc(C.Structure):
+    _fields_ = [
+        ("robot", C.c_int32),
+        ("n_pose", C.c_int32), ("pose_links", C.c_void_p), ("pose_weights", C.c_void_p),
+        ("n_pos", C.c_int32), ("pos_links", C.c_void_p), ("pos_weights", C.c_void_p),
+        ("limit", C.c_int32), ("limit_weights", C.c_void_p),
+        ("smooth", C.c_int32), ("smooth_weights", C.c_void_p),
+    ]
+
+
+class LMParamsC(C.Structure):
+    _fields_ = [(n, C.c_float) for n in
+                ("lm_lambda", "lambda_factor", "lambda_min", "lambda_max", "rho_min", "gain_ratio_epsilon")]
+
+
+_lib: Optional[C.CDLL] = None
+
+EXPORTS = [
+    "rk_last_error", "rk_abi_version", "rk_context_create", "rk_context_destroy", "rk_context_sync",
+    "rk_context_mem", "rk_robot_upload", "rk_fk", "rk_link_jacobian", "rk_se3_op", "rk_batch_create",
+    "rk_batch_destroy", "rk_batch_set", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics", "rk_group_create",
+    "rk_group_destroy", "rk_group_set_tuning", "rk_group_autotune", "rk_group_solve",
+]
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m robokit_amd.build` (hipcc, gfx950)")
+        L = C.CDLL(str(LIB_PATH))
+        L.rk_last_error.restype = C.c_char_p
+        for name in EXPORTS:
+            if name != "rk_last_error":
+                getattr(L, name).restype = C.c_int32
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != RK_OK:
+        msg = lib().rk_last_error().decode()
+        if rc in (RK_EINVAL, RK_ECAPACITY):
+            raise ValueError(msg)
+        raise RKError(rc, msg)
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+@dataclass
+class LMParams:
+    """LMOptimizerConfig damping fields (opt/lm_optimizer.py:502-523 defaults)."""
+
+    lm_lambda: float = 1e-3
+    lambda_factor: float = 2.0
+    lambda_min: float = 1e-5
+    lambda_max: float = 1e10
+    rho_min: float = 1e-3
+    gain_ratio_epsilon: float = 1e-8
+
+    def c(self) -> LMParamsC:
+        return LMParamsC(self.lm_lambda, self.lambda_factor, self.lambda_min, self.lambda_max, self.rho_min,
+                         self.gain_ratio_epsilon)
+
+
+class Context:
+    """One HIP device + stream + the resident robot models (rk_context)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().rk_context_create(int(device), C.byref(self._h)))
+        self.device = device
+        self._robots: Dict[int, int] = {}
+        self._keep: List = []
+
+    def close(self) -> None:
+        if self._h:
+            lib().rk_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def sync(self) -> None:
+        _check(lib().rk_context_sync(self._h))
+
+    def mem_bytes(self) -> int:
+        v = C.c_int64()
+        _check(lib().rk_context_mem(self._h, C.byref(v)))
+        return int(v.value)
+
+    def robot(self, spec) -> int:
+        """Upload ``spec`` once (resident on the device) and return its id."""
+        key = id(spec)
+        if key in self._robots:
+            return self._robots[key]
+        arrs = [
+            _i32(spec.joint_types), _f32(spec.joint_axes), _f32(spec.parent_joint_transforms),
+            _i32(spec.parent_joint_indices), _i32(spec.actuated_joint_indices),
+            _i32(spec.mimic_actuated_joint_indices), _f32(spec.mimic_multipliers), _f32(spec.mimic_offsets),
+            _i32(spec.link_parent_joint_indices), _f32(spec.actuated_joint_limits_f32),
+        ]
+        d = RobotDesc(spec.num_joints, spec.num_links, spec.num_actuated_joints, *[_p(a) for a in arrs])
+        rid = C.c_int32()
+        _check(lib().rk_robot_upload(self._h, C.byref(d), C.byref(rid)))
+        self._robots[key] = rid.value
+        self._keep.append(spec)
+        return rid.value
+
+    def fk(self, spec, q: np.ndarray) -> np.ndarray:
+        q = _f32(np.atleast_2d(q))
+        out = np.zeros((q.shape[0], spec.num_links, 7), np.float32)
+        _check(lib().rk_fk(self._h, self.robot(spec), q.shape[0], _p(q), _p(out)))
+        return out
+
+    def link_jacobian(self, spec, q: np.ndarray, link: int, frame: str = "body") -> np.ndarray:
+        q = _f32(np.atleast_2d(q))
+        out = np.zeros((q.shape[0], 6, spec.num_actuated_joints), np.float32)
+        _check(lib().rk_link_jacobian(self._h, self.robot(spec), q.shape[0], _p(q), int(link),
+                                      0 if frame == "spatial" else 1, _p(out)))
+        return out
+
+    def se3(self, op: str, a: np.ndarray, b: Optional[np.ndarray] = None) -> np.ndarray:
+        code, w = SE3_OPS[op]
+        a = _f32(np.atleast_2d(a))
+        bb = None if b is None else _f32(np.atleast_2d(b))
+        out = np.zeros((a.shape[0], w), np.float32)
+        _check(lib().rk_se3_op(self._h, code, a.shape[0], _p(a), _p(bb), _p(out)))
+        return out.reshape(a.shape[0], 6, 6) if w == 36 else out
+
+
+@dataclass
+class TermSpec:
+    """Uniform residual-term structure of a batch (row order: pose, position, limit, smoothness)."""
+
+    pose_links: Sequence[int] = ()
+    pose_weights: Optional[np.ndarray] = None  # [n_pose, 6]
+    pos_links: Sequence[int] = ()
+    pos_weights: Optional[np.ndarray] = None  # [n_pos, 3]
+    limit_weights: Optional[np.ndarray] = None  # [na] or None (no limit term)
+    smooth_weights: Optional[np.ndarray] = None  # [na] or None (no smoothness term)
+
+    def rows(self, na: int) -> int:
+        return 6 * len(self.pose_links) + 3 * len(self.pos_links) + na * (self.limit_weights is not None) + \
+            na * (self.smooth_weights is not None)
+
+
+class Batch:
+    """Device-resident SoA buffers for up to ``capacity`` problems of one term structure."""
+
+    def __init__(self, ctx: Context, spec, terms: TermSpec, capacity: int, trace_iters: int = 0):
+        self.ctx, self.spec, self.terms = ctx, spec, terms
+        self.capacity, self.trace_iters = int(capacity), int(trace_iters)
+        na = spec.num_actuated_joints
+        self._arrs = [
+            _i32(list(terms.pose_links)),
+            _f32(terms.pose_weights if terms.pose_weights is not None else np.zeros((0, 6))),
+            _i32(list(terms.pos_links)),
+            _f32(terms.pos_weights if terms.pos_weights is not None else np.zeros((0, 3))),
+            _f32(terms.limit_weights if terms.limit_weights is not None else np.zeros(na)),
+            _f32(terms.smooth_weights if terms.smooth_weights is not None else np.zeros(na)),
+        ]
+        a = self._arrs
+        d = TermsDesc(ctx.robot(spec), len(terms.pose_links), _p(a[0]), _p(a[1]), len(terms.pos_links), _p(a[2]),
+                      _p(a[3]), int(terms.limit_weights is not None), _p(a[4]), int(terms.smooth_weights is not None),
+                      _p(a[5]))
+        self._h = C.c_void_p()
+        _check(lib().rk_batch_create(ctx.handle, C.byref(d), self.capacity, self.trace_iters, C.byref(self._h)))
+        self.n = 0
+
+    def close(self) -> None:
+        if self._h:
+            lib().rk_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set(self, q: Optional[np.ndarray] = None, pose_targets: Optional[np.ndarray] = None,
+            pos_targets: Optional[np.ndarray] = None, q_prev: Optional[np.ndarray] = None, n: Optional[int] = None):
+        arrs = [None if x is None else _f32(x) for x in (q, pose_targets, pos_targets, q_prev)]
+        if n is None:
+            n = next(x.shape[0] for x in arrs if x is not None)
+        if arrs[2] is not None and arrs[2].shape[-1] == 7:
+            arrs[2] = _f32(arrs[2][..., :3])
+        _check(lib().rk_batch_set(self._h, int(n), *[_p(x) for x in arrs]))
+        self.n = int(n)
+
+    def solve(self, iters: int, params: LMParams, mode: int = MODE_FUSED, tpp: int = 0, from_seed: bool = False) -> float:
+        ms = C.c_float()
+        m = int(mode) | (FROM_SEED if from_seed else 0)
+        _check(lib().rk_batch_solve(self._h, int(iters), C.byref(params.c()), m, int(tpp), C.byref(ms)))
+        return float(ms.value)
+
+    def get(self, traces: bool = False):
+        na = self.spec.num_actuated_joints
+        q = np.zeros((self.n, na), np.float32)
+        cost = np.zeros(self.n, np.float32)
+        lam = np.zeros(self.n, np.float32)
+        ct = np.zeros((self.n, self.trace_iters + 1), np.float32) if traces else None
+        at = np.zeros((self.n, self.trace_iters), np.int32) if traces else None
+        _check(lib().rk_batch_get(self._h, _p(q), _p(cost), _p(lam), _p(ct), _p(at)))
+        out = dict(q=q, cost=cost, lam=lam)
+        if traces:
+            out.update(cost_trace=ct, accept=at)
+        return out
+
+    def iterates(self):
+        """q before each traced step [B, trace_iters, na] and the damping it used [B, trace_iters]."""
+        na = self.spec.num_actuated_joints
+        qt = np.zeros((self.n, self.trace_iters, na), np.float32)
+        lt = np.zeros((self.n, self.trace_iters), np.float32)
+        _check(lib().rk_batch_get_iterates(self._h, _p(qt), _p(lt)))
+        return qt, lt
+
+    def metrics(self, pos_thr: float = 1e-3, ang_thr: float = 1e-2):
+        k = len(self.terms.pose_links)
+        pe = np.zeros((self.n, k), np.float32)
+        ae = np.zeros((self.n, k), np.float32)
+        ok = np.zeros(self.n, np.int32)
+        _check(lib().rk_batch_metrics(self._h, C.c_float(pos_thr), C.c_float(ang_thr), _p(pe), _p(ae), _p(ok)))
+        return pe, ae, ok.astype(bool)
+
+
+class Group:
+    """Uneven batch: several Batches solved by one persistent kernel (rk_group)."""
+
+    def __init__(self, ctx: Context, batches: Sequence[Batch]):
+        self.ctx, self.batches = ctx, list(batches)
+        arr = (C.c_void_p * len(self.batches))(*[b.handle.value for b in self.batches])
+        self._h = C.c_void_p()
+        _check(lib().rk_group_create(ctx.handle, len(self.batches), arr, C.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().rk_group_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, tpp: Optional[Sequence[int]] = None, wg_per_cu: int = 0, iters_per_launch: int = 0) -> None:
+        t = None if tpp is None else _i32(list(tpp))
+        _check(lib().rk_group_set_tuning(self._h, _p(t), int(wg_per_cu), int(iters_per_launch)))
+
+    def autotune(self, iters: int, params: LMParams) -> List[int]:
+        out = np.zeros(len(self.batches), np.int32)
+        _check(lib().rk_group_autotune(self._h, int(iters), C.byref(params.c()), _p(out)))
+        return out.tolist()
+
+    def solve(self, iters: int, params: LMParams, use_graph: bool = True, from_seed: bool = False) -> float:
+        ms = C.c_float()
+        flags = (USE_GRAPH if use_graph else 0) | (FROM_SEED if from_seed else 0)
+        _check(lib().rk_group_solve(self._h, int(iters), C.byref(params.c()), flags, C.byref(ms)))
+        return float(ms.value)

@@ -1,0 +1,170 @@
+"""GPU parity of the batched LM solve (fused and plain forms) against the CPU oracle.
+
+per-iteration residual norms within 1e-4 relative (tests/parity.py explains
+the comparison), identical converged/failed classification at 1 mm / 0.01 rad, FP32.
+Plus the invariants the reference tests hold: a problem's result does not depend on the batch
+it is in (tests/opt/test_varied_batch.py:138-151, exact), capacity overflow raises
+ValueError (optimizer.py:321-322).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from robokit_amd import native as N
+from robokit_amd import workloads as W
+from tests.parity import compare_traces, shadow_check
+
+pytestmark = pytest.mark.gpu
+
+C0_PARAMS = N.LMParams(lm_lambda=1e-2, lambda_factor=10.0)
+ITERS = 50
+
+
+def _ik_terms(spec, links):
+    na = spec.num_actuated_joints
+    return N.TermSpec(pose_links=links, pose_weights=np.ones((len(links), 6), np.float32),
+                      limit_weights=np.ones(na, np.float32))
+
+
+def _oracle_ik(wl, iters=ITERS, params=C0_PARAMS):
+    na = wl.spec.num_actuated_joints
+    terms = O.ik_terms(wl.links, wl.targets, na)
+    p = O.LMParams(**params.__dict__)
+    res = O.lm_solve(wl.spec, terms, wl.q_seed, iters, p)
+    _, _, ok = O.ik_metrics(wl.spec, res["q"], wl.links, wl.targets)
+    return res, ok
+
+
+def _gpu_ik(ctx, wl, mode, tpp=0, iters=ITERS, params=C0_PARAMS, capacity=None):
+    b = N.Batch(ctx, wl.spec, _ik_terms(wl.spec, wl.links), capacity or wl.size, trace_iters=iters)
+    b.set(q=wl.q_seed, pose_targets=wl.targets)
+    b.solve(iters, params, mode=mode, tpp=tpp)
+    out = b.get(traces=True)
+    out["q_iter"], out["lam_iter"] = b.iterates()
+    _, _, ok = b.metrics()
+    out["ok"] = ok
+    return out
+
+
+def _check_parity(ctx, wl, mode, label):
+    """Shadowing check of every GPU step + identical classification + a free-running report."""
+    na = wl.spec.num_actuated_joints
+    ora, ok_o = _oracle_ik(wl)
+    gpu = _gpu_ik(ctx, wl, mode)
+    B = wl.size
+    rows = np.repeat(np.arange(B), ITERS)
+    args = (wl.spec, O.ik_terms(wl.links, wl.targets, na), gpu["q_iter"].reshape(-1, na), gpu["lam_iter"].reshape(-1),
+            O.LMParams(**C0_PARAMS.__dict__))
+    rep = shadow_check(gpu["cost_trace"], gpu["accept"], O.lm_step(*args, rows=rows),
+                       O.lm_step(*args, dtype="f64", rows=rows))
+    free = compare_traces(gpu["cost_trace"], ora["cost_trace"], gpu["accept"], ora["accept"])
+    print(f"\n{label} mode={mode}: {rep}\n  free-running: forks={free.forks}/{free.n} "
+          f"converged={ok_o.mean():.3f}")
+    assert rep.ok(), rep
+    assert np.array_equal(gpu["ok"], ok_o), f"classification differs on {np.sum(gpu['ok'] != ok_o)} problems"
+
+
+@pytest.mark.parametrize("mode", [N.MODE_FUSED, N.MODE_PLAIN])
+@pytest.mark.parametrize("robot", ["panda_dh", "ur10"])
+def test_ik_parity_serial(ctx, mode, robot):
+    spec = W.panda_dh() if robot == "panda_dh" else W.ur10()
+    wl = W.ik_batch(spec, [spec.num_links - 1], 256, seed=11)
+    _check_parity(ctx, wl, mode, robot)
+
+
+@pytest.mark.parametrize("mode", [N.MODE_FUSED, N.MODE_PLAIN])
+def test_ik_parity_humanoid(ctx, mode):
+    spec = W.humanoid29(0)
+    wl = W.ik_batch(spec, W.humanoid_ee_links(spec), 64, seed=12)
+    # seeds near the target joints so that a share of the problems converges
+    wl.q_seed = (wl.q_target + np.random.default_rng(1).normal(0, 0.3, wl.q_target.shape)).astype(np.float32)
+    _check_parity(ctx, wl, mode, "humanoid")
+
+
+def test_fused_and_plain_agree(ctx):
+    """Both forms evaluate the same formulas (the compiler may contract multiply-adds differently
+    in different kernels, so agreement is to rounding, not bitwise): the first step's costs agree
+    to 1e-5 relative and the converged/failed classification is identical."""
+    wl = W.config0(seed=3, batch=512)
+    f = _gpu_ik(ctx, wl, N.MODE_FUSED)
+    p = _gpu_ik(ctx, wl, N.MODE_PLAIN)
+    np.testing.assert_allclose(f["cost_trace"][:, :2], p["cost_trace"][:, :2], rtol=1e-5, atol=1e-9)
+    assert np.array_equal(f["ok"], p["ok"])
+
+
+@pytest.mark.parametrize("robot", ["panda_dh", "humanoid"])
+def test_threads_per_problem_invariance(ctx, robot):
+    """Each reduction is summed by one lane in a fixed order: results are identical for any G."""
+    if robot == "panda_dh":
+        spec, links, tpps = W.panda_dh(), [7], [4, 8, 16, 32, 64]
+    else:
+        spec = W.humanoid29(0)
+        links, tpps = W.humanoid_ee_links(spec), [16, 32, 64]
+    wl = W.ik_batch(spec, links, 100, seed=4)
+    outs = [_gpu_ik(ctx, wl, N.MODE_FUSED, tpp=t, iters=20) for t in tpps]
+    for o in outs[1:]:
+        assert np.array_equal(o["q"], outs[0]["q"])
+        assert np.array_equal(o["cost_trace"], outs[0]["cost_trace"])
+
+
+def test_batch_composition_invariance(ctx):
+    """prefix of a capacity-N batch == a fresh batch (tests/opt/test_varied_batch.py:138-151)."""
+    wl = W.config0(seed=5, batch=300)
+    full = _gpu_ik(ctx, wl, N.MODE_FUSED, capacity=1024)
+    sub = W.IKBatch(wl.spec, wl.links, wl.q_seed[:37], wl.targets[:37], wl.q_target[:37])
+    part = _gpu_ik(ctx, sub, N.MODE_FUSED)
+    assert np.array_equal(full["q"][:37], part["q"])
+    with pytest.raises(ValueError, match="exceeds"):
+        b = N.Batch(ctx, wl.spec, _ik_terms(wl.spec, wl.links), 16)
+        b.set(q=wl.q_seed[:17], pose_targets=wl.targets[:17])
+
+
+def test_group_uneven_batch_matches_single(ctx):
+    """Persistent uneven batch (3 robots in one launch, graph-captured) == each batch alone."""
+    mb = W.config1(seed=0, total=600)
+    params = C0_PARAMS
+    singles, batches = [], []
+    for bk in mb.buckets:
+        singles.append(_gpu_ik(ctx, bk, N.MODE_FUSED, iters=30))
+        b = N.Batch(ctx, bk.spec, _ik_terms(bk.spec, bk.links), bk.size, trace_iters=30)
+        b.set(q=bk.q_seed, pose_targets=bk.targets)
+        batches.append(b)
+    g = N.Group(ctx, batches)
+    g.solve(30, params, use_graph=True)
+    for s, b in zip(singles, batches):
+        out = b.get(traces=True)
+        assert np.array_equal(out["q"], s["q"])
+        assert np.array_equal(out["cost_trace"], s["cost_trace"])
+    # replaying the captured graph from the same seeds gives the same result again
+    for bk, b in zip(mb.buckets, batches):
+        b.set(q=bk.q_seed, pose_targets=bk.targets)
+    g.solve(30, params, use_graph=True)
+    for s, b in zip(singles, batches):
+        assert np.array_equal(b.get()["q"], s["q"])
+    # chunked launches (iteration quanta) reproduce the single launch
+    g.set_tuning(iters_per_launch=7)
+    for bk, b in zip(mb.buckets, batches):
+        b.set(q=bk.q_seed, pose_targets=bk.targets)
+    g.solve(30, params, use_graph=False)
+    for s, b in zip(singles, batches):
+        assert np.array_equal(b.get()["q"], s["q"])
+
+
+def test_autotune_keeps_results(ctx):
+    mb = W.config1(seed=1, total=400)
+    batches = []
+    for bk in mb.buckets:
+        b = N.Batch(ctx, bk.spec, _ik_terms(bk.spec, bk.links), bk.size)
+        b.set(q=bk.q_seed, pose_targets=bk.targets)
+        batches.append(b)
+    g = N.Group(ctx, batches)
+    ref = []
+    g.solve(10, C0_PARAMS, use_graph=False)
+    ref = [b.get()["q"] for b in batches]
+    for bk, b in zip(mb.buckets, batches):
+        b.set(q=bk.q_seed, pose_targets=bk.targets)
+    chosen = g.autotune(5, C0_PARAMS)
+    assert len(chosen) == 3 and all(c in (4, 8, 16, 32, 64) for c in chosen)
+    g.solve(10, C0_PARAMS, use_graph=True)
+    for r, b in zip(ref, batches):
+        assert np.array_equal(b.get()["q"], r)
