@@ -72,6 +72,10 @@ int rk_fk(rk_context* ctx, int32_t robot, int32_t B, const float* q, float* T_li
 int rk_link_jacobian(rk_context* ctx, int32_t robot, int32_t B, const float* q, int32_t link, int32_t frame,
                      float* J);
 
+/* measurement utility (no reference counterpart): two launches of a streaming copy of `bytes`
+ * (read once, written once), the second timed; calibrates the FETCH_SIZE / WRITE_SIZE counters */
+int rk_copy_probe(rk_context* ctx, int64_t bytes, float* kernel_ms);
+
 /* device SE(3) ops for tests (lie/se3.py:38-290 host API): op 0 compose(a,b) 1 inverse(a) 2 log(a)
  * 3 jlog(a) [36] 4 adjoint(a) [36] 5 apply(a, b[3]) ; a,b [n][7]; out [n][w] */
 int rk_se3_op(rk_context* ctx, int32_t op, int32_t n, const float* a, const float* b, float* out);
@@ -120,6 +124,10 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
 #define RK_MODE_FUSED 0 /* one launch: whole solve per problem in LDS/registers */
 #define RK_MODE_PLAIN 1 /* reference decomposition: per-term kernels, J in HBM, host loop */
 #define RK_FROM_SEED 0x100 /* flag: start from the seeds of the last rk_batch_set (kept on the device) */
+/* threads_per_problem codes: 1 = one lane per problem, register-resident (serial arms);
+ * RK_TPP_WAVE = one wavefront per problem, register-resident rows (trees, <= 32 dofs);
+ * 4, 8, 16, 32, 64 = that many lanes per problem, LDS-staged (any robot the batch accepts) */
+#define RK_TPP_WAVE (-64)
 #define RK_USE_GRAPH 0x1   /* rk_group_solve flag: capture the launch sequence in a hipGraph and replay it */
 
 /*
@@ -154,6 +162,11 @@ int rk_group_set_tuning(rk_group* g, const int32_t* tpp, int32_t wg_per_cu, int3
 int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t* chosen_tpp);
 /* solve all batches; flags: RK_USE_GRAPH (capture once per shape, then replay), RK_FROM_SEED */
 int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t flags, float* kernel_ms);
+/* per persistent kernel of the last eager (not graph-replayed) solve -- one per kernel variant,
+ * launched concurrently on forked streams: its threads-per-problem code, HIP-event time on its
+ * stream and the batches it ran (bit i = batch i of the group, first 32); *n = kernel count
+ * (0 after a graph replay) */
+int rk_group_kernel_times(rk_group* g, int32_t cap, int32_t* tpp, float* ms, int32_t* batch_mask, int32_t* n);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,8 @@ struct Mat3 {
 // ---------------------------------------------------------------------------------------------
 struct Robot {
   int nj, nl, na, nlevels;
+  int chain;    // serial arm: parent(j) = j-1, joints 0..na-1 actuated 1:1, the rest fixed
+  int nrounds;  // pointer-jumping rounds that cover the deepest chain: ceil(log2(depth + 1))
   int joint_type[RK_NJ_MAX];
   int parent[RK_NJ_MAX];
   int act[RK_NJ_MAX];    // actuated index or -1
@@ -41,7 +43,7 @@ struct Robot {
   float pT[RK_NJ_MAX][7];      // parent joint transforms
   int link_joint[RK_NL_MAX];   // link -> parent joint (-1 = base)
   float lo[RK_NA_MAX], hi[RK_NA_MAX];  // float32-narrowed actuated limits
-  int order[RK_NJ_MAX];        // joints sorted by depth (ties by index)
+  int order[RK_NJ_MAX];        // joints sorted by depth (identity: joints are renumbered in depth order)
   int level_start[RK_NJ_MAX + 1];
   unsigned long long anc[RK_NL_MAX];  // bit j: joint j is an ancestor of link (incl. its parent joint)
   int col_start[RK_NA_MAX + 1];       // CSR over actuated dofs: joints j (ascending) with map[j][a] != 0
@@ -52,7 +54,7 @@ struct Robot {
 // ---------------------------------------------------------------------------------------------
 // quaternion / SE(3)
 // ---------------------------------------------------------------------------------------------
-__host__ __device__ inline void cross3(const float* a, const float* b, float* o) {
+__host__ __device__ __forceinline__ void cross3(const float* a, const float* b, float* o) {
   float x = a[1] * b[2] - a[2] * b[1];
   float y = a[2] * b[0] - a[0] * b[2];
   float z = a[0] * b[1] - a[1] * b[0];
@@ -60,7 +62,7 @@ __host__ __device__ inline void cross3(const float* a, const float* b, float* o)
 }
 
 // quaternion_multiply_func + standardize (rotation_conversions.py:259-279)
-__host__ __device__ inline void quat_mul(const float* a, const float* b, float* o) {
+__host__ __device__ __forceinline__ void quat_mul(const float* a, const float* b, float* o) {
   float w = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
   float x = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
   float y = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
@@ -70,7 +72,7 @@ __host__ __device__ inline void quat_mul(const float* a, const float* b, float* 
 }
 
 // v + 2 (w (u x v) + u x (u x v))   (rotation_conversions.py:193-200)
-__host__ __device__ inline void quat_apply(const float* q, const float* v, float* o) {
+__host__ __device__ __forceinline__ void quat_apply(const float* q, const float* v, float* o) {
   float u[3] = {q[1], q[2], q[3]};
   float uv[3], uuv[3];
   cross3(u, v, uv);
@@ -80,7 +82,7 @@ __host__ __device__ inline void quat_apply(const float* q, const float* v, float
   o[2] = v[2] + 2.f * (q[0] * uv[2] + uuv[2]);
 }
 
-__host__ __device__ inline Mat3 quat_to_mat(const float* q) {
+__host__ __device__ __forceinline__ Mat3 quat_to_mat(const float* q) {
   float w = q[0], x = q[1], y = q[2], z = q[3];
   float s = 2.f / (w * w + x * x + y * y + z * z);
   Mat3 R;
@@ -96,14 +98,14 @@ __host__ __device__ inline Mat3 quat_to_mat(const float* q) {
   return R;
 }
 
-__host__ __device__ inline void mat3_vec(const Mat3& A, const float* v, float* o) {
+__host__ __device__ __forceinline__ void mat3_vec(const Mat3& A, const float* v, float* o) {
   float a = A.m[0] * v[0] + A.m[1] * v[1] + A.m[2] * v[2];
   float b = A.m[3] * v[0] + A.m[4] * v[1] + A.m[5] * v[2];
   float c = A.m[6] * v[0] + A.m[7] * v[1] + A.m[8] * v[2];
   o[0] = a; o[1] = b; o[2] = c;
 }
 
-__host__ __device__ inline Pose compose(const Pose& a, const Pose& b) {
+__host__ __device__ __forceinline__ Pose compose(const Pose& a, const Pose& b) {
   Pose o;
   quat_mul(a.q, b.q, o.q);
   float t[3];
@@ -112,7 +114,7 @@ __host__ __device__ inline Pose compose(const Pose& a, const Pose& b) {
   return o;
 }
 
-__host__ __device__ inline Pose inverse(const Pose& a) {
+__host__ __device__ __forceinline__ Pose inverse(const Pose& a) {
   Pose o;
   o.q[0] = a.q[0]; o.q[1] = -a.q[1]; o.q[2] = -a.q[2]; o.q[3] = -a.q[3];
   float t[3];
@@ -122,7 +124,7 @@ __host__ __device__ inline Pose inverse(const Pose& a) {
 }
 
 // quaternion_to_axis_angle_func (rotation_conversions.py:226-246)
-__device__ inline void quat_to_axis_angle(const float* q, float* aa) {
+__device__ __forceinline__ void quat_to_axis_angle(const float* q, float* aa) {
   float norm = sqrtf(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
   float half = atan2f(norm, q[0]);
   float angle = 2.f * half;
@@ -133,7 +135,7 @@ __device__ inline void quat_to_axis_angle(const float* q, float* aa) {
 }
 
 // hat(w)^2 = w w^T - |w|^2 I ; these helpers form I + a K + b K^2 applied to v without 3x3 temporaries.
-__device__ inline void ikk_apply(float a, float b, const float* w, const float* v, float* o) {
+__device__ __forceinline__ void ikk_apply(float a, float b, const float* w, const float* v, float* o) {
   // (I + a K + b K^2) v = v + a (w x v) + b (w x (w x v))
   float wv[3], wwv[3];
   cross3(w, v, wv);
@@ -144,7 +146,7 @@ __device__ inline void ikk_apply(float a, float b, const float* w, const float* 
 }
 
 // se3_log_map_func with se3_V_matrix_inverse_func (se3_kernels.py:230-309), eps = 1e-4
-__device__ inline void se3_log(const Pose& T, float* out) {
+__device__ __forceinline__ void se3_log(const Pose& T, float* out) {
   const float eps = 1e-4f;
   float w[3];
   quat_to_axis_angle(T.q, w);
@@ -167,7 +169,7 @@ __device__ inline void se3_log(const Pose& T, float* out) {
 
 // se3_jlog_func (se3_kernels.py:358-428), eps = 1e-4.  Returns the 3x3 blocks of
 // [[A, B],[0, A]] with A = so3_jac_left_inv(w)^T.
-__device__ inline void se3_jlog_blocks(const Pose& T, Mat3& A, Mat3& B) {
+__device__ __forceinline__ void se3_jlog_blocks(const Pose& T, Mat3& A, Mat3& B) {
   const float eps = 1e-4f;
   const float* p = T.t;
   float w[3];
@@ -226,7 +228,8 @@ __device__ inline void se3_jlog_blocks(const Pose& T, Mat3& A, Mat3& B) {
 }
 
 // joint local transform P_j * M(q_j)   (_compute_joint_local_transform_func, robot_kernels.py:264-297)
-__device__ inline Pose joint_local(const Robot* rb, int j, const float* q) {
+template <class RB>
+__device__ __forceinline__ Pose joint_local(RB rb, int j, const float* q) {
   float pos;
   int mi = rb->mimic[j];
   if (mi == -1) pos = rb->act[j] == -1 ? 0.f : q[rb->act[j]];
@@ -250,17 +253,18 @@ __device__ inline Pose joint_local(const Robot* rb, int j, const float* q) {
   return compose(P, X);
 }
 
-__device__ inline void load_pose(const float* p, Pose& T) {
+__device__ __forceinline__ void load_pose(const float* p, Pose& T) {
   T.t[0] = p[0]; T.t[1] = p[1]; T.t[2] = p[2];
   T.q[0] = p[3]; T.q[1] = p[4]; T.q[2] = p[5]; T.q[3] = p[6];
 }
-__device__ inline void store_pose(float* p, const Pose& T) {
+__device__ __forceinline__ void store_pose(float* p, const Pose& T) {
   p[0] = T.t[0]; p[1] = T.t[1]; p[2] = T.t[2];
   p[3] = T.q[0]; p[4] = T.q[1]; p[5] = T.q[2]; p[6] = T.q[3];
 }
 
 // world motion-subspace column S_j = Ad(T_wj) twist_j  (robot_kernels.py:650-664)
-__device__ inline void subspace_column(const Robot* rb, int j, const Pose& T, float* S6) {
+template <class RB>
+__device__ __forceinline__ void subspace_column(RB rb, int j, const Pose& T, float* S6) {
   int jt = rb->joint_type[j];
   float a[3] = {rb->axis[j][0], rb->axis[j][1], rb->axis[j][2]};
   float Ra[3];

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -43,6 +44,10 @@ struct Terms {
   float pos_w[RK_MAXQ][3];
   float limit_w[RK_NA_MAX];
   float smooth_w[RK_NA_MAX];
+  // structure of the Cholesky factor of J^T J + lam I (symbolic factorisation at batch creation):
+  // fill_col[k] bit j (j > k): L[j][k] can be nonzero; fill_row[k] bit i (i < k): L[k][i] can be
+  unsigned fill_col[RK_NA_MAX];
+  unsigned fill_row[RK_NA_MAX];
 };
 
 struct LMParams {
@@ -106,31 +111,32 @@ __device__ __forceinline__ void tri_index(int k, int& i, int& j) {
 // per-problem building blocks shared by the fused and plain forms
 // ---------------------------------------------------------------------------------------------
 
-// Pose residual of target t (+ the per-target Jacobian blocks when `tg` is non-null).
-// r = w * log(T_f^-1 T_tgt) (frame_task.py:505-554); the Jacobian uses jlog(T_tgt^-1 T_f) and
-// Ad(T_f^-1) (frame_task.py:391-456, 557-621).
-__device__ __forceinline__ void pose_target(const Pose& Tf, const Pose& Tt, const float* w, float* r6, float* tg) {
-  Pose Tfi = inverse(Tf);
-  Pose E = compose(Tfi, Tt);
+// Pose residual of a target: r = w * log(T_f^-1 T_tgt) (frame_task.py:505-554).
+__device__ __forceinline__ void pose_target(const Pose& Tf, const Pose& Tt, const float* w, float* r6) {
+  Pose E = compose(inverse(Tf), Tt);
   float e[6];
   se3_log(E, e);
 #pragma unroll
   for (int i = 0; i < 6; ++i) r6[i] = w[i] * e[i];
-  if (tg) {
-    Pose F = compose(inverse(Tt), Tf);
-    Mat3 A, B;
-    se3_jlog_blocks(F, A, B);
-    Mat3 R = quat_to_mat(Tfi.q);
+}
+
+// Per-target Jacobian blocks: jlog(T_tgt^-1 T_f) and Ad(T_f^-1) (frame_task.py:391-456, 557-621),
+// as R(T_f^-1) 9, t(T_f^-1) 3, jlog A 9, B 9, weights 6.
+__device__ __forceinline__ void pose_blocks(const Pose& Tf, const Pose& Tt, const float* w, float* tg) {
+  Pose Tfi = inverse(Tf);
+  Pose F = compose(inverse(Tt), Tf);
+  Mat3 A, B;
+  se3_jlog_blocks(F, A, B);
+  Mat3 R = quat_to_mat(Tfi.q);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) tg[i] = R.m[i];
-    tg[9] = Tfi.t[0]; tg[10] = Tfi.t[1]; tg[11] = Tfi.t[2];
+  for (int i = 0; i < 9; ++i) tg[i] = R.m[i];
+  tg[9] = Tfi.t[0]; tg[10] = Tfi.t[1]; tg[11] = Tfi.t[2];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) tg[12 + i] = A.m[i];
+  for (int i = 0; i < 9; ++i) tg[12 + i] = A.m[i];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) tg[21 + i] = B.m[i];
+  for (int i = 0; i < 9; ++i) tg[21 + i] = B.m[i];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) tg[30 + i] = w[i];
-  }
+  for (int i = 0; i < 6; ++i) tg[30 + i] = w[i];
 }
 
 // spatial column of link `link` for actuated dof a: sum of S_j * map[j][a] over ancestor joints
@@ -183,11 +189,14 @@ __device__ __forceinline__ void pos_jac_column(const float* sc, const float* p, 
   for (int i = 0; i < 3; ++i) out3[i] = -w[i] * (sc[i] + wxp[i]);
 }
 
-__device__ __forceinline__ float limit_row(float x, float lo, float hi, float w, float* jdiag) {
+// joint-limit row, "abs" mode (position_limit.py:508-534) and its diagonal Jacobian (:537-573)
+__device__ __forceinline__ float limit_row(float x, float lo, float hi, float w) {
   float eu = fmaxf(x - hi, 0.f);
   float el = fmaxf(lo - x, 0.f);
-  if (jdiag) *jdiag = w * (x > hi ? 1.f : (x < lo ? -1.f : 0.f));
   return w * (eu + el);
+}
+__device__ __forceinline__ float limit_diag(float x, float lo, float hi, float w) {
+  return w * (x > hi ? 1.f : (x < lo ? -1.f : 0.f));
 }
 
 // cost = sum over terms (in row order) of 0.5 * sum of squares of its rows
@@ -269,7 +278,8 @@ __device__ __forceinline__ void residual_rows(const Robot* rb, const Terms* tm, 
     Pose Tf = link_pose<C>(rb, T, tm->pose_link[t]);
     Pose Tt;
     load_pose(sh->tp[t], Tt);
-    pose_target(Tf, Tt, tm->pose_w[t], sh->r + 6 * t, with_jac ? sh->tg[t] : nullptr);
+    pose_target(Tf, Tt, tm->pose_w[t], sh->r + 6 * t);
+    if (with_jac) pose_blocks(Tf, Tt, tm->pose_w[t], sh->tg[t]);
   }
   for (int u = lane; u < tm->n_pos; u += G) {
     Pose Tf = link_pose<C>(rb, T, tm->pos_link[u]);
@@ -278,8 +288,10 @@ __device__ __forceinline__ void residual_rows(const Robot* rb, const Terms* tm, 
   }
   const int off_l = tm->Rf + tm->Rq;
   if (tm->limit)
-    for (int d = lane; d < na; d += G)
-      sh->r[off_l + d] = limit_row(q[d], rb->lo[d], rb->hi[d], tm->limit_w[d], with_jac ? &sh->jl[d] : nullptr);
+    for (int d = lane; d < na; d += G) {
+      sh->r[off_l + d] = limit_row(q[d], rb->lo[d], rb->hi[d], tm->limit_w[d]);
+      if (with_jac) sh->jl[d] = limit_diag(q[d], rb->lo[d], rb->hi[d], tm->limit_w[d]);
+    }
   const int off_s = off_l + (tm->limit ? na : 0);
   if (tm->smooth)
     for (int d = lane; d < na; d += G) {
@@ -512,13 +524,33 @@ __global__ __launch_bounds__(64) void k_fused(Problems P, LMParams lp, int it0, 
   solve_tile<G, C>(P, blockIdx.x * (WAVE / G), lp, it0, iters, lds);
 }
 
-// variants of (threads per problem, LDS configuration)
-enum Variant { V_S4 = 0, V_S8, V_S16, V_L16, V_L32, V_L64, NVARIANT };
+#include "rk_thread.h"
+#include "rk_wave.h"
+
+// kernel variants: (threads per problem, data placement)
+//   V_S*/V_L*: G lanes per problem, LDS-staged (small / large LDS configuration)
+//   V_T*:      one lane per problem, register-resident, serial arms with NA dofs
+//   V_W*:      one wavefront per problem, register-resident rows (trees, <= 32 dofs)
+enum Variant { V_S4 = 0, V_S8, V_S16, V_L16, V_L32, V_L64, V_T5, V_T6, V_T7, V_T8, V_W40, V_W012, NVARIANT };
+__host__ __device__ constexpr bool variant_wave(int v) { return v == V_W40 || v == V_W012; }
+__host__ __device__ constexpr bool variant_thread(int v) { return v >= V_T5 && v <= V_T8; }
 __host__ __device__ constexpr int variant_G(int v) {
-  return v == V_S4 ? 4 : v == V_S8 ? 8 : v == V_S16 ? 16 : v == V_L16 ? 16 : v == V_L32 ? 32 : 64;
+  return variant_thread(v) ? 1
+         : variant_wave(v) ? 64
+         : v == V_S4       ? 4
+         : v == V_S8       ? 8
+         : v == V_S16      ? 16
+         : v == V_L16      ? 16
+         : v == V_L32      ? 32
+                           : 64;
 }
 __host__ __device__ constexpr bool variant_small(int v) { return v <= V_S16; }
+// problems per workgroup (one wavefront)
+__host__ __device__ constexpr int variant_ppw(int v) { return variant_thread(v) ? WAVE : WAVE / variant_G(v); }
 inline size_t variant_lds(int v) {
+  if (variant_thread(v)) return 0;
+  if (v == V_W40) return sizeof(WaveLDS<4, 0>);
+  if (v == V_W012) return sizeof(WaveLDS<0, 12>);
   return variant_small(v) ? (WAVE / variant_G(v)) * sizeof(Shared<CfgS>)
                           : (WAVE / variant_G(v)) * sizeof(Shared<CfgL>);
 }
@@ -531,7 +563,30 @@ struct Bucket {
   int pad;
 };
 
-// persistent uneven-batch kernel: each workgroup (one wavefront) pulls tiles from the queue
+// one tile (one wavefront's worth of problems) of bucket bk
+template <int V>
+__device__ __forceinline__ void run_tile(const Bucket* bk, int local, const LMParams& lp, int it0, int iters,
+                                         char* lds) {
+  const Problems P = bk->P;
+  if constexpr (V == V_S4) solve_tile<4, CfgS>(P, local * 16, lp, it0, iters, lds);
+  else if constexpr (V == V_S8) solve_tile<8, CfgS>(P, local * 8, lp, it0, iters, lds);
+  else if constexpr (V == V_S16) solve_tile<16, CfgS>(P, local * 4, lp, it0, iters, lds);
+  else if constexpr (V == V_L16) solve_tile<16, CfgL>(P, local * 4, lp, it0, iters, lds);
+  else if constexpr (V == V_L32) solve_tile<32, CfgL>(P, local * 2, lp, it0, iters, lds);
+  else if constexpr (V == V_L64) solve_tile<64, CfgL>(P, local, lp, it0, iters, lds);
+  else if constexpr (V == V_T5) thread_solve<5, 7>(P, local * WAVE + threadIdx.x, lp, it0, iters);
+  else if constexpr (V == V_T6) thread_solve<6, 8>(P, local * WAVE + threadIdx.x, lp, it0, iters);
+  else if constexpr (V == V_T7) thread_solve<7, 9>(P, local * WAVE + threadIdx.x, lp, it0, iters);
+  else if constexpr (V == V_T8) thread_solve<8, 10>(P, local * WAVE + threadIdx.x, lp, it0, iters);
+  else if constexpr (V == V_W40) wave_solve<4, 0>(P, local, lp, it0, iters, lds);
+  else wave_solve<0, 12>(P, local, lp, it0, iters, lds);
+}
+
+// Persistent uneven-batch kernel of one variant: each workgroup (one wavefront) pulls tiles of
+// the variant's buckets from its queue until the queue is empty.  A group launches one such
+// kernel per variant it holds, concurrently (forked streams, one graph); a kernel per variant
+// gives each its own register allocation instead of the worst case of all.
+template <int V>
 __global__ __launch_bounds__(64) void k_persistent(const Bucket* __restrict__ buckets, int nb, int total_tiles,
                                                    int* counter, LMParams lp, int it0, int iters) {
   extern __shared__ __align__(16) char lds[];
@@ -544,16 +599,7 @@ __global__ __launch_bounds__(64) void k_persistent(const Bucket* __restrict__ bu
     if (tile >= total_tiles) break;
     int b = 0;
     while (b + 1 < nb && tile >= buckets[b + 1].tile_base) ++b;
-    const Bucket& bk = buckets[b];
-    const int local = tile - bk.tile_base;
-    switch (bk.variant) {
-      case V_S4: solve_tile<4, CfgS>(bk.P, local * 16, lp, it0, iters, lds); break;
-      case V_S8: solve_tile<8, CfgS>(bk.P, local * 8, lp, it0, iters, lds); break;
-      case V_S16: solve_tile<16, CfgS>(bk.P, local * 4, lp, it0, iters, lds); break;
-      case V_L16: solve_tile<16, CfgL>(bk.P, local * 4, lp, it0, iters, lds); break;
-      case V_L32: solve_tile<32, CfgL>(bk.P, local * 2, lp, it0, iters, lds); break;
-      default: solve_tile<64, CfgL>(bk.P, local * 1, lp, it0, iters, lds); break;
-    }
+    run_tile<V>(buckets + b, tile - buckets[b].tile_base, lp, it0, iters, lds);
   }
 }
 
@@ -630,7 +676,7 @@ __global__ void k_plain_pose_res(Problems P, const float* Tj, float* r) {
   for (int c = 0; c < 7; ++c) tv[c] = P.pose_tgt[(t * 7 + c) * S + b];
   load_pose(tv, Tt);
   float r6[6];
-  pose_target(Tf, Tt, tm->pose_w[t], r6, nullptr);
+  pose_target(Tf, Tt, tm->pose_w[t], r6);
   for (int i = 0; i < 6; ++i) r[(6 * t + i) * S + b] = r6[i];
 }
 
@@ -649,8 +695,8 @@ __global__ void k_plain_pose_jac(Problems P, const float* Tj, const float* Sw, f
     for (int c = 0; c < 7; ++c) tv[c] = P.pose_tgt[(t * 7 + c) * S + b];
     Pose Tt;
     load_pose(tv, Tt);
-    float r6[6], tg[36];
-    pose_target(Tf, Tt, tm->pose_w[t], r6, tg);
+    float tg[36];
+    pose_blocks(Tf, Tt, tm->pose_w[t], tg);
     float sc[6], c6[6];
     spatial_column(rb, tm->pose_link[t], a, S_at, sc);
     pose_jac_column(tg, sc, c6);
@@ -697,9 +743,8 @@ __global__ void k_plain_diag_rows(Problems P, const float* qsrc, float* r, float
   const float x = qsrc[d * S + b];
   int off = tm->Rf + tm->Rq;
   if (tm->limit) {
-    float jd;
-    r[(off + d) * S + b] = limit_row(x, rb->lo[d], rb->hi[d], tm->limit_w[d], &jd);
-    if (J) J[((off + d) * na + d) * S + b] = jd;
+    r[(off + d) * S + b] = limit_row(x, rb->lo[d], rb->hi[d], tm->limit_w[d]);
+    if (J) J[((off + d) * na + d) * S + b] = limit_diag(x, rb->lo[d], rb->hi[d], tm->limit_w[d]);
     off += na;
   }
   if (tm->smooth) {
@@ -877,6 +922,13 @@ __global__ void k_link_jacobian(const Robot* rb, int B, const float* q, int link
   }
 }
 
+// plain streaming copy (16 B per lane per step): a kernel of exactly known HBM traffic, used to
+// calibrate the FETCH_SIZE / WRITE_SIZE counters
+__global__ void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, long n4) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void k_se3_op(int op, int n, const float* a, const float* bb, float* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -986,6 +1038,7 @@ struct rk_context {
   int cu_count = 0;
   int64_t mem_bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::map<int, int> tuned;  // batch signature -> threads per problem chosen by the autotuner
 };
 
 struct rk_batch {
@@ -1008,7 +1061,12 @@ struct rk_group {
   std::vector<int> tpp;  // chosen threads per problem, 0 = default
   int wg_per_cu = 0, iters_per_launch = 0;
   Bucket* buckets_dev = nullptr;
-  int* counter = nullptr;
+  int* counter = nullptr;  // one queue counter per variant
+  hipStream_t aux[NVARIANT] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[NVARIANT] = {};
+  hipEvent_t ev_k0[NVARIANT] = {}, ev_k1[NVARIANT] = {};  // per-variant kernel timing
+  std::vector<int> last_var;                               // variants of the last solve, launch order
+  std::vector<unsigned> last_mask;                         // batches (bit = index) of each of those kernels
   hipGraphExec_t graph = nullptr;
   int graph_iters = -1;
   std::vector<float> graph_key;
@@ -1056,10 +1114,45 @@ static bool fits_small(const rk_batch* b, const Robot& r) {
   return r.nj <= CfgS::nj && r.na <= CfgS::na && b->th.n_pose <= CfgS::np && b->th.n_pos <= CfgS::nq;
 }
 
+// register thread-per-problem kernels: serial arms with 5..8 dofs, <= 2 fixed tail joints,
+// <= 1 pose target, no position targets
+static bool fits_thread(const rk_batch* b, const Robot& r) {
+  return r.chain && r.na >= 5 && r.na <= 8 && r.nj <= r.na + 2 && b->th.n_pose <= 1 && b->th.n_pos == 0;
+}
+
+// register wave-per-problem kernels: trees with <= 64 joints, <= 32 dofs, each dof driven by
+// <= 2 joints; <= 4 pose targets and no position targets, or <= 12 position targets and no pose
+static int wave_kind(const rk_batch* b, const Robot& r) {
+  if (r.nj > 64 || r.na > WV_NA) return -1;
+  for (int a = 0; a < r.na; ++a)
+    if (r.col_start[a + 1] - r.col_start[a] > 2) return -1;
+  if (b->th.n_pos == 0 && b->th.n_pose <= 4) return V_W40;
+  if (b->th.n_pose == 0 && b->th.n_pos <= 12) return V_W012;
+  return -1;
+}
+
+static int signature(const rk_batch* b) {
+  return ((b->th.robot * 8 + b->th.n_pose) * 16 + b->th.n_pos) * 4 + b->th.limit * 2 + b->th.smooth;
+}
+
+// threads per problem -> variant.  G = 0: the tuned choice for this batch's signature if the
+// autotuner made one on this context, else the default (registers where they fit).
 static int pick_variant(const rk_batch* b, int G, int* v) {
   const Robot& r = b->ctx->robots[b->th.robot];
   const bool small = fits_small(b, r);
-  if (G == 0) G = small ? 8 : 64;
+  if (G == 0) {
+    auto it = b->ctx->tuned.find(signature(b));
+    if (it != b->ctx->tuned.end()) G = it->second;
+    else G = fits_thread(b, r) ? 1 : wave_kind(b, r) >= 0 ? RK_TPP_WAVE : small ? 8 : 64;
+  }
+  if (G == 1 && fits_thread(b, r)) {
+    *v = V_T5 + (r.na - 5);
+    return RK_OK;
+  }
+  if (G == RK_TPP_WAVE && wave_kind(b, r) >= 0) {
+    *v = wave_kind(b, r);
+    return RK_OK;
+  }
   if (small && (G == 4 || G == 8 || G == 16)) {
     *v = G == 4 ? V_S4 : G == 8 ? V_S8 : V_S16;
     return RK_OK;
@@ -1068,12 +1161,18 @@ static int pick_variant(const rk_batch* b, int G, int* v) {
     *v = G == 16 ? V_L16 : G == 32 ? V_L32 : V_L64;
     return RK_OK;
   }
-  return fail(RK_EINVAL, "threads_per_problem must be 4, 8, 16 (small robots) or 16, 32, 64");
+  return fail(RK_EINVAL,
+              "threads_per_problem must be 1 (serial arms), -64 (register wave: trees), 4, 8, 16 (small robots) "
+              "or 16, 32, 64");
 }
 
 static void legal_tpps(const rk_batch* b, std::vector<int>& out) {
-  if (fits_small(b, b->ctx->robots[b->th.robot])) out = {4, 8, 16, 32, 64};
-  else out = {16, 32, 64};
+  const Robot& r = b->ctx->robots[b->th.robot];
+  out.clear();
+  if (fits_thread(b, r)) out.push_back(1);
+  if (wave_kind(b, r) >= 0) out.push_back(RK_TPP_WAVE);
+  if (fits_small(b, r)) out.insert(out.end(), {4, 8, 16, 32, 64});
+  else out.insert(out.end(), {16, 32, 64});
 }
 
 template <int G, class C>
@@ -1097,8 +1196,20 @@ static int launch_variant(rk_context* ctx, int v, const Problems& P, const LMPar
     case V_S16: return launch_fused<16, CfgS>(ctx, P, lp, it0, iters);
     case V_L16: return launch_fused<16, CfgL>(ctx, P, lp, it0, iters);
     case V_L32: return launch_fused<32, CfgL>(ctx, P, lp, it0, iters);
-    default: return launch_fused<64, CfgL>(ctx, P, lp, it0, iters);
+    case V_L64: return launch_fused<64, CfgL>(ctx, P, lp, it0, iters);
+    default: break;
   }
+  const dim3 grid((P.n + WAVE - 1) / WAVE);
+  switch (v) {
+    case V_T5: hipLaunchKernelGGL((k_thread<5, 7>), grid, dim3(WAVE), 0, ctx->stream, P, lp, it0, iters); break;
+    case V_T6: hipLaunchKernelGGL((k_thread<6, 8>), grid, dim3(WAVE), 0, ctx->stream, P, lp, it0, iters); break;
+    case V_T7: hipLaunchKernelGGL((k_thread<7, 9>), grid, dim3(WAVE), 0, ctx->stream, P, lp, it0, iters); break;
+    case V_T8: hipLaunchKernelGGL((k_thread<8, 10>), grid, dim3(WAVE), 0, ctx->stream, P, lp, it0, iters); break;
+    case V_W40: hipLaunchKernelGGL((k_wave<4, 0>), dim3(P.n), dim3(WAVE), 0, ctx->stream, P, lp, it0, iters); break;
+    default: hipLaunchKernelGGL((k_wave<0, 12>), dim3(P.n), dim3(WAVE), 0, ctx->stream, P, lp, it0, iters); break;
+  }
+  HIP_TRY(hipGetLastError());
+  return RK_OK;
 }
 
 static LMParams to_lp(const rk_lm_params* p) {
@@ -1268,39 +1379,22 @@ int rk_robot_upload(rk_context* ctx, const rk_robot_desc* d, int32_t* robot_id) 
   if ((int)ctx->robots.size() >= RK_MAX_ROBOTS) return fail(RK_ENOMEM, "too many robots on this context");
   if (d->nj < 1 || d->nj > RK_NJ_MAX || d->nl < 1 || d->nl > RK_NL_MAX || d->na < 1 || d->na > RK_NA_MAX)
     return fail(RK_EINVAL, "robot size out of range (nj, nl <= 64, 1 <= na <= 32)");
-  Robot r;
-  memset(&r, 0, sizeof(r));
-  r.nj = d->nj;
-  r.nl = d->nl;
-  r.na = d->na;
+  // validate, then renumber joints in depth order (parents before children) so that the
+  // register kernels can index joints with compile-time indices
   for (int j = 0; j < d->nj; ++j) {
-    r.joint_type[j] = d->joint_type[j];
-    r.parent[j] = d->parent_joint[j];
-    r.act[j] = d->act_idx[j];
-    r.mimic[j] = d->mimic_idx[j];
-    r.mul[j] = d->mimic_mul[j];
-    r.off[j] = d->mimic_off[j];
-    for (int k = 0; k < 3; ++k) r.axis[j][k] = d->joint_axis[3 * j + k];
-    for (int k = 0; k < 7; ++k) r.pT[j][k] = d->parent_T[7 * j + k];
-    if (r.parent[j] < -1 || r.parent[j] >= d->nj || r.act[j] >= d->na || r.mimic[j] >= d->na ||
-        r.joint_type[j] < 0 || r.joint_type[j] > 2)
+    if (d->parent_joint[j] < -1 || d->parent_joint[j] >= d->nj || d->act_idx[j] >= d->na ||
+        d->mimic_idx[j] >= d->na || d->joint_type[j] < 0 || d->joint_type[j] > 2)
       return fail(RK_EINVAL, "invalid joint index or type");
   }
-  for (int l = 0; l < d->nl; ++l) {
-    r.link_joint[l] = d->link_parent_joint[l];
-    if (r.link_joint[l] < -1 || r.link_joint[l] >= d->nj) return fail(RK_EINVAL, "invalid link parent joint");
-  }
-  for (int a = 0; a < d->na; ++a) {
-    r.lo[a] = d->limits[2 * a];
-    r.hi[a] = d->limits[2 * a + 1];
-  }
-  // depth order
+  for (int l = 0; l < d->nl; ++l)
+    if (d->link_parent_joint[l] < -1 || d->link_parent_joint[l] >= d->nj)
+      return fail(RK_EINVAL, "invalid link parent joint");
   std::vector<int> depth(d->nj, -1);
   for (int pass = 0; pass < d->nj + 1; ++pass) {
     bool changed = false;
     for (int j = 0; j < d->nj; ++j) {
       if (depth[j] >= 0) continue;
-      const int p = r.parent[j];
+      const int p = d->parent_joint[j];
       if (p < 0) depth[j] = 0, changed = true;
       else if (depth[p] >= 0) depth[j] = depth[p] + 1, changed = true;
     }
@@ -1311,14 +1405,51 @@ int rk_robot_upload(rk_context* ctx, const rk_robot_desc* d, int32_t* robot_id) 
     if (depth[j] < 0) return fail(RK_EINVAL, "circular joint parents");
     maxd = std::max(maxd, depth[j]);
   }
-  int k = 0;
-  r.nlevels = maxd + 1;
-  for (int lv = 0; lv <= maxd; ++lv) {
-    r.level_start[lv] = k;
+  std::vector<int> new2old, old2new(d->nj);
+  for (int lv = 0; lv <= maxd; ++lv)
     for (int j = 0; j < d->nj; ++j)
-      if (depth[j] == lv) r.order[k++] = j;
+      if (depth[j] == lv) new2old.push_back(j);
+  for (int n = 0; n < d->nj; ++n) old2new[new2old[n]] = n;
+  Robot r;
+  memset(&r, 0, sizeof(r));
+  r.nj = d->nj;
+  r.nl = d->nl;
+  r.na = d->na;
+  for (int n = 0; n < d->nj; ++n) {
+    const int j = new2old[n];
+    r.joint_type[n] = d->joint_type[j];
+    r.parent[n] = d->parent_joint[j] < 0 ? -1 : old2new[d->parent_joint[j]];
+    r.act[n] = d->act_idx[j];
+    r.mimic[n] = d->mimic_idx[j];
+    r.mul[n] = d->mimic_mul[j];
+    r.off[n] = d->mimic_off[j];
+    for (int k = 0; k < 3; ++k) r.axis[n][k] = d->joint_axis[3 * j + k];
+    for (int k = 0; k < 7; ++k) r.pT[n][k] = d->parent_T[7 * j + k];
+    r.order[n] = n;
   }
-  r.level_start[maxd + 1] = k;
+  for (int l = 0; l < d->nl; ++l) r.link_joint[l] = d->link_parent_joint[l] < 0 ? -1 : old2new[d->link_parent_joint[l]];
+  for (int a = 0; a < d->na; ++a) {
+    r.lo[a] = d->limits[2 * a];
+    r.hi[a] = d->limits[2 * a + 1];
+  }
+  r.nlevels = maxd + 1;
+  {
+    int k = 0;
+    for (int lv = 0; lv <= maxd; ++lv) {
+      r.level_start[lv] = k;
+      while (k < d->nj && depth[new2old[k]] == lv) ++k;
+    }
+    r.level_start[maxd + 1] = k;
+  }
+  r.nrounds = 0;
+  while ((1 << r.nrounds) < maxd + 1) ++r.nrounds;
+  // serial arm: joint j's parent is j-1, joints 0..na-1 actuated 1:1, the rest fixed, no mimic
+  r.chain = 1;
+  for (int j = 0; j < r.nj; ++j) {
+    if (r.parent[j] != j - 1 || r.mimic[j] != -1) r.chain = 0;
+    if (j < r.na && (r.act[j] != j || r.joint_type[j] == 0)) r.chain = 0;
+    if (j >= r.na && (r.act[j] != -1 || r.joint_type[j] != 0)) r.chain = 0;
+  }
   for (int l = 0; l < d->nl; ++l) {
     unsigned long long m = 0;
     for (int j = r.link_joint[l]; j != -1; j = r.parent[j]) m |= 1ull << j;
@@ -1399,6 +1530,27 @@ int rk_link_jacobian(rk_context* ctx, int32_t robot, int32_t B, const float* q, 
   return RK_OK;
 }
 
+int rk_copy_probe(rk_context* ctx, int64_t bytes, float* kernel_ms) {
+  if (!ctx || bytes < 16 || bytes % 16) return fail(RK_EINVAL, "bytes must be a positive multiple of 16");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  float4 *src = nullptr, *dst = nullptr;
+  HIP_TRY(hipMalloc(&src, bytes));
+  HIP_TRY(hipMalloc(&dst, bytes));
+  const long n4 = bytes / 16;
+  const int grid = (int)std::min<long>((n4 + 255) / 256, 4L * 256 * ctx->cu_count);
+  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, ctx->stream, (const float4*)src, dst, n4);
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, ctx->stream, (const float4*)src, dst, n4);
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  hipError_t e = hipEventSynchronize(ctx->ev1);
+  if (e == hipSuccess && kernel_ms) e = hipEventElapsedTime(kernel_ms, ctx->ev0, ctx->ev1);
+  hipFree(src);
+  hipFree(dst);
+  if (e != hipSuccess) return fail(RK_EHIP, std::string("rk_copy_probe: ") + hipGetErrorString(e));
+  return RK_OK;
+}
+
 int rk_se3_op(rk_context* ctx, int32_t op, int32_t n, const float* a, const float* b, float* out) {
   if (!ctx || n <= 0 || !a || !out || op < 0 || op > 5 || ((op == 0 || op == 5) && !b))
     return fail(RK_EINVAL, "bad argument to rk_se3_op");
@@ -1455,6 +1607,38 @@ int rk_batch_create(rk_context* ctx, const rk_terms_desc* t, int32_t capacity, i
   for (int d = 0; d < r.na; ++d) {
     th.limit_w[d] = th.limit ? t->limit_weights[d] : 0.f;
     th.smooth_w[d] = th.smooth ? t->smooth_weights[d] : 0.f;
+  }
+  {
+    // pattern of J^T J: dofs i, j couple when both move the link of one target row block;
+    // then the symbolic Cholesky (fill-in) in dof order
+    const int na = r.na;
+    auto moves = [&](int a, int link) {
+      for (int c = r.col_start[a]; c < r.col_start[a + 1]; ++c)
+        if ((r.anc[link] >> r.col_joint[c]) & 1ull) return true;
+      return false;
+    };
+    std::vector<std::vector<char>> P(na, std::vector<char>(na, 0));
+    for (int i = 0; i < na; ++i) P[i][i] = 1;
+    std::vector<int> links;
+    for (int t = 0; t < th.n_pose; ++t) links.push_back(th.pose_link[t]);
+    for (int u = 0; u < th.n_pos; ++u) links.push_back(th.pos_link[u]);
+    for (int link : links)
+      for (int i = 0; i < na; ++i)
+        if (moves(i, link))
+          for (int j = 0; j < na; ++j)
+            if (moves(j, link)) P[i][j] = 1;
+    for (int k = 0; k < na; ++k)
+      for (int i = k + 1; i < na; ++i)
+        if (P[i][k])
+          for (int j = k + 1; j <= i; ++j)
+            if (P[j][k]) P[i][j] = P[j][i] = 1;
+    for (int k = 0; k < na; ++k) {
+      th.fill_col[k] = th.fill_row[k] = 0u;
+      for (int j = k + 1; j < na; ++j)
+        if (P[j][k]) th.fill_col[k] |= 1u << j;
+      for (int i = 0; i < k; ++i)
+        if (P[k][i]) th.fill_row[k] |= 1u << i;
+    }
   }
   th.Rf = 6 * th.n_pose;
   th.Rq = 3 * th.n_pos;
@@ -1673,7 +1857,7 @@ int rk_group_create(rk_context* ctx, int32_t nb, rk_batch* const* batches, rk_gr
   g->tpp.assign(nb, 0);
   int rc;
   if ((rc = dmalloc(ctx, (void**)&g->buckets_dev, sizeof(Bucket) * nb, &g->bytes)) ||
-      (rc = dmalloc(ctx, (void**)&g->counter, sizeof(int) * 4, &g->bytes))) {
+      (rc = dmalloc(ctx, (void**)&g->counter, sizeof(int) * NVARIANT, &g->bytes))) {
     delete g;
     return rc;
   }
@@ -1687,6 +1871,13 @@ int rk_group_destroy(rk_group* g) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   if (g->graph) hipGraphExecDestroy(g->graph);
+  for (int k = 0; k < NVARIANT; ++k) {
+    if (g->aux[k]) hipStreamDestroy(g->aux[k]);
+    if (g->ev_join[k]) hipEventDestroy(g->ev_join[k]);
+    if (g->ev_k0[k]) hipEventDestroy(g->ev_k0[k]);
+    if (g->ev_k1[k]) hipEventDestroy(g->ev_k1[k]);
+  }
+  if (g->ev_fork) hipEventDestroy(g->ev_fork);
   hipFree(g->buckets_dev);
   hipFree(g->counter);
   ctx->mem_bytes -= g->bytes;
@@ -1713,74 +1904,160 @@ int rk_group_set_tuning(rk_group* g, const int32_t* tpp, int32_t wg_per_cu, int3
   return RK_OK;
 }
 
+}  // extern "C"
+
 struct GroupLaunch {
-  int nb = 0, total = 0, grid = 0;
-  size_t lds = 0;
+  // per variant present: first bucket, bucket count, tiles, grid, dynamic LDS, batch mask
+  std::vector<int> var, first, nb, tiles, grid;
+  std::vector<unsigned> mask;
+  std::vector<size_t> lds;
 };
 
-// upload the bucket table (outside any capture) and size the persistent grid
+template <int V>
+static int variant_grid(rk_context* ctx, int wg_per_cu, size_t lds, int tiles, int* grid) {
+  static size_t attr_lds = 0;
+  if (lds > attr_lds) {
+    HIP_TRY(hipFuncSetAttribute((const void*)k_persistent<V>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_lds = lds;
+  }
+  int per_cu = wg_per_cu;
+  if (per_cu <= 0) {
+    int occ = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_persistent<V>, WAVE, lds));
+    per_cu = std::max(1, occ);
+  }
+  *grid = std::max(1, std::min(tiles, per_cu * ctx->cu_count));
+  return RK_OK;
+}
+
+static int variant_grid_any(rk_context* ctx, int v, int wg_per_cu, size_t lds, int tiles, int* grid) {
+  switch (v) {
+    case V_S4: return variant_grid<V_S4>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_S8: return variant_grid<V_S8>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_S16: return variant_grid<V_S16>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_L16: return variant_grid<V_L16>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_L32: return variant_grid<V_L32>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_L64: return variant_grid<V_L64>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_T5: return variant_grid<V_T5>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_T6: return variant_grid<V_T6>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_T7: return variant_grid<V_T7>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_T8: return variant_grid<V_T8>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_W40: return variant_grid<V_W40>(ctx, wg_per_cu, lds, tiles, grid);
+    default: return variant_grid<V_W012>(ctx, wg_per_cu, lds, tiles, grid);
+  }
+}
+
+static void launch_persistent(int v, dim3 grid, size_t lds, hipStream_t st, const Bucket* bk, int nb, int total,
+                              int* counter, const LMParams& lp, int it0, int iters) {
+  switch (v) {
+#define RK_LAUNCH_P(VV) \
+  case VV: hipLaunchKernelGGL(k_persistent<VV>, grid, dim3(WAVE), lds, st, bk, nb, total, counter, lp, it0, iters); break;
+    RK_LAUNCH_P(V_S4)
+    RK_LAUNCH_P(V_S8)
+    RK_LAUNCH_P(V_S16)
+    RK_LAUNCH_P(V_L16)
+    RK_LAUNCH_P(V_L32)
+    RK_LAUNCH_P(V_L64)
+    RK_LAUNCH_P(V_T5)
+    RK_LAUNCH_P(V_T6)
+    RK_LAUNCH_P(V_T7)
+    RK_LAUNCH_P(V_T8)
+    RK_LAUNCH_P(V_W40)
+    RK_LAUNCH_P(V_W012)
+#undef RK_LAUNCH_P
+    default: break;
+  }
+}
+
+// upload the bucket table (outside any capture) and size each variant's persistent grid
 static int group_prepare(rk_group* g, GroupLaunch* gl, bool from_seed) {
   rk_context* ctx = g->ctx;
   std::vector<Bucket> bk(g->batches.size());
-  size_t lds = 0;
   for (size_t i = 0; i < g->batches.size(); ++i) {
     int v;
     int rc = pick_variant(g->batches[i], g->tpp[i], &v);
     if (rc) return rc;
     bk[i].variant = v;
     bk[i].P = make_problems(g->batches[i], from_seed);
-    const int ppw = WAVE / variant_G(v);
+    const int ppw = variant_ppw(v);
     bk[i].tiles = (g->batches[i]->n + ppw - 1) / ppw;
-    lds = std::max(lds, variant_lds(v));
   }
-  // big problems first (longest-processing-time order)
+  // group by variant; within a variant, big problems first (longest-processing-time order)
   std::vector<int> order(bk.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (bk[a].variant != bk[b].variant) return bk[a].variant < bk[b].variant;
     return g->batches[a]->na * g->batches[a]->th.R > g->batches[b]->na * g->batches[b]->th.R;
   });
   std::vector<Bucket> sorted(bk.size());
+  *gl = GroupLaunch();
   int base = 0;
   for (size_t i = 0; i < order.size(); ++i) {
     sorted[i] = bk[order[i]];
+    const int v = sorted[i].variant;
+    if (gl->var.empty() || gl->var.back() != v) {
+      gl->var.push_back(v);
+      gl->first.push_back((int)i);
+      gl->nb.push_back(0);
+      gl->tiles.push_back(0);
+      gl->mask.push_back(0u);
+      base = 0;
+    }
+    if (order[i] < 32) gl->mask.back() |= 1u << order[i];
     sorted[i].tile_base = base;
     base += sorted[i].tiles;
+    gl->nb.back() += 1;
+    gl->tiles.back() = base;
   }
   HIP_TRY(hipMemcpyAsync(g->buckets_dev, sorted.data(), sizeof(Bucket) * sorted.size(), hipMemcpyHostToDevice,
                          ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
-  static size_t attr_lds = 0;
-  if (lds > attr_lds) {
-    HIP_TRY(hipFuncSetAttribute((const void*)k_persistent, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_lds = lds;
+  for (size_t k = 0; k < gl->var.size(); ++k) {
+    if (!g->aux[k]) HIP_TRY(hipStreamCreateWithFlags(&g->aux[k], hipStreamNonBlocking));
+    if (!g->ev_join[k]) HIP_TRY(hipEventCreateWithFlags(&g->ev_join[k], hipEventDisableTiming));
+    if (!g->ev_k0[k]) HIP_TRY(hipEventCreate(&g->ev_k0[k]));
+    if (!g->ev_k1[k]) HIP_TRY(hipEventCreate(&g->ev_k1[k]));
   }
-  int per_cu = g->wg_per_cu;
-  if (per_cu <= 0) {
-    int occ = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_persistent, WAVE, lds));
-    per_cu = std::max(1, occ);
+  if (!g->ev_fork) HIP_TRY(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
+  for (size_t k = 0; k < gl->var.size(); ++k) {
+    const size_t lds = variant_lds(gl->var[k]);
+    int grid;
+    int rc = variant_grid_any(ctx, gl->var[k], g->wg_per_cu, lds, gl->tiles[k], &grid);
+    if (rc) return rc;
+    gl->lds.push_back(lds);
+    gl->grid.push_back(grid);
   }
-  gl->nb = (int)sorted.size();
-  gl->total = base;
-  gl->lds = lds;
-  gl->grid = std::max(1, std::min(base, per_cu * ctx->cu_count));
   return RK_OK;
 }
 
-// enqueue the solve's launches on the context stream (capturable: no allocation, copy or memset)
-static int group_launch(rk_group* g, const GroupLaunch& gl, int iters, const LMParams& lp) {
+// enqueue the solve on the context stream: fork one stream per variant, each runs its
+// persistent kernel over its queue, join.  Capturable: no allocation, copy or memset.
+static int group_launch(rk_group* g, const GroupLaunch& gl, int iters, const LMParams& lp, bool timed) {
   rk_context* ctx = g->ctx;
   const int chunk = g->iters_per_launch > 0 ? g->iters_per_launch : std::max(1, iters);
-  for (int it0 = 0; it0 < std::max(1, iters); it0 += chunk) {
-    const int n_it = std::max(0, std::min(chunk, iters - it0));
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, ctx->stream, g->counter);
-    hipLaunchKernelGGL(k_persistent, dim3(gl.grid), dim3(WAVE), gl.lds, ctx->stream, (const Bucket*)g->buckets_dev,
-                       gl.nb, gl.total, g->counter, lp, it0, n_it);
-    if (iters == 0) break;
+  HIP_TRY(hipEventRecord(g->ev_fork, ctx->stream));
+  for (size_t k = 0; k < gl.var.size(); ++k) {
+    hipStream_t st = gl.var.size() == 1 ? ctx->stream : g->aux[k];
+    if (st != ctx->stream) HIP_TRY(hipStreamWaitEvent(st, g->ev_fork, 0));
+    if (timed) HIP_TRY(hipEventRecord(g->ev_k0[k], st));
+    for (int it0 = 0; it0 < std::max(1, iters); it0 += chunk) {
+      const int n_it = std::max(0, std::min(chunk, iters - it0));
+      hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, g->counter + k);
+      launch_persistent(gl.var[k], dim3(gl.grid[k]), gl.lds[k], st, (const Bucket*)g->buckets_dev + gl.first[k],
+                        gl.nb[k], gl.tiles[k], g->counter + k, lp, it0, n_it);
+      if (iters == 0) break;
+    }
+    if (timed) HIP_TRY(hipEventRecord(g->ev_k1[k], st));
+    if (st != ctx->stream) {
+      HIP_TRY(hipEventRecord(g->ev_join[k], st));
+      HIP_TRY(hipStreamWaitEvent(ctx->stream, g->ev_join[k], 0));
+    }
   }
   HIP_TRY(hipGetLastError());
   return RK_OK;
 }
+
+extern "C" {
 
 int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t flags, float* kernel_ms) {
   const bool use_graph = (flags & RK_USE_GRAPH) != 0;
@@ -1810,7 +2087,7 @@ int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t fl
       if ((rc = group_prepare(g, &gl, from_seed))) return rc;
       hipGraph_t graph;
       HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      rc = group_launch(g, gl, iters, lp);
+      rc = group_launch(g, gl, iters, lp, false);
       hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
       if (rc) return rc;
       if (e != hipSuccess) return fail(RK_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -1821,15 +2098,31 @@ int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t fl
     }
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     HIP_TRY(hipGraphLaunch(g->graph, ctx->stream));
+    g->last_var.clear();  // per-kernel times are recorded by eager solves only
+    g->last_mask.clear();
   } else {
     GroupLaunch gl;
     if ((rc = group_prepare(g, &gl, from_seed))) return rc;
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-    if ((rc = group_launch(g, gl, iters, lp))) return rc;
+    if ((rc = group_launch(g, gl, iters, lp, true))) return rc;
+    g->last_var = gl.var;
+    g->last_mask = gl.mask;
   }
   HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
   HIP_TRY(hipEventSynchronize(ctx->ev1));
   if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, ctx->ev0, ctx->ev1));
+  return RK_OK;
+}
+
+int rk_group_kernel_times(rk_group* g, int32_t cap, int32_t* tpp, float* ms, int32_t* batch_mask, int32_t* n) {
+  if (!g || !n) return fail(RK_EINVAL, "null argument");
+  const int m = (int)g->last_var.size();
+  *n = m;
+  for (int k = 0; k < m && k < cap; ++k) {
+    if (tpp) tpp[k] = variant_wave(g->last_var[k]) ? RK_TPP_WAVE : variant_G(g->last_var[k]);
+    if (ms) HIP_TRY(hipEventElapsedTime(&ms[k], g->ev_k0[k], g->ev_k1[k]));
+    if (batch_mask) batch_mask[k] = (int32_t)g->last_mask[k];
+  }
   return RK_OK;
 }
 
@@ -1886,6 +2179,7 @@ int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t
     b->q_trace = qt;
     b->lam_trace = lt;
     g->tpp[i] = best_g;
+    ctx->tuned[signature(b)] = best_g;
     if (chosen) chosen[i] = best_g;
   }
   if (g->graph) {

@@ -1,0 +1,265 @@
+// rk_thread.h -- thread-per-problem LM for serial arms (included by rk_lm.hip inside namespace rk).
+//
+// A serial arm here: after depth renumbering joint j's parent is j-1, joints 0..na-1 are actuated
+// one-to-one (dof a = joint a), the remaining joints are fixed.  Then every index in the solve is
+// a compile-time constant: the motion-subspace columns, the Jacobian (6 rows x NA columns),
+// J^T J, its Cholesky factor and the solves all live in VGPRs of one lane, robot and term data
+// are read through the constant address space (scalar loads), and there is no LDS and no
+// synchronisation.  FK is streamed joint by joint (no pose array is kept): once at the iterate
+// (with the subspace columns) and once at the proposal (to the target link only).
+// Column a of the pose rows is structurally zero when joint a is not an ancestor of the target
+// link (a > its parent joint); those products are skipped, which adds exact zeros only, so the
+// per-entry operation sequence equals the dense one of the LDS kernel.
+
+typedef const __attribute__((address_space(4))) Robot RobotK;
+typedef const __attribute__((address_space(4))) Terms TermsK;
+
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(const T* p) {
+  return (const __attribute__((address_space(4))) T*)p;
+}
+
+__device__ __forceinline__ Pose identity_pose() {
+  Pose P;
+  P.t[0] = P.t[1] = P.t[2] = 0.f;
+  P.q[0] = 1.f;
+  P.q[1] = P.q[2] = P.q[3] = 0.f;
+  return P;
+}
+
+// FK of a serial arm streamed joint by joint: T_j = T_{j-1} o (P_j o M(q_j)) (fixed tail joints
+// compose P_j only).  Returns the world pose of joint jt; with WITH_S also the motion-subspace
+// column of every actuated joint, computed as its pose is produced.
+template <int NA, int NJ, bool WITH_S>
+__device__ __forceinline__ Pose chain_fk(RobotK* rb, int nj, const float* q, int jt, float (&S6)[NA][6]) {
+  Pose T = identity_pose(), Tt = identity_pose();
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    if (j < nj) {
+      Pose L;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) L.t[k] = rb->pT[j][k];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) L.q[k] = rb->pT[j][3 + k];
+      if (j < NA) {
+        Pose X = identity_pose();
+        const float ax0 = rb->axis[j][0], ax1 = rb->axis[j][1], ax2 = rb->axis[j][2];
+        if (rb->joint_type[j] == 2) {
+          float s, c;
+          sincosf(q[j] * 0.5f, &s, &c);
+          X.q[0] = c; X.q[1] = ax0 * s; X.q[2] = ax1 * s; X.q[3] = ax2 * s;
+        } else {
+          X.t[0] = ax0 * q[j]; X.t[1] = ax1 * q[j]; X.t[2] = ax2 * q[j];
+        }
+        L = compose(L, X);
+      }
+      T = j == 0 ? L : compose(T, L);
+      if (WITH_S && j < NA) subspace_column(rb, j, T, S6[j]);
+      if (j == jt) Tt = T;
+    }
+  }
+  return Tt;
+}
+
+// residual rows at a point: pose rows of the target, limit rows and smoothness rows; with JAC
+// also the target's Jacobian blocks and the diagonal Jacobian entries of limit / smoothness rows
+template <int NA, bool JAC>
+__device__ __forceinline__ void chain_rows(RobotK* rb, TermsK* tm, const Pose& Tf, const Pose& tgt, const float* q,
+                                           const float* qprev, float (&r_pose)[6], float (&r_lim)[NA],
+                                           float (&r_sm)[NA], float (&tg)[36], float (&jl)[NA], float (&js)[NA]) {
+  if (tm->n_pose) {
+    float w[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) w[i] = tm->pose_w[0][i];
+    pose_target(Tf, tgt, w, r_pose);
+    if (JAC) pose_blocks(Tf, tgt, w, tg);
+  }
+  if (tm->limit) {
+#pragma unroll
+    for (int d = 0; d < NA; ++d) {
+      r_lim[d] = limit_row(q[d], rb->lo[d], rb->hi[d], tm->limit_w[d]);
+      if (JAC) jl[d] = limit_diag(q[d], rb->lo[d], rb->hi[d], tm->limit_w[d]);
+    }
+  }
+  if (tm->smooth) {
+#pragma unroll
+    for (int d = 0; d < NA; ++d) {
+      r_sm[d] = tm->smooth_w[d] * (q[d] - qprev[d]);
+      if (JAC) js[d] = tm->smooth_w[d];
+    }
+  }
+}
+
+// cost = sum over terms (pose, limit, smoothness) of 0.5 * sum of squares of the term's rows
+template <int NA>
+__device__ __forceinline__ float chain_cost(TermsK* tm, const float* r_pose, const float* r_lim, const float* r_sm) {
+  float cost = 0.f;
+  if (tm->n_pose) {
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s += r_pose[k] * r_pose[k];
+    cost = cost + 0.5f * s;
+  }
+  if (tm->limit) {
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < NA; ++k) s += r_lim[k] * r_lim[k];
+    cost = cost + 0.5f * s;
+  }
+  if (tm->smooth) {
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < NA; ++k) s += r_sm[k] * r_sm[k];
+    cost = cost + 0.5f * s;
+  }
+  return cost;
+}
+
+// One serial-arm problem per lane: <= 1 pose target, limit and smoothness rows.
+template <int NA, int NJ>
+__device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, const LMParams& lp, int it0, int iters) {
+  RobotK* rb = as_const(Pg.robot);
+  TermsK* tm = as_const(Pg.terms);
+  const int S = Pg.stride;
+  const bool valid = s_idx < Pg.n;
+  const int s = valid ? s_idx : Pg.n - 1;
+  const int nj = rb->nj;
+  const int n_pose = tm->n_pose;
+  const int jt = n_pose ? rb->link_joint[tm->pose_link[0]] : -1;
+
+  float q[NA], qprev[NA];
+#pragma unroll
+  for (int d = 0; d < NA; ++d) {
+    q[d] = (it0 == 0 && Pg.q_seed) ? Pg.q_seed[d * S + s] : Pg.q[d * S + s];
+    qprev[d] = tm->smooth ? Pg.q_prev[d * S + s] : 0.f;
+  }
+  Pose tgt = identity_pose();
+  if (n_pose) {
+    float v[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) v[c] = Pg.pose_tgt[c * S + s];
+    load_pose(v, tgt);
+  }
+  float lam = it0 == 0 ? lp.lm_lambda : Pg.lam[s];
+  int nacc = it0 == 0 ? 0 : Pg.nacc[s];
+  float cost = 0.f;
+
+  for (int it = 0; it < iters; ++it) {
+    const int gi = it0 + it;
+    if (valid && Pg.q_trace && gi < Pg.trace_iters) {
+#pragma unroll
+      for (int d = 0; d < NA; ++d) Pg.q_trace[(gi * NA + d) * S + s] = q[d];
+      Pg.lam_trace[gi * S + s] = lam;
+    }
+    // ---- FK with motion subspace, residual rows, Jacobian blocks at q
+    float S6[NA][6];
+    const Pose Tf = chain_fk<NA, NJ, true>(rb, nj, q, jt, S6);
+    float r_pose[6], r_lim[NA], r_sm[NA], jl[NA], js[NA], tg[36];
+    chain_rows<NA, true>(rb, tm, Tf, tgt, q, qprev, r_pose, r_lim, r_sm, tg, jl, js);
+    // ---- pose-row Jacobian columns (zero for joints after the target link's joint)
+    float J[6][NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      float c6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (n_pose && a <= jt) pose_jac_column(tg, S6[a], c6);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) J[i][a] = c6[i];
+    }
+    // ---- normal equations (per-term partial sums, rows in order)
+    float H[NA][NA], g[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        float h = 0.f;
+        if (n_pose && i <= jt) {
+          float sp = 0.f;
+#pragma unroll
+          for (int r = 0; r < 6; ++r) sp += J[r][i] * J[r][j];
+          h = sp;
+        }
+        if (i == j) {
+          if (tm->limit) h = h + jl[i] * jl[i];
+          if (tm->smooth) h = h + js[i] * js[i];
+        }
+        H[i][j] = h;
+      }
+      float gv = 0.f;
+      if (n_pose && i <= jt) {
+        float sp = 0.f;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) sp += J[r][i] * r_pose[r];
+        gv = sp;
+      }
+      if (tm->limit) gv = gv + jl[i] * r_lim[i];
+      if (tm->smooth) gv = gv + js[i] * r_sm[i];
+      g[i] = gv;
+    }
+    cost = chain_cost<NA>(tm, r_pose, r_lim, r_sm);
+    // ---- damped Cholesky (left-looking; the per-entry sequence of the LDS kernel) and solves
+#pragma unroll
+    for (int j = 0; j < NA; ++j) {
+      float sd = H[j][j] + lam;
+#pragma unroll
+      for (int k = 0; k < j; ++k) sd -= H[j][k] * H[j][k];
+      const float d = sqrtf(sd);
+      H[j][j] = d;
+#pragma unroll
+      for (int i = j + 1; i < NA; ++i) {
+        float v = H[i][j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) v -= H[i][k] * H[j][k];
+        H[i][j] = v / d;
+      }
+    }
+    float dq[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      float v = -g[i];
+#pragma unroll
+      for (int k = 0; k < i; ++k) v -= H[i][k] * dq[k];
+      dq[i] = v / H[i][i];
+    }
+#pragma unroll
+    for (int i = NA - 1; i >= 0; --i) {
+      float v = dq[i];
+#pragma unroll
+      for (int k = i + 1; k < NA; ++k) v -= H[k][i] * dq[k];
+      dq[i] = v / H[i][i];
+    }
+    float pr = 0.f;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) pr += dq[i] * (lam * dq[i] - g[i]);
+    const float pred = 0.5f * pr;
+    // ---- proposal: FK to the target link only, residual rows, cost
+    float qp[NA];
+#pragma unroll
+    for (int d = 0; d < NA; ++d) qp[d] = q[d] + dq[d];
+    const Pose Tfp = chain_fk<NA, NJ, false>(rb, nj, qp, jt, S6);
+    chain_rows<NA, false>(rb, tm, Tfp, tgt, qp, qprev, r_pose, r_lim, r_sm, tg, jl, js);
+    const float cp = chain_cost<NA>(tm, r_pose, r_lim, r_sm);
+    if (valid && Pg.trace && gi == 0) Pg.trace[s] = cost;
+    const bool acc = lm_accept(lp, cost, cp, pred, lam);
+    if (acc) {
+      ++nacc;
+#pragma unroll
+      for (int d = 0; d < NA; ++d) q[d] = qp[d];
+    }
+    if (valid && Pg.trace && gi < Pg.trace_iters) {
+      Pg.trace[(gi + 1) * S + s] = cost;
+      Pg.acc_trace[gi * S + s] = acc;
+    }
+  }
+  if (valid) {
+#pragma unroll
+    for (int d = 0; d < NA; ++d) Pg.q[d * S + s] = q[d];
+    if (iters > 0) Pg.cost[s] = cost;
+    Pg.lam[s] = lam;
+    Pg.nacc[s] = nacc;
+  }
+}
+
+template <int NA, int NJ>
+__global__ __launch_bounds__(64) void k_thread(Problems P, LMParams lp, int it0, int iters) {
+  thread_solve<NA, NJ>(P, blockIdx.x * 64 + threadIdx.x, lp, it0, iters);
+}

@@ -1,0 +1,426 @@
+// rk_wave.h -- one wavefront per problem, register-resident (included by rk_lm.hip inside namespace rk).
+//
+// For trees up to 64 joints / 32 dofs (e.g. a 29-dof humanoid).  Lane roles inside the wave:
+//   lane j  (< nj): joint j -- its local transform, world pose, motion-subspace column;
+//   lane a  (< na): dof a   -- q_a, its Jacobian column (all pose / position rows), row a of
+//                              J^T J + lam I and of its Cholesky factor;
+//   lane t        : pose target t (then position target t - n_pose): residual rows and the
+//                   per-target Jacobian blocks, broadcast to the wave through LDS.
+// World poses come from pointer jumping over the parent tree (ceil(log2(depth+1)) rounds of
+// __shfl of the ancestor's partial product); the factorisation is the right-looking Cholesky on
+// register rows, entries of other rows read with v_readlane (no LDS round trip); the backward
+// solve reads the factor's columns after one transpose through LDS.
+// Row blocks of a target are structurally zero in the columns of dofs that do not move the
+// target's link; those (target, column) blocks are skipped in J^T J with a wave-uniform mask
+// (exact zeros only, so the per-entry sequence equals the dense one).
+
+constexpr int WV_NA = 32;  // padded dof count: rows na..31 of the system are the identity
+
+__device__ __forceinline__ float wv_readlane(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ Pose wv_shfl_pose(const Pose& P, int src) {
+  Pose o;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) o.t[k] = __shfl(P.t[k], src);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) o.q[k] = __shfl(P.q[k], src);
+  return o;
+}
+
+template <int NP, int NQ>
+struct WaveLDS {
+  static constexpr int RP = 6 * NP + 3 * NQ > 0 ? 6 * NP + 3 * NQ : 1;
+  float tg[NP > 0 ? NP : 1][36];        // pose-target Jacobian blocks
+  float rows[RP + 2 * WV_NA];           // residual rows, term order
+  float plink[NQ > 0 ? NQ : 1][4];      // world position of each position-target link
+  float J[WV_NA][RP + 1];               // Jacobian columns (dof-major), padded
+  float L[WV_NA][WV_NA + 1];            // Cholesky factor rows for the transpose
+};
+
+// per-lane joint data, loaded once per problem
+struct WaveJoint {
+  Pose P;  // parent joint transform
+  float ax[3], mul, off;
+  int type, parent, act, mimic;
+};
+
+// pointer-jumping FK: lane j ends with T_world_joint(j) (identity base)
+__device__ __forceinline__ Pose wave_fk(const WaveJoint& jd, int nj, int nrounds, float q_mine) {
+  const int lane = threadIdx.x & 63;
+  // q of the dof that drives this joint
+  const int src = jd.mimic >= 0 ? jd.mimic : (jd.act >= 0 ? jd.act : 0);
+  const float qs = __shfl(q_mine, src);
+  float pos = jd.mimic >= 0 ? qs * jd.mul + jd.off : (jd.act >= 0 ? qs : 0.f);
+  Pose X = identity_pose();
+  if (jd.type == 2) {
+    float s, c;
+    sincosf(pos * 0.5f, &s, &c);
+    X.q[0] = c; X.q[1] = jd.ax[0] * s; X.q[2] = jd.ax[1] * s; X.q[3] = jd.ax[2] * s;
+  } else if (jd.type == 1) {
+    X.t[0] = jd.ax[0] * pos; X.t[1] = jd.ax[1] * pos; X.t[2] = jd.ax[2] * pos;
+  }
+  Pose A = compose(jd.P, X);
+  int ptr = lane < nj ? jd.parent : -1;
+  for (int r = 0; r < nrounds; ++r) {
+    const int sp = ptr < 0 ? 0 : ptr;
+    const Pose Ap = wv_shfl_pose(A, sp);
+    const int pp = __shfl(ptr, sp);
+    if (ptr >= 0) {
+      A = compose(Ap, A);
+      ptr = pp;
+    }
+  }
+  return A;
+}
+
+template <int NP, int NQ>
+__device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const LMParams& lp, int it0, int iters,
+                                           char* lds_raw) {
+  constexpr int RP = 6 * NP + 3 * NQ;
+  WaveLDS<NP, NQ>* sh = reinterpret_cast<WaveLDS<NP, NQ>*>(lds_raw);
+  const int lane = threadIdx.x & 63;
+  RobotK* rb = as_const(Pg.robot);
+  TermsK* tm = as_const(Pg.terms);
+  const int S = Pg.stride;
+  const bool valid = s_idx < Pg.n;
+  const int s = valid ? s_idx : Pg.n - 1;
+  const int nj = rb->nj, na = rb->na, nrounds = rb->nrounds;
+  const int n_pose = tm->n_pose, n_pos = tm->n_pos;
+  const bool has_lim = tm->limit != 0, has_sm = tm->smooth != 0;
+  const int Rf = 6 * n_pose, Rq = 3 * n_pos;
+  const int off_l = Rf + Rq, off_s = off_l + (has_lim ? na : 0), R = off_s + (has_sm ? na : 0);
+
+  // ---- per-lane joint data
+  WaveJoint jd;
+  {
+    const int j = lane < nj ? lane : 0;
+    const Robot* rg = Pg.robot;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) jd.P.t[k] = rg->pT[j][k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) jd.P.q[k] = rg->pT[j][3 + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) jd.ax[k] = rg->axis[j][k];
+    jd.mul = rg->mul[j];
+    jd.off = rg->off[j];
+    jd.type = lane < nj ? rg->joint_type[j] : 0;
+    jd.parent = lane < nj ? rg->parent[j] : -1;
+    jd.act = rg->act[j];
+    jd.mimic = rg->mimic[j];
+  }
+  // ---- per-lane dof data: joints driving dof a (CSR, <= 2 entries), limits, weights
+  int cj0 = -1, cj1 = -1;
+  float cw0 = 0.f, cw1 = 0.f, lo = 0.f, hi = 0.f, wl = 0.f, ws = 0.f;
+  if (lane < na) {
+    const Robot* rg = Pg.robot;
+    const int c0 = rg->col_start[lane], c1 = rg->col_start[lane + 1];
+    if (c1 > c0) { cj0 = rg->col_joint[c0]; cw0 = rg->col_w[c0]; }
+    if (c1 > c0 + 1) { cj1 = rg->col_joint[c0 + 1]; cw1 = rg->col_w[c0 + 1]; }
+    lo = rg->lo[lane];
+    hi = rg->hi[lane];
+    wl = Pg.terms->limit_w[lane];
+    ws = Pg.terms->smooth_w[lane];
+  }
+  // which of this dof's joints move each target's link; wave-uniform column masks per target
+  unsigned in0 = 0, in1 = 0;  // bit t (pose targets), bit NP + u (position targets)
+  unsigned long long cmask[NP + NQ > 0 ? NP + NQ : 1];
+#pragma unroll
+  for (int t = 0; t < NP + NQ; ++t) {
+    const bool pose = t < NP;
+    const int tt = pose ? t : t - NP;
+    const bool live = pose ? tt < n_pose : tt < n_pos;
+    bool b0 = false, b1 = false;
+    if (live) {
+      const int link = pose ? tm->pose_link[tt] : tm->pos_link[tt];
+      const unsigned long long anc = rb->anc[link];
+      b0 = cj0 >= 0 && ((anc >> cj0) & 1ull);
+      b1 = cj1 >= 0 && ((anc >> cj1) & 1ull);
+    }
+    in0 |= (unsigned)b0 << t;
+    in1 |= (unsigned)b1 << t;
+    cmask[t] = __ballot(b0 || b1);
+  }
+
+  // ---- problem data
+  float q = 0.f, qprev = 0.f;
+  if (lane < na) {
+    q = (it0 == 0 && Pg.q_seed) ? Pg.q_seed[lane * S + s] : Pg.q[lane * S + s];
+    if (has_sm) qprev = Pg.q_prev[lane * S + s];
+  }
+  // pose / position target owned by this lane
+  Pose tgt = identity_pose();
+  float ptgt[3] = {0.f, 0.f, 0.f};
+  int tj = -1;  // world joint whose pose the lane's target reads (-1: base)
+  if (lane < n_pose) {
+    float v[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) v[c] = Pg.pose_tgt[(lane * 7 + c) * S + s];
+    load_pose(v, tgt);
+    tj = Pg.robot->link_joint[Pg.terms->pose_link[lane]];
+  } else if (lane < n_pose + n_pos) {
+    const int u = lane - n_pose;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ptgt[c] = Pg.pos_tgt[(u * 3 + c) * S + s];
+    tj = Pg.robot->link_joint[Pg.terms->pos_link[u]];
+  }
+  float pw[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (lane < n_pose) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) pw[i] = Pg.terms->pose_w[lane][i];
+  } else if (lane < n_pose + n_pos) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pw[i] = Pg.terms->pos_w[lane - n_pose][i];
+  }
+
+  float lam = it0 == 0 ? lp.lm_lambda : Pg.lam[s];
+  int nacc = it0 == 0 ? 0 : Pg.nacc[s];
+  float cost = 0.f;
+  Pose T = wave_fk(jd, nj, nrounds, q);
+
+  // residual rows of the lane's target at world poses Tw (all lanes hold their joint's pose);
+  // JAC also writes the pose target's Jacobian blocks / the position link's point to LDS
+  auto target_rows = [&](const Pose& Tw, bool jac) {
+    const Pose Tl = wv_shfl_pose(Tw, tj < 0 ? 0 : tj);
+    const Pose Tf = tj < 0 ? identity_pose() : Tl;
+    if (lane < n_pose) {
+      float r6[6];
+      pose_target(Tf, tgt, pw, r6);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) sh->rows[6 * lane + i] = r6[i];
+      if (jac) pose_blocks(Tf, tgt, pw, sh->tg[lane < NP ? lane : 0]);
+    } else if (lane < n_pose + n_pos) {
+      const int u = lane - n_pose;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) sh->rows[Rf + 3 * u + i] = pw[i] * (ptgt[i] - Tf.t[i]);
+      if (jac) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) sh->plink[u < NQ ? u : 0][i] = Tf.t[i];
+      }
+    }
+  };
+  // cost = sum over terms of 0.5 * sum of squares (rows in order), the same value on all lanes
+  auto term_cost_lds = [&]() {
+    float c = 0.f;
+    const int sizes[4] = {Rf, Rq, has_lim ? na : 0, has_sm ? na : 0};
+    int off = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (sizes[t] == 0) continue;
+      float sum = 0.f;
+      for (int k = 0; k < sizes[t]; ++k) sum += sh->rows[off + k] * sh->rows[off + k];
+      c = c + 0.5f * sum;
+      off += sizes[t];
+    }
+    return c;
+  };
+
+  for (int it = 0; it < iters; ++it) {
+    const int gi = it0 + it;
+    if (valid && Pg.q_trace && gi < Pg.trace_iters) {
+      if (lane < na) Pg.q_trace[(gi * na + lane) * S + s] = q;
+      if (lane == 0) Pg.lam_trace[gi * S + s] = lam;
+    }
+    // ---- motion subspace (lane j), the dof's joint columns (lane a)
+    float S6[6];
+    {
+      const int j = lane < nj ? lane : 0;
+      subspace_column(rb, j, T, S6);
+      if (lane >= nj) S6[0] = S6[1] = S6[2] = S6[3] = S6[4] = S6[5] = 0.f;
+    }
+    float Sc0[6], Sc1[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      Sc0[r] = __shfl(S6[r], cj0 < 0 ? 0 : cj0);
+      Sc1[r] = __shfl(S6[r], cj1 < 0 ? 0 : cj1);
+    }
+    // ---- residual rows and target blocks at q
+    target_rows(T, true);
+    float rl = 0.f, jl = 0.f, rs = 0.f, js = 0.f;
+    if (lane < na) {
+      if (has_lim) {
+        rl = limit_row(q, lo, hi, wl);
+        jl = limit_diag(q, lo, hi, wl);
+        sh->rows[off_l + lane] = rl;
+      }
+      if (has_sm) {
+        rs = ws * (q - qprev);
+        js = ws;
+        sh->rows[off_s + lane] = rs;
+      }
+    }
+    __syncthreads();
+    cost = term_cost_lds();
+    // ---- this dof's Jacobian column (pose rows, then position rows), also to LDS
+    float Jc[RP > 0 ? RP : 1];
+#pragma unroll
+    for (int t = 0; t < NP + NQ; ++t) {
+      const bool pose = t < NP;
+      const int tt = pose ? t : t - NP;
+      float sc[6];
+      const bool m0 = (in0 >> t) & 1u, m1 = (in1 >> t) & 1u;
+#pragma unroll
+      for (int r = 0; r < 6; ++r) sc[r] = (m0 ? Sc0[r] * cw0 : 0.f) + (m1 ? Sc1[r] * cw1 : 0.f);
+      if (pose) {
+        float c6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (tt < n_pose && (m0 || m1)) pose_jac_column(sh->tg[tt], sc, c6);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) Jc[6 * tt + i] = c6[i];
+      } else {
+        float c3[3] = {0.f, 0.f, 0.f};
+        if (tt < n_pos && (m0 || m1)) {
+          float w3[3] = {tm->pos_w[tt][0], tm->pos_w[tt][1], tm->pos_w[tt][2]};
+          pos_jac_column(sc, sh->plink[tt], w3, c3);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Jc[6 * NP + 3 * tt + i] = c3[i];
+      }
+    }
+    if (lane < WV_NA) {
+#pragma unroll
+      for (int r = 0; r < RP; ++r) sh->J[lane][r] = Jc[r];
+    }
+    __syncthreads();
+    // ---- row `lane` of J^T J + lam I and of J^T r (pose term, then position term, then diagonals)
+    float A[WV_NA];
+#pragma unroll
+    for (int j = 0; j < WV_NA; ++j) A[j] = 0.f;
+    float gv = 0.f;
+    if (NP > 0 && n_pose) {
+#pragma unroll
+      for (int t = 0; t < NP; ++t) {
+        if (t < n_pose) {
+          const unsigned long long m = cmask[t];
+#pragma unroll
+          for (int j = 0; j < WV_NA; ++j) {
+            if ((m >> j) & 1ull) {
+#pragma unroll
+              for (int r = 0; r < 6; ++r) A[j] += Jc[6 * t + r] * sh->J[j][6 * t + r];
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < 6; ++r) gv += Jc[6 * t + r] * sh->rows[6 * t + r];
+        }
+      }
+    }
+    if (NQ > 0 && n_pos) {
+      float A2[WV_NA];
+#pragma unroll
+      for (int j = 0; j < WV_NA; ++j) A2[j] = 0.f;
+      float g2 = 0.f;
+#pragma unroll
+      for (int u = 0; u < NQ; ++u) {
+        if (u < n_pos) {
+          const unsigned long long m = cmask[NP + u];
+#pragma unroll
+          for (int j = 0; j < WV_NA; ++j) {
+            if ((m >> j) & 1ull) {
+#pragma unroll
+              for (int r = 0; r < 3; ++r) A2[j] += Jc[6 * NP + 3 * u + r] * sh->J[j][6 * NP + 3 * u + r];
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < 3; ++r) g2 += Jc[6 * NP + 3 * u + r] * sh->rows[Rf + 3 * u + r];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < WV_NA; ++j) A[j] = A[j] + A2[j];
+      gv = gv + g2;
+    }
+#pragma unroll
+    for (int j = 0; j < WV_NA; ++j) {
+      if (j == lane) {
+        if (has_lim) A[j] = A[j] + jl * jl;
+        if (has_sm) A[j] = A[j] + js * js;
+      }
+    }
+    if (has_lim) gv = gv + jl * rl;
+    if (has_sm) gv = gv + js * rs;
+    // padding rows / columns: identity system
+#pragma unroll
+    for (int j = 0; j < WV_NA; ++j) {
+      if (lane >= na || j >= na) A[j] = (j == lane) ? 1.f : 0.f;
+      if (j == lane) A[j] = A[j] + lam;
+    }
+    if (lane >= na) gv = 0.f;
+    // ---- right-looking Cholesky on register rows (lane i holds row i; upper entries unused)
+    // (structurally zero L[j][k] -- fill_col from the symbolic factorisation -- are skipped: they
+    // would subtract exact zeros)
+#pragma unroll
+    for (int k = 0; k < WV_NA; ++k) {
+      const float d = sqrtf(wv_readlane(A[k], k));
+      A[k] = lane == k ? d : A[k] / d;
+      const unsigned fc = k < na ? tm->fill_col[k] : 0u;
+#pragma unroll
+      for (int j = k + 1; j < WV_NA; ++j)
+        if ((fc >> j) & 1u) A[j] -= A[k] * wv_readlane(A[k], j);
+    }
+    // forward substitution: y_k = (-g_k - sum_{i<k} L_ki y_i) / L_kk, column by column
+    float b = -gv, y = 0.f;
+#pragma unroll
+    for (int k = 0; k < WV_NA; ++k) {
+      const float yk = wv_readlane(b, k) / wv_readlane(A[k], k);
+      if (lane == k) y = yk;
+      if (k < na && tm->fill_col[k]) b -= A[k] * yk;
+    }
+    // backward substitution with the factor's columns (transposed through LDS)
+    if (lane < WV_NA) {
+#pragma unroll
+      for (int j = 0; j < WV_NA; ++j) sh->L[lane][j] = A[j];
+    }
+    __syncthreads();
+    float c = y, dq = 0.f;
+#pragma unroll
+    for (int k = WV_NA - 1; k >= 0; --k) {
+      const float xk = wv_readlane(c, k) / sh->L[k][k];
+      if (lane == k) dq = xk;
+      if (k < na && tm->fill_row[k]) c -= (lane < WV_NA ? sh->L[k][lane < WV_NA ? lane : 0] : 0.f) * xk;
+    }
+    float pr = 0.f;
+    for (int i = 0; i < na; ++i) {
+      const float di = wv_readlane(dq, i), gi_ = wv_readlane(gv, i);
+      pr += di * (lam * di - gi_);
+    }
+    const float pred = 0.5f * pr;
+    // ---- proposal
+    const float qp = q + dq;
+    const Pose Tp = wave_fk(jd, nj, nrounds, qp);
+    __syncthreads();  // the rows of the iterate have been consumed
+    target_rows(Tp, false);
+    if (lane < na) {
+      if (has_lim) sh->rows[off_l + lane] = limit_row(qp, lo, hi, wl);
+      if (has_sm) sh->rows[off_s + lane] = ws * (qp - qprev);
+    }
+    __syncthreads();
+    const float cp = term_cost_lds();
+    if (valid && Pg.trace && gi == 0 && lane == 0) Pg.trace[s] = cost;
+    const bool acc = lm_accept(lp, cost, cp, pred, lam);
+    if (acc) {
+      ++nacc;
+      q = qp;
+      T = Tp;
+    }
+    if (valid && Pg.trace && gi < Pg.trace_iters && lane == 0) {
+      Pg.trace[(gi + 1) * S + s] = cost;
+      Pg.acc_trace[gi * S + s] = acc;
+    }
+    __syncthreads();
+  }
+  (void)R;
+  if (valid) {
+    if (lane < na) Pg.q[lane * S + s] = q;
+    if (lane == 0) {
+      if (iters > 0) Pg.cost[s] = cost;
+      Pg.lam[s] = lam;
+      Pg.nacc[s] = nacc;
+    }
+  }
+  __syncthreads();
+}
+
+template <int NP, int NQ>
+__global__ __launch_bounds__(64) void k_wave(Problems P, LMParams lp, int it0, int iters) {
+  __shared__ __align__(16) char lds[sizeof(WaveLDS<NP, NQ>)];
+  wave_solve<NP, NQ>(P, blockIdx.x, lp, it0, iters, lds);
+}

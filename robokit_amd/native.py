@@ -5,7 +5,9 @@ raise.  The library is built in-tree by ``robokit_amd.build``.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 from pathlib import Path
 from typing import Dict, List, Optional, Sequence
@@ -54,11 +56,26 @@ class LMParamsC(C.Structure):
 
 _lib: Optional[C.CDLL] = None
 
+# Live handles, closed at interpreter exit (groups, then batches, then contexts) while the HIP
+# runtime is still up: objects freed during interpreter teardown would call into it afterwards.
+_LIVE = {"group": weakref.WeakSet(), "batch": weakref.WeakSet(), "context": weakref.WeakSet()}
+
+
+@atexit.register
+def _close_all() -> None:
+    for kind in ("group", "batch", "server", "context"):
+        for obj in list(_LIVE.get(kind, ())):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
 EXPORTS = [
     "rk_last_error", "rk_abi_version", "rk_context_create", "rk_context_destroy", "rk_context_sync",
-    "rk_context_mem", "rk_robot_upload", "rk_fk", "rk_link_jacobian", "rk_se3_op", "rk_batch_create",
+    "rk_context_mem", "rk_robot_upload", "rk_fk", "rk_link_jacobian", "rk_copy_probe", "rk_se3_op",
+    "rk_batch_create",
     "rk_batch_destroy", "rk_batch_set", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics", "rk_group_create",
-    "rk_group_destroy", "rk_group_set_tuning", "rk_group_autotune", "rk_group_solve",
+    "rk_group_destroy", "rk_group_set_tuning", "rk_group_autotune", "rk_group_solve", "rk_group_kernel_times",
 ]
 
 
@@ -118,6 +135,7 @@ class Context:
     def __init__(self, device: int = 0):
         self._h = C.c_void_p()
         _check(lib().rk_context_create(int(device), C.byref(self._h)))
+        _LIVE["context"].add(self)
         self.device = device
         self._robots: Dict[int, int] = {}
         self._keep: List = []
@@ -176,6 +194,12 @@ class Context:
                                       0 if frame == "spatial" else 1, _p(out)))
         return out
 
+    def copy_probe(self, nbytes: int) -> float:
+        """Streaming copy of nbytes (counter calibration); returns the timed launch's ms."""
+        ms = C.c_float()
+        _check(lib().rk_copy_probe(self._h, C.c_int64(int(nbytes)), C.byref(ms)))
+        return float(ms.value)
+
     def se3(self, op: str, a: np.ndarray, b: Optional[np.ndarray] = None) -> np.ndarray:
         code, w = SE3_OPS[op]
         a = _f32(np.atleast_2d(a))
@@ -222,6 +246,7 @@ class Batch:
                       _p(a[5]))
         self._h = C.c_void_p()
         _check(lib().rk_batch_create(ctx.handle, C.byref(d), self.capacity, self.trace_iters, C.byref(self._h)))
+        _LIVE["batch"].add(self)
         self.n = 0
 
     def close(self) -> None:
@@ -293,6 +318,7 @@ class Group:
         arr = (C.c_void_p * len(self.batches))(*[b.handle.value for b in self.batches])
         self._h = C.c_void_p()
         _check(lib().rk_group_create(ctx.handle, len(self.batches), arr, C.byref(self._h)))
+        _LIVE["group"].add(self)
 
     def close(self) -> None:
         if self._h:
@@ -319,3 +345,14 @@ class Group:
         flags = (USE_GRAPH if use_graph else 0) | (FROM_SEED if from_seed else 0)
         _check(lib().rk_group_solve(self._h, int(iters), C.byref(params.c()), flags, C.byref(ms)))
         return float(ms.value)
+
+    def kernel_times(self) -> List[tuple]:
+        """[(threads-per-problem code, ms, batch indices)] of each persistent kernel of the last solve."""
+        cap = 16
+        tpp = np.zeros(cap, np.int32)
+        ms = np.zeros(cap, np.float32)
+        mask = np.zeros(cap, np.int32)
+        n = C.c_int32()
+        _check(lib().rk_group_kernel_times(self._h, cap, _p(tpp), _p(ms), _p(mask), C.byref(n)))
+        return [(int(tpp[k]), float(ms[k]), [i for i in range(32) if (int(mask[k]) >> i) & 1])
+                for k in range(min(n.value, cap))]
