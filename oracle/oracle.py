@@ -89,7 +89,7 @@ class OracleRobot:
 class Term:
     kind: int
     links: Sequence[int] = ()
-    targets: Optional[np.ndarray] = None  # [B, k, 7]
+    targets: Optional[np.ndarray] = None  # FRAME [B, k, 7] poses; POSITION [B, k, 3] points (or poses)
     weights: Optional[np.ndarray] = None
     q_prev: Optional[np.ndarray] = None  # [B, na]
 
@@ -99,6 +99,12 @@ def _terms(terms: Sequence[Term], keep: list):
     for i, t in enumerate(terms):
         links = np.ascontiguousarray(t.links, np.int32)
         tg = None if t.targets is None else np.ascontiguousarray(t.targets, np.float32)
+        if tg is not None and tg.shape[-1] == 3:  # points: the C side reads 7-float pose records
+            pad = np.zeros(tg.shape[:-1] + (4,), np.float32)
+            pad[..., 0] = 1.0
+            tg = np.ascontiguousarray(np.concatenate([tg, pad], -1))
+        if tg is not None and tg.shape[-1] != 7:
+            raise ValueError(f"targets must end in 3 or 7 floats, got {tg.shape}")
         w = np.ascontiguousarray(t.weights, np.float32)
         qp = None if t.q_prev is None else np.ascontiguousarray(t.q_prev, np.float32)
         keep.extend([links, tg, w, qp])
