@@ -44,10 +44,6 @@ struct Terms {
   float pos_w[RK_MAXQ][3];
   float limit_w[RK_NA_MAX];
   float smooth_w[RK_NA_MAX];
-  // structure of the Cholesky factor of J^T J + lam I (symbolic factorisation at batch creation):
-  // fill_col[k] bit j (j > k): L[j][k] can be nonzero; fill_row[k] bit i (i < k): L[k][i] can be
-  unsigned fill_col[RK_NA_MAX];
-  unsigned fill_row[RK_NA_MAX];
 };
 
 struct LMParams {
@@ -547,12 +543,13 @@ __host__ __device__ constexpr int variant_G(int v) {
 __host__ __device__ constexpr bool variant_small(int v) { return v <= V_S16; }
 // problems per workgroup (one wavefront)
 __host__ __device__ constexpr int variant_ppw(int v) { return variant_thread(v) ? WAVE : WAVE / variant_G(v); }
-inline size_t variant_lds(int v) {
-  if (variant_thread(v)) return 0;
-  if (v == V_W40) return sizeof(WaveLDS<4, 0>);
-  if (v == V_W012) return sizeof(WaveLDS<0, 12>);
-  return variant_small(v) ? (WAVE / variant_G(v)) * sizeof(Shared<CfgS>)
-                          : (WAVE / variant_G(v)) * sizeof(Shared<CfgL>);
+// LDS bytes per workgroup of a variant
+__host__ __device__ constexpr size_t variant_lds(int v) {
+  return variant_thread(v) ? 0
+         : v == V_W40      ? sizeof(WaveLDS<4, 0>)
+         : v == V_W012     ? sizeof(WaveLDS<0, 12>)
+         : variant_small(v) ? (WAVE / variant_G(v)) * sizeof(Shared<CfgS>)
+                            : (WAVE / variant_G(v)) * sizeof(Shared<CfgL>);
 }
 
 struct Bucket {
@@ -589,7 +586,9 @@ __device__ __forceinline__ void run_tile(const Bucket* bk, int local, const LMPa
 template <int V>
 __global__ __launch_bounds__(64) void k_persistent(const Bucket* __restrict__ buckets, int nb, int total_tiles,
                                                    int* counter, LMParams lp, int it0, int iters) {
-  extern __shared__ __align__(16) char lds[];
+  // static LDS of the variant's size: its offsets fold into the LDS instructions (a dynamic
+  // base made every address a live scalar)
+  __shared__ __align__(16) char lds[variant_lds(V) > 0 ? variant_lds(V) : 16];
   __shared__ int tile_s;
   for (;;) {
     if (threadIdx.x == 0) tile_s = atomicAdd(counter, 1);
@@ -1608,38 +1607,6 @@ int rk_batch_create(rk_context* ctx, const rk_terms_desc* t, int32_t capacity, i
     th.limit_w[d] = th.limit ? t->limit_weights[d] : 0.f;
     th.smooth_w[d] = th.smooth ? t->smooth_weights[d] : 0.f;
   }
-  {
-    // pattern of J^T J: dofs i, j couple when both move the link of one target row block;
-    // then the symbolic Cholesky (fill-in) in dof order
-    const int na = r.na;
-    auto moves = [&](int a, int link) {
-      for (int c = r.col_start[a]; c < r.col_start[a + 1]; ++c)
-        if ((r.anc[link] >> r.col_joint[c]) & 1ull) return true;
-      return false;
-    };
-    std::vector<std::vector<char>> P(na, std::vector<char>(na, 0));
-    for (int i = 0; i < na; ++i) P[i][i] = 1;
-    std::vector<int> links;
-    for (int t = 0; t < th.n_pose; ++t) links.push_back(th.pose_link[t]);
-    for (int u = 0; u < th.n_pos; ++u) links.push_back(th.pos_link[u]);
-    for (int link : links)
-      for (int i = 0; i < na; ++i)
-        if (moves(i, link))
-          for (int j = 0; j < na; ++j)
-            if (moves(j, link)) P[i][j] = 1;
-    for (int k = 0; k < na; ++k)
-      for (int i = k + 1; i < na; ++i)
-        if (P[i][k])
-          for (int j = k + 1; j <= i; ++j)
-            if (P[j][k]) P[i][j] = P[j][i] = 1;
-    for (int k = 0; k < na; ++k) {
-      th.fill_col[k] = th.fill_row[k] = 0u;
-      for (int j = k + 1; j < na; ++j)
-        if (P[j][k]) th.fill_col[k] |= 1u << j;
-      for (int i = 0; i < k; ++i)
-        if (P[k][i]) th.fill_row[k] |= 1u << i;
-    }
-  }
   th.Rf = 6 * th.n_pose;
   th.Rq = 3 * th.n_pos;
   th.R = th.Rf + th.Rq + (th.limit ? r.na : 0) + (th.smooth ? r.na : 0);
@@ -1915,11 +1882,6 @@ struct GroupLaunch {
 
 template <int V>
 static int variant_grid(rk_context* ctx, int wg_per_cu, size_t lds, int tiles, int* grid) {
-  static size_t attr_lds = 0;
-  if (lds > attr_lds) {
-    HIP_TRY(hipFuncSetAttribute((const void*)k_persistent<V>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_lds = lds;
-  }
   int per_cu = wg_per_cu;
   if (per_cu <= 0) {
     int occ = 0;
@@ -2020,7 +1982,7 @@ static int group_prepare(rk_group* g, GroupLaunch* gl, bool from_seed) {
   }
   if (!g->ev_fork) HIP_TRY(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
   for (size_t k = 0; k < gl->var.size(); ++k) {
-    const size_t lds = variant_lds(gl->var[k]);
+    const size_t lds = 0;  // the persistent kernels' LDS is static (variant_lds bytes)
     int grid;
     int rc = variant_grid_any(ctx, gl->var[k], g->wg_per_cu, lds, gl->tiles[k], &grid);
     if (rc) return rc;

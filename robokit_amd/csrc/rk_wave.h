@@ -20,6 +20,14 @@ __device__ __forceinline__ float wv_readlane(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// sum over the 64 lanes by an xor butterfly: every lane ends with the bitwise-same total (each
+// pairwise add sees the same two operands, in commuted order)
+__device__ __forceinline__ float wv_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 __device__ __forceinline__ Pose wv_shfl_pose(const Pose& P, int src) {
   Pose o;
 #pragma unroll
@@ -28,6 +36,14 @@ __device__ __forceinline__ Pose wv_shfl_pose(const Pose& P, int src) {
   for (int k = 0; k < 4; ++k) o.q[k] = __shfl(P.q[k], src);
   return o;
 }
+
+// an empty asm the optimiser cannot see through: keeps a wave-uniform mask in one SGPR pair and
+// its bit tests inside the iteration loop (hoisted out, every (target, column) test became a live
+// 64-bit condition and the kernel spilled hundreds of SGPRs into VGPR lanes)
+__device__ __forceinline__ void wv_opaque(unsigned long long& m) { asm volatile("" : "+s"(m)); }
+__device__ __forceinline__ void wv_opaque(unsigned& m) { asm volatile("" : "+s"(m)); }
+template <class T>
+__device__ __forceinline__ void wv_opaque_v(T& v) { asm volatile("" : "+v"(v)); }
 
 template <int NP, int NQ>
 struct WaveLDS {
@@ -80,7 +96,7 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const 
                                            char* lds_raw) {
   constexpr int RP = 6 * NP + 3 * NQ;
   WaveLDS<NP, NQ>* sh = reinterpret_cast<WaveLDS<NP, NQ>*>(lds_raw);
-  const int lane = threadIdx.x & 63;
+  int lane = threadIdx.x & 63;
   RobotK* rb = as_const(Pg.robot);
   TermsK* tm = as_const(Pg.terms);
   const int S = Pg.stride;
@@ -200,23 +216,35 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const 
       }
     }
   };
-  // cost = sum over terms of 0.5 * sum of squares (rows in order), the same value on all lanes
+  // cost = sum over terms of 0.5 * (sum of the term's squared rows): the rows are read back from
+  // LDS by the lanes that own them (target t: lane t, dof a: lane a) and summed across the wave
   auto term_cost_lds = [&]() {
-    float c = 0.f;
-    const int sizes[4] = {Rf, Rq, has_lim ? na : 0, has_sm ? na : 0};
-    int off = 0;
+    float sp = 0.f, sq = 0.f, sl = 0.f, ss = 0.f;
+    if (lane < n_pose) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      if (sizes[t] == 0) continue;
-      float sum = 0.f;
-      for (int k = 0; k < sizes[t]; ++k) sum += sh->rows[off + k] * sh->rows[off + k];
-      c = c + 0.5f * sum;
-      off += sizes[t];
+      for (int i = 0; i < 6; ++i) sp += sh->rows[6 * lane + i] * sh->rows[6 * lane + i];
+    } else if (lane < n_pose + n_pos) {
+      const int u = lane - n_pose;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) sq += sh->rows[Rf + 3 * u + i] * sh->rows[Rf + 3 * u + i];
     }
+    if (lane < na) {
+      if (has_lim) sl = sh->rows[off_l + lane] * sh->rows[off_l + lane];
+      if (has_sm) ss = sh->rows[off_s + lane] * sh->rows[off_s + lane];
+    }
+    float c = 0.f;
+    if (n_pose) c = c + 0.5f * wv_sum(sp);
+    if (n_pos) c = c + 0.5f * wv_sum(sq);
+    if (has_lim) c = c + 0.5f * wv_sum(sl);
+    if (has_sm) c = c + 0.5f * wv_sum(ss);
     return c;
   };
 
   for (int it = 0; it < iters; ++it) {
+    // lane-derived predicates recomputed per iteration (hoisted, each is a live SGPR pair)
+    wv_opaque_v(lane);
+    wv_opaque_v(in0);
+    wv_opaque_v(in1);
     const int gi = it0 + it;
     if (valid && Pg.q_trace && gi < Pg.trace_iters) {
       if (lane < na) Pg.q_trace[(gi * na + lane) * S + s] = q;
@@ -291,7 +319,8 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const 
 #pragma unroll
       for (int t = 0; t < NP; ++t) {
         if (t < n_pose) {
-          const unsigned long long m = cmask[t];
+          unsigned long long m = cmask[t];
+          wv_opaque(m);
 #pragma unroll
           for (int j = 0; j < WV_NA; ++j) {
             if ((m >> j) & 1ull) {
@@ -312,7 +341,8 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const 
 #pragma unroll
       for (int u = 0; u < NQ; ++u) {
         if (u < n_pos) {
-          const unsigned long long m = cmask[NP + u];
+          unsigned long long m = cmask[NP + u];
+          wv_opaque(m);
 #pragma unroll
           for (int j = 0; j < WV_NA; ++j) {
             if ((m >> j) & 1ull) {
@@ -337,51 +367,55 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const 
     }
     if (has_lim) gv = gv + jl * rl;
     if (has_sm) gv = gv + js * rs;
-    // padding rows / columns: identity system
+    // padding rows na..31: identity (their columns in rows < na are exact zeros: no Jacobian
+    // column and no diagonal term reaches them).  Lane-wise select, no per-column test.
+    const bool pad = lane >= na;
 #pragma unroll
     for (int j = 0; j < WV_NA; ++j) {
-      if (lane >= na || j >= na) A[j] = (j == lane) ? 1.f : 0.f;
+      A[j] = pad ? (j == lane ? 1.f : 0.f) : A[j];
       if (j == lane) A[j] = A[j] + lam;
     }
-    if (lane >= na) gv = 0.f;
-    // ---- right-looking Cholesky on register rows (lane i holds row i; upper entries unused)
-    // (structurally zero L[j][k] -- fill_col from the symbolic factorisation -- are skipped: they
-    // would subtract exact zeros)
+    if (pad) gv = 0.f;
+    // ---- right-looking Cholesky on register rows (lane i holds row i; entries above the
+    // diagonal are left as scratch and never read)
+    float dd = 1.f;  // lane k: L_kk
 #pragma unroll
     for (int k = 0; k < WV_NA; ++k) {
       const float d = sqrtf(wv_readlane(A[k], k));
+      if (lane == k) dd = d;
       A[k] = lane == k ? d : A[k] / d;
-      const unsigned fc = k < na ? tm->fill_col[k] : 0u;
 #pragma unroll
-      for (int j = k + 1; j < WV_NA; ++j)
-        if ((fc >> j) & 1u) A[j] -= A[k] * wv_readlane(A[k], j);
+      for (int j = k + 1; j < WV_NA; ++j) A[j] -= A[k] * wv_readlane(A[k], j);
     }
-    // forward substitution: y_k = (-g_k - sum_{i<k} L_ki y_i) / L_kk, column by column
+    const float dinv = 1.f / dd;
+    // forward substitution L y = -g, column by column; step k's update of rows <= k only
+    // touches values already consumed
     float b = -gv, y = 0.f;
 #pragma unroll
     for (int k = 0; k < WV_NA; ++k) {
-      const float yk = wv_readlane(b, k) / wv_readlane(A[k], k);
-      if (lane == k) y = yk;
-      if (k < na && tm->fill_col[k]) b -= A[k] * yk;
+      const float t = b * dinv;
+      const float yk = wv_readlane(t, k);
+      if (lane == k) y = t;
+      b -= A[k] * yk;
     }
-    // backward substitution with the factor's columns (transposed through LDS)
+    // backward substitution L^T x = y with the factor's rows transposed through LDS
     if (lane < WV_NA) {
 #pragma unroll
       for (int j = 0; j < WV_NA; ++j) sh->L[lane][j] = A[j];
     }
     __syncthreads();
     float c = y, dq = 0.f;
+    const int lc = lane < WV_NA ? lane : 0;
 #pragma unroll
     for (int k = WV_NA - 1; k >= 0; --k) {
-      const float xk = wv_readlane(c, k) / sh->L[k][k];
-      if (lane == k) dq = xk;
-      if (k < na && tm->fill_row[k]) c -= (lane < WV_NA ? sh->L[k][lane < WV_NA ? lane : 0] : 0.f) * xk;
+      const float t = c * dinv;
+      const float xk = wv_readlane(t, k);
+      if (lane == k) dq = t;
+      c -= sh->L[k][lc] * xk;
     }
-    float pr = 0.f;
-    for (int i = 0; i < na; ++i) {
-      const float di = wv_readlane(dq, i), gi_ = wv_readlane(gv, i);
-      pr += di * (lam * di - gi_);
-    }
+    if (pad) dq = 0.f;
+    // predicted reduction 1/2 d.(lam d - g), summed over the dofs
+    const float pr = wv_sum(pad ? 0.f : dq * (lam * dq - gv));
     const float pred = 0.5f * pr;
     // ---- proposal
     const float qp = q + dq;
