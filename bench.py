@@ -42,6 +42,33 @@ def f_model(na: int, nj: int, n_pose: int) -> float:
     return 93 * nj * 1 + 39 * nj + 2 * 200 * n_pose + 294 * n_pose + 123 * na * n_pose + 2 * 4 * na + 2 * na
 
 
+PROFILE = os.path.join(ROOT, "profiles", "k_persistent_uneven_ik_16384_counters.md")
+
+
+def variant_name(tpp: int, spec, n_pos: int) -> str:
+    """kernel-variant label of a bucket (as the counter summary names them)"""
+    if tpp == -64:
+        return "V_W012" if n_pos else "V_W40"
+    if tpp == 1:
+        return f"V_T{spec.num_actuated_joints}"
+    return f"G{tpp}"
+
+
+def profiled_traffic(variant: str):
+    """HBM bytes per launch of k_persistent<variant> from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE runs, each scaled by the copy-probe calibration); None if absent"""
+    try:
+        with open(PROFILE) as f:
+            in_traffic = False
+            for line in f:
+                in_traffic = in_traffic or line.startswith("## HBM traffic")
+                if in_traffic and line.startswith(f"| k_persistent<{variant}>"):
+                    return int(line.strip().strip("|").split("|")[-1])
+    except OSError:
+        pass
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,7 +202,7 @@ def main() -> int:
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-            "traffic": None,
+            "traffic": profiled_traffic(variant_name(dom_tpp, mb.buckets[dom_batches[0]].spec, 0)),
             "kernel": f"k_persistent (threads/problem {dom_tpp}; batches {dom_batches})",
             "kernel_ms": round(kavg, 4),
             "flops_per_launch": flops,

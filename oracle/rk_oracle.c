@@ -16,6 +16,24 @@
  *
  * Layout conventions (same as the reference): pose = [x,y,z,qw,qx,qy,qz],
  * twist = [v; w], joint types FIXED 0 / PRISMATIC 1 / REVOLUTE 2.
+ *
+ * Pinned (tests/test_oracle.py) before it is trusted:
+ *   ork_se3_*            golden constants of tests/lie/test_se3.py:39-80, homogeneous-matrix
+ *                        algebra, exp(log T) = T, Ad(T^-1) Ad(T) = I, jlog = finite-difference
+ *                        derivative of log(T exp(xi))
+ *   ork_fk               tests/robo/fk_ground_truth.json (panda, ur10; with/without base),
+ *                        doctest literals of robo/robot.py:139-154
+ *   ork_link_jacobian    tests/robo/jacobian_ground_truth.json (joint + base columns),
+ *                        doctest literals of robo/robot.py:362-376, central differences of FK
+ *   ork_residual_jacobian  central differences of the stacked residual for every term kind,
+ *                        closed forms of the limit / smoothness rows, H = J^T J, g = J^T r
+ *   ork_damped_solve     numpy.linalg.solve(H + lam I, -g); pred = quadratic-model reduction
+ *   ork_accept           truth table of tests/opt/test_optimizer_acceptance.py:41-66 + clamps
+ *   ork_lm_solve/_step   one step = solve(J^T J + lam I, -J^T r) (tests/opt/test_optimizer.py:
+ *                        366-429); step == first solve iteration; IK convergence from near seeds
+ *   ork_ik_metrics       errors of targets displaced by known translations / rotations
+ * Parity unpinned: the exact FP32 rounding sequence of Warp's CPU tile_matmul / tile_cholesky
+ * (Warp is not importable here); only its tolerance-level effect is bounded.
  */
 #include <math.h>
 #include <stdint.h>
