@@ -138,6 +138,10 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
  */
 int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mode, int32_t threads_per_problem,
                    float* kernel_ms);
+/* warm start for a sequence of frames (helpers/retarget: the previous frame's solution is the
+ * next frame's seed and its SmoothnessTask reference, terms/dense/smoothness_task.py:350-365):
+ * on the device, q_prev <- q and seeds <- q; the next rk_batch_set then only brings new targets */
+int rk_batch_advance(rk_batch* b);
 /* outputs (nullable): q [B][na], cost [B], lambda [B], cost_trace [B][trace_iters+1], accept_trace [B][trace_iters] */
 int rk_batch_get(rk_batch* b, float* q, float* cost, float* lambda, float* cost_trace, int32_t* accept_trace);
 /* iterates of the last traced solve: q_trace [B][trace_iters][na] = q before step i,

@@ -1772,6 +1772,18 @@ int rk_batch_get(rk_batch* b, float* q, float* cost, float* lambda, float* cost_
   return RK_OK;
 }
 
+int rk_batch_advance(rk_batch* b) {
+  if (!b) return fail(RK_EINVAL, "null batch");
+  if (b->n <= 0) return fail(RK_EINVAL, "batch has no problems (call rk_batch_set)");
+  rk_context* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t bytes = sizeof(float) * b->na * b->capacity;
+  HIP_TRY(hipMemcpyAsync(b->q_prev, b->q, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(b->q_seed, b->q, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return RK_OK;
+}
+
 int rk_batch_get_iterates(rk_batch* b, float* q_trace, float* lam_trace) {
   if (!b || !q_trace || !lam_trace) return fail(RK_EINVAL, "null argument");
   if (!b->q_trace) return fail(RK_EINVAL, "batch was created without a trace");

@@ -19,6 +19,7 @@ LIB_PATH = PKG / "librkhip.so"
 
 RK_OK, RK_EINVAL, RK_EHIP, RK_ECAPACITY, RK_ENOMEM, RK_EBUSY = range(6)
 MODE_FUSED, MODE_PLAIN = 0, 1
+TPP_WAVE = -64  # one wavefront per problem, register rows
 FROM_SEED, USE_GRAPH = 0x100, 0x1
 SE3_OPS = {"compose": (0, 7), "inverse": (1, 7), "log": (2, 6), "jlog": (3, 36), "adjoint": (4, 36), "apply": (5, 3)}
 
@@ -74,7 +75,7 @@ EXPORTS = [
     "rk_last_error", "rk_abi_version", "rk_context_create", "rk_context_destroy", "rk_context_sync",
     "rk_context_mem", "rk_robot_upload", "rk_fk", "rk_link_jacobian", "rk_copy_probe", "rk_se3_op",
     "rk_batch_create",
-    "rk_batch_destroy", "rk_batch_set", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics", "rk_group_create",
+    "rk_batch_destroy", "rk_batch_set", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics", "rk_group_create",
     "rk_group_destroy", "rk_group_set_tuning", "rk_group_autotune", "rk_group_solve", "rk_group_kernel_times",
 ]
 
@@ -292,6 +293,10 @@ class Batch:
         if traces:
             out.update(cost_trace=ct, accept=at)
         return out
+
+    def advance(self) -> None:
+        """Warm start on the device: q_prev <- q and seeds <- q (next frame of a sequence)."""
+        _check(lib().rk_batch_advance(self._h))
 
     def iterates(self):
         """q before each traced step [B, trace_iters, na] and the damping it used [B, trace_iters]."""

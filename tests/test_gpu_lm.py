@@ -46,11 +46,11 @@ def _gpu_ik(ctx, wl, mode, tpp=0, iters=ITERS, params=C0_PARAMS, capacity=None):
     return out
 
 
-def _check_parity(ctx, wl, mode, label):
+def _check_parity(ctx, wl, mode, label, tpp=0):
     """Shadowing check of every GPU step + identical classification + a free-running report."""
     na = wl.spec.num_actuated_joints
     ora, ok_o = _oracle_ik(wl)
-    gpu = _gpu_ik(ctx, wl, mode)
+    gpu = _gpu_ik(ctx, wl, mode, tpp=tpp)
     B = wl.size
     rows = np.repeat(np.arange(B), ITERS)
     args = (wl.spec, O.ik_terms(wl.links, wl.targets, na), gpu["q_iter"].reshape(-1, na), gpu["lam_iter"].reshape(-1),
@@ -58,7 +58,7 @@ def _check_parity(ctx, wl, mode, label):
     rep = shadow_check(gpu["cost_trace"], gpu["accept"], O.lm_step(*args, rows=rows),
                        O.lm_step(*args, dtype="f64", rows=rows))
     free = compare_traces(gpu["cost_trace"], ora["cost_trace"], gpu["accept"], ora["accept"])
-    print(f"\n{label} mode={mode}: {rep}\n  free-running: forks={free.forks}/{free.n} "
+    print(f"\n{label} mode={mode} tpp={tpp}: {rep}\n  free-running: forks={free.forks}/{free.n} "
           f"converged={ok_o.mean():.3f}")
     assert rep.ok(), rep
     _check_classification(wl, gpu, ora, ok_o)
@@ -87,21 +87,23 @@ def _check_classification(wl, gpu, ora, ok_o):
     assert mism.size <= allowed, f"classification differs on {mism.size} problems (allowed {allowed})"
 
 
-@pytest.mark.parametrize("mode", [N.MODE_FUSED, N.MODE_PLAIN])
+# every kernel family: thread-per-problem registers (1), LDS-staged G lanes (8 small / 64 large),
+# wave-per-problem registers (-64), and the plain per-term launch
+@pytest.mark.parametrize("mode,tpp", [(N.MODE_FUSED, 1), (N.MODE_FUSED, 8), (N.MODE_FUSED, 64), (N.MODE_PLAIN, 0)])
 @pytest.mark.parametrize("robot", ["panda_dh", "ur10"])
-def test_ik_parity_serial(ctx, mode, robot):
+def test_ik_parity_serial(ctx, mode, tpp, robot):
     spec = W.panda_dh() if robot == "panda_dh" else W.ur10()
     wl = W.ik_batch(spec, [spec.num_links - 1], 256, seed=11)
-    _check_parity(ctx, wl, mode, robot)
+    _check_parity(ctx, wl, mode, robot, tpp)
 
 
-@pytest.mark.parametrize("mode", [N.MODE_FUSED, N.MODE_PLAIN])
-def test_ik_parity_humanoid(ctx, mode):
+@pytest.mark.parametrize("mode,tpp", [(N.MODE_FUSED, N.TPP_WAVE), (N.MODE_FUSED, 32), (N.MODE_PLAIN, 0)])
+def test_ik_parity_humanoid(ctx, mode, tpp):
     spec = W.humanoid29(0)
     wl = W.ik_batch(spec, W.humanoid_ee_links(spec), 64, seed=12)
     # seeds near the target joints so that a share of the problems converges
     wl.q_seed = (wl.q_target + np.random.default_rng(1).normal(0, 0.3, wl.q_target.shape)).astype(np.float32)
-    _check_parity(ctx, wl, mode, "humanoid")
+    _check_parity(ctx, wl, mode, "humanoid", tpp)
 
 
 def test_fused_and_plain_agree(ctx):
@@ -193,3 +195,79 @@ def test_autotune_keeps_results(ctx):
     g.solve(10, C0_PARAMS, use_graph=True)
     for r, b in zip(ref, batches):
         assert np.array_equal(b.get()["q"], r)
+
+
+# ----------------------------------------------------------------------------- retargeting (C2 shape)
+RT_PARAMS = N.LMParams(lm_lambda=1e-2, lambda_factor=10.0)
+RT_POS_W, RT_SMOOTH_W, RT_LIMIT_W = 1.0, 0.1, 1.0
+
+
+def _retarget_terms(spec, links):
+    na = spec.num_actuated_joints
+    ts = N.TermSpec(pos_links=links, pos_weights=np.full((len(links), 3), RT_POS_W, np.float32),
+                    limit_weights=np.full(na, RT_LIMIT_W, np.float32),
+                    smooth_weights=np.full(na, RT_SMOOTH_W, np.float32))
+    return ts
+
+
+def _oracle_retarget_terms(spec, links, targets, q_prev):
+    na = spec.num_actuated_joints
+    return [O.Term(O.POSITION, links, targets[..., :3], np.full(3 * len(links), RT_POS_W, np.float32)),
+            O.Term(O.LIMIT, (), None, np.full(na, RT_LIMIT_W, np.float32)),
+            O.Term(O.SMOOTH, (), None, np.full(na, RT_SMOOTH_W, np.float32), q_prev)]
+
+
+@pytest.mark.parametrize("mode,tpp", [(N.MODE_FUSED, N.TPP_WAVE), (N.MODE_FUSED, 64), (N.MODE_PLAIN, 0)])
+def test_retarget_parity(ctx, mode, tpp):
+    """12 link-position targets + smoothness to the previous frame + limits on the 29-dof tree:
+    frames t of a smooth clip, each seeded (and smoothed) at the clip's frame t-1 (+ noise)."""
+    clip = W.config2(seed=2, frames=129)
+    spec, links = clip.spec, clip.links
+    na = spec.num_actuated_joints
+    B, iters = 128, 20
+    rng = np.random.default_rng(3)
+    q_prev = (clip.q_traj[:-1] + rng.normal(0, 0.05, (B, na))).astype(np.float32)
+    targets = clip.targets[1:]
+    b = N.Batch(ctx, spec, _retarget_terms(spec, links), B, trace_iters=iters)
+    b.set(q=q_prev, pos_targets=targets, q_prev=q_prev)
+    b.solve(iters, RT_PARAMS, mode=mode, tpp=tpp)
+    gpu = b.get(traces=True)
+    q_it, lam_it = b.iterates()
+    terms = _oracle_retarget_terms(spec, links, targets, q_prev)
+    rows = np.repeat(np.arange(B), iters)
+    p = O.LMParams(**RT_PARAMS.__dict__)
+    rep = shadow_check(gpu["cost_trace"], gpu["accept"],
+                       O.lm_step(spec, terms, q_it.reshape(-1, na), lam_it.reshape(-1), p, rows=rows),
+                       O.lm_step(spec, terms, q_it.reshape(-1, na), lam_it.reshape(-1), p, dtype="f64", rows=rows))
+    ora = O.lm_solve(spec, terms, q_prev, iters, p)
+    print(f"\nretarget mode={mode} tpp={tpp}: {rep}; final cost gpu {gpu['cost'].mean():.4e} "
+          f"oracle {ora['cost'].mean():.4e}")
+    assert rep.ok(), rep
+    # converged costs: the same minimum to FP32 rounding of the noise-limited fit
+    np.testing.assert_allclose(gpu["cost"], ora["cost"], rtol=2e-3, atol=1e-7)
+
+
+def test_retarget_warm_start_resident(ctx):
+    """Sequence of frames with the warm start kept on the device (rk_batch_advance: q_prev and
+    seeds <- the last solution) equals the same sequence driven through host copies, bitwise."""
+    clip = W.config2(seed=4, frames=9)
+    spec, links = clip.spec, clip.links
+    na = spec.num_actuated_joints
+    S = 32  # independent streams: stream s follows the clip with its own offset in joint space
+    off = np.random.default_rng(5).normal(0, 0.02, (S, 1, na)).astype(np.float32)
+    q0 = (clip.q_traj[0][None] + off[:, 0]).astype(np.float32)
+    dev = N.Batch(ctx, spec, _retarget_terms(spec, links), S)
+    host = N.Batch(ctx, spec, _retarget_terms(spec, links), S)
+    dev.set(q=q0, q_prev=q0, pos_targets=np.repeat(clip.targets[1][None], S, 0))
+    host.set(q=q0, q_prev=q0, pos_targets=np.repeat(clip.targets[1][None], S, 0))
+    for t in range(1, 9):
+        tg = np.repeat(clip.targets[t][None], S, 0)
+        if t > 1:
+            dev.advance()
+            dev.set(pos_targets=tg)
+            qh = host.get()["q"]
+            host.set(q=qh, q_prev=qh, pos_targets=tg)
+        dev.solve(10, RT_PARAMS, from_seed=True)
+        host.solve(10, RT_PARAMS, from_seed=True)
+        a, h = dev.get(), host.get()
+        assert np.array_equal(a["q"], h["q"]) and np.array_equal(a["cost"], h["cost"]), t
