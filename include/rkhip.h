@@ -124,6 +124,8 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
 #define RK_MODE_FUSED 0 /* one launch: whole solve per problem in LDS/registers */
 #define RK_MODE_PLAIN 1 /* reference decomposition: per-term kernels, J in HBM, host loop */
 #define RK_FROM_SEED 0x100 /* flag: start from the seeds of the last rk_batch_set (kept on the device) */
+#define RK_CONTINUE 0x200  /* flag: continue the last solve (current q and damping, iteration count and
+                            * traces carry on) -- Optimizer.step after reset (opt/optimizer.py:280-287) */
 /* threads_per_problem codes: 1 = one lane per problem, register-resident (serial arms);
  * RK_TPP_WAVE = one wavefront per problem, register-resident rows (trees, <= 32 dofs);
  * 4, 8, 16, 32, 64 = that many lanes per problem, LDS-staged (any robot the batch accepts) */
@@ -135,6 +137,7 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
  * lm_optimizer.py:834-1005).  threads_per_problem (fused mode): 0 = library default, else one
  * of 4, 8, 16 (robots with nj <= 16, na <= 8, n_pose <= 2, n_pos <= 2) or 16, 32, 64.
  * kernel_ms (nullable) = HIP-event time of the solve's kernels on the context stream.
+ * iters = 0 only evaluates the cost at the current q (or the seeds): rk_batch_get returns it.
  */
 int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mode, int32_t threads_per_problem,
                    float* kernel_ms);

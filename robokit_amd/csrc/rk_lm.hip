@@ -805,6 +805,29 @@ __global__ void k_plain_integrate(Problems P, PlainBufs B) {
   B.qp[d * S + b] = P.q[d * S + b] + B.dq[d * S + b];
 }
 
+// cost at the current iterate from its residual rows (strided layout): rk_batch_solve with zero
+// iterations (LMOptimizer._start scores the start, opt/lm_optimizer.py:636-640)
+__global__ void k_plain_cost(Problems P, const float* __restrict__ r) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.n) return;
+  const int S = P.stride, na = P.robot->na;
+  const Terms* tm = P.terms;
+  float c = 0.f;
+  int off = 0;
+  const int sizes[4] = {tm->Rf, tm->Rq, tm->limit ? na : 0, tm->smooth ? na : 0};
+  for (int t = 0; t < 4; ++t) {
+    if (!sizes[t]) continue;
+    float s = 0.f;
+    for (int k = 0; k < sizes[t]; ++k) {
+      const float v = r[(off + k) * S + b];
+      s += v * v;
+    }
+    c = c + 0.5f * s;
+    off += sizes[t];
+  }
+  P.cost[b] = c;
+}
+
 __global__ void k_plain_accept(Problems P, PlainBufs B, LMParams lp, int it) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= P.n) return;
@@ -1045,6 +1068,7 @@ struct rk_batch {
   Terms th{};
   Terms* td = nullptr;
   int capacity = 0, n = 0, trace_iters = 0, na = 0;
+  int it_done = 0;  // iterations run since the last rk_batch_set / seeded solve (RK_CONTINUE offset)
   float *q = nullptr, *q_seed = nullptr, *pose_tgt = nullptr, *pos_tgt = nullptr, *q_prev = nullptr;
   float *cost = nullptr, *lam = nullptr, *trace = nullptr;
   int *nacc = nullptr, *acc_trace = nullptr;
@@ -1256,7 +1280,33 @@ static int launch_plain_solve(rk_context* ctx, const Problems& P, const PlainBuf
 }
 
 // the reference's GPU decomposition, one host-loop iteration = 13 launches for IK terms
-static int solve_plain(rk_batch* b, int iters, const LMParams& lp, bool from_seed) {
+// cost at the current q (or the seeds) without iterating, through the per-term residual kernels
+static int eval_cost(rk_batch* b, bool from_seed) {
+  int rc;
+  if ((rc = alloc_plain(b))) return rc;
+  rk_context* ctx = b->ctx;
+  if (from_seed)
+    HIP_TRY(hipMemcpyAsync(b->q, b->q_seed, sizeof(float) * b->na * b->capacity, hipMemcpyDeviceToDevice,
+                           ctx->stream));
+  Problems P = make_problems(b);
+  const int na = ctx->robots[b->th.robot].na;
+  const dim3 blk(256), g1((b->n + 255) / 256);
+  hipLaunchKernelGGL(k_plain_fk, g1, blk, 0, ctx->stream, P, (const float*)b->q, b->pb.Tj, (float*)nullptr);
+  if (b->th.n_pose)
+    hipLaunchKernelGGL(k_plain_pose_res, dim3(g1.x, b->th.n_pose), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
+                       b->pb.r);
+  if (b->th.n_pos)
+    hipLaunchKernelGGL(k_plain_pos_res, dim3(g1.x, b->th.n_pos), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
+                       b->pb.r);
+  if (b->th.limit || b->th.smooth)
+    hipLaunchKernelGGL(k_plain_diag_rows, dim3(g1.x, na), blk, 0, ctx->stream, P, (const float*)b->q, b->pb.r,
+                       (float*)nullptr);
+  hipLaunchKernelGGL(k_plain_cost, g1, blk, 0, ctx->stream, P, (const float*)b->pb.r);
+  HIP_TRY(hipGetLastError());
+  return RK_OK;
+}
+
+static int solve_plain(rk_batch* b, int it0, int iters, const LMParams& lp, bool from_seed) {
   int rc;
   if ((rc = alloc_plain(b))) return rc;
   rk_context* ctx = b->ctx;
@@ -1269,8 +1319,8 @@ static int solve_plain(rk_batch* b, int iters, const LMParams& lp, bool from_see
   const dim3 blk(256);
   const dim3 g1((b->n + 255) / 256);
   const bool small = fits_small(b, rob);
-  hipLaunchKernelGGL(k_init_lm, g1, blk, 0, ctx->stream, P, lp.lm_lambda);
-  for (int it = 0; it < iters; ++it) {
+  if (it0 == 0) hipLaunchKernelGGL(k_init_lm, g1, blk, 0, ctx->stream, P, lp.lm_lambda);
+  for (int it = it0; it < it0 + iters; ++it) {
     if (P.q_trace) hipLaunchKernelGGL(k_plain_trace_state, g1, blk, 0, ctx->stream, P, it);
     hipLaunchKernelGGL(k_plain_fk, g1, blk, 0, ctx->stream, P, (const float*)b->q, b->pb.Tj, b->pb.Sw);
     const long nJ = (long)b->th.R * na * S;
@@ -1701,6 +1751,7 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
   HIP_TRY(hipSetDevice(ctx->device));
   int rc;
   b->n = B;
+  b->it_done = 0;
   if (q) {
     if ((rc = upload_soa(ctx, b->q, q, B, b->na, b->capacity))) return rc;
     HIP_TRY(hipMemcpyAsync(b->q_seed, b->q, sizeof(float) * b->na * b->capacity, hipMemcpyDeviceToDevice,
@@ -1728,17 +1779,24 @@ int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mo
   int rc = RK_OK;
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
   const bool from_seed = (mode & RK_FROM_SEED) != 0;
-  mode &= ~RK_FROM_SEED;
-  if (mode == RK_MODE_PLAIN) {
-    rc = solve_plain(b, iters, lp, from_seed);
+  const bool cont = (mode & RK_CONTINUE) != 0;
+  mode &= ~(RK_FROM_SEED | RK_CONTINUE);
+  if (cont && from_seed) return fail(RK_EINVAL, "RK_CONTINUE and RK_FROM_SEED exclude each other");
+  const int it0 = cont ? b->it_done : 0;
+  if (b->trace && it0 + iters > b->trace_iters) return fail(RK_EINVAL, "iterations exceed the batch's trace length");
+  if (iters == 0 && (mode == RK_MODE_PLAIN || mode == RK_MODE_FUSED)) {
+    rc = eval_cost(b, from_seed);
+  } else if (mode == RK_MODE_PLAIN) {
+    rc = solve_plain(b, it0, iters, lp, from_seed);
   } else if (mode == RK_MODE_FUSED) {
     int v;
     if ((rc = pick_variant(b, tpp, &v))) return rc;
-    rc = launch_variant(ctx, v, make_problems(b, from_seed), lp, 0, iters);
+    rc = launch_variant(ctx, v, make_problems(b, from_seed), lp, it0, iters);
   } else {
     return fail(RK_EINVAL, "mode must be RK_MODE_FUSED or RK_MODE_PLAIN");
   }
   if (rc) return rc;
+  b->it_done = it0 + iters;
   HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
   HIP_TRY(hipEventSynchronize(ctx->ev1));
   if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, ctx->ev0, ctx->ev1));

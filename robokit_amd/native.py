@@ -20,7 +20,7 @@ LIB_PATH = PKG / "librkhip.so"
 RK_OK, RK_EINVAL, RK_EHIP, RK_ECAPACITY, RK_ENOMEM, RK_EBUSY = range(6)
 MODE_FUSED, MODE_PLAIN = 0, 1
 TPP_WAVE = -64  # one wavefront per problem, register rows
-FROM_SEED, USE_GRAPH = 0x100, 0x1
+FROM_SEED, CONTINUE, USE_GRAPH = 0x100, 0x200, 0x1
 SE3_OPS = {"compose": (0, 7), "inverse": (1, 7), "log": (2, 6), "jlog": (3, 36), "adjoint": (4, 36), "apply": (5, 3)}
 
 
@@ -137,12 +137,18 @@ class Context:
         self._h = C.c_void_p()
         _check(lib().rk_context_create(int(device), C.byref(self._h)))
         _LIVE["context"].add(self)
+        self._children = weakref.WeakSet()  # batches / groups / servers on this context
         self.device = device
         self._robots: Dict[int, int] = {}
         self._keep: List = []
 
     def close(self) -> None:
+        """Destroy the context; objects still built on it are closed first (their handles point
+        into it)."""
         if self._h:
+            kids = list(self._children)
+            for k in sorted(kids, key=lambda o: 0 if isinstance(o, Group) else 1):
+                k.close()
             lib().rk_context_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -248,10 +254,14 @@ class Batch:
         self._h = C.c_void_p()
         _check(lib().rk_batch_create(ctx.handle, C.byref(d), self.capacity, self.trace_iters, C.byref(self._h)))
         _LIVE["batch"].add(self)
+        ctx._children.add(self)
+        self._groups = weakref.WeakSet()
         self.n = 0
 
     def close(self) -> None:
         if self._h:
+            for g in list(self._groups):  # a group holds this batch's handle
+                g.close()
             lib().rk_batch_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -275,9 +285,11 @@ class Batch:
         _check(lib().rk_batch_set(self._h, int(n), *[_p(x) for x in arrs]))
         self.n = int(n)
 
-    def solve(self, iters: int, params: LMParams, mode: int = MODE_FUSED, tpp: int = 0, from_seed: bool = False) -> float:
+    def solve(self, iters: int, params: LMParams, mode: int = MODE_FUSED, tpp: int = 0, from_seed: bool = False,
+              cont: bool = False) -> float:
+        """Run ``iters`` LM iterations; ``cont`` continues the previous solve (q, damping, traces)."""
         ms = C.c_float()
-        m = int(mode) | (FROM_SEED if from_seed else 0)
+        m = int(mode) | (FROM_SEED if from_seed else 0) | (CONTINUE if cont else 0)
         _check(lib().rk_batch_solve(self._h, int(iters), C.byref(params.c()), m, int(tpp), C.byref(ms)))
         return float(ms.value)
 
@@ -324,6 +336,9 @@ class Group:
         self._h = C.c_void_p()
         _check(lib().rk_group_create(ctx.handle, len(self.batches), arr, C.byref(self._h)))
         _LIVE["group"].add(self)
+        ctx._children.add(self)
+        for b in self.batches:
+            b._groups.add(self)
 
     def close(self) -> None:
         if self._h:

@@ -1,0 +1,140 @@
+"""The reference-shaped solver API (robokit_amd.lm): construction and validation on the CPU,
+solves on the GPU through the C-ABI.
+
+Mirrors the reference's tests of the LM lifecycle: config defaults
+(tests/opt/test_optimizer_acceptance.py:66-67), capacity errors (opt/optimizer.py:321-322),
+unsupported residual modes raise ValueError (frame_task.py:87-88, position_limit.py:225-226),
+iteration callbacks see every step (opt/optimizer.py:226-236), early stopping at cost_tol
+(opt/optimizer.py:237-239).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from robokit_amd import lm as L
+from robokit_amd import native as N
+from robokit_amd import workloads as W
+
+
+def _ik_problem(B=64, seed=0, spread=0.4):
+    spec = W.panda_dh()
+    robot = L.Robot(spec)
+    wl = W.ik_batch(spec, [spec.num_links - 1], B, seed=seed)
+    lim = spec.actuated_joint_limits_f32
+    q0 = np.clip(wl.q_target + np.random.default_rng(seed).uniform(-spread, spread, wl.q_target.shape),
+                 lim[:, 0], lim[:, 1]).astype(np.float32)
+    return robot, wl, q0
+
+
+# ----------------------------------------------------------------------------- CPU
+def test_config_defaults_match_reference():
+    c = L.LMOptimizerConfig()
+    assert (c.max_iter, c.early_stopping_interval, c.cost_tol, c.cuda_graph_mode) == (100, 1, 1e-9, "none")
+    assert (c.lm_lambda, c.lambda_factor, c.lambda_min, c.lambda_max) == (1e-3, 2.0, 1e-5, 1e10)
+    assert (c.rho_min, c.gain_ratio_epsilon, c.linear_solver) == (1e-3, 1e-8, "cholesky")
+
+
+def test_term_validation():
+    robot = L.Robot(W.ur10())
+    T = np.array([0.5, 0.1, 0.4, 1, 0, 0, 0], np.float32)
+    with pytest.raises(ValueError):
+        L.FrameTask(robot, 7, T, residual_mode="bogus")
+    with pytest.raises(NotImplementedError):
+        L.FrameTask(robot, 7, T, residual_mode="distance")
+    with pytest.raises(ValueError):
+        L.PositionLimit(robot, residual_mode="bogus")
+    with pytest.raises(ValueError):
+        L.FrameTask(robot, 99, T)
+    with pytest.raises(ValueError):
+        L.FrameTask(robot, [6, 7], T[None])  # one target for two frames
+    ft = L.FrameTask(robot, 7, T, position_weight=2.0, orientation_weight=0.5)
+    np.testing.assert_array_equal(ft.weights, [[2, 2, 2, 0.5, 0.5, 0.5]])
+    assert ft.T_world_target.shape == (1, 1, 7) and ft.residual_dim == 6
+    pt = L.PositionTask(robot, [5, 7], np.zeros((2, 3)), weight=[1, 2, 3])
+    assert pt.weights.shape == (2, 3) and pt.T_world_target.shape == (1, 2, 3)
+    assert L.PositionTask(robot, 7, T).weights[0, 0] == 20.0  # reference default weight
+    np.testing.assert_array_equal(L.PositionLimit(robot).weights, np.ones(6))
+    with pytest.raises(ValueError):
+        L.RobotState(robot, np.zeros((3, 5)))
+
+
+# ----------------------------------------------------------------------------- GPU
+@pytest.mark.gpu
+def test_optimizer_matches_batch_api(ctx):
+    robot, wl, q0 = _ik_problem()
+    cfg = L.LMOptimizerConfig(max_iter=30, early_stopping_interval=0, lm_lambda=1e-2, lambda_factor=10.0)
+    terms = [L.FrameTask(robot, wl.links[0], wl.targets[:, 0]), L.PositionLimit(robot, weight=1.0)]
+    var = L.VarValues({L.VarKey.ROBOT: robot.state(q0)})
+    var, costs = cfg.build(terms).solve(var)
+    b = N.Batch(ctx, robot.spec, N.TermSpec(pose_links=wl.links, pose_weights=np.ones((1, 6), np.float32),
+                                            limit_weights=np.ones(7, np.float32)), 64)
+    b.set(q=q0, pose_targets=wl.targets)
+    b.solve(30, cfg.params())
+    ref = b.get()
+    np.testing.assert_array_equal(var.get(L.VarKey.ROBOT).q, ref["q"])
+    np.testing.assert_array_equal(costs, ref["cost"])
+
+
+@pytest.mark.gpu
+def test_steps_continue_the_solve(ctx):
+    """early_stopping_interval 5 runs the solve as continued chunks; with no early stop it equals
+    one 30-iteration launch bitwise (damping and iterate carry over on the device)."""
+    robot, wl, q0 = _ik_problem(seed=1, spread=2.0)
+    terms = [L.FrameTask(robot, wl.links[0], wl.targets[:, 0]), L.PositionLimit(robot)]
+    base = dict(max_iter=30, lm_lambda=1e-2, lambda_factor=10.0, cost_tol=0.0)
+    v1 = L.VarValues({L.VarKey.ROBOT: robot.state(q0)})
+    v2 = L.VarValues({L.VarKey.ROBOT: robot.state(q0)})
+    _, c1 = L.LMOptimizerConfig(early_stopping_interval=0, **base).build(terms).solve(v1)
+    _, c2 = L.LMOptimizerConfig(early_stopping_interval=5, **base).build(terms).solve(v2)
+    np.testing.assert_array_equal(v1.get(L.VarKey.ROBOT).q, v2.get(L.VarKey.ROBOT).q)
+    np.testing.assert_array_equal(c1, c2)
+
+
+@pytest.mark.gpu
+def test_iteration_callback_and_cost_trace(ctx):
+    robot, wl, q0 = _ik_problem(seed=2)
+    terms = [L.FrameTask(robot, wl.links[0], wl.targets[:, 0]), L.PositionLimit(robot)]
+    cfg = L.LMOptimizerConfig(max_iter=12, early_stopping_interval=0, lm_lambda=1e-2, lambda_factor=10.0)
+    seen = []
+    var = L.VarValues({L.VarKey.ROBOT: robot.state(q0)})
+    cfg.build(terms).solve(var, iteration_callback=lambda i, v, c: seen.append((i, c.copy())))
+    assert [i for i, _ in seen] == list(range(13))
+    costs = np.stack([c for _, c in seen])
+    assert np.all(np.diff(costs, axis=0) <= 0)  # LM never accepts an uphill step
+    # step 0 is the cost at the seeds: the oracle's seed cost
+    ora = O.lm_solve(robot.spec, O.ik_terms(wl.links, wl.targets, 7), q0, 0, O.LMParams())
+    np.testing.assert_allclose(costs[0], ora["cost"], rtol=1e-5, atol=1e-9)
+    with pytest.raises(ValueError):
+        L.LMOptimizerConfig(cuda_graph_mode="full").build(terms).solve(var, iteration_callback=lambda *a: None)
+
+
+@pytest.mark.gpu
+def test_early_stop_and_capacity(ctx):
+    robot, wl, q0 = _ik_problem(seed=3, spread=0.05)
+    terms = [L.FrameTask(robot, wl.links[0], wl.targets[:, 0]), L.PositionLimit(robot)]
+    calls = []
+    cfg = L.LMOptimizerConfig(max_iter=100, early_stopping_interval=1, cost_tol=1e-6, lm_lambda=1e-2,
+                              lambda_factor=10.0)
+    opt = cfg.build(terms)
+    var = L.VarValues({L.VarKey.ROBOT: robot.state(q0)})
+    _, costs = opt.solve(var, iteration_callback=lambda i, v, c: calls.append(i))
+    assert np.all(costs <= 1e-6) and len(calls) < 101  # stopped early once every cost met cost_tol
+    # a smaller batch is a prefix; a larger one is refused
+    small = L.VarValues({L.VarKey.ROBOT: robot.state(q0[:10])})
+    terms[0].set_target(wl.targets[:10, 0])
+    opt.solve(small)
+    big = L.VarValues({L.VarKey.ROBOT: robot.state(np.concatenate([q0, q0]))})
+    terms[0].set_target(np.concatenate([wl.targets[:, 0]] * 2))
+    with pytest.raises(ValueError, match="exceeds"):
+        opt.solve(big)
+
+
+@pytest.mark.gpu
+def test_ik_metrics_api(ctx):
+    robot, wl, q0 = _ik_problem(seed=4)
+    pe, ae, ok = L.ik_metrics(robot, wl.q_target, wl.links, wl.targets[:, 0])
+    assert ok.all() and pe.max() < 1e-5
+    pe2, ae2, ok2 = O.ik_metrics(robot.spec, q0, wl.links, wl.targets)
+    pe3, ae3, ok3 = L.ik_metrics(robot, q0, wl.links, wl.targets[:, 0])
+    np.testing.assert_allclose(pe3, pe2, atol=2e-6)
+    np.testing.assert_array_equal(ok3, ok2)
