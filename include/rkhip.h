@@ -175,6 +175,52 @@ int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t fl
  * (0 after a graph replay) */
 int rk_group_kernel_times(rk_group* g, int32_t cap, int32_t* tpp, float* ms, int32_t* batch_mask, int32_t* n);
 
+/*
+ * Request server: continuous batching of solves under a device-memory budget.  No reference
+ * counterpart (the reference solves one caller's batch at a time); it serves many callers of the
+ * LM path of opt/optimizer.py:216-240 concurrently.  A worker thread packs every queued request of
+ * one term structure and iteration count into one launch whenever one of `streams` launch streams
+ * is free and the `mem_budget`-byte device arena has room; launches on different streams overlap.
+ * Robots and term structures stay resident; a session keeps its last solution resident as the
+ * next request's seeds and smoothness reference.  success uses pos_thr / ang_thr
+ * (helpers/ik/ik.py:59-82) and needs pose targets.
+ */
+typedef struct {
+    int32_t terms;               /* id from rk_server_add_terms */
+    int32_t n;                   /* problems, 1..max_launch_problems */
+    int32_t iters;               /* fixed LM iterations */
+    rk_lm_params params;
+    const float* q;              /* [n][na] seeds; NULL: the session's last solution */
+    const float* pose_targets;   /* [n][n_pose][7] */
+    const float* pos_targets;    /* [n][n_pos][3] */
+    const float* q_prev;         /* [n][na] smoothness reference; NULL: the session's last solution, else q */
+    int64_t session;             /* 0: none */
+} rk_request;
+
+typedef struct {
+    int64_t submitted, completed, failed, queued, launches, problems, converged;
+    double p50_ms, p99_ms, mean_ms, max_ms;   /* submit -> results ready, completed requests */
+    double mean_problems_per_launch;
+    int64_t budget_bytes, peak_bytes, bytes_in_use;
+    double span_s;                            /* first submit -> last completion */
+} rk_serve_stats;
+
+int rk_server_create(rk_context* ctx, int64_t mem_budget, int32_t streams, int32_t max_launch_problems,
+                     float pos_thr, float ang_thr, rk_server** out);
+/* register a term structure (resident; its kernel variant is chosen once) */
+int rk_server_add_terms(rk_server* s, const rk_terms_desc* terms, int32_t* term_id);
+/* queue a request (inputs copied); returns at once with a ticket */
+int rk_server_submit(rk_server* s, const rk_request* req, int64_t* ticket);
+/* results of a ticket: wait_ms 0 = poll (RK_EBUSY while running), < 0 = block; outputs nullable:
+ * q [n][na], cost [n], success [n], pos_err / ang_err [n][n_pose], latency_ms; collects the ticket */
+int rk_server_poll(rk_server* s, int64_t ticket, int32_t wait_ms, float* q, float* cost, int32_t* success,
+                   float* pos_err, float* ang_err, float* latency_ms);
+/* release a session's resident state (RK_EBUSY while it has requests queued or running) */
+int rk_server_end_session(rk_server* s, int64_t session);
+int rk_server_stats(rk_server* s, rk_serve_stats* out);
+/* drains the queue, waits for the device, frees everything */
+int rk_server_destroy(rk_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,12 +16,17 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rkhip.h"
@@ -1133,47 +1138,47 @@ static Problems make_problems(rk_batch* b, bool from_seed = false) {
   return P;
 }
 
-static bool fits_small(const rk_batch* b, const Robot& r) {
-  return r.nj <= CfgS::nj && r.na <= CfgS::na && b->th.n_pose <= CfgS::np && b->th.n_pos <= CfgS::nq;
+static bool fits_small(const Terms& th, const Robot& r) {
+  return r.nj <= CfgS::nj && r.na <= CfgS::na && th.n_pose <= CfgS::np && th.n_pos <= CfgS::nq;
 }
 
 // register thread-per-problem kernels: serial arms with 5..8 dofs, <= 2 fixed tail joints,
 // <= 1 pose target, no position targets
-static bool fits_thread(const rk_batch* b, const Robot& r) {
-  return r.chain && r.na >= 5 && r.na <= 8 && r.nj <= r.na + 2 && b->th.n_pose <= 1 && b->th.n_pos == 0;
+static bool fits_thread(const Terms& th, const Robot& r) {
+  return r.chain && r.na >= 5 && r.na <= 8 && r.nj <= r.na + 2 && th.n_pose <= 1 && th.n_pos == 0;
 }
 
 // register wave-per-problem kernels: trees with <= 64 joints, <= 32 dofs, each dof driven by
 // <= 2 joints; <= 4 pose targets and no position targets, or <= 12 position targets and no pose
-static int wave_kind(const rk_batch* b, const Robot& r) {
+static int wave_kind(const Terms& th, const Robot& r) {
   if (r.nj > 64 || r.na > WV_NA) return -1;
   for (int a = 0; a < r.na; ++a)
     if (r.col_start[a + 1] - r.col_start[a] > 2) return -1;
-  if (b->th.n_pos == 0 && b->th.n_pose <= 4) return V_W40;
-  if (b->th.n_pose == 0 && b->th.n_pos <= 12) return V_W012;
+  if (th.n_pos == 0 && th.n_pose <= 4) return V_W40;
+  if (th.n_pose == 0 && th.n_pos <= 12) return V_W012;
   return -1;
 }
 
-static int signature(const rk_batch* b) {
-  return ((b->th.robot * 8 + b->th.n_pose) * 16 + b->th.n_pos) * 4 + b->th.limit * 2 + b->th.smooth;
+static int signature(const Terms& th) {
+  return ((th.robot * 8 + th.n_pose) * 16 + th.n_pos) * 4 + th.limit * 2 + th.smooth;
 }
 
-// threads per problem -> variant.  G = 0: the tuned choice for this batch's signature if the
+// threads per problem -> variant.  G = 0: the tuned choice for this term structure if the
 // autotuner made one on this context, else the default (registers where they fit).
-static int pick_variant(const rk_batch* b, int G, int* v) {
-  const Robot& r = b->ctx->robots[b->th.robot];
-  const bool small = fits_small(b, r);
+static int pick_variant(const rk_context* ctx, const Terms& th, int G, int* v) {
+  const Robot& r = ctx->robots[th.robot];
+  const bool small = fits_small(th, r);
   if (G == 0) {
-    auto it = b->ctx->tuned.find(signature(b));
-    if (it != b->ctx->tuned.end()) G = it->second;
-    else G = fits_thread(b, r) ? 1 : wave_kind(b, r) >= 0 ? RK_TPP_WAVE : small ? 8 : 64;
+    auto it = ctx->tuned.find(signature(th));
+    if (it != ctx->tuned.end()) G = it->second;
+    else G = fits_thread(th, r) ? 1 : wave_kind(th, r) >= 0 ? RK_TPP_WAVE : small ? 8 : 64;
   }
-  if (G == 1 && fits_thread(b, r)) {
+  if (G == 1 && fits_thread(th, r)) {
     *v = V_T5 + (r.na - 5);
     return RK_OK;
   }
-  if (G == RK_TPP_WAVE && wave_kind(b, r) >= 0) {
-    *v = wave_kind(b, r);
+  if (G == RK_TPP_WAVE && wave_kind(th, r) >= 0) {
+    *v = wave_kind(th, r);
     return RK_OK;
   }
   if (small && (G == 4 || G == 8 || G == 16)) {
@@ -1188,18 +1193,19 @@ static int pick_variant(const rk_batch* b, int G, int* v) {
               "threads_per_problem must be 1 (serial arms), -64 (register wave: trees), 4, 8, 16 (small robots) "
               "or 16, 32, 64");
 }
+static int pick_variant(const rk_batch* b, int G, int* v) { return pick_variant(b->ctx, b->th, G, v); }
 
 static void legal_tpps(const rk_batch* b, std::vector<int>& out) {
   const Robot& r = b->ctx->robots[b->th.robot];
   out.clear();
-  if (fits_thread(b, r)) out.push_back(1);
-  if (wave_kind(b, r) >= 0) out.push_back(RK_TPP_WAVE);
-  if (fits_small(b, r)) out.insert(out.end(), {4, 8, 16, 32, 64});
+  if (fits_thread(b->th, r)) out.push_back(1);
+  if (wave_kind(b->th, r) >= 0) out.push_back(RK_TPP_WAVE);
+  if (fits_small(b->th, r)) out.insert(out.end(), {4, 8, 16, 32, 64});
   else out.insert(out.end(), {16, 32, 64});
 }
 
 template <int G, class C>
-static int launch_fused(rk_context* ctx, const Problems& P, const LMParams& lp, int it0, int iters) {
+static int launch_fused(hipStream_t st, const Problems& P, const LMParams& lp, int it0, int iters) {
   const size_t lds = (WAVE / G) * sizeof(Shared<C>);
   static bool attr_set = false;
   if (!attr_set) {
@@ -1207,29 +1213,29 @@ static int launch_fused(rk_context* ctx, const Problems& P, const LMParams& lp, 
     attr_set = true;
   }
   const int grid = (P.n + (WAVE / G) - 1) / (WAVE / G);
-  hipLaunchKernelGGL((k_fused<G, C>), dim3(grid), dim3(WAVE), lds, ctx->stream, P, lp, it0, iters);
+  hipLaunchKernelGGL((k_fused<G, C>), dim3(grid), dim3(WAVE), lds, st, P, lp, it0, iters);
   HIP_TRY(hipGetLastError());
   return RK_OK;
 }
 
-static int launch_variant(rk_context* ctx, int v, const Problems& P, const LMParams& lp, int it0, int iters) {
+static int launch_variant(hipStream_t st, int v, const Problems& P, const LMParams& lp, int it0, int iters) {
   switch (v) {
-    case V_S4: return launch_fused<4, CfgS>(ctx, P, lp, it0, iters);
-    case V_S8: return launch_fused<8, CfgS>(ctx, P, lp, it0, iters);
-    case V_S16: return launch_fused<16, CfgS>(ctx, P, lp, it0, iters);
-    case V_L16: return launch_fused<16, CfgL>(ctx, P, lp, it0, iters);
-    case V_L32: return launch_fused<32, CfgL>(ctx, P, lp, it0, iters);
-    case V_L64: return launch_fused<64, CfgL>(ctx, P, lp, it0, iters);
+    case V_S4: return launch_fused<4, CfgS>(st, P, lp, it0, iters);
+    case V_S8: return launch_fused<8, CfgS>(st, P, lp, it0, iters);
+    case V_S16: return launch_fused<16, CfgS>(st, P, lp, it0, iters);
+    case V_L16: return launch_fused<16, CfgL>(st, P, lp, it0, iters);
+    case V_L32: return launch_fused<32, CfgL>(st, P, lp, it0, iters);
+    case V_L64: return launch_fused<64, CfgL>(st, P, lp, it0, iters);
     default: break;
   }
   const dim3 grid((P.n + WAVE - 1) / WAVE);
   switch (v) {
-    case V_T5: hipLaunchKernelGGL((k_thread<5, 7>), grid, dim3(WAVE), 0, ctx->stream, P, lp, it0, iters); break;
-    case V_T6: hipLaunchKernelGGL((k_thread<6, 8>), grid, dim3(WAVE), 0, ctx->stream, P, lp, it0, iters); break;
-    case V_T7: hipLaunchKernelGGL((k_thread<7, 9>), grid, dim3(WAVE), 0, ctx->stream, P, lp, it0, iters); break;
-    case V_T8: hipLaunchKernelGGL((k_thread<8, 10>), grid, dim3(WAVE), 0, ctx->stream, P, lp, it0, iters); break;
-    case V_W40: hipLaunchKernelGGL((k_wave<4, 0>), dim3(P.n), dim3(WAVE), 0, ctx->stream, P, lp, it0, iters); break;
-    default: hipLaunchKernelGGL((k_wave<0, 12>), dim3(P.n), dim3(WAVE), 0, ctx->stream, P, lp, it0, iters); break;
+    case V_T5: hipLaunchKernelGGL((k_thread<5, 7>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_T6: hipLaunchKernelGGL((k_thread<6, 8>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_T7: hipLaunchKernelGGL((k_thread<7, 9>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_T8: hipLaunchKernelGGL((k_thread<8, 10>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_W40: hipLaunchKernelGGL((k_wave<4, 0>), dim3(P.n), dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    default: hipLaunchKernelGGL((k_wave<0, 12>), dim3(P.n), dim3(WAVE), 0, st, P, lp, it0, iters); break;
   }
   HIP_TRY(hipGetLastError());
   return RK_OK;
@@ -1318,7 +1324,7 @@ static int solve_plain(rk_batch* b, int it0, int iters, const LMParams& lp, bool
   const int na = rob.na, S = b->capacity;
   const dim3 blk(256);
   const dim3 g1((b->n + 255) / 256);
-  const bool small = fits_small(b, rob);
+  const bool small = fits_small(b->th, rob);
   if (it0 == 0) hipLaunchKernelGGL(k_init_lm, g1, blk, 0, ctx->stream, P, lp.lm_lambda);
   for (int it = it0; it < it0 + iters; ++it) {
     if (P.q_trace) hipLaunchKernelGGL(k_plain_trace_state, g1, blk, 0, ctx->stream, P, it);
@@ -1624,12 +1630,13 @@ int rk_se3_op(rk_context* ctx, int32_t op, int32_t n, const float* a, const floa
   return RK_OK;
 }
 
-int rk_batch_create(rk_context* ctx, const rk_terms_desc* t, int32_t capacity, int32_t trace_iters, rk_batch** out) {
+}  // extern "C"
+
+// validated term structure of a descriptor (rk_batch_create, rk_server_add_terms)
+static int make_terms(rk_context* ctx, const rk_terms_desc* t, Terms* out) {
   int rc;
   if (!t || !out) return fail(RK_EINVAL, "null argument");
   if ((rc = check_robot(ctx, t->robot))) return rc;
-  if (capacity <= 0) return fail(RK_EINVAL, "capacity must be positive");
-  if (trace_iters < 0) return fail(RK_EINVAL, "trace_iters must be >= 0");
   const Robot& r = ctx->robots[t->robot];
   if (t->n_pose < 0 || t->n_pose > RK_MAXT || t->n_pos < 0 || t->n_pos > RK_MAXQ)
     return fail(RK_EINVAL, "at most 4 pose targets and 12 position targets per problem");
@@ -1660,6 +1667,20 @@ int rk_batch_create(rk_context* ctx, const rk_terms_desc* t, int32_t capacity, i
   th.Rf = 6 * th.n_pose;
   th.Rq = 3 * th.n_pos;
   th.R = th.Rf + th.Rq + (th.limit ? r.na : 0) + (th.smooth ? r.na : 0);
+  *out = th;
+  return RK_OK;
+}
+
+extern "C" {
+
+int rk_batch_create(rk_context* ctx, const rk_terms_desc* t, int32_t capacity, int32_t trace_iters, rk_batch** out) {
+  int rc;
+  if (!t || !out) return fail(RK_EINVAL, "null argument");
+  if (capacity <= 0) return fail(RK_EINVAL, "capacity must be positive");
+  if (trace_iters < 0) return fail(RK_EINVAL, "trace_iters must be >= 0");
+  Terms th;
+  if ((rc = make_terms(ctx, t, &th))) return rc;
+  const Robot& r = ctx->robots[t->robot];
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device));
   rk_batch* b = new rk_batch();
@@ -1791,7 +1812,7 @@ int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mo
   } else if (mode == RK_MODE_FUSED) {
     int v;
     if ((rc = pick_variant(b, tpp, &v))) return rc;
-    rc = launch_variant(ctx, v, make_problems(b, from_seed), lp, it0, iters);
+    rc = launch_variant(ctx->stream, v, make_problems(b, from_seed), lp, it0, iters);
   } else {
     return fail(RK_EINVAL, "mode must be RK_MODE_FUSED or RK_MODE_PLAIN");
   }
@@ -2191,7 +2212,7 @@ int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t
       for (int rep = 0; rep < 2; ++rep) {
         HIP_TRY(hipMemcpyAsync(b->q, q_save, sizeof(float) * b->na * S, hipMemcpyDeviceToDevice, ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        if ((rc = launch_variant(ctx, v, make_problems(b), lp, 0, iters))) return rc;
+        if ((rc = launch_variant(ctx->stream, v, make_problems(b), lp, 0, iters))) return rc;
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(hipEventSynchronize(ctx->ev1));
         float ms = 0.f;
@@ -2211,7 +2232,7 @@ int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t
     b->q_trace = qt;
     b->lam_trace = lt;
     g->tpp[i] = best_g;
-    ctx->tuned[signature(b)] = best_g;
+    ctx->tuned[signature(b->th)] = best_g;
     if (chosen) chosen[i] = best_g;
   }
   if (g->graph) {
@@ -2222,3 +2243,5 @@ int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t
 }
 
 }  // extern "C"
+
+#include "rk_serve.h"

@@ -55,11 +55,25 @@ class LMParamsC(C.Structure):
                 ("lm_lambda", "lambda_factor", "lambda_min", "lambda_max", "rho_min", "gain_ratio_epsilon")]
 
 
+class RequestC(C.Structure):
+    _fields_ = [("terms", C.c_int32), ("n", C.c_int32), ("iters", C.c_int32), ("params", LMParamsC),
+                ("q", C.c_void_p), ("pose_targets", C.c_void_p), ("pos_targets", C.c_void_p),
+                ("q_prev", C.c_void_p), ("session", C.c_int64)]
+
+
+class ServeStatsC(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("submitted", "completed", "failed", "queued", "launches", "problems",
+                                         "converged")] + \
+               [(n, C.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "mean_problems_per_launch")] + \
+               [(n, C.c_int64) for n in ("budget_bytes", "peak_bytes", "bytes_in_use")] + [("span_s", C.c_double)]
+
+
 _lib: Optional[C.CDLL] = None
 
 # Live handles, closed at interpreter exit (groups, then batches, then contexts) while the HIP
 # runtime is still up: objects freed during interpreter teardown would call into it afterwards.
-_LIVE = {"group": weakref.WeakSet(), "batch": weakref.WeakSet(), "context": weakref.WeakSet()}
+_LIVE = {"group": weakref.WeakSet(), "server": weakref.WeakSet(), "batch": weakref.WeakSet(),
+         "context": weakref.WeakSet()}
 
 
 @atexit.register
@@ -77,6 +91,8 @@ EXPORTS = [
     "rk_batch_create",
     "rk_batch_destroy", "rk_batch_set", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics", "rk_group_create",
     "rk_group_destroy", "rk_group_set_tuning", "rk_group_autotune", "rk_group_solve", "rk_group_kernel_times",
+    "rk_server_create", "rk_server_add_terms", "rk_server_submit", "rk_server_poll", "rk_server_end_session",
+    "rk_server_stats", "rk_server_destroy",
 ]
 
 
@@ -147,7 +163,7 @@ class Context:
         into it)."""
         if self._h:
             kids = list(self._children)
-            for k in sorted(kids, key=lambda o: 0 if isinstance(o, Group) else 1):
+            for k in sorted(kids, key=lambda o: 0 if isinstance(o, (Group, Server)) else 1):
                 k.close()
             lib().rk_context_destroy(self._h)
             self._h = C.c_void_p()
@@ -376,3 +392,90 @@ class Group:
         _check(lib().rk_group_kernel_times(self._h, cap, _p(tpp), _p(ms), _p(mask), C.byref(n)))
         return [(int(tpp[k]), float(ms[k]), [i for i in range(32) if (int(mask[k]) >> i) & 1])
                 for k in range(min(n.value, cap))]
+
+
+class Server:
+    """Continuous-batching request server (rk_server): submit returns a ticket at once; results are
+    collected with poll.  Requests of one term structure and iteration count that are queued when
+    a launch stream frees up share one launch; launches on different streams overlap."""
+
+    def __init__(self, ctx: Context, mem_budget: int, streams: int = 4, max_launch_problems: int = 4096,
+                 pos_thr: float = 1e-3, ang_thr: float = 1e-2):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        _check(lib().rk_server_create(ctx.handle, int(mem_budget), int(streams), int(max_launch_problems),
+                                      C.c_float(pos_thr), C.c_float(ang_thr), C.byref(self._h)))
+        _LIVE["server"].add(self)
+        ctx._children.add(self)
+        self._terms: List[tuple] = []  # (na, n_pose, n_pos)
+        self._pending: Dict[int, tuple] = {}  # ticket -> (term, n)
+
+    def close(self) -> None:
+        if self._h:
+            lib().rk_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_terms(self, spec, terms: TermSpec) -> int:
+        na = spec.num_actuated_joints
+        arrs = [
+            _i32(list(terms.pose_links)),
+            _f32(terms.pose_weights if terms.pose_weights is not None else np.zeros((0, 6))),
+            _i32(list(terms.pos_links)),
+            _f32(terms.pos_weights if terms.pos_weights is not None else np.zeros((0, 3))),
+            _f32(terms.limit_weights if terms.limit_weights is not None else np.zeros(na)),
+            _f32(terms.smooth_weights if terms.smooth_weights is not None else np.zeros(na)),
+        ]
+        d = TermsDesc(self.ctx.robot(spec), len(terms.pose_links), _p(arrs[0]), _p(arrs[1]), len(terms.pos_links),
+                      _p(arrs[2]), _p(arrs[3]), int(terms.limit_weights is not None), _p(arrs[4]),
+                      int(terms.smooth_weights is not None), _p(arrs[5]))
+        tid = C.c_int32()
+        _check(lib().rk_server_add_terms(self._h, C.byref(d), C.byref(tid)))
+        self._terms.append((na, len(terms.pose_links), len(terms.pos_links)))
+        return tid.value
+
+    def submit(self, term: int, iters: int, params: LMParams, q: Optional[np.ndarray] = None,
+               pose_targets: Optional[np.ndarray] = None, pos_targets: Optional[np.ndarray] = None,
+               q_prev: Optional[np.ndarray] = None, session: int = 0, n: Optional[int] = None) -> int:
+        arrs = [None if x is None else _f32(x) for x in (q, pose_targets, pos_targets, q_prev)]
+        if arrs[2] is not None and arrs[2].shape[-1] == 7:
+            arrs[2] = _f32(arrs[2][..., :3])
+        if n is None:
+            n = next(x.shape[0] for x in arrs if x is not None)
+        req = RequestC(int(term), int(n), int(iters), params.c(), *[_p(x) for x in arrs], int(session))
+        t = C.c_int64()
+        _check(lib().rk_server_submit(self._h, C.byref(req), C.byref(t)))
+        self._pending[t.value] = (int(term), int(n))
+        return t.value
+
+    def poll(self, ticket: int, wait_ms: int = -1) -> Optional[dict]:
+        """Results of a ticket (None while it is still running with wait_ms >= 0)."""
+        term, n = self._pending[ticket]
+        na, npose, _ = self._terms[term]
+        q = np.zeros((n, na), np.float32)
+        cost = np.zeros(n, np.float32)
+        ok = np.zeros(n, np.int32)
+        pe = np.zeros((n, max(npose, 1)), np.float32)
+        ae = np.zeros((n, max(npose, 1)), np.float32)
+        lat = C.c_float()
+        rc = lib().rk_server_poll(self._h, int(ticket), int(wait_ms), _p(q), _p(cost), _p(ok), _p(pe), _p(ae),
+                                  C.byref(lat))
+        if rc == RK_EBUSY:
+            return None
+        del self._pending[ticket]
+        _check(rc)
+        return dict(q=q, cost=cost, success=ok.astype(bool), pos_err=pe[:, :npose], ang_err=ae[:, :npose],
+                    latency_ms=float(lat.value))
+
+    def end_session(self, session: int) -> None:
+        _check(lib().rk_server_end_session(self._h, int(session)))
+
+    def stats(self) -> dict:
+        st = ServeStatsC()
+        _check(lib().rk_server_stats(self._h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in ServeStatsC._fields_}
