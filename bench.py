@@ -69,6 +69,113 @@ def profiled_traffic(variant: str):
     return None
 
 
+RT_POS_W, RT_SMOOTH_W = 1.0, 0.1  # retargeting frame: PositionTask weight, SmoothnessTask weight
+
+
+def _serve_terms(N, trace):
+    """term structure per request kind: IK = FrameTask(log) + PositionLimit; retarget = 12
+    PositionTasks + PositionLimit + SmoothnessTask"""
+    out = {}
+    for kind in trace.kinds:
+        if kind == "retarget":
+            clip = trace.pools[kind][0]
+            na = clip.spec.num_actuated_joints
+            out[kind] = (clip.spec, N.TermSpec(pos_links=clip.links,
+                                               pos_weights=np.full((len(clip.links), 3), RT_POS_W, np.float32),
+                                               limit_weights=np.ones(na, np.float32),
+                                               smooth_weights=np.full(na, RT_SMOOTH_W, np.float32)))
+        else:
+            wl = trace.pools[kind]
+            na = wl.spec.num_actuated_joints
+            out[kind] = (wl.spec, N.TermSpec(pose_links=wl.links, pose_weights=np.ones((len(wl.links), 6), np.float32),
+                                             limit_weights=np.ones(na, np.float32)))
+    return out
+
+
+def _request_inputs(trace, i):
+    """(kind, n, q, pose_targets, pos_targets, q_prev) of request i"""
+    kind = trace.kinds[int(trace.kind[i])]
+    n, o = int(trace.n[i]), int(trace.off[i])
+    if kind == "retarget":
+        clip, prev = trace.pools[kind]
+        return kind, n, prev[o:o + n], None, clip.targets[o + 1:o + 1 + n], prev[o:o + n]
+    wl = trace.pools[kind]
+    return kind, n, wl.q_seed[o:o + n], wl.targets[o:o + n], None, None
+
+
+def run_served(N, ctx, trace, params, iters, budget, streams):
+    """Replay the Poisson trace against the request server in real time; results collected by a
+    second thread (ctypes releases the GIL while a poll waits)."""
+    import queue
+    import threading
+
+    sv = N.Server(ctx, budget, streams=streams, max_launch_problems=8192)
+    terms = _serve_terms(N, trace)
+    tid = {k: sv.add_terms(spec, ts) for k, (spec, ts) in terms.items()}
+    inputs = [_request_inputs(trace, i) for i in range(trace.size)]
+    tq: "queue.Queue" = queue.Queue()
+    conv = {"ik": 0, "ik_problems": 0, "retarget_problems": 0}
+
+    def collect():
+        while True:
+            item = tq.get()
+            if item is None:
+                return
+            ticket, kind = item
+            r = sv.poll(ticket)
+            if kind == "retarget":
+                conv["retarget_problems"] += len(r["cost"])
+            else:
+                conv["ik"] += int(r["success"].sum())
+                conv["ik_problems"] += len(r["cost"])
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    for i in range(trace.size):
+        kind, n, q, pose, pos, qprev = inputs[i]
+        while True:
+            dt = t0 + trace.t[i] - time.perf_counter()
+            if dt <= 0:
+                break
+            if dt > 2e-3:
+                time.sleep(dt - 1e-3)
+        tq.put((sv.submit(tid[kind], iters, params, q=q, pose_targets=pose, pos_targets=pos, q_prev=qprev), kind))
+    tq.put(None)
+    th.join()
+    st = sv.stats()
+    sv.close()
+    return st, conv
+
+
+def run_plain_per_request(N, ctx, trace, params, iters, seconds):
+    """The same arrivals (first `seconds`), each request solved alone by the plain per-term launch
+    as it arrives, one after another on the context stream."""
+    terms = _serve_terms(N, trace)
+    batches = {k: N.Batch(ctx, spec, ts, 256) for k, (spec, ts) in terms.items()}
+    lat, conv_ik = [], 0
+    m = int(np.searchsorted(trace.t, seconds))
+    t0 = time.perf_counter()
+    for i in range(m):
+        kind, n, q, pose, pos, qprev = _request_inputs(trace, i)
+        wait = t0 + trace.t[i] - time.perf_counter()
+        if wait > 0:
+            time.sleep(wait)
+        b = batches[kind]
+        b.set(q=q, pose_targets=pose, pos_targets=pos, q_prev=qprev)
+        b.solve(iters, params, mode=N.MODE_PLAIN)
+        if kind != "retarget":
+            conv_ik += int(b.metrics()[2].sum())
+        else:
+            b.get()
+        lat.append((time.perf_counter() - (t0 + trace.t[i])) * 1e3)
+    span = time.perf_counter() - t0
+    for b in batches.values():
+        b.close()
+    return {"requests": m, "converged": conv_ik, "span_s": span, "value": conv_ik / span,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +184,11 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--problems", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline leg")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of the served Poisson trace (2000 requests/s); 0 skips the served leg")
+    ap.add_argument("--serve-plain-seconds", type=float, default=0.5,
+                    help="arrivals replayed through the plain per-request launch")
+    ap.add_argument("--serve-budget-gb", type=float, default=16.0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a streaming copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -213,6 +325,31 @@ def main() -> int:
             "whole_solve_achieved": round(flops_all / (float(np.mean(kern_ms)) * 1e-3) / 1e12, 4),
         },
     }
+
+    # ---- served: Poisson request stream through the continuous-batching server vs plain per request
+    if args.serve_seconds > 0:
+        trace = W.config4(seed=args.seed, rate=2000.0, seconds=args.serve_seconds)
+        st, conv = run_served(N, ctx, trace, params, ITERS, int(args.serve_budget_gb * (1 << 30)), streams=8)
+        served_value = conv["ik"] / st["span_s"]
+        print(f"[bench] served: {st['completed']} requests in {st['span_s']:.2f} s, p50 {st['p50_ms']:.3f} ms "
+              f"p99 {st['p99_ms']:.3f} ms, {st['mean_problems_per_launch']:.1f} problems/launch", file=sys.stderr,
+              flush=True)
+        plain_req = run_plain_per_request(N, ctx, trace, params, ITERS, args.serve_plain_seconds)
+        print(f"[bench] plain per request: {plain_req}", file=sys.stderr, flush=True)
+        out["served"] = {
+            "workload": f"Poisson 2000 requests/s for {args.serve_seconds:g} s, 1-256 problems per request "
+                        f"(log-uniform), kinds {trace.kinds} uniform; {ITERS} iterations; "
+                        f"budget {args.serve_budget_gb:g} GB, 8 launch streams",
+            "requests": st["completed"], "failed": st["failed"], "problems": st["problems"],
+            "value": round(served_value, 1), "unit": "converged IK solves/s",
+            "converged_ik": conv["ik"], "ik_problems": conv["ik_problems"],
+            "retarget_frames": conv["retarget_problems"],
+            "p50_ms": round(st["p50_ms"], 4), "p99_ms": round(st["p99_ms"], 4), "max_ms": round(st["max_ms"], 4),
+            "launches": st["launches"], "mean_problems_per_launch": round(st["mean_problems_per_launch"], 2),
+            "peak_device_bytes": st["peak_bytes"], "span_s": round(st["span_s"], 3),
+            "plain_per_request": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in plain_req.items()},
+            "speedup_vs_plain_per_request": round(served_value / max(plain_req["value"], 1e-9), 2),
+        }
 
     # ---- CPU baseline: the oracle (a C restatement of the reference's Warp-CPU path), 1 thread
     if not args.no_cpu:

@@ -403,7 +403,7 @@ class Server:
                  pos_thr: float = 1e-3, ang_thr: float = 1e-2):
         self.ctx = ctx
         self._h = C.c_void_p()
-        _check(lib().rk_server_create(ctx.handle, int(mem_budget), int(streams), int(max_launch_problems),
+        _check(lib().rk_server_create(ctx.handle, C.c_int64(int(mem_budget)), int(streams), int(max_launch_problems),
                                       C.c_float(pos_thr), C.c_float(ang_thr), C.byref(self._h)))
         _LIVE["server"].add(self)
         ctx._children.add(self)
@@ -463,7 +463,7 @@ class Server:
         pe = np.zeros((n, max(npose, 1)), np.float32)
         ae = np.zeros((n, max(npose, 1)), np.float32)
         lat = C.c_float()
-        rc = lib().rk_server_poll(self._h, int(ticket), int(wait_ms), _p(q), _p(cost), _p(ok), _p(pe), _p(ae),
+        rc = lib().rk_server_poll(self._h, C.c_int64(int(ticket)), int(wait_ms), _p(q), _p(cost), _p(ok), _p(pe), _p(ae),
                                   C.byref(lat))
         if rc == RK_EBUSY:
             return None
@@ -473,7 +473,7 @@ class Server:
                     latency_ms=float(lat.value))
 
     def end_session(self, session: int) -> None:
-        _check(lib().rk_server_end_session(self._h, int(session)))
+        _check(lib().rk_server_end_session(self._h, C.c_int64(int(session))))
 
     def stats(self) -> dict:
         st = ServeStatsC()

@@ -17,7 +17,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional
+from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
@@ -194,3 +194,49 @@ def config2(seed: int = 0, frames: int = 4096, humanoid_seed: int = 0) -> Retarg
     tg[..., :3] = P
     tg[..., 3] = 1.0
     return RetargetClip(spec, links, q.astype(np.float32), tg)
+
+
+@dataclass
+class ServeTrace:
+    """Poisson request stream over problem pools: request i arrives at t[i] with n[i] problems of
+    kind[i], taken from that kind's pool at offset off[i]."""
+
+    kinds: List[str]
+    pools: Dict[str, object]  # kind -> IKBatch, or (RetargetClip, seeds, q_prev) for "retarget"
+    t: np.ndarray
+    kind: np.ndarray
+    n: np.ndarray
+    off: np.ndarray
+
+    @property
+    def size(self) -> int:
+        return len(self.t)
+
+
+def config4(seed: int = 0, rate: float = 2000.0, seconds: float = 60.0, pool: int = 4096,
+            kinds: Sequence[str] = ("ik6", "ik7", "ik29", "retarget")) -> ServeTrace:
+    """Poisson arrivals at `rate` requests/s for `seconds`; problems per request floor(exp(U(0, ln 257)))
+    in [1, 256]; request kind uniform over `kinds` (IK 6/7/29-DOF as config 1, one retargeting frame
+    of config 2 per problem, seeded at the clip's previous frame)."""
+    s = np.random.SeedSequence(seed).spawn(8)
+    u, p, h = ur10(), panda_dh(), humanoid29(0)
+    pools: Dict[str, object] = {}
+    if "ik6" in kinds:
+        pools["ik6"] = ik_batch(u, [u.link_names.index("ee_link")], pool, int(s[0].generate_state(1)[0]))
+    if "ik7" in kinds:
+        pools["ik7"] = ik_batch(p, [p.link_names.index("link7")], pool, int(s[1].generate_state(1)[0]))
+    if "ik29" in kinds:
+        pools["ik29"] = ik_batch(h, humanoid_ee_links(h), pool, int(s[2].generate_state(1)[0]))
+    if "retarget" in kinds:
+        clip = config2(seed=int(s[3].generate_state(1)[0]) % 1000, frames=pool + 1)
+        prev = clip.q_traj[:-1] + np.random.default_rng(s[4]).normal(0, 0.02, clip.q_traj[:-1].shape)
+        pools["retarget"] = (clip, prev.astype(np.float32))
+    rng = np.random.default_rng(s[5])
+    n_req = int(rate * seconds * 1.2) + 16
+    t = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    t = t[t < seconds]
+    m = len(t)
+    kind = rng.integers(0, len(kinds), m)
+    n = np.floor(np.exp(rng.uniform(0, np.log(257), m))).astype(np.int64).clip(1, 256)
+    off = rng.integers(0, pool - 256, m)
+    return ServeTrace(list(kinds), pools, t, kind, n, off)
