@@ -279,3 +279,106 @@ def ik_terms(links: Sequence[int], targets: np.ndarray, na: int, limit_weight: f
     k = len(links)
     w = np.tile(np.array([pos_w] * 3 + [ori_w] * 3, np.float32), k)
     return [Term(FRAME, links, targets, w), Term(LIMIT, (), None, np.full(na, limit_weight, np.float32))]
+
+
+# ----------------------------------------------------------------------------- trajectories
+class _Traj(C.Structure):
+    _fields_ = [("T", C.c_int), ("goal_link", C.c_int), ("goal_targets", C.c_void_p), ("goal_w", C.c_void_p),
+                ("wv", C.c_float), ("wa", C.c_float), ("dt", C.c_float), ("nsph", C.c_int),
+                ("sph_link", C.c_void_p), ("sph_c", C.c_void_p), ("sph_r", C.c_void_p), ("coll_w", C.c_float),
+                ("margin", C.c_float), ("nobs_s", C.c_int), ("obs_s", C.c_void_p), ("nobs_b", C.c_int),
+                ("obs_b", C.c_void_p)]
+
+
+class OracleTraj:
+    """The C trajectory description of a workloads.TrajBatch (keeps its arrays alive)."""
+
+    def __init__(self, tb, goal_w=None, wv=None, wa=None, coll_w=None, margin=None):
+        sc = tb.scene
+        self.keep = [
+            np.ascontiguousarray(tb.goals, np.float32),
+            np.ascontiguousarray(np.ones(6) if goal_w is None else goal_w, np.float32),
+            np.ascontiguousarray(sc.sphere_links, np.int32),
+            np.ascontiguousarray(sc.sphere_centers, np.float32),
+            np.ascontiguousarray(sc.sphere_radii, np.float32),
+            np.ascontiguousarray(sc.obstacle_spheres, np.float32).reshape(-1, 4),
+            np.ascontiguousarray(sc.obstacle_boxes, np.float32).reshape(-1, 6),
+        ]
+        k = self.keep
+        self.struct = _Traj(tb.T, tb.goal_link, _ptr(k[0]), _ptr(k[1]), tb.wv if wv is None else wv,
+                            tb.wa if wa is None else wa, tb.dt, len(k[2]), _ptr(k[2]), _ptr(k[3]), _ptr(k[4]),
+                            tb.coll_w if coll_w is None else coll_w, tb.margin if margin is None else margin,
+                            len(k[5]), _ptr(k[5]), len(k[6]), _ptr(k[6]))
+        self.tb = tb
+
+    def rows(self):
+        tb, na = self.tb, self.tb.spec.num_actuated_joints
+        s = self.struct
+        return 6 + (tb.T - 1) * na * (s.wv != 0) + (tb.T - 1) * na * (s.wa != 0) + tb.T * s.nsph
+
+
+def traj_residual_jacobian(ot: OracleTraj, b: int, q: np.ndarray, dtype: str = "f32", jac: bool = True):
+    """r [R], dense J [R, T*na] (or None), H, g, cost_terms [4] of problem b at the trajectory q [T, na]."""
+    lib, rt = _lib(dtype), _real(dtype)
+    rb = OracleRobot(ot.tb.spec)
+    D = ot.tb.T * ot.tb.spec.num_actuated_joints
+    R = ot.rows()
+    q = np.ascontiguousarray(q, rt).reshape(-1)
+    r = np.zeros(R, rt)
+    J = np.zeros((R, D), rt) if jac else None
+    H = np.zeros((D, D), rt)
+    g = np.zeros(D, rt)
+    ct = np.zeros(4, rt)
+    lib.ork_traj_residual_jacobian(C.byref(rb.struct), C.byref(ot.struct), int(b), _ptr(q), _ptr(r), _ptr(J), _ptr(H),
+                                   _ptr(g), _ptr(ct))
+    return r, J, H, g, ct
+
+
+def traj_lm_step(ot: OracleTraj, q: np.ndarray, lam: np.ndarray, params: LMParams, dtype: str = "f32", rows=None):
+    """One trajectory LM step per problem from q [B, T, na] and lam [B]; ``rows`` maps entries to the
+    problem whose goal they use."""
+    lib, rt = _lib(dtype), _real(dtype)
+    rb = OracleRobot(ot.tb.spec)
+    q = np.ascontiguousarray(q, rt)
+    B = q.shape[0]
+    ot2 = ot
+    if rows is not None:
+        import copy
+        tb = copy.copy(ot.tb)
+        tb.goals = np.asarray(ot.tb.goals)[rows]
+        ot2 = OracleTraj(tb, goal_w=ot.keep[1], wv=ot.struct.wv, wa=ot.struct.wa, coll_w=ot.struct.coll_w,
+                         margin=ot.struct.margin)
+    lam = np.ascontiguousarray(lam, rt)
+    out = {k: np.zeros(B, rt) for k in ("cost", "cost_prop", "pred", "lam_new")}
+    acc = np.zeros(B, np.int32)
+    qn = np.zeros_like(q)
+    p = params.struct()
+    lib.ork_traj_lm_step(C.byref(rb.struct), C.byref(ot2.struct), C.byref(p), B, _ptr(q), _ptr(lam), _ptr(out["cost"]),
+                         _ptr(out["cost_prop"]), _ptr(out["pred"]), _ptr(acc), _ptr(qn), _ptr(out["lam_new"]))
+    out.update(acc=acc, q_new=qn)
+    return out
+
+
+def traj_lm_solve(ot: OracleTraj, q_seed: np.ndarray, iters: int, params: LMParams, dtype: str = "f32"):
+    lib, rt = _lib(dtype), _real(dtype)
+    rb = OracleRobot(ot.tb.spec)
+    q = np.ascontiguousarray(q_seed, rt).copy()
+    B = q.shape[0]
+    cost = np.zeros(B, rt)
+    ct = np.zeros((B, iters + 1), rt)
+    at = np.zeros((B, iters), np.int32)
+    p = params.struct()
+    lib.ork_traj_lm_solve(C.byref(rb.struct), C.byref(ot.struct), C.byref(p), B, int(iters), _ptr(q), _ptr(cost),
+                          _ptr(ct), _ptr(at))
+    return dict(q=q, cost=cost, cost_trace=ct, accept=at)
+
+
+def traj_metrics(ot: OracleTraj, q: np.ndarray, dtype: str = "f32"):
+    """goal position / angle error at the last frame [B] and the smallest sphere-scene distance [B]"""
+    lib, rt = _lib(dtype), _real(dtype)
+    rb = OracleRobot(ot.tb.spec)
+    q = np.ascontiguousarray(q, rt)
+    B = q.shape[0]
+    pe, ae, sd = np.zeros(B, rt), np.zeros(B, rt), np.zeros(B, rt)
+    lib.ork_traj_metrics(C.byref(rb.struct), C.byref(ot.struct), B, _ptr(q), _ptr(pe), _ptr(ae), _ptr(sd))
+    return pe, ae, sd

@@ -663,7 +663,7 @@ static int damped_cholesky(int D, const real *H, const real *g, real lam, real *
             L[i * D + j] = v / d;
         }
     }
-    real y[256];
+    real *y = (real *)malloc(sizeof(real) * D);
     for (int i = 0; i < D; ++i) {
         real v = -g[i];
         for (int k = 0; k < i; ++k) v -= L[i * D + k] * y[k];
@@ -674,6 +674,7 @@ static int damped_cholesky(int D, const real *H, const real *g, real lam, real *
         for (int k = i + 1; k < D; ++k) v -= L[k * D + i] * delta[k];
         delta[i] = v / L[i * D + i];
     }
+    free(y);
     real p = 0;
     for (int i = 0; i < D; ++i) p += delta[i] * (lam * delta[i] - g[i]);
     *pred = (real)0.5 * p;
@@ -919,4 +920,342 @@ EXPORT void ork_ik_metrics(const ork_robot *rb, int B, const real *q, const real
     free(Tl);
     free(Xl);
     free(Tj);
+}
+
+
+/* ------------------------------------------------------------------------- */
+/* trajectory LM: T frames of one serial robot, joint-major per frame         */
+/* ------------------------------------------------------------------------- */
+/*
+ * Terms, in this row order (the reference's cost sums term costs in list order):
+ *   goal      TrajectoryTask(FrameTask) on the last frame (terms/trajectory_task.py:65-295,
+ *             frame_task.py:459-502): 6 log rows
+ *   velocity  TrajectorySmoothnessTask order 1 (terms/sparse/trajectory_smoothness_task.py:506-557):
+ *             rows (row, dof), row = 0..T-2, r = (w / dt) (q[row+1] - q[row])
+ *   accel.    TrajectorySmoothnessTask order 2: rows row = 0..T-2 with window start row-1; the first
+ *             window reaches before frame 0 and has weight history_valid * w = 0 (no history), the
+ *             others r = (w / dt^2) (q[s] - 2 q[s+1] + q[s+2])
+ *   collision TrajectoryCollisionTask, primitive scene, penalty "plain", no sweep
+ *             (terms/sparse/trajectory_collision_task.py:176-293, sphere_collision_kernels.py:19-42):
+ *             rows (frame, sphere): sd = scene SDF at the sphere centre - radius,
+ *             r = w (margin - sd) if sd < margin else 0; J = -w n^T (v + w x c) with n the SDF
+ *             gradient (closest-point direction, trajectory_collision_task.py:62-72)
+ * Scene SDF (geom/sdf_kernels.py:223-295): minimum over the obstacle spheres, then the
+ * axis-aligned boxes, the first strict minimum kept.
+ */
+typedef struct {
+    int T;
+    int goal_link;
+    const float *goal_targets; /* [B][7] */
+    const float *goal_w;       /* [6] */
+    float wv, wa, dt;          /* velocity / acceleration weights (0: term absent) */
+    int nsph;
+    const int32_t *sph_link;   /* [nsph] */
+    const float *sph_c;        /* [nsph][3] link-frame centres */
+    const float *sph_r;        /* [nsph] */
+    float coll_w, margin;
+    int nobs_s;
+    const float *obs_s;        /* [n][4] centre, radius */
+    int nobs_b;
+    const float *obs_b;        /* [n][6] centre, half extents */
+} ork_traj;
+
+static int traj_rows(const ork_robot *rb, const ork_traj *tj, int *Rg, int *Rv, int *Ra, int *Rc) {
+    *Rg = 6;
+    *Rv = tj->wv != 0.f ? (tj->T - 1) * rb->na : 0;
+    *Ra = tj->wa != 0.f ? (tj->T - 1) * rb->na : 0;
+    *Rc = tj->nsph > 0 ? tj->T * tj->nsph : 0;
+    return *Rg + *Rv + *Ra + *Rc;
+}
+
+/* scene SDF at p: signed distance and gradient direction */
+static real scene_sdf(const ork_traj *tj, const real *p, real *n) {
+    real best = (real)1e6;
+    n[0] = 1; n[1] = 0; n[2] = 0;
+    for (int i = 0; i < tj->nobs_s; ++i) {
+        const float *o = tj->obs_s + 4 * i;
+        real d[3] = {p[0] - (real)o[0], p[1] - (real)o[1], p[2] - (real)o[2]};
+        real len = RSQRT(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        real dist = len - (real)o[3];
+        if (dist < best) {
+            best = dist;
+            if (len > (real)1e-10) { n[0] = d[0] / len; n[1] = d[1] / len; n[2] = d[2] / len; }
+            else { n[0] = 1; n[1] = 0; n[2] = 0; }
+        }
+    }
+    for (int i = 0; i < tj->nobs_b; ++i) {
+        const float *o = tj->obs_b + 6 * i;
+        real ql[3], qv[3], h[3];
+        for (int k = 0; k < 3; ++k) {
+            ql[k] = p[k] - (real)o[k];
+            h[k] = (real)o[3 + k];
+            qv[k] = RFABS(ql[k]) - h[k];
+        }
+        real qmax = qv[0] > qv[1] ? qv[0] : qv[1];
+        if (qv[2] > qmax) qmax = qv[2];
+        real qp[3] = {qv[0] > 0 ? qv[0] : 0, qv[1] > 0 ? qv[1] : 0, qv[2] > 0 ? qv[2] : 0};
+        real dist = RSQRT(qp[0] * qp[0] + qp[1] * qp[1] + qp[2] * qp[2]) + (qmax < 0 ? qmax : 0);
+        if (dist < best) {
+            best = dist;
+            real nn[3] = {0, 0, 1};
+            if (qmax < 0) {
+                if (qv[0] >= qv[1] && qv[0] >= qv[2]) { nn[0] = ql[0] < 0 ? -1 : 1; nn[2] = 0; }
+                else if (qv[1] >= qv[2]) { nn[1] = ql[1] < 0 ? -1 : 1; nn[2] = 0; }
+                else { nn[2] = ql[2] < 0 ? -1 : 1; }
+            } else {
+                real df[3];
+                for (int k = 0; k < 3; ++k) {
+                    real c = ql[k] < -h[k] ? -h[k] : (ql[k] > h[k] ? h[k] : ql[k]);
+                    df[k] = ql[k] - c;
+                }
+                real dl = RSQRT(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
+                if (dl > (real)1e-10) { nn[0] = df[0] / dl; nn[1] = df[1] / dl; nn[2] = df[2] / dl; }
+            }
+            n[0] = nn[0]; n[1] = nn[1]; n[2] = nn[2];
+        }
+    }
+    return best;
+}
+
+/* sparse row: up to 7 columns (one frame) or 3 (smoothness) */
+typedef struct { int n; int col[8]; real val[8]; } traj_row;
+
+/*
+ * All rows of problem b at the trajectory q[T*na]: r[R], the rows' sparse Jacobian entries
+ * (optionally dense J[R][T*na]) and, accumulated row by row like the sparse path
+ * (terms/task.py:373-400, _sparse_normal_equations_kernel :469-488), H and g.
+ * cost_terms[4] = 1/2 sum of squares of the goal, velocity, acceleration, collision rows.
+ */
+EXPORT int ork_traj_residual_jacobian(const ork_robot *rb, const ork_traj *tj, int b, const real *q, real *r,
+                                      real *J, real *H, real *g, real *cost_terms) {
+    const int na = rb->na, T = tj->T, D = T * na;
+    int Rg, Rv, Ra, Rc;
+    const int R = traj_rows(rb, tj, &Rg, &Rv, &Ra, &Rc);
+    real *Xl = (real *)calloc(7 * (rb->nj + 1), sizeof(real));
+    real *Tj = (real *)calloc(7 * (rb->nj + 1), sizeof(real));
+    real *Tl = (real *)calloc(7 * (rb->nl + 1), sizeof(real));
+    real *S = (real *)calloc(6 * (rb->nj + 1), sizeof(real));
+    real *Jf = (real *)calloc(6 * na, sizeof(real));
+    if (J) memset(J, 0, sizeof(real) * (size_t)R * D);
+    if (H) memset(H, 0, sizeof(real) * (size_t)D * D);
+    if (g) memset(g, 0, sizeof(real) * D);
+    real ct[4] = {0, 0, 0, 0};
+    int row = 0;
+    traj_row rw;
+#define EMIT(term, rv)                                                                        \
+    do {                                                                                      \
+        r[row] = (rv);                                                                        \
+        ct[term] += r[row] * r[row];                                                          \
+        for (int u = 0; u < rw.n; ++u) {                                                      \
+            if (J) J[(size_t)row * D + rw.col[u]] = rw.val[u];                                \
+            if (g) g[rw.col[u]] = g[rw.col[u]] + rw.val[u] * r[row];                          \
+            if (H)                                                                            \
+                for (int v = 0; v < rw.n; ++v)                                                \
+                    H[(size_t)rw.col[u] * D + rw.col[v]] =                                    \
+                        H[(size_t)rw.col[u] * D + rw.col[v]] + rw.val[u] * rw.val[v];          \
+        }                                                                                     \
+        ++row;                                                                                \
+    } while (0)
+    /* goal: the FrameTask log rows of the last frame */
+    {
+        const real *qT = q + (size_t)(T - 1) * na;
+        memset(S, 0, sizeof(real) * 6 * rb->nj);
+        fk(rb, qT, IDENT7, Xl, Tj, Tl, S);
+        ork_term ft = {ORK_FRAME, 1, &tj->goal_link, tj->goal_targets, tj->goal_w, NULL};
+        real rg[6];
+        memset(Jf, 0, sizeof(real) * 6 * na);
+        term_eval(rb, &ft, b, qT, Tl, S, rg, Jf);
+        for (int i = 0; i < 6; ++i) {
+            rw.n = na;
+            for (int a = 0; a < na; ++a) { rw.col[a] = (T - 1) * na + a; rw.val[a] = Jf[i * na + a]; }
+            EMIT(0, rg[i]);
+        }
+    }
+    /* velocity rows */
+    if (Rv) {
+        const real w = (real)tj->wv / (real)tj->dt;
+        for (int t = 0; t + 1 < T; ++t)
+            for (int d = 0; d < na; ++d) {
+                rw.n = 2;
+                rw.col[0] = t * na + d; rw.val[0] = -w;
+                rw.col[1] = (t + 1) * na + d; rw.val[1] = w;
+                EMIT(1, w * (q[(t + 1) * na + d] - q[t * na + d]));
+            }
+    }
+    /* acceleration rows (window start s = row - 1; the first window has zero weight) */
+    if (Ra) {
+        const real w = (real)tj->wa / ((real)tj->dt * (real)tj->dt);
+        for (int rr = 0; rr + 1 < T; ++rr)
+            for (int d = 0; d < na; ++d) {
+                const int s0 = rr - 1;
+                if (s0 < 0) {
+                    rw.n = 0;
+                    EMIT(2, (real)0);
+                    continue;
+                }
+                rw.n = 3;
+                rw.col[0] = s0 * na + d; rw.val[0] = w;
+                rw.col[1] = (s0 + 1) * na + d; rw.val[1] = (real)-2 * w;
+                rw.col[2] = (s0 + 2) * na + d; rw.val[2] = w;
+                EMIT(2, w * (q[s0 * na + d] + (real)-2 * q[(s0 + 1) * na + d] + q[(s0 + 2) * na + d]));
+            }
+    }
+    /* collision rows */
+    if (Rc) {
+        for (int t = 0; t < T; ++t) {
+            const real *qt = q + (size_t)t * na;
+            memset(S, 0, sizeof(real) * 6 * rb->nj);
+            fk(rb, qt, IDENT7, Xl, Tj, Tl, S);
+            for (int k = 0; k < tj->nsph; ++k) {
+                const int link = tj->sph_link[k];
+                real lc[3] = {(real)tj->sph_c[3 * k], (real)tj->sph_c[3 * k + 1], (real)tj->sph_c[3 * k + 2]};
+                real c[3];
+                const real *Tk = Tl + 7 * link;
+                quat_apply(Tk + 3, lc, c);
+                for (int i = 0; i < 3; ++i) c[i] += Tk[i];
+                real n[3];
+                const real sd = scene_sdf(tj, c, n) - (real)tj->sph_r[k];
+                const real w = (real)tj->coll_w;
+                real res = 0;
+                rw.n = 0;
+                if (sd < (real)tj->margin) {
+                    res = w * ((real)tj->margin - sd);
+                    real cn[3];
+                    cross3(c, n, cn);
+                    rw.n = na;
+                    for (int a = 0; a < na; ++a) {
+                        real sc[6];
+                        spatial_column(rb, S, link, a, sc);
+                        rw.col[a] = t * na + a;
+                        rw.val[a] = -w * ((n[0] * sc[0] + n[1] * sc[1] + n[2] * sc[2]) +
+                                          (cn[0] * sc[3] + cn[1] * sc[4] + cn[2] * sc[5]));
+                    }
+                }
+                EMIT(3, res);
+            }
+        }
+    }
+#undef EMIT
+    for (int i = 0; i < 4; ++i) ct[i] = (real)0.5 * ct[i];
+    if (cost_terms) memcpy(cost_terms, ct, sizeof(ct));
+    free(Xl); free(Tj); free(Tl); free(S); free(Jf);
+    return R;
+}
+
+static real traj_cost(const ork_robot *rb, const ork_traj *tj, int b, const real *q, real *r) {
+    real ct[4];
+    ork_traj_residual_jacobian(rb, tj, b, q, r, NULL, NULL, NULL, ct);
+    return ((ct[0] + ct[1]) + ct[2]) + ct[3];
+}
+
+/* one LM step of trajectory problem b from (qb, lam); as the body of ork_lm_step */
+typedef struct { real *r, *H, *L, *g, *dl, *qp; } traj_ws;
+
+static void traj_step_one(const ork_robot *rb, const ork_traj *tj, const ork_lm_params *p, int b, const real *qb,
+                          real lam, traj_ws *w, real *cost_cur, real *cost_prop, real *pred_out, int32_t *acc_out,
+                          real *q_new, real *lam_new) {
+    const int D = tj->T * rb->na;
+    real ct[4];
+    ork_traj_residual_jacobian(rb, tj, b, qb, w->r, NULL, w->H, w->g, ct);
+    real cost = ((ct[0] + ct[1]) + ct[2]) + ct[3];
+    *cost_cur = cost;
+    real l = lam, pred;
+    damped_cholesky(D, w->H, w->g, l, w->L, w->dl, &pred);
+    for (int d = 0; d < D; ++d) w->qp[d] = qb[d] + w->dl[d];
+    real cp = traj_cost(rb, tj, b, w->qp, w->r);
+    *cost_prop = cp;
+    *pred_out = pred;
+    int32_t acc;
+    ork_accept(1, &cost, &cp, &pred, p, &l, &acc);
+    *acc_out = acc;
+    *lam_new = l;
+    for (int d = 0; d < D; ++d) q_new[d] = acc ? w->qp[d] : qb[d];
+}
+
+static void traj_ws_alloc(traj_ws *w, const ork_robot *rb, const ork_traj *tj) {
+    const int D = tj->T * rb->na;
+    int Rg, Rv, Ra, Rc;
+    const int R = traj_rows(rb, tj, &Rg, &Rv, &Ra, &Rc);
+    w->r = (real *)malloc(sizeof(real) * R);
+    w->H = (real *)malloc(sizeof(real) * (size_t)D * D);
+    w->L = (real *)malloc(sizeof(real) * (size_t)D * D);
+    w->g = (real *)malloc(sizeof(real) * D);
+    w->dl = (real *)malloc(sizeof(real) * D);
+    w->qp = (real *)malloc(sizeof(real) * D);
+}
+
+static void traj_ws_free(traj_ws *w) { free(w->r); free(w->H); free(w->L); free(w->g); free(w->dl); free(w->qp); }
+
+/* one LM step of each trajectory problem from (q[b], lam[b]); as ork_lm_step */
+EXPORT void ork_traj_lm_step(const ork_robot *rb, const ork_traj *tj, const ork_lm_params *p, int B, const real *q,
+                             const real *lam, real *cost_cur, real *cost_prop, real *pred_out, int32_t *acc_out,
+                             real *q_new, real *lam_new) {
+    const int D = tj->T * rb->na;
+    traj_ws w;
+    traj_ws_alloc(&w, rb, tj);
+    for (int b = 0; b < B; ++b)
+        traj_step_one(rb, tj, p, b, q + (size_t)b * D, lam[b], &w, cost_cur + b, cost_prop + b, pred_out + b,
+                      acc_out + b, q_new + (size_t)b * D, lam_new + b);
+    traj_ws_free(&w);
+}
+
+/* fixed-iteration trajectory LM (as ork_lm_solve); q [B][T*na] in/out */
+EXPORT void ork_traj_lm_solve(const ork_robot *rb, const ork_traj *tj, const ork_lm_params *p, int B, int iters,
+                              real *q, real *cost_out, real *cost_trace, int32_t *accept_trace) {
+    const int D = tj->T * rb->na;
+    traj_ws w;
+    traj_ws_alloc(&w, rb, tj);
+    real *qn = (real *)malloc(sizeof(real) * D);
+    for (int b = 0; b < B; ++b) {
+        real *qb = q + (size_t)b * D;
+        real lam = (real)p->lm_lambda, cost = traj_cost(rb, tj, b, qb, w.r), cp, pred, ln, c0;
+        int32_t acc;
+        if (cost_trace) cost_trace[(size_t)b * (iters + 1)] = cost;
+        for (int it = 0; it < iters; ++it) {
+            traj_step_one(rb, tj, p, b, qb, lam, &w, &c0, &cp, &pred, &acc, qn, &ln);
+            cost = acc ? cp : c0;
+            memcpy(qb, qn, sizeof(real) * D);
+            lam = ln;
+            if (cost_trace) cost_trace[(size_t)b * (iters + 1) + it + 1] = cost;
+            if (accept_trace) accept_trace[(size_t)b * iters + it] = acc;
+        }
+        if (cost_out) cost_out[b] = cost;
+    }
+    free(qn);
+    traj_ws_free(&w);
+}
+
+/* goal pose errors at the last frame and the smallest sphere-to-scene distance over all frames */
+EXPORT void ork_traj_metrics(const ork_robot *rb, const ork_traj *tj, int B, const real *q, real *pos_err,
+                             real *ang_err, real *min_sd) {
+    const int na = rb->na, T = tj->T;
+    real *Xl = (real *)calloc(7 * (rb->nj + 1), sizeof(real));
+    real *Tjt = (real *)calloc(7 * (rb->nj + 1), sizeof(real));
+    real *Tl = (real *)calloc(7 * (rb->nl + 1), sizeof(real));
+    for (int b = 0; b < B; ++b) {
+        const real *qb = q + (size_t)b * T * na;
+        real msd = (real)1e6;
+        for (int t = 0; t < T; ++t) {
+            fk(rb, qb + (size_t)t * na, IDENT7, Xl, Tjt, Tl, NULL);
+            for (int k = 0; k < tj->nsph; ++k) {
+                const real *Tk = Tl + 7 * tj->sph_link[k];
+                real lc[3] = {(real)tj->sph_c[3 * k], (real)tj->sph_c[3 * k + 1], (real)tj->sph_c[3 * k + 2]}, c[3], n[3];
+                quat_apply(Tk + 3, lc, c);
+                for (int i = 0; i < 3; ++i) c[i] += Tk[i];
+                real sd = scene_sdf(tj, c, n) - (real)tj->sph_r[k];
+                if (sd < msd) msd = sd;
+            }
+            if (t == T - 1) {
+                real inv[7], e[7], tg[7];
+                for (int k = 0; k < 7; ++k) tg[k] = (real)tj->goal_targets[(size_t)b * 7 + k];
+                se3_inverse(Tl + 7 * tj->goal_link, inv);
+                se3_compose(inv, tg, e);
+                pos_err[b] = RSQRT(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+                real v = RSQRT(e[4] * e[4] + e[5] * e[5] + e[6] * e[6]);
+                ang_err[b] = (real)2 * RASIN(v < (real)1 ? v : (real)1);
+            }
+        }
+        min_sd[b] = msd;
+    }
+    free(Xl); free(Tjt); free(Tl);
 }

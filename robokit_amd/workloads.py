@@ -240,3 +240,84 @@ def config4(seed: int = 0, rate: float = 2000.0, seconds: float = 60.0, pool: in
     n = np.floor(np.exp(rng.uniform(0, np.log(257), m))).astype(np.int64).clip(1, 256)
     off = rng.integers(0, pool - 256, m)
     return ServeTrace(list(kinds), pools, t, kind, n, off)
+
+
+@dataclass
+class TrajScene:
+    """Collision model of a serial arm and a primitive scene (world frame)."""
+
+    sphere_links: np.ndarray  # [m] int32, link of each robot sphere
+    sphere_centers: np.ndarray  # [m, 3] float32, in the link frame
+    sphere_radii: np.ndarray  # [m] float32
+    obstacle_spheres: np.ndarray  # [ns, 4] float32: centre, radius
+    obstacle_boxes: np.ndarray  # [nb, 6] float32: centre, half extents (axis-aligned)
+
+
+@dataclass
+class TrajBatch:
+    """Trajectory problems: seeds [B, T, na], goal pose of the last frame [B, 7]."""
+
+    spec: RobotSpec
+    goal_link: int
+    T: int
+    q_seed: np.ndarray
+    goals: np.ndarray
+    scene: TrajScene
+    wv: float = 1e-2
+    wa: float = 1e-2
+    dt: float = 1.0
+    coll_w: float = 20.0
+    margin: float = 0.02
+
+    @property
+    def size(self) -> int:
+        return self.q_seed.shape[0]
+
+
+def arm_spheres(spec: RobotSpec, per_link: int, rng: np.random.Generator, r_lo=0.03, r_hi=0.08):
+    """`per_link` spheres on every link of a serial arm, spaced along the segment to the next
+    joint's origin (the last link: 0.1 m along its z axis); radii uniform in [r_lo, r_hi]."""
+    nl = spec.num_links
+    pT = np.asarray(spec.parent_joint_transforms)
+    child = {int(spec.link_parent_joint_indices[l]): l for l in range(nl)}
+    links, centres = [], []
+    for l in range(nl):
+        j_in = int(spec.link_parent_joint_indices[l])
+        nxt = [j for j in range(spec.num_joints) if int(spec.parent_joint_indices[j]) == j_in]
+        seg = pT[nxt[0], :3] if nxt else np.array([0.0, 0.0, 0.1])
+        for i in range(per_link):
+            links.append(l)
+            centres.append(seg * (i + 0.5) / per_link)
+    radii = rng.uniform(r_lo, r_hi, len(links))
+    del child
+    return np.asarray(links, np.int32), np.asarray(centres, np.float32), radii.astype(np.float32)
+
+
+def config3(seed: int = 0, batch: int = 512, T: int = 64, n_obs_spheres: int = 32, n_obs_boxes: int = 8) -> TrajBatch:
+    """Collision-aware trajectory LM on the 7-DOF chain: 48 robot spheres (6 per link, radius
+    3-8 cm); 32 obstacle spheres (radius 5-15 cm) + 8 axis-aligned boxes (half extents 5-15 cm),
+    centres uniform in the 1.5 m cube [-0.75, 0.75]^2 x [0, 1.5] around the base, none within
+    0.3 m of the base's vertical axis below 0.5 m; seeds = straight joint-space lines between two
+    uniform in-limit configurations; goal = FK of a third."""
+    spec = panda_dh()
+    r_sph, r_obs, r_q = _streams(seed, 3)
+    links, centres, radii = arm_spheres(spec, 6, r_sph)
+
+    def centres_ok(n):
+        out = []
+        while len(out) < n:
+            c = r_obs.uniform([-0.75, -0.75, 0.0], [0.75, 0.75, 1.5])
+            if np.hypot(c[0], c[1]) < 0.3 and c[2] < 0.5:
+                continue
+            out.append(c)
+        return np.asarray(out)
+
+    obs_s = np.concatenate([centres_ok(n_obs_spheres), r_obs.uniform(0.05, 0.15, (n_obs_spheres, 1))], 1)
+    obs_b = np.concatenate([centres_ok(n_obs_boxes), r_obs.uniform(0.05, 0.15, (n_obs_boxes, 3))], 1)
+    scene = TrajScene(links, centres, radii, obs_s.astype(np.float32), obs_b.astype(np.float32))
+    qa, qb, qg = uniform_q(r_q, spec, batch), uniform_q(r_q, spec, batch), uniform_q(r_q, spec, batch)
+    s = np.linspace(0.0, 1.0, T, dtype=np.float32)[None, :, None]
+    q_seed = (qa[:, None] * (1 - s) + qb[:, None] * s).astype(np.float32)
+    goal_link = spec.num_links - 1
+    goals = forward_kinematics(spec, qg.astype(np.float64))[:, goal_link].astype(np.float32)
+    return TrajBatch(spec, goal_link, T, q_seed, goals, scene)
