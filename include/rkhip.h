@@ -176,6 +176,43 @@ int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t fl
 int rk_group_kernel_times(rk_group* g, int32_t cap, int32_t* tpp, float* ms, int32_t* batch_mask, int32_t* n);
 
 /*
+ * Collision-aware trajectory LM (BASELINE config 3 shape): T frames of a serial arm per problem,
+ * residuals in this row order:
+ *   goal       TrajectoryTask(FrameTask log) on the last frame (terms/trajectory_task.py:65-295)
+ *   velocity   TrajectorySmoothnessTask order 1, weight / dt (trajectory_smoothness_task.py:506-557)
+ *   accel.     TrajectorySmoothnessTask order 2, weight / dt^2 (first window: no history, weight 0)
+ *   collision  TrajectoryCollisionTask, primitive scene, penalty "plain", no sweep
+ *              (terms/sparse/trajectory_collision_task.py:176-293): w * max(0, margin - sd)
+ * Banded normal equations (half-bandwidth 2 * dofs) solved per problem
+ * (opt/lm_optimizer.py:81-136).  The batch then takes rk_batch_set (q [B][T][na], pose_targets =
+ * goal [B][7]), rk_batch_solve (RK_MODE_FUSED), rk_batch_get, rk_batch_advance.
+ * Limits: serial arms with 6 or 7 dofs, T <= 64, <= 64 robot spheres, <= 64 obstacle spheres,
+ * <= 16 axis-aligned obstacle boxes.
+ */
+typedef struct {
+    int32_t robot;
+    int32_t frames;
+    int32_t goal_link;
+    const float* goal_weights;    /* [6] (NULL: ones) */
+    float vel_weight, acc_weight, dt;
+    int32_t n_spheres;
+    const int32_t* sphere_links;  /* [n] */
+    const float* sphere_centers;  /* [n][3] link frame */
+    const float* sphere_radii;    /* [n] */
+    float coll_weight, coll_margin;
+    int32_t n_obs_spheres;
+    const float* obs_spheres;     /* [n][4] centre, radius */
+    int32_t n_obs_boxes;
+    const float* obs_boxes;       /* [n][6] centre, half extents */
+} rk_traj_desc;
+
+int rk_traj_batch_create(rk_context* ctx, const rk_traj_desc* desc, int32_t capacity, int32_t trace_iters,
+                         rk_batch** out);
+/* goal position / angle error at the last frame and the smallest sphere-to-scene signed distance
+ * over all frames and spheres, per problem [B] */
+int rk_traj_metrics(rk_batch* b, float* pos_err, float* ang_err, float* min_sd);
+
+/*
  * Request server: continuous batching of solves under a device-memory budget.  No reference
  * counterpart (the reference solves one caller's batch at a time); it serves many callers of the
  * LM path of opt/optimizer.py:216-240 concurrently.  A worker thread packs every queued request of

@@ -527,6 +527,7 @@ __global__ __launch_bounds__(64) void k_fused(Problems P, LMParams lp, int it0, 
 
 #include "rk_thread.h"
 #include "rk_wave.h"
+#include "rk_traj.h"
 
 // kernel variants: (threads per problem, data placement)
 //   V_S*/V_L*: G lanes per problem, LDS-staged (small / large LDS configuration)
@@ -1074,6 +1075,9 @@ struct rk_batch {
   Terms* td = nullptr;
   int capacity = 0, n = 0, trace_iters = 0, na = 0;
   int it_done = 0;  // iterations run since the last rk_batch_set / seeded solve (RK_CONTINUE offset)
+  int traj_T = 0;   // > 0: a trajectory batch (na = traj_T * robot dofs), solved by k_traj
+  int traj_na = 0;  // robot dofs of a trajectory batch
+  TrajTerms* tt = nullptr;
   float *q = nullptr, *q_seed = nullptr, *pose_tgt = nullptr, *pos_tgt = nullptr, *q_prev = nullptr;
   float *cost = nullptr, *lam = nullptr, *trace = nullptr;
   int *nacc = nullptr, *acc_trace = nullptr;
@@ -1237,6 +1241,17 @@ static int launch_variant(hipStream_t st, int v, const Problems& P, const LMPara
     case V_W40: hipLaunchKernelGGL((k_wave<4, 0>), dim3(P.n), dim3(WAVE), 0, st, P, lp, it0, iters); break;
     default: hipLaunchKernelGGL((k_wave<0, 12>), dim3(P.n), dim3(WAVE), 0, st, P, lp, it0, iters); break;
   }
+  HIP_TRY(hipGetLastError());
+  return RK_OK;
+}
+
+// trajectory kernel of the batch's arm (one wavefront per problem)
+static int launch_traj(rk_batch* b, const Problems& P, const LMParams& lp, int it0, int iters) {
+  hipStream_t st = b->ctx->stream;
+  if (b->traj_na == 6)
+    hipLaunchKernelGGL((k_traj<6, 8>), dim3(P.n), dim3(WAVE), 0, st, P, (const TrajTerms*)b->tt, lp, it0, iters);
+  else
+    hipLaunchKernelGGL((k_traj<7, 9>), dim3(P.n), dim3(WAVE), 0, st, P, (const TrajTerms*)b->tt, lp, it0, iters);
   HIP_TRY(hipGetLastError());
   return RK_OK;
 }
@@ -1722,13 +1737,133 @@ int rk_batch_destroy(rk_batch* b) {
   rk_context* ctx = b->ctx;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
-  void* ptrs[] = {b->td, b->q, b->q_seed, b->pose_tgt, b->pos_tgt, b->q_prev, b->cost, b->lam, b->nacc, b->trace,
+  void* ptrs[] = {b->tt, b->td, b->q, b->q_seed, b->pose_tgt, b->pos_tgt, b->q_prev, b->cost, b->lam, b->nacc, b->trace,
                   b->acc_trace, b->q_trace, b->lam_trace, b->pb.Tj, b->pb.Sw, b->pb.J, b->pb.r, b->pb.rp, b->pb.qp, b->pb.dq, b->pb.pred,
                   b->pb.acc};
   for (void* p : ptrs)
     if (p) hipFree(p);
   ctx->mem_bytes -= b->bytes;
   delete b;
+  return RK_OK;
+}
+
+int rk_traj_batch_create(rk_context* ctx, const rk_traj_desc* d, int32_t capacity, int32_t trace_iters,
+                         rk_batch** out) {
+  int rc;
+  if (!ctx || !d || !out) return fail(RK_EINVAL, "null argument");
+  if ((rc = check_robot(ctx, d->robot))) return rc;
+  if (capacity <= 0 || trace_iters < 0) return fail(RK_EINVAL, "capacity must be positive, trace_iters >= 0");
+  const Robot& r = ctx->robots[d->robot];
+  if (!r.chain || (r.na != 6 && r.na != 7) || r.nj > r.na + 2)
+    return fail(RK_EINVAL, "trajectory batches take serial arms with 6 or 7 dofs (<= 2 fixed tail joints)");
+  if (d->frames < 3 || d->frames > RK_TRAJ_MAXT) return fail(RK_EINVAL, "frames must be 3..64");
+  if (d->goal_link < 0 || d->goal_link >= r.nl) return fail(RK_EINVAL, "goal link out of range");
+  if (d->n_spheres < 0 || d->n_spheres > RK_TRAJ_MAXS || d->n_obs_spheres < 0 || d->n_obs_spheres > RK_SCENE_MAXS ||
+      d->n_obs_boxes < 0 || d->n_obs_boxes > RK_SCENE_MAXB)
+    return fail(RK_EINVAL, "at most 64 robot spheres, 64 obstacle spheres and 16 obstacle boxes");
+  if (!(d->dt > 0.f)) return fail(RK_EINVAL, "dt must be positive");
+  TrajTerms tt;
+  memset(&tt, 0, sizeof(tt));
+  tt.robot = d->robot;
+  tt.T = d->frames;
+  tt.goal_link = d->goal_link;
+  for (int i = 0; i < 6; ++i) tt.goal_w[i] = d->goal_weights ? d->goal_weights[i] : 1.f;
+  tt.wv = d->vel_weight / d->dt;
+  tt.wa = d->acc_weight / (d->dt * d->dt);
+  tt.coll_w = d->coll_weight;
+  tt.margin = d->coll_margin;
+  // spheres sorted by the (depth-renumbered) joint that moves their link, base link first
+  std::vector<int> order(d->n_spheres);
+  for (int k = 0; k < d->n_spheres; ++k) {
+    order[k] = k;
+    if (d->sphere_links[k] < 0 || d->sphere_links[k] >= r.nl) return fail(RK_EINVAL, "sphere link out of range");
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return r.link_joint[d->sphere_links[a]] < r.link_joint[d->sphere_links[b]]; });
+  tt.nsph = d->n_spheres;
+  for (int i = 0; i < d->n_spheres; ++i) {
+    const int k = order[i];
+    tt.sph_joint[i] = r.link_joint[d->sphere_links[k]];
+    for (int c = 0; c < 3; ++c) tt.sph_c[i][c] = d->sphere_centers[3 * k + c];
+    tt.sph_r[i] = d->sphere_radii[k];
+  }
+  tt.ns = d->n_obs_spheres;
+  tt.nb = d->n_obs_boxes;
+  for (int i = 0; i < tt.ns; ++i)
+    for (int c = 0; c < 4; ++c) tt.obs_s[i][c] = d->obs_spheres[4 * i + c];
+  for (int i = 0; i < tt.nb; ++i)
+    for (int c = 0; c < 6; ++c) tt.obs_b[i][c] = d->obs_boxes[6 * i + c];
+  // the batch: na = frames * dofs per problem (SoA [k][stride]), one 7-float goal per problem
+  Terms th;
+  memset(&th, 0, sizeof(th));
+  th.robot = d->robot;
+  th.n_pose = 1;
+  th.pose_link[0] = d->goal_link;
+  for (int k = 0; k < 6; ++k) th.pose_w[0][k] = tt.goal_w[k];
+  th.Rf = 6;
+  th.R = 6;
+  const int D = d->frames * r.na;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  rk_batch* b = new rk_batch();
+  b->ctx = ctx;
+  b->th = th;
+  b->capacity = capacity;
+  b->trace_iters = trace_iters;
+  b->na = D;
+  b->traj_T = d->frames;
+  b->traj_na = r.na;
+  const size_t S = capacity;
+  auto A = [&](void** p, size_t bytes) { return dmalloc(ctx, p, bytes, &b->bytes); };
+  if ((rc = A((void**)&b->tt, sizeof(TrajTerms))) || (rc = A((void**)&b->td, sizeof(Terms))) ||
+      (rc = A((void**)&b->q, sizeof(float) * D * S)) || (rc = A((void**)&b->q_seed, sizeof(float) * D * S)) ||
+      (rc = A((void**)&b->pose_tgt, sizeof(float) * 7 * S)) || (rc = A((void**)&b->pos_tgt, sizeof(float) * S)) ||
+      (rc = A((void**)&b->q_prev, sizeof(float) * D * S)) || (rc = A((void**)&b->cost, sizeof(float) * S)) ||
+      (rc = A((void**)&b->lam, sizeof(float) * S)) || (rc = A((void**)&b->nacc, sizeof(int) * S))) {
+    rk_batch_destroy(b);
+    return rc;
+  }
+  if (trace_iters > 0) {
+    if ((rc = A((void**)&b->trace, sizeof(float) * (trace_iters + 1) * S)) ||
+        (rc = A((void**)&b->acc_trace, sizeof(int) * trace_iters * S)) ||
+        (rc = A((void**)&b->q_trace, sizeof(float) * (size_t)trace_iters * D * S)) ||
+        (rc = A((void**)&b->lam_trace, sizeof(float) * trace_iters * S))) {
+      rk_batch_destroy(b);
+      return rc;
+    }
+  }
+  hipMemcpyAsync(b->tt, &tt, sizeof(TrajTerms), hipMemcpyHostToDevice, ctx->stream);
+  hipMemcpyAsync(b->td, &th, sizeof(Terms), hipMemcpyHostToDevice, ctx->stream);
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  *out = b;
+  return RK_OK;
+}
+
+int rk_traj_metrics(rk_batch* b, float* pos_err, float* ang_err, float* min_sd) {
+  if (!b || !pos_err || !ang_err || !min_sd) return fail(RK_EINVAL, "null argument");
+  if (!b->traj_T) return fail(RK_EINVAL, "not a trajectory batch");
+  rk_context* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int B = b->n;
+  float* dbuf = nullptr;
+  HIP_TRY(hipMalloc(&dbuf, sizeof(float) * 3 * B));
+  const Problems P = make_problems(b);
+  if (b->traj_na == 6)
+    hipLaunchKernelGGL((k_traj_metrics<6, 8>), dim3(B), dim3(WAVE), 0, ctx->stream, P, (const TrajTerms*)b->tt, dbuf,
+                       dbuf + B, dbuf + 2 * B);
+  else
+    hipLaunchKernelGGL((k_traj_metrics<7, 9>), dim3(B), dim3(WAVE), 0, ctx->stream, P, (const TrajTerms*)b->tt, dbuf,
+                       dbuf + B, dbuf + 2 * B);
+  hipError_t e = hipGetLastError();
+  std::vector<float> h(3 * B);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dbuf, sizeof(float) * 3 * B, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(dbuf);
+  if (e != hipSuccess) return fail(RK_EHIP, std::string("rk_traj_metrics: ") + hipGetErrorString(e));
+  memcpy(pos_err, h.data(), sizeof(float) * B);
+  memcpy(ang_err, h.data() + B, sizeof(float) * B);
+  memcpy(min_sd, h.data() + 2 * B, sizeof(float) * B);
   return RK_OK;
 }
 
@@ -1805,7 +1940,10 @@ int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mo
   if (cont && from_seed) return fail(RK_EINVAL, "RK_CONTINUE and RK_FROM_SEED exclude each other");
   const int it0 = cont ? b->it_done : 0;
   if (b->trace && it0 + iters > b->trace_iters) return fail(RK_EINVAL, "iterations exceed the batch's trace length");
-  if (iters == 0 && (mode == RK_MODE_PLAIN || mode == RK_MODE_FUSED)) {
+  if (b->traj_T) {
+    if (mode != RK_MODE_FUSED) return fail(RK_EINVAL, "trajectory batches run the fused kernel only");
+    rc = launch_traj(b, make_problems(b, from_seed), lp, it0, iters);
+  } else if (iters == 0 && (mode == RK_MODE_PLAIN || mode == RK_MODE_FUSED)) {
     rc = eval_cost(b, from_seed);
   } else if (mode == RK_MODE_PLAIN) {
     rc = solve_plain(b, it0, iters, lp, from_seed);
@@ -1878,6 +2016,7 @@ int rk_batch_get_iterates(rk_batch* b, float* q_trace, float* lam_trace) {
 int rk_batch_metrics(rk_batch* b, float pos_thr, float ang_thr, float* pos_err, float* ang_err, int32_t* success) {
   if (!b || !pos_err || !ang_err || !success) return fail(RK_EINVAL, "null argument");
   if (b->th.n_pose == 0) return fail(RK_EINVAL, "metrics need pose targets");
+  if (b->traj_T) return fail(RK_EINVAL, "trajectory batches report rk_traj_metrics");
   rk_context* ctx = b->ctx;
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device));
@@ -1907,6 +2046,8 @@ int rk_group_create(rk_context* ctx, int32_t nb, rk_batch* const* batches, rk_gr
   if (!ctx || nb <= 0 || !batches || !out) return fail(RK_EINVAL, "bad argument to rk_group_create");
   for (int i = 0; i < nb; ++i)
     if (!batches[i] || batches[i]->ctx != ctx) return fail(RK_EINVAL, "batch of another context or null");
+  for (int i = 0; i < nb; ++i)
+    if (batches[i]->traj_T) return fail(RK_EINVAL, "trajectory batches are solved on their own (rk_batch_solve)");
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device));
   rk_group* g = new rk_group();

@@ -1,0 +1,460 @@
+// rk_traj.h -- collision-aware trajectory LM, one wavefront per problem (included by rk_lm.hip
+// inside namespace rk, after rk_thread.h).
+//
+// A problem is a trajectory of T <= 64 frames of a serial arm with NA dofs: D = T*NA variables,
+// frame-major (index t*NA + d).  Lane t of the wave owns frame t: its FK, its collision spheres
+// against the primitive scene, the goal pose on the last frame, its finite-difference rows, and
+// the NA x NA diagonal block of J^T J and the frame's part of J^T r, all in registers.  J^T J is
+// banded (half-bandwidth W = 2*NA: acceleration rows couple frames t and t+2): the band lives in
+// LDS, [D][W+1], and is factorised in place by the right-looking band Cholesky, the W(W+1)/2
+// updates of each column spread over the lanes; the two triangular solves follow the same
+// pattern.  Rows and term costs in this order: goal, velocity, acceleration, collision (the
+// reference sums term costs in list order).
+
+#define RK_TRAJ_MAXT 64
+#define RK_TRAJ_MAXS 64  // robot collision spheres
+#define RK_SCENE_MAXS 64  // obstacle spheres
+#define RK_SCENE_MAXB 16  // obstacle boxes (axis-aligned)
+
+struct TrajTerms {
+  int robot, T;
+  int goal_link;
+  float goal_w[6];
+  float wv, wa;  // finite-difference row weights already divided by dt, dt^2 (0: term absent)
+  float coll_w, margin;
+  int nsph;      // robot spheres, sorted by the joint that moves their link (-1: base link first)
+  int sph_joint[RK_TRAJ_MAXS];
+  float sph_c[RK_TRAJ_MAXS][3];
+  float sph_r[RK_TRAJ_MAXS];
+  int ns, nb;
+  float obs_s[RK_SCENE_MAXS][4];
+  float obs_b[RK_SCENE_MAXB][6];
+};
+typedef const __attribute__((address_space(4))) TrajTerms TrajK;
+
+// scene signed distance at p (minimum over obstacle spheres, then boxes; first strict minimum)
+// and its gradient direction n (geom/sdf_kernels.py:256-295, trajectory_collision_task.py:62-72)
+__device__ __forceinline__ float scene_sdf(TrajK* tt, const float* p, float* n) {
+  float best = 1e6f;
+  n[0] = 1.f;
+  n[1] = n[2] = 0.f;
+  for (int i = 0; i < tt->ns; ++i) {
+    const float d0 = p[0] - tt->obs_s[i][0], d1 = p[1] - tt->obs_s[i][1], d2 = p[2] - tt->obs_s[i][2];
+    const float len = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+    const float dist = len - tt->obs_s[i][3];
+    if (dist < best) {
+      best = dist;
+      if (len > 1e-10f) {
+        n[0] = d0 / len; n[1] = d1 / len; n[2] = d2 / len;
+      } else {
+        n[0] = 1.f; n[1] = n[2] = 0.f;
+      }
+    }
+  }
+  for (int i = 0; i < tt->nb; ++i) {
+    float ql[3], qv[3], h[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      ql[k] = p[k] - tt->obs_b[i][k];
+      h[k] = tt->obs_b[i][3 + k];
+      qv[k] = fabsf(ql[k]) - h[k];
+    }
+    float qmax = qv[0] > qv[1] ? qv[0] : qv[1];
+    if (qv[2] > qmax) qmax = qv[2];
+    const float p0 = qv[0] > 0.f ? qv[0] : 0.f, p1 = qv[1] > 0.f ? qv[1] : 0.f, p2 = qv[2] > 0.f ? qv[2] : 0.f;
+    const float dist = sqrtf(p0 * p0 + p1 * p1 + p2 * p2) + (qmax < 0.f ? qmax : 0.f);
+    if (dist < best) {
+      best = dist;
+      float nn[3] = {0.f, 0.f, 1.f};
+      if (qmax < 0.f) {
+        if (qv[0] >= qv[1] && qv[0] >= qv[2]) {
+          nn[0] = ql[0] < 0.f ? -1.f : 1.f;
+          nn[2] = 0.f;
+        } else if (qv[1] >= qv[2]) {
+          nn[1] = ql[1] < 0.f ? -1.f : 1.f;
+          nn[2] = 0.f;
+        } else {
+          nn[2] = ql[2] < 0.f ? -1.f : 1.f;
+        }
+      } else {
+        float df[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const float c = ql[k] < -h[k] ? -h[k] : (ql[k] > h[k] ? h[k] : ql[k]);
+          df[k] = ql[k] - c;
+        }
+        const float dl = sqrtf(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
+        if (dl > 1e-10f) {
+          nn[0] = df[0] / dl; nn[1] = df[1] / dl; nn[2] = df[2] / dl;
+        }
+      }
+      n[0] = nn[0]; n[1] = nn[1]; n[2] = nn[2];
+    }
+  }
+  return best;
+}
+
+// per-frame evaluation: FK streamed joint by joint with the spheres of each link processed as
+// its pose appears; JAC also accumulates the frame's block of J^T J (lower, hb[a][b], b <= a)
+// and J^T r (gf) of the goal and collision rows.  Returns the half sums of squares of the goal
+// rows (lane T-1 only) in cg and of the collision rows in cc.
+template <int NA, int NJ, bool JAC>
+__device__ __forceinline__ void traj_frame(RobotK* rb, TrajK* tt, const float (&q)[NA], bool last, const Pose& goal,
+                                           float (&hb)[NA][NA], float (&gf)[NA], float& cg, float& cc) {
+  const int nj = rb->nj;
+  float S6[NA][6];
+  Pose T = identity_pose(), Tg = identity_pose();
+  const int gj = rb->link_joint[tt->goal_link];
+  const float w = tt->coll_w, margin = tt->margin;
+  int k = 0;
+  auto spheres = [&](int joint, const Pose& P, int ndof) {
+    for (; k < tt->nsph && tt->sph_joint[k] == joint; ++k) {
+      const float lc[3] = {tt->sph_c[k][0], tt->sph_c[k][1], tt->sph_c[k][2]};
+      float c[3];
+      quat_apply(P.q, lc, c);
+      c[0] += P.t[0];
+      c[1] += P.t[1];
+      c[2] += P.t[2];
+      float n[3];
+      const float sd = scene_sdf(tt, c, n) - tt->sph_r[k];
+      if (sd < margin) {
+        const float res = w * (margin - sd);
+        cc += res * res;
+        if (JAC) {
+          const float cn0 = c[1] * n[2] - c[2] * n[1], cn1 = c[2] * n[0] - c[0] * n[2], cn2 = c[0] * n[1] - c[1] * n[0];
+          float jr[NA];
+#pragma unroll
+          for (int a = 0; a < NA; ++a)
+            jr[a] = a < ndof ? -w * ((n[0] * S6[a][0] + n[1] * S6[a][1] + n[2] * S6[a][2]) +
+                                     (cn0 * S6[a][3] + cn1 * S6[a][4] + cn2 * S6[a][5]))
+                             : 0.f;
+#pragma unroll
+          for (int a = 0; a < NA; ++a) {
+#pragma unroll
+            for (int b2 = 0; b2 <= a; ++b2) hb[a][b2] += jr[a] * jr[b2];
+            gf[a] += jr[a] * res;
+          }
+        }
+      }
+    }
+  };
+  spheres(-1, T, 0);
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    if (j < nj) {
+      Pose L;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) L.t[c] = rb->pT[j][c];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) L.q[c] = rb->pT[j][3 + c];
+      if (j < NA) {
+        Pose X = identity_pose();
+        const float ax0 = rb->axis[j][0], ax1 = rb->axis[j][1], ax2 = rb->axis[j][2];
+        if (rb->joint_type[j] == 2) {
+          float s, c;
+          sincosf(q[j] * 0.5f, &s, &c);
+          X.q[0] = c; X.q[1] = ax0 * s; X.q[2] = ax1 * s; X.q[3] = ax2 * s;
+        } else {
+          X.t[0] = ax0 * q[j]; X.t[1] = ax1 * q[j]; X.t[2] = ax2 * q[j];
+        }
+        L = compose(L, X);
+      }
+      T = j == 0 ? L : compose(T, L);
+      if (JAC && j < NA) subspace_column(rb, j, T, S6[j]);
+      if (j == gj) Tg = T;
+      spheres(j, T, j < NA ? j + 1 : NA);
+    }
+  }
+  // goal: the FrameTask log rows of the last frame (every lane evaluates; lane T-1 keeps them)
+  float gw[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) gw[i] = tt->goal_w[i];
+  float r6[6];
+  pose_target(Tg, goal, gw, r6);
+  float sg = 0.f;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) sg += r6[i] * r6[i];
+  cg = last ? sg : 0.f;
+  if (JAC && last) {
+    float tg[36];
+    pose_blocks(Tg, goal, gw, tg);
+    float J6[6][NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      float c6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (a <= gj) pose_jac_column(tg, S6[a], c6);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) J6[i][a] = c6[i];
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+#pragma unroll
+      for (int b2 = 0; b2 <= a; ++b2) {
+        float sp = 0.f;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) sp += J6[i][a] * J6[i][b2];
+        hb[a][b2] += sp;
+      }
+      float sp = 0.f;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) sp += J6[i][a] * r6[i];
+      gf[a] += sp;
+    }
+  }
+}
+
+template <int NA>
+struct TrajLDS {
+  static constexpr int W = 2 * NA;  // half-bandwidth
+  float band[RK_TRAJ_MAXT * NA][W + 1];
+  float rhs[RK_TRAJ_MAXT * NA];
+  float dinv[RK_TRAJ_MAXT * NA];
+};
+
+// finite-difference rows of lane t (velocity row t, acceleration window starting at t) and their
+// J^T r contribution to frame t; q of frames t-2..t+2 come from the neighbouring lanes
+template <int NA, bool JAC>
+__device__ __forceinline__ void traj_smooth(TrajK* tt, int t, int T, const float (&q)[NA], float (&gf)[NA], float& cv,
+                                            float& ca) {
+  const float wv = tt->wv, wa = tt->wa;
+  const int l1 = t + 1 < 64 ? t + 1 : 63, l2 = t + 2 < 64 ? t + 2 : 63;
+  const bool has_v = t + 1 < T, has_a = t + 2 < T;
+  float v[NA], ac[NA];
+#pragma unroll
+  for (int d = 0; d < NA; ++d) {
+    const float q1 = __shfl(q[d], l1), q2 = __shfl(q[d], l2);
+    v[d] = has_v ? wv * (q1 - q[d]) : 0.f;
+    ac[d] = has_a ? wa * (q[d] + -2.f * q1 + q2) : 0.f;
+    cv += v[d] * v[d];
+    ca += ac[d] * ac[d];
+  }
+  if (JAC) {
+    const int m1 = t >= 1 ? t - 1 : 0, m2 = t >= 2 ? t - 2 : 0;
+#pragma unroll
+    for (int d = 0; d < NA; ++d) {
+      // every lane takes part in each shuffle (a lane reading an inactive lane gets nothing)
+      const float vs1 = __shfl(v[d], m1), as1 = __shfl(ac[d], m1), as2 = __shfl(ac[d], m2);
+      const float vm1 = t >= 1 ? vs1 : 0.f;
+      const float am1 = t >= 1 ? as1 : 0.f;
+      const float am2 = t >= 2 ? as2 : 0.f;
+      gf[d] += (-wv * v[d] + wv * vm1) + (wa * ac[d] + -2.f * wa * am1 + wa * am2);
+    }
+  }
+}
+
+template <int NA, int NJ>
+__device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s, const LMParams& lp, int it0, int iters,
+                                           TrajLDS<NA>* sh) {
+  constexpr int W = TrajLDS<NA>::W;
+  RobotK* rb = as_const(Pg.robot);
+  const int lane = threadIdx.x & 63;
+  const int T = tt->T, D = T * NA, S = Pg.stride;
+  const int t = lane < T ? lane : T - 1;
+  const bool own = lane < T;  // lanes >= T mirror frame T-1 and contribute nothing
+  const bool last = lane == T - 1;
+  float q[NA];
+#pragma unroll
+  for (int d = 0; d < NA; ++d) {
+    const int k = t * NA + d;
+    q[d] = (it0 == 0 && Pg.q_seed) ? Pg.q_seed[k * S + s] : Pg.q[k * S + s];
+  }
+  Pose goal;
+  {
+    float v[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) v[c] = Pg.pose_tgt[c * S + s];
+    load_pose(v, goal);
+  }
+  float lam = it0 == 0 ? lp.lm_lambda : Pg.lam[s];
+  int nacc = it0 == 0 ? 0 : Pg.nacc[s];
+  // smoothness band of J^T J is constant: entries (t,d)-(t,d), (t+1,d)-(t,d), (t+2,d)-(t,d)
+  const float wv2 = tt->wv * tt->wv, wa2 = tt->wa * tt->wa;
+  auto in_a = [&](int st) { return st >= 0 && st <= T - 3; };
+  auto in_v = [&](int st) { return st >= 0 && st <= T - 2; };
+  const float hs0 = wv2 * (float)(in_v(t) + in_v(t - 1)) + wa2 * (1.f * in_a(t) + 4.f * in_a(t - 1) + 1.f * in_a(t - 2));
+  const float hs1m = wv2 * (in_v(t - 1) ? -1.f : 0.f) + wa2 * (-2.f * in_a(t - 1) - 2.f * in_a(t - 2));  // (t, t-1)
+  const float hs2m = wa2 * (float)in_a(t - 2);                                                            // (t, t-2)
+  // lane p's (a, b) pairs of the band update, 1 <= b <= a <= W
+  int pa[2] = {0, 0}, pbv[2] = {0, 0};
+  {
+    int idx = 0;
+    for (int a = 1; a <= W; ++a)
+      for (int b2 = 1; b2 <= a; ++b2, ++idx) {
+        if (idx == lane) { pa[0] = a; pbv[0] = b2; }
+        if (idx == lane + 64) { pa[1] = a; pbv[1] = b2; }
+      }
+  }
+  auto eval_cost = [&](const float (&qq)[NA]) {
+    float hb[NA][NA], gf[NA];
+    float cg = 0.f, cc = 0.f, cv = 0.f, ca = 0.f;
+    traj_frame<NA, NJ, false>(rb, tt, qq, last, goal, hb, gf, cg, cc);
+    traj_smooth<NA, false>(tt, t, T, qq, gf, cv, ca);
+    if (!own) cg = cc = cv = ca = 0.f;
+    const float c0 = 0.5f * wv_sum(cg), c1 = 0.5f * wv_sum(cv), c2 = 0.5f * wv_sum(ca), c3 = 0.5f * wv_sum(cc);
+    return ((c0 + c1) + c2) + c3;
+  };
+  float cost = 0.f;
+  if (iters == 0) cost = eval_cost(q);
+  for (int it = 0; it < iters; ++it) {
+    const int gi = it0 + it;
+    if (Pg.q_trace && gi < Pg.trace_iters) {
+      if (own)
+#pragma unroll
+        for (int d = 0; d < NA; ++d) Pg.q_trace[((size_t)gi * D + t * NA + d) * S + s] = q[d];
+      if (lane == 0) Pg.lam_trace[gi * S + s] = lam;
+    }
+    // ---- rows, block of J^T J and J^T r of this frame
+    float hb[NA][NA], gf[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      gf[a] = 0.f;
+#pragma unroll
+      for (int b2 = 0; b2 < NA; ++b2) hb[a][b2] = 0.f;
+    }
+    float cg = 0.f, cc = 0.f, cv = 0.f, ca = 0.f;
+    traj_frame<NA, NJ, true>(rb, tt, q, last, goal, hb, gf, cg, cc);
+    traj_smooth<NA, true>(tt, t, T, q, gf, cv, ca);
+    if (!own) cg = cc = cv = ca = 0.f;
+    cost = ((0.5f * wv_sum(cg) + 0.5f * wv_sum(cv)) + 0.5f * wv_sum(ca)) + 0.5f * wv_sum(cc);
+    // ---- band of J^T J + lam I (rows of frame t), right-hand side J^T r
+    if (own) {
+#pragma unroll
+      for (int i = 0; i < NA; ++i) {
+        const int row = t * NA + i;
+#pragma unroll
+        for (int kk = 0; kk <= W; ++kk) {
+          float v = 0.f;
+          if (kk <= i) v = hb[i][i - kk];
+          if (kk == 0) v = v + hs0 + lam;
+          if (kk == NA) v = hs1m;
+          if (kk == 2 * NA) v = hs2m;
+          sh->band[row][kk] = v;
+        }
+        sh->rhs[row] = -gf[i];
+      }
+    }
+    __syncthreads();
+    // ---- band Cholesky, column by column
+    for (int j = 0; j < D; ++j) {
+      const float d = sqrtf(sh->band[j][0]);
+      const float inv = 1.f / d;
+      float Lv = 0.f;
+      if (lane >= 1 && lane <= W && j + lane < D) {
+        Lv = sh->band[j + lane][lane] * inv;
+      }
+      __syncthreads();
+      if (lane >= 1 && lane <= W && j + lane < D) sh->band[j + lane][lane] = Lv;
+      if (lane == 0) {
+        sh->band[j][0] = d;
+        sh->dinv[j] = inv;
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int a = pa[u], b2 = pbv[u];
+        const float La = __shfl(Lv, a), Lb = __shfl(Lv, b2);
+        if (a > 0 && j + a < D) sh->band[j + a][a - b2] -= La * Lb;
+      }
+      __syncthreads();
+    }
+    // ---- L y = -g, then L^T delta = y (in place in rhs)
+    for (int j = 0; j < D; ++j) {
+      const float y = sh->rhs[j] * sh->dinv[j];
+      __syncthreads();
+      if (lane == 0) sh->rhs[j] = y;
+      if (lane >= 1 && lane <= W && j + lane < D) sh->rhs[j + lane] -= sh->band[j + lane][lane] * y;
+      __syncthreads();
+    }
+    for (int j = D - 1; j >= 0; --j) {
+      const float x = sh->rhs[j] * sh->dinv[j];
+      __syncthreads();
+      if (lane == 0) sh->rhs[j] = x;
+      if (lane >= 1 && lane <= W && j - lane >= 0) sh->rhs[j - lane] -= sh->band[j][lane] * x;
+      __syncthreads();
+    }
+    float dq[NA], qp[NA];
+    float pr = 0.f;
+#pragma unroll
+    for (int d = 0; d < NA; ++d) {
+      dq[d] = sh->rhs[t * NA + d];
+      qp[d] = q[d] + dq[d];
+      pr += dq[d] * (lam * dq[d] - gf[d]);
+    }
+    const float pred = 0.5f * wv_sum(own ? pr : 0.f);
+    const float cp = eval_cost(qp);
+    if (Pg.trace && gi == 0 && lane == 0) Pg.trace[s] = cost;
+    const bool acc = lm_accept(lp, cost, cp, pred, lam);
+    if (acc) {
+      ++nacc;
+#pragma unroll
+      for (int d = 0; d < NA; ++d) q[d] = qp[d];
+    }
+    if (Pg.trace && gi < Pg.trace_iters && lane == 0) {
+      Pg.trace[(gi + 1) * S + s] = cost;
+      Pg.acc_trace[gi * S + s] = acc;
+    }
+    __syncthreads();
+  }
+  if (own)
+#pragma unroll
+    for (int d = 0; d < NA; ++d) Pg.q[(t * NA + d) * S + s] = q[d];
+  if (lane == 0) {
+    Pg.cost[s] = cost;
+    Pg.lam[s] = lam;
+    Pg.nacc[s] = nacc;
+  }
+}
+
+template <int NA, int NJ>
+__global__ __launch_bounds__(64) void k_traj(Problems P, const TrajTerms* tt, LMParams lp, int it0, int iters) {
+  __shared__ __align__(16) TrajLDS<NA> sh;
+  if ((int)blockIdx.x >= P.n) return;
+  traj_solve<NA, NJ>(P, as_const(tt), blockIdx.x, lp, it0, iters, &sh);
+}
+
+// goal errors at the last frame and the smallest sphere-to-scene distance (one wave per problem)
+template <int NA, int NJ>
+__global__ __launch_bounds__(64) void k_traj_metrics(Problems P, const TrajTerms* ttg, float* pos_err, float* ang_err,
+                                                     float* min_sd) {
+  const int s = blockIdx.x, lane = threadIdx.x & 63;
+  if (s >= P.n) return;
+  TrajK* tt = as_const(ttg);
+  RobotK* rb = as_const(P.robot);
+  const int T = tt->T, S = P.stride, t = lane < T ? lane : T - 1;
+  float q[NA];
+#pragma unroll
+  for (int d = 0; d < NA; ++d) q[d] = P.q[(t * NA + d) * S + s];
+  float msd = 1e6f;
+  Pose Tt = identity_pose(), Tg = identity_pose();
+  const int gj = rb->link_joint[tt->goal_link];
+  int k = 0;
+  auto spheres = [&](int joint, const Pose& Pw) {
+    for (; k < tt->nsph && tt->sph_joint[k] == joint; ++k) {
+      const float lc[3] = {tt->sph_c[k][0], tt->sph_c[k][1], tt->sph_c[k][2]};
+      float c[3], n[3];
+      quat_apply(Pw.q, lc, c);
+      c[0] += Pw.t[0];
+      c[1] += Pw.t[1];
+      c[2] += Pw.t[2];
+      msd = fminf(msd, scene_sdf(tt, c, n) - tt->sph_r[k]);
+    }
+  };
+  spheres(-1, Tt);
+  for (int j = 0; j < rb->nj && j < NJ; ++j) {
+    Pose L = joint_local(rb, j, q);
+    Tt = j == 0 ? L : compose(Tt, L);
+    if (j == gj) Tg = Tt;
+    spheres(j, Tt);
+  }
+  for (int o = 32; o > 0; o >>= 1) msd = fminf(msd, __shfl_xor(msd, o));
+  if (lane == 0) min_sd[s] = msd;
+  if (lane == T - 1) {
+    float v[7];
+    for (int c = 0; c < 7; ++c) v[c] = P.pose_tgt[c * S + s];
+    Pose goal;
+    load_pose(v, goal);
+    const Pose E = compose(inverse(Tg), goal);
+    pos_err[s] = sqrtf(E.t[0] * E.t[0] + E.t[1] * E.t[1] + E.t[2] * E.t[2]);
+    const float vv = sqrtf(E.q[1] * E.q[1] + E.q[2] * E.q[2] + E.q[3] * E.q[3]);
+    ang_err[s] = 2.f * asinf(fminf(vv, 1.f));
+  }
+}

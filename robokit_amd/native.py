@@ -55,6 +55,14 @@ class LMParamsC(C.Structure):
                 ("lm_lambda", "lambda_factor", "lambda_min", "lambda_max", "rho_min", "gain_ratio_epsilon")]
 
 
+class TrajDescC(C.Structure):
+    _fields_ = [("robot", C.c_int32), ("frames", C.c_int32), ("goal_link", C.c_int32), ("goal_weights", C.c_void_p),
+                ("vel_weight", C.c_float), ("acc_weight", C.c_float), ("dt", C.c_float), ("n_spheres", C.c_int32),
+                ("sphere_links", C.c_void_p), ("sphere_centers", C.c_void_p), ("sphere_radii", C.c_void_p),
+                ("coll_weight", C.c_float), ("coll_margin", C.c_float), ("n_obs_spheres", C.c_int32),
+                ("obs_spheres", C.c_void_p), ("n_obs_boxes", C.c_int32), ("obs_boxes", C.c_void_p)]
+
+
 class RequestC(C.Structure):
     _fields_ = [("terms", C.c_int32), ("n", C.c_int32), ("iters", C.c_int32), ("params", LMParamsC),
                 ("q", C.c_void_p), ("pose_targets", C.c_void_p), ("pos_targets", C.c_void_p),
@@ -92,7 +100,7 @@ EXPORTS = [
     "rk_batch_destroy", "rk_batch_set", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics", "rk_group_create",
     "rk_group_destroy", "rk_group_set_tuning", "rk_group_autotune", "rk_group_solve", "rk_group_kernel_times",
     "rk_server_create", "rk_server_add_terms", "rk_server_submit", "rk_server_poll", "rk_server_end_session",
-    "rk_server_stats", "rk_server_destroy",
+    "rk_server_stats", "rk_server_destroy", "rk_traj_batch_create", "rk_traj_metrics",
 ]
 
 
@@ -255,6 +263,7 @@ class Batch:
         self.ctx, self.spec, self.terms = ctx, spec, terms
         self.capacity, self.trace_iters = int(capacity), int(trace_iters)
         na = spec.num_actuated_joints
+        self._na = na  # variables per problem
         self._arrs = [
             _i32(list(terms.pose_links)),
             _f32(terms.pose_weights if terms.pose_weights is not None else np.zeros((0, 6))),
@@ -310,7 +319,7 @@ class Batch:
         return float(ms.value)
 
     def get(self, traces: bool = False):
-        na = self.spec.num_actuated_joints
+        na = self._na
         q = np.zeros((self.n, na), np.float32)
         cost = np.zeros(self.n, np.float32)
         lam = np.zeros(self.n, np.float32)
@@ -328,7 +337,7 @@ class Batch:
 
     def iterates(self):
         """q before each traced step [B, trace_iters, na] and the damping it used [B, trace_iters]."""
-        na = self.spec.num_actuated_joints
+        na = self._na
         qt = np.zeros((self.n, self.trace_iters, na), np.float32)
         lt = np.zeros((self.n, self.trace_iters), np.float32)
         _check(lib().rk_batch_get_iterates(self._h, _p(qt), _p(lt)))
@@ -479,3 +488,50 @@ class Server:
         st = ServeStatsC()
         _check(lib().rk_server_stats(self._h, C.byref(st)))
         return {name: getattr(st, name) for name, _ in ServeStatsC._fields_}
+
+
+class TrajBatch(Batch):
+    """Trajectory problems (rk_traj_batch_create): q [B, T, na] per problem, a goal pose for the
+    last frame, finite-difference smoothness and a sphere-SDF collision hinge against a primitive
+    scene; solved by the banded trajectory kernel."""
+
+    def __init__(self, ctx: Context, tb, capacity: int, trace_iters: int = 0, goal_weights=None):
+        self.ctx, self.spec = ctx, tb.spec
+        self.T, self.robot_na = tb.T, tb.spec.num_actuated_joints
+        self._na = self.T * self.robot_na
+        self.terms = TermSpec(pose_links=[tb.goal_link], pose_weights=np.ones((1, 6), np.float32))
+        self.capacity, self.trace_iters = int(capacity), int(trace_iters)
+        sc = tb.scene
+        self._arrs = [
+            _f32(np.ones(6) if goal_weights is None else goal_weights),
+            _i32(sc.sphere_links), _f32(sc.sphere_centers), _f32(sc.sphere_radii),
+            _f32(np.asarray(sc.obstacle_spheres).reshape(-1, 4)), _f32(np.asarray(sc.obstacle_boxes).reshape(-1, 6)),
+        ]
+        a = self._arrs
+        d = TrajDescC(ctx.robot(tb.spec), tb.T, tb.goal_link, _p(a[0]), tb.wv, tb.wa, tb.dt, len(a[1]), _p(a[1]),
+                      _p(a[2]), _p(a[3]), tb.coll_w, tb.margin, len(a[4]), _p(a[4]), len(a[5]), _p(a[5]))
+        self._h = C.c_void_p()
+        _check(lib().rk_traj_batch_create(ctx.handle, C.byref(d), self.capacity, self.trace_iters, C.byref(self._h)))
+        _LIVE["batch"].add(self)
+        ctx._children.add(self)
+        self._groups = weakref.WeakSet()
+        self.n = 0
+
+    def set(self, q: Optional[np.ndarray] = None, goals: Optional[np.ndarray] = None, n: Optional[int] = None):
+        qq = None if q is None else _f32(q).reshape(len(q), -1)
+        gg = None if goals is None else _f32(goals).reshape(len(goals), 1, 7)
+        super().set(q=qq, pose_targets=gg, n=n)
+
+    def get(self, traces: bool = False):
+        out = super().get(traces)
+        out["q"] = out["q"].reshape(self.n, self.T, self.robot_na)
+        return out
+
+    def iterates(self):
+        qt, lt = super().iterates()
+        return qt.reshape(self.n, self.trace_iters, self.T, self.robot_na), lt
+
+    def metrics(self, *args):
+        pe, ae, sd = (np.zeros(self.n, np.float32) for _ in range(3))
+        _check(lib().rk_traj_metrics(self._h, _p(pe), _p(ae), _p(sd)))
+        return pe, ae, sd
