@@ -77,7 +77,9 @@ def _serve_terms(N, trace):
     PositionTasks + PositionLimit + SmoothnessTask"""
     out = {}
     for kind in trace.kinds:
-        if kind == "retarget":
+        if kind == "trajopt":
+            out[kind] = (trace.pools[kind].spec, None)
+        elif kind == "retarget":
             clip = trace.pools[kind][0]
             na = clip.spec.num_actuated_joints
             out[kind] = (clip.spec, N.TermSpec(pos_links=clip.links,
@@ -96,6 +98,9 @@ def _request_inputs(trace, i):
     """(kind, n, q, pose_targets, pos_targets, q_prev) of request i"""
     kind = trace.kinds[int(trace.kind[i])]
     n, o = int(trace.n[i]), int(trace.off[i])
+    if kind == "trajopt":
+        tb = trace.pools[kind]
+        return kind, n, tb.q_seed[o:o + n], tb.goals[o:o + n, None], None, None
     if kind == "retarget":
         clip, prev = trace.pools[kind]
         return kind, n, prev[o:o + n], None, clip.targets[o + 1:o + 1 + n], prev[o:o + n]
@@ -111,10 +116,11 @@ def run_served(N, ctx, trace, params, iters, budget, streams):
 
     sv = N.Server(ctx, budget, streams=streams, max_launch_problems=8192)
     terms = _serve_terms(N, trace)
-    tid = {k: sv.add_terms(spec, ts) for k, (spec, ts) in terms.items()}
+    tid = {k: (sv.add_traj(trace.pools[k]) if k == "trajopt" else sv.add_terms(spec, ts))
+           for k, (spec, ts) in terms.items()}
     inputs = [_request_inputs(trace, i) for i in range(trace.size)]
     tq: "queue.Queue" = queue.Queue()
-    conv = {"ik": 0, "ik_problems": 0, "retarget_problems": 0}
+    conv = {"ik": 0, "ik_problems": 0, "retarget_problems": 0, "traj": 0, "traj_problems": 0}
 
     def collect():
         while True:
@@ -125,6 +131,9 @@ def run_served(N, ctx, trace, params, iters, budget, streams):
             r = sv.poll(ticket)
             if kind == "retarget":
                 conv["retarget_problems"] += len(r["cost"])
+            elif kind == "trajopt":
+                conv["traj"] += int(r["success"].sum())
+                conv["traj_problems"] += len(r["cost"])
             else:
                 conv["ik"] += int(r["success"].sum())
                 conv["ik_problems"] += len(r["cost"])
@@ -152,7 +161,8 @@ def run_plain_per_request(N, ctx, trace, params, iters, seconds):
     """The same arrivals (first `seconds`), each request solved alone by the plain per-term launch
     as it arrives, one after another on the context stream."""
     terms = _serve_terms(N, trace)
-    batches = {k: N.Batch(ctx, spec, ts, 256) for k, (spec, ts) in terms.items()}
+    batches = {k: (N.TrajBatch(ctx, trace.pools[k], 256) if k == "trajopt" else N.Batch(ctx, spec, ts, 256))
+               for k, (spec, ts) in terms.items()}
     lat, conv_ik = [], 0
     m = int(np.searchsorted(trace.t, seconds))
     t0 = time.perf_counter()
@@ -162,12 +172,17 @@ def run_plain_per_request(N, ctx, trace, params, iters, seconds):
         if wait > 0:
             time.sleep(wait)
         b = batches[kind]
-        b.set(q=q, pose_targets=pose, pos_targets=pos, q_prev=qprev)
-        b.solve(iters, params, mode=N.MODE_PLAIN)
-        if kind != "retarget":
-            conv_ik += int(b.metrics()[2].sum())
-        else:
+        if kind == "trajopt":  # no per-term decomposition of the trajectory path: its own kernel, per request
+            b.set(q=q, goals=pose[:, 0])
+            b.solve(iters, params)
             b.get()
+        else:
+            b.set(q=q, pose_targets=pose, pos_targets=pos, q_prev=qprev)
+            b.solve(iters, params, mode=N.MODE_PLAIN)
+            if kind != "retarget":
+                conv_ik += int(b.metrics()[2].sum())
+            else:
+                b.get()
         lat.append((time.perf_counter() - (t0 + trace.t[i])) * 1e3)
     span = time.perf_counter() - t0
     for b in batches.values():
@@ -189,6 +204,7 @@ def main() -> int:
     ap.add_argument("--serve-plain-seconds", type=float, default=0.5,
                     help="arrivals replayed through the plain per-request launch")
     ap.add_argument("--serve-budget-gb", type=float, default=16.0)
+    ap.add_argument("--no-traj", dest="traj", action="store_false", help="skip the trajectory leg")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a streaming copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -326,6 +342,26 @@ def main() -> int:
         },
     }
 
+    # ---- trajectory LM (512 collision-aware trajectories of 64 frames): kernel time and outcome
+    if args.traj:
+        tb = W.config3(seed=args.seed, batch=512, T=64)
+        tbatch = N.TrajBatch(ctx, tb, 512)
+        tbatch.set(q=tb.q_seed, goals=tb.goals)
+        tbatch.solve(ITERS, params, from_seed=True)
+        t_ms = [tbatch.solve(ITERS, params, from_seed=True) for _ in range(3)]
+        pe, ae, sd = tbatch.metrics()
+        ok = (pe < 1e-3) & (ae < 1e-2) & (sd >= 0)
+        out["trajectory"] = {
+            "workload": "512 trajectories x 64 frames, 7-DOF chain, 48 robot spheres vs 32 spheres + 8 boxes, "
+                        f"goal + velocity/acceleration (w 1e-2) + collision hinge (w 20, margin 2 cm); {ITERS} iterations",
+            "ms_per_solve": round(float(np.mean(t_ms)), 3),
+            "trajectories_per_s": round(512 / (float(np.mean(t_ms)) * 1e-3), 1),
+            "goal_reached_and_collision_free": int(ok.sum()),
+            "median_goal_error_m": round(float(np.median(pe)), 5),
+            "median_min_clearance_m": round(float(np.median(sd)), 4),
+        }
+        print(f"[bench] trajectory: {out['trajectory']}", file=sys.stderr, flush=True)
+
     # ---- served: Poisson request stream through the continuous-batching server vs plain per request
     if args.serve_seconds > 0:
         trace = W.config4(seed=args.seed, rate=2000.0, seconds=args.serve_seconds)
@@ -344,6 +380,7 @@ def main() -> int:
             "value": round(served_value, 1), "unit": "converged IK solves/s",
             "converged_ik": conv["ik"], "ik_problems": conv["ik_problems"],
             "retarget_frames": conv["retarget_problems"],
+            "trajectories": conv["traj_problems"], "trajectories_goal_and_clear": conv["traj"],
             "p50_ms": round(st["p50_ms"], 4), "p99_ms": round(st["p99_ms"], 4), "max_ms": round(st["max_ms"], 4),
             "launches": st["launches"], "mean_problems_per_launch": round(st["mean_problems_per_launch"], 2),
             "peak_device_bytes": st["peak_bytes"], "span_s": round(st["span_s"], 3),

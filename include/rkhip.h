@@ -246,6 +246,9 @@ int rk_server_create(rk_context* ctx, int64_t mem_budget, int32_t streams, int32
                      float pos_thr, float ang_thr, rk_server** out);
 /* register a term structure (resident; its kernel variant is chosen once) */
 int rk_server_add_terms(rk_server* s, const rk_terms_desc* terms, int32_t* term_id);
+/* register a trajectory problem set (rk_traj_desc); its requests carry q [n][frames][na] and
+ * pose_targets = goals [n][7]; success = goal within the thresholds and no sphere penetrating */
+int rk_server_add_traj(rk_server* s, const rk_traj_desc* desc, int32_t* term_id);
 /* queue a request (inputs copied); returns at once with a ticket */
 int rk_server_submit(rk_server* s, const rk_request* req, int64_t* ticket);
 /* results of a ticket: wait_ms 0 = poll (RK_EBUSY while running), < 0 = block; outputs nullable:

@@ -1245,15 +1245,25 @@ static int launch_variant(hipStream_t st, int v, const Problems& P, const LMPara
   return RK_OK;
 }
 
-// trajectory kernel of the batch's arm (one wavefront per problem)
-static int launch_traj(rk_batch* b, const Problems& P, const LMParams& lp, int it0, int iters) {
-  hipStream_t st = b->ctx->stream;
-  if (b->traj_na == 6)
-    hipLaunchKernelGGL((k_traj<6, 8>), dim3(P.n), dim3(WAVE), 0, st, P, (const TrajTerms*)b->tt, lp, it0, iters);
+// trajectory kernel of an arm with `na` dofs (one wavefront per problem)
+static int launch_traj(hipStream_t st, int na, const TrajTerms* tt, const Problems& P, const LMParams& lp, int it0,
+                       int iters) {
+  if (na == 6)
+    hipLaunchKernelGGL((k_traj<6, 8>), dim3(P.n), dim3(WAVE), 0, st, P, tt, lp, it0, iters);
   else
-    hipLaunchKernelGGL((k_traj<7, 9>), dim3(P.n), dim3(WAVE), 0, st, P, (const TrajTerms*)b->tt, lp, it0, iters);
+    hipLaunchKernelGGL((k_traj<7, 9>), dim3(P.n), dim3(WAVE), 0, st, P, tt, lp, it0, iters);
   HIP_TRY(hipGetLastError());
   return RK_OK;
+}
+static int launch_traj(rk_batch* b, const Problems& P, const LMParams& lp, int it0, int iters) {
+  return launch_traj(b->ctx->stream, b->traj_na, b->tt, P, lp, it0, iters);
+}
+static void launch_traj_metrics(hipStream_t st, int na, const TrajTerms* tt, const Problems& P, float* pe, float* ae,
+                                float* sd) {
+  if (na == 6)
+    hipLaunchKernelGGL((k_traj_metrics<6, 8>), dim3(P.n), dim3(WAVE), 0, st, P, tt, pe, ae, sd);
+  else
+    hipLaunchKernelGGL((k_traj_metrics<7, 9>), dim3(P.n), dim3(WAVE), 0, st, P, tt, pe, ae, sd);
 }
 
 static LMParams to_lp(const rk_lm_params* p) {
@@ -1747,12 +1757,13 @@ int rk_batch_destroy(rk_batch* b) {
   return RK_OK;
 }
 
-int rk_traj_batch_create(rk_context* ctx, const rk_traj_desc* d, int32_t capacity, int32_t trace_iters,
-                         rk_batch** out) {
+}  // extern "C"
+
+// validated device description of a trajectory problem set (rk_traj_batch_create, rk_server_add_traj)
+static int make_traj_terms(rk_context* ctx, const rk_traj_desc* d, TrajTerms* out) {
   int rc;
-  if (!ctx || !d || !out) return fail(RK_EINVAL, "null argument");
+  if (!ctx || !d) return fail(RK_EINVAL, "null argument");
   if ((rc = check_robot(ctx, d->robot))) return rc;
-  if (capacity <= 0 || trace_iters < 0) return fail(RK_EINVAL, "capacity must be positive, trace_iters >= 0");
   const Robot& r = ctx->robots[d->robot];
   if (!r.chain || (r.na != 6 && r.na != 7) || r.nj > r.na + 2)
     return fail(RK_EINVAL, "trajectory batches take serial arms with 6 or 7 dofs (<= 2 fixed tail joints)");
@@ -1762,7 +1773,7 @@ int rk_traj_batch_create(rk_context* ctx, const rk_traj_desc* d, int32_t capacit
       d->n_obs_boxes < 0 || d->n_obs_boxes > RK_SCENE_MAXB)
     return fail(RK_EINVAL, "at most 64 robot spheres, 64 obstacle spheres and 16 obstacle boxes");
   if (!(d->dt > 0.f)) return fail(RK_EINVAL, "dt must be positive");
-  TrajTerms tt;
+  TrajTerms& tt = *out;
   memset(&tt, 0, sizeof(tt));
   tt.robot = d->robot;
   tt.T = d->frames;
@@ -1793,7 +1804,11 @@ int rk_traj_batch_create(rk_context* ctx, const rk_traj_desc* d, int32_t capacit
     for (int c = 0; c < 4; ++c) tt.obs_s[i][c] = d->obs_spheres[4 * i + c];
   for (int i = 0; i < tt.nb; ++i)
     for (int c = 0; c < 6; ++c) tt.obs_b[i][c] = d->obs_boxes[6 * i + c];
-  // the batch: na = frames * dofs per problem (SoA [k][stride]), one 7-float goal per problem
+  return RK_OK;
+}
+
+// the goal-only term header of a trajectory problem set (the goal pose is its one pose target)
+static Terms traj_goal_terms(const rk_traj_desc* d, const TrajTerms& tt) {
   Terms th;
   memset(&th, 0, sizeof(th));
   th.robot = d->robot;
@@ -1802,6 +1817,21 @@ int rk_traj_batch_create(rk_context* ctx, const rk_traj_desc* d, int32_t capacit
   for (int k = 0; k < 6; ++k) th.pose_w[0][k] = tt.goal_w[k];
   th.Rf = 6;
   th.R = 6;
+  return th;
+}
+
+extern "C" {
+
+int rk_traj_batch_create(rk_context* ctx, const rk_traj_desc* d, int32_t capacity, int32_t trace_iters,
+                         rk_batch** out) {
+  int rc;
+  if (!ctx || !d || !out) return fail(RK_EINVAL, "null argument");
+  if (capacity <= 0 || trace_iters < 0) return fail(RK_EINVAL, "capacity must be positive, trace_iters >= 0");
+  TrajTerms tt;
+  if ((rc = make_traj_terms(ctx, d, &tt))) return rc;
+  const Robot& r = ctx->robots[d->robot];
+  // the batch: na = frames * dofs per problem (SoA [k][stride]), one 7-float goal per problem
+  const Terms th = traj_goal_terms(d, tt);
   const int D = d->frames * r.na;
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device));
@@ -1849,12 +1879,7 @@ int rk_traj_metrics(rk_batch* b, float* pos_err, float* ang_err, float* min_sd) 
   float* dbuf = nullptr;
   HIP_TRY(hipMalloc(&dbuf, sizeof(float) * 3 * B));
   const Problems P = make_problems(b);
-  if (b->traj_na == 6)
-    hipLaunchKernelGGL((k_traj_metrics<6, 8>), dim3(B), dim3(WAVE), 0, ctx->stream, P, (const TrajTerms*)b->tt, dbuf,
-                       dbuf + B, dbuf + 2 * B);
-  else
-    hipLaunchKernelGGL((k_traj_metrics<7, 9>), dim3(B), dim3(WAVE), 0, ctx->stream, P, (const TrajTerms*)b->tt, dbuf,
-                       dbuf + B, dbuf + 2 * B);
+  launch_traj_metrics(ctx->stream, b->traj_na, b->tt, P, dbuf, dbuf + B, dbuf + 2 * B);
   hipError_t e = hipGetLastError();
   std::vector<float> h(3 * B);
   if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dbuf, sizeof(float) * 3 * B, hipMemcpyDeviceToHost, ctx->stream);

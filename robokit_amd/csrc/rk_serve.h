@@ -74,8 +74,17 @@ struct Arena {
 struct SrvTerms {
   Terms th{};
   Terms* td = nullptr;
-  int variant = 0, na = 0;
+  int variant = 0, na = 0;  // na: variables per problem (frames * dofs for trajectories)
+  int traj_na = 0;          // > 0: trajectory problems of an arm with this many dofs
+  TrajTerms* tt = nullptr;
 };
+
+// success of a trajectory problem: goal within the thresholds and every sphere clear of the scene
+__global__ void k_srv_traj_ok(const float* pe, const float* ae, const float* sd, int* ok, int n, float pos_thr,
+                              float ang_thr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ok[i] = (pe[i] < pos_thr && ae[i] < ang_thr && sd[i] >= 0.f) ? 1 : 0;
+}
 
 struct SrvReq {
   int64_t ticket = 0, session = 0;
@@ -187,6 +196,12 @@ static size_t srv_layout(const SrvTerms& t, int n, char* base, SrvLaunch* L, siz
   return o;
 }
 static size_t srv_launch_bytes(const SrvTerms& t, int n) { return srv_layout(t, n, nullptr, nullptr, nullptr, nullptr); }
+// do the staging buffers hold a launch of n problems of t?
+static bool srv_fits_stage(const rk_server* sv, const SrvTerms& t, int n) {
+  size_t ib = 0, ob = 0;
+  srv_layout(t, n, nullptr, nullptr, &ib, &ob);
+  return ib <= sizeof(float) * sv->stage_floats && ob <= sizeof(float) * sv->stage_floats;
+}
 
 // stage, launch and enqueue the copies of launch slot s (worker thread)
 static int srv_start(rk_server* sv, int s) {
@@ -243,11 +258,20 @@ static int srv_start(rk_server* sv, int s) {
   P.n = n;
   P.stride = n;
   const LMParams lp = L.reqs[0]->lp;
-  int rc = launch_variant(st, T.variant, P, lp, 0, L.reqs[0]->iters);
-  if (rc) return rc;
-  if (np)
-    hipLaunchKernelGGL(k_ik_metrics, dim3((n + 127) / 128), dim3(128), 0, st, P, sv->pos_thr, sv->ang_thr, L.pe, L.ae,
-                       L.ok);
+  if (T.traj_na) {
+    int rc = launch_traj(st, T.traj_na, T.tt, P, lp, 0, L.reqs[0]->iters);
+    if (rc) return rc;
+    // lambda is not read back: its slice holds the clearance for the success flags
+    launch_traj_metrics(st, T.traj_na, T.tt, P, L.pe, L.ae, L.lam);
+    hipLaunchKernelGGL(k_srv_traj_ok, dim3((n + 127) / 128), dim3(128), 0, st, L.pe, L.ae, L.lam, L.ok, n,
+                       sv->pos_thr, sv->ang_thr);
+  } else {
+    int rc = launch_variant(st, T.variant, P, lp, 0, L.reqs[0]->iters);
+    if (rc) return rc;
+    if (np)
+      hipLaunchKernelGGL(k_ik_metrics, dim3((n + 127) / 128), dim3(128), 0, st, P, sv->pos_thr, sv->ang_thr, L.pe,
+                         L.ae, L.ok);
+  }
   // session state <- this solution
   for (SrvReq* r : L.reqs) {
     if (!r->session) continue;
@@ -363,7 +387,7 @@ static bool srv_form(rk_server* sv, int s) {
   std::set<int64_t> sess;
   int n = 0;
   for (SrvReq* r : sv->queue) {
-    if (!srv_same_key(r, head) || n + r->n > sv->max_problems) continue;
+    if (!srv_same_key(r, head) || n + r->n > sv->max_problems || !srv_fits_stage(sv, T, n + r->n)) continue;
     if (r->session && (sess.count(r->session) || !srv_ready(sv, r))) continue;
     pick.push_back(r);
     if (r->session) sess.insert(r->session);
@@ -526,6 +550,31 @@ int rk_server_add_terms(rk_server* sv, const rk_terms_desc* t, int32_t* term_id)
   return RK_OK;
 }
 
+int rk_server_add_traj(rk_server* sv, const rk_traj_desc* d, int32_t* term_id) {
+  if (!sv || !d || !term_id) return fail(RK_EINVAL, "null argument");
+  rk_context* ctx = sv->ctx;
+  SrvTerms T;
+  int rc;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    TrajTerms tt;
+    if ((rc = make_traj_terms(ctx, d, &tt))) return rc;
+    T.th = traj_goal_terms(d, tt);
+    T.traj_na = ctx->robots[d->robot].na;
+    T.na = d->frames * T.traj_na;
+    if ((rc = dmalloc(ctx, (void**)&T.td, sizeof(Terms), nullptr))) return rc;
+    if ((rc = dmalloc(ctx, (void**)&T.tt, sizeof(TrajTerms), nullptr))) return rc;
+    HIP_TRY(hipMemcpyAsync(T.td, &T.th, sizeof(Terms), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(T.tt, &tt, sizeof(TrajTerms), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->terms.push_back(T);
+  *term_id = (int32_t)sv->terms.size() - 1;
+  return RK_OK;
+}
+
 int rk_server_submit(rk_server* sv, const rk_request* rq, int64_t* ticket) {
   if (!sv || !rq || !ticket) return fail(RK_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(sv->mu);
@@ -534,6 +583,7 @@ int rk_server_submit(rk_server* sv, const rk_request* rq, int64_t* ticket) {
   const SrvTerms& T = sv->terms[rq->terms];
   if (rq->n <= 0 || rq->n > sv->max_problems)
     return fail(RK_EINVAL, "request size must be 1.." + std::to_string(sv->max_problems));
+  if (!srv_fits_stage(sv, T, rq->n)) return fail(RK_EINVAL, "request too large for the server's staging buffers");
   if (rq->iters < 1) return fail(RK_EINVAL, "iters must be >= 1");
   if (T.th.n_pose && !rq->pose_targets) return fail(RK_EINVAL, "pose targets missing");
   if (T.th.n_pos && !rq->pos_targets) return fail(RK_EINVAL, "position targets missing");
@@ -673,7 +723,10 @@ int rk_server_destroy(rk_server* sv) {
     if (sv->h_in[s]) hipHostFree(sv->h_in[s]);
     if (sv->h_out[s]) hipHostFree(sv->h_out[s]);
   }
-  for (auto& T : sv->terms) dfree(ctx, T.td, sizeof(Terms));
+  for (auto& T : sv->terms) {
+    dfree(ctx, T.td, sizeof(Terms));
+    if (T.tt) dfree(ctx, T.tt, sizeof(TrajTerms));
+  }
   dfree(ctx, sv->arena_base, sv->arena.size);
   for (auto& kv : sv->reqs) delete kv.second;
   delete sv;

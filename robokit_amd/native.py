@@ -99,7 +99,7 @@ EXPORTS = [
     "rk_batch_create",
     "rk_batch_destroy", "rk_batch_set", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics", "rk_group_create",
     "rk_group_destroy", "rk_group_set_tuning", "rk_group_autotune", "rk_group_solve", "rk_group_kernel_times",
-    "rk_server_create", "rk_server_add_terms", "rk_server_submit", "rk_server_poll", "rk_server_end_session",
+    "rk_server_create", "rk_server_add_terms", "rk_server_add_traj", "rk_server_submit", "rk_server_poll", "rk_server_end_session",
     "rk_server_stats", "rk_server_destroy", "rk_traj_batch_create", "rk_traj_metrics",
 ]
 
@@ -448,6 +448,20 @@ class Server:
         self._terms.append((na, len(terms.pose_links), len(terms.pos_links)))
         return tid.value
 
+    def add_traj(self, tb, goal_weights=None) -> int:
+        """register a trajectory problem set (workloads.TrajBatch shape: robot, frames, scene)"""
+        sc = tb.scene
+        arrs = [_f32(np.ones(6) if goal_weights is None else goal_weights), _i32(sc.sphere_links),
+                _f32(sc.sphere_centers), _f32(sc.sphere_radii), _f32(np.asarray(sc.obstacle_spheres).reshape(-1, 4)),
+                _f32(np.asarray(sc.obstacle_boxes).reshape(-1, 6))]
+        d = TrajDescC(self.ctx.robot(tb.spec), tb.T, tb.goal_link, _p(arrs[0]), tb.wv, tb.wa, tb.dt, len(arrs[1]),
+                      _p(arrs[1]), _p(arrs[2]), _p(arrs[3]), tb.coll_w, tb.margin, len(arrs[4]), _p(arrs[4]),
+                      len(arrs[5]), _p(arrs[5]))
+        tid = C.c_int32()
+        _check(lib().rk_server_add_traj(self._h, C.byref(d), C.byref(tid)))
+        self._terms.append((tb.T * tb.spec.num_actuated_joints, 1, 0))
+        return tid.value
+
     def submit(self, term: int, iters: int, params: LMParams, q: Optional[np.ndarray] = None,
                pose_targets: Optional[np.ndarray] = None, pos_targets: Optional[np.ndarray] = None,
                q_prev: Optional[np.ndarray] = None, session: int = 0, n: Optional[int] = None) -> int:
@@ -456,6 +470,10 @@ class Server:
             arrs[2] = _f32(arrs[2][..., :3])
         if n is None:
             n = next(x.shape[0] for x in arrs if x is not None)
+        na = self._terms[term][0]
+        for i in (0, 3):
+            if arrs[i] is not None and arrs[i].size != int(n) * na:
+                raise ValueError(f"q / q_prev must hold {n} x {na} values, got {arrs[i].shape}")
         req = RequestC(int(term), int(n), int(iters), params.c(), *[_p(x) for x in arrs], int(session))
         t = C.c_int64()
         _check(lib().rk_server_submit(self._h, C.byref(req), C.byref(t)))

@@ -214,10 +214,10 @@ class ServeTrace:
 
 
 def config4(seed: int = 0, rate: float = 2000.0, seconds: float = 60.0, pool: int = 4096,
-            kinds: Sequence[str] = ("ik6", "ik7", "ik29", "retarget")) -> ServeTrace:
+            kinds: Sequence[str] = ("ik6", "ik7", "ik29", "retarget", "trajopt")) -> ServeTrace:
     """Poisson arrivals at `rate` requests/s for `seconds`; problems per request floor(exp(U(0, ln 257)))
     in [1, 256]; request kind uniform over `kinds` (IK 6/7/29-DOF as config 1, one retargeting frame
-    of config 2 per problem, seeded at the clip's previous frame)."""
+    of config 2 per problem seeded at the clip's previous frame, T = 64 trajectories of config 3)."""
     s = np.random.SeedSequence(seed).spawn(8)
     u, p, h = ur10(), panda_dh(), humanoid29(0)
     pools: Dict[str, object] = {}
@@ -231,6 +231,8 @@ def config4(seed: int = 0, rate: float = 2000.0, seconds: float = 60.0, pool: in
         clip = config2(seed=int(s[3].generate_state(1)[0]) % 1000, frames=pool + 1)
         prev = clip.q_traj[:-1] + np.random.default_rng(s[4]).normal(0, 0.02, clip.q_traj[:-1].shape)
         pools["retarget"] = (clip, prev.astype(np.float32))
+    if "trajopt" in kinds:
+        pools["trajopt"] = config3(seed=int(s[6].generate_state(1)[0]) % 1000, batch=pool, T=64)
     rng = np.random.default_rng(s[5])
     n_req = int(rate * seconds * 1.2) + 16
     t = np.cumsum(rng.exponential(1.0 / rate, n_req))

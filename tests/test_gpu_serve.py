@@ -142,3 +142,31 @@ def test_streaming_arrivals_overlap(ctx):
     print(f"\nstreaming: {st}")
     assert st["completed"] == len(reqs) and st["p99_ms"] >= st["p50_ms"] > 0
     assert all(r["latency_ms"] > 0 for r in res)
+
+
+def test_trajectory_requests_equal_batch(ctx):
+    """trajectory requests (T = 32) mixed with IK requests: each equals the trajectory batch's solve"""
+    tb = W.config3(seed=8, batch=24, T=32)
+    sv, tids = _server(ctx)
+    ttid = sv.add_traj(tb)
+    ik = _requests(4, 8)
+    tickets = []
+    cuts = [0, 5, 6, 13, 24]
+    for i in range(4):
+        lo, hi = cuts[i], cuts[i + 1]
+        tickets.append(sv.submit(ttid, 10, P, q=tb.q_seed[lo:hi], pose_targets=tb.goals[lo:hi, None]))
+        kind, wl = ik[i]
+        sv.submit(tids[kind], ITERS, P, q=wl.q_seed, pose_targets=wl.targets)
+    res = [sv.poll(t) for t in tickets]
+    b = N.TrajBatch(ctx, tb, 24)
+    b.set(q=tb.q_seed, goals=tb.goals)
+    b.solve(10, P)
+    ref = b.get()
+    pe, ae, sd = b.metrics()
+    ok = (pe < 1e-3) & (ae < 1e-2) & (sd >= 0)
+    for i in range(4):
+        lo, hi = cuts[i], cuts[i + 1]
+        assert np.array_equal(res[i]["q"].reshape(hi - lo, 32, -1), ref["q"][lo:hi]), i
+        assert np.array_equal(res[i]["cost"], ref["cost"][lo:hi])
+        assert np.array_equal(res[i]["success"], ok[lo:hi])
+    sv.close()
