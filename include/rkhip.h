@@ -84,9 +84,11 @@ int rk_se3_op(rk_context* ctx, int32_t op, int32_t n, const float* a, const floa
  * Residual-term structure of a batch (uniform over its problems), in this row order:
  *   FrameTask log mode   (terms/dense/frame_task.py:505-621)      6 rows per pose target
  *   PositionTask         (terms/dense/position_task.py:188-272)   3 rows per position target
+ *   RotationTask         (terms/dense/rotation_task.py:323-401)   3 rows per rotation target
  *   PositionLimit "abs"  (terms/dense/position_limit.py:508-573)  na rows
  *   SmoothnessTask       (terms/dense/smoothness_task.py:350-422) na rows
- * Limits: n_pose <= 4, n_pos <= 12.
+ * Limits: n_pose <= 4, n_pos <= 12, n_pose + n_rot <= 4.  Batches with rotation targets run the
+ * LDS-staged fused kernels (threads_per_problem 4..64).
  */
 typedef struct {
     int32_t robot;
@@ -100,6 +102,9 @@ typedef struct {
     const float* limit_weights; /* [na] */
     int32_t smooth;
     const float* smooth_weights; /* [na] */
+    int32_t n_rot;               /* RotationTask (terms/dense/rotation_task.py:323-401), rows after the position rows */
+    const int32_t* rot_links;
+    const float* rot_weights;    /* [n_rot][3] */
 } rk_terms_desc;
 
 /* LMOptimizerConfig fields (opt/lm_optimizer.py:502-523) */
@@ -141,6 +146,8 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
  */
 int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mode, int32_t threads_per_problem,
                    float* kernel_ms);
+/* rotation targets [B][n_rot][4] (w, x, y, z) of the live batch (after rk_batch_set) */
+int rk_batch_set_rotations(rk_batch* b, int32_t B, const float* rot_targets);
 /* warm start for a sequence of frames (helpers/retarget: the previous frame's solution is the
  * next frame's seed and its SmoothnessTask reference, terms/dense/smoothness_task.py:350-365):
  * on the device, q_prev <- q and seeds <- q; the next rk_batch_set then only brings new targets */

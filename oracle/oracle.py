@@ -20,7 +20,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 _LIBS = {}
 
-FRAME, POSITION, LIMIT, SMOOTH = 0, 1, 2, 3
+FRAME, POSITION, LIMIT, SMOOTH, ROTATION = 0, 1, 2, 3, 4
 
 
 def build() -> None:
@@ -162,7 +162,8 @@ def residual_jacobian(spec, terms: Sequence[Term], b: int, q: np.ndarray, dtype:
     keep: list = []
     ta = _terms(terms, keep)
     na = spec.num_actuated_joints
-    R = sum(6 * len(t.links) if t.kind == FRAME else 3 * len(t.links) if t.kind == POSITION else na for t in terms)
+    R = sum(6 * len(t.links) if t.kind == FRAME else 3 * len(t.links) if t.kind in (POSITION, ROTATION) else na
+            for t in terms)
     q = np.ascontiguousarray(q, rt)
     r = np.zeros(R, rt)
     J = np.zeros((R, na), rt)

@@ -527,7 +527,7 @@ EXPORT void ork_link_jacobian(const ork_robot *rb, int B, const real *q, const r
 /* ------------------------------------------------------------------------- */
 /* residual terms                                                            */
 /* ------------------------------------------------------------------------- */
-enum { ORK_FRAME = 0, ORK_POSITION = 1, ORK_LIMIT = 2, ORK_SMOOTH = 3 };
+enum { ORK_FRAME = 0, ORK_POSITION = 1, ORK_LIMIT = 2, ORK_SMOOTH = 3, ORK_ROTATION = 4 };
 
 typedef struct {
     int kind;
@@ -542,6 +542,7 @@ static int term_rows(const ork_robot *rb, const ork_term *t) {
     switch (t->kind) {
     case ORK_FRAME: return 6 * t->nframes;
     case ORK_POSITION: return 3 * t->nframes;
+    case ORK_ROTATION: return 3 * t->nframes;
     default: return rb->na;
     }
 }
@@ -566,6 +567,8 @@ static void spatial_column(const ork_robot *rb, const real *S, int link, int a, 
  *   POSITION position_task.py:188-212, 215-272
  *   LIMIT    compute_position_limit_weighted_residual/jacobian_kernel position_limit.py:508-573 ("abs")
  *   SMOOTH   compute_smoothness_task_weighted_residual/jacobian_kernel smoothness_task.py:350-365, 404-422
+ *   ROTATION compute_rotation_task_weighted_residual_kernel rotation_task.py:323-349 (axis-angle of
+ *            q_link * q_target^-1) and the geometric Jacobian rotation_task.py:352-401 (w * omega column)
  */
 static void term_eval(const ork_robot *rb, const ork_term *t, int b, const real *q, const real *Tl, const real *S,
                       real *r, real *J) {
@@ -623,6 +626,23 @@ static void term_eval(const ork_robot *rb, const ork_term *t, int b, const real 
                 if (x > hi) sign = 1;
                 else if (x < lo) sign = -1;
                 J[d * na + d] = (real)t->weights[d] * sign;
+            }
+        }
+    } else if (t->kind == ORK_ROTATION) {
+        for (int f = 0; f < t->nframes; ++f) {
+            const real *Tf = Tl + 7 * t->links[f];
+            const float *tg = t->targets + ((size_t)b * t->nframes + f) * 7;
+            const float *w = t->weights + 3 * f;
+            real qinv[4] = {(real)tg[3], -(real)tg[4], -(real)tg[5], -(real)tg[6]}, qe[4], aa[3];
+            quat_mul(Tf + 3, qinv, qe);
+            quat_to_axis_angle(qe, aa);
+            for (int i = 0; i < 3; ++i) r[3 * f + i] = (real)w[i] * aa[i];
+            if (J) {
+                for (int a = 0; a < na; ++a) {
+                    real sc[6];
+                    spatial_column(rb, S, t->links[f], a, sc);
+                    for (int i = 0; i < 3; ++i) J[(3 * f + i) * na + a] = (real)w[i] * sc[3 + i];
+                }
             }
         }
     } else if (t->kind == ORK_SMOOTH) {

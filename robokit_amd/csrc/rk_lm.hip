@@ -49,6 +49,10 @@ struct Terms {
   float pos_w[RK_MAXQ][3];
   float limit_w[RK_NA_MAX];
   float smooth_w[RK_NA_MAX];
+  // RotationTask rows (after the position rows): n_rot + n_pose <= RK_MAXT
+  int n_rot, Rr;
+  int rot_link[RK_MAXT];
+  float rot_w[RK_MAXT][3];
 };
 
 struct LMParams {
@@ -63,6 +67,7 @@ struct Problems {
   const float* q_seed;    // [na][stride] start iterate for solves that restart from the seeds, or null
   const float* pose_tgt;  // [n_pose*7][stride]
   const float* pos_tgt;   // [n_pos*3][stride]
+  const float* rot_tgt;   // [n_rot*4][stride] target quaternions (w, x, y, z)
   const float* q_prev;    // [na][stride]
   float* cost;            // [stride]
   float* lam;             // [stride]
@@ -200,13 +205,22 @@ __device__ __forceinline__ float limit_diag(float x, float lo, float hi, float w
   return w * (x > hi ? 1.f : (x < lo ? -1.f : 0.f));
 }
 
+// RotationTask residual direction: axis-angle of q_link * q_target^-1 (rotation_task.py:323-349;
+// the product is standardised to w >= 0 by quat_mul)
+__device__ __forceinline__ void rotation_target(const Pose& Tf, const float* qt, float* aa) {
+  const float qinv[4] = {qt[0], -qt[1], -qt[2], -qt[3]};
+  float qe[4];
+  quat_mul(Tf.q, qinv, qe);
+  quat_to_axis_angle(qe, aa);
+}
+
 // cost = sum over terms (in row order) of 0.5 * sum of squares of its rows
 __device__ __forceinline__ float term_cost(const Terms* tm, int na, const float* r) {
   float cost = 0.f;
   int off = 0;
-  const int sizes[4] = {tm->Rf, tm->Rq, tm->limit ? na : 0, tm->smooth ? na : 0};
+  const int sizes[5] = {tm->Rf, tm->Rq, tm->Rr, tm->limit ? na : 0, tm->smooth ? na : 0};
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
+  for (int t = 0; t < 5; ++t) {
     if (sizes[t] == 0) continue;
     float s = 0.f;
     for (int k = 0; k < sizes[t]; ++k) s += r[off + k] * r[off + k];
@@ -287,7 +301,14 @@ __device__ __forceinline__ void residual_rows(const Robot* rb, const Terms* tm, 
 #pragma unroll
     for (int i = 0; i < 3; ++i) sh->r[tm->Rf + 3 * u + i] = tm->pos_w[u][i] * (sh->tq[u][i] - Tf.t[i]);
   }
-  const int off_l = tm->Rf + tm->Rq;
+  for (int u = lane; u < tm->n_rot; u += G) {
+    Pose Tf = link_pose<C>(rb, T, tm->rot_link[u]);
+    float aa[3];
+    rotation_target(Tf, sh->tp[tm->n_pose + u] + 3, aa);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) sh->r[tm->Rf + tm->Rq + 3 * u + i] = tm->rot_w[u][i] * aa[i];
+  }
+  const int off_l = tm->Rf + tm->Rq + tm->Rr;
   if (tm->limit)
     for (int d = lane; d < na; d += G) {
       sh->r[off_l + d] = limit_row(q[d], rb->lo[d], rb->hi[d], tm->limit_w[d]);
@@ -305,8 +326,8 @@ __device__ __forceinline__ void residual_rows(const Robot* rb, const Terms* tm, 
 template <int G, class C>
 __device__ __forceinline__ void normal_equations(const Terms* tm, Shared<C>* sh, int na, int lane) {
   const int ntri = na * (na + 1) / 2;
-  const int Rf = tm->Rf, Rq = tm->Rq;
-  const int off_l = Rf + Rq, off_s = off_l + (tm->limit ? na : 0);
+  const int Rf = tm->Rf, Rq = tm->Rq, Rr = tm->Rr;
+  const int off_l = Rf + Rq + Rr, off_s = off_l + (tm->limit ? na : 0);
   for (int k = lane; k <= ntri + na; k += G) {
     if (k < ntri) {
       int i, j;
@@ -318,6 +339,11 @@ __device__ __forceinline__ void normal_equations(const Terms* tm, Shared<C>* sh,
         float s2 = 0.f;
         for (int r = Rf; r < Rf + Rq; ++r) s2 += sh->J[r][i] * sh->J[r][j];
         h = h + s2;
+      }
+      if (Rr) {
+        float s3 = 0.f;
+        for (int r = Rf + Rq; r < Rf + Rq + Rr; ++r) s3 += sh->J[r][i] * sh->J[r][j];
+        h = h + s3;
       }
       if (i == j) {
         if (tm->limit) h = h + sh->jl[i] * sh->jl[i];
@@ -333,6 +359,11 @@ __device__ __forceinline__ void normal_equations(const Terms* tm, Shared<C>* sh,
         float s2 = 0.f;
         for (int r = Rf; r < Rf + Rq; ++r) s2 += sh->J[r][i] * sh->r[r];
         gi = gi + s2;
+      }
+      if (Rr) {
+        float s3 = 0.f;
+        for (int r = Rf + Rq; r < Rf + Rq + Rr; ++r) s3 += sh->J[r][i] * sh->r[r];
+        gi = gi + s3;
       }
       if (tm->limit) gi = gi + sh->jl[i] * sh->r[off_l + i];
       if (tm->smooth) gi = gi + sh->js[i] * sh->r[off_s + i];
@@ -405,7 +436,7 @@ template <int G, class C>
 __device__ __forceinline__ void jacobian_columns(const Robot* rb, const Terms* tm, Shared<C>* sh, float (*T)[8],
                                                  int lane) {
   const int na = rb->na;
-  const int nitems = (tm->n_pose + tm->n_pos) * na;
+  const int nitems = (tm->n_pose + tm->n_pos + tm->n_rot) * na;
   auto S_at = [&](int j, int r) { return sh->S[j][r]; };
   for (int k = lane; k < nitems; k += G) {
     const int t = k / na, a = k - t * na;
@@ -416,7 +447,7 @@ __device__ __forceinline__ void jacobian_columns(const Robot* rb, const Terms* t
       pose_jac_column(sh->tg[t], sc, c6);
 #pragma unroll
       for (int i = 0; i < 6; ++i) sh->J[6 * t + i][a] = c6[i];
-    } else {
+    } else if (t < tm->n_pose + tm->n_pos) {
       const int u = t - tm->n_pose;
       const int link = tm->pos_link[u];
       spatial_column(rb, link, a, S_at, sc);
@@ -425,6 +456,12 @@ __device__ __forceinline__ void jacobian_columns(const Robot* rb, const Terms* t
       pos_jac_column(sc, Tf.t, tm->pos_w[u], c3);
 #pragma unroll
       for (int i = 0; i < 3; ++i) sh->J[tm->Rf + 3 * u + i][a] = c3[i];
+    } else {
+      // geometric Jacobian of the rotation rows: w * angular column (rotation_task.py:352-401)
+      const int u = t - tm->n_pose - tm->n_pos;
+      spatial_column(rb, tm->rot_link[u], a, S_at, sc);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) sh->J[tm->Rf + tm->Rq + 3 * u + i][a] = tm->rot_w[u][i] * sc[3 + i];
     }
   }
 }
@@ -452,6 +489,7 @@ __device__ __forceinline__ void solve_tile(const Problems& P, int first, const L
   }
   for (int k = lane; k < tm->n_pose * 7; k += G) sh->tp[k / 7][k % 7] = P.pose_tgt[k * S + s];
   for (int k = lane; k < tm->n_pos * 3; k += G) sh->tq[k / 3][k % 3] = P.pos_tgt[k * S + s];
+  for (int k = lane; k < tm->n_rot * 4; k += G) sh->tp[tm->n_pose + k / 4][3 + k % 4] = P.rot_tgt[k * S + s];
   if (lane == 0) {
     sh->lam = it0 == 0 ? lp.lm_lambda : P.lam[s];
     sh->nacc = it0 == 0 ? 0 : P.nacc[s];
@@ -1079,6 +1117,7 @@ struct rk_batch {
   int traj_na = 0;  // robot dofs of a trajectory batch
   TrajTerms* tt = nullptr;
   float *q = nullptr, *q_seed = nullptr, *pose_tgt = nullptr, *pos_tgt = nullptr, *q_prev = nullptr;
+  float* rot_tgt = nullptr;
   float *cost = nullptr, *lam = nullptr, *trace = nullptr;
   int *nacc = nullptr, *acc_trace = nullptr;
   float *q_trace = nullptr, *lam_trace = nullptr;
@@ -1128,6 +1167,7 @@ static Problems make_problems(rk_batch* b, bool from_seed = false) {
   P.q_seed = from_seed ? b->q_seed : nullptr;
   P.pose_tgt = b->pose_tgt;
   P.pos_tgt = b->pos_tgt;
+  P.rot_tgt = b->rot_tgt;
   P.q_prev = b->q_prev;
   P.cost = b->cost;
   P.lam = b->lam;
@@ -1143,19 +1183,19 @@ static Problems make_problems(rk_batch* b, bool from_seed = false) {
 }
 
 static bool fits_small(const Terms& th, const Robot& r) {
-  return r.nj <= CfgS::nj && r.na <= CfgS::na && th.n_pose <= CfgS::np && th.n_pos <= CfgS::nq;
+  return r.nj <= CfgS::nj && r.na <= CfgS::na && th.n_pose + th.n_rot <= CfgS::np && th.n_pos <= CfgS::nq;
 }
 
 // register thread-per-problem kernels: serial arms with 5..8 dofs, <= 2 fixed tail joints,
 // <= 1 pose target, no position targets
 static bool fits_thread(const Terms& th, const Robot& r) {
-  return r.chain && r.na >= 5 && r.na <= 8 && r.nj <= r.na + 2 && th.n_pose <= 1 && th.n_pos == 0;
+  return r.chain && r.na >= 5 && r.na <= 8 && r.nj <= r.na + 2 && th.n_pose <= 1 && th.n_pos == 0 && th.n_rot == 0;
 }
 
 // register wave-per-problem kernels: trees with <= 64 joints, <= 32 dofs, each dof driven by
 // <= 2 joints; <= 4 pose targets and no position targets, or <= 12 position targets and no pose
 static int wave_kind(const Terms& th, const Robot& r) {
-  if (r.nj > 64 || r.na > WV_NA) return -1;
+  if (r.nj > 64 || r.na > WV_NA || th.n_rot) return -1;
   for (int a = 0; a < r.na; ++a)
     if (r.col_start[a + 1] - r.col_start[a] > 2) return -1;
   if (th.n_pos == 0 && th.n_pose <= 4) return V_W40;
@@ -1164,7 +1204,7 @@ static int wave_kind(const Terms& th, const Robot& r) {
 }
 
 static int signature(const Terms& th) {
-  return ((th.robot * 8 + th.n_pose) * 16 + th.n_pos) * 4 + th.limit * 2 + th.smooth;
+  return (((th.robot * 8 + th.n_pose) * 16 + th.n_pos) * 4 + th.limit * 2 + th.smooth) * 8 + th.n_rot;
 }
 
 // threads per problem -> variant.  G = 0: the tuned choice for this term structure if the
@@ -1665,7 +1705,9 @@ static int make_terms(rk_context* ctx, const rk_terms_desc* t, Terms* out) {
   const Robot& r = ctx->robots[t->robot];
   if (t->n_pose < 0 || t->n_pose > RK_MAXT || t->n_pos < 0 || t->n_pos > RK_MAXQ)
     return fail(RK_EINVAL, "at most 4 pose targets and 12 position targets per problem");
-  if (t->n_pose + t->n_pos + (t->limit ? 1 : 0) + (t->smooth ? 1 : 0) == 0)
+  if (t->n_rot < 0 || t->n_pose + t->n_rot > RK_MAXT)
+    return fail(RK_EINVAL, "at most 4 pose and rotation targets together per problem");
+  if (t->n_pose + t->n_pos + t->n_rot + (t->limit ? 1 : 0) + (t->smooth ? 1 : 0) == 0)
     return fail(RK_EINVAL, "a batch needs at least one residual term");
   if (r.nj > CfgL::nj) return fail(RK_EINVAL, "the LM kernels take robots with nj <= 32");
   Terms th;
@@ -1689,9 +1731,16 @@ static int make_terms(rk_context* ctx, const rk_terms_desc* t, Terms* out) {
     th.limit_w[d] = th.limit ? t->limit_weights[d] : 0.f;
     th.smooth_w[d] = th.smooth ? t->smooth_weights[d] : 0.f;
   }
+  th.n_rot = t->n_rot;
+  for (int i = 0; i < t->n_rot; ++i) {
+    th.rot_link[i] = t->rot_links[i];
+    if (th.rot_link[i] < 0 || th.rot_link[i] >= r.nl) return fail(RK_EINVAL, "rotation link out of range");
+    for (int k = 0; k < 3; ++k) th.rot_w[i][k] = t->rot_weights[3 * i + k];
+  }
   th.Rf = 6 * th.n_pose;
   th.Rq = 3 * th.n_pos;
-  th.R = th.Rf + th.Rq + (th.limit ? r.na : 0) + (th.smooth ? r.na : 0);
+  th.Rr = 3 * th.n_rot;
+  th.R = th.Rf + th.Rq + th.Rr + (th.limit ? r.na : 0) + (th.smooth ? r.na : 0);
   *out = th;
   return RK_OK;
 }
@@ -1720,6 +1769,7 @@ int rk_batch_create(rk_context* ctx, const rk_terms_desc* t, int32_t capacity, i
       (rc = A((void**)&b->q_seed, sizeof(float) * r.na * S)) ||
       (rc = A((void**)&b->pose_tgt, sizeof(float) * std::max(1, th.n_pose * 7) * S)) ||
       (rc = A((void**)&b->pos_tgt, sizeof(float) * std::max(1, th.n_pos * 3) * S)) ||
+      (rc = A((void**)&b->rot_tgt, sizeof(float) * std::max(1, th.n_rot * 4) * S)) ||
       (rc = A((void**)&b->q_prev, sizeof(float) * r.na * S)) || (rc = A((void**)&b->cost, sizeof(float) * S)) ||
       (rc = A((void**)&b->lam, sizeof(float) * S)) || (rc = A((void**)&b->nacc, sizeof(int) * S))) {
     rk_batch_destroy(b);
@@ -1747,7 +1797,7 @@ int rk_batch_destroy(rk_batch* b) {
   rk_context* ctx = b->ctx;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
-  void* ptrs[] = {b->tt, b->td, b->q, b->q_seed, b->pose_tgt, b->pos_tgt, b->q_prev, b->cost, b->lam, b->nacc, b->trace,
+  void* ptrs[] = {b->rot_tgt, b->tt, b->td, b->q, b->q_seed, b->pose_tgt, b->pos_tgt, b->q_prev, b->cost, b->lam, b->nacc, b->trace,
                   b->acc_trace, b->q_trace, b->lam_trace, b->pb.Tj, b->pb.Sw, b->pb.J, b->pb.r, b->pb.rp, b->pb.qp, b->pb.dq, b->pb.pred,
                   b->pb.acc};
   for (void* p : ptrs)
@@ -1948,6 +1998,16 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
   return RK_OK;
 }
 
+int rk_batch_set_rotations(rk_batch* b, int32_t B, const float* rot_targets) {
+  if (!b || !rot_targets) return fail(RK_EINVAL, "null argument");
+  if (!b->th.n_rot) return fail(RK_EINVAL, "the batch has no rotation targets");
+  if (B != b->n) return fail(RK_EINVAL, "rotation targets must cover the live batch (rk_batch_set first)");
+  rk_context* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  return upload_soa(ctx, b->rot_tgt, rot_targets, B, b->th.n_rot * 4, b->capacity);
+}
+
 int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mode, int32_t tpp, float* kernel_ms) {
   if (!b || !p) return fail(RK_EINVAL, "null argument");
   if (b->n <= 0) return fail(RK_EINVAL, "batch has no problems (call rk_batch_set)");
@@ -1968,6 +2028,8 @@ int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mo
   if (b->traj_T) {
     if (mode != RK_MODE_FUSED) return fail(RK_EINVAL, "trajectory batches run the fused kernel only");
     rc = launch_traj(b, make_problems(b, from_seed), lp, it0, iters);
+  } else if ((mode == RK_MODE_PLAIN || iters == 0) && b->th.n_rot) {
+    return fail(RK_EINVAL, "rotation targets run the fused kernels only (with iters >= 1)");
   } else if (iters == 0 && (mode == RK_MODE_PLAIN || mode == RK_MODE_FUSED)) {
     rc = eval_cost(b, from_seed);
   } else if (mode == RK_MODE_PLAIN) {

@@ -7,6 +7,7 @@ caller of the reference's Levenberg-Marquardt path can switch without edits:
   Robot, RobotState            robo/robot.py:105-130, robo/robot_state.py:41-476 (q, batch)
   FrameTask                    terms/dense/frame_task.py:72-82 (residual_mode "log")
   PositionTask                 terms/dense/position_task.py:42-50
+  RotationTask                 terms/dense/rotation_task.py:25-80 (axis-angle rows, geometric Jacobian)
   PositionLimit                terms/dense/position_limit.py:204-220 (residual_mode "abs")
   SmoothnessTask               terms/dense/smoothness_task.py:41-50
   LMOptimizerConfig            opt/lm_optimizer.py:502-530 (+ OptimizerConfig, opt/optimizer.py:159-174)
@@ -235,6 +236,36 @@ class PositionTask(Task):
         return 3 * len(self.frame_indices)
 
 
+class RotationTask(Task):
+    """Orientation residual r = w * axis_angle(q_link * q_target^-1) per frame (rotation_task.py:323-349)."""
+
+    def __init__(self, robot: Robot, frame_index: Union[int, Sequence[int]], T_world_target,
+                 weight: Union[float, Sequence[float]] = 10.0, fixed_target: bool = False, name: str = ""):
+        self.robot, self.name, self.weight, self.fixed_target = robot, name, weight, fixed_target
+        self.frame_indices = _frames(frame_index)
+        k = len(self.frame_indices)
+        for f in self.frame_indices:
+            if not 0 <= f < robot.spec.num_links:
+                raise ValueError(f"frame_index {f} out of range (robot has {robot.spec.num_links} links)")
+        w = np.asarray(weight, dtype=np.float32)
+        if w.ndim == 0:
+            self.weights = np.full((k, 3), float(w), np.float32)
+        elif w.size == 3:
+            self.weights = np.tile(w.reshape(1, 3), (k, 1))
+        elif w.size == 3 * k:
+            self.weights = w.reshape(k, 3)
+        else:
+            raise ValueError(f"weight must be a scalar, 3 values or 3 per frame, got {w.size}")
+        self.T_world_target = _targets(T_world_target, k, 7)
+
+    def set_target(self, T_world_target) -> None:
+        self.T_world_target = _targets(T_world_target, len(self.frame_indices), 7)
+
+    @property
+    def residual_dim(self) -> int:
+        return 3 * len(self.frame_indices)
+
+
 class PositionLimit(Task):
     """Joint-limit hinge r_i = w_i (max(0, q_i - hi_i) + max(0, lo_i - q_i)) (position_limit.py:508-534)."""
 
@@ -341,9 +372,11 @@ class LMOptimizer:
         self.robot: Robot = self.terms[0].robot
         self._frame = [t for t in self.terms if isinstance(t, FrameTask)]
         self._pos = [t for t in self.terms if isinstance(t, PositionTask)]
+        self._rot = [t for t in self.terms if isinstance(t, RotationTask)]
         lim = [t for t in self.terms if isinstance(t, PositionLimit)]
         sm = [t for t in self.terms if isinstance(t, SmoothnessTask)]
-        other = [t for t in self.terms if not isinstance(t, (FrameTask, PositionTask, PositionLimit, SmoothnessTask))]
+        other = [t for t in self.terms
+                 if not isinstance(t, (FrameTask, PositionTask, RotationTask, PositionLimit, SmoothnessTask))]
         if other:
             raise NotImplementedError(f"terms not on this path: {[type(t).__name__ for t in other]}")
         if len(lim) > 1 or len(sm) > 1:
@@ -352,8 +385,9 @@ class LMOptimizer:
         self._smooth = sm[0] if sm else None
         pose_links = [f for t in self._frame for f in t.frame_indices]
         pos_links = [f for t in self._pos for f in t.frame_indices]
-        if len(pose_links) > MAX_POSE_TARGETS or len(pos_links) > MAX_POSITION_TARGETS:
-            raise ValueError(f"at most {MAX_POSE_TARGETS} pose and {MAX_POSITION_TARGETS} position targets")
+        rot_links = [f for t in self._rot for f in t.frame_indices]
+        if len(pose_links) + len(rot_links) > MAX_POSE_TARGETS or len(pos_links) > MAX_POSITION_TARGETS:
+            raise ValueError(f"at most {MAX_POSE_TARGETS} pose + rotation and {MAX_POSITION_TARGETS} position targets")
         self.termspec = N.TermSpec(
             pose_links=pose_links,
             pose_weights=np.concatenate([t.weights for t in self._frame]) if self._frame else None,
@@ -361,6 +395,8 @@ class LMOptimizer:
             pos_weights=np.concatenate([t.weights for t in self._pos]) if self._pos else None,
             limit_weights=self._limit.weights if self._limit else None,
             smooth_weights=self._smooth.weights if self._smooth else None,
+            rot_links=rot_links,
+            rot_weights=np.concatenate([t.weights for t in self._rot]) if self._rot else None,
         )
 
     @property
@@ -379,6 +415,7 @@ class LMOptimizer:
                 parts.append(np.broadcast_to(tg, (B,) + tg.shape[1:]))
             return np.ascontiguousarray(np.concatenate(parts, axis=1), dtype=np.float32)
 
+        self._rot_targets = stack(self._rot, 7)
         q_prev = None
         if self._smooth is not None:
             if self._smooth.prev_var is None:
@@ -398,7 +435,8 @@ class LMOptimizer:
         if B > self.max_batch_size:
             raise ValueError(f"batch size {B} exceeds the {self.max_batch_size} this optimizer was built for")
         pose, pos, q_prev = self._gather_targets(B)
-        self._batch.set(q=state.q, pose_targets=pose, pos_targets=pos, q_prev=q_prev, n=B)
+        self._batch.set(q=state.q, pose_targets=pose, pos_targets=pos, q_prev=q_prev, n=B,
+                        rot_targets=self._rot_targets)
         self.batch_size = B
         self._num_steps = self.config.max_iter if num_steps is None else num_steps
         self._var = var

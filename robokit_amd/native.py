@@ -47,6 +47,7 @@ class TermsDesc(C.Structure):
         ("n_pos", C.c_int32), ("pos_links", C.c_void_p), ("pos_weights", C.c_void_p),
         ("limit", C.c_int32), ("limit_weights", C.c_void_p),
         ("smooth", C.c_int32), ("smooth_weights", C.c_void_p),
+        ("n_rot", C.c_int32), ("rot_links", C.c_void_p), ("rot_weights", C.c_void_p),
     ]
 
 
@@ -97,7 +98,7 @@ EXPORTS = [
     "rk_last_error", "rk_abi_version", "rk_context_create", "rk_context_destroy", "rk_context_sync",
     "rk_context_mem", "rk_robot_upload", "rk_fk", "rk_link_jacobian", "rk_copy_probe", "rk_se3_op",
     "rk_batch_create",
-    "rk_batch_destroy", "rk_batch_set", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics", "rk_group_create",
+    "rk_batch_destroy", "rk_batch_set", "rk_batch_set_rotations", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics", "rk_group_create",
     "rk_group_destroy", "rk_group_set_tuning", "rk_group_autotune", "rk_group_solve", "rk_group_kernel_times",
     "rk_server_create", "rk_server_add_terms", "rk_server_add_traj", "rk_server_submit", "rk_server_poll", "rk_server_end_session",
     "rk_server_stats", "rk_server_destroy", "rk_traj_batch_create", "rk_traj_metrics",
@@ -242,7 +243,8 @@ class Context:
 
 @dataclass
 class TermSpec:
-    """Uniform residual-term structure of a batch (row order: pose, position, limit, smoothness)."""
+    """Uniform residual-term structure of a batch (row order: pose, position, rotation, limit,
+    smoothness)."""
 
     pose_links: Sequence[int] = ()
     pose_weights: Optional[np.ndarray] = None  # [n_pose, 6]
@@ -250,10 +252,30 @@ class TermSpec:
     pos_weights: Optional[np.ndarray] = None  # [n_pos, 3]
     limit_weights: Optional[np.ndarray] = None  # [na] or None (no limit term)
     smooth_weights: Optional[np.ndarray] = None  # [na] or None (no smoothness term)
+    rot_links: Sequence[int] = ()
+    rot_weights: Optional[np.ndarray] = None  # [n_rot, 3]
 
     def rows(self, na: int) -> int:
-        return 6 * len(self.pose_links) + 3 * len(self.pos_links) + na * (self.limit_weights is not None) + \
-            na * (self.smooth_weights is not None)
+        return 6 * len(self.pose_links) + 3 * len(self.pos_links) + 3 * len(self.rot_links) + \
+            na * (self.limit_weights is not None) + na * (self.smooth_weights is not None)
+
+    def desc(self, robot_id: int, na: int):
+        """(rk_terms_desc, the arrays it points at)"""
+        arrs = [
+            _i32(list(self.pose_links)),
+            _f32(self.pose_weights if self.pose_weights is not None else np.zeros((0, 6))),
+            _i32(list(self.pos_links)),
+            _f32(self.pos_weights if self.pos_weights is not None else np.zeros((0, 3))),
+            _f32(self.limit_weights if self.limit_weights is not None else np.zeros(na)),
+            _f32(self.smooth_weights if self.smooth_weights is not None else np.zeros(na)),
+            _i32(list(self.rot_links)),
+            _f32(self.rot_weights if self.rot_weights is not None else np.zeros((0, 3))),
+        ]
+        a = arrs
+        d = TermsDesc(robot_id, len(self.pose_links), _p(a[0]), _p(a[1]), len(self.pos_links), _p(a[2]), _p(a[3]),
+                      int(self.limit_weights is not None), _p(a[4]), int(self.smooth_weights is not None), _p(a[5]),
+                      len(self.rot_links), _p(a[6]), _p(a[7]))
+        return d, arrs
 
 
 class Batch:
@@ -264,18 +286,7 @@ class Batch:
         self.capacity, self.trace_iters = int(capacity), int(trace_iters)
         na = spec.num_actuated_joints
         self._na = na  # variables per problem
-        self._arrs = [
-            _i32(list(terms.pose_links)),
-            _f32(terms.pose_weights if terms.pose_weights is not None else np.zeros((0, 6))),
-            _i32(list(terms.pos_links)),
-            _f32(terms.pos_weights if terms.pos_weights is not None else np.zeros((0, 3))),
-            _f32(terms.limit_weights if terms.limit_weights is not None else np.zeros(na)),
-            _f32(terms.smooth_weights if terms.smooth_weights is not None else np.zeros(na)),
-        ]
-        a = self._arrs
-        d = TermsDesc(ctx.robot(spec), len(terms.pose_links), _p(a[0]), _p(a[1]), len(terms.pos_links), _p(a[2]),
-                      _p(a[3]), int(terms.limit_weights is not None), _p(a[4]), int(terms.smooth_weights is not None),
-                      _p(a[5]))
+        d, self._arrs = terms.desc(ctx.robot(spec), na)
         self._h = C.c_void_p()
         _check(lib().rk_batch_create(ctx.handle, C.byref(d), self.capacity, self.trace_iters, C.byref(self._h)))
         _LIVE["batch"].add(self)
@@ -301,14 +312,21 @@ class Batch:
         return self._h
 
     def set(self, q: Optional[np.ndarray] = None, pose_targets: Optional[np.ndarray] = None,
-            pos_targets: Optional[np.ndarray] = None, q_prev: Optional[np.ndarray] = None, n: Optional[int] = None):
+            pos_targets: Optional[np.ndarray] = None, q_prev: Optional[np.ndarray] = None, n: Optional[int] = None,
+            rot_targets: Optional[np.ndarray] = None):
+        """rot_targets [B, n_rot, 4] quaternions (w, x, y, z), or [B, n_rot, 7] poses (rotation used)"""
         arrs = [None if x is None else _f32(x) for x in (q, pose_targets, pos_targets, q_prev)]
         if n is None:
-            n = next(x.shape[0] for x in arrs if x is not None)
+            n = next(x.shape[0] for x in arrs + [rot_targets] if x is not None)
         if arrs[2] is not None and arrs[2].shape[-1] == 7:
             arrs[2] = _f32(arrs[2][..., :3])
         _check(lib().rk_batch_set(self._h, int(n), *[_p(x) for x in arrs]))
         self.n = int(n)
+        if rot_targets is not None:
+            rt = _f32(rot_targets)
+            if rt.shape[-1] == 7:
+                rt = _f32(rt[..., 3:])
+            _check(lib().rk_batch_set_rotations(self._h, int(n), _p(rt)))
 
     def solve(self, iters: int, params: LMParams, mode: int = MODE_FUSED, tpp: int = 0, from_seed: bool = False,
               cont: bool = False) -> float:
@@ -432,17 +450,7 @@ class Server:
 
     def add_terms(self, spec, terms: TermSpec) -> int:
         na = spec.num_actuated_joints
-        arrs = [
-            _i32(list(terms.pose_links)),
-            _f32(terms.pose_weights if terms.pose_weights is not None else np.zeros((0, 6))),
-            _i32(list(terms.pos_links)),
-            _f32(terms.pos_weights if terms.pos_weights is not None else np.zeros((0, 3))),
-            _f32(terms.limit_weights if terms.limit_weights is not None else np.zeros(na)),
-            _f32(terms.smooth_weights if terms.smooth_weights is not None else np.zeros(na)),
-        ]
-        d = TermsDesc(self.ctx.robot(spec), len(terms.pose_links), _p(arrs[0]), _p(arrs[1]), len(terms.pos_links),
-                      _p(arrs[2]), _p(arrs[3]), int(terms.limit_weights is not None), _p(arrs[4]),
-                      int(terms.smooth_weights is not None), _p(arrs[5]))
+        d, arrs = terms.desc(self.ctx.robot(spec), na)
         tid = C.c_int32()
         _check(lib().rk_server_add_terms(self._h, C.byref(d), C.byref(tid)))
         self._terms.append((na, len(terms.pose_links), len(terms.pos_links)))

@@ -65,6 +65,7 @@ class ShadowReport:
     prop_max_excess: float
     flips_gpu: int  # accept decisions differing from the FP64 twin
     flips_f32: int  # the FP32 oracle's own flips against the FP64 twin
+    worst: tuple = ()  # (step index, n_gpu, n_f32, n_f64) of the largest norm excess
 
     def ok(self) -> bool:
         return (self.norm_bad == 0 and self.prop_bad == 0
@@ -119,4 +120,6 @@ def shadow_check(gpu_trace, gpu_acc, step32, step64, rtol: float = 1e-4, atol: f
         prop_max_excess=float(prop_excess),
         flips_gpu=int((a_g != a64).sum()),
         flips_f32=int((a32 != a64).sum()),
+        worst=(int(ex.argmax()), float(n_g[ex.argmax()]), float(n32[ex.argmax()]), float(n64[ex.argmax()]))
+        if ex.size else (),
     )

@@ -271,3 +271,54 @@ def test_retarget_warm_start_resident(ctx):
         host.solve(10, RT_PARAMS, from_seed=True)
         a, h = dev.get(), host.get()
         assert np.array_equal(a["q"], h["q"]) and np.array_equal(a["cost"], h["cost"]), t
+
+
+# ----------------------------------------------------------------------------- RotationTask
+@pytest.mark.parametrize("robot,tpp", [("panda_dh", 8), ("humanoid", 32)])
+def test_rotation_task_parity(ctx, robot, tpp):
+    """PositionTask + RotationTask (the presets.basic term pair) [+ FrameTask on the humanoid] +
+    PositionLimit, LDS-staged kernels, shadow-checked against the oracle's ROTATION rows"""
+    if robot == "panda_dh":
+        spec = W.panda_dh()
+        pose_links, rot_links = [], [spec.num_links - 1]
+        pos_links = [spec.num_links - 1]
+    else:
+        spec = W.humanoid29(1)
+        ee = W.humanoid_ee_links(spec)
+        pose_links, rot_links, pos_links = ee[:2], ee[2:], ee[2:]
+    na = spec.num_actuated_joints
+    B, iters = 64, 20
+    rng = np.random.default_rng(17)
+    qt = W.uniform_q(rng, spec, B)
+    Tl, _ = O.fk(spec, qt, dtype="f64")
+    q0 = (qt + rng.normal(0, 0.3, qt.shape)).astype(np.float32)
+    pose_t = Tl[:, pose_links].astype(np.float32)
+    rot_t = Tl[:, rot_links].astype(np.float32)
+    pos_t = Tl[:, pos_links].astype(np.float32)
+    wr = np.tile(np.array([[2.0, 1.5, 1.0]], np.float32), (len(rot_links), 1))
+    wp = np.full((len(pos_links), 3), 5.0, np.float32)
+    ts = N.TermSpec(pose_links=pose_links, pose_weights=np.ones((len(pose_links), 6), np.float32) if pose_links else None,
+                    pos_links=pos_links, pos_weights=wp, limit_weights=np.ones(na, np.float32),
+                    rot_links=rot_links, rot_weights=wr)
+    b = N.Batch(ctx, spec, ts, B, trace_iters=iters)
+    b.set(q=q0, pose_targets=pose_t if pose_links else None, pos_targets=pos_t, rot_targets=rot_t)
+    b.solve(iters, C0_PARAMS, tpp=tpp)
+    gpu = b.get(traces=True)
+    q_it, lam_it = b.iterates()
+    terms = []
+    if pose_links:
+        terms.append(O.Term(O.FRAME, pose_links, pose_t, np.ones(6 * len(pose_links), np.float32)))
+    terms += [O.Term(O.POSITION, pos_links, pos_t, wp.reshape(-1)), O.Term(O.ROTATION, rot_links, rot_t, wr.reshape(-1)),
+              O.Term(O.LIMIT, (), None, np.ones(na, np.float32))]
+    rows = np.repeat(np.arange(B), iters)
+    p = O.LMParams(**C0_PARAMS.__dict__)
+    args = (spec, terms, q_it.reshape(-1, na), lam_it.reshape(-1), p)
+    # a weighted residual resolves to weight x the FP32 resolution of the unweighted one, so the
+    # absolute floor of the norm comparison (1e-6 at unit weight) scales with the largest weight
+    atol = 1e-6 * float(max(wp.max(), wr.max()))
+    rep = shadow_check(gpu["cost_trace"], gpu["accept"], O.lm_step(*args, rows=rows),
+                       O.lm_step(*args, dtype="f64", rows=rows), atol=atol)
+    print(f"\nrotation {robot} tpp={tpp}: {rep}")
+    assert rep.ok(), rep
+    with pytest.raises(ValueError):  # rotation rows run the LDS-staged kernels only
+        b.solve(iters, C0_PARAMS, mode=N.MODE_PLAIN)

@@ -138,3 +138,18 @@ def test_ik_metrics_api(ctx):
     pe3, ae3, ok3 = L.ik_metrics(robot, q0, wl.links, wl.targets[:, 0])
     np.testing.assert_allclose(pe3, pe2, atol=2e-6)
     np.testing.assert_array_equal(ok3, ok2)
+
+
+@pytest.mark.gpu
+def test_rotation_position_tasks_solve(ctx):
+    """the presets.basic pair (PositionTask + RotationTask) through the solver API reaches the targets"""
+    robot, wl, q0 = _ik_problem(seed=5, spread=0.3)
+    link = wl.links[0]
+    T = wl.targets[:, 0]
+    terms = [L.PositionTask(robot, link, T, weight=20.0), L.RotationTask(robot, link, T, weight=10.0),
+             L.PositionLimit(robot)]
+    cfg = L.LMOptimizerConfig(max_iter=50, early_stopping_interval=0, lm_lambda=1e-2, lambda_factor=10.0)
+    var = L.VarValues({L.VarKey.ROBOT: robot.state(q0)})
+    var, costs = cfg.build(terms).solve(var)
+    pe, ae, ok = L.ik_metrics(robot, var.get(L.VarKey.ROBOT).q, wl.links, T)
+    assert ok.mean() > 0.8, (pe.max(), ae.max())

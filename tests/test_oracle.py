@@ -418,3 +418,52 @@ def test_ik_metrics_definition():
     tg0 = Tl[:1, link][:, None].copy()
     tg0[0, 0, 0] += 1e-3 * 2  # well above
     assert not O.ik_metrics(spec, q[:1], [link], tg0.astype(np.float32), pos_thr=2e-3 * 0.5, dtype="f64")[2][0]
+
+
+# ----------------------------------------------------------------------------- RotationTask
+def _qmul(a, b):
+    w1, x1, y1, z1 = a
+    w2, x2, y2, z2 = b
+    q = np.array([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                  w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2])
+    return q if q[0] >= 0 else -q
+
+
+def test_rotation_task_rows():
+    """rotation_task.py:323-401: r = w * axis_angle(q_link q_target^-1) (closed form with numpy
+    quaternions); J = w * angular part of the link's spatial Jacobian; at the target orientation the
+    geometric Jacobian is the derivative of the residual (central differences)"""
+    spec = W.humanoid29(2)
+    rng = np.random.default_rng(30)
+    links = [spec.num_links - 1, 5]
+    w = np.array([2.0, 1.0, 0.5, 1.5, 0.7, 1.0], np.float32)
+    q = W.uniform_q(rng, spec, 2).astype(np.float64)
+    tg = O.fk(spec, W.uniform_q(rng, spec, 2), dtype="f64")[0][:, links].astype(np.float32)
+    terms = [O.Term(O.ROTATION, links, tg, w)]
+    for b in range(2):
+        r, J, *_ = O.residual_jacobian(spec, terms, b, q[b], dtype="f64")
+        Tl, _ = O.fk(spec, q[b], dtype="f64")
+        for f, link in enumerate(links):
+            qt = tg[b, f, 3:].astype(np.float64)
+            qe = _qmul(Tl[0, link, 3:], np.r_[qt[0], -qt[1:]])
+            ang = 2 * np.arctan2(np.linalg.norm(qe[1:]), qe[0])
+            aa = qe[1:] / (np.sin(ang / 2) / ang)
+            np.testing.assert_allclose(r[3 * f:3 * f + 3], w[3 * f:3 * f + 3] * aa, atol=1e-12)
+            Js = O.link_jacobian(spec, q[b], link, "spatial", dtype="f64")[0]
+            np.testing.assert_allclose(J[3 * f:3 * f + 3], w[3 * f:3 * f + 3, None] * Js[3:], atol=1e-12)
+    # at the target orientation: finite differences of the residual = the geometric Jacobian
+    tg0 = O.fk(spec, q[:1], dtype="f64")[0][:, links].astype(np.float64)
+    terms0 = [O.Term(O.ROTATION, links, tg0.astype(np.float32), w)]
+    q0 = q[0].copy()
+    # re-derive the float32-rounded target exactly: evaluate at the orientation of the rounded target
+    r0, J0, *_ = O.residual_jacobian(spec, terms0, 0, q0, dtype="f64")
+    assert np.abs(r0).max() < 1e-6
+    h = 1e-6
+    Jfd = np.zeros_like(J0)
+    for a in range(spec.num_actuated_joints):
+        qp, qm = q0.copy(), q0.copy()
+        qp[a] += h
+        qm[a] -= h
+        Jfd[:, a] = (O.residual_jacobian(spec, terms0, 0, qp, dtype="f64")[0]
+                     - O.residual_jacobian(spec, terms0, 0, qm, dtype="f64")[0]) / (2 * h)
+    np.testing.assert_allclose(J0, Jfd, atol=2e-5)
