@@ -49,6 +49,8 @@ def variant_name(tpp: int, spec, n_pos: int) -> str:
     """kernel-variant label of a bucket (as the counter summary names them)"""
     if tpp == -64:
         return "V_W012" if n_pos else "V_W40"
+    if tpp == -32:
+        return "V_H012" if n_pos else "V_H40"
     if tpp == 1:
         return f"V_T{spec.num_actuated_joints}"
     return f"G{tpp}"

@@ -132,9 +132,12 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
 #define RK_CONTINUE 0x200  /* flag: continue the last solve (current q and damping, iteration count and
                             * traces carry on) -- Optimizer.step after reset (opt/optimizer.py:280-287) */
 /* threads_per_problem codes: 1 = one lane per problem, register-resident (serial arms);
- * RK_TPP_WAVE = one wavefront per problem, register-resident rows (trees, <= 32 dofs);
+ * RK_TPP_WAVE = one wavefront per problem, register-resident rows (trees, <= 64 joints, <= 32 dofs);
+ * RK_TPP_HALF = two problems per wavefront, 32 lanes each, register-resident rows (trees,
+ *               <= 32 joints, <= 32 dofs);
  * 4, 8, 16, 32, 64 = that many lanes per problem, LDS-staged (any robot the batch accepts) */
 #define RK_TPP_WAVE (-64)
+#define RK_TPP_HALF (-32)
 #define RK_USE_GRAPH 0x1   /* rk_group_solve flag: capture the launch sequence in a hipGraph and replay it */
 
 /*

@@ -570,13 +570,18 @@ __global__ __launch_bounds__(64) void k_fused(Problems P, LMParams lp, int it0, 
 // kernel variants: (threads per problem, data placement)
 //   V_S*/V_L*: G lanes per problem, LDS-staged (small / large LDS configuration)
 //   V_T*:      one lane per problem, register-resident, serial arms with NA dofs
-//   V_W*:      one wavefront per problem, register-resident rows (trees, <= 32 dofs)
-enum Variant { V_S4 = 0, V_S8, V_S16, V_L16, V_L32, V_L64, V_T5, V_T6, V_T7, V_T8, V_W40, V_W012, NVARIANT };
+//   V_W*:      one wavefront per problem, register-resident rows (trees, <= 64 joints, <= 32 dofs)
+//   V_H*:      two problems per wavefront, 32 lanes each, otherwise as V_W* (trees, <= 32 joints)
+enum Variant {
+  V_S4 = 0, V_S8, V_S16, V_L16, V_L32, V_L64, V_T5, V_T6, V_T7, V_T8, V_W40, V_W012, V_H40, V_H012, NVARIANT
+};
 __host__ __device__ constexpr bool variant_wave(int v) { return v == V_W40 || v == V_W012; }
+__host__ __device__ constexpr bool variant_half(int v) { return v == V_H40 || v == V_H012; }
 __host__ __device__ constexpr bool variant_thread(int v) { return v >= V_T5 && v <= V_T8; }
 __host__ __device__ constexpr int variant_G(int v) {
   return variant_thread(v) ? 1
          : variant_wave(v) ? 64
+         : variant_half(v) ? 32
          : v == V_S4       ? 4
          : v == V_S8       ? 8
          : v == V_S16      ? 16
@@ -590,8 +595,10 @@ __host__ __device__ constexpr int variant_ppw(int v) { return variant_thread(v) 
 // LDS bytes per workgroup of a variant
 __host__ __device__ constexpr size_t variant_lds(int v) {
   return variant_thread(v) ? 0
-         : v == V_W40      ? sizeof(WaveLDS<4, 0>)
-         : v == V_W012     ? sizeof(WaveLDS<0, 12>)
+         : v == V_W40      ? wave_lds_bytes<4, 0, 64>()
+         : v == V_W012     ? wave_lds_bytes<0, 12, 64>()
+         : v == V_H40      ? wave_lds_bytes<4, 0, 32>()
+         : v == V_H012     ? wave_lds_bytes<0, 12, 32>()
          : variant_small(v) ? (WAVE / variant_G(v)) * sizeof(Shared<CfgS>)
                             : (WAVE / variant_G(v)) * sizeof(Shared<CfgL>);
 }
@@ -619,16 +626,21 @@ __device__ __forceinline__ void run_tile(const Bucket* bk, int local, const LMPa
   else if constexpr (V == V_T6) thread_solve<6, 8>(P, local * WAVE + threadIdx.x, lp, it0, iters);
   else if constexpr (V == V_T7) thread_solve<7, 9>(P, local * WAVE + threadIdx.x, lp, it0, iters);
   else if constexpr (V == V_T8) thread_solve<8, 10>(P, local * WAVE + threadIdx.x, lp, it0, iters);
-  else if constexpr (V == V_W40) wave_solve<4, 0>(P, local, lp, it0, iters, lds);
-  else wave_solve<0, 12>(P, local, lp, it0, iters, lds);
+  else if constexpr (V == V_W40) wave_solve<4, 0, 64>(P, local, lp, it0, iters, lds);
+  else if constexpr (V == V_W012) wave_solve<0, 12, 64>(P, local, lp, it0, iters, lds);
+  else if constexpr (V == V_H40) wave_solve<4, 0, 32>(P, local, lp, it0, iters, lds);
+  else wave_solve<0, 12, 32>(P, local, lp, it0, iters, lds);
 }
 
 // Persistent uneven-batch kernel of one variant: each workgroup (one wavefront) pulls tiles of
 // the variant's buckets from its queue until the queue is empty.  A group launches one such
 // kernel per variant it holds, concurrently (forked streams, one graph); a kernel per variant
 // gives each its own register allocation instead of the worst case of all.
+// register budget of a variant's persistent kernel: waves per SIMD the allocation must allow
+__host__ __device__ constexpr int variant_waves_per_eu(int v) { return variant_wave(v) || variant_half(v) ? 2 : 1; }
+
 template <int V>
-__global__ __launch_bounds__(64) void k_persistent(const Bucket* __restrict__ buckets, int nb, int total_tiles,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(variant_waves_per_eu(V)))) void k_persistent(const Bucket* __restrict__ buckets, int nb, int total_tiles,
                                                    int* counter, LMParams lp, int it0, int iters) {
   // static LDS of the variant's size: its offsets fold into the LDS instructions (a dynamic
   // base made every address a live scalar)
@@ -1192,14 +1204,15 @@ static bool fits_thread(const Terms& th, const Robot& r) {
   return r.chain && r.na >= 5 && r.na <= 8 && r.nj <= r.na + 2 && th.n_pose <= 1 && th.n_pos == 0 && th.n_rot == 0;
 }
 
-// register wave-per-problem kernels: trees with <= 64 joints, <= 32 dofs, each dof driven by
-// <= 2 joints; <= 4 pose targets and no position targets, or <= 12 position targets and no pose
-static int wave_kind(const Terms& th, const Robot& r) {
-  if (r.nj > 64 || r.na > WV_NA || th.n_rot) return -1;
+// register wave kernels: trees with <= 32 dofs, each dof driven by <= 2 joints; <= 4 pose
+// targets and no position targets, or <= 12 position targets and no pose targets.  hw = 64: one
+// problem per wavefront, <= 64 joints; hw = 32: two per wavefront, <= 32 joints.
+static int wave_kind(const Terms& th, const Robot& r, int hw) {
+  if (r.nj > hw || r.na > WV_NA || th.n_rot) return -1;
   for (int a = 0; a < r.na; ++a)
     if (r.col_start[a + 1] - r.col_start[a] > 2) return -1;
-  if (th.n_pos == 0 && th.n_pose <= 4) return V_W40;
-  if (th.n_pose == 0 && th.n_pos <= 12) return V_W012;
+  if (th.n_pos == 0 && th.n_pose <= 4) return hw == 64 ? V_W40 : V_H40;
+  if (th.n_pose == 0 && th.n_pos <= 12) return hw == 64 ? V_W012 : V_H012;
   return -1;
 }
 
@@ -1215,14 +1228,19 @@ static int pick_variant(const rk_context* ctx, const Terms& th, int G, int* v) {
   if (G == 0) {
     auto it = ctx->tuned.find(signature(th));
     if (it != ctx->tuned.end()) G = it->second;
-    else G = fits_thread(th, r) ? 1 : wave_kind(th, r) >= 0 ? RK_TPP_WAVE : small ? 8 : 64;
+    else
+      G = fits_thread(th, r)               ? 1
+          : wave_kind(th, r, 32) >= 0      ? RK_TPP_HALF
+          : wave_kind(th, r, 64) >= 0      ? RK_TPP_WAVE
+          : small                          ? 8
+                                           : 64;
   }
   if (G == 1 && fits_thread(th, r)) {
     *v = V_T5 + (r.na - 5);
     return RK_OK;
   }
-  if (G == RK_TPP_WAVE && wave_kind(th, r) >= 0) {
-    *v = wave_kind(th, r);
+  if ((G == RK_TPP_WAVE || G == RK_TPP_HALF) && wave_kind(th, r, -G) >= 0) {
+    *v = wave_kind(th, r, -G);
     return RK_OK;
   }
   if (small && (G == 4 || G == 8 || G == 16)) {
@@ -1243,7 +1261,8 @@ static void legal_tpps(const rk_batch* b, std::vector<int>& out) {
   const Robot& r = b->ctx->robots[b->th.robot];
   out.clear();
   if (fits_thread(b->th, r)) out.push_back(1);
-  if (wave_kind(b->th, r) >= 0) out.push_back(RK_TPP_WAVE);
+  if (wave_kind(b->th, r, 64) >= 0) out.push_back(RK_TPP_WAVE);
+  if (wave_kind(b->th, r, 32) >= 0) out.push_back(RK_TPP_HALF);
   if (fits_small(b->th, r)) out.insert(out.end(), {4, 8, 16, 32, 64});
   else out.insert(out.end(), {16, 32, 64});
 }
@@ -1278,8 +1297,14 @@ static int launch_variant(hipStream_t st, int v, const Problems& P, const LMPara
     case V_T6: hipLaunchKernelGGL((k_thread<6, 8>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_T7: hipLaunchKernelGGL((k_thread<7, 9>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_T8: hipLaunchKernelGGL((k_thread<8, 10>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
-    case V_W40: hipLaunchKernelGGL((k_wave<4, 0>), dim3(P.n), dim3(WAVE), 0, st, P, lp, it0, iters); break;
-    default: hipLaunchKernelGGL((k_wave<0, 12>), dim3(P.n), dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_W40: hipLaunchKernelGGL((k_wave<4, 0, 64>), dim3(P.n), dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_W012: hipLaunchKernelGGL((k_wave<0, 12, 64>), dim3(P.n), dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_H40:
+      hipLaunchKernelGGL((k_wave<4, 0, 32>), dim3((P.n + 1) / 2), dim3(WAVE), 0, st, P, lp, it0, iters);
+      break;
+    default:
+      hipLaunchKernelGGL((k_wave<0, 12, 32>), dim3((P.n + 1) / 2), dim3(WAVE), 0, st, P, lp, it0, iters);
+      break;
   }
   HIP_TRY(hipGetLastError());
   return RK_OK;
@@ -2224,7 +2249,9 @@ static int variant_grid_any(rk_context* ctx, int v, int wg_per_cu, size_t lds, i
     case V_T7: return variant_grid<V_T7>(ctx, wg_per_cu, lds, tiles, grid);
     case V_T8: return variant_grid<V_T8>(ctx, wg_per_cu, lds, tiles, grid);
     case V_W40: return variant_grid<V_W40>(ctx, wg_per_cu, lds, tiles, grid);
-    default: return variant_grid<V_W012>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_W012: return variant_grid<V_W012>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_H40: return variant_grid<V_H40>(ctx, wg_per_cu, lds, tiles, grid);
+    default: return variant_grid<V_H012>(ctx, wg_per_cu, lds, tiles, grid);
   }
 }
 
@@ -2245,6 +2272,8 @@ static void launch_persistent(int v, dim3 grid, size_t lds, hipStream_t st, cons
     RK_LAUNCH_P(V_T8)
     RK_LAUNCH_P(V_W40)
     RK_LAUNCH_P(V_W012)
+    RK_LAUNCH_P(V_H40)
+    RK_LAUNCH_P(V_H012)
 #undef RK_LAUNCH_P
     default: break;
   }
@@ -2400,7 +2429,10 @@ int rk_group_kernel_times(rk_group* g, int32_t cap, int32_t* tpp, float* ms, int
   const int m = (int)g->last_var.size();
   *n = m;
   for (int k = 0; k < m && k < cap; ++k) {
-    if (tpp) tpp[k] = variant_wave(g->last_var[k]) ? RK_TPP_WAVE : variant_G(g->last_var[k]);
+    if (tpp)
+      tpp[k] = variant_wave(g->last_var[k])   ? RK_TPP_WAVE
+               : variant_half(g->last_var[k]) ? RK_TPP_HALF
+                                              : variant_G(g->last_var[k]);
     if (ms) HIP_TRY(hipEventElapsedTime(&ms[k], g->ev_k0[k], g->ev_k1[k]));
     if (batch_mask) batch_mask[k] = (int32_t)g->last_mask[k];
   }

@@ -1,15 +1,18 @@
-// rk_wave.h -- one wavefront per problem, register-resident (included by rk_lm.hip inside namespace rk).
+// rk_wave.h -- register-resident wave kernels for trees (included by rk_lm.hip inside namespace rk).
 //
-// For trees up to 64 joints / 32 dofs (e.g. a 29-dof humanoid).  Lane roles inside the wave:
+// HW lanes per problem: HW = 64 (one problem per wavefront, trees up to 64 joints) or HW = 32
+// (two problems per wavefront, trees up to 32 joints).  For trees up to 32 dofs (e.g. a 29-dof
+// humanoid).  Lane roles inside a problem's lanes (lane = threadIdx.x mod HW):
 //   lane j  (< nj): joint j -- its local transform, world pose, motion-subspace column;
 //   lane a  (< na): dof a   -- q_a, its Jacobian column (all pose / position rows), row a of
 //                              J^T J + lam I and of its Cholesky factor;
 //   lane t        : pose target t (then position target t - n_pose): residual rows and the
-//                   per-target Jacobian blocks, broadcast to the wave through LDS.
+//                   per-target Jacobian blocks, broadcast to the problem's lanes through LDS.
 // World poses come from pointer jumping over the parent tree (ceil(log2(depth+1)) rounds of
-// __shfl of the ancestor's partial product); the factorisation is the right-looking Cholesky on
-// register rows, entries of other rows read with v_readlane (no LDS round trip); the backward
-// solve reads the factor's columns after one transpose through LDS.
+// __shfl of the ancestor's partial product).  The factorisation is the right-looking Cholesky on
+// register rows; each step stages its column (and the right-hand side entry) in LDS, from where
+// every lane reads the entries it needs as broadcasts, and the forward substitution rides along.
+// The backward solve reads the factor's columns after one transpose through LDS.
 // Row blocks of a target are structurally zero in the columns of dofs that do not move the
 // target's link; those (target, column) blocks are skipped in J^T J with a wave-uniform mask
 // (exact zeros only, so the per-entry sequence equals the dense one).
@@ -20,20 +23,33 @@ __device__ __forceinline__ float wv_readlane(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// sum over the 64 lanes by an xor butterfly: every lane ends with the bitwise-same total (each
-// pairwise add sees the same two operands, in commuted order)
+// value of lane k of the caller's problem (HW lanes per problem)
+template <int HW>
+__device__ __forceinline__ float wv_bcast(float v, int k) {
+  if constexpr (HW == 64) {
+    return wv_readlane(v, k);
+  } else {
+    const float lo = wv_readlane(v, k), hi = wv_readlane(v, k + 32);
+    return (threadIdx.x & 32) ? hi : lo;
+  }
+}
+
+// sum over a problem's HW lanes by an xor butterfly: every lane ends with the bitwise-same total
+// (each pairwise add sees the same two operands, in commuted order)
+template <int HW = 64>
 __device__ __forceinline__ float wv_sum(float v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  for (int o = HW / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
 
+template <int HW = 64>
 __device__ __forceinline__ Pose wv_shfl_pose(const Pose& P, int src) {
   Pose o;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) o.t[k] = __shfl(P.t[k], src);
+  for (int k = 0; k < 3; ++k) o.t[k] = __shfl(P.t[k], src, HW);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) o.q[k] = __shfl(P.q[k], src);
+  for (int k = 0; k < 4; ++k) o.q[k] = __shfl(P.q[k], src, HW);
   return o;
 }
 
@@ -48,12 +64,29 @@ __device__ __forceinline__ void wv_opaque_v(T& v) { asm volatile("" : "+v"(v)); 
 template <int NP, int NQ>
 struct WaveLDS {
   static constexpr int RP = 6 * NP + 3 * NQ > 0 ? 6 * NP + 3 * NQ : 1;
+  static constexpr int JS = ((RP + 3) & ~3) + 4;  // row stride of J: 16-byte rows, 4-way at worst
   float tg[NP > 0 ? NP : 1][36];        // pose-target Jacobian blocks
   float rows[RP + 2 * WV_NA];           // residual rows, term order
   float plink[NQ > 0 ? NQ : 1][4];      // world position of each position-target link
-  float J[WV_NA][RP + 1];               // Jacobian columns (dof-major), padded
-  float L[WV_NA][WV_NA + 1];            // Cholesky factor rows for the transpose
+  alignas(16) float col[2][WV_NA];      // Cholesky step: column k of the trailing matrix (2 buffers)
+  alignas(16) float rhs[2][WV_NA];      //                and the right-hand side
+  union {
+    alignas(16) float J[WV_NA][JS];     // Jacobian columns (dof-major)
+    alignas(16) float L[WV_NA][36];     // Cholesky factor rows, for the transposed backward solve
+  };
 };
+
+// per-problem LDS stride in floats: 16-float aligned and an odd multiple of 16, so the two
+// problems of a wavefront (HW = 32) read their broadcasts from disjoint banks
+template <int NP, int NQ>
+constexpr int wave_lds_stride() {
+  const int f = (int)((sizeof(WaveLDS<NP, NQ>) + 63) / 64) * 16;
+  return (f / 16) % 2 ? f : f + 16;
+}
+template <int NP, int NQ, int HW>
+constexpr size_t wave_lds_bytes() {
+  return (size_t)(64 / HW) * wave_lds_stride<NP, NQ>() * sizeof(float);
+}
 
 // per-lane joint data, loaded once per problem
 struct WaveJoint {
@@ -63,11 +96,12 @@ struct WaveJoint {
 };
 
 // pointer-jumping FK: lane j ends with T_world_joint(j) (identity base)
+template <int HW>
 __device__ __forceinline__ Pose wave_fk(const WaveJoint& jd, int nj, int nrounds, float q_mine) {
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & (HW - 1);
   // q of the dof that drives this joint
   const int src = jd.mimic >= 0 ? jd.mimic : (jd.act >= 0 ? jd.act : 0);
-  const float qs = __shfl(q_mine, src);
+  const float qs = __shfl(q_mine, src, HW);
   float pos = jd.mimic >= 0 ? qs * jd.mul + jd.off : (jd.act >= 0 ? qs : 0.f);
   Pose X = identity_pose();
   if (jd.type == 2) {
@@ -81,8 +115,8 @@ __device__ __forceinline__ Pose wave_fk(const WaveJoint& jd, int nj, int nrounds
   int ptr = lane < nj ? jd.parent : -1;
   for (int r = 0; r < nrounds; ++r) {
     const int sp = ptr < 0 ? 0 : ptr;
-    const Pose Ap = wv_shfl_pose(A, sp);
-    const int pp = __shfl(ptr, sp);
+    const Pose Ap = wv_shfl_pose<HW>(A, sp);
+    const int pp = __shfl(ptr, sp, HW);
     if (ptr >= 0) {
       A = compose(Ap, A);
       ptr = pp;
@@ -91,12 +125,17 @@ __device__ __forceinline__ Pose wave_fk(const WaveJoint& jd, int nj, int nrounds
   return A;
 }
 
-template <int NP, int NQ>
-__device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const LMParams& lp, int it0, int iters,
+// tile = index of the wavefront's problem group: problems tile * (64 / HW) + threadIdx.x / HW
+template <int NP, int NQ, int HW>
+__device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const LMParams& lp, int it0, int iters,
                                            char* lds_raw) {
+  static_assert(HW == 64 || HW == 32, "HW lanes per problem: 64 or 32");
   constexpr int RP = 6 * NP + 3 * NQ;
-  WaveLDS<NP, NQ>* sh = reinterpret_cast<WaveLDS<NP, NQ>*>(lds_raw);
-  int lane = threadIdx.x & 63;
+  const int slot = threadIdx.x / HW;
+  WaveLDS<NP, NQ>* sh =
+      reinterpret_cast<WaveLDS<NP, NQ>*>(reinterpret_cast<float*>(lds_raw) + slot * wave_lds_stride<NP, NQ>());
+  const int s_idx = tile * (64 / HW) + slot;
+  int lane = threadIdx.x & (HW - 1);
   RobotK* rb = as_const(Pg.robot);
   TermsK* tm = as_const(Pg.terms);
   const int S = Pg.stride;
@@ -193,12 +232,12 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const 
   float lam = it0 == 0 ? lp.lm_lambda : Pg.lam[s];
   int nacc = it0 == 0 ? 0 : Pg.nacc[s];
   float cost = 0.f;
-  Pose T = wave_fk(jd, nj, nrounds, q);
+  Pose T = wave_fk<HW>(jd, nj, nrounds, q);
 
   // residual rows of the lane's target at world poses Tw (all lanes hold their joint's pose);
   // JAC also writes the pose target's Jacobian blocks / the position link's point to LDS
   auto target_rows = [&](const Pose& Tw, bool jac) {
-    const Pose Tl = wv_shfl_pose(Tw, tj < 0 ? 0 : tj);
+    const Pose Tl = wv_shfl_pose<HW>(Tw, tj < 0 ? 0 : tj);
     const Pose Tf = tj < 0 ? identity_pose() : Tl;
     if (lane < n_pose) {
       float r6[6];
@@ -233,10 +272,10 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const 
       if (has_sm) ss = sh->rows[off_s + lane] * sh->rows[off_s + lane];
     }
     float c = 0.f;
-    if (n_pose) c = c + 0.5f * wv_sum(sp);
-    if (n_pos) c = c + 0.5f * wv_sum(sq);
-    if (has_lim) c = c + 0.5f * wv_sum(sl);
-    if (has_sm) c = c + 0.5f * wv_sum(ss);
+    if (n_pose) c = c + 0.5f * wv_sum<HW>(sp);
+    if (n_pos) c = c + 0.5f * wv_sum<HW>(sq);
+    if (has_lim) c = c + 0.5f * wv_sum<HW>(sl);
+    if (has_sm) c = c + 0.5f * wv_sum<HW>(ss);
     return c;
   };
 
@@ -260,8 +299,8 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const 
     float Sc0[6], Sc1[6];
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
-      Sc0[r] = __shfl(S6[r], cj0 < 0 ? 0 : cj0);
-      Sc1[r] = __shfl(S6[r], cj1 < 0 ? 0 : cj1);
+      Sc0[r] = __shfl(S6[r], cj0 < 0 ? 0 : cj0, HW);
+      Sc1[r] = __shfl(S6[r], cj1 < 0 ? 0 : cj1, HW);
     }
     // ---- residual rows and target blocks at q
     target_rows(T, true);
@@ -310,7 +349,11 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const 
       for (int r = 0; r < RP; ++r) sh->J[lane][r] = Jc[r];
     }
     __syncthreads();
-    // ---- row `lane` of J^T J + lam I and of J^T r (pose term, then position term, then diagonals)
+    // ---- row `lane` of J^T J + lam I and of J^T r (pose term, then position term, then diagonals).
+    // Dense over the 32 columns, branch-free: a (target, column) block whose column does not move
+    // the target's link holds exact zeros and adds nothing, and without branches the broadcast
+    // reads of every column issue back to back (masked blocks cost a wait each and register
+    // copies at every merge).
     float A[WV_NA];
 #pragma unroll
     for (int j = 0; j < WV_NA; ++j) A[j] = 0.f;
@@ -319,15 +362,10 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const 
 #pragma unroll
       for (int t = 0; t < NP; ++t) {
         if (t < n_pose) {
-          unsigned long long m = cmask[t];
-          wv_opaque(m);
 #pragma unroll
-          for (int j = 0; j < WV_NA; ++j) {
-            if ((m >> j) & 1ull) {
+          for (int j = 0; j < WV_NA; ++j)
 #pragma unroll
-              for (int r = 0; r < 6; ++r) A[j] += Jc[6 * t + r] * sh->J[j][6 * t + r];
-            }
-          }
+            for (int r = 0; r < 6; ++r) A[j] += Jc[6 * t + r] * sh->J[j][6 * t + r];
 #pragma unroll
           for (int r = 0; r < 6; ++r) gv += Jc[6 * t + r] * sh->rows[6 * t + r];
         }
@@ -341,15 +379,10 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const 
 #pragma unroll
       for (int u = 0; u < NQ; ++u) {
         if (u < n_pos) {
-          unsigned long long m = cmask[NP + u];
-          wv_opaque(m);
 #pragma unroll
-          for (int j = 0; j < WV_NA; ++j) {
-            if ((m >> j) & 1ull) {
+          for (int j = 0; j < WV_NA; ++j)
 #pragma unroll
-              for (int r = 0; r < 3; ++r) A2[j] += Jc[6 * NP + 3 * u + r] * sh->J[j][6 * NP + 3 * u + r];
-            }
-          }
+            for (int r = 0; r < 3; ++r) A2[j] += Jc[6 * NP + 3 * u + r] * sh->J[j][6 * NP + 3 * u + r];
 #pragma unroll
           for (int r = 0; r < 3; ++r) g2 += Jc[6 * NP + 3 * u + r] * sh->rows[Rf + 3 * u + r];
         }
@@ -358,68 +391,100 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const 
       for (int j = 0; j < WV_NA; ++j) A[j] = A[j] + A2[j];
       gv = gv + g2;
     }
-#pragma unroll
-    for (int j = 0; j < WV_NA; ++j) {
-      if (j == lane) {
-        if (has_lim) A[j] = A[j] + jl * jl;
-        if (has_sm) A[j] = A[j] + js * js;
-      }
-    }
     if (has_lim) gv = gv + jl * rl;
     if (has_sm) gv = gv + js * rs;
-    // padding rows na..31: identity (their columns in rows < na are exact zeros: no Jacobian
-    // column and no diagonal term reaches them).  Lane-wise select, no per-column test.
+    // diagonal terms (limit, smoothness, then lam) enter row k's pivot when column k is staged.
+    // Rows na..31 (padding) and their columns are exact zeros: no Jacobian column, diagonal term
+    // or right-hand side reaches them.
     const bool pad = lane >= na;
-#pragma unroll
-    for (int j = 0; j < WV_NA; ++j) {
-      A[j] = pad ? (j == lane ? 1.f : 0.f) : A[j];
-      if (j == lane) A[j] = A[j] + lam;
-    }
-    if (pad) gv = 0.f;
+    const float dl = jl * jl, ds = js * js;
     // ---- right-looking Cholesky on register rows (lane i holds row i; entries above the
-    // diagonal are left as scratch and never read)
-    float dd = 1.f;  // lane k: L_kk
+    // diagonal are left as scratch and never read), forward substitution L y = -g alongside.
+    // Step k stages the trailing column k and the right-hand side in LDS; every lane reads the
+    // pivot A_kk, b_k and the entries A_jk (j > k) back as broadcasts (16-byte quads):
+    //   L_kk = sqrt(A_kk), L_ik = A_ik / L_kk, y_k = b_k / L_kk,
+    //   A_ij -= (L_ik / L_kk) A_jk,  b_i -= L_ik y_k        (rows i > k, columns k < j <= i).
+    // Two buffers: step k + 1 writes the other one, and its wait orders step k's reads before
+    // step k + 2 overwrites theirs.  Padding rows k >= na stage a zero column with pivot lam:
+    // they keep y = 0, give dq = 0 and change no other row.
+    // The pivot's square root and reciprocal are the hardware's (v_sqrt_f32, v_rcp_f32, 1 ulp).
+    float b = -gv, y = 0.f, dinv = 1.f;
+    float keep = 0.f;  // sums the quad entries below k: every quad read stays one 16-byte read
+    const bool stage = HW == 32 || lane < WV_NA;
+    float* const wcol = &sh->col[0][lane & (WV_NA - 1)];
+    float* const wrhs = &sh->rhs[0][lane & (WV_NA - 1)];
 #pragma unroll
     for (int k = 0; k < WV_NA; ++k) {
-      const float d = sqrtf(wv_readlane(A[k], k));
-      if (lane == k) dd = d;
-      A[k] = lane == k ? d : A[k] / d;
+      {
+        const int buf = k & 1;
+        int ln = lane;
+        wv_opaque_v(ln);  // one lane == k test per step (kept for all 32 steps they spilled SGPRs)
+        const bool mine = ln == k;
+        float akk = A[k];
+        if (has_lim) akk = akk + dl;
+        if (has_sm) akk = akk + ds;
+        akk = akk + lam;
+        if (stage) {
+          wcol[buf * WV_NA] = mine ? akk : A[k];
+          wrhs[buf * WV_NA] = b;
+        }
+        __syncthreads();
+        float cv[WV_NA];  // staged column, entries k..31 (quads from the one holding k)
+        const float4* c4 = reinterpret_cast<const float4*>(__builtin_assume_aligned(sh->col[buf], 16));
 #pragma unroll
-      for (int j = k + 1; j < WV_NA; ++j) A[j] -= A[k] * wv_readlane(A[k], j);
+        for (int q4 = k / 4; q4 < WV_NA / 4; ++q4) {
+          const float4 v = c4[q4];
+          cv[4 * q4] = v.x;
+          cv[4 * q4 + 1] = v.y;
+          cv[4 * q4 + 2] = v.z;
+          cv[4 * q4 + 3] = v.w;
+        }
+#pragma unroll
+        for (int j = (k / 4) * 4; j < k; ++j) keep += cv[j];
+        const float d = __builtin_amdgcn_sqrtf(cv[k]);
+        const float di = __builtin_amdgcn_rcpf(d);
+        const float yk = sh->rhs[buf][k] * di;
+        const float lik = A[k] * di;
+        const float u = lik * di;
+        y = mine ? yk : y;
+        dinv = mine ? di : dinv;
+        A[k] = mine ? d : lik;
+        b -= lik * yk;
+#pragma unroll
+        for (int j = k + 1; j < WV_NA; ++j) {
+          A[j] -= u * cv[j];
+          // pinned here: left free, the compiler sank every update to the step consuming column
+          // j and kept all multipliers and staged entries live (hundreds of registers)
+          wv_opaque_v(A[j]);
+        }
+      }
     }
-    const float dinv = 1.f / dd;
-    // forward substitution L y = -g, column by column; step k's update of rows <= k only
-    // touches values already consumed
-    float b = -gv, y = 0.f;
+    wv_opaque_v(keep);
+    // backward substitution L^T x = y with the factor's rows transposed through LDS (the
+    // Jacobian columns sharing that space were consumed before the first step's wait)
+    if (stage) {
 #pragma unroll
-    for (int k = 0; k < WV_NA; ++k) {
-      const float t = b * dinv;
-      const float yk = wv_readlane(t, k);
-      if (lane == k) y = t;
-      b -= A[k] * yk;
-    }
-    // backward substitution L^T x = y with the factor's rows transposed through LDS
-    if (lane < WV_NA) {
-#pragma unroll
-      for (int j = 0; j < WV_NA; ++j) sh->L[lane][j] = A[j];
+      for (int j = 0; j < WV_NA; ++j) sh->L[lane & (WV_NA - 1)][j] = A[j];
     }
     __syncthreads();
     float c = y, dq = 0.f;
-    const int lc = lane < WV_NA ? lane : 0;
+    const int lc = lane & (WV_NA - 1);
 #pragma unroll
     for (int k = WV_NA - 1; k >= 0; --k) {
+      int ln = lane;
+      wv_opaque_v(ln);
       const float t = c * dinv;
-      const float xk = wv_readlane(t, k);
-      if (lane == k) dq = t;
+      const float xk = wv_bcast<HW>(t, k);
+      dq = ln == k ? t : dq;
       c -= sh->L[k][lc] * xk;
     }
     if (pad) dq = 0.f;
     // predicted reduction 1/2 d.(lam d - g), summed over the dofs
-    const float pr = wv_sum(pad ? 0.f : dq * (lam * dq - gv));
+    const float pr = wv_sum<HW>(pad ? 0.f : dq * (lam * dq - gv));
     const float pred = 0.5f * pr;
     // ---- proposal
     const float qp = q + dq;
-    const Pose Tp = wave_fk(jd, nj, nrounds, qp);
+    const Pose Tp = wave_fk<HW>(jd, nj, nrounds, qp);
     __syncthreads();  // the rows of the iterate have been consumed
     target_rows(Tp, false);
     if (lane < na) {
@@ -453,8 +518,8 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int s_idx, const 
   __syncthreads();
 }
 
-template <int NP, int NQ>
+template <int NP, int NQ, int HW>
 __global__ __launch_bounds__(64) void k_wave(Problems P, LMParams lp, int it0, int iters) {
-  __shared__ __align__(16) char lds[sizeof(WaveLDS<NP, NQ>)];
-  wave_solve<NP, NQ>(P, blockIdx.x, lp, it0, iters, lds);
+  __shared__ __align__(16) char lds[wave_lds_bytes<NP, NQ, HW>()];
+  wave_solve<NP, NQ, HW>(P, blockIdx.x, lp, it0, iters, lds);
 }

@@ -330,7 +330,7 @@ class LMOptimizerConfig(OptimizerConfig):
     cg_check_interval: int = 4
     threads_per_problem: int = 0
     """Kernel choice (no reference counterpart): 0 library default / autotuned, 1 one lane per problem,
-    -64 one wavefront per problem, 4..64 lanes per problem staged in LDS."""
+    -64 one wavefront per problem, -32 two problems per wavefront, 4..64 lanes per problem staged in LDS."""
 
     def build(self, terms: Sequence[Task], device=None, active_dof_mask=None) -> "LMOptimizer":
         return LMOptimizer(terms, self, device, active_dof_mask)

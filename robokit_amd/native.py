@@ -20,6 +20,7 @@ LIB_PATH = PKG / "librkhip.so"
 RK_OK, RK_EINVAL, RK_EHIP, RK_ECAPACITY, RK_ENOMEM, RK_EBUSY = range(6)
 MODE_FUSED, MODE_PLAIN = 0, 1
 TPP_WAVE = -64  # one wavefront per problem, register rows
+TPP_HALF = -32  # two problems per wavefront (32 lanes each), register rows
 FROM_SEED, CONTINUE, USE_GRAPH = 0x100, 0x200, 0x1
 SE3_OPS = {"compose": (0, 7), "inverse": (1, 7), "log": (2, 6), "jlog": (3, 36), "adjoint": (4, 36), "apply": (5, 3)}
 

@@ -97,7 +97,8 @@ def test_ik_parity_serial(ctx, mode, tpp, robot):
     _check_parity(ctx, wl, mode, robot, tpp)
 
 
-@pytest.mark.parametrize("mode,tpp", [(N.MODE_FUSED, N.TPP_WAVE), (N.MODE_FUSED, 32), (N.MODE_PLAIN, 0)])
+@pytest.mark.parametrize("mode,tpp", [(N.MODE_FUSED, N.TPP_WAVE), (N.MODE_FUSED, N.TPP_HALF), (N.MODE_FUSED, 32),
+                                      (N.MODE_PLAIN, 0)])
 def test_ik_parity_humanoid(ctx, mode, tpp):
     spec = W.humanoid29(0)
     wl = W.ik_batch(spec, W.humanoid_ee_links(spec), 64, seed=12)
@@ -117,6 +118,18 @@ def test_fused_and_plain_agree(ctx):
     np.testing.assert_allclose(f["cost_trace"][:, 0], p["cost_trace"][:, 0], rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(f["cost_trace"][:, 1], p["cost_trace"][:, 1], rtol=1e-3, atol=1e-7)
     assert np.sum(f["ok"] != p["ok"]) <= 1
+
+
+def test_half_wave_matches_wave(ctx):
+    """two problems per wavefront (32 lanes each) run the one-problem-per-wavefront arithmetic
+    lane for lane; the 64-lane butterfly sums only add exact zeros from lanes 32..63: same bits,
+    for an odd problem count too (the last wavefront's second slot is idle)"""
+    spec = W.humanoid29(0)
+    wl = W.ik_batch(spec, W.humanoid_ee_links(spec), 99, seed=21)
+    w = _gpu_ik(ctx, wl, N.MODE_FUSED, tpp=N.TPP_WAVE, iters=20)
+    h = _gpu_ik(ctx, wl, N.MODE_FUSED, tpp=N.TPP_HALF, iters=20)
+    assert np.array_equal(w["q"], h["q"]) and np.array_equal(w["cost_trace"], h["cost_trace"])
+    assert np.array_equal(w["accept"], h["accept"])
 
 
 @pytest.mark.parametrize("robot", ["panda_dh", "humanoid"])
@@ -217,7 +230,8 @@ def _oracle_retarget_terms(spec, links, targets, q_prev):
             O.Term(O.SMOOTH, (), None, np.full(na, RT_SMOOTH_W, np.float32), q_prev)]
 
 
-@pytest.mark.parametrize("mode,tpp", [(N.MODE_FUSED, N.TPP_WAVE), (N.MODE_FUSED, 64), (N.MODE_PLAIN, 0)])
+@pytest.mark.parametrize("mode,tpp", [(N.MODE_FUSED, N.TPP_WAVE), (N.MODE_FUSED, N.TPP_HALF), (N.MODE_FUSED, 64),
+                                      (N.MODE_PLAIN, 0)])
 def test_retarget_parity(ctx, mode, tpp):
     """12 link-position targets + smoothness to the previous frame + limits on the 29-dof tree:
     frames t of a smooth clip, each seeded (and smoothed) at the clip's frame t-1 (+ noise)."""
