@@ -117,32 +117,124 @@ __device__ __forceinline__ void tri_index(int k, int& i, int& j) {
 // per-problem building blocks shared by the fused and plain forms
 // ---------------------------------------------------------------------------------------------
 
-// Pose residual of a target: r = w * log(T_f^-1 T_tgt) (frame_task.py:505-554).
-__device__ __forceinline__ void pose_target(const Pose& Tf, const Pose& Tt, const float* w, float* r6) {
-  Pose E = compose(inverse(Tf), Tt);
-  float e[6];
-  se3_log(E, e);
-#pragma unroll
-  for (int i = 0; i < 6; ++i) r6[i] = w[i] * e[i];
+// FrameTask (log mode) quantities of one target, evaluated from the half-angle of the error
+// rotation.  E = T_f^-1 T_tgt; with n = |q_xyz(E)| and r = |q(E)|:
+//   half = atan2(n, q_w)             (quaternion_to_axis_angle, rotation_conversions.py:226-246)
+//   sin(half) = n / r, cos(half) = q_w / r,
+//   sin(a) = 2 sin(half) cos(half),  1 - cos(a) = 2 sin(half)^2       (a = |log_so3(E)|)
+// which replaces the sincos(a) of se3_log_map (se3_kernels.py:230-309) and the sincos(a / 2),
+// sincos(a) of se3_jlog (se3_kernels.py:358-428) by identities: one atan2, no sin / cos calls.
+// The thresholds (eps = 1e-4) and the branch structure are the reference's.
+struct TargetLog {
+  Pose Tfi;      // T_f^-1
+  float e[6];    // log(E) = (v, w)
+  float a, sh, ch;  // |w|, sin(half), cos(half)
+};
+
+__device__ __forceinline__ void target_log(const Pose& Tf, const Pose& Tt, TargetLog& L) {
+  const float eps = 1e-4f;
+  L.Tfi = inverse(Tf);
+  const Pose E = compose(L.Tfi, Tt);
+  const float* q = E.q;
+  const float n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+  const float n = sqrtf(n2);
+  const float half = atan2f(n, q[0]);
+  const float angle = 2.f * half;
+  const float rinv = 1.f / sqrtf(n2 + q[0] * q[0]);
+  L.sh = n * rinv;
+  L.ch = q[0] * rinv;
+  const float s = fabsf(angle) < 1e-6f ? 0.5f - (angle * angle) / 48.f : L.sh / angle;  // sin(half) / angle
+  float w[3] = {q[1] / s, q[2] / s, q[3] / s};
+  const float a = sqrtf(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  float c2 = 1.f / 12.f;
+  if (a >= eps) {
+    const float omc = 2.f * L.sh * L.sh;
+    if (fabsf(omc) >= eps) c2 = (1.f / (a * a)) * (1.f - (a * L.ch) / (2.f * L.sh));
+  }
+  float v[3];
+  ikk_apply(-0.5f, c2, w, E.t, v);
+  L.e[0] = v[0]; L.e[1] = v[1]; L.e[2] = v[2];
+  L.e[3] = w[0]; L.e[4] = w[1]; L.e[5] = w[2];
+  L.a = a;
 }
 
-// Per-target Jacobian blocks: jlog(T_tgt^-1 T_f) and Ad(T_f^-1) (frame_task.py:391-456, 557-621),
-// as R(T_f^-1) 9, t(T_f^-1) 3, jlog A 9, B 9, weights 6.
-__device__ __forceinline__ void pose_blocks(const Pose& Tf, const Pose& Tt, const float* w, float* tg) {
-  Pose Tfi = inverse(Tf);
-  Pose F = compose(inverse(Tt), Tf);
-  Mat3 A, B;
-  se3_jlog_blocks(F, A, B);
-  Mat3 R = quat_to_mat(Tfi.q);
+// Jacobian blocks of the target (frame_task.py:391-456, 557-621): jlog(F), F = T_tgt^-1 T_f = E^-1
+// (its rotation vector is -w), and Ad(T_f^-1), packed as R(T_f^-1) 9, t(T_f^-1) 3, A 9, B 9, w 6
+// with jlog = [[A, B], [0, A]], A = so3_jac_left_inv(w_F)^T (so3_kernels.py:13-40).
+__device__ __forceinline__ void target_blocks(const Pose& Tt, const Pose& Tf, const TargetLog& L, const float* wt,
+                                              float* tg) {
+  const float eps = 1e-4f;
+  const float w[3] = {-L.e[3], -L.e[4], -L.e[5]};
+  float p[3];  // translation of F
+  {
+    const Pose F = compose(inverse(Tt), Tf);
+    p[0] = F.t[0]; p[1] = F.t[1]; p[2] = F.t[2];
+  }
+  const float t2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  const bool taylor = t2 < eps;
+  const float th = L.a;
+  const float coeff = taylor ? 1.f / 12.f : (1.f - th * L.ch / (2.f * L.sh)) / t2;
+  const float K2[9] = {w[0] * w[0] - t2, w[0] * w[1], w[0] * w[2],
+                       w[1] * w[0], w[1] * w[1] - t2, w[1] * w[2],
+                       w[2] * w[0], w[2] * w[1], w[2] * w[2] - t2};
+  const float K[9] = {0.f, -w[2], w[1], w[2], 0.f, -w[0], -w[1], w[0], 0.f};
+  float A[9], B[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) A[i] = ((i % 4) == 0 ? 1.f : 0.f) + 0.5f * K[i] + coeff * K2[i];
+  const float sk[9] = {0.f, -p[2], p[1], p[2], 0.f, -p[0], -p[1], p[0], 0.f};
+  if (taylor) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) B[i] = 0.5f * sk[i];
+  } else {
+    const float th_inv = 1.f / th;
+    const float th2_inv = th_inv * th_inv;
+    const float st = 2.f * L.sh * L.ch;
+    const float inv_2_2ct = 1.f / (4.f * L.sh * L.sh);  // 1 / (2 (1 - cos th))
+    const float beta = th2_inv - st * th_inv * inv_2_2ct;
+    const float bdot = -2.f * th2_inv * th2_inv + (1.f + st * th_inv) * th2_inv * inv_2_2ct;
+    const float wTp = w[0] * p[0] + w[1] * p[1] + w[2] * p[2];
+    float v3[3], Cm[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) v3[i] = bdot * wTp * w[i] - (t2 * bdot + 2.f * beta) * p[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        Cm[i * 3 + j] = v3[i] * w[j] + beta * (w[i] * p[j]) + (i == j ? wTp * beta : 0.f) + 0.5f * sk[i * 3 + j];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        B[i * 3 + j] = Cm[i * 3 + 0] * A[0 * 3 + j] + Cm[i * 3 + 1] * A[1 * 3 + j] + Cm[i * 3 + 2] * A[2 * 3 + j];
+  }
+  const Mat3 R = quat_to_mat(L.Tfi.q);
 #pragma unroll
   for (int i = 0; i < 9; ++i) tg[i] = R.m[i];
-  tg[9] = Tfi.t[0]; tg[10] = Tfi.t[1]; tg[11] = Tfi.t[2];
+  tg[9] = L.Tfi.t[0]; tg[10] = L.Tfi.t[1]; tg[11] = L.Tfi.t[2];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) tg[12 + i] = A.m[i];
+  for (int i = 0; i < 9; ++i) tg[12 + i] = A[i];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) tg[21 + i] = B.m[i];
+  for (int i = 0; i < 9; ++i) tg[21 + i] = B[i];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) tg[30 + i] = w[i];
+  for (int i = 0; i < 6; ++i) tg[30 + i] = wt[i];
+}
+
+// Pose residual of a target r = w * log(T_f^-1 T_tgt) (frame_task.py:505-554); with tg != nullptr
+// also its Jacobian blocks.
+__device__ __forceinline__ void pose_target(const Pose& Tf, const Pose& Tt, const float* w, float* r6,
+                                            float* tg = nullptr) {
+  TargetLog L;
+  target_log(Tf, Tt, L);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) r6[i] = w[i] * L.e[i];
+  if (tg) target_blocks(Tt, Tf, L, w, tg);
+}
+
+// the Jacobian blocks alone
+__device__ __forceinline__ void pose_blocks(const Pose& Tf, const Pose& Tt, const float* w, float* tg) {
+  TargetLog L;
+  target_log(Tf, Tt, L);
+  target_blocks(Tt, Tf, L, w, tg);
 }
 
 // spatial column of link `link` for actuated dof a: sum of S_j * map[j][a] over ancestor joints
@@ -293,8 +385,7 @@ __device__ __forceinline__ void residual_rows(const Robot* rb, const Terms* tm, 
     Pose Tf = link_pose<C>(rb, T, tm->pose_link[t]);
     Pose Tt;
     load_pose(sh->tp[t], Tt);
-    pose_target(Tf, Tt, tm->pose_w[t], sh->r + 6 * t);
-    if (with_jac) pose_blocks(Tf, Tt, tm->pose_w[t], sh->tg[t]);
+    pose_target(Tf, Tt, tm->pose_w[t], sh->r + 6 * t, with_jac ? sh->tg[t] : nullptr);
   }
   for (int u = lane; u < tm->n_pos; u += G) {
     Pose Tf = link_pose<C>(rb, T, tm->pos_link[u]);

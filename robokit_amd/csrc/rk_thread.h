@@ -71,8 +71,7 @@ __device__ __forceinline__ void chain_rows(RobotK* rb, TermsK* tm, const Pose& T
     float w[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) w[i] = tm->pose_w[0][i];
-    pose_target(Tf, tgt, w, r_pose);
-    if (JAC) pose_blocks(Tf, tgt, w, tg);
+    pose_target(Tf, tgt, w, r_pose, JAC ? tg : nullptr);
   }
   if (tm->limit) {
 #pragma unroll

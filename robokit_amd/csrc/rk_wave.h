@@ -241,10 +241,9 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     const Pose Tf = tj < 0 ? identity_pose() : Tl;
     if (lane < n_pose) {
       float r6[6];
-      pose_target(Tf, tgt, pw, r6);
+      pose_target(Tf, tgt, pw, r6, jac ? sh->tg[lane < NP ? lane : 0] : nullptr);
 #pragma unroll
       for (int i = 0; i < 6; ++i) sh->rows[6 * lane + i] = r6[i];
-      if (jac) pose_blocks(Tf, tgt, pw, sh->tg[lane < NP ? lane : 0]);
     } else if (lane < n_pose + n_pos) {
       const int u = lane - n_pose;
 #pragma unroll

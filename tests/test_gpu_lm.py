@@ -204,7 +204,7 @@ def test_autotune_keeps_results(ctx):
     for bk, b in zip(mb.buckets, batches):
         b.set(q=bk.q_seed, pose_targets=bk.targets)
     chosen = g.autotune(5, C0_PARAMS)
-    assert len(chosen) == 3 and all(c in (1, -64, 4, 8, 16, 32, 64) for c in chosen)
+    assert len(chosen) == 3 and all(c in (1, N.TPP_WAVE, N.TPP_HALF, 4, 8, 16, 32, 64) for c in chosen)
     g.solve(10, C0_PARAMS, use_graph=True)
     for r, b in zip(ref, batches):
         assert np.array_equal(b.get()["q"], r)
