@@ -278,6 +278,13 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     return c;
   };
 
+  // J^T J (row `lane`, without the diagonal terms), J^T r, the squared limit / smoothness
+  // Jacobian diagonal and the cost at the current iterate.  A rejected step leaves the iterate,
+  // hence all of these, unchanged: they are rebuilt only when some problem of the wavefront
+  // moved (the others recompute the same bits).  Two problems per wavefront only.
+  float Aj[WV_NA];
+  float gv = 0.f, dl = 0.f, ds = 0.f;
+  bool moved = true;
   for (int it = 0; it < iters; ++it) {
     // lane-derived predicates recomputed per iteration (hoisted, each is a live SGPR pair)
     wv_opaque_v(lane);
@@ -288,6 +295,7 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
       if (lane < na) Pg.q_trace[(gi * na + lane) * S + s] = q;
       if (lane == 0) Pg.lam_trace[gi * S + s] = lam;
     }
+    if (HW == 64 || __ballot(moved)) {  // (one problem per wavefront: no registers to spare)
     // ---- motion subspace (lane j), the dof's joint columns (lane a)
     float S6[6];
     {
@@ -353,10 +361,10 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     // the target's link holds exact zeros and adds nothing, and without branches the broadcast
     // reads of every column issue back to back (masked blocks cost a wait each and register
     // copies at every merge).
-    float A[WV_NA];
+    float* const A = Aj;
 #pragma unroll
     for (int j = 0; j < WV_NA; ++j) A[j] = 0.f;
-    float gv = 0.f;
+    gv = 0.f;
     if (NP > 0 && n_pose) {
 #pragma unroll
       for (int t = 0; t < NP; ++t) {
@@ -392,11 +400,16 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     }
     if (has_lim) gv = gv + jl * rl;
     if (has_sm) gv = gv + js * rs;
+    dl = jl * jl;
+    ds = js * js;
+    }  // moved
+    float A[WV_NA];
+#pragma unroll
+    for (int j = 0; j < WV_NA; ++j) A[j] = Aj[j];
     // diagonal terms (limit, smoothness, then lam) enter row k's pivot when column k is staged.
     // Rows na..31 (padding) and their columns are exact zeros: no Jacobian column, diagonal term
     // or right-hand side reaches them.
     const bool pad = lane >= na;
-    const float dl = jl * jl, ds = js * js;
     // ---- right-looking Cholesky on register rows (lane i holds row i; entries above the
     // diagonal are left as scratch and never read), forward substitution L y = -g alongside.
     // Step k stages the trailing column k and the right-hand side in LDS; every lane reads the
@@ -494,6 +507,7 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     const float cp = term_cost_lds();
     if (valid && Pg.trace && gi == 0 && lane == 0) Pg.trace[s] = cost;
     const bool acc = lm_accept(lp, cost, cp, pred, lam);
+    moved = acc;
     if (acc) {
       ++nacc;
       q = qp;

@@ -191,6 +191,9 @@ def test_group_uneven_batch_matches_single(ctx):
 
 
 def test_autotune_keeps_results(ctx):
+    """the tuner's trial solves leave the batches as set; the tuned graph solve equals an eager
+    solve with the same kernel choice from the same seeds (the tuner may pick a different kernel
+    family than the default, whose rounding differs)"""
     mb = W.config1(seed=1, total=400)
     batches = []
     for bk in mb.buckets:
@@ -198,16 +201,18 @@ def test_autotune_keeps_results(ctx):
         b.set(q=bk.q_seed, pose_targets=bk.targets)
         batches.append(b)
     g = N.Group(ctx, batches)
-    ref = []
-    g.solve(10, C0_PARAMS, use_graph=False)
-    ref = [b.get()["q"] for b in batches]
-    for bk, b in zip(mb.buckets, batches):
-        b.set(q=bk.q_seed, pose_targets=bk.targets)
     chosen = g.autotune(5, C0_PARAMS)
+    print(f"\nautotune chose {chosen}")
     assert len(chosen) == 3 and all(c in (1, N.TPP_WAVE, N.TPP_HALF, 4, 8, 16, 32, 64) for c in chosen)
     g.solve(10, C0_PARAMS, use_graph=True)
-    for r, b in zip(ref, batches):
-        assert np.array_equal(b.get()["q"], r)
+    tuned = [b.get()["q"] for b in batches]
+    for bk, b in zip(mb.buckets, batches):
+        b.set(q=bk.q_seed, pose_targets=bk.targets)
+    g.set_tuning(chosen)
+    g.solve(10, C0_PARAMS, use_graph=False)
+    for k, (t, b) in enumerate(zip(tuned, batches)):
+        ref = b.get()["q"]
+        assert np.array_equal(t, ref), (k, np.abs(t - ref).max())
 
 
 # ----------------------------------------------------------------------------- retargeting (C2 shape)
