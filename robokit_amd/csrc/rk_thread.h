@@ -195,20 +195,25 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
       g[i] = gv;
     }
     cost = chain_cost<NA>(tm, r_pose, r_lim, r_sm);
-    // ---- damped Cholesky (left-looking; the per-entry sequence of the LDS kernel) and solves
+    // ---- damped Cholesky (left-looking) and solves.  Each pivot's square root and reciprocal
+    // are the hardware's (v_sqrt_f32, v_rcp_f32, 1 ulp) and the column scales by the reciprocal:
+    // a correctly rounded division is a dependent chain of ~10 instructions, and this kernel is
+    // bound by dependency latency.
+    float dinv[NA];
 #pragma unroll
     for (int j = 0; j < NA; ++j) {
       float sd = H[j][j] + lam;
 #pragma unroll
       for (int k = 0; k < j; ++k) sd -= H[j][k] * H[j][k];
-      const float d = sqrtf(sd);
+      const float d = __builtin_amdgcn_sqrtf(sd);
+      dinv[j] = __builtin_amdgcn_rcpf(d);
       H[j][j] = d;
 #pragma unroll
       for (int i = j + 1; i < NA; ++i) {
         float v = H[i][j];
 #pragma unroll
         for (int k = 0; k < j; ++k) v -= H[i][k] * H[j][k];
-        H[i][j] = v / d;
+        H[i][j] = v * dinv[j];
       }
     }
     float dq[NA];
@@ -217,14 +222,14 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
       float v = -g[i];
 #pragma unroll
       for (int k = 0; k < i; ++k) v -= H[i][k] * dq[k];
-      dq[i] = v / H[i][i];
+      dq[i] = v * dinv[i];
     }
 #pragma unroll
     for (int i = NA - 1; i >= 0; --i) {
       float v = dq[i];
 #pragma unroll
       for (int k = i + 1; k < NA; ++k) v -= H[k][i] * dq[k];
-      dq[i] = v / H[i][i];
+      dq[i] = v * dinv[i];
     }
     float pr = 0.f;
 #pragma unroll

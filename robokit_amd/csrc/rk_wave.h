@@ -68,8 +68,8 @@ struct WaveLDS {
   float tg[NP > 0 ? NP : 1][36];        // pose-target Jacobian blocks
   float rows[RP + 2 * WV_NA];           // residual rows, term order
   float plink[NQ > 0 ? NQ : 1][4];      // world position of each position-target link
-  alignas(16) float col[2][WV_NA];      // Cholesky step: column k of the trailing matrix (2 buffers)
-  alignas(16) float rhs[2][WV_NA];      //                and the right-hand side
+  float4 blk[2][WV_NA];                 // Cholesky block step: row i's 4 entries of the block columns
+  alignas(16) float rhs[2][WV_NA];      //                      and the right-hand side (2 buffers)
   union {
     alignas(16) float J[WV_NA][JS];     // Jacobian columns (dof-major)
     alignas(16) float L[WV_NA][36];     // Cholesky factor rows, for the transposed backward solve
@@ -411,67 +411,104 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     // or right-hand side reaches them.
     const bool pad = lane >= na;
     // ---- right-looking Cholesky on register rows (lane i holds row i; entries above the
-    // diagonal are left as scratch and never read), forward substitution L y = -g alongside.
-    // Step k stages the trailing column k and the right-hand side in LDS; every lane reads the
-    // pivot A_kk, b_k and the entries A_jk (j > k) back as broadcasts (16-byte quads):
-    //   L_kk = sqrt(A_kk), L_ik = A_ik / L_kk, y_k = b_k / L_kk,
-    //   A_ij -= (L_ik / L_kk) A_jk,  b_i -= L_ik y_k        (rows i > k, columns k < j <= i).
-    // Two buffers: step k + 1 writes the other one, and its wait orders step k's reads before
-    // step k + 2 overwrites theirs.  Padding rows k >= na stage a zero column with pivot lam:
-    // they keep y = 0, give dq = 0 and change no other row.
-    // The pivot's square root and reciprocal are the hardware's (v_sqrt_f32, v_rcp_f32, 1 ulp).
+    // diagonal are left as scratch and never read), forward substitution L y = -g alongside,
+    // four columns B = [k0, k0 + 4) per step.  Each row stages its entries a_iB (and b_i) in LDS
+    // and reads back the diagonal block D = A_BB, b_B and every trailing row's a_jB as
+    // broadcasts.  Every lane factors D = L_D L_D^T itself (4 x 4), then
+    //   L_iB = L_D^-1 a_iB,  y_B = L_D^-1 b_B,  b_i -= L_iB . y_B,
+    //   A_ij -= (D^-1 a_iB) . a_jB = (L_D^-T L_iB) . a_jB        (rows i, j past the block)
+    // (the block form of the column-by-column recurrence: one LDS round trip per four columns).
+    // Two buffers: step s + 1 writes the other one, and its wait orders step s's reads before
+    // step s + 2 overwrites theirs.  Padding rows k >= na stage zeros with pivot lam: they keep
+    // y = 0, give dq = 0 and change no other row.  Square roots and reciprocals of the pivots are
+    // the hardware's (v_sqrt_f32, v_rcp_f32, 1 ulp).
     float b = -gv, y = 0.f, dinv = 1.f;
-    float keep = 0.f;  // sums the quad entries below k: every quad read stays one 16-byte read
     const bool stage = HW == 32 || lane < WV_NA;
-    float* const wcol = &sh->col[0][lane & (WV_NA - 1)];
-    float* const wrhs = &sh->rhs[0][lane & (WV_NA - 1)];
+    float dsum = lam;  // this row's diagonal terms: limit, smoothness, then lam
+    if (has_sm) dsum = ds + dsum;
+    if (has_lim) dsum = dl + dsum;
 #pragma unroll
-    for (int k = 0; k < WV_NA; ++k) {
-      {
-        const int buf = k & 1;
-        int ln = lane;
-        wv_opaque_v(ln);  // one lane == k test per step (kept for all 32 steps they spilled SGPRs)
-        const bool mine = ln == k;
-        float akk = A[k];
-        if (has_lim) akk = akk + dl;
-        if (has_sm) akk = akk + ds;
-        akk = akk + lam;
-        if (stage) {
-          wcol[buf * WV_NA] = mine ? akk : A[k];
-          wrhs[buf * WV_NA] = b;
-        }
-        __syncthreads();
-        float cv[WV_NA];  // staged column, entries k..31 (quads from the one holding k)
-        const float4* c4 = reinterpret_cast<const float4*>(__builtin_assume_aligned(sh->col[buf], 16));
+    for (int s4 = 0; s4 < WV_NA / 4; ++s4) {
+      const int k0 = 4 * s4, buf = s4 & 1;
+      int ln = lane;
+      wv_opaque_v(ln);  // lane tests recomputed per step (kept for all steps they spilled SGPRs)
+      const int m = ln - k0;  // position of this row in the block (0..3 inside)
+      float4 a4 = make_float4(A[k0], A[k0 + 1], A[k0 + 2], A[k0 + 3]);
+      a4.x += m == 0 ? dsum : 0.f;
+      a4.y += m == 1 ? dsum : 0.f;
+      a4.z += m == 2 ? dsum : 0.f;
+      a4.w += m == 3 ? dsum : 0.f;
+      if (stage) {
+        sh->blk[buf][lane & (WV_NA - 1)] = a4;
+        sh->rhs[buf][lane & (WV_NA - 1)] = b;
+      }
+      __syncthreads();
+      const float4 d0 = sh->blk[buf][k0], d1 = sh->blk[buf][k0 + 1], d2 = sh->blk[buf][k0 + 2],
+                   d3 = sh->blk[buf][k0 + 3];
+      const float4 bB = *reinterpret_cast<const float4*>(&sh->rhs[buf][k0]);
+      // L_D (lower) and its reciprocal diagonal
+      const float l00 = __builtin_amdgcn_sqrtf(d0.x), i00 = __builtin_amdgcn_rcpf(l00);
+      const float l10 = d1.x * i00, l20 = d2.x * i00, l30 = d3.x * i00;
+      const float l11 = __builtin_amdgcn_sqrtf(d1.y - l10 * l10), i11 = __builtin_amdgcn_rcpf(l11);
+      const float l21 = (d2.y - l20 * l10) * i11, l31 = (d3.y - l30 * l10) * i11;
+      const float l22 = __builtin_amdgcn_sqrtf(d2.z - l20 * l20 - l21 * l21), i22 = __builtin_amdgcn_rcpf(l22);
+      const float l32 = (d3.z - l30 * l20 - l31 * l21) * i22;
+      const float l33 = __builtin_amdgcn_sqrtf(d3.w - l30 * l30 - l31 * l31 - l32 * l32),
+                  i33 = __builtin_amdgcn_rcpf(l33);
+      // y_B
+      const float y0 = bB.x * i00;
+      const float y1 = (bB.y - l10 * y0) * i11;
+      const float y2 = (bB.z - l20 * y0 - l21 * y1) * i22;
+      const float y3 = (bB.w - l30 * y0 - l31 * y1 - l32 * y2) * i33;
+      // this row's L_iB and D^-1 a_iB
+      const float li0 = a4.x * i00;
+      const float li1 = (a4.y - li0 * l10) * i11;
+      const float li2 = (a4.z - li0 * l20 - li1 * l21) * i22;
+      const float li3 = (a4.w - li0 * l30 - li1 * l31 - li2 * l32) * i33;
+      const float g3 = li3 * i33;
+      const float g2 = (li2 - l32 * g3) * i22;
+      const float g1 = (li1 - l21 * g2 - l31 * g3) * i11;
+      const float g0 = (li0 - l10 * g1 - l20 * g2 - l30 * g3) * i00;
+      b -= li0 * y0 + li1 * y1 + li2 * y2 + li3 * y3;
+      // a row m of the block solves L_D x = D e_m = L_D (row m of L_D)^T: its L_iB is its row of
+      // L_D (to rounding; the entries past the diagonal are scratch).  It keeps y_m and 1 / L_mm.
+      A[k0] = li0;
+      A[k0 + 1] = li1;
+      A[k0 + 2] = li2;
+      A[k0 + 3] = li3;
+      const bool s0 = m == 0, s1 = m == 1, s2 = m == 2, s3 = m == 3;
+      y = s0 ? y0 : y;
+      y = s1 ? y1 : y;
+      y = s2 ? y2 : y;
+      y = s3 ? y3 : y;
+      dinv = s0 ? i00 : dinv;
+      dinv = s1 ? i11 : dinv;
+      dinv = s2 ? i22 : dinv;
+      dinv = s3 ? i33 : dinv;
+      // trailing rows in chunks of 4, software-pipelined: the next chunk's reads issue before this
+      // chunk's updates.  The updated entries are pinned at each chunk's end: left free, the
+      // compiler sank every update to the step consuming its column and kept all multipliers and
+      // staged entries live (hundreds of registers).
+      if (k0 + 4 < WV_NA) {
+        float4 cur[4], nxt[4];
 #pragma unroll
-        for (int q4 = k / 4; q4 < WV_NA / 4; ++q4) {
-          const float4 v = c4[q4];
-          cv[4 * q4] = v.x;
-          cv[4 * q4 + 1] = v.y;
-          cv[4 * q4 + 2] = v.z;
-          cv[4 * q4 + 3] = v.w;
-        }
+        for (int u = 0; u < 4; ++u) cur[u] = sh->blk[buf][k0 + 4 + u];
 #pragma unroll
-        for (int j = (k / 4) * 4; j < k; ++j) keep += cv[j];
-        const float d = __builtin_amdgcn_sqrtf(cv[k]);
-        const float di = __builtin_amdgcn_rcpf(d);
-        const float yk = sh->rhs[buf][k] * di;
-        const float lik = A[k] * di;
-        const float u = lik * di;
-        y = mine ? yk : y;
-        dinv = mine ? di : dinv;
-        A[k] = mine ? d : lik;
-        b -= lik * yk;
+        for (int jc = k0 + 4; jc < WV_NA; jc += 4) {
+          if (jc + 4 < WV_NA) {
 #pragma unroll
-        for (int j = k + 1; j < WV_NA; ++j) {
-          A[j] -= u * cv[j];
-          // pinned here: left free, the compiler sank every update to the step consuming column
-          // j and kept all multipliers and staged entries live (hundreds of registers)
-          wv_opaque_v(A[j]);
+            for (int u = 0; u < 4; ++u) nxt[u] = sh->blk[buf][jc + 4 + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            A[jc + u] -= g0 * cur[u].x + g1 * cur[u].y + g2 * cur[u].z + g3 * cur[u].w;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) wv_opaque_v(A[jc + u]);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
         }
       }
     }
-    wv_opaque_v(keep);
     // backward substitution L^T x = y with the factor's rows transposed through LDS (the
     // Jacobian columns sharing that space were consumed before the first step's wait)
     if (stage) {
