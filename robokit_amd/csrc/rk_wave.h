@@ -19,6 +19,33 @@
 
 constexpr int WV_NA = 32;  // padded dof count: rows na..31 of the system are the identity
 
+typedef float wv_v32f __attribute__((ext_vector_type(32)));
+
+// J^T J of the two problems of a wavefront (32 lanes each) on the matrix cores.
+// v_mfma_f32_32x32x1_2b_f32 multiplies two independent blocks: lane l supplies entry l % 32 of
+// block l / 32's column and row vectors, so with row r of J (lane = dof) as both operands each
+// instruction adds J_r^T J_r of both problems; NR of them chained in row order sum the rows.
+// Block b's result D_b[i][j] lands in lane j + 32 ((i / 4) % 2), register 16 b + (i % 4) + 4 (i / 8).
+// D_b is symmetric, so lane j of either half holds row j's entries at the columns i with
+// (i / 4) % 2 == its half; 16 v_permlane32_swap (upper half of register m <-> lower half of
+// register 16 + m) bring the other half's entries of its own block over.  Afterwards every lane
+// holds its problem's row: column i in register (i % 4) + 4 (i / 8) + 16 ((i / 4) % 2).
+template <int NR>
+__device__ __forceinline__ void wv_jtj_mfma(const float* Jr, float* A) {
+  wv_v32f acc = {};
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc = __builtin_amdgcn_mfma_f32_32x32x1f32(Jr[r], Jr[r], acc, 0, 0, 0);
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc[m]), __float_as_uint(acc[16 + m]), false,
+                                                     false);
+    acc[m] = __uint_as_float(sw[0]);
+    acc[16 + m] = __uint_as_float(sw[1]);
+  }
+#pragma unroll
+  for (int i = 0; i < WV_NA; ++i) A[i] = acc[(i % 4) + 4 * (i / 8) + 16 * ((i / 4) % 2)];
+}
+
 __device__ __forceinline__ float wv_readlane(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
@@ -351,52 +378,86 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
         for (int i = 0; i < 3; ++i) Jc[6 * NP + 3 * tt + i] = c3[i];
       }
     }
-    if (lane < WV_NA) {
-#pragma unroll
-      for (int r = 0; r < RP; ++r) sh->J[lane][r] = Jc[r];
-    }
-    __syncthreads();
-    // ---- row `lane` of J^T J + lam I and of J^T r (pose term, then position term, then diagonals).
-    // Dense over the 32 columns, branch-free: a (target, column) block whose column does not move
-    // the target's link holds exact zeros and adds nothing, and without branches the broadcast
-    // reads of every column issue back to back (masked blocks cost a wait each and register
-    // copies at every merge).
     float* const A = Aj;
-#pragma unroll
-    for (int j = 0; j < WV_NA; ++j) A[j] = 0.f;
     gv = 0.f;
-    if (NP > 0 && n_pose) {
+    if constexpr (HW == 32) {
+      // ---- row `lane` of J^T J (pose term, then position term) on the matrix cores; J^T r
+      if constexpr (NP > 0) {
+        wv_jtj_mfma<6 * NP>(Jc, A);
+      } else {
 #pragma unroll
-      for (int t = 0; t < NP; ++t) {
-        if (t < n_pose) {
+        for (int j = 0; j < WV_NA; ++j) A[j] = 0.f;
+      }
+      if constexpr (NQ > 0) {
+        float A2[WV_NA];
+        wv_jtj_mfma<3 * NQ>(Jc + 6 * NP, A2);
 #pragma unroll
-          for (int j = 0; j < WV_NA; ++j)
+        for (int j = 0; j < WV_NA; ++j) A[j] = A[j] + A2[j];
+      }
+      if (NP > 0 && n_pose) {
 #pragma unroll
-            for (int r = 0; r < 6; ++r) A[j] += Jc[6 * t + r] * sh->J[j][6 * t + r];
+        for (int t = 0; t < NP; ++t)
+          if (t < n_pose) {
 #pragma unroll
-          for (int r = 0; r < 6; ++r) gv += Jc[6 * t + r] * sh->rows[6 * t + r];
+            for (int r = 0; r < 6; ++r) gv += Jc[6 * t + r] * sh->rows[6 * t + r];
+          }
+      }
+      if (NQ > 0 && n_pos) {
+        float g2 = 0.f;
+#pragma unroll
+        for (int u = 0; u < NQ; ++u)
+          if (u < n_pos) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) g2 += Jc[6 * NP + 3 * u + r] * sh->rows[Rf + 3 * u + r];
+          }
+        gv = gv + g2;
+      }
+    } else {
+      if (lane < WV_NA) {
+#pragma unroll
+        for (int r = 0; r < RP; ++r) sh->J[lane][r] = Jc[r];
+      }
+      __syncthreads();
+      // ---- row `lane` of J^T J and of J^T r (pose term, then position term) on the VALU.
+      // Dense over the 32 columns, branch-free: a (target, column) block whose column does not move
+      // the target's link holds exact zeros and adds nothing, and without branches the broadcast
+      // reads of every column issue back to back (masked blocks cost a wait each and register
+      // copies at every merge).
+#pragma unroll
+      for (int j = 0; j < WV_NA; ++j) A[j] = 0.f;
+      if (NP > 0 && n_pose) {
+#pragma unroll
+        for (int t = 0; t < NP; ++t) {
+          if (t < n_pose) {
+#pragma unroll
+            for (int j = 0; j < WV_NA; ++j)
+#pragma unroll
+              for (int r = 0; r < 6; ++r) A[j] += Jc[6 * t + r] * sh->J[j][6 * t + r];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) gv += Jc[6 * t + r] * sh->rows[6 * t + r];
+          }
         }
       }
-    }
-    if (NQ > 0 && n_pos) {
-      float A2[WV_NA];
+      if (NQ > 0 && n_pos) {
+        float A2[WV_NA];
 #pragma unroll
-      for (int j = 0; j < WV_NA; ++j) A2[j] = 0.f;
-      float g2 = 0.f;
+        for (int j = 0; j < WV_NA; ++j) A2[j] = 0.f;
+        float g2 = 0.f;
 #pragma unroll
-      for (int u = 0; u < NQ; ++u) {
-        if (u < n_pos) {
+        for (int u = 0; u < NQ; ++u) {
+          if (u < n_pos) {
 #pragma unroll
-          for (int j = 0; j < WV_NA; ++j)
+            for (int j = 0; j < WV_NA; ++j)
 #pragma unroll
-            for (int r = 0; r < 3; ++r) A2[j] += Jc[6 * NP + 3 * u + r] * sh->J[j][6 * NP + 3 * u + r];
+              for (int r = 0; r < 3; ++r) A2[j] += Jc[6 * NP + 3 * u + r] * sh->J[j][6 * NP + 3 * u + r];
 #pragma unroll
-          for (int r = 0; r < 3; ++r) g2 += Jc[6 * NP + 3 * u + r] * sh->rows[Rf + 3 * u + r];
+            for (int r = 0; r < 3; ++r) g2 += Jc[6 * NP + 3 * u + r] * sh->rows[Rf + 3 * u + r];
+          }
         }
-      }
 #pragma unroll
-      for (int j = 0; j < WV_NA; ++j) A[j] = A[j] + A2[j];
-      gv = gv + g2;
+        for (int j = 0; j < WV_NA; ++j) A[j] = A[j] + A2[j];
+        gv = gv + g2;
+      }
     }
     if (has_lim) gv = gv + jl * rl;
     if (has_sm) gv = gv + js * rs;
