@@ -61,12 +61,34 @@ __device__ __forceinline__ float wv_bcast(float v, int k) {
   }
 }
 
-// sum over a problem's HW lanes by an xor butterfly: every lane ends with the bitwise-same total
-// (each pairwise add sees the same two operands, in commuted order)
+// sum over a problem's HW lanes by a butterfly of partner exchanges that stay in the VALU (DPP
+// lane permutes and the 16 / 32-lane row swaps): pairs, quads (quad_perm), 8 lanes
+// (row_half_mirror: lane i <-> 7 - i), 16 (row_mirror: i <-> 15 - i), 32 (permlane16_swap),
+// 64 (permlane32_swap).  Before each exchange every lane of a group holds the same partial, so
+// every lane ends with the bitwise-same total (each add sees the same two operands, commuted).
+__device__ __forceinline__ float wv_dpp(float v, int ctrl_case) {
+  const int b = __float_as_int(v);
+  switch (ctrl_case) {
+    case 0: return __int_as_float(__builtin_amdgcn_mov_dpp(b, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    case 1: return __int_as_float(__builtin_amdgcn_mov_dpp(b, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    case 2: return __int_as_float(__builtin_amdgcn_mov_dpp(b, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    default: return __int_as_float(__builtin_amdgcn_mov_dpp(b, 0x140, 0xF, 0xF, false)); // row_mirror
+  }
+}
 template <int HW = 64>
 __device__ __forceinline__ float wv_sum(float v) {
-#pragma unroll
-  for (int o = HW / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  v += wv_dpp(v, 0);
+  v += wv_dpp(v, 1);
+  v += wv_dpp(v, 2);
+  v += wv_dpp(v, 3);
+  {
+    const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);
+  }
+  if constexpr (HW == 64) {
+    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);
+  }
   return v;
 }
 
@@ -97,6 +119,8 @@ struct WaveLDS {
   float plink[NQ > 0 ? NQ : 1][4];      // world position of each position-target link
   float4 blk[2][WV_NA];                 // Cholesky block step: row i's 4 entries of the block columns
   alignas(16) float rhs[2][WV_NA];      //                      and the right-hand side (2 buffers)
+  alignas(16) float cst[2][WV_NA];      // backward block step: the running right-hand side (2 buffers)
+  alignas(16) float dvs[WV_NA];         //                      and 1 / L_ii
   union {
     alignas(16) float J[WV_NA][JS];     // Jacobian columns (dof-major)
     alignas(16) float L[WV_NA][36];     // Cholesky factor rows, for the transposed backward solve
@@ -562,7 +586,7 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u)
-            A[jc + u] -= g0 * cur[u].x + g1 * cur[u].y + g2 * cur[u].z + g3 * cur[u].w;
+            A[jc + u] = fmaf(-g3, cur[u].w, fmaf(-g2, cur[u].z, fmaf(-g1, cur[u].y, fmaf(-g0, cur[u].x, A[jc + u]))));
 #pragma unroll
           for (int u = 0; u < 4; ++u) wv_opaque_v(A[jc + u]);
 #pragma unroll
@@ -571,22 +595,42 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
       }
     }
     // backward substitution L^T x = y with the factor's rows transposed through LDS (the
-    // Jacobian columns sharing that space were consumed before the first step's wait)
+    // Jacobian columns sharing that space were consumed before the first step's wait), four
+    // unknowns per step from the last block down: the running right-hand side c is staged, every
+    // lane solves L_D^T x_B = c_B (rows of L_D and 1 / L_mm read back as broadcasts), and
+    // c_j -= sum_m L_{k0+m, j} x_m reads the lane's own column of the block's rows.
+    const int lc = lane & (WV_NA - 1);
     if (stage) {
 #pragma unroll
-      for (int j = 0; j < WV_NA; ++j) sh->L[lane & (WV_NA - 1)][j] = A[j];
+      for (int j = 0; j < WV_NA; ++j) sh->L[lc][j] = A[j];
+      sh->dvs[lc] = dinv;
     }
     __syncthreads();
     float c = y, dq = 0.f;
-    const int lc = lane & (WV_NA - 1);
 #pragma unroll
-    for (int k = WV_NA - 1; k >= 0; --k) {
+    for (int s4 = WV_NA / 4 - 1; s4 >= 0; --s4) {
+      const int k0 = 4 * s4, buf = s4 & 1;
+      // this lane's entries of the block's rows (independent of c: issued ahead of the wait)
+      const float e0 = sh->L[k0][lc], e1 = sh->L[k0 + 1][lc], e2 = sh->L[k0 + 2][lc], e3 = sh->L[k0 + 3][lc];
+      if (stage) sh->cst[buf][lc] = c;
+      __syncthreads();
+      const float4 cB = *reinterpret_cast<const float4*>(&sh->cst[buf][k0]);
+      const float4 dB = *reinterpret_cast<const float4*>(&sh->dvs[k0]);
+      const float4 r1 = *reinterpret_cast<const float4*>(&sh->L[k0 + 1][k0]);
+      const float4 r2 = *reinterpret_cast<const float4*>(&sh->L[k0 + 2][k0]);
+      const float4 r3 = *reinterpret_cast<const float4*>(&sh->L[k0 + 3][k0]);
+      const float x3 = cB.w * dB.w;
+      const float x2 = (cB.z - r3.z * x3) * dB.z;
+      const float x1 = (cB.y - r2.y * x2 - r3.y * x3) * dB.y;
+      const float x0 = (cB.x - r1.x * x1 - r2.x * x2 - r3.x * x3) * dB.x;
       int ln = lane;
       wv_opaque_v(ln);
-      const float t = c * dinv;
-      const float xk = wv_bcast<HW>(t, k);
-      dq = ln == k ? t : dq;
-      c -= sh->L[k][lc] * xk;
+      const int m = ln - k0;
+      dq = m == 0 ? x0 : dq;
+      dq = m == 1 ? x1 : dq;
+      dq = m == 2 ? x2 : dq;
+      dq = m == 3 ? x3 : dq;
+      c -= e0 * x0 + e1 * x1 + e2 * x2 + e3 * x3;
     }
     if (pad) dq = 0.f;
     // predicted reduction 1/2 d.(lam d - g), summed over the dofs
