@@ -124,32 +124,46 @@ __device__ __forceinline__ void tri_index(int k, int& i, int& j) {
 //   sin(a) = 2 sin(half) cos(half),  1 - cos(a) = 2 sin(half)^2       (a = |log_so3(E)|)
 // which replaces the sincos(a) of se3_log_map (se3_kernels.py:230-309) and the sincos(a / 2),
 // sincos(a) of se3_jlog (se3_kernels.py:358-428) by identities: one atan2, no sin / cos calls.
-// The thresholds (eps = 1e-4) and the branch structure are the reference's.
+// The thresholds (eps = 1e-4) and the branch structure are the reference's.  With FAST, square
+// roots and reciprocals are the hardware's (v_sqrt_f32, v_rsq_f32, v_rcp_f32, 1 ulp): these run on
+// one lane per target and the correctly rounded sequences dominated their instruction count.  The
+// trajectory kernel's goal term keeps the correctly rounded forms (its long banded solves amplify
+// the difference).
+template <bool FAST>
+__device__ __forceinline__ float tl_sqrt(float x) { return FAST ? __builtin_amdgcn_sqrtf(x) : sqrtf(x); }
+template <bool FAST>
+__device__ __forceinline__ float tl_rcp(float x) { return FAST ? __builtin_amdgcn_rcpf(x) : 1.f / x; }
+template <bool FAST>
+__device__ __forceinline__ float tl_rsq(float x) { return FAST ? __builtin_amdgcn_rsqf(x) : 1.f / sqrtf(x); }
 struct TargetLog {
   Pose Tfi;      // T_f^-1
   float e[6];    // log(E) = (v, w)
   float a, sh, ch;  // |w|, sin(half), cos(half)
 };
 
+template <bool FAST>
 __device__ __forceinline__ void target_log(const Pose& Tf, const Pose& Tt, TargetLog& L) {
   const float eps = 1e-4f;
   L.Tfi = inverse(Tf);
   const Pose E = compose(L.Tfi, Tt);
   const float* q = E.q;
   const float n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
-  const float n = sqrtf(n2);
+  const float n = tl_sqrt<FAST>(n2);
   const float half = atan2f(n, q[0]);
   const float angle = 2.f * half;
-  const float rinv = 1.f / sqrtf(n2 + q[0] * q[0]);
+  const float rinv = tl_rsq<FAST>(n2 + q[0] * q[0]);
   L.sh = n * rinv;
   L.ch = q[0] * rinv;
-  const float s = fabsf(angle) < 1e-6f ? 0.5f - (angle * angle) / 48.f : L.sh / angle;  // sin(half) / angle
-  float w[3] = {q[1] / s, q[2] / s, q[3] / s};
-  const float a = sqrtf(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  // 1 / s with s = sin(half) / angle
+  const float inv_s = fabsf(angle) < 1e-6f ? tl_rcp<FAST>(0.5f - (angle * angle) * (1.f / 48.f))
+                                            : angle * tl_rcp<FAST>(L.sh);
+  float w[3] = {q[1] * inv_s, q[2] * inv_s, q[3] * inv_s};
+  const float a = tl_sqrt<FAST>(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
   float c2 = 1.f / 12.f;
   if (a >= eps) {
     const float omc = 2.f * L.sh * L.sh;
-    if (fabsf(omc) >= eps) c2 = (1.f / (a * a)) * (1.f - (a * L.ch) / (2.f * L.sh));
+    if (fabsf(omc) >= eps)
+      c2 = tl_rcp<FAST>(a * a) * (1.f - (a * L.ch) * tl_rcp<FAST>(2.f * L.sh));
   }
   float v[3];
   ikk_apply(-0.5f, c2, w, E.t, v);
@@ -161,6 +175,7 @@ __device__ __forceinline__ void target_log(const Pose& Tf, const Pose& Tt, Targe
 // Jacobian blocks of the target (frame_task.py:391-456, 557-621): jlog(F), F = T_tgt^-1 T_f = E^-1
 // (its rotation vector is -w), and Ad(T_f^-1), packed as R(T_f^-1) 9, t(T_f^-1) 3, A 9, B 9, w 6
 // with jlog = [[A, B], [0, A]], A = so3_jac_left_inv(w_F)^T (so3_kernels.py:13-40).
+template <bool FAST>
 __device__ __forceinline__ void target_blocks(const Pose& Tt, const Pose& Tf, const TargetLog& L, const float* wt,
                                               float* tg) {
   const float eps = 1e-4f;
@@ -173,7 +188,8 @@ __device__ __forceinline__ void target_blocks(const Pose& Tt, const Pose& Tf, co
   const float t2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
   const bool taylor = t2 < eps;
   const float th = L.a;
-  const float coeff = taylor ? 1.f / 12.f : (1.f - th * L.ch / (2.f * L.sh)) / t2;
+  const float coeff =
+      taylor ? 1.f / 12.f : (1.f - th * L.ch * tl_rcp<FAST>(2.f * L.sh)) * tl_rcp<FAST>(t2);
   const float K2[9] = {w[0] * w[0] - t2, w[0] * w[1], w[0] * w[2],
                        w[1] * w[0], w[1] * w[1] - t2, w[1] * w[2],
                        w[2] * w[0], w[2] * w[1], w[2] * w[2] - t2};
@@ -186,10 +202,10 @@ __device__ __forceinline__ void target_blocks(const Pose& Tt, const Pose& Tf, co
 #pragma unroll
     for (int i = 0; i < 9; ++i) B[i] = 0.5f * sk[i];
   } else {
-    const float th_inv = 1.f / th;
+    const float th_inv = tl_rcp<FAST>(th);
     const float th2_inv = th_inv * th_inv;
     const float st = 2.f * L.sh * L.ch;
-    const float inv_2_2ct = 1.f / (4.f * L.sh * L.sh);  // 1 / (2 (1 - cos th))
+    const float inv_2_2ct = tl_rcp<FAST>(4.f * L.sh * L.sh);  // 1 / (2 (1 - cos th))
     const float beta = th2_inv - st * th_inv * inv_2_2ct;
     const float bdot = -2.f * th2_inv * th2_inv + (1.f + st * th_inv) * th2_inv * inv_2_2ct;
     const float wTp = w[0] * p[0] + w[1] * p[1] + w[2] * p[2];
@@ -221,20 +237,22 @@ __device__ __forceinline__ void target_blocks(const Pose& Tt, const Pose& Tf, co
 
 // Pose residual of a target r = w * log(T_f^-1 T_tgt) (frame_task.py:505-554); with tg != nullptr
 // also its Jacobian blocks.
+template <bool FAST = true>
 __device__ __forceinline__ void pose_target(const Pose& Tf, const Pose& Tt, const float* w, float* r6,
                                             float* tg = nullptr) {
   TargetLog L;
-  target_log(Tf, Tt, L);
+  target_log<FAST>(Tf, Tt, L);
 #pragma unroll
   for (int i = 0; i < 6; ++i) r6[i] = w[i] * L.e[i];
-  if (tg) target_blocks(Tt, Tf, L, w, tg);
+  if (tg) target_blocks<FAST>(Tt, Tf, L, w, tg);
 }
 
 // the Jacobian blocks alone
+template <bool FAST = true>
 __device__ __forceinline__ void pose_blocks(const Pose& Tf, const Pose& Tt, const float* w, float* tg) {
   TargetLog L;
-  target_log(Tf, Tt, L);
-  target_blocks(Tt, Tf, L, w, tg);
+  target_log<FAST>(Tf, Tt, L);
+  target_blocks<FAST>(Tt, Tf, L, w, tg);
 }
 
 // spatial column of link `link` for actuated dof a: sum of S_j * map[j][a] over ancestor joints

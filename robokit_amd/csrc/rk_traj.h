@@ -170,14 +170,14 @@ __device__ __forceinline__ void traj_frame(RobotK* rb, TrajK* tt, const float (&
 #pragma unroll
   for (int i = 0; i < 6; ++i) gw[i] = tt->goal_w[i];
   float r6[6];
-  pose_target(Tg, goal, gw, r6);
+  pose_target<false>(Tg, goal, gw, r6);
   float sg = 0.f;
 #pragma unroll
   for (int i = 0; i < 6; ++i) sg += r6[i] * r6[i];
   cg = last ? sg : 0.f;
   if (JAC && last) {
     float tg[36];
-    pose_blocks(Tg, goal, gw, tg);
+    pose_blocks<false>(Tg, goal, gw, tg);
     float J6[6][NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
