@@ -120,16 +120,22 @@ def test_fused_and_plain_agree(ctx):
     assert np.sum(f["ok"] != p["ok"]) <= 1
 
 
-def test_half_wave_matches_wave(ctx):
-    """two problems per wavefront (32 lanes each) run the one-problem-per-wavefront arithmetic
-    lane for lane; the 64-lane butterfly sums only add exact zeros from lanes 32..63: same bits,
-    for an odd problem count too (the last wavefront's second slot is idle)"""
+def test_half_wave_mate_independence(ctx):
+    """two problems share a wavefront (32 lanes each, one matrix-core block each): a problem's
+    result does not depend on its wave-mate -- the same problems in reversed order (every pair
+    different) and an odd count (the last wavefront's second slot idle) give the same bits"""
     spec = W.humanoid29(0)
     wl = W.ik_batch(spec, W.humanoid_ee_links(spec), 99, seed=21)
-    w = _gpu_ik(ctx, wl, N.MODE_FUSED, tpp=N.TPP_WAVE, iters=20)
     h = _gpu_ik(ctx, wl, N.MODE_FUSED, tpp=N.TPP_HALF, iters=20)
+    rev = W.ik_batch(spec, W.humanoid_ee_links(spec), 99, seed=21)
+    rev.q_seed, rev.targets = wl.q_seed[::-1].copy(), wl.targets[::-1].copy()
+    r = _gpu_ik(ctx, rev, N.MODE_FUSED, tpp=N.TPP_HALF, iters=20)
+    assert np.array_equal(h["q"], r["q"][::-1]) and np.array_equal(h["cost_trace"], r["cost_trace"][::-1])
+    assert np.array_equal(h["accept"], r["accept"][::-1])
+    # the one-problem-per-wavefront kernel runs the same arithmetic lane for lane (its 64-lane
+    # sums add exact zeros from lanes 32..63; its J^T J is the same FMA chain on the VALU)
+    w = _gpu_ik(ctx, wl, N.MODE_FUSED, tpp=N.TPP_WAVE, iters=20)
     assert np.array_equal(w["q"], h["q"]) and np.array_equal(w["cost_trace"], h["cost_trace"])
-    assert np.array_equal(w["accept"], h["accept"])
 
 
 @pytest.mark.parametrize("robot", ["panda_dh", "humanoid"])
