@@ -139,6 +139,8 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
 #define RK_TPP_WAVE (-64)
 #define RK_TPP_HALF (-32)
 #define RK_USE_GRAPH 0x1   /* rk_group_solve flag: capture the launch sequence in a hipGraph and replay it */
+#define RK_ASYNC 0x2       /* rk_group_solve flag: return once the solve is enqueued on the context's stream
+                            * (kernel_ms is not written; rk_context_sync or a later synchronous call waits) */
 
 /*
  * Optimizer.solve with a fixed iteration count (opt/optimizer.py:216-240, LM step
@@ -177,7 +179,7 @@ int rk_group_set_tuning(rk_group* g, const int32_t* tpp, int32_t wg_per_cu, int3
 /* time every legal threads-per-problem for each batch on a short run and keep the fastest;
  * chosen_tpp [nbatches] (nullable) receives the choice */
 int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t* chosen_tpp);
-/* solve all batches; flags: RK_USE_GRAPH (capture once per shape, then replay), RK_FROM_SEED */
+/* solve all batches; flags: RK_USE_GRAPH (capture once per shape, then replay), RK_FROM_SEED, RK_ASYNC */
 int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t flags, float* kernel_ms);
 /* per persistent kernel of the last eager (not graph-replayed) solve -- one per kernel variant,
  * launched concurrently on forked streams: its threads-per-problem code, HIP-event time on its

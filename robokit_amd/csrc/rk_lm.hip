@@ -745,6 +745,9 @@ __device__ __forceinline__ void run_tile(const Bucket* bk, int local, const LMPa
 // the variant's buckets from its queue until the queue is empty.  A group launches one such
 // kernel per variant it holds, concurrently (forked streams, one graph); a kernel per variant
 // gives each its own register allocation instead of the worst case of all.
+// counter[0] is the tile queue, counter[1] counts workgroups that left it; the last one out
+// (every other workgroup has taken its final tile) puts both back to zero for the next launch,
+// so no reset kernel precedes a launch.  Both start at zero when the group is created.
 // register budget of a variant's persistent kernel: waves per SIMD the allocation must allow
 __host__ __device__ constexpr int variant_waves_per_eu(int v) { return variant_wave(v) || variant_half(v) ? 2 : 1; }
 
@@ -765,10 +768,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(variant_wave
     while (b + 1 < nb && tile >= buckets[b + 1].tile_base) ++b;
     run_tile<V>(buckets + b, tile - buckets[b].tile_base, lp, it0, iters, lds);
   }
-}
-
-__global__ void k_reset_counter(int* counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+  if (threadIdx.x == 0 && atomicAdd(counter + 1, 1) == (int)gridDim.x - 1) {
+    atomicExch(counter, 0);
+    atomicExch(counter + 1, 0);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1253,7 +1256,7 @@ struct rk_group {
   std::vector<int> tpp;  // chosen threads per problem, 0 = default
   int wg_per_cu = 0, iters_per_launch = 0;
   Bucket* buckets_dev = nullptr;
-  int* counter = nullptr;  // one queue counter per variant
+  int* counter = nullptr;  // per variant: tile queue and exit count (self-resetting)
   hipStream_t aux[NVARIANT] = {};
   hipEvent_t ev_fork = nullptr, ev_join[NVARIANT] = {};
   hipEvent_t ev_k0[NVARIANT] = {}, ev_k1[NVARIANT] = {};  // per-variant kernel timing
@@ -2277,9 +2280,17 @@ int rk_group_create(rk_context* ctx, int32_t nb, rk_batch* const* batches, rk_gr
   g->tpp.assign(nb, 0);
   int rc;
   if ((rc = dmalloc(ctx, (void**)&g->buckets_dev, sizeof(Bucket) * nb, &g->bytes)) ||
-      (rc = dmalloc(ctx, (void**)&g->counter, sizeof(int) * NVARIANT, &g->bytes))) {
+      (rc = dmalloc(ctx, (void**)&g->counter, sizeof(int) * 2 * NVARIANT, &g->bytes))) {
     delete g;
     return rc;
+  }
+  // (creation is never captured into a graph)
+  if (hipMemsetAsync(g->counter, 0, sizeof(int) * 2 * NVARIANT, ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    dfree(ctx, g->buckets_dev, sizeof(Bucket) * nb);
+    dfree(ctx, g->counter, sizeof(int) * 2 * NVARIANT);
+    delete g;
+    return fail(RK_EHIP, "could not clear the group's queue counters");
   }
   *out = g;
   return RK_OK;
@@ -2461,9 +2472,8 @@ static int group_launch(rk_group* g, const GroupLaunch& gl, int iters, const LMP
     if (timed) HIP_TRY(hipEventRecord(g->ev_k0[k], st));
     for (int it0 = 0; it0 < std::max(1, iters); it0 += chunk) {
       const int n_it = std::max(0, std::min(chunk, iters - it0));
-      hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, g->counter + k);
       launch_persistent(gl.var[k], dim3(gl.grid[k]), gl.lds[k], st, (const Bucket*)g->buckets_dev + gl.first[k],
-                        gl.nb[k], gl.tiles[k], g->counter + k, lp, it0, n_it);
+                        gl.nb[k], gl.tiles[k], g->counter + 2 * k, lp, it0, n_it);
       if (iters == 0) break;
     }
     if (timed) HIP_TRY(hipEventRecord(g->ev_k1[k], st));
@@ -2527,6 +2537,7 @@ int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t fl
     g->last_var = gl.var;
     g->last_mask = gl.mask;
   }
+  if (flags & RK_ASYNC) return RK_OK;  // enqueued; the caller synchronises
   HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
   HIP_TRY(hipEventSynchronize(ctx->ev1));
   if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, ctx->ev0, ctx->ev1));

@@ -21,7 +21,7 @@ RK_OK, RK_EINVAL, RK_EHIP, RK_ECAPACITY, RK_ENOMEM, RK_EBUSY = range(6)
 MODE_FUSED, MODE_PLAIN = 0, 1
 TPP_WAVE = -64  # one wavefront per problem, register rows
 TPP_HALF = -32  # two problems per wavefront (32 lanes each), register rows
-FROM_SEED, CONTINUE, USE_GRAPH = 0x100, 0x200, 0x1
+FROM_SEED, CONTINUE, USE_GRAPH, ASYNC = 0x100, 0x200, 0x1, 0x2
 SE3_OPS = {"compose": (0, 7), "inverse": (1, 7), "log": (2, 6), "jlog": (3, 36), "adjoint": (4, 36), "apply": (5, 3)}
 
 
@@ -404,11 +404,14 @@ class Group:
         _check(lib().rk_group_autotune(self._h, int(iters), C.byref(params.c()), _p(out)))
         return out.tolist()
 
-    def solve(self, iters: int, params: LMParams, use_graph: bool = True, from_seed: bool = False) -> float:
+    def solve(self, iters: int, params: LMParams, use_graph: bool = True, from_seed: bool = False,
+              sync: bool = True) -> float:
+        """Solve every batch; returns the event-timed milliseconds, or 0.0 with ``sync=False`` (the
+        solve is only enqueued: ``Context.sync`` or a later synchronous call waits for it)."""
         ms = C.c_float()
-        flags = (USE_GRAPH if use_graph else 0) | (FROM_SEED if from_seed else 0)
+        flags = (USE_GRAPH if use_graph else 0) | (FROM_SEED if from_seed else 0) | (0 if sync else ASYNC)
         _check(lib().rk_group_solve(self._h, int(iters), C.byref(params.c()), flags, C.byref(ms)))
-        return float(ms.value)
+        return float(ms.value) if sync else 0.0
 
     def kernel_times(self) -> List[tuple]:
         """[(threads-per-problem code, ms, batch indices)] of each persistent kernel of the last solve."""
