@@ -287,40 +287,39 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
 
   // residual rows of the lane's target at world poses Tw (all lanes hold their joint's pose);
   // JAC also writes the pose target's Jacobian blocks / the position link's point to LDS
-  auto target_rows = [&](const Pose& Tw, bool jac) {
+  // (and the lane's sums of squares sp / sq; with jac false -- the proposal -- nothing goes to
+  // LDS.  Both paths sum the same rows in the same order, so a cost at the proposal and the cost
+  // recomputed at the accepted iterate agree bitwise)
+  auto target_rows = [&](const Pose& Tw, bool jac, float& sp, float& sq) {
     const Pose Tl = wv_shfl_pose<HW>(Tw, tj < 0 ? 0 : tj);
     const Pose Tf = tj < 0 ? identity_pose() : Tl;
     if (lane < n_pose) {
       float r6[6];
       pose_target(Tf, tgt, pw, r6, jac ? sh->tg[lane < NP ? lane : 0] : nullptr);
-#pragma unroll
-      for (int i = 0; i < 6; ++i) sh->rows[6 * lane + i] = r6[i];
-    } else if (lane < n_pose + n_pos) {
-      const int u = lane - n_pose;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) sh->rows[Rf + 3 * u + i] = pw[i] * (ptgt[i] - Tf.t[i]);
       if (jac) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) sh->plink[u < NQ ? u : 0][i] = Tf.t[i];
+        for (int i = 0; i < 6; ++i) sh->rows[6 * lane + i] = r6[i];
       }
-    }
-  };
-  // cost = sum over terms of 0.5 * (sum of the term's squared rows): the rows are read back from
-  // LDS by the lanes that own them (target t: lane t, dof a: lane a) and summed across the wave
-  auto term_cost_lds = [&]() {
-    float sp = 0.f, sq = 0.f, sl = 0.f, ss = 0.f;
-    if (lane < n_pose) {
 #pragma unroll
-      for (int i = 0; i < 6; ++i) sp += sh->rows[6 * lane + i] * sh->rows[6 * lane + i];
+      for (int i = 0; i < 6; ++i) sp += r6[i] * r6[i];
     } else if (lane < n_pose + n_pos) {
       const int u = lane - n_pose;
+      float r3[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) sq += sh->rows[Rf + 3 * u + i] * sh->rows[Rf + 3 * u + i];
+      for (int i = 0; i < 3; ++i) r3[i] = pw[i] * (ptgt[i] - Tf.t[i]);
+      if (jac) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          sh->rows[Rf + 3 * u + i] = r3[i];
+          sh->plink[u < NQ ? u : 0][i] = Tf.t[i];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i) sq += r3[i] * r3[i];
     }
-    if (lane < na) {
-      if (has_lim) sl = sh->rows[off_l + lane] * sh->rows[off_l + lane];
-      if (has_sm) ss = sh->rows[off_s + lane] * sh->rows[off_s + lane];
-    }
+  };
+  // cost = sum over terms of 0.5 * (sum of the term's squared rows) from the lanes' partial sums
+  auto term_cost = [&](float sp, float sq, float sl, float ss) {
     float c = 0.f;
     if (n_pose) c = c + 0.5f * wv_sum<HW>(sp);
     if (n_pos) c = c + 0.5f * wv_sum<HW>(sq);
@@ -361,7 +360,8 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
       Sc1[r] = __shfl(S6[r], cj1 < 0 ? 0 : cj1, HW);
     }
     // ---- residual rows and target blocks at q
-    target_rows(T, true);
+    float sp = 0.f, sq = 0.f;
+    target_rows(T, true, sp, sq);
     float rl = 0.f, jl = 0.f, rs = 0.f, js = 0.f;
     if (lane < na) {
       if (has_lim) {
@@ -376,7 +376,7 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
       }
     }
     __syncthreads();
-    cost = term_cost_lds();
+    cost = term_cost(sp, sq, rl * rl, rs * rs);
     // ---- this dof's Jacobian column (pose rows, then position rows), also to LDS
     float Jc[RP > 0 ? RP : 1];
 #pragma unroll
@@ -636,17 +636,22 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     // predicted reduction 1/2 d.(lam d - g), summed over the dofs
     const float pr = wv_sum<HW>(pad ? 0.f : dq * (lam * dq - gv));
     const float pred = 0.5f * pr;
-    // ---- proposal
+    // ---- proposal: FK, rows and cost in registers (no LDS)
     const float qp = q + dq;
     const Pose Tp = wave_fk<HW>(jd, nj, nrounds, qp);
-    __syncthreads();  // the rows of the iterate have been consumed
-    target_rows(Tp, false);
+    float psp = 0.f, psq = 0.f, psl = 0.f, pss = 0.f;
+    target_rows(Tp, false, psp, psq);
     if (lane < na) {
-      if (has_lim) sh->rows[off_l + lane] = limit_row(qp, lo, hi, wl);
-      if (has_sm) sh->rows[off_s + lane] = ws * (qp - qprev);
+      if (has_lim) {
+        const float r = limit_row(qp, lo, hi, wl);
+        psl = r * r;
+      }
+      if (has_sm) {
+        const float r = ws * (qp - qprev);
+        pss = r * r;
+      }
     }
-    __syncthreads();
-    const float cp = term_cost_lds();
+    const float cp = term_cost(psp, psq, psl, pss);
     if (valid && Pg.trace && gi == 0 && lane == 0) Pg.trace[s] = cost;
     const bool acc = lm_accept(lp, cost, cp, pred, lam);
     moved = acc;
