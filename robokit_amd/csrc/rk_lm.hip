@@ -53,7 +53,66 @@ struct Terms {
   int n_rot, Rr;
   int rot_link[RK_MAXT];
   float rot_w[RK_MAXT][3];
+  // register wave kernels: elimination order of the dofs (lane k holds dof wperm[k]; lanes
+  // >= na are padding and map to themselves), its inverse, and per 4-column block s of the
+  // factorisation the 4-row chunks c > s whose rows have structural nonzeros in the block's
+  // columns (bit c): the other chunks' trailing updates add exact zeros and are skipped
+  int wperm[RK_NA_MAX], wiperm[RK_NA_MAX];
+  unsigned wchol[RK_NA_MAX / 4];
 };
+
+// Dof elimination order and block fill pattern of the damped normal equations of one term
+// structure (host, at batch creation).  J^T J has a nonzero (a, b) iff some target's link moves
+// with both dofs (limit / smoothness rows are diagonal).  A minimum-degree order (ties: lowest
+// dof) eliminates the leaves of a kinematic tree first -- for a humanoid the legs, then the arms,
+// then the waist they share -- which keeps the Cholesky factor free of fill between branches;
+// the symbolic factorisation in that order gives the structurally nonzero (row chunk, column
+// block) pairs.  The ordering only changes which exact zeros are skipped and the rounding
+// order, not the system solved.
+static void wave_elimination(const Robot& r, Terms& th) {
+  const int na = r.na;
+  for (int k = 0; k < RK_NA_MAX; ++k) th.wperm[k] = th.wiperm[k] = k;
+  for (int s4 = 0; s4 < RK_NA_MAX / 4; ++s4) th.wchol[s4] = ~0u;
+  if (na > RK_NA_MAX) return;
+  std::vector<int> links;
+  for (int i = 0; i < th.n_pose; ++i) links.push_back(th.pose_link[i]);
+  for (int i = 0; i < th.n_pos; ++i) links.push_back(th.pos_link[i]);
+  for (int i = 0; i < th.n_rot; ++i) links.push_back(th.rot_link[i]);
+  std::vector<unsigned> dep(links.size(), 0u);  // bit a: the target's link moves with dof a
+  for (size_t t = 0; t < links.size(); ++t)
+    for (int a = 0; a < na; ++a)
+      for (int c = r.col_start[a]; c < r.col_start[a + 1]; ++c)
+        if ((r.anc[links[t]] >> r.col_joint[c]) & 1ull) dep[t] |= 1u << a;
+  std::vector<unsigned> adj(na, 0u);
+  for (unsigned m : dep)
+    for (int a = 0; a < na; ++a)
+      if ((m >> a) & 1u) adj[a] |= m & ~(1u << a);
+  unsigned left = na >= 32 ? ~0u : ((1u << na) - 1u);
+  for (int k = 0; k < na; ++k) {
+    int best = -1, bdeg = 1 << 30;
+    for (int a = 0; a < na; ++a)
+      if ((left >> a) & 1u) {
+        const int d = __builtin_popcount(adj[a] & left);
+        if (d < bdeg) { bdeg = d; best = a; }
+      }
+    th.wperm[k] = best;
+    const unsigned nb = adj[best] & left & ~(1u << best);
+    for (int a = 0; a < na; ++a)
+      if ((nb >> a) & 1u) adj[a] |= nb & ~(1u << a);  // fill: the neighbours become a clique
+    left &= ~(1u << best);
+  }
+  for (int k = 0; k < na; ++k) th.wiperm[th.wperm[k]] = k;
+  // filled pattern in the new order: row i has a nonzero in column k < i iff dofs wperm[i],
+  // wperm[k] are adjacent in the filled graph (adj now holds it)
+  for (int s4 = 0; s4 < RK_NA_MAX / 4; ++s4) {
+    unsigned m = 0;
+    for (int c = s4 + 1; c < RK_NA_MAX / 4; ++c)
+      for (int i = 4 * c; i < 4 * c + 4 && i < na; ++i)
+        for (int k = 4 * s4; k < 4 * s4 + 4 && k < na; ++k)
+          if ((adj[th.wperm[i]] >> th.wperm[k]) & 1u) m |= 1u << c;
+    th.wchol[s4] = m;
+  }
+}
 
 struct LMParams {
   float lm_lambda, lambda_factor, lambda_min, lambda_max, rho_min, gain_ratio_epsilon;
@@ -1878,6 +1937,7 @@ static int make_terms(rk_context* ctx, const rk_terms_desc* t, Terms* out) {
   th.Rq = 3 * th.n_pos;
   th.Rr = 3 * th.n_rot;
   th.R = th.Rf + th.Rq + th.Rr + (th.limit ? r.na : 0) + (th.smooth ? r.na : 0);
+  wave_elimination(r, th);
   *out = th;
   return RK_OK;
 }

@@ -213,21 +213,24 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     jd.off = rg->off[j];
     jd.type = lane < nj ? rg->joint_type[j] : 0;
     jd.parent = lane < nj ? rg->parent[j] : -1;
-    jd.act = rg->act[j];
-    jd.mimic = rg->mimic[j];
+    // the lane holding the dof that drives the joint (dofs sit on lanes in elimination order)
+    jd.act = rg->act[j] >= 0 ? Pg.terms->wiperm[rg->act[j]] : -1;
+    jd.mimic = rg->mimic[j] >= 0 ? Pg.terms->wiperm[rg->mimic[j]] : -1;
   }
-  // ---- per-lane dof data: joints driving dof a (CSR, <= 2 entries), limits, weights
+  // ---- per-lane dof data (lane a < na holds dof wperm[a], the elimination order of the term
+  // structure): joints driving the dof (CSR, <= 2 entries), limits, weights
+  const int dof = lane < na ? Pg.terms->wperm[lane] : 0;
   int cj0 = -1, cj1 = -1;
   float cw0 = 0.f, cw1 = 0.f, lo = 0.f, hi = 0.f, wl = 0.f, ws = 0.f;
   if (lane < na) {
     const Robot* rg = Pg.robot;
-    const int c0 = rg->col_start[lane], c1 = rg->col_start[lane + 1];
+    const int c0 = rg->col_start[dof], c1 = rg->col_start[dof + 1];
     if (c1 > c0) { cj0 = rg->col_joint[c0]; cw0 = rg->col_w[c0]; }
     if (c1 > c0 + 1) { cj1 = rg->col_joint[c0 + 1]; cw1 = rg->col_w[c0 + 1]; }
-    lo = rg->lo[lane];
-    hi = rg->hi[lane];
-    wl = Pg.terms->limit_w[lane];
-    ws = Pg.terms->smooth_w[lane];
+    lo = rg->lo[dof];
+    hi = rg->hi[dof];
+    wl = Pg.terms->limit_w[dof];
+    ws = Pg.terms->smooth_w[dof];
   }
   // which of this dof's joints move each target's link; wave-uniform column masks per target
   unsigned in0 = 0, in1 = 0;  // bit t (pose targets), bit NP + u (position targets)
@@ -252,8 +255,8 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
   // ---- problem data
   float q = 0.f, qprev = 0.f;
   if (lane < na) {
-    q = (it0 == 0 && Pg.q_seed) ? Pg.q_seed[lane * S + s] : Pg.q[lane * S + s];
-    if (has_sm) qprev = Pg.q_prev[lane * S + s];
+    q = (it0 == 0 && Pg.q_seed) ? Pg.q_seed[dof * S + s] : Pg.q[dof * S + s];
+    if (has_sm) qprev = Pg.q_prev[dof * S + s];
   }
   // pose / position target owned by this lane
   Pose tgt = identity_pose();
@@ -342,7 +345,7 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     wv_opaque_v(in1);
     const int gi = it0 + it;
     if (valid && Pg.q_trace && gi < Pg.trace_iters) {
-      if (lane < na) Pg.q_trace[(gi * na + lane) * S + s] = q;
+      if (lane < na) Pg.q_trace[(gi * na + dof) * S + s] = q;
       if (lane == 0) Pg.lam_trace[gi * S + s] = lam;
     }
     if (HW == 64 || __ballot(moved)) {  // (one problem per wavefront: no registers to spare)
@@ -570,27 +573,24 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
       dinv = s1 ? i11 : dinv;
       dinv = s2 ? i22 : dinv;
       dinv = s3 ? i33 : dinv;
-      // trailing rows in chunks of 4, software-pipelined: the next chunk's reads issue before this
-      // chunk's updates.  The updated entries are pinned at each chunk's end: left free, the
-      // compiler sank every update to the step consuming its column and kept all multipliers and
-      // staged entries live (hundreds of registers).
-      if (k0 + 4 < WV_NA) {
-        float4 cur[4], nxt[4];
+      // trailing rows in chunks of 4.  A chunk whose rows have no structural nonzero in the
+      // block's columns (the term structure's fill pattern) would subtract exact zeros: it is
+      // skipped (wave-uniform branch).  The updated entries are pinned at each chunk's end: left
+      // free, the compiler sank every update to the step consuming its column and kept all
+      // multipliers and staged entries live (hundreds of registers).
+      unsigned cm = tm->wchol[s4];
+      wv_opaque(cm);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) cur[u] = sh->blk[buf][k0 + 4 + u];
+      for (int jc = k0 + 4; jc < WV_NA; jc += 4) {
+        if ((cm >> (jc / 4)) & 1u) {
+          float4 c[4];
 #pragma unroll
-        for (int jc = k0 + 4; jc < WV_NA; jc += 4) {
-          if (jc + 4 < WV_NA) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) nxt[u] = sh->blk[buf][jc + 4 + u];
-          }
+          for (int u = 0; u < 4; ++u) c[u] = sh->blk[buf][jc + u];
 #pragma unroll
           for (int u = 0; u < 4; ++u)
-            A[jc + u] = fmaf(-g3, cur[u].w, fmaf(-g2, cur[u].z, fmaf(-g1, cur[u].y, fmaf(-g0, cur[u].x, A[jc + u]))));
+            A[jc + u] = fmaf(-g3, c[u].w, fmaf(-g2, c[u].z, fmaf(-g1, c[u].y, fmaf(-g0, c[u].x, A[jc + u]))));
 #pragma unroll
           for (int u = 0; u < 4; ++u) wv_opaque_v(A[jc + u]);
-#pragma unroll
-          for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
         }
       }
     }
@@ -668,7 +668,7 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
   }
   (void)R;
   if (valid) {
-    if (lane < na) Pg.q[lane * S + s] = q;
+    if (lane < na) Pg.q[dof * S + s] = q;
     if (lane == 0) {
       if (iters > 0) Pg.cost[s] = cost;
       Pg.lam[s] = lam;
