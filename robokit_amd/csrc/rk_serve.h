@@ -373,11 +373,20 @@ static void srv_fail(rk_server* sv, SrvReq* r, const std::string& msg) {
   }
 }
 
-// form one launch on free stream slot s from the queue; false if nothing could start
+// form one launch on free stream slot s from the queue; false if nothing could start.
+// The head is the oldest ready request, except that at most a quarter of the streams (at least
+// one) carry trajectory launches at a time: a trajectory launch runs for tens of milliseconds
+// (one wavefront per trajectory through ~T * dofs serial band columns), an IK launch for well
+// under one, and with every stream holding a trajectory launch the IK requests would queue
+// behind them.
 static bool srv_form(rk_server* sv, int s) {
+  int traj_active = 0;
+  for (const auto& L : sv->launches)
+    if (L.active && sv->terms[L.term].traj_na) ++traj_active;
+  const bool traj_ok = traj_active < std::max(1, sv->nstreams / 4);
   SrvReq* head = nullptr;
   for (SrvReq* r : sv->queue)
-    if (srv_ready(sv, r)) {
+    if ((traj_ok || !sv->terms[r->term].traj_na) && srv_ready(sv, r)) {
       head = r;
       break;
     }
