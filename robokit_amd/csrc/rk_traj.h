@@ -32,66 +32,94 @@ struct TrajTerms {
 };
 typedef const __attribute__((address_space(4))) TrajTerms TrajK;
 
-// scene signed distance at p (minimum over obstacle spheres, then boxes; first strict minimum)
-// and its gradient direction n (geom/sdf_kernels.py:256-295, trajectory_collision_task.py:62-72)
-__device__ __forceinline__ float scene_sdf(TrajK* tt, const float* p, float* n) {
+// scene signed distance at p: minimum over obstacle spheres, then boxes, the first strict minimum
+// (geom/sdf_kernels.py:256-295, trajectory_collision_task.py:62-72).  Returns the distance and
+// the winner (win >= 0: sphere win; win < 0: box -1 - win; RK_SDF_NONE: no primitive closer
+// than 1e6); scene_normal gives the gradient
+// direction of that primitive, computed only where a caller needs it.  A sphere's square root is
+// taken only when its squared distance can reach below the current best (the bound carries a
+// 1e-5 relative slack, so the exact comparison of the reference's formula decides every case).
+#define RK_SDF_NONE (1 << 30)
+__device__ __forceinline__ float scene_sdf(TrajK* tt, const float* p, int& win) {
   float best = 1e6f;
-  n[0] = 1.f;
-  n[1] = n[2] = 0.f;
+  win = RK_SDF_NONE;
+  // (unrolled: the scene is read through the scalar cache, and a rolled loop waits on every load)
+#pragma unroll 8
   for (int i = 0; i < tt->ns; ++i) {
     const float d0 = p[0] - tt->obs_s[i][0], d1 = p[1] - tt->obs_s[i][1], d2 = p[2] - tt->obs_s[i][2];
-    const float len = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
-    const float dist = len - tt->obs_s[i][3];
-    if (dist < best) {
-      best = dist;
-      if (len > 1e-10f) {
-        n[0] = d0 / len; n[1] = d1 / len; n[2] = d2 / len;
-      } else {
-        n[0] = 1.f; n[1] = n[2] = 0.f;
+    const float r = tt->obs_s[i][3];
+    const float d2s = d0 * d0 + d1 * d1 + d2 * d2, br = best + r;
+    if (br > 0.f && d2s <= br * br * (1.f + 1e-5f) + 1e-30f) {
+      const float dist = sqrtf(d2s) - r;
+      if (dist < best) {
+        best = dist;
+        win = i;
       }
     }
   }
+#pragma unroll 4
   for (int i = 0; i < tt->nb; ++i) {
-    float ql[3], qv[3], h[3];
+    float qv[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      ql[k] = p[k] - tt->obs_b[i][k];
-      h[k] = tt->obs_b[i][3 + k];
-      qv[k] = fabsf(ql[k]) - h[k];
-    }
+    for (int k = 0; k < 3; ++k) qv[k] = fabsf(p[k] - tt->obs_b[i][k]) - tt->obs_b[i][3 + k];
     float qmax = qv[0] > qv[1] ? qv[0] : qv[1];
     if (qv[2] > qmax) qmax = qv[2];
     const float p0 = qv[0] > 0.f ? qv[0] : 0.f, p1 = qv[1] > 0.f ? qv[1] : 0.f, p2 = qv[2] > 0.f ? qv[2] : 0.f;
     const float dist = sqrtf(p0 * p0 + p1 * p1 + p2 * p2) + (qmax < 0.f ? qmax : 0.f);
     if (dist < best) {
       best = dist;
-      float nn[3] = {0.f, 0.f, 1.f};
-      if (qmax < 0.f) {
-        if (qv[0] >= qv[1] && qv[0] >= qv[2]) {
-          nn[0] = ql[0] < 0.f ? -1.f : 1.f;
-          nn[2] = 0.f;
-        } else if (qv[1] >= qv[2]) {
-          nn[1] = ql[1] < 0.f ? -1.f : 1.f;
-          nn[2] = 0.f;
-        } else {
-          nn[2] = ql[2] < 0.f ? -1.f : 1.f;
-        }
-      } else {
-        float df[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const float c = ql[k] < -h[k] ? -h[k] : (ql[k] > h[k] ? h[k] : ql[k]);
-          df[k] = ql[k] - c;
-        }
-        const float dl = sqrtf(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
-        if (dl > 1e-10f) {
-          nn[0] = df[0] / dl; nn[1] = df[1] / dl; nn[2] = df[2] / dl;
-        }
-      }
-      n[0] = nn[0]; n[1] = nn[1]; n[2] = nn[2];
+      win = -1 - i;
     }
   }
   return best;
+}
+
+__device__ __forceinline__ void scene_normal(TrajK* tt, const float* p, int win, float* n) {
+  n[0] = 1.f;
+  n[1] = n[2] = 0.f;
+  if (win == RK_SDF_NONE) return;  // the reference's initial direction
+  if (win >= 0) {
+    const float d0 = p[0] - tt->obs_s[win][0], d1 = p[1] - tt->obs_s[win][1], d2 = p[2] - tt->obs_s[win][2];
+    const float len = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+    if (len > 1e-10f) {
+      n[0] = d0 / len; n[1] = d1 / len; n[2] = d2 / len;
+    }
+    return;
+  }
+  const int i = -1 - win;
+  float ql[3], qv[3], h[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    ql[k] = p[k] - tt->obs_b[i][k];
+    h[k] = tt->obs_b[i][3 + k];
+    qv[k] = fabsf(ql[k]) - h[k];
+  }
+  float qmax = qv[0] > qv[1] ? qv[0] : qv[1];
+  if (qv[2] > qmax) qmax = qv[2];
+  float nn[3] = {0.f, 0.f, 1.f};
+  if (qmax < 0.f) {
+    if (qv[0] >= qv[1] && qv[0] >= qv[2]) {
+      nn[0] = ql[0] < 0.f ? -1.f : 1.f;
+      nn[2] = 0.f;
+    } else if (qv[1] >= qv[2]) {
+      nn[1] = ql[1] < 0.f ? -1.f : 1.f;
+      nn[2] = 0.f;
+    } else {
+      nn[2] = ql[2] < 0.f ? -1.f : 1.f;
+    }
+  } else {
+    float df[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float c = ql[k] < -h[k] ? -h[k] : (ql[k] > h[k] ? h[k] : ql[k]);
+      df[k] = ql[k] - c;
+    }
+    const float dl = sqrtf(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
+    if (dl > 1e-10f) {
+      nn[0] = df[0] / dl; nn[1] = df[1] / dl; nn[2] = df[2] / dl;
+    }
+  }
+  n[0] = nn[0]; n[1] = nn[1]; n[2] = nn[2];
 }
 
 // per-frame evaluation: FK streamed joint by joint with the spheres of each link processed as
@@ -115,12 +143,14 @@ __device__ __forceinline__ void traj_frame(RobotK* rb, TrajK* tt, const float (&
       c[0] += P.t[0];
       c[1] += P.t[1];
       c[2] += P.t[2];
-      float n[3];
-      const float sd = scene_sdf(tt, c, n) - tt->sph_r[k];
+      int win;
+      const float sd = scene_sdf(tt, c, win) - tt->sph_r[k];
       if (sd < margin) {
         const float res = w * (margin - sd);
         cc += res * res;
         if (JAC) {
+          float n[3];
+          scene_normal(tt, c, win, n);
           const float cn0 = c[1] * n[2] - c[2] * n[1], cn1 = c[2] * n[0] - c[0] * n[2], cn2 = c[0] * n[1] - c[1] * n[0];
           float jr[NA];
 #pragma unroll
@@ -430,12 +460,13 @@ __global__ __launch_bounds__(64) void k_traj_metrics(Problems P, const TrajTerms
   auto spheres = [&](int joint, const Pose& Pw) {
     for (; k < tt->nsph && tt->sph_joint[k] == joint; ++k) {
       const float lc[3] = {tt->sph_c[k][0], tt->sph_c[k][1], tt->sph_c[k][2]};
-      float c[3], n[3];
+      float c[3];
       quat_apply(Pw.q, lc, c);
       c[0] += Pw.t[0];
       c[1] += Pw.t[1];
       c[2] += Pw.t[2];
-      msd = fminf(msd, scene_sdf(tt, c, n) - tt->sph_r[k]);
+      int win;
+      msd = fminf(msd, scene_sdf(tt, c, win) - tt->sph_r[k]);
     }
   };
   spheres(-1, Tt);
