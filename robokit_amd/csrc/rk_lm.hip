@@ -2047,10 +2047,12 @@ static int make_traj_terms(rk_context* ctx, const rk_traj_desc* d, TrajTerms* ou
   }
   tt.ns = d->n_obs_spheres;
   tt.nb = d->n_obs_boxes;
-  for (int i = 0; i < tt.ns; ++i)
-    for (int c = 0; c < 4; ++c) tt.obs_s[i][c] = d->obs_spheres[4 * i + c];
-  for (int i = 0; i < tt.nb; ++i)
-    for (int c = 0; c < 6; ++c) tt.obs_b[i][c] = d->obs_boxes[6 * i + c];
+  // the kernels read the scene in groups of 8 spheres / 4 boxes: the rest of the last group is
+  // far-away padding that never wins the minimum
+  for (int i = 0; i < RK_SCENE_MAXS; ++i)
+    for (int c = 0; c < 4; ++c) tt.obs_s[i][c] = i < tt.ns ? d->obs_spheres[4 * i + c] : (c < 3 ? RK_SDF_FAR : 0.f);
+  for (int i = 0; i < RK_SCENE_MAXB; ++i)
+    for (int c = 0; c < 6; ++c) tt.obs_b[i][c] = i < tt.nb ? d->obs_boxes[6 * i + c] : (c < 3 ? RK_SDF_FAR : 0.f);
   return RK_OK;
 }
 

@@ -35,40 +35,46 @@ typedef const __attribute__((address_space(4))) TrajTerms TrajK;
 // scene signed distance at p: minimum over obstacle spheres, then boxes, the first strict minimum
 // (geom/sdf_kernels.py:256-295, trajectory_collision_task.py:62-72).  Returns the distance and
 // the winner (win >= 0: sphere win; win < 0: box -1 - win; RK_SDF_NONE: no primitive closer
-// than 1e6); scene_normal gives the gradient
-// direction of that primitive, computed only where a caller needs it.  A sphere's square root is
-// taken only when its squared distance can reach below the current best (the bound carries a
-// 1e-5 relative slack, so the exact comparison of the reference's formula decides every case).
+// than 1e6); scene_normal gives the gradient direction of that primitive, computed only where a
+// caller needs it.  The tables are padded with far-away primitives to whole groups (8 spheres,
+// 4 boxes); a group's data is read through the scalar cache before any of it is used, and every
+// distance is branch-free.
 #define RK_SDF_NONE (1 << 30)
+#define RK_SDF_FAR 1e18f  // centre coordinate of a padding primitive (its squared distance stays finite)
 __device__ __forceinline__ float scene_sdf(TrajK* tt, const float* p, int& win) {
   float best = 1e6f;
   win = RK_SDF_NONE;
-  // (unrolled: the scene is read through the scalar cache, and a rolled loop waits on every load)
-#pragma unroll 8
-  for (int i = 0; i < tt->ns; ++i) {
-    const float d0 = p[0] - tt->obs_s[i][0], d1 = p[1] - tt->obs_s[i][1], d2 = p[2] - tt->obs_s[i][2];
-    const float r = tt->obs_s[i][3];
-    const float d2s = d0 * d0 + d1 * d1 + d2 * d2, br = best + r;
-    if (br > 0.f && d2s <= br * br * (1.f + 1e-5f) + 1e-30f) {
-      const float dist = sqrtf(d2s) - r;
-      if (dist < best) {
-        best = dist;
-        win = i;
-      }
+  const int ns = tt->ns, nb = tt->nb;
+  for (int i0 = 0; i0 < ns; i0 += 8) {
+    float o[8][4];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) o[u][c] = tt->obs_s[i0 + u][c];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const float d0 = p[0] - o[u][0], d1 = p[1] - o[u][1], d2 = p[2] - o[u][2];
+      const float dist = __builtin_amdgcn_sqrtf(d0 * d0 + d1 * d1 + d2 * d2) - o[u][3];
+      win = dist < best ? i0 + u : win;
+      best = dist < best ? dist : best;
     }
   }
-#pragma unroll 4
-  for (int i = 0; i < tt->nb; ++i) {
-    float qv[3];
+  for (int i0 = 0; i0 < nb; i0 += 4) {
+    float o[4][6];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) qv[k] = fabsf(p[k] - tt->obs_b[i][k]) - tt->obs_b[i][3 + k];
-    float qmax = qv[0] > qv[1] ? qv[0] : qv[1];
-    if (qv[2] > qmax) qmax = qv[2];
-    const float p0 = qv[0] > 0.f ? qv[0] : 0.f, p1 = qv[1] > 0.f ? qv[1] : 0.f, p2 = qv[2] > 0.f ? qv[2] : 0.f;
-    const float dist = sqrtf(p0 * p0 + p1 * p1 + p2 * p2) + (qmax < 0.f ? qmax : 0.f);
-    if (dist < best) {
-      best = dist;
-      win = -1 - i;
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) o[u][c] = tt->obs_b[i0 + u][c];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      float qv[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) qv[k] = fabsf(p[k] - o[u][k]) - o[u][3 + k];
+      const float qmax = fmaxf(fmaxf(qv[0], qv[1]), qv[2]);
+      const float p0 = fmaxf(qv[0], 0.f), p1 = fmaxf(qv[1], 0.f), p2 = fmaxf(qv[2], 0.f);
+      const float dist = __builtin_amdgcn_sqrtf(p0 * p0 + p1 * p1 + p2 * p2) + fminf(qmax, 0.f);
+      win = dist < best ? -1 - (i0 + u) : win;
+      best = dist < best ? dist : best;
     }
   }
   return best;
@@ -235,11 +241,23 @@ __device__ __forceinline__ void traj_frame(RobotK* rb, TrajK* tt, const float (&
 
 template <int NA>
 struct TrajLDS {
-  static constexpr int W = 2 * NA;  // half-bandwidth
-  float band[RK_TRAJ_MAXT * NA][W + 1];
-  float rhs[RK_TRAJ_MAXT * NA];
-  float dinv[RK_TRAJ_MAXT * NA];
+  static constexpr int W = 2 * NA;          // half-bandwidth
+  static constexpr int BS = (W + 4) & ~3;   // band row stride (>= W + 1, 16-byte rows)
+  // band[r][k] = A[r][r-k]; after the solve L[r][r-k], 1/L[r][r] at k = 0.  Rows D..D+W stay
+  // zero (they enter the register window past the end); sink takes the stores of idle lanes.
+  float band[RK_TRAJ_MAXT * NA + W + 1][BS];
+  float rhs[RK_TRAJ_MAXT * NA + W + 1];
+  float sink[64];
 };
+
+// one-lane wavefront shifts (DPP wave_shl1 / wave_shr1): lane l takes lane l+1 (l-1); the lane
+// with no source (63 / 0) takes `fill`
+__device__ __forceinline__ float wv_shl1(float src, float fill) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(src), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float wv_shr1(float src, float fill) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(src), 0x138, 0xF, 0xF, false));
+}
 
 // finite-difference rows of lane t (velocity row t, acceleration window starting at t) and their
 // J^T r contribution to frame t; q of frames t-2..t+2 come from the neighbouring lanes
@@ -323,6 +341,9 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
     const float c0 = 0.5f * wv_sum(cg), c1 = 0.5f * wv_sum(cv), c2 = 0.5f * wv_sum(ca), c3 = 0.5f * wv_sum(cc);
     return ((c0 + c1) + c2) + c3;
   };
+  // rows D..D+W of the band and the right-hand side are zero: they enter the Cholesky window past the end
+  for (int e = lane; e < (W + 1) * TrajLDS<NA>::BS; e += 64) sh->band[D + e / TrajLDS<NA>::BS][e % TrajLDS<NA>::BS] = 0.f;
+  if (lane <= W) sh->rhs[D + lane] = 0.f;
   float cost = 0.f;
   if (iters == 0) cost = eval_cost(q);
   for (int it = 0; it < iters; ++it) {
@@ -364,43 +385,99 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
       }
     }
     __syncthreads();
-    // ---- band Cholesky, column by column
-    for (int j = 0; j < D; ++j) {
-      const float d = sqrtf(sh->band[j][0]);
-      const float inv = 1.f / d;
-      float Lv = 0.f;
-      if (lane >= 1 && lane <= W && j + lane < D) {
-        Lv = sh->band[j + lane][lane] * inv;
-      }
-      __syncthreads();
-      if (lane >= 1 && lane <= W && j + lane < D) sh->band[j + lane][lane] = Lv;
-      if (lane == 0) {
-        sh->band[j][0] = d;
-        sh->dinv[j] = inv;
-      }
+    // ---- band Cholesky with the forward substitution L y = -g fused, in registers.  The active
+    // rows j..j+W sit on lanes WL..63 (row j+a on lane WL+a) in column-relative form
+    // C[c] = A[j+a][j+c], so column j is C[0] on every lane.  Per column: the pivot from lane WL,
+    // the multipliers L = C[0] / d, the rank-1 update C[c] -= L * (L of lane WL+c), then a
+    // one-lane DPP shift moves the window down and row j+W+1 enters at lane 63 as the shift's fill
+    // value.  L goes back to the band in LDS for the backward substitution, 1/d to its diagonal.
+    {
+      constexpr int WL = 63 - W;
+      const int wa = lane - WL;
+      float C[W + 1];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int a = pa[u], b2 = pbv[u];
-        const float La = __shfl(Lv, a), Lb = __shfl(Lv, b2);
-        if (a > 0 && j + a < D) sh->band[j + a][a - b2] -= La * Lb;
+      for (int c = 0; c <= W; ++c) {
+        const bool ok = wa >= c && wa < D;
+        C[c] = sh->band[ok ? wa : 0][ok ? wa - c : 0];
+        C[c] = ok ? C[c] : 0.f;
       }
-      __syncthreads();
+      const bool okr = wa >= 0 && wa < D;
+      float rc = sh->rhs[okr ? wa : 0];
+      rc = okr ? rc : 0.f;
+      float piv = wv_readlane(C[0], WL);
+      for (int j = 0; j < D; ++j) {
+        const int rn = j + W + 1;  // the row entering the window (a zero row past D), reversed
+        float P[W + 1];
+#pragma unroll
+        for (int c = 0; c <= W; ++c) P[c] = sh->band[rn][W - c];
+        const float pr = sh->rhs[rn];
+        const float inv = __builtin_amdgcn_rsqf(piv);  // 1 / L[j][j]
+        const float Lv = C[0] * inv;
+        const float y = wv_readlane(rc, WL) * inv;
+        float* const dl = wa >= 1 && j + wa < D ? &sh->band[j + wa][wa] : &sh->sink[lane];
+        *dl = Lv;
+        float* const di = wa == 0 ? &sh->band[j][0] : &sh->sink[lane];
+        float* const dy = wa == 0 ? &sh->rhs[j] : &sh->sink[lane];
+        *di = inv;
+        *dy = y;
+        rc = fmaf(-Lv, y, rc);
+#pragma unroll
+        for (int c = 1; c <= W; ++c) C[c] = fmaf(-Lv, wv_readlane(Lv, WL + c), C[c]);
+        piv = wv_readlane(C[1], WL + 1);  // the next pivot (lane WL+1 moves to lane WL)
+#pragma unroll
+        for (int c = 0; c < W; ++c) C[c] = wv_shl1(C[c + 1], P[c]);
+        C[W] = P[W];  // (lanes below 63 hold the upper triangle there: never read)
+        rc = wv_shl1(rc, pr);
+      }
     }
-    // ---- L y = -g, then L^T delta = y (in place in rhs)
-    for (int j = 0; j < D; ++j) {
-      const float y = sh->rhs[j] * sh->dinv[j];
-      __syncthreads();
-      if (lane == 0) sh->rhs[j] = y;
-      if (lane >= 1 && lane <= W && j + lane < D) sh->rhs[j + lane] -= sh->band[j + lane][lane] * y;
-      __syncthreads();
+    __syncthreads();
+    // ---- L^T delta = y the same way: rows j-W..j on lanes 0..W (row j-W+l on lane l), the pivot
+    // on lane W; the window moves one lane up and row j-W-1 enters at lane 0.  The band values of
+    // the next TB columns are loaded while the current TB are solved.
+    {
+      constexpr int TB = 8;
+      const int wb = W - lane;  // this lane's row is j - wb
+      const bool okb = lane <= W && D - 1 - wb >= 0;
+      float rb = sh->rhs[okb ? D - 1 - wb : 0];
+      rb = okb ? rb : 0.f;
+      float Lc[TB], ic[TB], pc[TB];
+      auto load = [&](int jb, float (&Lo)[TB], float (&io)[TB], float (&po)[TB]) {
+#pragma unroll
+        for (int u = 0; u < TB; ++u) {
+          const int j = jb - u;
+          const bool okj = j >= 0;
+          const bool okl = okj && wb >= 1 && wb <= W && j - wb >= 0;
+          Lo[u] = sh->band[okl ? j : 0][okl ? wb : 0];
+          Lo[u] = okl ? Lo[u] : 0.f;
+          io[u] = sh->band[okj ? j : 0][0];
+          const int rn = j - W - 1;
+          po[u] = sh->rhs[rn >= 0 ? rn : 0];
+          po[u] = rn >= 0 ? po[u] : 0.f;
+        }
+      };
+      load(D - 1, Lc, ic, pc);
+      for (int jb = D - 1; jb >= 0; jb -= TB) {
+        float Ln[TB], in[TB], pn[TB];
+        load(jb - TB, Ln, in, pn);
+#pragma unroll
+        for (int u = 0; u < TB; ++u) {
+          const int j = jb - u;
+          if (j >= 0) {
+            const float x = wv_readlane(rb, W) * ic[u];
+            if (lane == W) sh->rhs[j] = x;
+            rb = fmaf(-Lc[u], x, rb);
+            rb = wv_shr1(rb, pc[u]);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < TB; ++u) {
+          Lc[u] = Ln[u];
+          ic[u] = in[u];
+          pc[u] = pn[u];
+        }
+      }
     }
-    for (int j = D - 1; j >= 0; --j) {
-      const float x = sh->rhs[j] * sh->dinv[j];
-      __syncthreads();
-      if (lane == 0) sh->rhs[j] = x;
-      if (lane >= 1 && lane <= W && j - lane >= 0) sh->rhs[j - lane] -= sh->band[j][lane] * x;
-      __syncthreads();
-    }
+    __syncthreads();
     float dq[NA], qp[NA];
     float pr = 0.f;
 #pragma unroll
