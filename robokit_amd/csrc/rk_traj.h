@@ -243,10 +243,10 @@ template <int NA>
 struct TrajLDS {
   static constexpr int W = 2 * NA;          // half-bandwidth
   static constexpr int BS = (W + 4) & ~3;   // band row stride (>= W + 1, 16-byte rows)
-  // band[r][k] = A[r][r-k]; after the solve L[r][r-k], 1/L[r][r] at k = 0.  Rows D..D+W stay
+  // band[r][k] = A[r][r-k]; after the solve L[r][r-k], 1/L[r][r] at k = 0.  Rows D..D+W+1 stay
   // zero (they enter the register window past the end); sink takes the stores of idle lanes.
-  float band[RK_TRAJ_MAXT * NA + W + 1][BS];
-  float rhs[RK_TRAJ_MAXT * NA + W + 1];
+  float band[RK_TRAJ_MAXT * NA + W + 2][BS];
+  float rhs[RK_TRAJ_MAXT * NA + W + 2];
   float sink[64];
 };
 
@@ -342,8 +342,8 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
     return ((c0 + c1) + c2) + c3;
   };
   // rows D..D+W of the band and the right-hand side are zero: they enter the Cholesky window past the end
-  for (int e = lane; e < (W + 1) * TrajLDS<NA>::BS; e += 64) sh->band[D + e / TrajLDS<NA>::BS][e % TrajLDS<NA>::BS] = 0.f;
-  if (lane <= W) sh->rhs[D + lane] = 0.f;
+  for (int e = lane; e < (W + 2) * TrajLDS<NA>::BS; e += 64) sh->band[D + e / TrajLDS<NA>::BS][e % TrajLDS<NA>::BS] = 0.f;
+  if (lane <= W + 1) sh->rhs[D + lane] = 0.f;
   float cost = 0.f;
   if (iters == 0) cost = eval_cost(q);
   for (int it = 0; it < iters; ++it) {
@@ -405,29 +405,44 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
       float rc = sh->rhs[okr ? wa : 0];
       rc = okr ? rc : 0.f;
       float piv = wv_readlane(C[0], WL);
-      for (int j = 0; j < D; ++j) {
-        const int rn = j + W + 1;  // the row entering the window (a zero row past D), reversed
-        float P[W + 1];
+      // one column: X is the window, Y the row entering it at this step (j + W + 1, a zero row
+      // past D, reversed to column-relative order), Z receives the row of the next step.  The
+      // shifted window is written over Y (the DPP fill operand), so three register sets rotate
+      // through the steps without copies.
+      auto step = [&](int j, float (&X)[W + 1], float (&Y)[W + 1], float (&Z)[W + 1], float& xr, float& yr,
+                      float& zr) {
+        {
+          const int rn = j + W + 2;
 #pragma unroll
-        for (int c = 0; c <= W; ++c) P[c] = sh->band[rn][W - c];
-        const float pr = sh->rhs[rn];
+          for (int c = 0; c <= W; ++c) Z[c] = sh->band[rn][W - c];
+          zr = sh->rhs[rn];
+        }
         const float inv = __builtin_amdgcn_rsqf(piv);  // 1 / L[j][j]
-        const float Lv = C[0] * inv;
-        const float y = wv_readlane(rc, WL) * inv;
+        const float Lv = X[0] * inv;
+        const float y = wv_readlane(xr, WL) * inv;
         float* const dl = wa >= 1 && j + wa < D ? &sh->band[j + wa][wa] : &sh->sink[lane];
         *dl = Lv;
         float* const di = wa == 0 ? &sh->band[j][0] : &sh->sink[lane];
         float* const dy = wa == 0 ? &sh->rhs[j] : &sh->sink[lane];
         *di = inv;
         *dy = y;
-        rc = fmaf(-Lv, y, rc);
+        xr = fmaf(-Lv, y, xr);
 #pragma unroll
-        for (int c = 1; c <= W; ++c) C[c] = fmaf(-Lv, wv_readlane(Lv, WL + c), C[c]);
-        piv = wv_readlane(C[1], WL + 1);  // the next pivot (lane WL+1 moves to lane WL)
+        for (int c = 1; c <= W; ++c) X[c] = fmaf(-Lv, wv_readlane(Lv, WL + c), X[c]);
+        piv = wv_readlane(X[1], WL + 1);  // the next pivot (lane WL+1 moves to lane WL)
 #pragma unroll
-        for (int c = 0; c < W; ++c) C[c] = wv_shl1(C[c + 1], P[c]);
-        C[W] = P[W];  // (lanes below 63 hold the upper triangle there: never read)
-        rc = wv_shl1(rc, pr);
+        for (int c = 0; c < W; ++c) Y[c] = wv_shl1(X[c + 1], Y[c]);
+        // (Y[W] keeps the entering row's diagonal; lanes below 63 hold the upper triangle there)
+        yr = wv_shl1(xr, yr);
+      };
+      float P1[W + 1], P2[W + 1], r1, r2;
+#pragma unroll
+      for (int c = 0; c <= W; ++c) P1[c] = sh->band[W + 1][W - c];
+      r1 = sh->rhs[W + 1];
+      for (int j = 0; j < D; j += 3) {
+        step(j, C, P1, P2, rc, r1, r2);
+        if (j + 1 < D) step(j + 1, P1, P2, C, r1, r2, rc);
+        if (j + 2 < D) step(j + 2, P2, C, P1, r2, rc, r1);
       }
     }
     __syncthreads();
