@@ -1485,9 +1485,9 @@ static int launch_variant(hipStream_t st, int v, const Problems& P, const LMPara
 static int launch_traj(hipStream_t st, int na, const TrajTerms* tt, const Problems& P, const LMParams& lp, int it0,
                        int iters) {
   if (na == 6)
-    hipLaunchKernelGGL((k_traj<6, 8>), dim3(P.n), dim3(WAVE), 0, st, P, tt, lp, it0, iters);
+    hipLaunchKernelGGL((k_traj<6, 8>), dim3(P.n), dim3(2 * WAVE), 0, st, P, tt, lp, it0, iters);
   else
-    hipLaunchKernelGGL((k_traj<7, 9>), dim3(P.n), dim3(WAVE), 0, st, P, tt, lp, it0, iters);
+    hipLaunchKernelGGL((k_traj<7, 9>), dim3(P.n), dim3(2 * WAVE), 0, st, P, tt, lp, it0, iters);
   HIP_TRY(hipGetLastError());
   return RK_OK;
 }

@@ -1,15 +1,17 @@
-// rk_traj.h -- collision-aware trajectory LM, one wavefront per problem (included by rk_lm.hip
+// rk_traj.h -- collision-aware trajectory LM, two wavefronts per problem (included by rk_lm.hip
 // inside namespace rk, after rk_thread.h).
 //
 // A problem is a trajectory of T <= 64 frames of a serial arm with NA dofs: D = T*NA variables,
-// frame-major (index t*NA + d).  Lane t of the wave owns frame t: its FK, its collision spheres
+// frame-major (index t*NA + d).  Lane t of a wave owns frame t: its FK, its collision spheres
 // against the primitive scene, the goal pose on the last frame, its finite-difference rows, and
-// the NA x NA diagonal block of J^T J and the frame's part of J^T r, all in registers.  J^T J is
-// banded (half-bandwidth W = 2*NA: acceleration rows couple frames t and t+2): the band lives in
-// LDS, [D][W+1], and is factorised in place by the right-looking band Cholesky, the W(W+1)/2
-// updates of each column spread over the lanes; the two triangular solves follow the same
-// pattern.  Rows and term costs in this order: goal, velocity, acceleration, collision (the
-// reference sums term costs in list order).
+// the NA x NA diagonal block of J^T J and the frame's part of J^T r, all in registers.  The two
+// wavefronts of a problem split the robot spheres; the first adds the second's rows from LDS and
+// solves.  J^T J is banded (half-bandwidth W = 2*NA: acceleration rows couple frames t and t+2):
+// the band is built in LDS, [D][W+1], and factorised by a right-looking band Cholesky whose W+1
+// active rows live in registers of the top W+1 lanes, shifted one lane per column; the forward
+// substitution is fused into it and the backward one runs the same way.  Rows and term costs in
+// this order: goal, velocity, acceleration, collision (the reference sums term costs in list
+// order).
 
 #define RK_TRAJ_MAXT 64
 #define RK_TRAJ_MAXS 64  // robot collision spheres
@@ -131,8 +133,10 @@ __device__ __forceinline__ void scene_normal(TrajK* tt, const float* p, int win,
 // per-frame evaluation: FK streamed joint by joint with the spheres of each link processed as
 // its pose appears; JAC also accumulates the frame's block of J^T J (lower, hb[a][b], b <= a)
 // and J^T r (gf) of the goal and collision rows.  Returns the half sums of squares of the goal
-// rows (lane T-1 only) in cg and of the collision rows in cc.
-template <int NA, int NJ, bool JAC>
+// rows (lane T-1 only) in cg and of the collision rows in cc.  The two wavefronts of a problem
+// split the robot spheres: PART 0 takes the even-numbered ones and the goal rows, PART 1 the
+// odd-numbered ones.
+template <int NA, int NJ, bool JAC, int PART>
 __device__ __forceinline__ void traj_frame(RobotK* rb, TrajK* tt, const float (&q)[NA], bool last, const Pose& goal,
                                            float (&hb)[NA][NA], float (&gf)[NA], float& cg, float& cc) {
   const int nj = rb->nj;
@@ -143,6 +147,7 @@ __device__ __forceinline__ void traj_frame(RobotK* rb, TrajK* tt, const float (&
   int k = 0;
   auto spheres = [&](int joint, const Pose& P, int ndof) {
     for (; k < tt->nsph && tt->sph_joint[k] == joint; ++k) {
+      if ((k & 1) != PART) continue;
       const float lc[3] = {tt->sph_c[k][0], tt->sph_c[k][1], tt->sph_c[k][2]};
       float c[3];
       quat_apply(P.q, lc, c);
@@ -201,6 +206,7 @@ __device__ __forceinline__ void traj_frame(RobotK* rb, TrajK* tt, const float (&
       spheres(j, T, j < NA ? j + 1 : NA);
     }
   }
+  if (PART != 0) return;
   // goal: the FrameTask log rows of the last frame (every lane evaluates; lane T-1 keeps them)
   float gw[6];
 #pragma unroll
@@ -248,6 +254,12 @@ struct TrajLDS {
   float band[RK_TRAJ_MAXT * NA + W + 2][BS];
   float rhs[RK_TRAJ_MAXT * NA + W + 2];
   float sink[64];
+  // from the second wavefront: its frame block of J^T J (lower), J^T r and collision cost at the
+  // iterate, and its collision cost at the proposal; to it: the iterate after each step
+  static constexpr int NPART = NA * (NA + 1) / 2 + NA + 1;
+  float part[NPART][64];
+  float pcc[64];
+  float qx[NA][64];
 };
 
 // one-lane wavefront shifts (DPP wave_shl1 / wave_shr1): lane l takes lane l+1 (l-1); the lane
@@ -322,21 +334,13 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
   const float hs0 = wv2 * (float)(in_v(t) + in_v(t - 1)) + wa2 * (1.f * in_a(t) + 4.f * in_a(t - 1) + 1.f * in_a(t - 2));
   const float hs1m = wv2 * (in_v(t - 1) ? -1.f : 0.f) + wa2 * (-2.f * in_a(t - 1) - 2.f * in_a(t - 2));  // (t, t-1)
   const float hs2m = wa2 * (float)in_a(t - 2);                                                            // (t, t-2)
-  // lane p's (a, b) pairs of the band update, 1 <= b <= a <= W
-  int pa[2] = {0, 0}, pbv[2] = {0, 0};
-  {
-    int idx = 0;
-    for (int a = 1; a <= W; ++a)
-      for (int b2 = 1; b2 <= a; ++b2, ++idx) {
-        if (idx == lane) { pa[0] = a; pbv[0] = b2; }
-        if (idx == lane + 64) { pa[1] = a; pbv[1] = b2; }
-      }
-  }
   auto eval_cost = [&](const float (&qq)[NA]) {
     float hb[NA][NA], gf[NA];
     float cg = 0.f, cc = 0.f, cv = 0.f, ca = 0.f;
-    traj_frame<NA, NJ, false>(rb, tt, qq, last, goal, hb, gf, cg, cc);
+    traj_frame<NA, NJ, false, 0>(rb, tt, qq, last, goal, hb, gf, cg, cc);
     traj_smooth<NA, false>(tt, t, T, qq, gf, cv, ca);
+    __syncthreads();  // the second wavefront's collision cost (odd spheres)
+    cc += sh->pcc[lane];
     if (!own) cg = cc = cv = ca = 0.f;
     const float c0 = 0.5f * wv_sum(cg), c1 = 0.5f * wv_sum(cv), c2 = 0.5f * wv_sum(ca), c3 = 0.5f * wv_sum(cc);
     return ((c0 + c1) + c2) + c3;
@@ -363,8 +367,19 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
       for (int b2 = 0; b2 < NA; ++b2) hb[a][b2] = 0.f;
     }
     float cg = 0.f, cc = 0.f, cv = 0.f, ca = 0.f;
-    traj_frame<NA, NJ, true>(rb, tt, q, last, goal, hb, gf, cg, cc);
+    traj_frame<NA, NJ, true, 0>(rb, tt, q, last, goal, hb, gf, cg, cc);
     traj_smooth<NA, true>(tt, t, T, q, gf, cv, ca);
+    __syncthreads();  // the second wavefront's rows (odd spheres)
+    {
+      int e = 0;
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b2 = 0; b2 <= a; ++b2) hb[a][b2] += sh->part[e++][lane];
+#pragma unroll
+      for (int a = 0; a < NA; ++a) gf[a] += sh->part[e++][lane];
+      cc += sh->part[e][lane];
+    }
     if (!own) cg = cc = cv = ca = 0.f;
     cost = ((0.5f * wv_sum(cg) + 0.5f * wv_sum(cv)) + 0.5f * wv_sum(ca)) + 0.5f * wv_sum(cc);
     // ---- band of J^T J + lam I (rows of frame t), right-hand side J^T r
@@ -447,43 +462,42 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
     }
     __syncthreads();
     // ---- L^T delta = y the same way: rows j-W..j on lanes 0..W (row j-W+l on lane l), the pivot
-    // on lane W; the window moves one lane up and row j-W-1 enters at lane 0.  The band values of
-    // the next TB columns are loaded while the current TB are solved.
+    // on lane W; the window moves one lane up and row j-W-1 enters at lane 0.  Lane l applies
+    // L[j][j-(W-l)] = band[j][W-l]; entries left of column 0 are zero from the build, and the
+    // lanes outside the window (and the pivot lane, which leaves it) read a harmless in-range
+    // entry.  The values of the next TB columns are loaded while the current TB are solved.
     {
       constexpr int TB = 8;
-      const int wb = W - lane;  // this lane's row is j - wb
+      const int wb = W - lane;
+      const int wbc = wb < 0 ? 0 : wb;
       const bool okb = lane <= W && D - 1 - wb >= 0;
       float rb = sh->rhs[okb ? D - 1 - wb : 0];
       rb = okb ? rb : 0.f;
+      float* const dx = lane == W ? &sh->rhs[0] : &sh->sink[lane];
+      const int dxs = lane == W ? 1 : 0;
+      auto step = [&](int j, float Lj, float ij, float pj) {
+        const float x = wv_readlane(rb, W) * ij;
+        dx[dxs * j] = x;
+        rb = fmaf(-Lj, x, rb);
+        rb = wv_shr1(rb, pj);
+      };
       float Lc[TB], ic[TB], pc[TB];
       auto load = [&](int jb, float (&Lo)[TB], float (&io)[TB], float (&po)[TB]) {
 #pragma unroll
         for (int u = 0; u < TB; ++u) {
-          const int j = jb - u;
-          const bool okj = j >= 0;
-          const bool okl = okj && wb >= 1 && wb <= W && j - wb >= 0;
-          Lo[u] = sh->band[okl ? j : 0][okl ? wb : 0];
-          Lo[u] = okl ? Lo[u] : 0.f;
-          io[u] = sh->band[okj ? j : 0][0];
-          const int rn = j - W - 1;
-          po[u] = sh->rhs[rn >= 0 ? rn : 0];
-          po[u] = rn >= 0 ? po[u] : 0.f;
+          const int j = max(jb - u, 0);
+          Lo[u] = sh->band[j][wbc];
+          io[u] = sh->band[j][0];
+          po[u] = sh->rhs[max(jb - u - W - 1, 0)];  // (the rows before 0 never become pivots)
         }
       };
-      load(D - 1, Lc, ic, pc);
-      for (int jb = D - 1; jb >= 0; jb -= TB) {
+      int jb = D - 1;
+      load(jb, Lc, ic, pc);
+      for (; jb >= TB - 1; jb -= TB) {
         float Ln[TB], in[TB], pn[TB];
         load(jb - TB, Ln, in, pn);
 #pragma unroll
-        for (int u = 0; u < TB; ++u) {
-          const int j = jb - u;
-          if (j >= 0) {
-            const float x = wv_readlane(rb, W) * ic[u];
-            if (lane == W) sh->rhs[j] = x;
-            rb = fmaf(-Lc[u], x, rb);
-            rb = wv_shr1(rb, pc[u]);
-          }
-        }
+        for (int u = 0; u < TB; ++u) step(jb - u, Lc[u], ic[u], pc[u]);
 #pragma unroll
         for (int u = 0; u < TB; ++u) {
           Lc[u] = Ln[u];
@@ -491,6 +505,9 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
           pc[u] = pn[u];
         }
       }
+#pragma unroll
+      for (int u = 0; u < TB - 1; ++u)
+        if (jb - u >= 0) step(jb - u, Lc[u], ic[u], pc[u]);
     }
     __syncthreads();
     float dq[NA], qp[NA];
@@ -514,6 +531,8 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
       Pg.trace[(gi + 1) * S + s] = cost;
       Pg.acc_trace[gi * S + s] = acc;
     }
+#pragma unroll
+    for (int d = 0; d < NA; ++d) sh->qx[d][lane] = q[d];
     __syncthreads();
   }
   if (own)
@@ -526,11 +545,77 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
   }
 }
 
+// the second wavefront of a trajectory problem: the collision rows of the odd-numbered robot
+// spheres at the iterate and at each proposal, handed over in LDS.  Its barriers pair one for
+// one with traj_solve's: rows ready, band built, factorised, solved, proposal cost ready, iterate
+// published.
 template <int NA, int NJ>
-__global__ __launch_bounds__(64) void k_traj(Problems P, const TrajTerms* tt, LMParams lp, int it0, int iters) {
+__device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s, int it0, int iters,
+                                            TrajLDS<NA>* sh) {
+  RobotK* rb = as_const(Pg.robot);
+  const int lane = threadIdx.x & 63;
+  const int T = tt->T, S = Pg.stride;
+  const int t = lane < T ? lane : T - 1;
+  float q[NA];
+#pragma unroll
+  for (int d = 0; d < NA; ++d) {
+    const int k = t * NA + d;
+    q[d] = (it0 == 0 && Pg.q_seed) ? Pg.q_seed[k * S + s] : Pg.q[k * S + s];
+  }
+  const Pose goal = identity_pose();  // (no goal rows in this part)
+  auto proposal_cost = [&](const float (&qq)[NA]) {
+    float hb[NA][NA], gf[NA];
+    float cg = 0.f, cc = 0.f;
+    traj_frame<NA, NJ, false, 1>(rb, tt, qq, false, goal, hb, gf, cg, cc);
+    sh->pcc[lane] = cc;
+    __syncthreads();
+  };
+  if (iters == 0) {
+    proposal_cost(q);
+    return;
+  }
+  for (int it = 0; it < iters; ++it) {
+    {
+      float hb[NA][NA], gf[NA];
+#pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        gf[a] = 0.f;
+#pragma unroll
+        for (int b2 = 0; b2 < NA; ++b2) hb[a][b2] = 0.f;
+      }
+      float cg = 0.f, cc = 0.f;
+      traj_frame<NA, NJ, true, 1>(rb, tt, q, false, goal, hb, gf, cg, cc);
+      int e = 0;
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b2 = 0; b2 <= a; ++b2) sh->part[e++][lane] = hb[a][b2];
+#pragma unroll
+      for (int a = 0; a < NA; ++a) sh->part[e++][lane] = gf[a];
+      sh->part[e][lane] = cc;
+    }
+    __syncthreads();  // rows ready
+    __syncthreads();  // band built
+    __syncthreads();  // factorised
+    __syncthreads();  // solved
+    float qp[NA];
+#pragma unroll
+    for (int d = 0; d < NA; ++d) qp[d] = q[d] + sh->rhs[t * NA + d];
+    proposal_cost(qp);  // (+ barrier: proposal cost ready)
+    __syncthreads();  // iterate published
+#pragma unroll
+    for (int d = 0; d < NA; ++d) q[d] = sh->qx[d][lane];
+  }
+}
+
+template <int NA, int NJ>
+__global__ __launch_bounds__(128) void k_traj(Problems P, const TrajTerms* tt, LMParams lp, int it0, int iters) {
   __shared__ __align__(16) TrajLDS<NA> sh;
   if ((int)blockIdx.x >= P.n) return;
-  traj_solve<NA, NJ>(P, as_const(tt), blockIdx.x, lp, it0, iters, &sh);
+  if (threadIdx.x < 64)
+    traj_solve<NA, NJ>(P, as_const(tt), blockIdx.x, lp, it0, iters, &sh);
+  else
+    traj_worker<NA, NJ>(P, as_const(tt), blockIdx.x, it0, iters, &sh);
 }
 
 // goal errors at the last frame and the smallest sphere-to-scene distance (one wave per problem)
