@@ -260,6 +260,9 @@ struct TrajLDS {
   float part[NPART][64];
   float pcc[64];
   float qx[NA][64];
+  // the second wavefront's Schur update of the separator rows (upper triangle) and right-hand side
+  float sx[W][W];
+  float rx[W];
 };
 
 // one-lane wavefront shifts (DPP wave_shl1 / wave_shr1): lane l takes lane l+1 (l-1); the lane
@@ -269,6 +272,247 @@ __device__ __forceinline__ float wv_shl1(float src, float fill) {
 }
 __device__ __forceinline__ float wv_shr1(float src, float fill) {
   return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(src), 0x138, 0xF, 0xF, false));
+}
+
+// ---- the damped band solve (J^T J + lam I) dq = -g, both wavefronts taking part.
+//
+// Register window (one wavefront): the active rows j..j+W sit on lanes WL..63 (row j+a on lane
+// WL+a) in column-relative form C[c] = A[j+a][j+c], so column j is C[0] on every lane.  Per
+// column: the pivot from lane WL, multipliers L = C[0] / d, the rank-1 update C[c] -= L * (L of
+// lane WL+c), the forward substitution on rc, then a one-lane DPP shift moves the window down and
+// row j+W+1 enters at lane 63 as the shift's fill value.  L goes back to the band in LDS, 1/d to
+// its diagonal slot, y to rhs.  REV runs the same on the reversed band: index i is row D-1-i, and
+// entry (r, c), c <= r, of the reversed matrix sits at band[D-1-c][r-c].  ZERO: the entering
+// rows are zero (nothing follows the columns factorised).  Factorises columns [j0, j0 + n) from
+// the window in (C, rc) and leaves the next window there; L is stored for rows below lim.
+template <int NA, bool REV, bool ZERO>
+__device__ __forceinline__ void band_factor(TrajLDS<NA>* sh, int D, int j0, int n, int lim, float (&C)[2 * NA + 1],
+                                            float& rc) {
+  constexpr int W = 2 * NA, WL = 63 - W;
+  const int lane = threadIdx.x & 63, wa = lane - WL;
+  auto at = [&](int r, int c) -> float* { return REV ? &sh->band[D - 1 - c][r - c] : &sh->band[r][r - c]; };
+  auto orig = [&](int i) { return REV ? D - 1 - i : i; };
+  float piv = wv_readlane(C[0], WL);
+  // one column: X is the window, Y the row entering it at this step (j + W + 1), Z receives the
+  // row of the next step.  The shifted window is written over Y (the DPP fill operand), so three
+  // register sets rotate through the steps without copies.
+  auto step = [&](int j, float (&X)[W + 1], float (&Y)[W + 1], float (&Z)[W + 1], float& xr, float& yr,
+                  float& zr) {
+    if (ZERO) {
+#pragma unroll
+      for (int c = 0; c <= W; ++c) Z[c] = 0.f;
+      zr = 0.f;
+    } else {
+      const int rn = j + W + 2;
+#pragma unroll
+      for (int c = 0; c <= W; ++c) Z[c] = *at(rn, j + 2 + c);
+      zr = sh->rhs[orig(rn)];
+    }
+    const float inv = __builtin_amdgcn_rsqf(piv);  // 1 / L[j][j]
+    const float Lv = X[0] * inv;
+    const float y = wv_readlane(xr, WL) * inv;
+    float* const dl = wa >= 1 && j + wa < lim ? at(j + wa, j) : &sh->sink[lane];
+    *dl = Lv;
+    float* const di = wa == 0 ? at(j, j) : &sh->sink[lane];
+    float* const dy = wa == 0 ? &sh->rhs[orig(j)] : &sh->sink[lane];
+    *di = inv;
+    *dy = y;
+    xr = fmaf(-Lv, y, xr);
+#pragma unroll
+    for (int c = 1; c <= W; ++c) X[c] = fmaf(-Lv, wv_readlane(Lv, WL + c), X[c]);
+    piv = wv_readlane(X[1], WL + 1);  // the next pivot (lane WL+1 moves to lane WL)
+#pragma unroll
+    for (int c = 0; c < W; ++c) Y[c] = wv_shl1(X[c + 1], Y[c]);
+    // (Y[W] keeps the entering row's diagonal; lanes below 63 hold the upper triangle there)
+    yr = wv_shl1(xr, yr);
+  };
+  float P1[W + 1], P2[W + 1], r1, r2;
+  if (ZERO) {
+#pragma unroll
+    for (int c = 0; c <= W; ++c) P1[c] = 0.f;
+    r1 = 0.f;
+  } else {
+    const int rn = j0 + W + 1;
+#pragma unroll
+    for (int c = 0; c <= W; ++c) P1[c] = *at(rn, j0 + 1 + c);
+    r1 = sh->rhs[orig(rn)];
+  }
+  const int je = j0 + n;
+  int j = j0;
+  for (; j + 2 < je; j += 3) {
+    step(j, C, P1, P2, rc, r1, r2);
+    step(j + 1, P1, P2, C, r1, r2, rc);
+    step(j + 2, P2, C, P1, r2, rc, r1);
+  }
+  if (j < je) {
+    step(j, C, P1, P2, rc, r1, r2);
+    if (j + 1 < je) {
+      step(j + 1, P1, P2, C, r1, r2, rc);
+#pragma unroll
+      for (int c = 0; c <= W; ++c) C[c] = P2[c];
+      rc = r2;
+    } else {
+#pragma unroll
+      for (int c = 0; c <= W; ++c) C[c] = P1[c];
+      rc = r1;
+    }
+  }
+}
+
+// the first window of a factorisation from column 0: rows 0..W of the (reversed) band
+template <int NA, bool REV>
+__device__ __forceinline__ void band_window0(TrajLDS<NA>* sh, int D, float (&C)[2 * NA + 1], float& rc) {
+  constexpr int W = 2 * NA, WL = 63 - W;
+  const int lane = threadIdx.x & 63, wa = lane - WL;
+#pragma unroll
+  for (int c = 0; c <= W; ++c) {
+    const bool ok = wa >= c;
+    const int r = ok ? wa : c;
+    C[c] = REV ? sh->band[D - 1 - c][r - c] : sh->band[r][r - c];
+    C[c] = ok ? C[c] : 0.f;
+  }
+  const int r = wa >= 0 ? wa : 0;
+  rc = sh->rhs[REV ? D - 1 - r : r];
+  rc = wa >= 0 ? rc : 0.f;
+}
+
+// L^T x = y for pivots jtop down to jstop, the same way: rows j-W..j on lanes 0..W (row j-W+l on
+// lane l), the pivot on lane W; the window (rb) moves one lane up and row j-W-1 enters at lane 0.
+// Lane l applies L[j][j-(W-l)]; entries left of column 0 are zero, and the lanes outside the
+// window (and the pivot lane, which leaves it) read a harmless in-range entry.  The values of the
+// next TB pivots are loaded while the current TB are solved.
+template <int NA, bool REV>
+__device__ __forceinline__ void band_back(TrajLDS<NA>* sh, int D, int jtop, int jstop, float& rb) {
+  constexpr int W = 2 * NA, TB = 8;
+  const int lane = threadIdx.x & 63;
+  const int wb = W - lane;
+  const int wbc = wb < 0 ? 0 : wb;
+  auto orig = [&](int i) { return REV ? D - 1 - i : i; };
+  float* const dx = lane == W ? &sh->rhs[0] : &sh->sink[lane];
+  const int dxs = lane == W ? 1 : 0;
+  auto step = [&](int j, float Lj, float ij, float pj) {
+    const float x = wv_readlane(rb, W) * ij;
+    dx[dxs * orig(j)] = x;
+    rb = fmaf(-Lj, x, rb);
+    rb = wv_shr1(rb, pj);
+  };
+  float Lc[TB], ic[TB], pc[TB];
+  auto load = [&](int jb, float (&Lo)[TB], float (&io)[TB], float (&po)[TB]) {
+#pragma unroll
+    for (int u = 0; u < TB; ++u) {
+      const int j = max(jb - u, jstop);
+      Lo[u] = REV ? sh->band[D - 1 - j + wbc][wbc] : sh->band[j][wbc];
+      io[u] = sh->band[orig(j)][0];
+      const int e = jb - u - W - 1;  // (the rows before 0 never become pivots)
+      po[u] = sh->rhs[REV ? D - 1 - max(e, -W - 2) : max(e, 0)];
+    }
+  };
+  int jb = jtop;
+  load(jb, Lc, ic, pc);
+  for (; jb - (TB - 1) >= jstop; jb -= TB) {
+    float Ln[TB], in[TB], pn[TB];
+    load(jb - TB, Ln, in, pn);
+#pragma unroll
+    for (int u = 0; u < TB; ++u) step(jb - u, Lc[u], ic[u], pc[u]);
+#pragma unroll
+    for (int u = 0; u < TB; ++u) {
+      Lc[u] = Ln[u];
+      ic[u] = in[u];
+      pc[u] = pn[u];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < TB - 1; ++u)
+    if (jb - u >= jstop) step(jb - u, Lc[u], ic[u], pc[u]);
+}
+
+// the first backward window: rows jtop-W..jtop
+template <int NA, bool REV>
+__device__ __forceinline__ float band_back_window(TrajLDS<NA>* sh, int D, int jtop) {
+  constexpr int W = 2 * NA;
+  const int lane = threadIdx.x & 63;
+  const int r = jtop - (W - lane);
+  const bool ok = lane <= W && r >= 0;
+  const float v = sh->rhs[ok ? (REV ? D - 1 - r : r) : 0];
+  return ok ? v : 0.f;
+}
+
+// Nested dissection over the two wavefronts.  The frames split into A = [0, m), a separator
+// S = {m, m+1} (W = 2*NA rows) and B = [m+2, T); no row of A couples to a row of B (the band
+// spans two frames).  The first wavefront (wv 0) eliminates A in order while the second
+// eliminates B from the end (the reversed band), each leaving its update of S in its window; the
+// first then factorises S0 + S1 - A_SS, solves S backwards and publishes x_S; then each runs the
+// backward substitution of its part.  Short trajectories (T < 6) are solved whole by the first.
+// Returns with dq in sh->rhs; takes three barriers in both wavefronts.
+template <int NA>
+__device__ __forceinline__ void band_solve(TrajLDS<NA>* sh, int T, int wv) {
+  constexpr int W = 2 * NA, WL = 63 - W;
+  const int lane = threadIdx.x & 63, wa = lane - WL;
+  const int D = T * NA;
+  const bool split = T >= 6;
+  const int m = (T - 2) / 2;
+  const int nA = m * NA, nB = (T - m - 2) * NA;
+  float C[W + 1], rc;
+  if (wv == 0) {
+    band_window0<NA, false>(sh, D, C, rc);
+    if (split)
+      band_factor<NA, false, false>(sh, D, 0, nA, nA + W, C, rc);
+    else
+      band_factor<NA, false, false>(sh, D, 0, D, D, C, rc);
+    __syncthreads();  // the second wavefront's update of S is in sx, rx
+    float rb = 0.f;
+    if (split) {
+      if (wa >= 0 && wa < W) {
+#pragma unroll
+        for (int c = 0; c < W; ++c)
+          if (c <= wa) C[c] += sh->sx[c][wa] - sh->band[nA + wa][wa - c];
+        rc += sh->rx[wa] - sh->rhs[nA + wa];
+      }
+      if (wa == W) {
+#pragma unroll
+        for (int c = 0; c <= W; ++c) C[c] = 0.f;
+        rc = 0.f;
+      }
+      band_factor<NA, false, true>(sh, D, nA, W, nA + W, C, rc);
+      rb = band_back_window<NA, false>(sh, D, nA + W - 1);
+      band_back<NA, false>(sh, D, nA + W - 1, nA, rb);
+    }
+    __syncthreads();  // x_S published
+    if (split)
+      band_back<NA, false>(sh, D, nA - 1, 0, rb);
+    else {
+      rb = band_back_window<NA, false>(sh, D, D - 1);
+      band_back<NA, false>(sh, D, D - 1, 0, rb);
+    }
+  } else {
+    if (split) {
+      band_window0<NA, true>(sh, D, C, rc);
+      band_factor<NA, true, false>(sh, D, 0, nB, D, C, rc);
+      // the window now holds the reversed separator rows: lane WL+a is row W-1-a of S
+      if (wa >= 0 && wa < W) {
+#pragma unroll
+        for (int c = 0; c < W; ++c)
+          if (c <= wa) sh->sx[W - 1 - wa][W - 1 - c] = C[c];
+        sh->rx[W - 1 - wa] = rc;
+      }
+    }
+    __syncthreads();  // update of S handed over
+    __syncthreads();  // x_S published
+    if (split) {
+      // the separator pivots (reversed nB+W-1 .. nB) with x known: their L^T rows update B only
+      float rb = band_back_window<NA, true>(sh, D, nB + W - 1);
+      const int wb = W - lane, wbc = wb < 0 ? 0 : wb;
+      for (int j = nB + W - 1; j >= nB; --j) {
+        const float x = sh->rhs[D - 1 - j];
+        const float Lj = sh->band[D - 1 - j + wbc][wbc];
+        const float pj = sh->rhs[D - 1 - (j - W - 1)];
+        rb = fmaf(-Lj, x, rb);
+        rb = wv_shr1(rb, pj);
+      }
+      band_back<NA, true>(sh, D, nB - 1, 0, rb);
+    }
+  }
+  __syncthreads();  // dq complete
 }
 
 // finite-difference rows of lane t (velocity row t, acceleration window starting at t) and their
@@ -399,117 +643,8 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
         sh->rhs[row] = -gf[i];
       }
     }
-    __syncthreads();
-    // ---- band Cholesky with the forward substitution L y = -g fused, in registers.  The active
-    // rows j..j+W sit on lanes WL..63 (row j+a on lane WL+a) in column-relative form
-    // C[c] = A[j+a][j+c], so column j is C[0] on every lane.  Per column: the pivot from lane WL,
-    // the multipliers L = C[0] / d, the rank-1 update C[c] -= L * (L of lane WL+c), then a
-    // one-lane DPP shift moves the window down and row j+W+1 enters at lane 63 as the shift's fill
-    // value.  L goes back to the band in LDS for the backward substitution, 1/d to its diagonal.
-    {
-      constexpr int WL = 63 - W;
-      const int wa = lane - WL;
-      float C[W + 1];
-#pragma unroll
-      for (int c = 0; c <= W; ++c) {
-        const bool ok = wa >= c && wa < D;
-        C[c] = sh->band[ok ? wa : 0][ok ? wa - c : 0];
-        C[c] = ok ? C[c] : 0.f;
-      }
-      const bool okr = wa >= 0 && wa < D;
-      float rc = sh->rhs[okr ? wa : 0];
-      rc = okr ? rc : 0.f;
-      float piv = wv_readlane(C[0], WL);
-      // one column: X is the window, Y the row entering it at this step (j + W + 1, a zero row
-      // past D, reversed to column-relative order), Z receives the row of the next step.  The
-      // shifted window is written over Y (the DPP fill operand), so three register sets rotate
-      // through the steps without copies.
-      auto step = [&](int j, float (&X)[W + 1], float (&Y)[W + 1], float (&Z)[W + 1], float& xr, float& yr,
-                      float& zr) {
-        {
-          const int rn = j + W + 2;
-#pragma unroll
-          for (int c = 0; c <= W; ++c) Z[c] = sh->band[rn][W - c];
-          zr = sh->rhs[rn];
-        }
-        const float inv = __builtin_amdgcn_rsqf(piv);  // 1 / L[j][j]
-        const float Lv = X[0] * inv;
-        const float y = wv_readlane(xr, WL) * inv;
-        float* const dl = wa >= 1 && j + wa < D ? &sh->band[j + wa][wa] : &sh->sink[lane];
-        *dl = Lv;
-        float* const di = wa == 0 ? &sh->band[j][0] : &sh->sink[lane];
-        float* const dy = wa == 0 ? &sh->rhs[j] : &sh->sink[lane];
-        *di = inv;
-        *dy = y;
-        xr = fmaf(-Lv, y, xr);
-#pragma unroll
-        for (int c = 1; c <= W; ++c) X[c] = fmaf(-Lv, wv_readlane(Lv, WL + c), X[c]);
-        piv = wv_readlane(X[1], WL + 1);  // the next pivot (lane WL+1 moves to lane WL)
-#pragma unroll
-        for (int c = 0; c < W; ++c) Y[c] = wv_shl1(X[c + 1], Y[c]);
-        // (Y[W] keeps the entering row's diagonal; lanes below 63 hold the upper triangle there)
-        yr = wv_shl1(xr, yr);
-      };
-      float P1[W + 1], P2[W + 1], r1, r2;
-#pragma unroll
-      for (int c = 0; c <= W; ++c) P1[c] = sh->band[W + 1][W - c];
-      r1 = sh->rhs[W + 1];
-      for (int j = 0; j < D; j += 3) {
-        step(j, C, P1, P2, rc, r1, r2);
-        if (j + 1 < D) step(j + 1, P1, P2, C, r1, r2, rc);
-        if (j + 2 < D) step(j + 2, P2, C, P1, r2, rc, r1);
-      }
-    }
-    __syncthreads();
-    // ---- L^T delta = y the same way: rows j-W..j on lanes 0..W (row j-W+l on lane l), the pivot
-    // on lane W; the window moves one lane up and row j-W-1 enters at lane 0.  Lane l applies
-    // L[j][j-(W-l)] = band[j][W-l]; entries left of column 0 are zero from the build, and the
-    // lanes outside the window (and the pivot lane, which leaves it) read a harmless in-range
-    // entry.  The values of the next TB columns are loaded while the current TB are solved.
-    {
-      constexpr int TB = 8;
-      const int wb = W - lane;
-      const int wbc = wb < 0 ? 0 : wb;
-      const bool okb = lane <= W && D - 1 - wb >= 0;
-      float rb = sh->rhs[okb ? D - 1 - wb : 0];
-      rb = okb ? rb : 0.f;
-      float* const dx = lane == W ? &sh->rhs[0] : &sh->sink[lane];
-      const int dxs = lane == W ? 1 : 0;
-      auto step = [&](int j, float Lj, float ij, float pj) {
-        const float x = wv_readlane(rb, W) * ij;
-        dx[dxs * j] = x;
-        rb = fmaf(-Lj, x, rb);
-        rb = wv_shr1(rb, pj);
-      };
-      float Lc[TB], ic[TB], pc[TB];
-      auto load = [&](int jb, float (&Lo)[TB], float (&io)[TB], float (&po)[TB]) {
-#pragma unroll
-        for (int u = 0; u < TB; ++u) {
-          const int j = max(jb - u, 0);
-          Lo[u] = sh->band[j][wbc];
-          io[u] = sh->band[j][0];
-          po[u] = sh->rhs[max(jb - u - W - 1, 0)];  // (the rows before 0 never become pivots)
-        }
-      };
-      int jb = D - 1;
-      load(jb, Lc, ic, pc);
-      for (; jb >= TB - 1; jb -= TB) {
-        float Ln[TB], in[TB], pn[TB];
-        load(jb - TB, Ln, in, pn);
-#pragma unroll
-        for (int u = 0; u < TB; ++u) step(jb - u, Lc[u], ic[u], pc[u]);
-#pragma unroll
-        for (int u = 0; u < TB; ++u) {
-          Lc[u] = Ln[u];
-          ic[u] = in[u];
-          pc[u] = pn[u];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < TB - 1; ++u)
-        if (jb - u >= 0) step(jb - u, Lc[u], ic[u], pc[u]);
-    }
-    __syncthreads();
+    __syncthreads();  // band built
+    band_solve<NA>(sh, T, 0);
     float dq[NA], qp[NA];
     float pr = 0.f;
 #pragma unroll
@@ -546,9 +681,9 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
 }
 
 // the second wavefront of a trajectory problem: the collision rows of the odd-numbered robot
-// spheres at the iterate and at each proposal, handed over in LDS.  Its barriers pair one for
-// one with traj_solve's: rows ready, band built, factorised, solved, proposal cost ready, iterate
-// published.
+// spheres at the iterate and at each proposal, handed over in LDS, and its half of the band
+// solve.  Its barriers pair one for one with traj_solve's: rows ready, band built, the three of
+// band_solve, proposal cost ready, iterate published.
 template <int NA, int NJ>
 __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s, int it0, int iters,
                                             TrajLDS<NA>* sh) {
@@ -596,8 +731,7 @@ __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s
     }
     __syncthreads();  // rows ready
     __syncthreads();  // band built
-    __syncthreads();  // factorised
-    __syncthreads();  // solved
+    band_solve<NA>(sh, T, 1);
     float qp[NA];
 #pragma unroll
     for (int d = 0; d < NA; ++d) qp[d] = q[d] + sh->rhs[t * NA + d];
