@@ -295,13 +295,15 @@ def arm_spheres(spec: RobotSpec, per_link: int, rng: np.random.Generator, r_lo=0
     return np.asarray(links, np.int32), np.asarray(centres, np.float32), radii.astype(np.float32)
 
 
-def config3(seed: int = 0, batch: int = 512, T: int = 64, n_obs_spheres: int = 32, n_obs_boxes: int = 8) -> TrajBatch:
-    """Collision-aware trajectory LM on the 7-DOF chain: 48 robot spheres (6 per link, radius
-    3-8 cm); 32 obstacle spheres (radius 5-15 cm) + 8 axis-aligned boxes (half extents 5-15 cm),
-    centres uniform in the 1.5 m cube [-0.75, 0.75]^2 x [0, 1.5] around the base, none within
-    0.3 m of the base's vertical axis below 0.5 m; seeds = straight joint-space lines between two
-    uniform in-limit configurations; goal = FK of a third."""
-    spec = panda_dh()
+def config3(seed: int = 0, batch: int = 512, T: int = 64, n_obs_spheres: int = 32, n_obs_boxes: int = 8,
+            arm: str = "panda") -> TrajBatch:
+    """Collision-aware trajectory LM on the 7-DOF chain (arm "ur10": the 6-DOF UR10): 6 robot
+    spheres per link (radius 3-8 cm; 48 on the 7-DOF chain); 32 obstacle spheres (radius 5-15 cm)
+    + 8 axis-aligned boxes (half extents 5-15 cm), centres uniform in the 1.5 m cube
+    [-0.75, 0.75]^2 x [0, 1.5] around the base, none within 0.3 m of the base's vertical axis
+    below 0.5 m; seeds = straight joint-space lines between two uniform in-limit configurations;
+    goal = FK of a third."""
+    spec = {"panda": panda_dh, "ur10": ur10}[arm]()
     r_sph, r_obs, r_q = _streams(seed, 3)
     links, centres, radii = arm_spheres(spec, 6, r_sph)
 

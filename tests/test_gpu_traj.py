@@ -18,9 +18,15 @@ pytestmark = pytest.mark.gpu
 P = N.LMParams(lm_lambda=1e-2, lambda_factor=10.0)
 
 
-@pytest.mark.parametrize("T,B,iters", [(64, 12, 12), (16, 24, 20)])
-def test_traj_parity(ctx, T, B, iters):
-    tb = W.config3(seed=5, batch=B, T=T)
+# T = 64 / 16: the band solve split over the two wavefronts (nested dissection, equal halves);
+# T = 7: unequal halves (2 frames before the separator, 3 after); T = 5: too short to split, one
+# wavefront solves it whole.  The UR10 case runs the 6-DOF kernel; 13 spheres + 3 boxes leave
+# the padded obstacle groups partly filled.
+@pytest.mark.parametrize("T,B,iters,arm,ns,nb", [(64, 12, 12, "panda", 32, 8), (16, 24, 20, "panda", 32, 8),
+                                                 (7, 16, 12, "panda", 32, 8), (5, 16, 12, "panda", 13, 3),
+                                                 (16, 16, 12, "ur10", 13, 3)])
+def test_traj_parity(ctx, T, B, iters, arm, ns, nb):
+    tb = W.config3(seed=5, batch=B, T=T, n_obs_spheres=ns, n_obs_boxes=nb, arm=arm)
     b = N.TrajBatch(ctx, tb, B, trace_iters=iters)
     b.set(q=tb.q_seed, goals=tb.goals)
     b.solve(iters, P)
@@ -33,7 +39,7 @@ def test_traj_parity(ctx, T, B, iters):
     s32 = O.traj_lm_step(ot, qi, lam_it.reshape(-1), p, rows=rows)
     s64 = O.traj_lm_step(ot, qi, lam_it.reshape(-1), p, dtype="f64", rows=rows)
     rep = shadow_check(gpu["cost_trace"], gpu["accept"], s32, s64)
-    print(f"\ntrajectory T={T} B={B}: {rep}")
+    print(f"\ntrajectory {arm} T={T} B={B} scene {ns}+{nb}: {rep}")
     assert rep.ok(), rep
     # the seed cost (first trace entry) against the FP64 oracle
     r64 = np.array([O.traj_residual_jacobian(ot, i, tb.q_seed[i], dtype="f64", jac=False)[4].sum() for i in range(B)])
