@@ -34,18 +34,58 @@ struct TrajTerms {
 };
 typedef const __attribute__((address_space(4))) TrajTerms TrajK;
 
-// scene signed distance at p: minimum over obstacle spheres, then boxes, the first strict minimum
-// (geom/sdf_kernels.py:256-295, trajectory_collision_task.py:62-72).  Returns the distance and
-// the winner (win >= 0: sphere win; win < 0: box -1 - win; RK_SDF_NONE: no primitive closer
-// than 1e6); scene_normal gives the gradient direction of that primitive, computed only where a
-// caller needs it.  The tables are padded with far-away primitives to whole groups (8 spheres,
-// 4 boxes); a group's data is read through the scalar cache before any of it is used, and every
-// distance is branch-free.
+// scene signed distance at p: minimum over obstacle spheres, then boxes
+// (geom/sdf_kernels.py:256-295, trajectory_collision_task.py:62-72); 1e6 when no primitive is
+// closer.  The tables are padded with far-away primitives to whole groups (8 spheres, 4 boxes);
+// a group's data is read through the scalar cache before any of it is used, and its distances
+// are reduced by a min tree (exact, so the order does not change the value).
 #define RK_SDF_NONE (1 << 30)
 #define RK_SDF_FAR 1e18f  // centre coordinate of a padding primitive (its squared distance stays finite)
-__device__ __forceinline__ float scene_sdf(TrajK* tt, const float* p, int& win) {
+__device__ __forceinline__ float sdf_sphere(const float* p, const float (&o)[4]) {
+  const float d0 = p[0] - o[0], d1 = p[1] - o[1], d2 = p[2] - o[2];
+  return __builtin_amdgcn_sqrtf(d0 * d0 + d1 * d1 + d2 * d2) - o[3];
+}
+__device__ __forceinline__ float sdf_box(const float* p, const float (&o)[6]) {
+  float qv[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) qv[k] = fabsf(p[k] - o[k]) - o[3 + k];
+  const float qmax = fmaxf(fmaxf(qv[0], qv[1]), qv[2]);
+  const float p0 = fmaxf(qv[0], 0.f), p1 = fmaxf(qv[1], 0.f), p2 = fmaxf(qv[2], 0.f);
+  return __builtin_amdgcn_sqrtf(p0 * p0 + p1 * p1 + p2 * p2) + fminf(qmax, 0.f);
+}
+__device__ __forceinline__ float scene_sdf(TrajK* tt, const float* p) {
   float best = 1e6f;
-  win = RK_SDF_NONE;
+  const int ns = tt->ns, nb = tt->nb;
+  for (int i0 = 0; i0 < ns; i0 += 8) {
+    float o[8][4], d[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) o[u][c] = tt->obs_s[i0 + u][c];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) d[u] = sdf_sphere(p, o[u]);
+    best = fminf(best, fminf(fminf(fminf(d[0], d[1]), fminf(d[2], d[3])), fminf(fminf(d[4], d[5]), fminf(d[6], d[7]))));
+  }
+  for (int i0 = 0; i0 < nb; i0 += 4) {
+    float o[4][6], d[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) o[u][c] = tt->obs_b[i0 + u][c];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) d[u] = sdf_box(p, o[u]);
+    best = fminf(best, fminf(fminf(d[0], d[1]), fminf(d[2], d[3])));
+  }
+  return best;
+}
+
+// the primitive that decides the scene distance at p: the first strict minimum in the
+// reference's order (spheres, then boxes).  win >= 0: sphere win; win < 0: box -1 - win;
+// RK_SDF_NONE: no primitive closer than 1e6.  Only needed for a gradient, so only where a sphere
+// is within the collision margin.
+__device__ __forceinline__ int scene_winner(TrajK* tt, const float* p) {
+  float best = 1e6f;
+  int win = RK_SDF_NONE;
   const int ns = tt->ns, nb = tt->nb;
   for (int i0 = 0; i0 < ns; i0 += 8) {
     float o[8][4];
@@ -55,10 +95,9 @@ __device__ __forceinline__ float scene_sdf(TrajK* tt, const float* p, int& win) 
       for (int c = 0; c < 4; ++c) o[u][c] = tt->obs_s[i0 + u][c];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const float d0 = p[0] - o[u][0], d1 = p[1] - o[u][1], d2 = p[2] - o[u][2];
-      const float dist = __builtin_amdgcn_sqrtf(d0 * d0 + d1 * d1 + d2 * d2) - o[u][3];
-      win = dist < best ? i0 + u : win;
-      best = dist < best ? dist : best;
+      const float d = sdf_sphere(p, o[u]);
+      win = d < best ? i0 + u : win;
+      best = d < best ? d : best;
     }
   }
   for (int i0 = 0; i0 < nb; i0 += 4) {
@@ -69,17 +108,12 @@ __device__ __forceinline__ float scene_sdf(TrajK* tt, const float* p, int& win) 
       for (int c = 0; c < 6; ++c) o[u][c] = tt->obs_b[i0 + u][c];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      float qv[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) qv[k] = fabsf(p[k] - o[u][k]) - o[u][3 + k];
-      const float qmax = fmaxf(fmaxf(qv[0], qv[1]), qv[2]);
-      const float p0 = fmaxf(qv[0], 0.f), p1 = fmaxf(qv[1], 0.f), p2 = fmaxf(qv[2], 0.f);
-      const float dist = __builtin_amdgcn_sqrtf(p0 * p0 + p1 * p1 + p2 * p2) + fminf(qmax, 0.f);
-      win = dist < best ? -1 - (i0 + u) : win;
-      best = dist < best ? dist : best;
+      const float d = sdf_box(p, o[u]);
+      win = d < best ? -1 - (i0 + u) : win;
+      best = d < best ? d : best;
     }
   }
-  return best;
+  return win;
 }
 
 __device__ __forceinline__ void scene_normal(TrajK* tt, const float* p, int win, float* n) {
@@ -154,14 +188,14 @@ __device__ __forceinline__ void traj_frame(RobotK* rb, TrajK* tt, const float (&
       c[0] += P.t[0];
       c[1] += P.t[1];
       c[2] += P.t[2];
-      int win;
-      const float sd = scene_sdf(tt, c, win) - tt->sph_r[k];
+      const float sdist = scene_sdf(tt, c);
+      const float sd = sdist - tt->sph_r[k];
       if (sd < margin) {
         const float res = w * (margin - sd);
         cc += res * res;
         if (JAC) {
           float n[3];
-          scene_normal(tt, c, win, n);
+          scene_normal(tt, c, scene_winner(tt, c), n);
           const float cn0 = c[1] * n[2] - c[2] * n[1], cn1 = c[2] * n[0] - c[0] * n[2], cn2 = c[0] * n[1] - c[1] * n[0];
           float jr[NA];
 #pragma unroll
@@ -288,7 +322,7 @@ __device__ __forceinline__ float wv_shr1(float src, float fill) {
 template <int NA, bool REV, bool ZERO>
 __device__ __forceinline__ void band_factor(TrajLDS<NA>* sh, int D, int j0, int n, int lim, float (&C)[2 * NA + 1],
                                             float& rc) {
-  constexpr int W = 2 * NA, WL = 63 - W;
+  constexpr int W = 2 * NA, WL = 63 - W, BS = TrajLDS<NA>::BS;
   const int lane = threadIdx.x & 63, wa = lane - WL;
   auto at = [&](int r, int c) -> float* { return REV ? &sh->band[D - 1 - c][r - c] : &sh->band[r][r - c]; };
   auto orig = [&](int i) { return REV ? D - 1 - i : i; };
@@ -303,9 +337,14 @@ __device__ __forceinline__ void band_factor(TrajLDS<NA>* sh, int D, int j0, int 
       for (int c = 0; c <= W; ++c) Z[c] = 0.f;
       zr = 0.f;
     } else {
+      // entry (j+W+2, j+2+c): one base address and constant offsets (a diagonal walk when REV);
+      // the base goes through a vector register so that the offsets fold into the LDS reads
       const int rn = j + W + 2;
+      int bo = (REV ? D - 3 - j - W : rn) * BS;
+      asm("" : "+v"(bo));
+      const float* const b = &sh->band[0][0] + bo;
 #pragma unroll
-      for (int c = 0; c <= W; ++c) Z[c] = *at(rn, j + 2 + c);
+      for (int c = 0; c <= W; ++c) Z[c] = b[REV ? (W - c) * (BS + 1) : W - c];
       zr = sh->rhs[orig(rn)];
     }
     const float inv = __builtin_amdgcn_rsqf(piv);  // 1 / L[j][j]
@@ -333,8 +372,9 @@ __device__ __forceinline__ void band_factor(TrajLDS<NA>* sh, int D, int j0, int 
     r1 = 0.f;
   } else {
     const int rn = j0 + W + 1;
+    const float* const b = REV ? &sh->band[D - 2 - j0 - W][0] : &sh->band[rn][0];
 #pragma unroll
-    for (int c = 0; c <= W; ++c) P1[c] = *at(rn, j0 + 1 + c);
+    for (int c = 0; c <= W; ++c) P1[c] = b[REV ? (W - c) * (BS + 1) : W - c];
     r1 = sh->rhs[orig(rn)];
   }
   const int je = j0 + n;
@@ -383,7 +423,7 @@ __device__ __forceinline__ void band_window0(TrajLDS<NA>* sh, int D, float (&C)[
 // next TB pivots are loaded while the current TB are solved.
 template <int NA, bool REV>
 __device__ __forceinline__ void band_back(TrajLDS<NA>* sh, int D, int jtop, int jstop, float& rb) {
-  constexpr int W = 2 * NA, TB = 8;
+  constexpr int W = 2 * NA, TB = 8, BS = TrajLDS<NA>::BS;
   const int lane = threadIdx.x & 63;
   const int wb = W - lane;
   const int wbc = wb < 0 ? 0 : wb;
@@ -397,11 +437,12 @@ __device__ __forceinline__ void band_back(TrajLDS<NA>* sh, int D, int jtop, int 
     rb = wv_shr1(rb, pj);
   };
   float Lc[TB], ic[TB], pc[TB];
+  const float* const lb = &sh->band[0][0] + (REV ? wbc * (BS + 1) : wbc);  // this lane's L entry of row 0
   auto load = [&](int jb, float (&Lo)[TB], float (&io)[TB], float (&po)[TB]) {
 #pragma unroll
     for (int u = 0; u < TB; ++u) {
       const int j = max(jb - u, jstop);
-      Lo[u] = REV ? sh->band[D - 1 - j + wbc][wbc] : sh->band[j][wbc];
+      Lo[u] = lb[orig(j) * BS];
       io[u] = sh->band[orig(j)][0];
       const int e = jb - u - W - 1;  // (the rows before 0 never become pivots)
       po[u] = sh->rhs[REV ? D - 1 - max(e, -W - 2) : max(e, 0)];
@@ -502,12 +543,18 @@ __device__ __forceinline__ void band_solve(TrajLDS<NA>* sh, int T, int wv) {
       // the separator pivots (reversed nB+W-1 .. nB) with x known: their L^T rows update B only
       float rb = band_back_window<NA, true>(sh, D, nB + W - 1);
       const int wb = W - lane, wbc = wb < 0 ? 0 : wb;
-      for (int j = nB + W - 1; j >= nB; --j) {
-        const float x = sh->rhs[D - 1 - j];
-        const float Lj = sh->band[D - 1 - j + wbc][wbc];
-        const float pj = sh->rhs[D - 1 - (j - W - 1)];
-        rb = fmaf(-Lj, x, rb);
-        rb = wv_shr1(rb, pj);
+      float xs[W], Ls[W], ps[W];
+#pragma unroll
+      for (int u = 0; u < W; ++u) {
+        const int j = nB + W - 1 - u;
+        xs[u] = sh->rhs[D - 1 - j];
+        Ls[u] = sh->band[D - 1 - j + wbc][wbc];
+        ps[u] = sh->rhs[D - 1 - (j - W - 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < W; ++u) {
+        rb = fmaf(-Ls[u], xs[u], rb);
+        rb = wv_shr1(rb, ps[u]);
       }
       band_back<NA, true>(sh, D, nB - 1, 0, rb);
     }
@@ -776,8 +823,7 @@ __global__ __launch_bounds__(64) void k_traj_metrics(Problems P, const TrajTerms
       c[0] += Pw.t[0];
       c[1] += Pw.t[1];
       c[2] += Pw.t[2];
-      int win;
-      msd = fminf(msd, scene_sdf(tt, c, win) - tt->sph_r[k]);
+      msd = fminf(msd, scene_sdf(tt, c) - tt->sph_r[k]);
     }
   };
   spheres(-1, Tt);
