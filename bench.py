@@ -303,6 +303,13 @@ def main() -> int:
     hbm_bytes = sum(mb.buckets[i].size * (4 * mb.buckets[i].spec.num_actuated_joints * 2 + 28 * len(mb.buckets[i].links)
                                           + 12) for i in dom_batches)
 
+    def kernel_roofline(k):
+        tpp_k, _, batches_k = per_kernel[0][k]
+        ach = sum(bucket_flops(mb.buckets[i]) for i in batches_k) / (k_ms[k] * 1e-3) / 1e12
+        variant = variant_name(tpp_k, mb.buckets[batches_k[0]].spec, 0)
+        return {"variant": variant, "batches": list(batches_k), "ms": round(k_ms[k], 4), "achieved": round(ach, 4),
+                "frac": round(ach / FP32_PEAK_TFLOPS, 5), "traffic": profiled_traffic(variant)}
+
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -348,6 +355,9 @@ def main() -> int:
             "all_kernels_ms": [round(x, 4) for x in k_ms],
             "whole_solve_ms": round(float(np.mean(kern_ms)), 4),
             "whole_solve_achieved": round(flops_all / (float(np.mean(kern_ms)) * 1e-3) / 1e12, 4),
+            # every persistent kernel of the group (they run at once): F_LM of its buckets over its
+            # own HIP-event time, and its measured HBM bytes per launch
+            "kernels": [kernel_roofline(k) for k in range(n_k)],
         },
     }
 

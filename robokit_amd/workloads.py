@@ -296,7 +296,7 @@ def arm_spheres(spec: RobotSpec, per_link: int, rng: np.random.Generator, r_lo=0
 
 
 def config3(seed: int = 0, batch: int = 512, T: int = 64, n_obs_spheres: int = 32, n_obs_boxes: int = 8,
-            arm: str = "panda") -> TrajBatch:
+            arm: str = "panda", spheres_per_link: int = 6) -> TrajBatch:
     """Collision-aware trajectory LM on the 7-DOF chain (arm "ur10": the 6-DOF UR10): 6 robot
     spheres per link (radius 3-8 cm; 48 on the 7-DOF chain); 32 obstacle spheres (radius 5-15 cm)
     + 8 axis-aligned boxes (half extents 5-15 cm), centres uniform in the 1.5 m cube
@@ -305,7 +305,7 @@ def config3(seed: int = 0, batch: int = 512, T: int = 64, n_obs_spheres: int = 3
     goal = FK of a third."""
     spec = {"panda": panda_dh, "ur10": ur10}[arm]()
     r_sph, r_obs, r_q = _streams(seed, 3)
-    links, centres, radii = arm_spheres(spec, 6, r_sph)
+    links, centres, radii = arm_spheres(spec, spheres_per_link, r_sph)
 
     def centres_ok(n):
         out = []

@@ -20,13 +20,14 @@ P = N.LMParams(lm_lambda=1e-2, lambda_factor=10.0)
 
 # T = 64 / 16: the band solve split over the two wavefronts (nested dissection, equal halves);
 # T = 7: unequal halves (2 frames before the separator, 3 after); T = 5: too short to split, one
-# wavefront solves it whole.  The UR10 case runs the 6-DOF kernel; 13 spheres + 3 boxes leave
-# the padded obstacle groups partly filled.
-@pytest.mark.parametrize("T,B,iters,arm,ns,nb", [(64, 12, 12, "panda", 32, 8), (16, 24, 20, "panda", 32, 8),
-                                                 (7, 16, 12, "panda", 32, 8), (5, 16, 12, "panda", 13, 3),
-                                                 (16, 16, 12, "ur10", 13, 3)])
-def test_traj_parity(ctx, T, B, iters, arm, ns, nb):
-    tb = W.config3(seed=5, batch=B, T=T, n_obs_spheres=ns, n_obs_boxes=nb, arm=arm)
+# wavefront solves it whole.  The UR10 cases run the 6-DOF kernel; 13 spheres + 3 boxes leave
+# the padded obstacle groups partly filled; the last case fills every table to its maximum
+# (64 robot spheres, 64 obstacle spheres, 16 boxes, 64 frames).
+@pytest.mark.parametrize("T,B,iters,arm,ns,nb,spl", [(64, 12, 12, "panda", 32, 8, 6), (16, 24, 20, "panda", 32, 8, 6),
+                                                     (7, 16, 12, "panda", 32, 8, 6), (5, 16, 12, "panda", 13, 3, 6),
+                                                     (16, 16, 12, "ur10", 13, 3, 6), (64, 6, 8, "ur10", 64, 16, 8)])
+def test_traj_parity(ctx, T, B, iters, arm, ns, nb, spl):
+    tb = W.config3(seed=5, batch=B, T=T, n_obs_spheres=ns, n_obs_boxes=nb, arm=arm, spheres_per_link=spl)
     b = N.TrajBatch(ctx, tb, B, trace_iters=iters)
     b.set(q=tb.q_seed, goals=tb.goals)
     b.solve(iters, P)
