@@ -53,8 +53,11 @@ __device__ __forceinline__ float sdf_box(const float* p, const float (&o)[6]) {
   const float p0 = fmaxf(qv[0], 0.f), p1 = fmaxf(qv[1], 0.f), p2 = fmaxf(qv[2], 0.f);
   return __builtin_amdgcn_sqrtf(p0 * p0 + p1 * p1 + p2 * p2) + fminf(qmax, 0.f);
 }
-__device__ __forceinline__ float scene_sdf(TrajK* tt, const float* p) {
+// also returns in grp the first group holding the minimum (sphere groups 0..7, box groups 8..11;
+// -1: none closer than 1e6), for scene_winner
+__device__ __forceinline__ float scene_sdf(TrajK* tt, const float* p, int& grp) {
   float best = 1e6f;
+  grp = -1;
   const int ns = tt->ns, nb = tt->nb;
   for (int i0 = 0; i0 < ns; i0 += 8) {
     float o[8][4], d[8];
@@ -64,7 +67,9 @@ __device__ __forceinline__ float scene_sdf(TrajK* tt, const float* p) {
       for (int c = 0; c < 4; ++c) o[u][c] = tt->obs_s[i0 + u][c];
 #pragma unroll
     for (int u = 0; u < 8; ++u) d[u] = sdf_sphere(p, o[u]);
-    best = fminf(best, fminf(fminf(fminf(d[0], d[1]), fminf(d[2], d[3])), fminf(fminf(d[4], d[5]), fminf(d[6], d[7]))));
+    const float gm = fminf(fminf(fminf(d[0], d[1]), fminf(d[2], d[3])), fminf(fminf(d[4], d[5]), fminf(d[6], d[7])));
+    grp = gm < best ? i0 / 8 : grp;
+    best = fminf(best, gm);
   }
   for (int i0 = 0; i0 < nb; i0 += 4) {
     float o[4][6], d[4];
@@ -74,42 +79,41 @@ __device__ __forceinline__ float scene_sdf(TrajK* tt, const float* p) {
       for (int c = 0; c < 6; ++c) o[u][c] = tt->obs_b[i0 + u][c];
 #pragma unroll
     for (int u = 0; u < 4; ++u) d[u] = sdf_box(p, o[u]);
-    best = fminf(best, fminf(fminf(d[0], d[1]), fminf(d[2], d[3])));
+    const float gm = fminf(fminf(d[0], d[1]), fminf(d[2], d[3]));
+    grp = gm < best ? 8 + i0 / 4 : grp;
+    best = fminf(best, gm);
   }
   return best;
 }
 
 // the primitive that decides the scene distance at p: the first strict minimum in the
-// reference's order (spheres, then boxes).  win >= 0: sphere win; win < 0: box -1 - win;
-// RK_SDF_NONE: no primitive closer than 1e6.  Only needed for a gradient, so only where a sphere
-// is within the collision margin.
-__device__ __forceinline__ int scene_winner(TrajK* tt, const float* p) {
+// reference's order (spheres, then boxes), found inside grp, the first group holding the minimum
+// (the groups' minima are compared strictly in order, so the first group wins, and inside it the
+// strict scan keeps the first).  win >= 0: sphere win; win < 0: box -1 - win; RK_SDF_NONE: no
+// primitive closer than 1e6.  Only needed for a gradient, so only where a sphere is within the
+// collision margin; grp differs between lanes, so the group is read with per-lane addresses.
+__device__ __forceinline__ int scene_winner(TrajK* tt, const float* p, int grp) {
+  if (grp < 0) return RK_SDF_NONE;
   float best = 1e6f;
   int win = RK_SDF_NONE;
-  const int ns = tt->ns, nb = tt->nb;
-  for (int i0 = 0; i0 < ns; i0 += 8) {
-    float o[8][4];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) o[u][c] = tt->obs_s[i0 + u][c];
+  if (grp < 8) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const float d = sdf_sphere(p, o[u]);
-      win = d < best ? i0 + u : win;
+      float o[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) o[c] = tt->obs_s[8 * grp + u][c];
+      const float d = sdf_sphere(p, o);
+      win = d < best ? 8 * grp + u : win;
       best = d < best ? d : best;
     }
-  }
-  for (int i0 = 0; i0 < nb; i0 += 4) {
-    float o[4][6];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int c = 0; c < 6; ++c) o[u][c] = tt->obs_b[i0 + u][c];
+  } else {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const float d = sdf_box(p, o[u]);
-      win = d < best ? -1 - (i0 + u) : win;
+      float o[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) o[c] = tt->obs_b[4 * (grp - 8) + u][c];
+      const float d = sdf_box(p, o);
+      win = d < best ? -1 - (4 * (grp - 8) + u) : win;
       best = d < best ? d : best;
     }
   }
@@ -188,14 +192,14 @@ __device__ __forceinline__ void traj_frame(RobotK* rb, TrajK* tt, const float (&
       c[0] += P.t[0];
       c[1] += P.t[1];
       c[2] += P.t[2];
-      const float sdist = scene_sdf(tt, c);
-      const float sd = sdist - tt->sph_r[k];
+      int grp;
+      const float sd = scene_sdf(tt, c, grp) - tt->sph_r[k];
       if (sd < margin) {
         const float res = w * (margin - sd);
         cc += res * res;
         if (JAC) {
           float n[3];
-          scene_normal(tt, c, scene_winner(tt, c), n);
+          scene_normal(tt, c, scene_winner(tt, c, grp), n);
           const float cn0 = c[1] * n[2] - c[2] * n[1], cn1 = c[2] * n[0] - c[0] * n[2], cn2 = c[0] * n[1] - c[1] * n[0];
           float jr[NA];
 #pragma unroll
@@ -823,7 +827,8 @@ __global__ __launch_bounds__(64) void k_traj_metrics(Problems P, const TrajTerms
       c[0] += Pw.t[0];
       c[1] += Pw.t[1];
       c[2] += Pw.t[2];
-      msd = fminf(msd, scene_sdf(tt, c) - tt->sph_r[k]);
+      int grp;
+      msd = fminf(msd, scene_sdf(tt, c, grp) - tt->sph_r[k]);
     }
   };
   spheres(-1, Tt);
