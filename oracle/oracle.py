@@ -1,7 +1,7 @@
 """ctypes wrapper over the CPU oracle (liboracle_f32.so / liboracle_f64.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg, as the checker.  The product package (robokit_amd) never imports it.
+cpu_baseline leg, to check results.  The product package (robokit_amd) never imports it.
 
 The C source (rk_oracle.c) restates the reference's Warp-CPU LM path; see its header for
 the file:line each function follows.

@@ -1,5 +1,5 @@
 /*
- * rk_oracle.c -- TEST INFRASTRUCTURE ONLY (the CPU oracle / checker).
+ * rk_oracle.c -- TEST INFRASTRUCTURE ONLY (the CPU oracle that results are checked against).
  *
  * A plain-C restatement of RoboKit's Warp-CPU Levenberg-Marquardt path, used by
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg to check and

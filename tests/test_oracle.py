@@ -1,4 +1,4 @@
-"""CPU checks that pin the oracle (oracle/rk_oracle.c) before it is trusted as the checker.
+"""CPU checks that pin the oracle (oracle/rk_oracle.c) before the GPU results are checked against it.
 
 Every oracle function is checked against data the reference's own tests hold, or against
 what the mathematics fixes independently of the formula the oracle restates:
