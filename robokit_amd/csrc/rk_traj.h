@@ -7,11 +7,11 @@
 // the NA x NA diagonal block of J^T J and the frame's part of J^T r, all in registers.  The two
 // wavefronts of a problem split the robot spheres; the first adds the second's rows from LDS and
 // solves.  J^T J is banded (half-bandwidth W = 2*NA: acceleration rows couple frames t and t+2):
-// the band is built in LDS, [D][W+1], and factorised by a right-looking band Cholesky whose W+1
-// active rows live in registers of the top W+1 lanes, shifted one lane per column; the forward
-// substitution is fused into it and the backward one runs the same way.  Rows and term costs in
-// this order: goal, velocity, acceleration, collision (the reference sums term costs in list
-// order).
+// the band is built in LDS ([D][W+1], rows padded to 16 floats) and factorised by a
+// right-looking band Cholesky whose W+1 active rows live in registers of the top W+1 lanes,
+// shifted one lane per column; the forward substitution is fused into it and the backward one
+// runs the same way.  Rows and term costs in this order: goal, velocity, acceleration,
+// collision (the reference sums term costs in list order).
 
 #define RK_TRAJ_MAXT 64
 #define RK_TRAJ_MAXS 64  // robot collision spheres
