@@ -206,6 +206,10 @@ def main() -> int:
     ap.add_argument("--serve-plain-seconds", type=float, default=0.5,
                     help="arrivals replayed through the plain per-request launch")
     ap.add_argument("--serve-budget-gb", type=float, default=16.0)
+    ap.add_argument("--serve-capacity-rate", type=float, default=16000.0,
+                    help="arrival rate of the IK-only capacity trace (requests/s)")
+    ap.add_argument("--serve-capacity-seconds", type=float, default=2.0,
+                    help="length of the capacity trace; 0 skips it")
     ap.add_argument("--no-traj", dest="traj", action="store_false", help="skip the trajectory leg")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a streaming copy of this many MiB (HBM counter calibration) and exit")
@@ -406,6 +410,22 @@ def main() -> int:
             "plain_per_request": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in plain_req.items()},
             "speedup_vs_plain_per_request": round(served_value / max(plain_req["value"], 1e-9), 2),
         }
+        # capacity: IK requests only, offered faster than the server completes them
+        if args.serve_capacity_seconds > 0:
+            cap_trace = W.config4(seed=args.seed + 1, rate=args.serve_capacity_rate,
+                                  seconds=args.serve_capacity_seconds, kinds=("ik6", "ik7", "ik29"))
+            st2, conv2 = run_served(N, ctx, cap_trace, params, ITERS, int(args.serve_budget_gb * (1 << 30)), streams=8)
+            out["served"]["capacity"] = {
+                "workload": f"Poisson {args.serve_capacity_rate:g} IK requests/s for {args.serve_capacity_seconds:g} s "
+                            f"(kinds {cap_trace.kinds}, 1-256 problems per request)",
+                "requests": st2["completed"], "failed": st2["failed"], "problems": st2["problems"],
+                "value": round(conv2["ik"] / st2["span_s"], 1), "unit": "converged IK solves/s",
+                "problems_per_s": round(st2["problems"] / st2["span_s"], 1),
+                "requests_per_s": round(st2["completed"] / st2["span_s"], 1),
+                "p50_ms": round(st2["p50_ms"], 4), "p99_ms": round(st2["p99_ms"], 4),
+                "mean_problems_per_launch": round(st2["mean_problems_per_launch"], 2), "span_s": round(st2["span_s"], 3),
+            }
+            print(f"[bench] served capacity: {out['served']['capacity']}", file=sys.stderr, flush=True)
 
     # ---- CPU baseline: the oracle (a C restatement of the reference's Warp-CPU path), 1 thread
     if not args.no_cpu:
