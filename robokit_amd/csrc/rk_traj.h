@@ -298,6 +298,7 @@ struct TrajLDS {
   float part[NPART][64];
   float pcc[64];
   float qx[NA][64];
+  int moved;  // the last step was accepted (the iterate in qx is new)
   // the second wavefront's Schur update of the separator rows (upper triangle) and right-hand side
   float sx[W][W];
   float rx[W];
@@ -645,6 +646,11 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
   if (lane <= W + 1) sh->rhs[D + lane] = 0.f;
   float cost = 0.f;
   if (iters == 0) cost = eval_cost(q);
+  // the frame's block of J^T J, J^T r and the cost at the iterate: a rejected step leaves the
+  // iterate, hence these, unchanged, so they are rebuilt only after an accepted step (both
+  // wavefronts skip the rows together: the decision is published with the iterate)
+  float hb[NA][NA], gf[NA];
+  bool moved = true;
   for (int it = 0; it < iters; ++it) {
     const int gi = it0 + it;
     if (Pg.q_trace && gi < Pg.trace_iters) {
@@ -654,29 +660,30 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
       if (lane == 0) Pg.lam_trace[gi * S + s] = lam;
     }
     // ---- rows, block of J^T J and J^T r of this frame
-    float hb[NA][NA], gf[NA];
+    if (moved) {
 #pragma unroll
-    for (int a = 0; a < NA; ++a) {
-      gf[a] = 0.f;
+      for (int a = 0; a < NA; ++a) {
+        gf[a] = 0.f;
 #pragma unroll
-      for (int b2 = 0; b2 < NA; ++b2) hb[a][b2] = 0.f;
+        for (int b2 = 0; b2 < NA; ++b2) hb[a][b2] = 0.f;
+      }
+      float cg = 0.f, cc = 0.f, cv = 0.f, ca = 0.f;
+      traj_frame<NA, NJ, true, 0>(rb, tt, q, last, goal, hb, gf, cg, cc);
+      traj_smooth<NA, true>(tt, t, T, q, gf, cv, ca);
+      __syncthreads();  // the second wavefront's rows (odd spheres)
+      {
+        int e = 0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int b2 = 0; b2 <= a; ++b2) hb[a][b2] += sh->part[e++][lane];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) gf[a] += sh->part[e++][lane];
+        cc += sh->part[e][lane];
+      }
+      if (!own) cg = cc = cv = ca = 0.f;
+      cost = ((0.5f * wv_sum(cg) + 0.5f * wv_sum(cv)) + 0.5f * wv_sum(ca)) + 0.5f * wv_sum(cc);
     }
-    float cg = 0.f, cc = 0.f, cv = 0.f, ca = 0.f;
-    traj_frame<NA, NJ, true, 0>(rb, tt, q, last, goal, hb, gf, cg, cc);
-    traj_smooth<NA, true>(tt, t, T, q, gf, cv, ca);
-    __syncthreads();  // the second wavefront's rows (odd spheres)
-    {
-      int e = 0;
-#pragma unroll
-      for (int a = 0; a < NA; ++a)
-#pragma unroll
-        for (int b2 = 0; b2 <= a; ++b2) hb[a][b2] += sh->part[e++][lane];
-#pragma unroll
-      for (int a = 0; a < NA; ++a) gf[a] += sh->part[e++][lane];
-      cc += sh->part[e][lane];
-    }
-    if (!own) cg = cc = cv = ca = 0.f;
-    cost = ((0.5f * wv_sum(cg) + 0.5f * wv_sum(cv)) + 0.5f * wv_sum(ca)) + 0.5f * wv_sum(cc);
     // ---- band of J^T J + lam I (rows of frame t), right-hand side J^T r
     if (own) {
 #pragma unroll
@@ -717,8 +724,11 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
       Pg.trace[(gi + 1) * S + s] = cost;
       Pg.acc_trace[gi * S + s] = acc;
     }
+    moved = acc;
+    if (acc)
 #pragma unroll
-    for (int d = 0; d < NA; ++d) sh->qx[d][lane] = q[d];
+      for (int d = 0; d < NA; ++d) sh->qx[d][lane] = q[d];
+    if (lane == 0) sh->moved = acc ? 1 : 0;
     __syncthreads();
   }
   if (own)
@@ -732,9 +742,10 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
 }
 
 // the second wavefront of a trajectory problem: the collision rows of the odd-numbered robot
-// spheres at the iterate and at each proposal, handed over in LDS, and its half of the band
-// solve.  Its barriers pair one for one with traj_solve's: rows ready, band built, the three of
-// band_solve, proposal cost ready, iterate published.
+// spheres at the iterate (after an accepted step) and at each proposal, handed over in LDS, and
+// its half of the band solve.  Its barriers pair one for one with traj_solve's: rows ready (after
+// an accepted step only), band built, the three of band_solve, proposal cost ready, iterate
+// published.
 template <int NA, int NJ>
 __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s, int it0, int iters,
                                             TrajLDS<NA>* sh) {
@@ -760,8 +771,9 @@ __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s
     proposal_cost(q);
     return;
   }
+  bool moved = true;
   for (int it = 0; it < iters; ++it) {
-    {
+    if (moved) {
       float hb[NA][NA], gf[NA];
 #pragma unroll
       for (int a = 0; a < NA; ++a) {
@@ -779,8 +791,8 @@ __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s
 #pragma unroll
       for (int a = 0; a < NA; ++a) sh->part[e++][lane] = gf[a];
       sh->part[e][lane] = cc;
+      __syncthreads();  // rows ready
     }
-    __syncthreads();  // rows ready
     __syncthreads();  // band built
     band_solve<NA>(sh, T, 1);
     float qp[NA];
@@ -788,8 +800,10 @@ __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s
     for (int d = 0; d < NA; ++d) qp[d] = q[d] + sh->rhs[t * NA + d];
     proposal_cost(qp);  // (+ barrier: proposal cost ready)
     __syncthreads();  // iterate published
+    moved = sh->moved != 0;
+    if (moved)
 #pragma unroll
-    for (int d = 0; d < NA; ++d) q[d] = sh->qx[d][lane];
+      for (int d = 0; d < NA; ++d) q[d] = sh->qx[d][lane];
   }
 }
 
