@@ -188,8 +188,8 @@ int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t fl
 int rk_group_kernel_times(rk_group* g, int32_t cap, int32_t* tpp, float* ms, int32_t* batch_mask, int32_t* n);
 
 /*
- * Collision-aware trajectory LM (512 trajectories x 64 frames in the benchmark): T frames of a serial arm per problem,
- * residuals in this row order:
+ * Collision-aware trajectory LM (512 trajectories x 64 frames in the benchmark): T frames of a
+ * serial arm per problem, residuals in this row order:
  *   goal       TrajectoryTask(FrameTask log) on the last frame (terms/trajectory_task.py:65-295)
  *   velocity   TrajectorySmoothnessTask order 1, weight / dt (trajectory_smoothness_task.py:506-557)
  *   accel.     TrajectorySmoothnessTask order 2, weight / dt^2 (first window: no history, weight 0)
