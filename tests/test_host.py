@@ -152,3 +152,36 @@ def test_config2_clip():
     assert clip.targets.shape == (64, 12, 7) and clip.q_traj.shape == (64, 29)
     # five sinusoids of 0.5 rad at <= 2 Hz: |dq/dt| <= 5 * 0.5 * 2 pi * 2 rad/s, at 30 fps per frame
     assert np.abs(np.diff(clip.q_traj, axis=0)).max() <= 5 * 0.5 * 2 * np.pi * 2 / 30
+
+
+# ctypes structures of the binding and the C structs of include/rkhip.h they mirror
+_STRUCTS = [(N.RobotDesc, "rk_robot_desc"), (N.TermsDesc, "rk_terms_desc"), (N.LMParamsC, "rk_lm_params"),
+            (N.TrajDescC, "rk_traj_desc"), (N.RequestC, "rk_request"), (N.ServeStatsC, "rk_serve_stats")]
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """every field of every ctypes structure sits at the offset the C compiler gives it, and the
+    sizes agree (gcc on a probe that includes the header)"""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cls, cname in _STRUCTS:
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True, capture_output=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, field, value = line.split()
+        got[(cname, field)] = int(value)
+    for cls, cname in _STRUCTS:
+        assert C.sizeof(cls) == got[(cname, "sizeof")], cname
+        for fname, _ in cls._fields_:
+            assert getattr(cls, fname).offset == got[(cname, fname)], (cname, fname)
