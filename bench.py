@@ -71,6 +71,23 @@ def profiled_traffic(variant: str):
     return None
 
 
+FLOPS_PROFILE = os.path.join(ROOT, "profiles", "valu_fp32_flops_counters.md")
+
+
+def profiled_flops(variant: str):
+    """FP32 TFLOP/s of k_persistent<variant> counted by the hardware (VALU, and with the matrix
+    cores) from the committed rocprofv3 summary; None if absent"""
+    try:
+        with open(FLOPS_PROFILE) as f:
+            for line in f:
+                if line.startswith(f"| k_persistent<{variant}>"):
+                    cells = line.strip().strip("|").split("|")
+                    return {"valu": float(cells[-2]), "with_matrix_cores": float(cells[-1])}
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 RT_POS_W, RT_SMOOTH_W = 1.0, 0.1  # retargeting frame: PositionTask weight, SmoothnessTask weight
 
 
@@ -312,7 +329,8 @@ def main() -> int:
         ach = sum(bucket_flops(mb.buckets[i]) for i in batches_k) / (k_ms[k] * 1e-3) / 1e12
         variant = variant_name(tpp_k, mb.buckets[batches_k[0]].spec, 0)
         return {"variant": variant, "batches": list(batches_k), "ms": round(k_ms[k], 4), "achieved": round(ach, 4),
-                "frac": round(ach / FP32_PEAK_TFLOPS, 5), "traffic": profiled_traffic(variant)}
+                "frac": round(ach / FP32_PEAK_TFLOPS, 5), "traffic": profiled_traffic(variant),
+                "fp32_tflops_counted": profiled_flops(variant)}
 
     out = {
         "metric": METRIC,
