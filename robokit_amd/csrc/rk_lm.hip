@@ -2528,7 +2528,14 @@ static int group_launch(rk_group* g, const GroupLaunch& gl, int iters, const LMP
   rk_context* ctx = g->ctx;
   const int chunk = g->iters_per_launch > 0 ? g->iters_per_launch : std::max(1, iters);
   HIP_TRY(hipEventRecord(g->ev_fork, ctx->stream));
-  for (size_t k = 0; k < gl.var.size(); ++k) {
+  // the variant with the most tiles is enqueued first: its workgroups reach the CUs first, and the
+  // few wavefronts of the others fill in around them (about 1 % faster on the uneven benchmark
+  // than the thread-per-problem kernels first)
+  size_t order[NVARIANT];
+  for (size_t k = 0; k < gl.var.size(); ++k) order[k] = k;
+  std::stable_sort(order, order + gl.var.size(), [&](size_t a, size_t b) { return gl.tiles[a] > gl.tiles[b]; });
+  for (size_t kk = 0; kk < gl.var.size(); ++kk) {
+    const size_t k = order[kk];
     hipStream_t st = gl.var.size() == 1 ? ctx->stream : g->aux[k];
     if (st != ctx->stream) HIP_TRY(hipStreamWaitEvent(st, g->ev_fork, 0));
     if (timed) HIP_TRY(hipEventRecord(g->ev_k0[k], st));
