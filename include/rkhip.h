@@ -40,6 +40,11 @@ int rk_context_destroy(rk_context* ctx);
 int rk_context_sync(rk_context* ctx);
 /* bytes of device memory currently allocated by the context (batches, groups, servers, robots) */
 int rk_context_mem(rk_context* ctx, int64_t* bytes);
+/* timing marks on the context stream (no counterpart in the reference; for benchmarks):
+ * rk_context_mark records HIP event `slot` (0..7) on the stream; rk_context_elapsed waits for
+ * mark b and gives the milliseconds from mark a to mark b */
+int rk_context_mark(rk_context* ctx, int32_t slot);
+int rk_context_elapsed(rk_context* ctx, int32_t a, int32_t b, float* ms);
 
 /*
  * Robot topology (robo/robot_spec.py:45-138 fields).  Copied to a resident device model.
@@ -179,7 +184,10 @@ int rk_group_set_tuning(rk_group* g, const int32_t* tpp, int32_t wg_per_cu, int3
 /* time every legal threads-per-problem for each batch on a short run and keep the fastest;
  * chosen_tpp [nbatches] (nullable) receives the choice */
 int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t* chosen_tpp);
-/* solve all batches; flags: RK_USE_GRAPH (capture once per shape, then replay), RK_FROM_SEED, RK_ASYNC */
+/* solve all batches; flags: RK_USE_GRAPH (capture once per shape, then replay), RK_FROM_SEED, RK_ASYNC.
+ * A graph or async solve of register-resident variants (threads-per-problem 1 up to 7 dofs,
+ * RK_TPP_WAVE, RK_TPP_HALF) runs as one launch; otherwise one kernel per variant on forked
+ * streams.  Results are the same bits either way. */
 int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t flags, float* kernel_ms);
 /* per persistent kernel of the last eager (not graph-replayed) solve -- one per kernel variant,
  * launched concurrently on forked streams: its threads-per-problem code, HIP-event time on its

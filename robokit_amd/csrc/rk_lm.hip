@@ -810,13 +810,10 @@ __device__ __forceinline__ void run_tile(const Bucket* bk, int local, const LMPa
 // register budget of a variant's persistent kernel: waves per SIMD the allocation must allow
 __host__ __device__ constexpr int variant_waves_per_eu(int v) { return variant_wave(v) || variant_half(v) ? 2 : 1; }
 
+// the tile loop of one workgroup of variant V (nblocks workgroups serve the variant's queue)
 template <int V>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(variant_waves_per_eu(V)))) void k_persistent(const Bucket* __restrict__ buckets, int nb, int total_tiles,
-                                                   int* counter, LMParams lp, int it0, int iters) {
-  // static LDS of the variant's size: its offsets fold into the LDS instructions (a dynamic
-  // base made every address a live scalar)
-  __shared__ __align__(16) char lds[variant_lds(V) > 0 ? variant_lds(V) : 16];
-  __shared__ int tile_s;
+__device__ __forceinline__ void persist_loop(const Bucket* buckets, int nb, int total_tiles, int* counter, int nblocks,
+                                             const LMParams& lp, int it0, int iters, char* lds, int& tile_s) {
   for (;;) {
     if (threadIdx.x == 0) tile_s = atomicAdd(counter, 1);
     __syncthreads();
@@ -827,9 +824,59 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(variant_wave
     while (b + 1 < nb && tile >= buckets[b + 1].tile_base) ++b;
     run_tile<V>(buckets + b, tile - buckets[b].tile_base, lp, it0, iters, lds);
   }
-  if (threadIdx.x == 0 && atomicAdd(counter + 1, 1) == (int)gridDim.x - 1) {
+  if (threadIdx.x == 0 && atomicAdd(counter + 1, 1) == nblocks - 1) {
     atomicExch(counter, 0);
     atomicExch(counter + 1, 0);
+  }
+}
+
+template <int V>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(variant_waves_per_eu(V)))) void k_persistent(const Bucket* __restrict__ buckets, int nb, int total_tiles,
+                                                   int* counter, LMParams lp, int it0, int iters) {
+  // static LDS of the variant's size: its offsets fold into the LDS instructions (a dynamic
+  // base made every address a live scalar)
+  __shared__ __align__(16) char lds[variant_lds(V) > 0 ? variant_lds(V) : 16];
+  __shared__ int tile_s;
+  persist_loop<V>(buckets, nb, total_tiles, counter, (int)gridDim.x, lp, it0, iters, lds, tile_s);
+}
+
+// The register-resident variants (thread-per-problem up to 7 dofs, wave and half-wave) fit one
+// kernel's budget (<= 256 registers, <= 15.2 KB LDS, 2 wavefronts per SIMD): a group made of
+// them runs as ONE launch, consecutive block ranges serving the variants' queues, instead of a
+// kernel per variant on forked streams (the fork, the join and the streams' start skew cost
+// ~50 us per solve on the uneven benchmark).
+__host__ __device__ constexpr bool variant_mixable(int v) {
+  return (variant_thread(v) && v != V_T8) || variant_wave(v) || variant_half(v);
+}
+constexpr size_t mix_lds() {
+  size_t m = 16;
+  for (int v = 0; v < NVARIANT; ++v)
+    if (variant_mixable(v) && variant_lds(v) > m) m = variant_lds(v);
+  return m;
+}
+struct MixArgs {
+  int nslots;
+  int variant[NVARIANT], block0[NVARIANT], nblocks[NVARIANT], first[NVARIANT], nb[NVARIANT], tiles[NVARIANT],
+      ctr[NVARIANT];
+};
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_persistent_reg(const Bucket* __restrict__ buckets, MixArgs ma, int* counter,
+                                                                                        LMParams lp, int it0, int iters) {
+  __shared__ __align__(16) char lds[mix_lds()];
+  __shared__ int tile_s;
+  int k = 0;
+  while (k + 1 < ma.nslots && (int)blockIdx.x >= ma.block0[k + 1]) ++k;
+  const Bucket* bk = buckets + ma.first[k];
+  int* const ctr = counter + 2 * ma.ctr[k];
+  const int nb = ma.nb[k], tiles = ma.tiles[k], nbl = ma.nblocks[k];
+  switch (ma.variant[k]) {
+    case V_T5: persist_loop<V_T5>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
+    case V_T6: persist_loop<V_T6>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
+    case V_T7: persist_loop<V_T7>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
+    case V_W40: persist_loop<V_W40>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
+    case V_W012: persist_loop<V_W012>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
+    case V_H40: persist_loop<V_H40>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
+    default: persist_loop<V_H012>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
   }
 }
 
@@ -1287,6 +1334,7 @@ struct rk_context {
   int cu_count = 0;
   int64_t mem_bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t marks[8] = {};  // rk_context_mark slots, created on first use
   std::map<int, int> tuned;  // batch signature -> threads per problem chosen by the autotuner
 };
 
@@ -1672,6 +1720,8 @@ int rk_context_destroy(rk_context* ctx) {
   hipFree(ctx->robots_dev);
   hipEventDestroy(ctx->ev0);
   hipEventDestroy(ctx->ev1);
+  for (hipEvent_t e : ctx->marks)
+    if (e) hipEventDestroy(e);
   hipStreamDestroy(ctx->stream);
   delete ctx;
   return RK_OK;
@@ -1680,6 +1730,22 @@ int rk_context_destroy(rk_context* ctx) {
 int rk_context_sync(rk_context* ctx) {
   if (!ctx) return fail(RK_EINVAL, "null context");
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return RK_OK;
+}
+
+int rk_context_mark(rk_context* ctx, int32_t slot) {
+  if (!ctx || slot < 0 || slot >= 8) return fail(RK_EINVAL, "null context or mark slot outside 0..7");
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (!ctx->marks[slot]) HIP_TRY(hipEventCreate(&ctx->marks[slot]));
+  HIP_TRY(hipEventRecord(ctx->marks[slot], ctx->stream));
+  return RK_OK;
+}
+
+int rk_context_elapsed(rk_context* ctx, int32_t a, int32_t b, float* ms) {
+  if (!ctx || !ms || a < 0 || a >= 8 || b < 0 || b >= 8) return fail(RK_EINVAL, "bad argument");
+  if (!ctx->marks[a] || !ctx->marks[b]) return fail(RK_EINVAL, "mark slot not recorded");
+  HIP_TRY(hipEventSynchronize(ctx->marks[b]));
+  HIP_TRY(hipEventElapsedTime(ms, ctx->marks[a], ctx->marks[b]));
   return RK_OK;
 }
 
@@ -2523,7 +2589,8 @@ static int group_prepare(rk_group* g, GroupLaunch* gl, bool from_seed) {
 }
 
 // enqueue the solve on the context stream: fork one stream per variant, each runs its
-// persistent kernel over its queue, join.  Capturable: no allocation, copy or memset.
+// persistent kernel over its queue, join; or, for a group of register-resident variants outside
+// the timed path, one k_persistent_reg launch.  Capturable: no allocation, copy or memset.
 static int group_launch(rk_group* g, const GroupLaunch& gl, int iters, const LMParams& lp, bool timed) {
   rk_context* ctx = g->ctx;
   const int chunk = g->iters_per_launch > 0 ? g->iters_per_launch : std::max(1, iters);
@@ -2534,6 +2601,34 @@ static int group_launch(rk_group* g, const GroupLaunch& gl, int iters, const LMP
   size_t order[NVARIANT];
   for (size_t k = 0; k < gl.var.size(); ++k) order[k] = k;
   std::stable_sort(order, order + gl.var.size(), [&](size_t a, size_t b) { return gl.tiles[a] > gl.tiles[b]; });
+  bool mix = !timed && gl.var.size() > 1;
+  for (size_t k = 0; k < gl.var.size(); ++k) mix = mix && variant_mixable(gl.var[k]);
+  if (mix) {
+    // one launch: block ranges in the same order (the variant with the most tiles first)
+    MixArgs ma;
+    memset(&ma, 0, sizeof(ma));
+    ma.nslots = (int)gl.var.size();
+    int blocks = 0;
+    for (size_t kk = 0; kk < gl.var.size(); ++kk) {
+      const size_t k = order[kk];
+      ma.variant[kk] = gl.var[k];
+      ma.block0[kk] = blocks;
+      ma.nblocks[kk] = gl.grid[k];
+      ma.first[kk] = gl.first[k];
+      ma.nb[kk] = gl.nb[k];
+      ma.tiles[kk] = gl.tiles[k];
+      ma.ctr[kk] = (int)k;
+      blocks += gl.grid[k];
+    }
+    for (int it0 = 0; it0 < std::max(1, iters); it0 += chunk) {
+      const int n_it = std::max(0, std::min(chunk, iters - it0));
+      hipLaunchKernelGGL(k_persistent_reg, dim3(blocks), dim3(WAVE), 0, ctx->stream, (const Bucket*)g->buckets_dev, ma,
+                         g->counter, lp, it0, n_it);
+      if (iters == 0) break;
+    }
+    HIP_TRY(hipGetLastError());
+    return RK_OK;
+  }
   for (size_t kk = 0; kk < gl.var.size(); ++kk) {
     const size_t k = order[kk];
     hipStream_t st = gl.var.size() == 1 ? ctx->stream : g->aux[k];

@@ -97,7 +97,7 @@ def _close_all() -> None:
 
 EXPORTS = [
     "rk_last_error", "rk_abi_version", "rk_context_create", "rk_context_destroy", "rk_context_sync",
-    "rk_context_mem", "rk_robot_upload", "rk_fk", "rk_link_jacobian", "rk_copy_probe", "rk_se3_op",
+    "rk_context_mem", "rk_context_mark", "rk_context_elapsed", "rk_robot_upload", "rk_fk", "rk_link_jacobian", "rk_copy_probe", "rk_se3_op",
     "rk_batch_create",
     "rk_batch_destroy", "rk_batch_set", "rk_batch_set_rotations", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics", "rk_group_create",
     "rk_group_destroy", "rk_group_set_tuning", "rk_group_autotune", "rk_group_solve", "rk_group_kernel_times",
@@ -190,6 +190,16 @@ class Context:
 
     def sync(self) -> None:
         _check(lib().rk_context_sync(self._h))
+
+    def mark(self, slot: int) -> None:
+        """record HIP event ``slot`` (0..7) on the context stream"""
+        _check(lib().rk_context_mark(self._h, int(slot)))
+
+    def elapsed(self, a: int, b: int) -> float:
+        """milliseconds from mark ``a`` to mark ``b`` (waits for ``b``)"""
+        ms = C.c_float()
+        _check(lib().rk_context_elapsed(self._h, int(a), int(b), C.byref(ms)))
+        return float(ms.value)
 
     def mem_bytes(self) -> int:
         v = C.c_int64()

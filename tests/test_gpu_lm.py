@@ -196,6 +196,60 @@ def test_group_uneven_batch_matches_single(ctx):
         assert np.array_equal(b.get()["q"], s["q"])
 
 
+@pytest.fixture(scope="module")
+def mb900():
+    # one workload (three robots uploaded once) for every tuning case below
+    return W.config1(seed=2, total=900)
+
+
+@pytest.mark.parametrize("tpp", [(1, 1, N.TPP_HALF), (1, 1, N.TPP_WAVE), (N.TPP_WAVE, N.TPP_HALF, N.TPP_HALF),
+                                 (1, 1, 32)])
+def test_group_single_launch_matches_per_variant(ctx, mb900, tpp):
+    """a graph solve of register-resident variants runs as one launch (block ranges per variant);
+    the eager solve runs one kernel per variant on forked streams: same bits.  (1, 1, 32) holds an
+    LDS variant and keeps the forked kernels in both."""
+    mb = mb900
+    batches = []
+    for bk in mb.buckets:
+        b = N.Batch(ctx, bk.spec, _ik_terms(bk.spec, bk.links), bk.size, trace_iters=12)
+        b.set(q=bk.q_seed, pose_targets=bk.targets)
+        batches.append(b)
+    g = N.Group(ctx, batches)
+    g.set_tuning(list(tpp))
+    g.solve(12, C0_PARAMS, use_graph=False, from_seed=True)
+    eager = [b.get(traces=True) for b in batches]
+    for _ in range(2):  # two replays of the captured graph
+        g.solve(12, C0_PARAMS, use_graph=True, from_seed=True)
+        for e, b in zip(eager, batches):
+            out = b.get(traces=True)
+            assert np.array_equal(out["q"], e["q"])
+            assert np.array_equal(out["cost_trace"], e["cost_trace"])
+    g.set_tuning(iters_per_launch=5)  # iteration quanta: three launches per solve
+    g.solve(12, C0_PARAMS, use_graph=True, from_seed=True)
+    for e, b in zip(eager, batches):
+        assert np.array_equal(b.get()["q"], e["q"])
+
+
+def test_context_marks_bracket_enqueued_solves(ctx, mb900):
+    """rk_context_mark / rk_context_elapsed: HIP events on the solve stream around enqueued work
+    measure at least the solve itself; slots outside 0..7 and unrecorded slots are refused"""
+    bk = mb900.buckets[2]
+    b = N.Batch(ctx, bk.spec, _ik_terms(bk.spec, bk.links), bk.size)
+    b.set(q=bk.q_seed, pose_targets=bk.targets)
+    g = N.Group(ctx, [b])
+    one = g.solve(10, C0_PARAMS, use_graph=True, from_seed=True)
+    ctx.mark(4)
+    for _ in range(3):
+        g.solve(10, C0_PARAMS, use_graph=True, from_seed=True, sync=False)
+    ctx.mark(5)
+    three = ctx.elapsed(4, 5)
+    assert three > 2.0 * one > 0.0, (three, one)
+    with pytest.raises(ValueError):
+        ctx.mark(8)
+    with pytest.raises(ValueError):
+        ctx.elapsed(4, 7)
+
+
 def test_autotune_keeps_results(ctx):
     """the tuner's trial solves leave the batches as set; the tuned graph solve equals an eager
     solve with the same kernel choice from the same seeds (the tuner may pick a different kernel
