@@ -233,6 +233,8 @@ def main() -> int:
     ap.add_argument("--serve-capacity-seconds", type=float, default=2.0,
                     help="length of the capacity trace; 0 skips it")
     ap.add_argument("--no-traj", dest="traj", action="store_false", help="skip the trajectory leg")
+    ap.add_argument("--population-problems", type=int, default=2048,
+                    help="Panda-DH problems of the population IK leg (0 skips it)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a streaming copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -405,6 +407,45 @@ def main() -> int:
             "kernels": [kernel_roofline(k) for k in range(n_k)],
         },
     }
+
+    # ---- population IK (the IK front end's schedule: 64 Roberts candidates per problem x 6 iterations
+    # at lambda 10, best 4 x 10 at lambda 1, best 1), against one candidate from the mid-range
+    if args.population_problems > 0:
+        from robokit_amd import population as P
+        pspec = W.panda_dh()
+        pn = args.population_problems
+        pwl = W.ik_batch(pspec, [pspec.num_links - 1], pn, seed=args.seed + 7)
+        pterms = N.TermSpec(pose_links=pwl.links, pose_weights=np.ones((1, 6), np.float32),
+                            limit_weights=np.ones(7, np.float32))
+        pop = P.PopulationIK(ctx, pspec, pterms, pn)
+        pop.set_targets(pose_targets=pwl.targets)
+        pop.solve()
+        p_ms = []
+        for _ in range(3):
+            ctx.mark(2)
+            win = pop.solve()
+            ctx.mark(3)
+            p_ms.append(ctx.elapsed(2, 3))
+        p_ok = int(win.metrics()[2].sum())
+        one = N.Batch(ctx, pspec, pterms, pn)
+        one.set(q=np.tile(pop.seeds[:1], (pn, 1)), pose_targets=pwl.targets)
+        one.solve(16, N.LMParams(lm_lambda=10.0), from_seed=True)
+        one_ok = int(one.metrics()[2].sum())
+        pm = float(np.mean(p_ms))
+        out["population"] = {
+            "workload": f"{pn} Panda-DH 7-DOF IK problems, 64 Roberts candidates each around the joint mid-range: "
+                        "6 LM iterations (lambda0 10), best 4 x 10 iterations (lambda0 1), best 1; "
+                        "FrameTask(log) + PositionLimit; 1 mm / 0.01 rad",
+            "ms_per_solve": round(pm, 4),
+            "what": "HIP events on the solve stream around the two LM launches and two selects (host waits between "
+                    "the calls included)",
+            "problems_per_s": round(pn / (pm * 1e-3), 1),
+            "converged": p_ok,
+            "converged_per_s": round(p_ok / (pm * 1e-3), 1),
+            "single_candidate_converged": one_ok,
+            "single_candidate_what": "the same problems, one candidate at the mid-range, 16 LM iterations (lambda0 10)",
+        }
+        print(f"[bench] population: {out['population']}", file=sys.stderr, flush=True)
 
     # ---- trajectory LM (512 collision-aware trajectories of 64 frames): kernel time and outcome
     if args.traj:

@@ -158,6 +158,15 @@ int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mo
                    float* kernel_ms);
 /* rotation targets [B][n_rot][4] (w, x, y, z) of the live batch (after rk_batch_set) */
 int rk_batch_set_rotations(rk_batch* b, int32_t B, const float* rot_targets);
+/* Population select (PopulationSolver._select, opt/population_solver.py:436-449, with the order
+ * of _select_best_k_kernel, :24-44): src holds n_inst problems x width_in candidates (row
+ * i * width_in + j = candidate j of problem i); for each problem keep the width_out candidates of
+ * lowest cost (the cost at the current iterate; NaN last; equal costs in candidate order) and
+ * write them best first into dst rows i * width_out + r: q (also as dst's seeds), targets,
+ * q_prev and the cost (rk_batch_get reads it).  dst takes n_inst * width_out problems, must share src's robot and term structure, and
+ * its next solve starts afresh (lambda = lm_lambda).  1 <= width_out <= width_in <= 64.
+ * src_rows [n_inst * width_out] (nullable, host): the source row of each dst row. */
+int rk_batch_select(rk_batch* src, int32_t width_in, rk_batch* dst, int32_t width_out, int32_t* src_rows);
 /* warm start for a sequence of frames (helpers/retarget: the previous frame's solution is the
  * next frame's seed and its SmoothnessTask reference, terms/dense/smoothness_task.py:350-365):
  * on the device, q_prev <- q and seeds <- q; the next rk_batch_set then only brings new targets */

@@ -1,0 +1,72 @@
+"""CPU restatement of the reference's population IK path (numpy over the C LM oracle).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ and bench.py's baseline leg to check results; the
+product package (robokit_amd) never imports it.
+
+  roberts_samples      opt/sampler.py:246-264
+  select_best_k        opt/population_solver.py:24-44 (_select_best_k_kernel: cost ascending,
+                       NaN as +inf, equal costs in candidate order), restated as a stable sort
+  population_ik        opt/population_solver.py:291-395 for the IK schedule of
+                       helpers/ik/config.py:37-45: each segment a fresh LM (oracle.lm_solve) on
+                       the kept candidates, then a select; the last select keeps one winner
+
+Pins (tests/test_population.py): the R_d closed forms (golden ratio for d = 1, plastic number for
+d = 2), the reference test's coverage property (tests/opt/test_sampler.py:57-61), the ranking
+golden case of tests/helpers/motion_plan/test_motion_plan.py:2871-2889, and the reference's
+plain selection loop on random costs with ties, infinities and NaNs.
+"""
+from __future__ import annotations
+
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+from . import oracle as O
+
+
+def roberts_samples(num_points: int, dim: int, offset: int = 0) -> np.ndarray:
+    """frac(n * (1 - phi^-(k+1))), phi the root > 1 of x^(d+1) = x + 1 (Newton from 1)"""
+    x = 1.0
+    for _ in range(10000):
+        f = x ** (dim + 1) - x - 1.0
+        x = x - f / ((dim + 1) * x ** dim - 1.0)
+        if abs(f) < 1e-10:
+            break
+    alpha = 1.0 - 1.0 / x ** (1.0 + np.arange(dim))
+    n = (np.arange(num_points) + offset).astype(np.float64)
+    return np.modf(np.outer(n, alpha))[0]
+
+
+def select_best_k(costs: np.ndarray, width_in: int, width_out: int) -> np.ndarray:
+    """[n_inst, width_out] flat source rows, best first (stable sort on the NaN-as-inf cost)"""
+    c = np.asarray(costs).reshape(-1, width_in)
+    key = np.where(np.isnan(c), np.inf, c)
+    order = np.argsort(key, axis=1, kind="stable")[:, :width_out]
+    return order + (np.arange(c.shape[0]) * width_in)[:, None]
+
+
+def population_ik(spec, links: Sequence[int], targets: np.ndarray, seeds: np.ndarray,
+                  stages: List[Tuple[int, int, O.LMParams]], dtype: str = "f32"):
+    """targets [n_inst, n_pose, 7]; seeds [width0, na] shared by every problem; stages
+    [(iters, width, params)].  Returns (q [n_inst, na], cost [n_inst], rows of every select)."""
+    n_inst, na = targets.shape[0], seeds.shape[1]
+    w = stages[0][1]
+    inst = np.repeat(np.arange(n_inst), w)
+    rt = np.float64 if dtype == "f64" else np.float32
+    q = np.tile(seeds, (n_inst, 1)).astype(rt)
+    cost = None
+    selects = []
+    for k, (iters, width, params) in enumerate(stages):
+        if k > 0 and width < w:
+            rows = select_best_k(cost, w, width).reshape(-1)
+            selects.append(rows)
+            q, cost, inst, w = q[rows], cost[rows], inst[rows], width
+        if iters > 0 or k == 0:
+            terms = O.ik_terms(links, targets[inst], na)
+            res = O.lm_solve(spec, terms, q, iters, params, dtype=dtype, traces=False)
+            q, cost = res["q"].astype(rt), res["cost"].astype(rt)
+    if w > 1:
+        rows = select_best_k(cost, w, 1).reshape(-1)
+        selects.append(rows)
+        q, cost = q[rows], cost[rows]
+    return q, cost, selects

@@ -34,7 +34,7 @@
 
 #define RK_MAXT 4   // pose targets per problem
 #define RK_MAXQ 12  // position targets per problem
-#define RK_MAX_ROBOTS 64
+#define RK_MAX_ROBOTS 256
 
 namespace rk {
 
@@ -1325,6 +1325,50 @@ static int fail(int code, const std::string& msg) {
       return fail(RK_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));             \
   } while (0)
 
+// ---------------------------------------------------------------- population: best-k select + gather
+struct SelectArgs {
+  const float *cost, *q, *pose, *pos, *rot, *qprev;
+  float *q_out, *seed_out, *pose_out, *pos_out, *rot_out, *qprev_out, *cost_out;
+  int* sel;
+  int cap_in, cap_out, width_in, width_out, na, n_pose7, n_pos3, n_rot4;
+};
+
+// One wavefront per problem; lane j holds candidate j (width_in <= 64).  A candidate's rank is
+// the number of candidates before it in the order (cost ascending, NaN as +inf, then candidate
+// index), the order of _select_best_k_kernel (opt/population_solver.py:24-44): the keys are
+// distinct, so the ranks are a permutation and rank r < width_out names output row r.  The
+// selected rows (q, also written as the next stage's seeds, every target field and the cost) are then
+// copied field by field, lanes over the fields of one row (problem-major SoA, stride = capacity).
+__global__ __launch_bounds__(64) void k_select_gather(SelectArgs a) {
+  __shared__ int pick[WAVE];
+  const int b = blockIdx.x, j = threadIdx.x;
+  float c = j < a.width_in ? a.cost[b * a.width_in + j] : INFINITY;
+  if (c != c) c = INFINITY;
+  int rank = 0;
+  for (int i = 0; i < a.width_in; ++i) {
+    const float ci = __shfl(c, i);
+    rank += (ci < c || (ci == c && i < j)) ? 1 : 0;
+  }
+  if (j < a.width_in && rank < a.width_out) pick[rank] = j;
+  __syncthreads();
+  for (int r = 0; r < a.width_out; ++r) {
+    const int si = b * a.width_in + pick[r], di = b * a.width_out + r;
+    if (j == 0) {
+      a.sel[di] = si;
+      a.cost_out[di] = a.cost[si];
+    }
+    for (int f = j; f < a.na; f += WAVE) {
+      const float v = a.q[(size_t)f * a.cap_in + si];
+      a.q_out[(size_t)f * a.cap_out + di] = v;
+      a.seed_out[(size_t)f * a.cap_out + di] = v;
+      a.qprev_out[(size_t)f * a.cap_out + di] = a.qprev[(size_t)f * a.cap_in + si];
+    }
+    for (int f = j; f < a.n_pose7; f += WAVE) a.pose_out[(size_t)f * a.cap_out + di] = a.pose[(size_t)f * a.cap_in + si];
+    for (int f = j; f < a.n_pos3; f += WAVE) a.pos_out[(size_t)f * a.cap_out + di] = a.pos[(size_t)f * a.cap_in + si];
+    for (int f = j; f < a.n_rot4; f += WAVE) a.rot_out[(size_t)f * a.cap_out + di] = a.rot[(size_t)f * a.cap_in + si];
+  }
+}
+
 struct rk_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -1354,6 +1398,7 @@ struct rk_batch {
   float *q_trace = nullptr, *lam_trace = nullptr;
   PlainBufs pb{};
   bool plain_alloc = false;
+  int* sel = nullptr;  // rk_batch_select: source row of each row (allocated on first select into the batch)
   int64_t bytes = 0;
 };
 
@@ -2061,7 +2106,7 @@ int rk_batch_destroy(rk_batch* b) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   void* ptrs[] = {b->rot_tgt, b->tt, b->td, b->q, b->q_seed, b->pose_tgt, b->pos_tgt, b->q_prev, b->cost, b->lam, b->nacc, b->trace,
-                  b->acc_trace, b->q_trace, b->lam_trace, b->pb.Tj, b->pb.Sw, b->pb.J, b->pb.r, b->pb.rp, b->pb.qp, b->pb.dq, b->pb.pred,
+                  b->acc_trace, b->q_trace, b->lam_trace, b->sel, b->pb.Tj, b->pb.Sw, b->pb.J, b->pb.r, b->pb.rp, b->pb.qp, b->pb.dq, b->pb.pred,
                   b->pb.acc};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -2260,6 +2305,43 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
   if (pos_targets && b->th.n_pos && (rc = upload_soa(ctx, b->pos_tgt, pos_targets, B, b->th.n_pos * 3, b->capacity)))
     return rc;
   if (q_prev && (rc = upload_soa(ctx, b->q_prev, q_prev, B, b->na, b->capacity))) return rc;
+  return RK_OK;
+}
+
+int rk_batch_select(rk_batch* src, int32_t width_in, rk_batch* dst, int32_t width_out, int32_t* src_rows) {
+  if (!src || !dst) return fail(RK_EINVAL, "null batch");
+  if (src == dst) return fail(RK_EINVAL, "select needs two batches (the source is read while the target is written)");
+  if (src->ctx != dst->ctx) return fail(RK_EINVAL, "batches of different contexts");
+  if (src->traj_T || dst->traj_T) return fail(RK_EINVAL, "trajectory batches have no candidate select");
+  if (width_in < 1 || width_in > WAVE || width_out < 1 || width_out > width_in)
+    return fail(RK_EINVAL, "need 1 <= width_out <= width_in <= 64");
+  if (src->n <= 0 || src->n % width_in)
+    return fail(RK_EINVAL, "the source batch size " + std::to_string(src->n) + " is not a multiple of width_in " +
+                               std::to_string(width_in));
+  const Terms &a = src->th, &c = dst->th;
+  if (a.robot != c.robot || a.n_pose != c.n_pose || a.n_pos != c.n_pos || a.n_rot != c.n_rot || src->na != dst->na)
+    return fail(RK_EINVAL, "source and target batches have different robots or term structures");
+  const int n_inst = src->n / width_in, n_out = n_inst * width_out;
+  if (n_out > dst->capacity)
+    return fail(RK_ECAPACITY, "batch size " + std::to_string(n_out) + " exceeds the " + std::to_string(dst->capacity) +
+                                  " this batch was built for");
+  rk_context* ctx = src->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc;
+  if (!dst->sel && (rc = dmalloc(ctx, (void**)&dst->sel, sizeof(int) * dst->capacity, &dst->bytes))) return rc;
+  SelectArgs sa{src->cost,     src->q,         src->pose_tgt, src->pos_tgt, src->rot_tgt, src->q_prev,
+                dst->q,        dst->q_seed,    dst->pose_tgt, dst->pos_tgt, dst->rot_tgt, dst->q_prev,
+                dst->cost,     dst->sel,      src->capacity,  dst->capacity, width_in,     width_out,    src->na,
+                a.n_pose * 7,  a.n_pos * 3,    a.n_rot * 4};
+  hipLaunchKernelGGL(k_select_gather, dim3(n_inst), dim3(WAVE), 0, ctx->stream, sa);
+  HIP_TRY(hipGetLastError());
+  dst->n = n_out;
+  dst->it_done = 0;
+  if (src_rows) {
+    HIP_TRY(hipMemcpyAsync(src_rows, dst->sel, sizeof(int) * n_out, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
   return RK_OK;
 }
 

@@ -99,7 +99,7 @@ EXPORTS = [
     "rk_last_error", "rk_abi_version", "rk_context_create", "rk_context_destroy", "rk_context_sync",
     "rk_context_mem", "rk_context_mark", "rk_context_elapsed", "rk_robot_upload", "rk_fk", "rk_link_jacobian", "rk_copy_probe", "rk_se3_op",
     "rk_batch_create",
-    "rk_batch_destroy", "rk_batch_set", "rk_batch_set_rotations", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics", "rk_group_create",
+    "rk_batch_destroy", "rk_batch_set", "rk_batch_set_rotations", "rk_batch_select", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics", "rk_group_create",
     "rk_group_destroy", "rk_group_set_tuning", "rk_group_autotune", "rk_group_solve", "rk_group_kernel_times",
     "rk_server_create", "rk_server_add_terms", "rk_server_add_traj", "rk_server_submit", "rk_server_poll", "rk_server_end_session",
     "rk_server_stats", "rk_server_destroy", "rk_traj_batch_create", "rk_traj_metrics",
@@ -358,6 +358,16 @@ class Batch:
         out = dict(q=q, cost=cost, lam=lam)
         if traces:
             out.update(cost_trace=ct, accept=at)
+        return out
+
+    def select(self, width_in: int, dst: "Batch", width_out: int, rows: bool = False) -> Optional[np.ndarray]:
+        """Keep each problem's best ``width_out`` of its ``width_in`` candidates (rows
+        i * width_in + j) in ``dst``, best first: lowest cost at the current iterate, NaN last,
+        equal costs in candidate order.  Returns the source row of each ``dst`` row if ``rows``."""
+        n_out = self.n // int(width_in) * int(width_out)
+        out = np.zeros(n_out, np.int32) if rows else None
+        _check(lib().rk_batch_select(self._h, int(width_in), dst.handle, int(width_out), _p(out)))
+        dst.n = n_out
         return out
 
     def advance(self) -> None:

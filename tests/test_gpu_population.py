@@ -1,0 +1,117 @@
+"""GPU parity of the population IK path (rk_batch_select + the staged schedule of
+robokit_amd.population) against the oracle.
+
+Bars: the select is integer work, so its rows must equal the oracle's ranking of the GPU's own
+costs exactly, and the gathered q and costs must be the source rows' bits; the whole schedule
+(64 candidates x 6 iterations at lambda 10, best 4 x 10 at lambda 1, best 1) must classify the
+winners at 1 mm / 0.01 rad like the FP32 oracle population, with no more disagreements than the
+FP32 oracle has with its FP64 twin (LM steps and hence selects fork on rounding-level costs).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import population as OP
+from robokit_amd import native as N
+from robokit_amd import population as P
+from robokit_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _terms(spec, links):
+    na = spec.num_actuated_joints
+    return N.TermSpec(pose_links=links, pose_weights=np.ones((len(links), 6), np.float32),
+                      limit_weights=np.ones(na, np.float32))
+
+
+@pytest.fixture(scope="module")
+def panda():
+    spec = W.panda_dh()
+    return spec, W.ik_batch(spec, [spec.num_links - 1], 48, seed=31)
+
+
+def test_select_gathers_the_oracle_ranking(ctx, panda):
+    spec, wl = panda
+    pop = P.PopulationIK(ctx, spec, _terms(spec, wl.links), wl.size)
+    pop.set_targets(pose_targets=wl.targets)
+    b64, b4, b1 = pop._batches[64], pop._batches[4], pop._batches[1]
+    print(f"\n[select] panda {wl.size} problems x 64 candidates", flush=True)
+    b64.solve(6, pop.config.optimizer[0], from_seed=True)
+    src = b64.get()
+    rows = b64.select(64, b4, 4, rows=True)
+    np.testing.assert_array_equal(rows, OP.select_best_k(src["cost"], 64, 4).reshape(-1))
+    dst = b4.get()
+    np.testing.assert_array_equal(dst["q"], src["q"][rows])
+    np.testing.assert_array_equal(dst["cost"], src["cost"][rows])
+    # the targets travelled with their rows: the kept candidates' errors against their own problem
+    pe, ae, _ = b4.metrics()
+    pe_o, ae_o, _ = O.ik_metrics(spec, dst["q"], wl.links, wl.targets[rows // 64])
+    np.testing.assert_allclose(pe, pe_o, rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(ae, ae_o, rtol=1e-4, atol=1e-5)
+    # second select (4 -> 1) from the kept population, after its own LM segment
+    b4.solve(10, pop.config.optimizer[1])
+    s4 = b4.get()
+    rows1 = b4.select(4, b1, 1, rows=True)
+    np.testing.assert_array_equal(rows1, OP.select_best_k(s4["cost"], 4, 1).reshape(-1))
+    np.testing.assert_array_equal(b1.get()["q"], s4["q"][rows1])
+
+
+def test_select_ties_keep_candidate_order(ctx, panda):
+    """scale 0: every candidate of a problem is the mid-range seed, so their LM runs give the same
+    bits and the costs tie: the select keeps candidates 0..3 in order"""
+    spec, wl = panda
+    pop = P.PopulationIK(ctx, spec, _terms(spec, wl.links), 8, scale=0.0)
+    pop.set_targets(pose_targets=wl.targets[:8])
+    b64, b4 = pop._batches[64], pop._batches[4]
+    b64.solve(6, pop.config.optimizer[0], from_seed=True)
+    c = b64.get()["cost"].reshape(8, 64)
+    assert np.all(c == c[:, :1])
+    rows = b64.select(64, b4, 4, rows=True)
+    np.testing.assert_array_equal(rows, (np.arange(8)[:, None] * 64 + np.arange(4)).reshape(-1))
+
+
+def test_select_rejects_bad_shapes(ctx, panda):
+    spec, wl = panda
+    terms = _terms(spec, wl.links)
+    a = N.Batch(ctx, spec, terms, 64)
+    b = N.Batch(ctx, spec, terms, 4)
+    a.set(q=np.zeros((60, 7), np.float32), pose_targets=wl.targets[:1].repeat(60, 0))
+    with pytest.raises(ValueError, match="multiple"):
+        a.select(64, b, 4)
+    a.set(q=np.zeros((64, 7), np.float32), pose_targets=wl.targets[:1].repeat(64, 0))
+    with pytest.raises(ValueError, match="exceeds"):
+        a.select(8, b, 2)  # 8 problems x 2 > capacity 4
+    with pytest.raises(ValueError, match="width"):
+        a.select(64, b, 65)
+    other = W.ur10()
+    c = N.Batch(ctx, other, _terms(other, [other.num_links - 1]), 4)
+    with pytest.raises(ValueError, match="robots or term"):
+        a.select(64, c, 4)
+
+
+def test_population_parity(ctx, panda):
+    spec, wl = panda
+    pop = P.PopulationIK(ctx, spec, _terms(spec, wl.links), wl.size)
+    pop.set_targets(pose_targets=wl.targets)
+    print(f"\n[population] panda {wl.size} problems, default IK schedule", flush=True)
+    win = pop.solve()
+    out = win.get()
+    _, _, ok = win.metrics()
+    stages = [(s.iters, w, O.LMParams(**p.__dict__)) for s, w, p in
+              zip(pop.config.stages, pop.config.num_candidates, pop.config.optimizer)]
+    q32, c32, sel32 = OP.population_ik(spec, wl.links, wl.targets, pop.seeds, stages)
+    q64, c64, _ = OP.population_ik(spec, wl.links, wl.targets, pop.seeds, stages, dtype="f64")
+    ok32 = O.ik_metrics(spec, q32, wl.links, wl.targets)[2]
+    ok64 = O.ik_metrics(spec, q64, wl.links, wl.targets, dtype="f64")[2]
+    mism, n_ref = int(np.sum(ok != ok32)), int(np.sum(ok64 != ok32))
+    same = float(np.mean(np.abs(out["cost"] - c32) <= 1e-4 * np.maximum(np.abs(c32), 1e-6)))
+    print(f"  converged gpu {ok.mean():.3f} f32 {ok32.mean():.3f} f64 {ok64.mean():.3f}; classification "
+          f"differs gpu/f32 {mism}, f32/f64 {n_ref}; winner cost within 1e-4: {same:.3f}")
+    assert mism <= max(n_ref, 1)
+    assert np.all(np.isfinite(out["cost"]))
+    # the population beats one candidate from the mid-range: plain LM, same 16 iterations
+    one = N.Batch(ctx, spec, _terms(spec, wl.links), wl.size)
+    one.set(q=np.tile(pop.seeds[:1], (wl.size, 1)), pose_targets=wl.targets)
+    one.solve(16, N.LMParams(lm_lambda=10.0), from_seed=True)
+    assert ok.sum() >= one.metrics()[2].sum()
