@@ -124,28 +124,49 @@ class PopulationIK:
         self._batches: Dict[int, N.Batch] = {}
         for w in dict.fromkeys(self.widths + [1]):
             self._batches[w] = N.Batch(ctx, spec, terms, self.batch_size * w)
-        self.seeds = self.candidate_seeds(seed, keep_first, scale)
+        self.offsets = candidate_offsets(self.widths[0], spec.num_actuated_joints,
+                                         None if seed is None else int(seed), keep_first)
+        self.seeds = self.candidate_seeds(scale)
+        self._global_loaded = False  # the tiled global seeds are resident in the widest batch
         self._ready = False
 
-    def candidate_seeds(self, seed: Optional[int], keep_first: bool, scale: float) -> np.ndarray:
-        """[width0, na] joint seeds shared by every problem: mid-range + offset x joint range, clamped
-        to the float32 limits"""
+    def candidate_seeds(self, scale: float = 1.0, center: Optional[np.ndarray] = None) -> np.ndarray:
+        """Joint seeds: centre + offset x (scale x joint range), clamped to the float32 limits.
+        center None: the joint mid-range, [width0, na] shared by every problem; else
+        center [batch, na]: [batch * width0, na], problem i's candidates in rows i * width0 + j."""
         lim = np.asarray(self.spec.actuated_joint_limits)
         rng = (lim.astype(np.float32)[:, 1] - lim.astype(np.float32)[:, 0]) * np.float32(scale)
-        mid = (0.5 * (lim[:, 0] + lim[:, 1])).astype(np.float32)
-        off = candidate_offsets(self.widths[0], lim.shape[0], None if seed is None else int(seed), keep_first)
-        q = mid[None, :] + off * rng[None, :]
+        step = self.offsets * rng[None, :]
+        if center is None:
+            q = (0.5 * (lim[:, 0] + lim[:, 1])).astype(np.float32)[None, :] + step
+        else:
+            c = np.asarray(center, np.float32).reshape(-1, lim.shape[0])
+            q = (c[:, None, :] + step[None, :, :]).reshape(-1, lim.shape[0])
         lo, hi = self.spec.actuated_joint_limits_f32[:, 0], self.spec.actuated_joint_limits_f32[:, 1]
         return np.clip(q, lo, hi).astype(np.float32)
 
     def set_targets(self, pose_targets: Optional[np.ndarray] = None, pos_targets: Optional[np.ndarray] = None,
-                    q_prev: Optional[np.ndarray] = None) -> None:
-        """each problem's targets [batch, ...], broadcast over its candidates (rows i * width + j)"""
+                    q_prev: Optional[np.ndarray] = None, init_q: Optional[np.ndarray] = None,
+                    init_scale: float = 0.2) -> None:
+        """each problem's targets [batch, ...], broadcast over its candidates (rows i * width + j).
+        Seeds: init_q None: the global mid-range candidates (uploaded once, then kept resident);
+        else init_q [batch, na] spread by init_scale x joint range (helpers/ik/ik.py:509-532)."""
         w0 = self.widths[0]
         rep = lambda a: None if a is None else np.repeat(np.asarray(a, np.float32), w0, axis=0)  # noqa: E731
         b0 = self._batches[w0]
-        b0.set(q=np.tile(self.seeds, (self.batch_size, 1)), pose_targets=rep(pose_targets),
-               pos_targets=rep(pos_targets), q_prev=rep(q_prev))
+        q = None
+        if init_q is not None:
+            if w0 == 1:  # one candidate sits on its centre
+                q = np.clip(np.asarray(init_q, np.float32), self.spec.actuated_joint_limits_f32[:, 0],
+                            self.spec.actuated_joint_limits_f32[:, 1])
+            else:
+                q = self.candidate_seeds(init_scale, center=init_q)
+            self._global_loaded = False
+        elif not self._global_loaded:
+            q = np.tile(self.seeds, (self.batch_size, 1))
+            self._global_loaded = True
+        b0.set(q=q, pose_targets=rep(pose_targets), pos_targets=rep(pos_targets), q_prev=rep(q_prev),
+               n=self.batch_size * w0)
         self._ready = True
 
     def solve(self, tpp: int = 0) -> N.Batch:

@@ -1,0 +1,95 @@
+"""GPU parity of the IKSolver front end (robokit_amd.ik) against the oracle population.
+
+Bars: the winners classify like the FP32 oracle population (no more disagreements than FP32 vs
+FP64); compute_metrics_numpy equals the oracle's metrics at the reference's default thresholds;
+the previous state is kept exactly where its cost is no worse than the winner's; candidates
+around an initial state put candidate 0 on it.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import population as OP
+from robokit_amd import ik as IK
+from robokit_amd import population as P
+from robokit_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def panda():
+    spec = W.panda_dh()
+    return spec, W.ik_batch(spec, [spec.num_links - 1], 32, seed=41)
+
+
+def _stages(cfg):
+    return [(s.iters, w, O.LMParams(**p.__dict__)) for s, w, p in
+            zip(cfg.stages, cfg.num_candidates, cfg.optimizer)]
+
+
+def test_solver_matches_oracle_population(ctx, panda):
+    spec, wl = panda
+    solver = IK.IKSolver(IK.IKSolverConfig(), spec, wl.links, ctx=ctx)
+    print(f"\n[ik] panda {wl.size} targets, default schedule", flush=True)
+    q = solver.solve_numpy(wl.targets)
+    q32, c32, _ = OP.population_ik(spec, wl.links, wl.targets, solver._pop.seeds, _stages(solver.config.solver))
+    q64, _, _ = OP.population_ik(spec, wl.links, wl.targets, solver._pop.seeds, _stages(solver.config.solver),
+                                 dtype="f64")
+    ok = O.ik_metrics(spec, q, wl.links, wl.targets)[2]
+    ok32 = O.ik_metrics(spec, q32, wl.links, wl.targets)[2]
+    ok64 = O.ik_metrics(spec, q64, wl.links, wl.targets, dtype="f64")[2]
+    mism, n_ref = int(np.sum(ok != ok32)), int(np.sum(ok64 != ok32))
+    print(f"  converged gpu {ok.mean():.3f} f32 {ok32.mean():.3f}; differ gpu/f32 {mism}, f32/f64 {n_ref}")
+    assert mism <= max(n_ref, 1)
+    # metrics at the reference's defaults (5 mm, 0.05 rad)
+    pe, ae, succ = solver.compute_metrics_numpy(q, wl.targets)
+    pe_o, ae_o, succ_o = O.ik_metrics(spec, q, wl.links, wl.targets, pos_thr=0.005, ang_thr=0.05)
+    np.testing.assert_allclose(pe, pe_o, rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(ae, ae_o, rtol=1e-4, atol=1e-5)
+    assert np.sum(succ != succ_o) <= 1
+    with pytest.raises(ValueError, match="mismatch"):
+        solver.solve_numpy(wl.targets[:5])
+    with pytest.raises(ValueError, match="shape"):
+        solver.solve_numpy(np.zeros((32, 6), np.float32))
+
+
+def test_prev_state_kept_on_ties(ctx, panda):
+    """select_better_score_kernel (helpers/ik/ik.py:43-57): prev_q wins where its cost <= the
+    winner's; the solve itself is reproducible, so the other rows are the plain solve's winners"""
+    spec, wl = panda
+    solver = IK.IKSolver(IK.IKSolverConfig(), spec, wl.links, ctx=ctx)
+    q1 = solver.solve_numpy(wl.targets)
+    q2 = solver.solve_numpy(wl.targets)
+    np.testing.assert_array_equal(q1, q2)  # same seeds, same bits
+    # a previous state that is the winner itself ties and is kept; a perturbed one loses
+    prev = q1.copy()
+    prev[::2] += 0.05
+    prev = np.clip(prev, spec.actuated_joint_limits_f32[:, 0], spec.actuated_joint_limits_f32[:, 1])
+    q3 = solver.solve_numpy(wl.targets, prev_q=prev)
+    keep = solver.last_prev_cost <= solver.last_cost
+    np.testing.assert_array_equal(q3[keep], prev[keep])
+    np.testing.assert_array_equal(q3[~keep], q1[~keep])
+    # the unperturbed rows score the winner itself: its cost again, to rounding
+    np.testing.assert_allclose(solver.last_prev_cost[1::2], solver.last_cost[1::2], rtol=1e-4, atol=1e-9)
+    # the previous state's score is the terms' cost at prev_q
+    c_o = O.lm_solve(spec, O.ik_terms(wl.links, wl.targets, 7), prev, 0, O.LMParams(), traces=False)["cost"]
+    np.testing.assert_allclose(solver.last_prev_cost, c_o, rtol=1e-4, atol=1e-7)
+
+
+def test_candidates_around_an_initial_state(ctx, panda):
+    """zero-iteration schedule: the select keeps the best seed, and candidate 0 sits on init_q, the
+    exact joint solution, so it wins"""
+    spec, wl = panda
+    cfg = IK.IKSolverConfig(solver=P.PopulationSolverConfig([P.Stage(0, 64), P.Stage(0, 1)]))
+    solver = IK.IKSolver(cfg, spec, wl.links, ctx=ctx)
+    init = np.clip(wl.q_target, spec.actuated_joint_limits_f32[:, 0], spec.actuated_joint_limits_f32[:, 1])
+    q = solver.solve_numpy(wl.targets, init_q=init, init_sample_range=0.1)
+    np.testing.assert_array_equal(q, init)
+    # full schedule from the initial states converges at least as often as from the mid-range
+    s_mid = IK.IKSolver(IK.IKSolverConfig(), spec, wl.links, ctx=ctx)
+    ok_mid = s_mid.compute_metrics_numpy(s_mid.solve_numpy(wl.targets), wl.targets)[2]
+    noisy = np.clip(wl.q_target + np.random.default_rng(2).normal(0, 0.05, wl.q_target.shape).astype(np.float32),
+                    spec.actuated_joint_limits_f32[:, 0], spec.actuated_joint_limits_f32[:, 1])
+    ok_init = s_mid.compute_metrics_numpy(s_mid.solve_numpy(wl.targets, init_q=noisy), wl.targets)[2]
+    assert ok_init.sum() >= ok_mid.sum()

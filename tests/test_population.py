@@ -102,8 +102,9 @@ def test_default_ik_schedule():
 class _NoDevice(P.PopulationIK):
     """the seeding of PopulationIK without its device batches"""
 
-    def __init__(self, spec, config):
+    def __init__(self, spec, config, seed=None, keep_first=True):
         self.spec, self.config, self.widths = spec, config, config.num_candidates
+        self.offsets = P.candidate_offsets(self.widths[0], spec.num_actuated_joints, seed, keep_first)
 
 
 @pytest.mark.parametrize("robot", ["panda_dh", "ur10", "humanoid"])
@@ -111,7 +112,7 @@ def test_candidate_seeds(robot):
     """helpers/ik/ik.py:509-532: mid-range + offset x (float32 upper - lower), float32 arithmetic,
     clamped to the float32 limits; candidate 0 is the mid-range"""
     spec = {"panda_dh": W.panda_dh, "ur10": W.ur10, "humanoid": lambda: W.humanoid29(0)}[robot]()
-    seeds = _NoDevice(spec, P.default_ik_config()).candidate_seeds(None, True, 1.0)
+    seeds = _NoDevice(spec, P.default_ik_config()).candidate_seeds(1.0)
     lim = np.asarray(spec.actuated_joint_limits)
     lim32 = spec.actuated_joint_limits_f32
     assert seeds.dtype == np.float32 and seeds.shape == (64, spec.num_actuated_joints)
@@ -130,7 +131,7 @@ def test_population_oracle_select_consistency():
     they belong to, and the winner's cost is the lowest of its final candidates"""
     spec = W.panda_dh()
     wl = W.ik_batch(spec, [spec.num_links - 1], 3, seed=5)
-    seeds = _NoDevice(spec, P.default_ik_config()).candidate_seeds(None, True, 1.0)[:8]
+    seeds = _NoDevice(spec, P.default_ik_config()).candidate_seeds(1.0)[:8]
     stages = [(3, 8, OP.O.LMParams(lm_lambda=10.0)), (4, 2, OP.O.LMParams(lm_lambda=1.0)), (0, 1, None)]
     q, cost, sel = OP.population_ik(spec, wl.links, wl.targets, seeds, stages)
     assert q.shape == (3, 7) and cost.shape == (3,)
@@ -141,3 +142,32 @@ def test_population_oracle_select_consistency():
 
 def test_select_abi_declared():
     assert "rk_batch_select" in N.EXPORTS
+
+
+def test_candidate_seeds_around_an_initial_state():
+    """helpers/ik/ik.py:513-532: with init_state, problem i's candidates are init_q[i] + offset x
+    (scale x joint range), clamped; candidate 0 stays on init_q (keep_first)"""
+    spec = W.panda_dh()
+    nd = _NoDevice(spec, P.default_ik_config())
+    lim32 = spec.actuated_joint_limits_f32
+    rng = np.random.default_rng(3)
+    init = (lim32[:, 0] + (lim32[:, 1] - lim32[:, 0]) * rng.random((5, 7))).astype(np.float32)
+    s = nd.candidate_seeds(0.2, center=init)
+    assert s.shape == (5 * 64, 7)
+    np.testing.assert_array_equal(s[::64], np.clip(init, lim32[:, 0], lim32[:, 1]))
+    lim = np.asarray(spec.actuated_joint_limits)
+    half = 0.1 * (lim[:, 1] - lim[:, 0])
+    d = s.reshape(5, 64, 7) - init[:, None, :]
+    assert np.all(np.abs(d) <= half + 1e-6)
+    assert np.all(s >= lim32[:, 0]) and np.all(s <= lim32[:, 1])
+
+
+def test_ik_solver_rejects_bad_arguments_before_touching_the_device():
+    from robokit_amd import ik as IK
+    spec = W.panda_dh()
+    with pytest.raises(ValueError, match="out of range"):
+        IK.IKSolver(IK.IKSolverConfig(), spec, [spec.num_links + 3])
+    with pytest.raises(ValueError, match="at least one"):
+        IK.IKSolver(IK.IKSolverConfig(), spec, [])
+    with pytest.raises(NotImplementedError):
+        IK.IKSolverConfig(enable_T_world_base=True)
