@@ -136,6 +136,8 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
 #define RK_FROM_SEED 0x100 /* flag: start from the seeds of the last rk_batch_set (kept on the device) */
 #define RK_CONTINUE 0x200  /* flag: continue the last solve (current q and damping, iteration count and
                             * traces carry on) -- Optimizer.step after reset (opt/optimizer.py:280-287) */
+#define RK_NOWAIT 0x400    /* flag: return once the solve is enqueued on the context's stream (kernel_ms is
+                            * not written; rk_batch_get, rk_batch_select's rows or rk_context_sync wait) */
 /* threads_per_problem codes: 1 = one lane per problem, register-resident (serial arms);
  * RK_TPP_WAVE = one wavefront per problem, register-resident rows (trees, <= 64 joints, <= 32 dofs);
  * RK_TPP_HALF = two problems per wavefront, 32 lanes each, register-resident rows (trees,

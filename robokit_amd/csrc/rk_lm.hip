@@ -2368,7 +2368,8 @@ int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mo
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
   const bool from_seed = (mode & RK_FROM_SEED) != 0;
   const bool cont = (mode & RK_CONTINUE) != 0;
-  mode &= ~(RK_FROM_SEED | RK_CONTINUE);
+  const bool nowait = (mode & RK_NOWAIT) != 0;
+  mode &= ~(RK_FROM_SEED | RK_CONTINUE | RK_NOWAIT);
   if (cont && from_seed) return fail(RK_EINVAL, "RK_CONTINUE and RK_FROM_SEED exclude each other");
   const int it0 = cont ? b->it_done : 0;
   if (b->trace && it0 + iters > b->trace_iters) return fail(RK_EINVAL, "iterations exceed the batch's trace length");
@@ -2390,6 +2391,7 @@ int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mo
   }
   if (rc) return rc;
   b->it_done = it0 + iters;
+  if (nowait) return RK_OK;  // enqueued; a later synchronous call waits
   HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
   HIP_TRY(hipEventSynchronize(ctx->ev1));
   if (kernel_ms) HIP_TRY(hipEventElapsedTime(kernel_ms, ctx->ev0, ctx->ev1));

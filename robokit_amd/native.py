@@ -21,7 +21,7 @@ RK_OK, RK_EINVAL, RK_EHIP, RK_ECAPACITY, RK_ENOMEM, RK_EBUSY = range(6)
 MODE_FUSED, MODE_PLAIN = 0, 1
 TPP_WAVE = -64  # one wavefront per problem, register rows
 TPP_HALF = -32  # two problems per wavefront (32 lanes each), register rows
-FROM_SEED, CONTINUE, USE_GRAPH, ASYNC = 0x100, 0x200, 0x1, 0x2
+FROM_SEED, CONTINUE, USE_GRAPH, ASYNC, NOWAIT = 0x100, 0x200, 0x1, 0x2, 0x400
 SE3_OPS = {"compose": (0, 7), "inverse": (1, 7), "log": (2, 6), "jlog": (3, 36), "adjoint": (4, 36), "apply": (5, 3)}
 
 
@@ -340,12 +340,13 @@ class Batch:
             _check(lib().rk_batch_set_rotations(self._h, int(n), _p(rt)))
 
     def solve(self, iters: int, params: LMParams, mode: int = MODE_FUSED, tpp: int = 0, from_seed: bool = False,
-              cont: bool = False) -> float:
-        """Run ``iters`` LM iterations; ``cont`` continues the previous solve (q, damping, traces)."""
+              cont: bool = False, wait: bool = True) -> float:
+        """Run ``iters`` LM iterations; ``cont`` continues the previous solve (q, damping, traces).
+        ``wait=False`` only enqueues it (returns 0.0; a later synchronous call waits)."""
         ms = C.c_float()
-        m = int(mode) | (FROM_SEED if from_seed else 0) | (CONTINUE if cont else 0)
+        m = int(mode) | (FROM_SEED if from_seed else 0) | (CONTINUE if cont else 0) | (0 if wait else NOWAIT)
         _check(lib().rk_batch_solve(self._h, int(iters), C.byref(params.c()), m, int(tpp), C.byref(ms)))
-        return float(ms.value)
+        return float(ms.value) if wait else 0.0
 
     def get(self, traces: bool = False):
         na = self._na

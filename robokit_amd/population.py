@@ -170,7 +170,9 @@ class PopulationIK:
         self._ready = True
 
     def solve(self, tpp: int = 0) -> N.Batch:
-        """Run the schedule from the seeds; returns the width-1 batch of winners (get / metrics)."""
+        """Run the schedule from the seeds; returns the width-1 batch of winners (get / metrics).
+        The launches and selects are only enqueued on the context's stream: reading the winners
+        (get / metrics) waits for them."""
         if not self._ready:
             raise RuntimeError("set_targets first")
         cfg, widths = self.config, self.widths
@@ -182,7 +184,7 @@ class PopulationIK:
                 cur.select(w, nxt, widths[k])
                 cur, w = nxt, widths[k]
             if stage.iters > 0 or k == 0:  # a zero-iteration first stage scores the seeds
-                cur.solve(stage.iters, params[k], tpp=tpp, from_seed=(k == 0))
+                cur.solve(stage.iters, params[k], tpp=tpp, from_seed=(k == 0), wait=False)
         if w > 1:  # one winner per problem
             win = self._batches[1]
             cur.select(w, win, 1)

@@ -55,6 +55,11 @@ def test_select_gathers_the_oracle_ranking(ctx, panda):
     rows1 = b4.select(4, b1, 1, rows=True)
     np.testing.assert_array_equal(rows1, OP.select_best_k(s4["cost"], 4, 1).reshape(-1))
     np.testing.assert_array_equal(b1.get()["q"], s4["q"][rows1])
+    # the schedule enqueued without waits (PopulationIK.solve) gives the same winners
+    manual = b1.get()
+    win = pop.solve().get()
+    np.testing.assert_array_equal(win["q"], manual["q"])
+    np.testing.assert_array_equal(win["cost"], manual["cost"])
 
 
 def test_select_ties_keep_candidate_order(ctx, panda):
