@@ -47,13 +47,14 @@ def select_best_k(costs: np.ndarray, width_in: int, width_out: int) -> np.ndarra
 
 def population_ik(spec, links: Sequence[int], targets: np.ndarray, seeds: np.ndarray,
                   stages: List[Tuple[int, int, O.LMParams]], dtype: str = "f32"):
-    """targets [n_inst, n_pose, 7]; seeds [width0, na] shared by every problem; stages
-    [(iters, width, params)].  Returns (q [n_inst, na], cost [n_inst], rows of every select)."""
+    """targets [n_inst, n_pose, 7]; seeds [width0, na] shared by every problem, or
+    [n_inst * width0, na] per problem; stages [(iters, width, params)].  Returns (q [n_inst, na],
+    cost [n_inst], rows of every select)."""
     n_inst, na = targets.shape[0], seeds.shape[1]
     w = stages[0][1]
     inst = np.repeat(np.arange(n_inst), w)
     rt = np.float64 if dtype == "f64" else np.float32
-    q = np.tile(seeds, (n_inst, 1)).astype(rt)
+    q = (np.tile(seeds, (n_inst, 1)) if seeds.shape[0] == w else np.asarray(seeds)).astype(rt)
     cost = None
     selects = []
     for k, (iters, width, params) in enumerate(stages):

@@ -93,3 +93,29 @@ def test_candidates_around_an_initial_state(ctx, panda):
                     spec.actuated_joint_limits_f32[:, 0], spec.actuated_joint_limits_f32[:, 1])
     ok_init = s_mid.compute_metrics_numpy(s_mid.solve_numpy(wl.targets, init_q=noisy), wl.targets)[2]
     assert ok_init.sum() >= ok_mid.sum()
+
+
+def test_solver_humanoid_four_targets(ctx):
+    """a tree with four pose targets (the stages run the half-wave register kernels), candidates
+    around noisy initial states: the winners classify like the oracle population"""
+    spec = W.humanoid29(0)
+    links = W.humanoid_ee_links(spec)
+    wl = W.ik_batch(spec, links, 12, seed=43)
+    lim32 = spec.actuated_joint_limits_f32
+    init = np.clip(wl.q_target + np.random.default_rng(4).normal(0, 0.15, wl.q_target.shape).astype(np.float32),
+                   lim32[:, 0], lim32[:, 1])
+    solver = IK.IKSolver(IK.IKSolverConfig(), spec, links, ctx=ctx)
+    print(f"\n[ik] humanoid {wl.size} x {len(links)} targets", flush=True)
+    q = solver.solve_numpy(wl.targets, init_q=init, init_sample_range=0.1)
+    seeds = solver._pop.candidate_seeds(0.1, center=init)
+    stages = _stages(solver.config.solver)
+    q32, _, _ = OP.population_ik(spec, links, wl.targets, seeds, stages)
+    q64, _, _ = OP.population_ik(spec, links, wl.targets, seeds, stages, dtype="f64")
+    ok = O.ik_metrics(spec, q, links, wl.targets, pos_thr=0.005, ang_thr=0.05)[2]
+    ok32 = O.ik_metrics(spec, q32, links, wl.targets, pos_thr=0.005, ang_thr=0.05)[2]
+    ok64 = O.ik_metrics(spec, q64, links, wl.targets, pos_thr=0.005, ang_thr=0.05, dtype="f64")[2]
+    mism, n_ref = int(np.sum(ok != ok32)), int(np.sum(ok64 != ok32))
+    print(f"  converged (5 mm) gpu {ok.mean():.3f} f32 {ok32.mean():.3f}; differ gpu/f32 {mism}, f32/f64 {n_ref}")
+    assert mism <= max(n_ref, 1)
+    assert ok32.sum() > 0  # the comparison is not vacuous
+    assert np.sum(solver.compute_metrics_numpy(q, wl.targets)[2] != ok) <= 1
