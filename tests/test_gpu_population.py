@@ -120,3 +120,25 @@ def test_population_parity(ctx, panda):
     one.set(q=np.tile(pop.seeds[:1], (wl.size, 1)), pose_targets=wl.targets)
     one.solve(16, N.LMParams(lm_lambda=10.0), from_seed=True)
     assert ok.sum() >= one.metrics()[2].sum()
+
+
+def test_select_ranks_nan_costs_last(ctx, panda):
+    """candidates seeded with NaN joints end with NaN costs: the full 64 -> 64 ranking puts them
+    after every finite cost, in candidate order (NaN as +inf); a 1 -> 1 select is the identity"""
+    spec, wl = panda
+    terms = _terms(spec, wl.links)
+    a = N.Batch(ctx, spec, terms, 4 * 64)
+    c = N.Batch(ctx, spec, terms, 4 * 64)
+    lim = spec.actuated_joint_limits_f32
+    q = (lim[:, 0] + (lim[:, 1] - lim[:, 0]) * np.random.default_rng(9).random((4 * 64, 7))).astype(np.float32)
+    q.reshape(4, 64, 7)[:, [0, 5, 63]] = np.nan
+    a.set(q=q, pose_targets=np.repeat(wl.targets[:4], 64, axis=0))
+    a.solve(2, N.LMParams(lm_lambda=10.0), tpp=1, from_seed=True)
+    cost = a.get()["cost"]
+    assert np.isnan(cost.reshape(4, 64)[:, [0, 5, 63]]).all() and np.isfinite(cost.reshape(4, 64)[:, 1:5]).all()
+    rows = a.select(64, c, 64, rows=True)
+    np.testing.assert_array_equal(rows, OP.select_best_k(cost, 64, 64).reshape(-1))
+    assert (rows.reshape(4, 64)[:, -3:] % 64 == [0, 5, 63]).all()
+    ident = a.select(1, c, 1, rows=True)
+    np.testing.assert_array_equal(ident, np.arange(4 * 64))
+    np.testing.assert_array_equal(c.get()["q"], a.get()["q"])
