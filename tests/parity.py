@@ -77,7 +77,7 @@ def shadow_check(gpu_trace, gpu_acc, step32, step64, rtol: float = 1e-4, atol: f
     """Check every GPU step from the GPU's own iterate (q_i, lam_i).
 
     ``step32`` / ``step64`` are the FP32 oracle's and its FP64 twin's one-step results from the
-    GPU iterates, flattened over (problem, iteration) row-major.  The north-star quantity is the
+    GPU iterates, flattened over (problem, iteration) row-major.  The compared quantity is the
     residual norm |r| = sqrt(2 cost).  A GPU norm n_g passes against the FP64 value n_64 when
         |n_g - n_64| <= max(rtol * n_64 + atol, k_f32 * |n_32 - n_64|),
     i.e. within 1e-4 relative (plus the FP32 resolution atol of a pose residual) wherever the

@@ -1,6 +1,6 @@
 """GPU parity of the batched LM solve (fused and plain forms) against the CPU oracle.
 
-per-iteration residual norms within 1e-4 relative (tests/parity.py explains
+Parity bar: per-iteration residual norms within 1e-4 relative (tests/parity.py explains
 the comparison), identical converged/failed classification at 1 mm / 0.01 rad, FP32.
 Plus the invariants the reference tests hold: a problem's result does not depend on the batch
 it is in (tests/opt/test_varied_batch.py:138-151, exact), capacity overflow raises
