@@ -142,3 +142,26 @@ def test_select_ranks_nan_costs_last(ctx, panda):
     ident = a.select(1, c, 1, rows=True)
     np.testing.assert_array_equal(ident, np.arange(4 * 64))
     np.testing.assert_array_equal(c.get()["q"], a.get()["q"])
+
+
+def test_population_full_size_properties(ctx):
+    """the bench leg's size (2048 problems x 64 candidates): every kept row belongs to its own
+    problem, each winner's cost is the lowest of its four finalists (the same bits), and the
+    winners are finite (LM iterates are not clamped: the limits are a soft residual, as the
+    reference's clamp_q_on_integrate = False default, robo/robot_state.py:59)"""
+    spec = W.panda_dh()
+    wl = W.ik_batch(spec, [spec.num_links - 1], 2048, seed=7)
+    pop = P.PopulationIK(ctx, spec, _terms(spec, wl.links), wl.size)
+    pop.set_targets(pose_targets=wl.targets)
+    b64, b4, b1 = pop._batches[64], pop._batches[4], pop._batches[1]
+    print(f"\n[population] full size {wl.size} x 64", flush=True)
+    b64.solve(6, pop.config.optimizer[0], from_seed=True)
+    rows = b64.select(64, b4, 4, rows=True)
+    assert np.array_equal(rows // 64, np.repeat(np.arange(wl.size), 4))
+    b4.solve(10, pop.config.optimizer[1])
+    fin = b4.get()["cost"].reshape(-1, 4)
+    rows1 = b4.select(4, b1, 1, rows=True)
+    assert np.array_equal(rows1 // 4, np.arange(wl.size))
+    win = b1.get()
+    np.testing.assert_array_equal(win["cost"], np.nanmin(np.where(np.isnan(fin), np.inf, fin), axis=1))
+    assert np.all(np.isfinite(win["q"])) and np.all(np.isfinite(win["cost"]))
