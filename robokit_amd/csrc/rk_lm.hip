@@ -1338,7 +1338,7 @@ struct SelectArgs {
 // index), the order of _select_best_k_kernel (opt/population_solver.py:24-44): the keys are
 // distinct, so the ranks are a permutation and rank r < width_out names output row r.  The
 // selected rows (q, also written as the next stage's seeds, every target field and the cost) are then
-// copied field by field, lanes over the fields of one row (problem-major SoA, stride = capacity).
+// copied, lanes over (row, field) pairs (problem-major SoA, stride = capacity).
 __global__ __launch_bounds__(64) void k_select_gather(SelectArgs a) {
   __shared__ int pick[WAVE];
   const int b = blockIdx.x, j = threadIdx.x;
@@ -1351,21 +1351,32 @@ __global__ __launch_bounds__(64) void k_select_gather(SelectArgs a) {
   }
   if (j < a.width_in && rank < a.width_out) pick[rank] = j;
   __syncthreads();
-  for (int r = 0; r < a.width_out; ++r) {
-    const int si = b * a.width_in + pick[r], di = b * a.width_out + r;
-    if (j == 0) {
-      a.sel[di] = si;
-      a.cost_out[di] = a.cost[si];
-    }
-    for (int f = j; f < a.na; f += WAVE) {
-      const float v = a.q[(size_t)f * a.cap_in + si];
-      a.q_out[(size_t)f * a.cap_out + di] = v;
-      a.seed_out[(size_t)f * a.cap_out + di] = v;
-      a.qprev_out[(size_t)f * a.cap_out + di] = a.qprev[(size_t)f * a.cap_in + si];
-    }
-    for (int f = j; f < a.n_pose7; f += WAVE) a.pose_out[(size_t)f * a.cap_out + di] = a.pose[(size_t)f * a.cap_in + si];
-    for (int f = j; f < a.n_pos3; f += WAVE) a.pos_out[(size_t)f * a.cap_out + di] = a.pos[(size_t)f * a.cap_in + si];
-    for (int f = j; f < a.n_rot4; f += WAVE) a.rot_out[(size_t)f * a.cap_out + di] = a.rot[(size_t)f * a.cap_in + si];
+  // lanes over (kept row r, field f) pairs, so the loads of every kept row are in flight at once
+  const int src0 = b * a.width_in, dst0 = b * a.width_out;
+  for (int e = j; e < a.width_out; e += WAVE) {
+    const int si = src0 + pick[e];
+    a.sel[dst0 + e] = si;
+    a.cost_out[dst0 + e] = a.cost[si];
+  }
+  for (int e = j; e < a.width_out * a.na; e += WAVE) {
+    const int r = e / a.na, f = e - r * a.na;
+    const int si = src0 + pick[r], di = dst0 + r;
+    const float v = a.q[(size_t)f * a.cap_in + si];
+    a.q_out[(size_t)f * a.cap_out + di] = v;
+    a.seed_out[(size_t)f * a.cap_out + di] = v;
+    a.qprev_out[(size_t)f * a.cap_out + di] = a.qprev[(size_t)f * a.cap_in + si];
+  }
+  for (int e = j; e < a.width_out * a.n_pose7; e += WAVE) {
+    const int r = e / a.n_pose7, f = e - r * a.n_pose7;
+    a.pose_out[(size_t)f * a.cap_out + dst0 + r] = a.pose[(size_t)f * a.cap_in + src0 + pick[r]];
+  }
+  for (int e = j; e < a.width_out * a.n_pos3; e += WAVE) {
+    const int r = e / a.n_pos3, f = e - r * a.n_pos3;
+    a.pos_out[(size_t)f * a.cap_out + dst0 + r] = a.pos[(size_t)f * a.cap_in + src0 + pick[r]];
+  }
+  for (int e = j; e < a.width_out * a.n_rot4; e += WAVE) {
+    const int r = e / a.n_rot4, f = e - r * a.n_rot4;
+    a.rot_out[(size_t)f * a.cap_out + dst0 + r] = a.rot[(size_t)f * a.cap_in + src0 + pick[r]];
   }
 }
 
