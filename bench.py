@@ -93,56 +93,17 @@ def profiled_flops(kernel: str):
     return None
 
 
-RT_POS_W, RT_SMOOTH_W = 1.0, 0.1  # retargeting frame: PositionTask weight, SmoothnessTask weight
-
-
-def _serve_terms(N, trace):
-    """term structure per request kind: IK = FrameTask(log) + PositionLimit; retarget = 12
-    PositionTasks + PositionLimit + SmoothnessTask"""
-    out = {}
-    for kind in trace.kinds:
-        if kind == "trajopt":
-            out[kind] = (trace.pools[kind].spec, None)
-        elif kind == "retarget":
-            clip = trace.pools[kind][0]
-            na = clip.spec.num_actuated_joints
-            out[kind] = (clip.spec, N.TermSpec(pos_links=clip.links,
-                                               pos_weights=np.full((len(clip.links), 3), RT_POS_W, np.float32),
-                                               limit_weights=np.ones(na, np.float32),
-                                               smooth_weights=np.full(na, RT_SMOOTH_W, np.float32)))
-        else:
-            wl = trace.pools[kind]
-            na = wl.spec.num_actuated_joints
-            out[kind] = (wl.spec, N.TermSpec(pose_links=wl.links, pose_weights=np.ones((len(wl.links), 6), np.float32),
-                                             limit_weights=np.ones(na, np.float32)))
-    return out
-
-
-def _request_inputs(trace, i):
-    """(kind, n, q, pose_targets, pos_targets, q_prev) of request i"""
-    kind = trace.kinds[int(trace.kind[i])]
-    n, o = int(trace.n[i]), int(trace.off[i])
-    if kind == "trajopt":
-        tb = trace.pools[kind]
-        return kind, n, tb.q_seed[o:o + n], tb.goals[o:o + n, None], None, None
-    if kind == "retarget":
-        clip, prev = trace.pools[kind]
-        return kind, n, prev[o:o + n], None, clip.targets[o + 1:o + 1 + n], prev[o:o + n]
-    wl = trace.pools[kind]
-    return kind, n, wl.q_seed[o:o + n], wl.targets[o:o + n], None, None
-
-
 def run_served(N, ctx, trace, params, iters, budget, streams):
     """Replay the Poisson trace against the request server in real time; results collected by a
     second thread (ctypes releases the GIL while a poll waits)."""
     import queue
     import threading
 
+    from robokit_amd import serving
+
     sv = N.Server(ctx, budget, streams=streams, max_launch_problems=8192)
-    terms = _serve_terms(N, trace)
-    tid = {k: (sv.add_traj(trace.pools[k]) if k == "trajopt" else sv.add_terms(spec, ts))
-           for k, (spec, ts) in terms.items()}
-    inputs = [_request_inputs(trace, i) for i in range(trace.size)]
+    tid = serving.register(sv, trace)
+    inputs = [serving.request_inputs(trace, i) for i in range(trace.size)]
     tq: "queue.Queue" = queue.Queue()
     conv = {"ik": 0, "ik_problems": 0, "retarget_problems": 0, "traj": 0, "traj_problems": 0}
 
@@ -184,14 +145,15 @@ def run_served(N, ctx, trace, params, iters, budget, streams):
 def run_plain_per_request(N, ctx, trace, params, iters, seconds):
     """The same arrivals (first `seconds`), each request solved alone by the plain per-term launch
     as it arrives, one after another on the context stream."""
-    terms = _serve_terms(N, trace)
+    from robokit_amd import serving
+
     batches = {k: (N.TrajBatch(ctx, trace.pools[k], 256) if k == "trajopt" else N.Batch(ctx, spec, ts, 256))
-               for k, (spec, ts) in terms.items()}
+               for k, (spec, ts) in serving.term_specs(trace).items()}
     lat, conv_ik = [], 0
     m = int(np.searchsorted(trace.t, seconds))
     t0 = time.perf_counter()
     for i in range(m):
-        kind, n, q, pose, pos, qprev = _request_inputs(trace, i)
+        kind, n, q, pose, pos, qprev = serving.request_inputs(trace, i)
         wait = t0 + trace.t[i] - time.perf_counter()
         if wait > 0:
             time.sleep(wait)

@@ -215,9 +215,39 @@ def accept(cost, cost_prop, pred, lam, params: LMParams, dtype: str = "f32"):
     return cost, lam, acc
 
 
+def _slice_terms(terms: Sequence[Term], lo: int, hi: int) -> List[Term]:
+    """the terms of problems lo..hi-1 (per-problem targets and smoothness references sliced)"""
+    return [Term(t.kind, t.links, None if t.targets is None else np.asarray(t.targets)[lo:hi], t.weights,
+                 None if t.q_prev is None else np.asarray(t.q_prev)[lo:hi]) for t in terms]
+
+
+def _chunks(B: int, threads: int):
+    threads = max(1, min(int(threads), B))
+    cuts = np.linspace(0, B, threads + 1).round().astype(int)
+    return [(int(a), int(b)) for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
+
+
+def _parallel(B: int, threads: int, fn):
+    """run fn(lo, hi) over contiguous chunks of the batch on `threads` host threads (each chunk is
+    one C call, which releases the GIL) and concatenate the per-key results in order"""
+    parts = _chunks(B, threads)
+    if len(parts) == 1:
+        return fn(*parts[0])
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(len(parts)) as ex:
+        outs = list(ex.map(lambda c: fn(*c), parts))
+    return {k: (None if outs[0][k] is None else np.concatenate([o[k] for o in outs])) for k in outs[0]}
+
+
 def lm_solve(spec, terms: Sequence[Term], q_seed: np.ndarray, iters: int, params: LMParams, dtype: str = "f32",
-             traces: bool = True):
-    """Fixed-iteration LM over a batch; returns dict(q, cost, cost_trace[B, iters+1], accept[B, iters], lam[B, iters])."""
+             traces: bool = True, threads: int = 1):
+    """Fixed-iteration LM over a batch; returns dict(q, cost, cost_trace[B, iters+1], accept[B, iters], lam[B, iters]).
+    Problems are independent: ``threads`` > 1 solves contiguous chunks of the batch at once (the
+    same per-problem arithmetic, so the same bits)."""
+    if threads > 1:
+        return _parallel(len(q_seed), threads, lambda lo, hi: lm_solve(spec, _slice_terms(terms, lo, hi), q_seed[lo:hi],
+                                                                     iters, params, dtype, traces))
     lib, rt = _lib(dtype), _real(dtype)
     rb = OracleRobot(spec)
     keep: list = []
@@ -235,7 +265,7 @@ def lm_solve(spec, terms: Sequence[Term], q_seed: np.ndarray, iters: int, params
 
 
 def lm_step(spec, terms: Sequence[Term], q: np.ndarray, lam: np.ndarray, params: LMParams, dtype: str = "f32",
-            rows: Optional[np.ndarray] = None):
+            rows: Optional[np.ndarray] = None, threads: int = 1):
     """One LM step per problem from iterate q[B, na] with damping lam[B].  ``rows`` maps each
     entry to the problem index its targets belong to (default: entry i is problem i)."""
     lib, rt = _lib(dtype), _real(dtype)
@@ -245,6 +275,10 @@ def lm_step(spec, terms: Sequence[Term], q: np.ndarray, lam: np.ndarray, params:
     if rows is not None:
         terms = [Term(t.kind, t.links, None if t.targets is None else np.asarray(t.targets)[rows], t.weights,
                       None if t.q_prev is None else np.asarray(t.q_prev)[rows]) for t in terms]
+    if threads > 1:
+        lam_a = np.asarray(lam)
+        return _parallel(B, threads, lambda lo, hi: lm_step(spec, _slice_terms(terms, lo, hi), q[lo:hi], lam_a[lo:hi],
+                                                            params, dtype))
     keep: list = []
     ta = _terms(terms, keep)
     lam = np.ascontiguousarray(lam, rt)

@@ -1418,7 +1418,8 @@ struct rk_group {
   std::vector<rk_batch*> batches;
   std::vector<int> tpp;  // chosen threads per problem, 0 = default
   int wg_per_cu = 0, iters_per_launch = 0;
-  Bucket* buckets_dev = nullptr;
+  Bucket* buckets_dev = nullptr;    // bucket table of eager solves (rewritten by each one)
+  Bucket* buckets_graph = nullptr;  // bucket table the captured graph reads (written only at capture)
   int* counter = nullptr;  // per variant: tile queue and exit count (self-resetting)
   hipStream_t aux[NVARIANT] = {};
   hipEvent_t ev_fork = nullptr, ev_join[NVARIANT] = {};
@@ -2503,7 +2504,11 @@ int rk_group_create(rk_context* ctx, int32_t nb, rk_batch* const* batches, rk_gr
   g->tpp.assign(nb, 0);
   int rc;
   if ((rc = dmalloc(ctx, (void**)&g->buckets_dev, sizeof(Bucket) * nb, &g->bytes)) ||
+      (rc = dmalloc(ctx, (void**)&g->buckets_graph, sizeof(Bucket) * nb, &g->bytes)) ||
       (rc = dmalloc(ctx, (void**)&g->counter, sizeof(int) * 2 * NVARIANT, &g->bytes))) {
+    dfree(ctx, g->buckets_dev, sizeof(Bucket) * nb);
+    dfree(ctx, g->buckets_graph, sizeof(Bucket) * nb);
+    dfree(ctx, g->counter, sizeof(int) * 2 * NVARIANT);
     delete g;
     return rc;
   }
@@ -2511,6 +2516,7 @@ int rk_group_create(rk_context* ctx, int32_t nb, rk_batch* const* batches, rk_gr
   if (hipMemsetAsync(g->counter, 0, sizeof(int) * 2 * NVARIANT, ctx->stream) != hipSuccess ||
       hipStreamSynchronize(ctx->stream) != hipSuccess) {
     dfree(ctx, g->buckets_dev, sizeof(Bucket) * nb);
+    dfree(ctx, g->buckets_graph, sizeof(Bucket) * nb);
     dfree(ctx, g->counter, sizeof(int) * 2 * NVARIANT);
     delete g;
     return fail(RK_EHIP, "could not clear the group's queue counters");
@@ -2533,6 +2539,7 @@ int rk_group_destroy(rk_group* g) {
   }
   if (g->ev_fork) hipEventDestroy(g->ev_fork);
   hipFree(g->buckets_dev);
+  hipFree(g->buckets_graph);
   hipFree(g->counter);
   ctx->mem_bytes -= g->bytes;
   delete g;
@@ -2541,14 +2548,15 @@ int rk_group_destroy(rk_group* g) {
 
 int rk_group_set_tuning(rk_group* g, const int32_t* tpp, int32_t wg_per_cu, int32_t iters_per_launch) {
   if (!g) return fail(RK_EINVAL, "null group");
-  if (tpp)
+  if (wg_per_cu < 0 || iters_per_launch < 0) return fail(RK_EINVAL, "negative tuning value");
+  if (tpp) {  // every entry is checked before any is applied
     for (size_t i = 0; i < g->batches.size(); ++i) {
       int v;
       int rc = pick_variant(g->batches[i], tpp[i], &v);
       if (rc) return rc;
-      g->tpp[i] = tpp[i];
     }
-  if (wg_per_cu < 0 || iters_per_launch < 0) return fail(RK_EINVAL, "negative tuning value");
+    for (size_t i = 0; i < g->batches.size(); ++i) g->tpp[i] = tpp[i];
+  }
   g->wg_per_cu = wg_per_cu;
   g->iters_per_launch = iters_per_launch;
   if (g->graph) {
@@ -2561,6 +2569,7 @@ int rk_group_set_tuning(rk_group* g, const int32_t* tpp, int32_t wg_per_cu, int3
 }  // extern "C"
 
 struct GroupLaunch {
+  const Bucket* table = nullptr;  // device bucket table the launches read
   // per variant present: first bucket, bucket count, tiles, grid, dynamic LDS, batch mask
   std::vector<int> var, first, nb, tiles, grid;
   std::vector<unsigned> mask;
@@ -2622,8 +2631,11 @@ static void launch_persistent(int v, dim3 grid, size_t lds, hipStream_t st, cons
   }
 }
 
-// upload the bucket table (outside any capture) and size each variant's persistent grid
-static int group_prepare(rk_group* g, GroupLaunch* gl, bool from_seed) {
+// upload the bucket table into `table` (outside any capture) and size each variant's persistent
+// grid.  Eager solves upload into g->buckets_dev; a capture uploads into g->buckets_graph, which
+// only the next capture rewrites, so a later eager solve (other seeds flag, other sizes) cannot
+// change what a replay of the captured graph reads.
+static int group_prepare(rk_group* g, GroupLaunch* gl, bool from_seed, Bucket* table) {
   rk_context* ctx = g->ctx;
   std::vector<Bucket> bk(g->batches.size());
   for (size_t i = 0; i < g->batches.size(); ++i) {
@@ -2662,7 +2674,8 @@ static int group_prepare(rk_group* g, GroupLaunch* gl, bool from_seed) {
     gl->nb.back() += 1;
     gl->tiles.back() = base;
   }
-  HIP_TRY(hipMemcpyAsync(g->buckets_dev, sorted.data(), sizeof(Bucket) * sorted.size(), hipMemcpyHostToDevice,
+  gl->table = table;
+  HIP_TRY(hipMemcpyAsync(table, sorted.data(), sizeof(Bucket) * sorted.size(), hipMemcpyHostToDevice,
                          ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   for (size_t k = 0; k < gl->var.size(); ++k) {
@@ -2717,7 +2730,7 @@ static int group_launch(rk_group* g, const GroupLaunch& gl, int iters, const LMP
     }
     for (int it0 = 0; it0 < std::max(1, iters); it0 += chunk) {
       const int n_it = std::max(0, std::min(chunk, iters - it0));
-      hipLaunchKernelGGL(k_persistent_reg, dim3(blocks), dim3(WAVE), 0, ctx->stream, (const Bucket*)g->buckets_dev, ma,
+      hipLaunchKernelGGL(k_persistent_reg, dim3(blocks), dim3(WAVE), 0, ctx->stream, gl.table, ma,
                          g->counter, lp, it0, n_it);
       if (iters == 0) break;
     }
@@ -2731,7 +2744,7 @@ static int group_launch(rk_group* g, const GroupLaunch& gl, int iters, const LMP
     if (timed) HIP_TRY(hipEventRecord(g->ev_k0[k], st));
     for (int it0 = 0; it0 < std::max(1, iters); it0 += chunk) {
       const int n_it = std::max(0, std::min(chunk, iters - it0));
-      launch_persistent(gl.var[k], dim3(gl.grid[k]), gl.lds[k], st, (const Bucket*)g->buckets_dev + gl.first[k],
+      launch_persistent(gl.var[k], dim3(gl.grid[k]), gl.lds[k], st, gl.table + gl.first[k],
                         gl.nb[k], gl.tiles[k], g->counter + 2 * k, lp, it0, n_it);
       if (iters == 0) break;
     }
@@ -2772,13 +2785,17 @@ int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t fl
         g->graph = nullptr;
       }
       GroupLaunch gl;
-      if ((rc = group_prepare(g, &gl, from_seed))) return rc;
-      hipGraph_t graph;
+      if ((rc = group_prepare(g, &gl, from_seed, g->buckets_graph))) return rc;
+      hipGraph_t graph = nullptr;
       HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       rc = group_launch(g, gl, iters, lp, false);
+      const std::string launch_err = g_err;
       hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
-      if (rc) return rc;
-      if (e != hipSuccess) return fail(RK_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      if (rc || e != hipSuccess) {
+        if (e == hipSuccess && graph) hipGraphDestroy(graph);
+        if (rc) return fail(rc, launch_err);
+        return fail(RK_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      }
       e = hipGraphInstantiate(&g->graph, graph, nullptr, nullptr, 0);
       hipGraphDestroy(graph);
       if (e != hipSuccess) return fail(RK_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
@@ -2790,7 +2807,7 @@ int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t fl
     g->last_mask.clear();
   } else {
     GroupLaunch gl;
-    if ((rc = group_prepare(g, &gl, from_seed))) return rc;
+    if ((rc = group_prepare(g, &gl, from_seed, g->buckets_dev))) return rc;
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     if ((rc = group_launch(g, gl, iters, lp, true))) return rc;
     g->last_var = gl.var;
@@ -2818,6 +2835,68 @@ int rk_group_kernel_times(rk_group* g, int32_t cap, int32_t* tpp, float* ms, int
   return RK_OK;
 }
 
+// one batch of a group: time every legal threads-per-problem code on `iters` iterations from the
+// batch's current q (restored after each trial and at the end, traces detached meanwhile); on any
+// exit the batch's q, trace pointers and the scratch copy are restored / freed
+static int autotune_batch(rk_group* g, rk_batch* b, int iters, const LMParams& lp, int* best_out) {
+  rk_context* ctx = g->ctx;
+  const size_t S = b->capacity, qbytes = sizeof(float) * b->na * S;
+  float* q_save = nullptr;
+  int rc;
+  if ((rc = dmalloc(ctx, (void**)&q_save, qbytes, nullptr))) return rc;
+  float* const tr = b->trace;
+  int* const at = b->acc_trace;
+  float* const qt = b->q_trace;
+  float* const lt = b->lam_trace;
+  auto restore = [&](int code) {
+    if (hipMemcpyAsync(b->q, q_save, qbytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      if (code == RK_OK) code = fail(RK_EHIP, "rk_group_autotune: could not restore the batch's q");
+    }
+    dfree(ctx, q_save, qbytes);
+    b->trace = tr;
+    b->acc_trace = at;
+    b->q_trace = qt;
+    b->lam_trace = lt;
+    return code;
+  };
+  if (hipMemcpyAsync(q_save, b->q, qbytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
+    dfree(ctx, q_save, qbytes);
+    return fail(RK_EHIP, "rk_group_autotune: could not save the batch's q");
+  }
+  b->trace = nullptr;
+  b->acc_trace = nullptr;
+  b->q_trace = nullptr;
+  b->lam_trace = nullptr;
+  std::vector<int> cands;
+  legal_tpps(b, cands);
+  float best = 1e30f;
+  int best_g = cands[0];
+  for (int G : cands) {
+    int v;
+    if (pick_variant(b, G, &v)) continue;
+    float ms_sum = 0.f;
+    for (int rep = 0; rep < 2; ++rep) {
+      hipError_t e = hipMemcpyAsync(b->q, q_save, qbytes, hipMemcpyDeviceToDevice, ctx->stream);
+      if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+      if (e != hipSuccess) return restore(fail(RK_EHIP, std::string("rk_group_autotune: ") + hipGetErrorString(e)));
+      if ((rc = launch_variant(ctx->stream, v, make_problems(b), lp, 0, iters))) return restore(rc);
+      float ms = 0.f;
+      e = hipEventRecord(ctx->ev1, ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+      if (e != hipSuccess) return restore(fail(RK_EHIP, std::string("rk_group_autotune: ") + hipGetErrorString(e)));
+      if (rep == 1) ms_sum += ms;
+    }
+    if (ms_sum < best) {
+      best = ms_sum;
+      best_g = G;
+    }
+  }
+  *best_out = best_g;
+  return restore(RK_OK);
+}
+
 int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t* chosen) {
   if (!g || !p || iters <= 0) return fail(RK_EINVAL, "bad argument to rk_group_autotune");
   rk_context* ctx = g->ctx;
@@ -2827,49 +2906,9 @@ int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t
     rk_batch* b = g->batches[i];
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
-    const size_t S = b->capacity;
-    float* q_save = nullptr;
-    int rc;
-    if ((rc = dmalloc(ctx, (void**)&q_save, sizeof(float) * b->na * S, nullptr))) return rc;
-    HIP_TRY(hipMemcpyAsync(q_save, b->q, sizeof(float) * b->na * S, hipMemcpyDeviceToDevice, ctx->stream));
-    float* tr = b->trace;
-    int* at = b->acc_trace;
-    float* qt = b->q_trace;
-    float* lt = b->lam_trace;
-    b->trace = nullptr;
-    b->acc_trace = nullptr;
-    b->q_trace = nullptr;
-    b->lam_trace = nullptr;
-    std::vector<int> cands;
-    legal_tpps(b, cands);
-    float best = 1e30f;
-    int best_g = cands[0];
-    for (int G : cands) {
-      int v;
-      if (pick_variant(b, G, &v)) continue;
-      float ms_sum = 0.f;
-      for (int rep = 0; rep < 2; ++rep) {
-        HIP_TRY(hipMemcpyAsync(b->q, q_save, sizeof(float) * b->na * S, hipMemcpyDeviceToDevice, ctx->stream));
-        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        if ((rc = launch_variant(ctx->stream, v, make_problems(b), lp, 0, iters))) return rc;
-        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-        HIP_TRY(hipEventSynchronize(ctx->ev1));
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        if (rep == 1) ms_sum += ms;
-      }
-      if (ms_sum < best) {
-        best = ms_sum;
-        best_g = G;
-      }
-    }
-    HIP_TRY(hipMemcpyAsync(b->q, q_save, sizeof(float) * b->na * S, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    dfree(ctx, q_save, sizeof(float) * b->na * S);
-    b->trace = tr;
-    b->acc_trace = at;
-    b->q_trace = qt;
-    b->lam_trace = lt;
+    int best_g = 0;
+    const int rc = autotune_batch(g, b, iters, lp, &best_g);
+    if (rc) return rc;
     g->tpp[i] = best_g;
     ctx->tuned[signature(b->th)] = best_g;
     if (chosen) chosen[i] = best_g;

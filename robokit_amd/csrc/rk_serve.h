@@ -236,6 +236,7 @@ static int srv_start(rk_server* sv, int s) {
   for (SrvReq* r : L.reqs) {
     if (!r->session || r->has_q) continue;
     const SrvSession& ss = sv->sessions[r->session];
+    if (!ss.valid) return fail(RK_EINVAL, "warm start: the session holds no solution");
     const float* src = reinterpret_cast<const float*>(sv->arena_base + ss.off);
     const int tot = r->n * na, blk = 256;
     hipLaunchKernelGGL(k_srv_gather, dim3((tot + blk - 1) / blk), dim3(blk), 0, st, L.qs, n, r->first, src, r->n, na);
@@ -384,22 +385,37 @@ static bool srv_form(rk_server* sv, int s) {
   for (const auto& L : sv->launches)
     if (L.active && sv->terms[L.term].traj_na) ++traj_active;
   const bool traj_ok = traj_active < std::max(1, sv->nstreams / 4);
+  // Requests of one session run in submission order: once the scan passes over a request of a
+  // session (not ready, other key, no room), every later request of that session waits too.
+  std::set<int64_t> passed;
   SrvReq* head = nullptr;
-  for (SrvReq* r : sv->queue)
+  for (SrvReq* r : sv->queue) {
+    if (r->session && passed.count(r->session)) continue;
     if ((traj_ok || !sv->terms[r->term].traj_na) && srv_ready(sv, r)) {
       head = r;
       break;
     }
+    if (r->session) passed.insert(r->session);
+  }
   if (!head) return false;
+  // a warm start needs the session's resident solution: when the session's earlier request
+  // failed there is none, and the request fails instead of reading unwritten arena memory
+  if (head->session && !head->has_q && !sv->sessions[head->session].valid) {
+    srv_fail(sv, head, "warm start: the session holds no solution (its earlier request failed)");
+    sv->queue.erase(std::find(sv->queue.begin(), sv->queue.end(), head));
+    return true;
+  }
   const SrvTerms& T = sv->terms[head->term];
   std::vector<SrvReq*> pick;
-  std::set<int64_t> sess;
+  std::set<int64_t> sess;  // sessions with a request picked or passed over by this scan
   int n = 0;
   for (SrvReq* r : sv->queue) {
-    if (!srv_same_key(r, head) || n + r->n > sv->max_problems || !srv_fits_stage(sv, T, n + r->n)) continue;
-    if (r->session && (sess.count(r->session) || !srv_ready(sv, r))) continue;
-    pick.push_back(r);
+    if (r->session && sess.count(r->session)) continue;
+    const bool take = srv_same_key(r, head) && n + r->n <= sv->max_problems && srv_fits_stage(sv, T, n + r->n) &&
+                      (!r->session || (srv_ready(sv, r) && (r->has_q || sv->sessions[r->session].valid)));
     if (r->session) sess.insert(r->session);
+    if (!take) continue;
+    pick.push_back(r);
     n += r->n;
   }
   size_t off = 0, bytes = 0;

@@ -170,3 +170,150 @@ def test_trajectory_requests_equal_batch(ctx):
         assert np.array_equal(res[i]["cost"], ref["cost"][lo:hi])
         assert np.array_equal(res[i]["success"], ok[lo:hi])
     sv.close()
+
+
+def test_session_requests_run_in_submission_order(ctx):
+    """Requests of one session with different keys (iteration counts) run in submission order
+    even when a launch of another request's key is formed first: r1 (seeds, 10 iterations) ->
+    r2 (warm, 20) -> r3 (warm, 10) equal the batch API's advance chain, while a sessionless
+    request with r3's key was queued ahead of them."""
+    clip = W.config2(seed=9, frames=5)
+    spec, links = clip.spec, clip.links
+    na, S = spec.num_actuated_joints, 8
+    ts = N.TermSpec(pos_links=links, pos_weights=np.ones((len(links), 3), np.float32),
+                    limit_weights=np.ones(na, np.float32), smooth_weights=np.full(na, 0.1, np.float32))
+    q0 = np.repeat(clip.q_traj[:1], S, 0) + np.random.default_rng(1).normal(0, 0.02, (S, na)).astype(np.float32)
+    tg = [np.repeat(clip.targets[t][None], S, 0) for t in range(1, 4)]
+    tb = W.config3(seed=10, batch=64, T=64)
+    sv = N.Server(ctx, 256 << 20, streams=1)
+    tid = sv.add_terms(spec, ts)
+    ttid = sv.add_traj(tb)
+    # a trajectory launch (several ms) holds the only stream while the rest queue up behind it
+    tt = sv.submit(ttid, ITERS, P, q=tb.q_seed, pose_targets=tb.goals[:, None])
+    other = sv.submit(tid, 10, P, q=q0, q_prev=q0, pos_targets=tg[0])
+    r1 = sv.submit(tid, 20, P, q=q0, q_prev=q0, pos_targets=tg[0], session=7)
+    r2 = sv.submit(tid, 10, P, pos_targets=tg[1], session=7)
+    r3 = sv.submit(tid, 20, P, pos_targets=tg[2], session=7)
+    res = [sv.poll(t) for t in (tt, other, r1, r2, r3)]
+    assert sv.stats()["failed"] == 0
+    b = N.Batch(ctx, spec, ts, S)
+    b.set(q=q0, q_prev=q0, pos_targets=tg[0])
+    for k, it in enumerate((20, 10, 20)):
+        if k:
+            b.advance()
+            b.set(pos_targets=tg[k])
+        b.solve(it, P, from_seed=True)
+        assert np.array_equal(res[2 + k]["q"], b.get()["q"]), k
+    sv.end_session(7)
+    sv.close()
+
+
+def test_served_trace_matches_batch_and_oracle(ctx):
+    """Half a second of the mixed Poisson trace (2000 requests/s; IK 6/7/29-dof, retargeting
+    frames, T = 64 trajectories; 1-256 problems each) replayed in real time through the server:
+    every request's result equals the batch API's solve of the same rows, bitwise; per kind, a
+    sample of the requests is checked against the oracle (IK: every step shadow-checked and the
+    converged/failed classification; retargeting: every step shadow-checked; trajectories: every
+    step of three trajectories shadow-checked against the trajectory oracle)."""
+    import dataclasses
+    import time
+
+    from oracle import oracle as O
+    from robokit_amd import serving
+    from tests.parity import shadow_check
+
+    trace = W.config4(seed=11, rate=2000.0, seconds=0.5, pool=1024)
+    sv = N.Server(ctx, 1 << 30, streams=4, max_launch_problems=8192)
+    tid = serving.register(sv, trace)
+    reqs = [serving.request_inputs(trace, i) for i in range(trace.size)]
+    tickets = []
+    t0 = time.perf_counter()
+    for i, (kind, n, q, pose, pos, qprev) in enumerate(reqs):
+        while time.perf_counter() < t0 + trace.t[i]:
+            pass
+        tickets.append(sv.submit(tid[kind], ITERS, P, q=q, pose_targets=pose, pos_targets=pos, q_prev=qprev))
+    res = [sv.poll(t) for t in tickets]
+    st = sv.stats()
+    sv.close()
+    print(f"\nserved trace: {st['completed']} requests, {st['launches']} launches, p50 {st['p50_ms']:.3f} ms "
+          f"p99 {st['p99_ms']:.3f} ms")
+    assert st["failed"] == 0 and st["completed"] == trace.size
+    kinds = {k for k, *_ in reqs}
+    assert kinds == set(trace.kinds), kinds
+    # every request == the batch API's solve of its rows (the same kernel variant), bitwise
+    specs = serving.term_specs(trace)
+    batches = {k: (N.TrajBatch(ctx, trace.pools[k], 256, trace_iters=ITERS) if k == "trajopt"
+                   else N.Batch(ctx, spec, ts, 256, trace_iters=ITERS)) for k, (spec, ts) in specs.items()}
+    sample = {k: [] for k in specs}
+    for (kind, n, q, pose, pos, qprev), r in zip(reqs, res):
+        b = batches[kind]
+        if kind == "trajopt":
+            b.set(q=q, goals=pose[:, 0])
+        else:
+            b.set(q=q, pose_targets=pose, pos_targets=pos, q_prev=qprev)
+        b.solve(ITERS, P)
+        out = b.get(traces=True)
+        assert np.array_equal(out["q"].reshape(r["q"].shape), r["q"]), kind
+        assert np.array_equal(out["cost"], r["cost"]), kind
+        if kind in serving.IK_KINDS:
+            assert np.array_equal(b.metrics()[2], r["success"]), kind
+        # oracle sample: requests of the kind until 64 problems (rounding forks make the accept-flip
+        # count of a few problems noise), one request of trajectories
+        have = sum(it[0] for it in sample[kind])
+        if (kind == "trajopt" and not sample[kind]) or (kind != "trajopt" and have < 64):
+            q_it, lam_it = b.iterates()
+            sample[kind].append((n, q, pose, pos, qprev, out, q_it, lam_it, r))
+    p = O.LMParams(**P.__dict__)
+
+    def pooled(items):
+        """the sampled requests of one IK-shaped kind as one batch"""
+        cat = lambda xs: None if xs[0] is None else np.concatenate(xs)  # noqa: E731
+        cols = list(zip(*items))
+        out = {k: np.concatenate([o[k] for o in cols[5]]) for k in ("cost_trace", "accept")}
+        res = {"success": np.concatenate([x["success"] for x in cols[8]])}
+        return (sum(cols[0]), cat(cols[1]), cat(cols[2]), cat(cols[3]), cat(cols[4]), out, np.concatenate(cols[6]),
+                np.concatenate(cols[7]), res)
+
+    for kind, items in sample.items():
+        if kind != "trajopt":
+            items = [pooled(items)]
+        for n, q, pose, pos, qprev, out, q_it, lam_it, r in items:
+            if kind == "trajopt":
+                m = min(3, n)
+                sub = dataclasses.replace(trace.pools[kind], q_seed=q[:m], goals=pose[:m, 0])
+                ot = O.OracleTraj(sub)
+                rows = np.repeat(np.arange(m), ITERS)
+                qi = q_it[:m].reshape(m * ITERS, 64, -1)
+                rep = shadow_check(out["cost_trace"][:m], out["accept"][:m],
+                                   O.traj_lm_step(ot, qi, lam_it[:m].reshape(-1), p, rows=rows),
+                                   O.traj_lm_step(ot, qi, lam_it[:m].reshape(-1), p, dtype="f64", rows=rows))
+                print(f"  {kind} x {m}: {rep}")
+                assert rep.ok(), rep
+                continue
+            spec = specs[kind][0]
+            na = spec.num_actuated_joints
+            if kind == "retarget":
+                links = trace.pools[kind][0].links
+                terms = [O.Term(O.POSITION, links, pos, np.full(3 * len(links), serving.RT_POS_W, np.float32)),
+                         O.Term(O.LIMIT, (), None, np.full(na, serving.RT_LIMIT_W, np.float32)),
+                         O.Term(O.SMOOTH, (), None, np.full(na, serving.RT_SMOOTH_W, np.float32), qprev)]
+            else:
+                links = trace.pools[kind].links
+                terms = O.ik_terms(links, pose, na)
+            rows = np.repeat(np.arange(n), ITERS)
+            args = (spec, terms, q_it.reshape(-1, na), lam_it.reshape(-1), p)
+            rep = shadow_check(out["cost_trace"], out["accept"], O.lm_step(*args, rows=rows, threads=8),
+                               O.lm_step(*args, dtype="f64", rows=rows, threads=8))
+            msg = f"  {kind} x {n}: {rep}"
+            assert rep.ok(), rep
+            if kind in serving.IK_KINDS:
+                r32 = O.lm_solve(spec, terms, q, ITERS, p, traces=False)
+                r64 = O.lm_solve(spec, terms, q, ITERS, p, dtype="f64", traces=False)
+                ok32 = O.ik_metrics(spec, r32["q"], links, pose)[2]
+                ok64 = O.ik_metrics(spec, r64["q"], links, pose, dtype="f64")[2]
+                mism, n_ref = int(np.sum(r["success"] != ok32)), int(np.sum(ok64 != ok32))
+                msg += f"; converged gpu {int(r['success'].sum())} f32 {int(ok32.sum())}, differ {mism} (f32 vs f64 {n_ref})"
+                assert mism <= n_ref, (kind, mism, n_ref)
+            print(msg)
+    for b in batches.values():
+        b.close()

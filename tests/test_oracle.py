@@ -467,3 +467,20 @@ def test_rotation_task_rows():
         Jfd[:, a] = (O.residual_jacobian(spec, terms0, 0, qp, dtype="f64")[0]
                      - O.residual_jacobian(spec, terms0, 0, qm, dtype="f64")[0]) / (2 * h)
     np.testing.assert_allclose(J0, Jfd, atol=2e-5)
+
+
+def test_threaded_oracle_is_bitwise_the_serial_one():
+    """problems are independent: solving contiguous chunks of the batch on several host threads
+    (the all-core CPU baseline) gives the serial solve's bits, and so does the one-step form"""
+    wl = W.config0(seed=3, batch=96)
+    terms = O.ik_terms(wl.links, wl.targets, 7)
+    p = O.LMParams(lm_lambda=1e-2, lambda_factor=10.0)
+    a = O.lm_solve(wl.spec, terms, wl.q_seed, 12, p)
+    b = O.lm_solve(wl.spec, terms, wl.q_seed, 12, p, threads=5)
+    assert all(np.array_equal(a[k], b[k]) for k in a)
+    rows = np.repeat(np.arange(96), 3)
+    qi = np.repeat(wl.q_seed, 3, 0)
+    lam = np.geomspace(1e-3, 1e1, qi.shape[0]).astype(np.float32)
+    x = O.lm_step(wl.spec, terms, qi, lam, p, rows=rows)
+    y = O.lm_step(wl.spec, terms, qi, lam, p, rows=rows, threads=7)
+    assert all(np.array_equal(x[k], y[k]) for k in x)
