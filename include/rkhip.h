@@ -181,6 +181,15 @@ int rk_batch_get(rk_batch* b, float* q, float* cost, float* lambda, float* cost_
 int rk_batch_get_iterates(rk_batch* b, float* q_trace, float* lam_trace);
 /* _ik_metrics_kernel (helpers/ik/ik.py:59-82) over the batch's pose targets at its current q */
 int rk_batch_metrics(rk_batch* b, float pos_thr, float ang_thr, float* pos_err, float* ang_err, int32_t* success);
+/* No counterpart in the reference: for callers that cannot allocate device memory (tight serving
+ * loops, code around HIP graph capture).  The same results as rk_batch_metrics, bit for bit, but
+ * nothing is allocated: the per-problem results are staged in a static device array of the library
+ * (one per GPU, held until the process ends) for at most RK_NOALLOC_MAX_PROBLEMS problems (larger
+ * batches fail with RK_EINVAL).  Calls that overlap on one GPU (several threads or contexts on the
+ * same device at once) share that array and race; calls on one context are serialised. */
+#define RK_NOALLOC_MAX_PROBLEMS 65536
+int rk_batch_metrics_noalloc(rk_batch* b, float pos_thr, float ang_thr, float* pos_err, float* ang_err,
+                             int32_t* success);
 
 /*
  * Uneven batch: several batches (different robots / sizes / terms) solved by ONE persistent

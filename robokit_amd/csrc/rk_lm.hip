@@ -1318,6 +1318,39 @@ static int fail(int code, const std::string& msg) {
   return code;
 }
 
+// ---------------------------------------------------------------- allocation-free metrics staging
+// rk_batch_metrics_noalloc stages its per-problem results in this file-scope device array instead
+// of allocating them per call: position errors [RK_NOALLOC_MAX_PROBLEMS * RK_MAXT], angle errors
+// [same], success flags [RK_NOALLOC_MAX_PROBLEMS] (2.25 MiB, held by every device the library runs
+// on until the process ends).  In host code `scratch` is only a stand-in for the symbol, never a
+// device address: each device's copy is found with hipGetSymbolAddress when the first context is
+// created on that device (with the device current) and kept in g_scratch_dev, and only that
+// pointer is passed to kernels and copies.  The metrics kernel writes every entry it later reads,
+// so nothing needs clearing between calls.
+constexpr int RK_MAX_DEVICES = 64;
+constexpr size_t RK_NOALLOC_PAIRS = (size_t)RK_NOALLOC_MAX_PROBLEMS * RK_MAXT;
+static __device__ __align__(256) float scratch[2 * RK_NOALLOC_PAIRS + RK_NOALLOC_MAX_PROBLEMS];
+static std::mutex g_scratch_mu;
+static float* g_scratch_dev[RK_MAX_DEVICES] = {};
+
+// set-up of the current device's staging pointer (once per device; the device must be current)
+static int scratch_setup(int device) {
+  if (device < 0 || device >= RK_MAX_DEVICES) return fail(RK_EINVAL, "device ordinal beyond the library's table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return RK_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(RK_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_dev[device] = static_cast<float*>(p);
+  return RK_OK;
+}
+
+static float* scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device >= 0 && device < RK_MAX_DEVICES ? g_scratch_dev[device] : nullptr;
+}
+
 #define HIP_TRY(expr)                                                                      \
   do {                                                                                     \
     hipError_t e_ = (expr);                                                                \
@@ -1762,7 +1795,10 @@ int rk_context_create(int32_t device, rk_context** out) {
   HIP_TRY(hipEventCreate(&c->ev0));
   HIP_TRY(hipEventCreate(&c->ev1));
   int rc = dmalloc(c, (void**)&c->robots_dev, sizeof(Robot) * RK_MAX_ROBOTS, nullptr);
+  if (!rc) rc = scratch_setup(device);
   if (rc) {
+    if (c->robots_dev) dfree(c, c->robots_dev, sizeof(Robot) * RK_MAX_ROBOTS);
+    hipStreamDestroy(c->stream);
     delete c;
     return rc;
   }
@@ -2484,6 +2520,34 @@ int rk_batch_metrics(rk_batch* b, float pos_thr, float ang_thr, float* pos_err, 
   hipFree(dae);
   hipFree(dok);
   if (e != hipSuccess) return fail(RK_EHIP, std::string("rk_batch_metrics: ") + hipGetErrorString(e));
+  return RK_OK;
+}
+
+int rk_batch_metrics_noalloc(rk_batch* b, float pos_thr, float ang_thr, float* pos_err, float* ang_err,
+                             int32_t* success) {
+  if (!b || !pos_err || !ang_err || !success) return fail(RK_EINVAL, "null argument");
+  if (b->th.n_pose == 0) return fail(RK_EINVAL, "metrics need pose targets");
+  if (b->traj_T) return fail(RK_EINVAL, "trajectory batches report rk_traj_metrics");
+  const int B = b->n, k = b->th.n_pose;
+  if (B > RK_NOALLOC_MAX_PROBLEMS)
+    return fail(RK_EINVAL, "rk_batch_metrics_noalloc: " + std::to_string(B) + " problems exceed the static staging of " +
+                               std::to_string(RK_NOALLOC_MAX_PROBLEMS));
+  rk_context* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  float* const base = scratch_of(ctx->device);
+  if (!base) return fail(RK_EHIP, "rk_batch_metrics_noalloc: no staging address for this device");
+  float* const dpe = base;
+  float* const dae = base + RK_NOALLOC_PAIRS;
+  int* const dok = reinterpret_cast<int*>(base + 2 * RK_NOALLOC_PAIRS);
+  hipLaunchKernelGGL(k_ik_metrics, dim3((B + 127) / 128), dim3(128), 0, ctx->stream, make_problems(b), pos_thr,
+                     ang_thr, dpe, dae, dok);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(pos_err, dpe, sizeof(float) * B * k, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(ang_err, dae, sizeof(float) * B * k, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(success, dok, sizeof(int) * B, hipMemcpyDeviceToHost, ctx->stream));
+  const hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail(RK_EHIP, std::string("rk_batch_metrics_noalloc: ") + hipGetErrorString(e));
   return RK_OK;
 }
 

@@ -99,7 +99,8 @@ EXPORTS = [
     "rk_last_error", "rk_abi_version", "rk_context_create", "rk_context_destroy", "rk_context_sync",
     "rk_context_mem", "rk_context_mark", "rk_context_elapsed", "rk_robot_upload", "rk_fk", "rk_link_jacobian", "rk_copy_probe", "rk_se3_op",
     "rk_batch_create",
-    "rk_batch_destroy", "rk_batch_set", "rk_batch_set_rotations", "rk_batch_select", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics", "rk_group_create",
+    "rk_batch_destroy", "rk_batch_set", "rk_batch_set_rotations", "rk_batch_select", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics",
+    "rk_batch_metrics_noalloc", "rk_group_create",
     "rk_group_destroy", "rk_group_set_tuning", "rk_group_autotune", "rk_group_solve", "rk_group_kernel_times",
     "rk_server_create", "rk_server_add_terms", "rk_server_add_traj", "rk_server_submit", "rk_server_poll", "rk_server_end_session",
     "rk_server_stats", "rk_server_destroy", "rk_traj_batch_create", "rk_traj_metrics",
@@ -383,12 +384,16 @@ class Batch:
         _check(lib().rk_batch_get_iterates(self._h, _p(qt), _p(lt)))
         return qt, lt
 
-    def metrics(self, pos_thr: float = 1e-3, ang_thr: float = 1e-2):
+    def metrics(self, pos_thr: float = 1e-3, ang_thr: float = 1e-2, noalloc: bool = False):
+        """(pos_err [B, k], ang_err [B, k], success [B]) at the current q.  ``noalloc`` stages the
+        device results in the library's static array (rk_batch_metrics_noalloc: no device
+        allocation, at most RK_NOALLOC_MAX_PROBLEMS problems) instead of per-call buffers."""
         k = len(self.terms.pose_links)
         pe = np.zeros((self.n, k), np.float32)
         ae = np.zeros((self.n, k), np.float32)
         ok = np.zeros(self.n, np.int32)
-        _check(lib().rk_batch_metrics(self._h, C.c_float(pos_thr), C.c_float(ang_thr), _p(pe), _p(ae), _p(ok)))
+        fn = lib().rk_batch_metrics_noalloc if noalloc else lib().rk_batch_metrics
+        _check(fn(self._h, C.c_float(pos_thr), C.c_float(ang_thr), _p(pe), _p(ae), _p(ok)))
         return pe, ae, ok.astype(bool)
 
 
