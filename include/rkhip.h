@@ -77,6 +77,15 @@ int rk_fk(rk_context* ctx, int32_t robot, int32_t B, const float* q, float* T_li
 int rk_link_jacobian(rk_context* ctx, int32_t robot, int32_t B, const float* q, int32_t link, int32_t frame,
                      float* J);
 
+/* the same with the robot's base at T_world_base [B][7] (xyz-wxyz; NULL = identity):
+ * Robot.forward_kinematics(q, T_world_base) (robo/robot.py:131-271, accumulating FK
+ * robot_kernels.py:327-377) -> T_link [B][nl][7]; and the floating-base link Jacobian
+ * (robot_kernels.py:667-737, has_floating_base) -> J [B][6][6 + na], the six base columns first:
+ * Ad(T_world_base) (spatial) or Ad(T_link^-1 T_world_base) (body), then the joint columns */
+int rk_fk_base(rk_context* ctx, int32_t robot, int32_t B, const float* q, const float* T_world_base, float* T_link);
+int rk_link_jacobian_base(rk_context* ctx, int32_t robot, int32_t B, const float* q, const float* T_world_base,
+                          int32_t link, int32_t frame, float* J);
+
 /* measurement utility (no reference counterpart): two launches of a streaming copy of `bytes`
  * (read once, written once), the second timed; calibrates the FETCH_SIZE / WRITE_SIZE counters */
 int rk_copy_probe(rk_context* ctx, int64_t bytes, float* kernel_ms);
@@ -93,7 +102,9 @@ int rk_se3_op(rk_context* ctx, int32_t op, int32_t n, const float* a, const floa
  *   PositionLimit "abs"  (terms/dense/position_limit.py:508-573)  na rows
  *   SmoothnessTask       (terms/dense/smoothness_task.py:350-422) na rows
  * Limits: n_pose <= 4, n_pos <= 12, n_pose + n_rot <= 4.  Batches with rotation targets run the
- * LDS-staged fused kernels (threads_per_problem 4..64).
+ * LDS-staged fused kernels (threads_per_problem 4..64), or, for a serial arm with one position and
+ * one rotation target on links of one parent joint (the IK presets' term pair), the one-lane-per-
+ * problem register kernel (threads_per_problem 1).
  */
 typedef struct {
     int32_t robot;

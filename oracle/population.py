@@ -46,10 +46,14 @@ def select_best_k(costs: np.ndarray, width_in: int, width_out: int) -> np.ndarra
 
 
 def population_ik(spec, links: Sequence[int], targets: np.ndarray, seeds: np.ndarray,
-                  stages: List[Tuple[int, int, O.LMParams]], dtype: str = "f32"):
+                  stages: List[Tuple[int, int, O.LMParams]], dtype: str = "f32", terms_fn=None, threads: int = 1):
     """targets [n_inst, n_pose, 7]; seeds [width0, na] shared by every problem, or
-    [n_inst * width0, na] per problem; stages [(iters, width, params)].  Returns (q [n_inst, na],
-    cost [n_inst], rows of every select)."""
+    [n_inst * width0, na] per problem; stages [(iters, width, params)].  terms_fn(inst) gives the
+    oracle terms of the candidate rows whose problems are ``inst`` (default: FrameTask on every
+    link + PositionLimit, oracle.ik_terms).  Returns (q [n_inst, na], cost [n_inst], rows of every
+    select)."""
+    if terms_fn is None:
+        terms_fn = lambda inst: O.ik_terms(links, targets[inst], seeds.shape[1])  # noqa: E731
     n_inst, na = targets.shape[0], seeds.shape[1]
     w = stages[0][1]
     inst = np.repeat(np.arange(n_inst), w)
@@ -63,8 +67,7 @@ def population_ik(spec, links: Sequence[int], targets: np.ndarray, seeds: np.nda
             selects.append(rows)
             q, cost, inst, w = q[rows], cost[rows], inst[rows], width
         if iters > 0 or k == 0:
-            terms = O.ik_terms(links, targets[inst], na)
-            res = O.lm_solve(spec, terms, q, iters, params, dtype=dtype, traces=False)
+            res = O.lm_solve(spec, terms_fn(inst), q, iters, params, dtype=dtype, traces=False, threads=threads)
             q, cost = res["q"].astype(rt), res["cost"].astype(rt)
     if w > 1:
         rows = select_best_k(cost, w, 1).reshape(-1)

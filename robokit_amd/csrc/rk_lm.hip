@@ -1005,6 +1005,38 @@ __global__ void k_plain_pos_jac(Problems P, const float* Tj, const float* Sw, fl
   }
 }
 
+// RotationTask rows r = w axis_angle(q_link q_tgt^-1) (rotation_task.py:323-349), thread per
+// (problem, target)
+__global__ void k_plain_rot_res(Problems P, const float* Tj, float* r) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int u = blockIdx.y;
+  if (b >= P.n) return;
+  const Terms* tm = P.terms;
+  const int S = P.stride;
+  const Pose Tf = plain_pose(Tj, P.robot->link_joint[tm->rot_link[u]], S, b);
+  float qt[4], aa[3];
+  for (int c = 0; c < 4; ++c) qt[c] = P.rot_tgt[(u * 4 + c) * S + b];
+  rotation_target(Tf, qt, aa);
+  for (int i = 0; i < 3; ++i) r[(tm->Rf + tm->Rq + 3 * u + i) * S + b] = tm->rot_w[u][i] * aa[i];
+}
+
+// RotationTask Jacobian: w * the angular part of the link's spatial column (rotation_task.py:352-401),
+// thread per (problem, dof)
+__global__ void k_plain_rot_jac(Problems P, const float* Sw, float* J) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y;
+  if (b >= P.n) return;
+  const Robot* rb = P.robot;
+  const Terms* tm = P.terms;
+  const int S = P.stride, na = rb->na;
+  auto S_at = [&](int j, int r) { return Sw[(j * 6 + r) * S + b]; };
+  for (int u = 0; u < tm->n_rot; ++u) {
+    float sc[6];
+    spatial_column(rb, tm->rot_link[u], a, S_at, sc);
+    for (int i = 0; i < 3; ++i) J[((tm->Rf + tm->Rq + 3 * u + i) * na + a) * S + b] = tm->rot_w[u][i] * sc[3 + i];
+  }
+}
+
 // limit + smoothness rows (and their diagonal Jacobian), thread per (problem, dof)
 __global__ void k_plain_diag_rows(Problems P, const float* qsrc, float* r, float* J) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1014,7 +1046,7 @@ __global__ void k_plain_diag_rows(Problems P, const float* qsrc, float* r, float
   const Terms* tm = P.terms;
   const int S = P.stride, na = rb->na;
   const float x = qsrc[d * S + b];
-  int off = tm->Rf + tm->Rq;
+  int off = tm->Rf + tm->Rq + tm->Rr;
   if (tm->limit) {
     r[(off + d) * S + b] = limit_row(x, rb->lo[d], rb->hi[d], tm->limit_w[d]);
     if (J) J[((off + d) * na + d) * S + b] = limit_diag(x, rb->lo[d], rb->hi[d], tm->limit_w[d]);
@@ -1040,7 +1072,7 @@ __global__ __launch_bounds__(64) void k_plain_solve(Problems P, PlainBufs B) {
   const int idx = blockIdx.x * (WAVE / G) + slot;
   const bool valid = idx < P.n;
   const int s = valid ? idx : P.n - 1;
-  const int Rp = tm->Rf + tm->Rq;
+  const int Rp = tm->Rf + tm->Rq + tm->Rr;
   const int R = tm->R;
   for (int k = lane; k < Rp * na; k += G) sh->J[k / na][k % na] = B.J[k * S + s];
   for (int k = lane; k < R; k += G) sh->r[k] = B.r[k * S + s];
@@ -1088,8 +1120,8 @@ __global__ void k_plain_cost(Problems P, const float* __restrict__ r) {
   const Terms* tm = P.terms;
   float c = 0.f;
   int off = 0;
-  const int sizes[4] = {tm->Rf, tm->Rq, tm->limit ? na : 0, tm->smooth ? na : 0};
-  for (int t = 0; t < 4; ++t) {
+  const int sizes[5] = {tm->Rf, tm->Rq, tm->Rr, tm->limit ? na : 0, tm->smooth ? na : 0};
+  for (int t = 0; t < 5; ++t) {
     if (!sizes[t]) continue;
     float s = 0.f;
     for (int k = 0; k < sizes[t]; ++k) {
@@ -1110,8 +1142,8 @@ __global__ void k_plain_accept(Problems P, PlainBufs B, LMParams lp, int it) {
   // proposed cost: per-term sums of the proposal's rows (strided layout)
   float cp = 0.f;
   int off = 0;
-  const int sizes[4] = {tm->Rf, tm->Rq, tm->limit ? na : 0, tm->smooth ? na : 0};
-  for (int t = 0; t < 4; ++t) {
+  const int sizes[5] = {tm->Rf, tm->Rq, tm->Rr, tm->limit ? na : 0, tm->smooth ? na : 0};
+  for (int t = 0; t < 5; ++t) {
     if (!sizes[t]) continue;
     float s = 0.f;
     for (int k = 0; k < sizes[t]; ++k) {
@@ -1162,38 +1194,70 @@ __device__ void fk_thread(const Robot* rb, const float* q, Pose* Tw) {
   }
 }
 
-__global__ void k_fk_host_layout(const Robot* rb, int B, const float* q, float* T_link) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  Pose Tw[RK_NJ_MAX];
-  fk_thread(rb, q + (size_t)b * rb->na, Tw);
-  for (int l = 0; l < rb->nl; ++l) {
-    const int j = rb->link_joint[l];
-    Pose P;
-    if (j >= 0) P = Tw[j];
-    else {
-      P.t[0] = P.t[1] = P.t[2] = 0.f;
-      P.q[0] = 1.f; P.q[1] = P.q[2] = P.q[3] = 0.f;
-    }
-    store_pose(T_link + ((size_t)b * rb->nl + l) * 7, P);
+// column k of Ad(T) = [[R, t^ R], [0, R]] (se3_kernels.py:323-346): the twist e_k seen through T
+__device__ __forceinline__ void adjoint_column(const Pose& T, int k, float* out) {
+  const Mat3 R = quat_to_mat(T.q);
+  float v[3] = {0.f, 0.f, 0.f}, w[3] = {0.f, 0.f, 0.f}, Rv[3], Rw[3], txRw[3];
+  if (k < 3) v[k] = 1.f;
+  else w[k - 3] = 1.f;
+  mat3_vec(R, v, Rv);
+  mat3_vec(R, w, Rw);
+  cross3(T.t, Rw, txRw);
+  for (int i = 0; i < 3; ++i) {
+    out[i] = Rv[i] + txRw[i];
+    out[3 + i] = Rw[i];
   }
 }
 
-__global__ void k_link_jacobian(const Robot* rb, int B, const float* q, int link, int frame, float* J) {
+// world poses of every joint of problem b with the robot's base at T_world_base (null: the
+// identity): the chain is composed from the root, then the base on the left, as the accumulating
+// FK composes each link's chain and then T_world_base (robot_kernels.py:327-377)
+__device__ __forceinline__ Pose fk_with_base(const Robot* rb, const float* q, const float* base, Pose* Tw) {
+  fk_thread(rb, q, Tw);
+  Pose Bp;
+  if (base) {
+    load_pose(base, Bp);
+    for (int j = 0; j < rb->nj; ++j) Tw[j] = compose(Bp, Tw[j]);
+  } else {
+    Bp.t[0] = Bp.t[1] = Bp.t[2] = 0.f;
+    Bp.q[0] = 1.f; Bp.q[1] = Bp.q[2] = Bp.q[3] = 0.f;
+  }
+  return Bp;
+}
+
+// T_link [B][nl][7]; base [B][7] or null (robot.py:131-271: a link without a parent joint sits at
+// the base)
+__global__ void k_fk_host_layout(const Robot* rb, int B, const float* q, const float* base, float* T_link) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   Pose Tw[RK_NJ_MAX];
-  fk_thread(rb, q + (size_t)b * rb->na, Tw);
-  const int na = rb->na;
-  const int jl = rb->link_joint[link];
-  Pose Tf;
-  if (jl >= 0) Tf = Tw[jl];
-  else {
-    Tf.t[0] = Tf.t[1] = Tf.t[2] = 0.f;
-    Tf.q[0] = 1.f; Tf.q[1] = Tf.q[2] = Tf.q[3] = 0.f;
+  const Pose Bp = fk_with_base(rb, q + (size_t)b * rb->na, base ? base + (size_t)b * 7 : nullptr, Tw);
+  for (int l = 0; l < rb->nl; ++l) {
+    const int j = rb->link_joint[l];
+    store_pose(T_link + ((size_t)b * rb->nl + l) * 7, j >= 0 ? Tw[j] : Bp);
   }
-  Pose Tfi = inverse(Tf);
-  Mat3 R = quat_to_mat(Tfi.q);
+}
+
+// J [B][6][fb + na], fb = 6 floating-base columns first when base is given (robot_kernels.py:667-737):
+// base column k is Ad(T_world_base) e_k (spatial) or Ad(T_link^-1 T_world_base) e_k (body); joint
+// column a sums the world motion-subspace columns of the link's ancestor joints driving dof a
+// (spatial), seen from the link frame through Ad(T_link^-1) (body)
+__global__ void k_link_jacobian(const Robot* rb, int B, const float* q, const float* base, int link, int frame,
+                                float* J) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  Pose Tw[RK_NJ_MAX];
+  const Pose Bp = fk_with_base(rb, q + (size_t)b * rb->na, base ? base + (size_t)b * 7 : nullptr, Tw);
+  const int na = rb->na, fb = base ? 6 : 0, ncol = fb + na;
+  const int jl = rb->link_joint[link];
+  const Pose Tf = jl >= 0 ? Tw[jl] : Bp;
+  const Pose Tfi = inverse(Tf);
+  const Mat3 R = quat_to_mat(Tfi.q);
+  for (int k = 0; k < fb; ++k) {
+    float col[6];
+    adjoint_column(frame == 1 ? compose(Tfi, Bp) : Bp, k, col);
+    for (int i = 0; i < 6; ++i) J[((size_t)b * 6 + i) * ncol + k] = col[i];
+  }
   auto S_at = [&](int j, int r) {
     float s6[6];
     subspace_column(rb, j, Tw[j], s6);
@@ -1214,9 +1278,10 @@ __global__ void k_link_jacobian(const Robot* rb, int B, const float* q, int link
     } else {
       for (int i = 0; i < 6; ++i) col[i] = sc[i];
     }
-    for (int i = 0; i < 6; ++i) J[((size_t)b * 6 + i) * na + a] = col[i];
+    for (int i = 0; i < 6; ++i) J[((size_t)b * 6 + i) * ncol + fb + a] = col[i];
   }
 }
+
 
 // plain streaming copy (16 B per lane per step): a kernel of exactly known HBM traffic, used to
 // calibrate the FETCH_SIZE / WRITE_SIZE counters
@@ -1507,10 +1572,13 @@ static bool fits_small(const Terms& th, const Robot& r) {
   return r.nj <= CfgS::nj && r.na <= CfgS::na && th.n_pose + th.n_rot <= CfgS::np && th.n_pos <= CfgS::nq;
 }
 
-// register thread-per-problem kernels: serial arms with 5..8 dofs, <= 2 fixed tail joints,
-// <= 1 pose target, no position targets
+// register thread-per-problem kernels: serial arms with 5..8 dofs, <= 2 fixed tail joints; one
+// pose target, or <= 1 position and <= 1 rotation target whose links share a parent joint
 static bool fits_thread(const Terms& th, const Robot& r) {
-  return r.chain && r.na >= 5 && r.na <= 8 && r.nj <= r.na + 2 && th.n_pose <= 1 && th.n_pos == 0 && th.n_rot == 0;
+  if (!(r.chain && r.na >= 5 && r.na <= 8 && r.nj <= r.na + 2)) return false;
+  if (th.n_pose) return th.n_pose == 1 && th.n_pos == 0 && th.n_rot == 0;
+  if (th.n_pos > 1 || th.n_rot > 1) return false;
+  return !(th.n_pos && th.n_rot && r.link_joint[th.pos_link[0]] != r.link_joint[th.rot_link[0]]);
 }
 
 // register wave kernels: trees with <= 32 dofs, each dof driven by <= 2 joints; <= 4 pose
@@ -1703,6 +1771,9 @@ static int eval_cost(rk_batch* b, bool from_seed) {
   if (b->th.n_pos)
     hipLaunchKernelGGL(k_plain_pos_res, dim3(g1.x, b->th.n_pos), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
                        b->pb.r);
+  if (b->th.n_rot)
+    hipLaunchKernelGGL(k_plain_rot_res, dim3(g1.x, b->th.n_rot), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
+                       b->pb.r);
   if (b->th.limit || b->th.smooth)
     hipLaunchKernelGGL(k_plain_diag_rows, dim3(g1.x, na), blk, 0, ctx->stream, P, (const float*)b->q, b->pb.r,
                        (float*)nullptr);
@@ -1742,6 +1813,11 @@ static int solve_plain(rk_batch* b, int it0, int iters, const LMParams& lp, bool
       hipLaunchKernelGGL(k_plain_pos_jac, dim3(g1.x, na), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
                          (const float*)b->pb.Sw, b->pb.J);
     }
+    if (b->th.n_rot) {
+      hipLaunchKernelGGL(k_plain_rot_res, dim3(g1.x, b->th.n_rot), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
+                         b->pb.r);
+      hipLaunchKernelGGL(k_plain_rot_jac, dim3(g1.x, na), blk, 0, ctx->stream, P, (const float*)b->pb.Sw, b->pb.J);
+    }
     if (b->th.limit || b->th.smooth)
       hipLaunchKernelGGL(k_plain_diag_rows, dim3(g1.x, na), blk, 0, ctx->stream, P, (const float*)b->q, b->pb.r,
                          b->pb.J);
@@ -1757,6 +1833,9 @@ static int solve_plain(rk_batch* b, int it0, int iters, const LMParams& lp, bool
                          b->pb.rp);
     if (b->th.n_pos)
       hipLaunchKernelGGL(k_plain_pos_res, dim3(g1.x, b->th.n_pos), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
+                         b->pb.rp);
+    if (b->th.n_rot)
+      hipLaunchKernelGGL(k_plain_rot_res, dim3(g1.x, b->th.n_rot), blk, 0, ctx->stream, P, (const float*)b->pb.Tj,
                          b->pb.rp);
     if (b->th.limit || b->th.smooth)
       hipLaunchKernelGGL(k_plain_diag_rows, dim3(g1.x, na), blk, 0, ctx->stream, P, (const float*)b->pb.qp,
@@ -1961,29 +2040,37 @@ static int check_robot(rk_context* ctx, int32_t robot) {
   return RK_OK;
 }
 
-int rk_fk(rk_context* ctx, int32_t robot, int32_t B, const float* q, float* T_link) {
+int rk_fk_base(rk_context* ctx, int32_t robot, int32_t B, const float* q, const float* T_world_base, float* T_link) {
   int rc;
   if ((rc = check_robot(ctx, robot))) return rc;
   if (B <= 0 || !q || !T_link) return fail(RK_EINVAL, "bad batch or null array");
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device));
   const Robot& r = ctx->robots[robot];
-  float *dq = nullptr, *dT = nullptr;
-  HIP_TRY(hipMalloc(&dq, sizeof(float) * B * r.na));
-  HIP_TRY(hipMalloc(&dT, sizeof(float) * B * r.nl * 7));
-  hipMemcpyAsync(dq, q, sizeof(float) * B * r.na, hipMemcpyHostToDevice, ctx->stream);
+  const size_t qb = sizeof(float) * B * r.na, bb = T_world_base ? sizeof(float) * B * 7 : 0,
+               tb = sizeof(float) * B * r.nl * 7;
+  char* d = nullptr;
+  HIP_TRY(hipMalloc(&d, qb + bb + tb));
+  float* dq = reinterpret_cast<float*>(d);
+  float* dbase = T_world_base ? reinterpret_cast<float*>(d + qb) : nullptr;
+  float* dT = reinterpret_cast<float*>(d + qb + bb);
+  hipMemcpyAsync(dq, q, qb, hipMemcpyHostToDevice, ctx->stream);
+  if (dbase) hipMemcpyAsync(dbase, T_world_base, bb, hipMemcpyHostToDevice, ctx->stream);
   hipLaunchKernelGGL(k_fk_host_layout, dim3((B + 127) / 128), dim3(128), 0, ctx->stream, ctx->robots_dev + robot, B,
-                     (const float*)dq, dT);
-  hipMemcpyAsync(T_link, dT, sizeof(float) * B * r.nl * 7, hipMemcpyDeviceToHost, ctx->stream);
+                     (const float*)dq, (const float*)dbase, dT);
+  hipMemcpyAsync(T_link, dT, tb, hipMemcpyDeviceToHost, ctx->stream);
   hipError_t e = hipStreamSynchronize(ctx->stream);
-  hipFree(dq);
-  hipFree(dT);
+  hipFree(d);
   if (e != hipSuccess) return fail(RK_EHIP, std::string("rk_fk: ") + hipGetErrorString(e));
   return RK_OK;
 }
 
-int rk_link_jacobian(rk_context* ctx, int32_t robot, int32_t B, const float* q, int32_t link, int32_t frame,
-                     float* J) {
+int rk_fk(rk_context* ctx, int32_t robot, int32_t B, const float* q, float* T_link) {
+  return rk_fk_base(ctx, robot, B, q, nullptr, T_link);
+}
+
+int rk_link_jacobian_base(rk_context* ctx, int32_t robot, int32_t B, const float* q, const float* T_world_base,
+                          int32_t link, int32_t frame, float* J) {
   int rc;
   if ((rc = check_robot(ctx, robot))) return rc;
   const Robot& r = ctx->robots[robot];
@@ -1991,18 +2078,28 @@ int rk_link_jacobian(rk_context* ctx, int32_t robot, int32_t B, const float* q, 
     return fail(RK_EINVAL, "bad argument to rk_link_jacobian");
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device));
-  float *dq = nullptr, *dJ = nullptr;
-  HIP_TRY(hipMalloc(&dq, sizeof(float) * B * r.na));
-  HIP_TRY(hipMalloc(&dJ, sizeof(float) * B * 6 * r.na));
-  hipMemcpyAsync(dq, q, sizeof(float) * B * r.na, hipMemcpyHostToDevice, ctx->stream);
+  const int ncol = (T_world_base ? 6 : 0) + r.na;
+  const size_t qb = sizeof(float) * B * r.na, bb = T_world_base ? sizeof(float) * B * 7 : 0,
+               jb = sizeof(float) * B * 6 * ncol;
+  char* d = nullptr;
+  HIP_TRY(hipMalloc(&d, qb + bb + jb));
+  float* dq = reinterpret_cast<float*>(d);
+  float* dbase = T_world_base ? reinterpret_cast<float*>(d + qb) : nullptr;
+  float* dJ = reinterpret_cast<float*>(d + qb + bb);
+  hipMemcpyAsync(dq, q, qb, hipMemcpyHostToDevice, ctx->stream);
+  if (dbase) hipMemcpyAsync(dbase, T_world_base, bb, hipMemcpyHostToDevice, ctx->stream);
   hipLaunchKernelGGL(k_link_jacobian, dim3((B + 127) / 128), dim3(128), 0, ctx->stream, ctx->robots_dev + robot, B,
-                     (const float*)dq, link, frame, dJ);
-  hipMemcpyAsync(J, dJ, sizeof(float) * B * 6 * r.na, hipMemcpyDeviceToHost, ctx->stream);
+                     (const float*)dq, (const float*)dbase, link, frame, dJ);
+  hipMemcpyAsync(J, dJ, jb, hipMemcpyDeviceToHost, ctx->stream);
   hipError_t e = hipStreamSynchronize(ctx->stream);
-  hipFree(dq);
-  hipFree(dJ);
+  hipFree(d);
   if (e != hipSuccess) return fail(RK_EHIP, std::string("rk_link_jacobian: ") + hipGetErrorString(e));
   return RK_OK;
+}
+
+int rk_link_jacobian(rk_context* ctx, int32_t robot, int32_t B, const float* q, int32_t link, int32_t frame,
+                     float* J) {
+  return rk_link_jacobian_base(ctx, robot, B, q, nullptr, link, frame, J);
 }
 
 int rk_copy_probe(rk_context* ctx, int64_t bytes, float* kernel_ms) {
@@ -2424,8 +2521,6 @@ int rk_batch_solve(rk_batch* b, int32_t iters, const rk_lm_params* p, int32_t mo
   if (b->traj_T) {
     if (mode != RK_MODE_FUSED) return fail(RK_EINVAL, "trajectory batches run the fused kernel only");
     rc = launch_traj(b, make_problems(b, from_seed), lp, it0, iters);
-  } else if ((mode == RK_MODE_PLAIN || iters == 0) && b->th.n_rot) {
-    return fail(RK_EINVAL, "rotation targets run the fused kernels only (with iters >= 1)");
   } else if (iters == 0 && (mode == RK_MODE_PLAIN || mode == RK_MODE_FUSED)) {
     rc = eval_cost(b, from_seed);
   } else if (mode == RK_MODE_PLAIN) {

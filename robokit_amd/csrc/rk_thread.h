@@ -61,8 +61,13 @@ __device__ __forceinline__ Pose chain_fk(RobotK* rb, int nj, const float* q, int
   return Tt;
 }
 
-// residual rows at a point: pose rows of the target, limit rows and smoothness rows; with JAC
-// also the target's Jacobian blocks and the diagonal Jacobian entries of limit / smoothness rows
+// residual rows at a point: the target's six rows, limit rows and smoothness rows; with JAC also
+// the target's Jacobian data and the diagonal Jacobian entries of limit / smoothness rows.  The
+// target rows are either a FrameTask target (log, 6 rows; tg = its Jacobian blocks) or a
+// PositionTask target (rows 0..2, r = w (p_tgt - p_link), position_task.py:188-212; tg[0..2] =
+// p_link) and / or a RotationTask target (rows 3..5, r = w axis_angle(q_link q_tgt^-1),
+// rotation_task.py:323-349) on links of one parent joint -- the term pair of the IK presets.  In
+// that form tgt.t holds the position target and tgt.q the target rotation.
 template <int NA, bool JAC>
 __device__ __forceinline__ void chain_rows(RobotK* rb, TermsK* tm, const Pose& Tf, const Pose& tgt, const float* q,
                                            const float* qprev, float (&r_pose)[6], float (&r_lim)[NA],
@@ -72,6 +77,22 @@ __device__ __forceinline__ void chain_rows(RobotK* rb, TermsK* tm, const Pose& T
 #pragma unroll
     for (int i = 0; i < 6; ++i) w[i] = tm->pose_w[0][i];
     pose_target(Tf, tgt, w, r_pose, JAC ? tg : nullptr);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) r_pose[i] = 0.f;
+    if (tm->n_pos) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        r_pose[i] = tm->pos_w[0][i] * (tgt.t[i] - Tf.t[i]);
+        if (JAC) tg[i] = Tf.t[i];
+      }
+    }
+    if (tm->n_rot) {
+      float aa[3];
+      rotation_target(Tf, tgt.q, aa);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) r_pose[3 + i] = tm->rot_w[0][i] * aa[i];
+    }
   }
   if (tm->limit) {
 #pragma unroll
@@ -98,6 +119,19 @@ __device__ __forceinline__ float chain_cost(TermsK* tm, const float* r_pose, con
 #pragma unroll
     for (int k = 0; k < 6; ++k) s += r_pose[k] * r_pose[k];
     cost = cost + 0.5f * s;
+  } else {
+    if (tm->n_pos) {
+      float s = 0.f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s += r_pose[k] * r_pose[k];
+      cost = cost + 0.5f * s;
+    }
+    if (tm->n_rot) {
+      float s = 0.f;
+#pragma unroll
+      for (int k = 3; k < 6; ++k) s += r_pose[k] * r_pose[k];
+      cost = cost + 0.5f * s;
+    }
   }
   if (tm->limit) {
     float s = 0.f;
@@ -114,7 +148,8 @@ __device__ __forceinline__ float chain_cost(TermsK* tm, const float* r_pose, con
   return cost;
 }
 
-// One serial-arm problem per lane: <= 1 pose target, limit and smoothness rows.
+// One serial-arm problem per lane: <= 1 pose target, or <= 1 position and <= 1 rotation target on
+// links of one parent joint; limit and smoothness rows.
 template <int NA, int NJ>
 __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, const LMParams& lp, int it0, int iters) {
   RobotK* rb = as_const(Pg.robot);
@@ -123,8 +158,11 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
   const bool valid = s_idx < Pg.n;
   const int s = valid ? s_idx : Pg.n - 1;
   const int nj = rb->nj;
-  const int n_pose = tm->n_pose;
-  const int jt = n_pose ? rb->link_joint[tm->pose_link[0]] : -1;
+  const int n_pose = tm->n_pose, n_pos = tm->n_pos, n_rot = tm->n_rot;
+  const int jt = n_pose ? rb->link_joint[tm->pose_link[0]]
+                 : n_pos ? rb->link_joint[tm->pos_link[0]]
+                 : n_rot ? rb->link_joint[tm->rot_link[0]]
+                         : -1;
 
   float q[NA], qprev[NA];
 #pragma unroll
@@ -138,6 +176,15 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
 #pragma unroll
     for (int c = 0; c < 7; ++c) v[c] = Pg.pose_tgt[c * S + s];
     load_pose(v, tgt);
+  } else {
+    if (n_pos) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) tgt.t[c] = Pg.pos_tgt[c * S + s];
+    }
+    if (n_rot) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) tgt.q[c] = Pg.rot_tgt[c * S + s];
+    }
   }
   float lam = it0 == 0 ? lp.lm_lambda : Pg.lam[s];
   int nacc = it0 == 0 ? 0 : Pg.nacc[s];
@@ -155,12 +202,27 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
     const Pose Tf = chain_fk<NA, NJ, true>(rb, nj, q, jt, S6);
     float r_pose[6], r_lim[NA], r_sm[NA], jl[NA], js[NA], tg[36];
     chain_rows<NA, true>(rb, tm, Tf, tgt, q, qprev, r_pose, r_lim, r_sm, tg, jl, js);
-    // ---- pose-row Jacobian columns (zero for joints after the target link's joint)
+    // ---- target-row Jacobian columns (zero for joints after the target link's joint): FrameTask
+    // -w jlog Ad(T_f^-1) S; PositionTask -w (v + w x p_link) (position_task.py:253-262); RotationTask
+    // w * (the column's angular part) (rotation_task.py:352-401)
     float J[6][NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
       float c6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      if (n_pose && a <= jt) pose_jac_column(tg, S6[a], c6);
+      if (a <= jt) {
+        if (n_pose) {
+          pose_jac_column(tg, S6[a], c6);
+        } else {
+          if (n_pos) {
+            const float w3[3] = {tm->pos_w[0][0], tm->pos_w[0][1], tm->pos_w[0][2]};
+            pos_jac_column(S6[a], tg, w3, c6);
+          }
+          if (n_rot) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) c6[3 + i] = tm->rot_w[0][i] * S6[a][3 + i];
+          }
+        }
+      }
 #pragma unroll
       for (int i = 0; i < 6; ++i) J[i][a] = c6[i];
     }
@@ -176,6 +238,19 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
 #pragma unroll
           for (int r = 0; r < 6; ++r) sp += J[r][i] * J[r][j];
           h = sp;
+        } else if (i <= jt) {  // position term, then rotation term
+          if (n_pos) {
+            float sp = 0.f;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) sp += J[r][i] * J[r][j];
+            h = sp;
+          }
+          if (n_rot) {
+            float sr = 0.f;
+#pragma unroll
+            for (int r = 3; r < 6; ++r) sr += J[r][i] * J[r][j];
+            h = h + sr;
+          }
         }
         if (i == j) {
           if (tm->limit) h = h + jl[i] * jl[i];
@@ -189,6 +264,19 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
 #pragma unroll
         for (int r = 0; r < 6; ++r) sp += J[r][i] * r_pose[r];
         gv = sp;
+      } else if (i <= jt) {
+        if (n_pos) {
+          float sp = 0.f;
+#pragma unroll
+          for (int r = 0; r < 3; ++r) sp += J[r][i] * r_pose[r];
+          gv = sp;
+        }
+        if (n_rot) {
+          float sr = 0.f;
+#pragma unroll
+          for (int r = 3; r < 6; ++r) sr += J[r][i] * r_pose[r];
+          gv = gv + sr;
+        }
       }
       if (tm->limit) gv = gv + jl[i] * r_lim[i];
       if (tm->smooth) gv = gv + js[i] * r_sm[i];

@@ -1,8 +1,11 @@
 """IKSolver front end on the MI355X population path.
 
-Mirrors the reference's IK helper (helpers/ik/ik.py, helpers/ik/config.py) for its default use:
+Mirrors the reference's IK helper (helpers/ik/ik.py, helpers/ik/config.py, helpers/ik/presets.py):
 
-  IKSolverConfig      helpers/ik/config.py:49-66 (solver schedule, Roberts sampler settings)
+  IKContext           helpers/ik/config.py:12-33 (the values an IKSolver gives its term templates)
+  IKSolverConfig      helpers/ik/config.py:49-92 (term templates added with ``add``, the solver
+                      schedule, the Roberts sampler settings)
+  presets             helpers/ik/presets.py:13-36 (basic, smooth, single_candidate)
   IKSolver.solve_numpy  helpers/ik/ik.py:223-304, 349-372: candidates around the joint
                       mid-range (drawn once, kept resident) or around init_q (init_sample_range
                       x joint range), the population schedule, and prev_q kept when its cost is no
@@ -10,20 +13,39 @@ Mirrors the reference's IK helper (helpers/ik/ik.py, helpers/ik/config.py) for i
   compute_metrics_numpy  helpers/ik/ik.py:387-470 (_ik_metrics_kernel, :60-82): per-target
                       position / rotation error and success at 5 mm / 0.05 rad by default
 
-The optimisation terms are FrameTask(log) on every target link (position / orientation weights)
-plus PositionLimit: the terms the reference's IK presets start from.  Every solve, score and
-select runs on the GPU through librkhip.so.  Score terms, rest_state pinning, scenes, the
-floating base and the torch adapter are not implemented (ValueError / NotImplementedError).
+Term templates (robokit_amd.lm FrameTask / PositionTask / RotationTask / PositionLimit /
+SmoothnessTask built without a robot) take IKContext.TARGET_LINKS / TARGET_LINK as frames,
+IKContext.TARGET as targets and IKContext.PREVIOUS_STATE as the smoothness reference.  A config
+without terms solves FrameTask(log) on every target link (``position_weight`` /
+``orientation_weight``) plus PositionLimit(``limit_weight``).  Every solve, score and select runs on
+the GPU through librkhip.so.  Score terms, rest_state pinning, scenes, the floating base and the
+torch adapter are not implemented (ValueError / NotImplementedError).
 """
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import Optional, Sequence, Tuple, Union
+from typing import List, Optional, Sequence, Tuple, Union
 
 import numpy as np
 
+from . import lm as L
 from . import native as N
-from .population import PopulationIK, PopulationSolverConfig, default_ik_config
+from .population import PopulationIK, PopulationSolverConfig, Stage, default_ik_config
+
+
+class IKContext(L.Context):
+    """Values an IKSolver gives its term templates (helpers/ik/config.py:12-33)."""
+
+    TARGET_LINKS = 0
+    TARGET_LINK = 1
+    TARGET = 2
+    PREVIOUS_STATE = 3
+    REST_Q = 4
+    REST_BASE = 5
+    NUM_SOLUTIONS = 6
+    OPTIMIZATION_TERMS = 7
+    SCENE = 8
+    SCENE_INDICES = 9
 
 
 @dataclass(frozen=True)
@@ -38,21 +60,63 @@ class RobertsSamplerConfig:
 
 @dataclass
 class IKSolverConfig:
+    terms: List[L.Task] = field(default_factory=list)
     solver: PopulationSolverConfig = field(default_factory=default_ik_config)
     sampler: RobertsSamplerConfig = field(default_factory=lambda: RobertsSamplerConfig(keep_first=True))
     position_weight: float = 1.0
     orientation_weight: float = 1.0
     limit_weight: float = 1.0
     enable_T_world_base: bool = False
+    score_terms: List[L.Task] = field(default_factory=list)
 
     def __post_init__(self):
         if self.enable_T_world_base:
             raise NotImplementedError("the floating base is not implemented on this path")
+        if self.score_terms:
+            raise NotImplementedError("score terms are not implemented on this path (candidates rank by cost)")
+
+    def add(self, term: L.Task, *, name: Optional[str] = None) -> "IKSolverConfig":
+        """append a term template (helpers/ik/config.py:73-76)"""
+        if name is not None:
+            term.name = name
+            if term._template is not None:
+                term._template["name"] = name
+        self.terms.append(term)
+        return self
+
+    def add_score(self, term: L.Task, *, name: Optional[str] = None) -> "IKSolverConfig":
+        raise NotImplementedError("score terms are not implemented on this path (candidates rank by cost)")
+
+    def task(self, name: str) -> L.Task:
+        for t in self.terms:
+            if t.name == name:
+                return t
+        raise KeyError(name)
+
+
+class presets:
+    """The reference's IK presets (helpers/ik/presets.py:13-36)."""
+
+    basic = (IKSolverConfig(sampler=RobertsSamplerConfig(keep_first=True, scale=1.0))
+             .add(L.PositionTask(frame_index=IKContext.TARGET_LINKS, T_world_target=IKContext.TARGET, weight=20.0))
+             .add(L.RotationTask(frame_index=IKContext.TARGET_LINKS, T_world_target=IKContext.TARGET, weight=10.0))
+             .add(L.PositionLimit(weight=50.0)))
+    smooth = (IKSolverConfig(sampler=RobertsSamplerConfig(keep_first=True, scale=1.0))
+              .add(L.PositionTask(frame_index=IKContext.TARGET_LINKS, T_world_target=IKContext.TARGET, weight=20.0))
+              .add(L.RotationTask(frame_index=IKContext.TARGET_LINKS, T_world_target=IKContext.TARGET, weight=10.0))
+              .add(L.PositionLimit(weight=50.0))
+              .add(L.SmoothnessTask(prev_var=IKContext.PREVIOUS_STATE, weight=0.1)))
+    single_candidate = (
+        IKSolverConfig(solver=PopulationSolverConfig(stages=[Stage(30, 1, N.LMParams(lm_lambda=1.0))]))
+        .add(L.PositionTask(frame_index=IKContext.TARGET_LINKS, T_world_target=IKContext.TARGET, weight=20.0))
+        .add(L.RotationTask(frame_index=IKContext.TARGET_LINKS, T_world_target=IKContext.TARGET, weight=10.0))
+        .add(L.PositionLimit(weight=50.0))
+        .add(L.SmoothnessTask(prev_var=IKContext.PREVIOUS_STATE, weight=0.1)))
 
 
 class IKSolver:
-    """IK for one robot and a fixed list of target links; batches of any size up to the first
-    solve's (buffers are sized at the first solve, as the reference's)."""
+    """IK for one robot and a fixed list of target links; the batch size is fixed by the first
+    solve (buffers are sized then, as the reference's)."""
 
     def __init__(self, config: IKSolverConfig, spec, target_links: Union[int, Sequence[int]],
                  ctx: Optional[N.Context] = None):
@@ -64,11 +128,7 @@ class IKSolver:
         if any(not 0 <= l < spec.num_links for l in self.target_links):
             raise ValueError(f"target link out of range (robot has {spec.num_links} links)")
         self.ctx = ctx or N.Context(0)
-        na = spec.num_actuated_joints
-        k = len(self.target_links)
-        w = np.tile(np.array([config.position_weight] * 3 + [config.orientation_weight] * 3, np.float32), (k, 1))
-        self.terms = N.TermSpec(pose_links=self.target_links, pose_weights=w,
-                                limit_weights=np.full(na, config.limit_weight, np.float32))
+        self._plan_terms()
         if config.solver.num_candidates[-1] != 1:
             raise NotImplementedError("a final stage of more than one solution per target is not implemented")
         self._pop: Optional[PopulationIK] = None
@@ -76,6 +136,60 @@ class IKSolver:
         self._metrics: Optional[N.Batch] = None
         self.last_cost: Optional[np.ndarray] = None
         self.last_prev_cost: Optional[np.ndarray] = None
+
+    def _plan_terms(self) -> None:
+        """resolve the config's term templates against the robot and the target links: each term's
+        frames become indices into the target list, whose poses feed its targets at solve time"""
+        k = len(self.target_links)
+        templates = self.config.terms or [
+            L.FrameTask(frame_index=IKContext.TARGET_LINKS, T_world_target=IKContext.TARGET,
+                        position_weight=self.config.position_weight, orientation_weight=self.config.orientation_weight),
+            L.PositionLimit(weight=self.config.limit_weight)]
+        robot = L.Robot(self.spec)
+        ctxv = {IKContext.TARGET_LINKS: list(self.target_links), IKContext.TARGET_LINK: self.target_links[0],
+                IKContext.TARGET: np.tile(np.array([0, 0, 0, 1, 0, 0, 0], np.float32), (1, k, 1)),
+                IKContext.PREVIOUS_STATE: L.RobotState(robot, self.spec.zero_q)}
+        self._sel = {"pose": [], "pos": [], "rot": []}  # target-list index of each row block
+        w = {"pose": [], "pos": [], "rot": []}
+        limit = smooth = None
+        self.needs_prev = False
+        for t in templates:
+            tmpl = t._template or {}
+            if isinstance(t, (L.FrameTask, L.PositionTask, L.RotationTask)) and tmpl.get("T_world_target") is not IKContext.TARGET:
+                raise NotImplementedError("IK term targets must be IKContext.TARGET on this path")
+            if isinstance(t, L.SmoothnessTask) and tmpl.get("prev_var") is not IKContext.PREVIOUS_STATE:
+                raise NotImplementedError("the IK smoothness reference must be IKContext.PREVIOUS_STATE")
+            r = t.resolve(robot, ctxv)
+            key = {L.FrameTask: "pose", L.PositionTask: "pos", L.RotationTask: "rot"}.get(type(r))
+            if key:
+                for f in r.frame_indices:
+                    if f not in self.target_links:
+                        raise ValueError(f"frame {f} is not a target link")
+                    self._sel[key].append(self.target_links.index(f))
+                w[key].append(r.weights)
+            elif isinstance(r, L.PositionLimit):
+                limit = r.weights
+            elif isinstance(r, L.SmoothnessTask):
+                smooth, self.needs_prev = r.weights, True
+            else:
+                raise NotImplementedError(f"IK term {type(r).__name__} is not on this path")
+        cat = lambda key: np.concatenate(w[key]) if w[key] else None  # noqa: E731
+        self.terms = N.TermSpec(pose_links=[self.target_links[i] for i in self._sel["pose"]], pose_weights=cat("pose"),
+                                pos_links=[self.target_links[i] for i in self._sel["pos"]], pos_weights=cat("pos"),
+                                rot_links=[self.target_links[i] for i in self._sel["rot"]], rot_weights=cat("rot"),
+                                limit_weights=limit, smooth_weights=smooth)
+        self._metric_terms = N.TermSpec(pose_links=self.target_links, pose_weights=np.ones((k, 6), np.float32))
+
+    def term_targets(self, tg: np.ndarray) -> dict:
+        """the batch's target arrays (pose [B, n, 7], position [B, n, 3], rotation [B, n, 4]) from
+        the target poses [B, k, 7]"""
+        out = {}
+        for key, sl in (("pose_targets", "pose"), ("pos_targets", "pos"), ("rot_targets", "rot")):
+            idx = self._sel[sl]
+            if idx:
+                a = tg[:, idx]
+                out[key] = np.ascontiguousarray(a[..., :3] if sl == "pos" else a[..., 3:] if sl == "rot" else a)
+        return out
 
     @property
     def num_targets(self) -> int:
@@ -107,15 +221,18 @@ class IKSolver:
             raise ValueError(f"init_q batch size mismatch: expected {B}")
         if prev_q is not None and np.asarray(prev_q).reshape(-1, na).shape[0] != B:
             raise ValueError(f"prev_state batch size mismatch: expected {B}, got {np.asarray(prev_q).reshape(-1, na).shape[0]}")
+        if self.needs_prev and prev_q is None:
+            raise ValueError("this configuration's SmoothnessTask needs prev_q (IKContext.PREVIOUS_STATE)")
+        pq = None if prev_q is None else np.asarray(prev_q, np.float32).reshape(B, na)
         scale = init_sample_range if init_sample_range is not None else self.config.sampler.scale
-        self._pop.set_targets(pose_targets=tg, init_q=init_q, init_scale=scale)
+        self._pop.set_targets(**self.term_targets(tg), q_prev=pq if self.needs_prev else None, init_q=init_q,
+                              init_scale=scale)
         win = self._pop.solve()
         out = win.get()
         q, cost = out["q"], out["cost"]
         self.last_cost, self.last_prev_cost = cost, None
-        if prev_q is not None:  # score prev with the solver's ranking rule; keep it on a tie
-            pq = np.asarray(prev_q, np.float32).reshape(B, na)
-            self._score.set(q=pq, pose_targets=tg)
+        if pq is not None:  # score prev with the solver's ranking rule; keep it on a tie
+            self._score.set(q=pq, q_prev=pq if self.needs_prev else None, **self.term_targets(tg))
             self._score.solve(0, self.config.solver.optimizer[-1], from_seed=True)
             pc = self._score.get()["cost"]
             keep = pc <= cost
@@ -132,7 +249,7 @@ class IKSolver:
         if q.shape[0] != tg.shape[0]:
             raise ValueError(f"Target batch size ({tg.shape[0]}) must match state batch size ({q.shape[0]})")
         if self._metrics is None or self._metrics.capacity < q.shape[0]:
-            self._metrics = N.Batch(self.ctx, self.spec, self.terms, q.shape[0])
+            self._metrics = N.Batch(self.ctx, self.spec, self._metric_terms, q.shape[0])
         self._metrics.set(q=q, pose_targets=tg)
         pe, ae, ok = self._metrics.metrics(position_threshold_m, orientation_threshold_rad)
         return pe, ae, ok.astype(bool)

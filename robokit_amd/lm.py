@@ -169,17 +169,53 @@ def _per_frame_weights(w, nframes: int, name: str) -> np.ndarray:
     return a
 
 
+class Context(Enum):
+    """Base of the keys that stand for a helper's values in task arguments (terms/task.py:13-15)."""
+
+
+def _has_context(*args) -> bool:
+    return any(isinstance(a, Context) or (isinstance(a, tuple) and any(isinstance(x, Context) for x in a))
+               for a in args)
+
+
 class Task:
+    """A cost term.  Built without a robot, or with a Context placeholder among its arguments, it
+    is a template (the IK presets' form, helpers/ik/presets.py:13-18): ``resolve`` swaps the
+    placeholders for a helper's values and builds the concrete term for a robot."""
+
     name: str = ""
+    _template: Optional[dict] = None
+
+    @property
+    def is_template(self) -> bool:
+        return self._template is not None
+
+    def _as_template(self, **kwargs) -> None:
+        self._template = kwargs
+        self.robot, self.name = None, kwargs.get("name", "")
+        self.weight = kwargs.get("weight")
+
+    def resolve(self, robot: "Robot", context: Dict["Context", object]) -> "Task":
+        """the concrete term for ``robot`` with every Context placeholder replaced from ``context``"""
+        if self._template is None:
+            return self
+        sub = lambda v: context[v] if isinstance(v, Context) else v  # noqa: E731
+        args = {k: (tuple(sub(x) for x in v) if isinstance(v, tuple) else sub(v)) for k, v in self._template.items()}
+        return type(self)(robot, **args)
 
 
 class FrameTask(Task):
     """Pose residual r = w * log(T_ee^-1 T_target) per frame ([v; w] rows, terms/dense/frame_task.py:505-554)."""
 
-    def __init__(self, robot: Robot, frame_index: Union[int, Sequence[int]], T_world_target,
-                 position_weight: Union[float, Sequence[float]] = 1.0,
+    def __init__(self, robot: Optional[Robot] = None, frame_index: Union[int, Sequence[int], Context, None] = None,
+                 T_world_target=None, position_weight: Union[float, Sequence[float]] = 1.0,
                  orientation_weight: Union[float, Sequence[float]] = 1.0,
                  residual_mode: Literal["log", "distance"] = "log", trajectory_frames: int = 1, name: str = ""):
+        if robot is None or frame_index is None or T_world_target is None or _has_context(frame_index, T_world_target):
+            self._as_template(frame_index=frame_index, T_world_target=T_world_target, position_weight=position_weight,
+                              orientation_weight=orientation_weight, residual_mode=residual_mode,
+                              trajectory_frames=trajectory_frames, name=name)
+            return
         if residual_mode not in {"log", "distance"}:
             raise ValueError(f"Unsupported residual_mode: {residual_mode}")
         if residual_mode != "log":
@@ -209,8 +245,13 @@ class FrameTask(Task):
 class PositionTask(Task):
     """Point residual r = w * (p_target - p_link) per frame (terms/dense/position_task.py:188-212)."""
 
-    def __init__(self, robot: Robot, frame_index: Union[int, Sequence[int]], T_world_target,
-                 weight: Union[float, Sequence[float]] = 20.0, fixed_target: bool = False, name: str = ""):
+    def __init__(self, robot: Optional[Robot] = None, frame_index: Union[int, Sequence[int], Context, None] = None,
+                 T_world_target=None, weight: Union[float, Sequence[float]] = 20.0, fixed_target: bool = False,
+                 name: str = ""):
+        if robot is None or frame_index is None or T_world_target is None or _has_context(frame_index, T_world_target):
+            self._as_template(frame_index=frame_index, T_world_target=T_world_target, weight=weight,
+                              fixed_target=fixed_target, name=name)
+            return
         self.robot, self.name, self.weight, self.fixed_target = robot, name, weight, fixed_target
         self.frame_indices = _frames(frame_index)
         k = len(self.frame_indices)
@@ -239,8 +280,13 @@ class PositionTask(Task):
 class RotationTask(Task):
     """Orientation residual r = w * axis_angle(q_link * q_target^-1) per frame (rotation_task.py:323-349)."""
 
-    def __init__(self, robot: Robot, frame_index: Union[int, Sequence[int]], T_world_target,
-                 weight: Union[float, Sequence[float]] = 10.0, fixed_target: bool = False, name: str = ""):
+    def __init__(self, robot: Optional[Robot] = None, frame_index: Union[int, Sequence[int], Context, None] = None,
+                 T_world_target=None, weight: Union[float, Sequence[float]] = 10.0, fixed_target: bool = False,
+                 name: str = ""):
+        if robot is None or frame_index is None or T_world_target is None or _has_context(frame_index, T_world_target):
+            self._as_template(frame_index=frame_index, T_world_target=T_world_target, weight=weight,
+                              fixed_target=fixed_target, name=name)
+            return
         self.robot, self.name, self.weight, self.fixed_target = robot, name, weight, fixed_target
         self.frame_indices = _frames(frame_index)
         k = len(self.frame_indices)
@@ -269,8 +315,11 @@ class RotationTask(Task):
 class PositionLimit(Task):
     """Joint-limit hinge r_i = w_i (max(0, q_i - hi_i) + max(0, lo_i - q_i)) (position_limit.py:508-534)."""
 
-    def __init__(self, robot: Robot, weight: Optional[Union[float, Sequence[float]]] = None,
+    def __init__(self, robot: Optional[Robot] = None, weight: Optional[Union[float, Sequence[float]]] = None,
                  residual_mode: Literal["abs", "sqrt_abs", "exp_barrier"] = "abs", name: str = ""):
+        if robot is None:
+            self._as_template(weight=weight, residual_mode=residual_mode, name=name)
+            return
         if residual_mode not in {"abs", "sqrt_abs", "exp_barrier"}:
             raise ValueError(f"Unsupported residual_mode: {residual_mode}")
         if residual_mode != "abs":
@@ -290,8 +339,11 @@ class PositionLimit(Task):
 class SmoothnessTask(Task):
     """Temporal smoothness r = w (q - q_prev) (terms/dense/smoothness_task.py:350-365)."""
 
-    def __init__(self, robot: Robot, prev_var: Optional[RobotState] = None,
+    def __init__(self, robot: Optional[Robot] = None, prev_var: Union[RobotState, Context, None] = None,
                  weight: Union[float, Sequence[float]] = 1.0, name: str = ""):
+        if robot is None or _has_context(prev_var):
+            self._as_template(prev_var=prev_var, weight=weight, name=name)
+            return
         self.robot, self.name, self.weight = robot, name, weight
         na = robot.spec.num_actuated_joints
         w = np.zeros(na, np.float32)
@@ -366,6 +418,8 @@ class LMOptimizer:
 
     # -- term structure
     def _plan(self) -> None:
+        if any(t.is_template for t in self.terms):
+            raise ValueError("a term template (no robot, or Context placeholders) must be resolved first")
         robots = {id(t.robot) for t in self.terms}
         if len(robots) != 1:
             raise ValueError("all terms of one optimizer must act on the same robot")

@@ -97,7 +97,8 @@ def _close_all() -> None:
 
 EXPORTS = [
     "rk_last_error", "rk_abi_version", "rk_context_create", "rk_context_destroy", "rk_context_sync",
-    "rk_context_mem", "rk_context_mark", "rk_context_elapsed", "rk_robot_upload", "rk_fk", "rk_link_jacobian", "rk_copy_probe", "rk_se3_op",
+    "rk_context_mem", "rk_context_mark", "rk_context_elapsed", "rk_robot_upload", "rk_fk", "rk_fk_base", "rk_link_jacobian",
+    "rk_link_jacobian_base", "rk_copy_probe", "rk_se3_op",
     "rk_batch_create",
     "rk_batch_destroy", "rk_batch_set", "rk_batch_set_rotations", "rk_batch_select", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics",
     "rk_batch_metrics_noalloc", "rk_group_create",
@@ -225,17 +226,31 @@ class Context:
         self._keep.append(spec)
         return rid.value
 
-    def fk(self, spec, q: np.ndarray) -> np.ndarray:
+    def fk(self, spec, q: np.ndarray, T_world_base: Optional[np.ndarray] = None) -> np.ndarray:
+        """T_world_link [B, nl, 7]; T_world_base [B, 7] or [7] (None: the identity)"""
         q = _f32(np.atleast_2d(q))
         out = np.zeros((q.shape[0], spec.num_links, 7), np.float32)
-        _check(lib().rk_fk(self._h, self.robot(spec), q.shape[0], _p(q), _p(out)))
+        if T_world_base is None:
+            _check(lib().rk_fk(self._h, self.robot(spec), q.shape[0], _p(q), _p(out)))
+        else:
+            base = _f32(np.broadcast_to(np.asarray(T_world_base, np.float32), (q.shape[0], 7)))
+            _check(lib().rk_fk_base(self._h, self.robot(spec), q.shape[0], _p(q), _p(base), _p(out)))
         return out
 
-    def link_jacobian(self, spec, q: np.ndarray, link: int, frame: str = "body") -> np.ndarray:
+    def link_jacobian(self, spec, q: np.ndarray, link: int, frame: str = "body",
+                      T_world_base: Optional[np.ndarray] = None) -> np.ndarray:
+        """J [B, 6, na]; with T_world_base ([B, 7] or [7]) the floating-base Jacobian [B, 6, 6 + na],
+        base columns first"""
         q = _f32(np.atleast_2d(q))
-        out = np.zeros((q.shape[0], 6, spec.num_actuated_joints), np.float32)
-        _check(lib().rk_link_jacobian(self._h, self.robot(spec), q.shape[0], _p(q), int(link),
-                                      0 if frame == "spatial" else 1, _p(out)))
+        fr = 0 if frame == "spatial" else 1
+        if T_world_base is None:
+            out = np.zeros((q.shape[0], 6, spec.num_actuated_joints), np.float32)
+            _check(lib().rk_link_jacobian(self._h, self.robot(spec), q.shape[0], _p(q), int(link), fr, _p(out)))
+        else:
+            base = _f32(np.broadcast_to(np.asarray(T_world_base, np.float32), (q.shape[0], 7)))
+            out = np.zeros((q.shape[0], 6, 6 + spec.num_actuated_joints), np.float32)
+            _check(lib().rk_link_jacobian_base(self._h, self.robot(spec), q.shape[0], _p(q), _p(base), int(link), fr,
+                                               _p(out)))
         return out
 
     def copy_probe(self, nbytes: int) -> float:

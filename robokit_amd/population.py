@@ -147,7 +147,7 @@ class PopulationIK:
 
     def set_targets(self, pose_targets: Optional[np.ndarray] = None, pos_targets: Optional[np.ndarray] = None,
                     q_prev: Optional[np.ndarray] = None, init_q: Optional[np.ndarray] = None,
-                    init_scale: float = 0.2) -> None:
+                    init_scale: float = 0.2, rot_targets: Optional[np.ndarray] = None) -> None:
         """each problem's targets [batch, ...], broadcast over its candidates (rows i * width + j).
         Seeds: init_q None: the global mid-range candidates (uploaded once, then kept resident);
         else init_q [batch, na] spread by init_scale x joint range (helpers/ik/ik.py:509-532)."""
@@ -166,7 +166,7 @@ class PopulationIK:
             q = np.tile(self.seeds, (self.batch_size, 1))
             self._global_loaded = True
         b0.set(q=q, pose_targets=rep(pose_targets), pos_targets=rep(pos_targets), q_prev=rep(q_prev),
-               n=self.batch_size * w0)
+               n=self.batch_size * w0, rot_targets=rep(rot_targets))
         self._ready = True
 
     def solve(self, tpp: int = 0) -> N.Batch:

@@ -63,6 +63,34 @@ def ur10(limit: float = LIMIT) -> RobotSpec:
     return RobotSpec.from_joints("ur10", links, joints, "base_link")
 
 
+def panda_tcp() -> RobotSpec:
+    """The Franka Panda arm as the reference's IK benchmark builds it (finger joints fixed,
+    scripts/benchmark/ik/helper.py:47-61): the seven revolute joints of franka_description with
+    their limits, link7 -> panda_hand as one fixed joint (0.107 m along z, -pi/4 about z), and
+    panda_hand -> panda_hand_tcp (0.1034 m along z), the benchmark's target link.  Its FK is checked
+    against the reference's Panda ground truth for panda_hand."""
+    rev = lambda i, xyz, rpy, lo, hi: {  # noqa: E731
+        "name": f"panda_joint{i}", "type": "revolute", "parent": f"panda_link{i - 1}", "child": f"panda_link{i}",
+        "xyz": xyz, "rpy": rpy, "axis": (0, 0, 1), "lower": lo, "upper": hi,
+    }
+    h = PI / 2
+    joints = [
+        rev(1, (0, 0, 0.333), (0, 0, 0), -2.8973, 2.8973),
+        rev(2, (0, 0, 0), (-h, 0, 0), -1.7628, 1.7628),
+        rev(3, (0, -0.316, 0), (h, 0, 0), -2.8973, 2.8973),
+        rev(4, (0.0825, 0, 0), (h, 0, 0), -3.0718, -0.0698),
+        rev(5, (-0.0825, 0.384, 0), (-h, 0, 0), -2.8973, 2.8973),
+        rev(6, (0, 0, 0), (h, 0, 0), -0.0175, 3.7525),
+        rev(7, (0.088, 0, 0), (h, 0, 0), -2.8973, 2.8973),
+        {"name": "panda_hand_joint", "type": "fixed", "parent": "panda_link7", "child": "panda_hand",
+         "xyz": (0, 0, 0.107), "rpy": (0, 0, -PI / 4)},
+        {"name": "panda_hand_tcp_joint", "type": "fixed", "parent": "panda_hand", "child": "panda_hand_tcp",
+         "xyz": (0, 0, 0.1034), "rpy": (0, 0, 0)},
+    ]
+    links = [f"panda_link{i}" for i in range(8)] + ["panda_hand", "panda_hand_tcp"]
+    return RobotSpec.from_joints("panda_tcp", links, joints, "panda_link0")
+
+
 HUMANOID_LIMBS = (("waist", "pelvis", 3), ("left_leg", "pelvis", 6), ("right_leg", "pelvis", 6),
                   ("left_arm", "chest", 7), ("right_arm", "chest", 7))
 

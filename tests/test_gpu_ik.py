@@ -119,3 +119,68 @@ def test_solver_humanoid_four_targets(ctx):
     assert mism <= max(n_ref, 1)
     assert ok32.sum() > 0  # the comparison is not vacuous
     assert np.sum(solver.compute_metrics_numpy(q, wl.targets)[2] != ok) <= 1
+
+
+def _preset_terms(links, targets, na):
+    """the oracle terms of presets.basic: PositionTask w 20 + RotationTask w 10 on every target link,
+    PositionLimit w 50 (helpers/ik/presets.py:13-18)"""
+    k = len(links)
+    return [O.Term(O.POSITION, links, targets, np.full(3 * k, 20.0, np.float32)),
+            O.Term(O.ROTATION, links, targets, np.full(3 * k, 10.0, np.float32)),
+            O.Term(O.LIMIT, (), None, np.full(na, 50.0, np.float32))]
+
+
+def test_presets_basic_matches_oracle_population(ctx):
+    """IKSolver(presets.basic) on the Panda with its TCP link (the reference benchmark's robot and
+    target link): 64 Roberts candidates (keep_first, scale 1) x 6 iterations at lambda 10, best 4 x 10
+    at lambda 1, best 1; the stages run the one-lane-per-problem register kernel's position +
+    rotation rows.  The winners classify like the FP32 oracle population at the reference
+    benchmark's 5 mm / 0.05 rad, and the winners' costs agree with the oracle's to its FP32 spread."""
+    spec = W.panda_tcp()
+    tcp = spec.link_names.index("panda_hand_tcp")
+    wl = W.ik_batch(spec, [tcp], 96, seed=45)
+    solver = IK.IKSolver(IK.presets.basic, spec, tcp, ctx=ctx)
+    print(f"\n[ik] presets.basic panda_tcp {wl.size} targets", flush=True)
+    q = solver.solve_numpy(wl.targets)
+    stages = _stages(solver.config.solver)
+    fn = lambda inst: _preset_terms([tcp], wl.targets[inst], 7)  # noqa: E731
+    q32, c32, _ = OP.population_ik(spec, [tcp], wl.targets, solver._pop.seeds, stages, terms_fn=fn, threads=8)
+    q64, c64, _ = OP.population_ik(spec, [tcp], wl.targets, solver._pop.seeds, stages, dtype="f64", terms_fn=fn,
+                                   threads=8)
+    thr = dict(pos_thr=0.005, ang_thr=0.05)
+    ok = O.ik_metrics(spec, q, [tcp], wl.targets, **thr)[2]
+    ok32 = O.ik_metrics(spec, q32, [tcp], wl.targets, **thr)[2]
+    ok64 = O.ik_metrics(spec, q64, [tcp], wl.targets, dtype="f64", **thr)[2]
+    mism, n_ref = int(np.sum(ok != ok32)), int(np.sum(ok64 != ok32))
+    print(f"  converged (5 mm) gpu {ok.mean():.3f} f32 {ok32.mean():.3f} f64 {ok64.mean():.3f}; "
+          f"differ gpu/f32 {mism}, f32/f64 {n_ref}")
+    assert mism <= max(n_ref, 1)
+    assert ok32.sum() > 0.5 * wl.size  # the preset solves most targets: the comparison is not vacuous
+    # compute_metrics_numpy of the preset solver (its own pose-target metrics batch) = the oracle's
+    pe, ae, succ = solver.compute_metrics_numpy(q, wl.targets)
+    pe_o, ae_o, _ = O.ik_metrics(spec, q, [tcp], wl.targets, **thr)
+    np.testing.assert_allclose(pe, pe_o, rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(ae, ae_o, rtol=1e-4, atol=1e-5)
+    # a previous state is scored with the preset's terms: the oracle's cost at prev_q
+    prev = np.clip(q + 0.01, spec.actuated_joint_limits_f32[:, 0], spec.actuated_joint_limits_f32[:, 1])
+    solver.solve_numpy(wl.targets, prev_q=prev)
+    c_o = O.lm_solve(spec, _preset_terms([tcp], wl.targets, 7), prev, 0, O.LMParams(), traces=False)["cost"]
+    np.testing.assert_allclose(solver.last_prev_cost, c_o, rtol=1e-4, atol=1e-6)
+
+
+def test_presets_smooth_needs_and_uses_previous_state(ctx):
+    """presets.smooth adds SmoothnessTask(prev_var=PREVIOUS_STATE, w 0.1): a solve without prev_q is
+    refused; with prev_q the winners' costs include the smoothness rows (the oracle's cost of the
+    returned joints with prev_q as reference agrees with the solver's)"""
+    spec = W.panda_tcp()
+    tcp = spec.link_names.index("panda_hand_tcp")
+    wl = W.ik_batch(spec, [tcp], 16, seed=46)
+    solver = IK.IKSolver(IK.presets.smooth, spec, tcp, ctx=ctx)
+    with pytest.raises(ValueError, match="prev_q"):
+        solver.solve_numpy(wl.targets)
+    prev = wl.q_target.copy()
+    q = solver.solve_numpy(wl.targets, prev_q=prev)
+    terms = _preset_terms([tcp], wl.targets, 7) + [O.Term(O.SMOOTH, (), None, np.full(7, 0.1, np.float32), prev)]
+    c_o = O.lm_solve(spec, terms, q, 0, O.LMParams(), traces=False)["cost"]
+    best = np.minimum(solver.last_cost, solver.last_prev_cost)
+    np.testing.assert_allclose(best, c_o, rtol=1e-3, atol=1e-6)

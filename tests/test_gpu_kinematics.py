@@ -73,9 +73,8 @@ def test_fk_ground_truth(ctx, panda_urdf_spec):
         if robot["robot_description_name"] != "panda_description":
             continue
         for cfg in robot["configurations"]:
-            if cfg["T_world_base"] is not None:
-                continue  # fixed-base device path; the base variants are checked on the oracle
-            T = ctx.fk(spec, np.array(cfg["qpos"], np.float32))[0]
+            base = None if cfg["T_world_base"] is None else np.array(cfg["T_world_base"], np.float32)
+            T = ctx.fk(spec, np.array(cfg["qpos"], np.float32), T_world_base=base)[0]
             for lp in cfg["link_poses"]:
                 if lp["link_name"] not in spec.link_names:
                     continue  # panda_hand_tcp is not in the fixture URDF
@@ -98,24 +97,28 @@ def test_fk_vs_oracle(ctx, panda_urdf_spec, robot):
 def test_jacobian_ground_truth(ctx, panda_urdf_spec):
     gt = json.loads((GOLDEN / "jacobian_ground_truth.json").read_text())
     spec = panda_urdf_spec
-    n = 0
+    n = nb = 0
     for robot in gt["robots"]:
         if robot["robot_description_name"] != "panda_description":
             continue
         for cfg in robot["configurations"]:
-            if cfg["T_world_base"] is not None:
-                continue
+            base = None if cfg["T_world_base"] is None else np.array(cfg["T_world_base"], np.float32)
             q = np.array(cfg["qpos"], np.float32)
             for lj in cfg["link_jacobians"]:
                 if lj["link_name"] not in spec.link_names:
                     continue
                 link = spec.link_names.index(lj["link_name"])
                 for frame in ("body", "spatial"):
-                    J = ctx.link_jacobian(spec, q, link, frame)[0]
-                    np.testing.assert_allclose(J, np.array(lj[f"{frame}_J_joints"]), atol=1e-5,
-                                               err_msg=f"{cfg['name']} {lj['link_name']} {frame}")
+                    J = ctx.link_jacobian(spec, q, link, frame, T_world_base=base)[0]
+                    msg = f"{cfg['name']} {lj['link_name']} {frame}"
+                    if base is None:
+                        np.testing.assert_allclose(J, np.array(lj[f"{frame}_J_joints"]), atol=1e-5, err_msg=msg)
+                    else:  # floating base: the six base columns first, then the joints
+                        np.testing.assert_allclose(J[:, :6], np.array(lj[f"{frame}_J_base"]), atol=1e-5, err_msg=msg)
+                        np.testing.assert_allclose(J[:, 6:], np.array(lj[f"{frame}_J_joints"]), atol=1e-5, err_msg=msg)
+                        nb += 1
                     n += 1
-    assert n > 0
+    assert n > 0 and nb > 0
 
 
 @pytest.mark.parametrize("robot", ["ur10", "humanoid"])
@@ -128,3 +131,34 @@ def test_jacobian_vs_oracle(ctx, robot):
         gpu = ctx.link_jacobian(spec, q, link, frame)
         ora = O.link_jacobian(spec, q, link, frame, dtype="f64")
         np.testing.assert_allclose(gpu, ora, atol=2e-5, rtol=1e-5)
+
+
+@pytest.mark.parametrize("robot", ["ur10", "humanoid"])
+def test_floating_base_vs_oracle(ctx, robot):
+    """FK and the floating-base link Jacobian (base columns first) with a random base pose per
+    problem, against the FP64 oracle (robot_kernels.py:327-377, 667-737); the base columns satisfy
+    their closed forms: spatial = Ad(T_world_base), body = Ad(T_link^-1 T_world_base)"""
+    spec = {"ur10": W.ur10(), "humanoid": W.humanoid29(0)}[robot]
+    rng = np.random.default_rng(9)
+    B = 64
+    q = W.uniform_q(rng, spec, B)
+    quat = rng.normal(size=(B, 4))
+    quat /= np.linalg.norm(quat, axis=1, keepdims=True)
+    base = np.concatenate([rng.uniform(-2, 2, (B, 3)), quat], 1).astype(np.float32)
+    gpu = ctx.fk(spec, q, T_world_base=base)
+    ora, _ = O.fk(spec, q, base=base, dtype="f64")
+    ok, err = _pose_close(gpu, ora, 4e-5)
+    assert ok, err
+    link = spec.num_links - 1
+    for frame in ("body", "spatial"):
+        J = ctx.link_jacobian(spec, q, link, frame, T_world_base=base)
+        Jo = O.link_jacobian(spec, q, link, frame, base=base, dtype="f64")
+        np.testing.assert_allclose(J, Jo, atol=5e-5, rtol=1e-5)
+    # closed form of the spatial base block: Ad(T_world_base)
+    adj = O.se3("adjoint", base.astype(np.float64), dtype="f64").reshape(B, 6, 6)
+    Js = ctx.link_jacobian(spec, q, link, "spatial", T_world_base=base)
+    np.testing.assert_allclose(Js[:, :, :6], adj, atol=2e-6)
+    # without a base the base-free entry points give the joint columns of the identity base
+    ident = np.tile(np.array([0, 0, 0, 1, 0, 0, 0], np.float32), (B, 1))
+    np.testing.assert_allclose(ctx.link_jacobian(spec, q, link, "body", T_world_base=ident)[:, :, 6:],
+                               ctx.link_jacobian(spec, q, link, "body"), atol=1e-6)

@@ -353,12 +353,13 @@ def test_retarget_warm_start_resident(ctx):
 
 
 # ----------------------------------------------------------------------------- RotationTask
-@pytest.mark.parametrize("robot,tpp", [("panda_dh", 8), ("humanoid", 32)])
+@pytest.mark.parametrize("robot,tpp", [("panda_dh", 8), ("humanoid", 32), ("panda_dh", 1), ("panda_tcp", 1)])
 def test_rotation_task_parity(ctx, robot, tpp):
     """PositionTask + RotationTask (the presets.basic term pair) [+ FrameTask on the humanoid] +
-    PositionLimit, LDS-staged kernels, shadow-checked against the oracle's ROTATION rows"""
-    if robot == "panda_dh":
-        spec = W.panda_dh()
+    PositionLimit, shadow-checked against the oracle's ROTATION rows: the LDS-staged kernels (8,
+    32) and the one-lane-per-problem register kernel (1; panda_tcp with its two fixed tail joints)"""
+    if robot in ("panda_dh", "panda_tcp"):
+        spec = W.panda_dh() if robot == "panda_dh" else W.panda_tcp()
         pose_links, rot_links = [], [spec.num_links - 1]
         pos_links = [spec.num_links - 1]
     else:
@@ -399,8 +400,25 @@ def test_rotation_task_parity(ctx, robot, tpp):
                        O.lm_step(*args, dtype="f64", rows=rows), atol=atol)
     print(f"\nrotation {robot} tpp={tpp}: {rep}")
     assert rep.ok(), rep
-    with pytest.raises(ValueError):  # rotation rows run the LDS-staged kernels only
-        b.solve(iters, C0_PARAMS, mode=N.MODE_PLAIN)
+    if tpp == 1:  # the library's default for this term structure is the register kernel
+        b.set(q=q0, pose_targets=pose_t if pose_links else None, pos_targets=pos_t, rot_targets=rot_t)
+        b.solve(iters, C0_PARAMS)
+        assert np.array_equal(b.get()["q"], gpu["q"])
+    # the plain per-term launch (rotation residual / Jacobian kernels writing J to HBM): same bar
+    b.set(q=q0, pose_targets=pose_t if pose_links else None, pos_targets=pos_t, rot_targets=rot_t)
+    b.solve(iters, C0_PARAMS, mode=N.MODE_PLAIN)
+    gp = b.get(traces=True)
+    qp_it, lamp_it = b.iterates()
+    argp = (spec, terms, qp_it.reshape(-1, na), lamp_it.reshape(-1), p)
+    repp = shadow_check(gp["cost_trace"], gp["accept"], O.lm_step(*argp, rows=rows),
+                        O.lm_step(*argp, dtype="f64", rows=rows), atol=atol)
+    print(f"  plain: {repp}")
+    assert repp.ok(), repp
+    # zero iterations score the seeds through the same per-term kernels
+    b.set(q=q0, pose_targets=pose_t if pose_links else None, pos_targets=pos_t, rot_targets=rot_t)
+    b.solve(0, C0_PARAMS)
+    c0 = O.lm_solve(spec, terms, q0, 0, p, dtype="f64", traces=False)["cost"]
+    np.testing.assert_allclose(b.get()["cost"], c0, rtol=1e-5, atol=1e-9)
 
 
 # ----------------------------------------------------------------------------- the headline uneven batch

@@ -484,3 +484,28 @@ def test_threaded_oracle_is_bitwise_the_serial_one():
     x = O.lm_step(wl.spec, terms, qi, lam, p, rows=rows)
     y = O.lm_step(wl.spec, terms, qi, lam, p, rows=rows, threads=7)
     assert all(np.array_equal(x[k], y[k]) for k in x)
+
+
+def test_panda_tcp_hand_matches_the_reference_ground_truth():
+    """workloads.panda_tcp (the reference IK benchmark's Panda, fingers fixed, TCP link): its
+    panda_hand equals tests/robo/fk_ground_truth.json's panda_hand for every fixed-base case"""
+    spec = W.panda_tcp()
+    gt = json.loads((GOLDEN / "fk_ground_truth.json").read_text())
+    n = 0
+    for robot in gt["robots"]:
+        if robot["robot_description_name"] != "panda_description":
+            continue
+        for cfg in robot["configurations"]:
+            if cfg["T_world_base"] is not None:
+                continue
+            q = np.array(cfg["qpos"][:7], np.float32)
+            T, _ = O.fk(spec, q[None], dtype="f64")
+            ref = {lp["link_name"]: np.array(lp["xyz_wxyz"]) for lp in cfg["link_poses"]}
+            for name in ("panda_link7", "panda_link8", "panda_hand"):
+                if name not in spec.link_names:
+                    continue
+                t = T[0, spec.link_names.index(name)]
+                assert np.abs(t[:3] - ref[name][:3]).max() < 1e-5, name
+                assert min(np.abs(t[3:] - ref[name][3:]).max(), np.abs(t[3:] + ref[name][3:]).max()) < 1e-5, name
+                n += 1
+    assert n >= 4
