@@ -14,6 +14,7 @@ Prints ONE JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one GPU
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -26,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "converged IK solves/sec @1mm/0.01rad, batched+served vs plain; p50/p99 ms"
 ITERS = 50
+SERVE_STREAMS = 4  # the process's hardware queues (HIP's default); stream 0 carries the trajectory launches
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 
@@ -40,6 +42,22 @@ def f_model(na: int, nj: int, n_pose: int) -> float:
     """Whole-iteration model work besides F_LM (FK, motion subspace, pose residuals at the
     iterate and the proposal, Jacobian set-up and columns, limit rows, integrate) per problem."""
     return 93 * nj * 1 + 39 * nj + 2 * 200 * n_pose + 294 * n_pose + 123 * na * n_pose + 2 * 4 * na + 2 * na
+
+
+def f_lm_traj(T: int, na: int, n_spheres: int, w: int = 14) -> float:
+    """Algorithmic flops of one banded LM step of a trajectory (goal rows on the last frame, order-1
+    and order-2 finite differences, one collision row per sphere and frame; scalar half-bandwidth
+    w): sum over rows of nnz (nnz + 1) [J^T J] + 2 nnz [J^T r] + D w^2 [band Cholesky] + 4 D w
+    [band solves, damping]"""
+    rows = [(6, na), ((T - 1) * na, 2), ((T - 2) * na, 3), (T * n_spheres, na)]
+    D = T * na
+    return sum(r * z * (z + 1) for r, z in rows) + 2 * sum(r * z for r, z in rows) + D * w * w + 4 * D * w
+
+
+def f_model_traj(T: int, nj: int, n_spheres: int, n_prims: int) -> float:
+    """Model work of a trajectory iteration besides F_LM: FK and motion subspace per frame, and each
+    robot sphere's distance to every scene primitive (about 36 flops per sphere-primitive pair)"""
+    return T * (93 + 39) * nj + T * n_spheres * (36 * n_prims + 12)
 
 
 PROFILE = os.path.join(ROOT, "profiles", "k_persistent_uneven_ik_16384_counters.md")
@@ -105,7 +123,8 @@ def run_served(N, ctx, trace, params, iters, budget, streams):
     tid = serving.register(sv, trace)
     inputs = [serving.request_inputs(trace, i) for i in range(trace.size)]
     tq: "queue.Queue" = queue.Queue()
-    conv = {"ik": 0, "ik_problems": 0, "retarget_problems": 0, "traj": 0, "traj_problems": 0}
+    conv = {"ik": 0, "ik_problems": 0, "retarget_problems": 0, "traj": 0, "traj_problems": 0,
+            "latency_ms": {k: [] for k in trace.kinds}}
 
     def collect():
         while True:
@@ -114,6 +133,7 @@ def run_served(N, ctx, trace, params, iters, budget, streams):
                 return
             ticket, kind = item
             r = sv.poll(ticket)
+            conv["latency_ms"][kind].append(r["latency_ms"])
             if kind == "retarget":
                 conv["retarget_problems"] += len(r["cost"])
             elif kind == "trajopt":
@@ -185,7 +205,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--problems", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline leg")
-    ap.add_argument("--serve-seconds", type=float, default=10.0,
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the served Poisson trace (2000 requests/s); 0 skips the served leg")
     ap.add_argument("--serve-plain-seconds", type=float, default=0.5,
                     help="arrivals replayed through the plain per-request launch")
@@ -197,6 +217,11 @@ def main() -> int:
     ap.add_argument("--no-traj", dest="traj", action="store_false", help="skip the trajectory leg")
     ap.add_argument("--population-problems", type=int, default=2048,
                     help="Panda-DH problems of the population IK leg (0 skips it)")
+    ap.add_argument("--preset-problems", type=int, default=2000,
+                    help="Panda IK targets of the presets.basic leg (the reference's published batch; 0 skips it)")
+    ap.add_argument("--clips", type=int, default=16, help="retargeting clips solved side by side (0 skips the leg)")
+    ap.add_argument("--clip-frames", type=int, default=4096)
+    ap.add_argument("--cpu-threads", type=int, default=16, help="host threads of the all-core CPU baseline")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a streaming copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -446,21 +471,43 @@ def main() -> int:
         t_ms = [tbatch.solve(ITERS, params, from_seed=True) for _ in range(3)]
         pe, ae, sd = tbatch.metrics()
         ok = (pe < 1e-3) & (ae < 1e-2) & (sd >= 0)
+        tm = float(np.mean(t_ms))
+        nsph, nprim = len(tb.scene.sphere_radii), len(tb.scene.obstacle_spheres) + len(tb.scene.obstacle_boxes)
+        flm = f_lm_traj(64, 7, nsph)
+        fmod = f_model_traj(64, tb.spec.num_joints, nsph, nprim)
+        ach = 512 * ITERS * flm / (tm * 1e-3) / 1e12
         out["trajectory"] = {
             "workload": "512 trajectories x 64 frames, 7-DOF chain, 48 robot spheres vs 32 spheres + 8 boxes, "
                         f"goal + velocity/acceleration (w 1e-2) + collision hinge (w 20, margin 2 cm); {ITERS} iterations",
-            "ms_per_solve": round(float(np.mean(t_ms)), 3),
-            "trajectories_per_s": round(512 / (float(np.mean(t_ms)) * 1e-3), 1),
+            "ms_per_solve": round(tm, 3),
+            "trajectories_per_s": round(512 / (tm * 1e-3), 1),
             "goal_reached_and_collision_free": int(ok.sum()),
             "median_goal_error_m": round(float(np.median(pe)), 5),
             "median_min_clearance_m": round(float(np.median(sd)), 4),
+            "roofline": {"bound": "valu-fp32", "kernel": "k_traj<7, 9>", "kernel_ms": round(tm, 4),
+                         "flops_per_launch": 512 * ITERS * flm, "achieved": round(ach, 4), "peak": FP32_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 5),
+                         "achieved_with_model_work": round(512 * ITERS * (flm + fmod) / (tm * 1e-3) / 1e12, 4),
+                         "per_problem_iteration": {"f_lm": flm, "f_model": fmod}},
         }
+        if not args.no_cpu:  # the trajectory oracle (dense 448 x 448 solves) on a few trajectories
+            from oracle import oracle as O
+
+            m = 8
+            sub = dataclasses.replace(tb, q_seed=tb.q_seed[:m], goals=tb.goals[:m])
+            s0 = time.perf_counter()
+            O.traj_lm_solve(O.OracleTraj(sub), sub.q_seed, ITERS, O.LMParams(**params.__dict__))
+            t_c = time.perf_counter() - s0
+            out["trajectory"]["cpu_baseline"] = {
+                "value": round(m / t_c, 3), "unit": "trajectories/s", "cores": 1, "kind": "port",
+                "sample": f"the first {m} trajectories, {ITERS} iterations, {t_c:.1f} s single-threaded oracle "
+                          "(dense normal equations, the reference's generic step)"}
         print(f"[bench] trajectory: {out['trajectory']}", file=sys.stderr, flush=True)
 
     # ---- served: Poisson request stream through the continuous-batching server vs plain per request
     if args.serve_seconds > 0:
         trace = W.config4(seed=args.seed, rate=2000.0, seconds=args.serve_seconds)
-        st, conv = run_served(N, ctx, trace, params, ITERS, int(args.serve_budget_gb * (1 << 30)), streams=8)
+        st, conv = run_served(N, ctx, trace, params, ITERS, int(args.serve_budget_gb * (1 << 30)), streams=SERVE_STREAMS)
         served_value = conv["ik"] / st["span_s"]
         print(f"[bench] served: {st['completed']} requests in {st['span_s']:.2f} s, p50 {st['p50_ms']:.3f} ms "
               f"p99 {st['p99_ms']:.3f} ms, {st['mean_problems_per_launch']:.1f} problems/launch", file=sys.stderr,
@@ -470,13 +517,19 @@ def main() -> int:
         out["served"] = {
             "workload": f"Poisson 2000 requests/s for {args.serve_seconds:g} s, 1-256 problems per request "
                         f"(log-uniform), kinds {trace.kinds} uniform; {ITERS} iterations; "
-                        f"budget {args.serve_budget_gb:g} GB, 8 launch streams",
+                        f"budget {args.serve_budget_gb:g} GB, {SERVE_STREAMS} launch streams (one for trajectories)",
             "requests": st["completed"], "failed": st["failed"], "problems": st["problems"],
             "value": round(served_value, 1), "unit": "converged IK solves/s",
             "converged_ik": conv["ik"], "ik_problems": conv["ik_problems"],
             "retarget_frames": conv["retarget_problems"],
             "trajectories": conv["traj_problems"], "trajectories_goal_and_clear": conv["traj"],
             "p50_ms": round(st["p50_ms"], 4), "p99_ms": round(st["p99_ms"], 4), "max_ms": round(st["max_ms"], 4),
+            "latency_by_kind_ms": {k: {"requests": len(v), "p50": round(float(np.percentile(v, 50)), 4),
+                                       "p99": round(float(np.percentile(v, 99)), 4),
+                                       "max": round(float(np.max(v)), 4)}
+                                   for k, v in conv["latency_ms"].items() if v},
+            "ik_p99_ms": round(float(np.percentile(sum((conv["latency_ms"][k] for k in ("ik6", "ik7", "ik29")
+                                                        if k in conv["latency_ms"]), []), 99)), 4),
             "launches": st["launches"], "mean_problems_per_launch": round(st["mean_problems_per_launch"], 2),
             "peak_device_bytes": st["peak_bytes"], "span_s": round(st["span_s"], 3),
             "plain_per_request": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in plain_req.items()},
@@ -486,7 +539,8 @@ def main() -> int:
         if args.serve_capacity_seconds > 0:
             cap_trace = W.config4(seed=args.seed + 1, rate=args.serve_capacity_rate,
                                   seconds=args.serve_capacity_seconds, kinds=("ik6", "ik7", "ik29"))
-            st2, conv2 = run_served(N, ctx, cap_trace, params, ITERS, int(args.serve_budget_gb * (1 << 30)), streams=8)
+            st2, conv2 = run_served(N, ctx, cap_trace, params, ITERS, int(args.serve_budget_gb * (1 << 30)),
+                                    streams=SERVE_STREAMS)
             out["served"]["capacity"] = {
                 "workload": f"Poisson {args.serve_capacity_rate:g} IK requests/s for {args.serve_capacity_seconds:g} s "
                             f"(kinds {cap_trace.kinds}, 1-256 problems per request)",
@@ -536,8 +590,191 @@ def main() -> int:
                 "converged": ok_c,
                 "sample": f"first {pn_cpu} problems of the leg (x 64 candidates), {t_pc:.2f} s single-threaded oracle",
             }
+    # ---- the reference's published IK benchmark shape: IKSolver(presets.basic) on the Panda TCP link,
+    # batch 2000, 5 mm / 0.05 rad (helpers/ik/presets.py:13-18, scripts/benchmark/ik/helper.py:66-67)
+    if args.preset_problems > 0:
+        out["preset_basic"] = run_preset_leg(N, W, ctx, args)
+        print(f"[bench] presets.basic: {out['preset_basic']}", file=sys.stderr, flush=True)
+
+    # ---- the plain LM IK configuration as stated: 1024 Panda-DH problems, 50 iterations
+    out["config0"] = run_config0(N, W, ctx, params, args)
+    print(f"[bench] config0: {out['config0']}", file=sys.stderr, flush=True)
+
+    # ---- retargeting clips: each frame warm-started from the previous frame's solution on the device
+    if args.clips > 0:
+        out["retarget_clips"] = run_clips(N, W, ctx, params, args)
+        print(f"[bench] retarget clips: {out['retarget_clips']}", file=sys.stderr, flush=True)
+
+    # ---- the oracle on all host cores over the whole uneven batch
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        thr = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        t_all, conv_all = 0.0, 0
+        for bk in mb.buckets:
+            terms = O.ik_terms(bk.links, bk.targets, bk.spec.num_actuated_joints)
+            s0 = time.perf_counter()
+            res = O.lm_solve(bk.spec, terms, bk.q_seed, ITERS, O.LMParams(**params.__dict__), traces=False, threads=thr)
+            t_all += time.perf_counter() - s0
+            conv_all += int(O.ik_metrics(bk.spec, res["q"], bk.links, bk.targets)[2].sum())
+        out["cpu_baseline_all_cores"] = {
+            "value": round(conv_all / t_all, 2), "unit": "converged solves/s", "cores": thr, "kind": "port",
+            "converged": conv_all,
+            "sample": f"all {mb.size} problems of the uneven batch, {ITERS} iterations, {t_all:.1f} s on {thr} host "
+                      "threads (contiguous chunks of each bucket per thread)"}
+        print(f"[bench] cpu all cores: {out['cpu_baseline_all_cores']}", file=sys.stderr, flush=True)
     print(json.dumps(out))
     return 0
+
+
+def run_preset_leg(N, W, ctx, args):
+    """IKSolver(presets.basic): 64 Roberts candidates x 6 iterations (lambda 10), best 4 x 10 (lambda 1),
+    best 1; PositionTask w 20 + RotationTask w 10 + PositionLimit w 50 on panda_hand_tcp"""
+    from robokit_amd import ik as IK
+
+    spec = W.panda_tcp()
+    tcp = spec.link_names.index("panda_hand_tcp")
+    n = args.preset_problems
+    wl = W.ik_batch(spec, [tcp], n, seed=args.seed + 10)
+    solver = IK.IKSolver(IK.presets.basic, spec, tcp, ctx=ctx)
+    solver.solve_numpy(wl.targets)  # buffers and resident candidates
+    dev_ms, wall_ms = [], []
+    for _ in range(5):
+        ctx.sync()
+        s0 = time.perf_counter()
+        q = solver.solve_numpy(wl.targets)  # targets from host memory, winners back to host memory
+        wall_ms.append((time.perf_counter() - s0) * 1e3)
+        ctx.mark(2)
+        solver._pop.solve()  # the schedule alone, targets resident
+        ctx.mark(3)
+        dev_ms.append(ctx.elapsed(2, 3))
+    pe, ae, ok = solver.compute_metrics_numpy(q, wl.targets)
+    dm, wm = float(np.mean(dev_ms)), float(np.mean(wall_ms))
+    out = {
+        "workload": f"{n} Panda IK targets (panda_hand_tcp, FK of uniform in-limit joints), IKSolver(presets.basic): "
+                    "64 Roberts candidates x 6 LM iterations (lambda0 10), best 4 x 10 (lambda0 1), best 1; "
+                    "PositionTask w 20 + RotationTask w 10 + PositionLimit w 50",
+        "ms_per_solve_device": round(dm, 4),
+        "what_device": "HIP events around the schedule's two LM launches and two selects (targets resident)",
+        "ms_per_solve_wall": round(wm, 4),
+        "what_wall": "host clock around IKSolver.solve_numpy: targets uploaded from host memory, the schedule, "
+                     "the winners copied back",
+        "success_5mm_0.05rad": round(float(ok.mean()), 4),
+        "solves_per_s": round(n / (wm * 1e-3), 1),
+        "p98_position_mm": round(float(np.percentile(pe[ok].max(1), 98)) * 1e3, 4) if ok.any() else None,
+        "p98_orientation_rad": round(float(np.percentile(ae[ok].max(1), 98)), 5) if ok.any() else None,
+        "reference_published": {"ms": 7.124, "success": 0.9991, "hardware": "RTX 4090 + Xeon Gold 5120",
+                                "note": "context only: another GPU; the same preset, batch and thresholds"},
+    }
+    if not args.no_cpu:  # the oracle population on a sample of the targets
+        from oracle import oracle as O
+        from oracle import population as OP
+
+        m = min(256, n)
+        stages = [(st.iters, w, O.LMParams(**p.__dict__)) for st, w, p in
+                  zip(solver.config.solver.stages, solver.config.solver.num_candidates, solver.config.solver.optimizer)]
+        k = 1
+        fn = lambda inst: [O.Term(O.POSITION, [tcp], wl.targets[:m][inst], np.full(3 * k, 20.0, np.float32)),  # noqa: E731
+                           O.Term(O.ROTATION, [tcp], wl.targets[:m][inst], np.full(3 * k, 10.0, np.float32)),
+                           O.Term(O.LIMIT, (), None, np.full(7, 50.0, np.float32))]
+        s0 = time.perf_counter()
+        q_c, _, _ = OP.population_ik(spec, [tcp], wl.targets[:m], solver._pop.seeds, stages, terms_fn=fn)
+        t_c = time.perf_counter() - s0
+        ok_c = O.ik_metrics(spec, q_c, [tcp], wl.targets[:m], pos_thr=0.005, ang_thr=0.05)[2]
+        out["cpu_baseline"] = {"value": round(m / t_c, 2), "unit": "IK solves/s", "cores": 1, "kind": "port",
+                               "success_5mm_0.05rad": round(float(ok_c.mean()), 4),
+                               "sample": f"the first {m} targets x 64 candidates, {t_c:.2f} s single-threaded oracle"}
+    return out
+
+
+def run_config0(N, W, ctx, params, args):
+    """1024 Panda-DH problems, FrameTask(log) + PositionLimit, 50 iterations from uniform seeds: the
+    fused batch solve, the plain per-term launch and the oracle on the host"""
+    wl = W.config0(seed=args.seed, batch=1024)
+    ts = N.TermSpec(pose_links=wl.links, pose_weights=np.ones((1, 6), np.float32), limit_weights=np.ones(7, np.float32))
+    b = N.Batch(ctx, wl.spec, ts, wl.size)
+    b.set(q=wl.q_seed, pose_targets=wl.targets)
+    b.solve(ITERS, params, from_seed=True)
+    f_ms = [b.solve(ITERS, params, from_seed=True) for _ in range(5)]
+    ok = int(b.metrics()[2].sum())
+    p_ms = []
+    for _ in range(3):
+        ctx.sync()
+        s0 = time.perf_counter()
+        b.solve(ITERS, params, mode=N.MODE_PLAIN, from_seed=True)
+        p_ms.append((time.perf_counter() - s0) * 1e3)
+    ok_p = int(b.metrics()[2].sum())
+    fm, pm = float(np.mean(f_ms)), float(np.mean(p_ms))
+    out = {"workload": "1024 Panda-DH 7-DOF IK problems, FrameTask(log) + PositionLimit, 50 LM iterations, "
+                       "lambda0 1e-2 x10, 1 mm / 0.01 rad",
+           "fused_ms": round(fm, 4), "fused_converged_per_s": round(ok / (fm * 1e-3), 1), "converged": ok,
+           "plain_ms": round(pm, 4), "plain_converged_per_s": round(ok_p / (pm * 1e-3), 1)}
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        terms = O.ik_terms(wl.links, wl.targets, 7)
+        for thr in (1, max(1, min(args.cpu_threads, os.cpu_count() or 1))):
+            s0 = time.perf_counter()
+            r = O.lm_solve(wl.spec, terms, wl.q_seed, ITERS, O.LMParams(**params.__dict__), traces=False, threads=thr)
+            t_c = time.perf_counter() - s0
+            ok_c = int(O.ik_metrics(wl.spec, r["q"], wl.links, wl.targets)[2].sum())
+            out[f"cpu_{thr}_threads"] = {"value": round(ok_c / t_c, 1), "unit": "converged solves/s", "cores": thr,
+                                         "kind": "port", "converged": ok_c, "seconds": round(t_c, 3)}
+    return out
+
+
+def run_clips(N, W, ctx, params, args):
+    """Retargeting: `clips` independent 4096-frame clips of the 29-DOF humanoid side by side (one
+    problem per clip per frame), 12 link-position targets (FK + 5 mm noise) + SmoothnessTask to the
+    previous frame + PositionLimit.  Frame t starts from frame t-1's solution, which stays on the
+    device (rk_batch_advance); only the frame's targets are uploaded."""
+    from robokit_amd import serving
+    from robokit_amd.robo.kinematics_np import forward_kinematics
+
+    C, F, iters = args.clips, args.clip_frames, 10
+    clips = [W.config2(seed=args.seed + 100 + c, frames=F) for c in range(C)]
+    spec, links = clips[0].spec, clips[0].links
+    na = spec.num_actuated_joints
+    ts = N.TermSpec(pos_links=links, pos_weights=np.full((len(links), 3), serving.RT_POS_W, np.float32),
+                    limit_weights=np.full(na, serving.RT_LIMIT_W, np.float32),
+                    smooth_weights=np.full(na, serving.RT_SMOOTH_W, np.float32))
+    tg = np.stack([c.targets for c in clips], 1)  # [F, C, 12, 7]
+    q0 = np.stack([c.q_traj[0] for c in clips]).astype(np.float32)
+    b = N.Batch(ctx, spec, ts, C)
+    b.set(q=q0, q_prev=q0, pos_targets=tg[0])
+    lat, qs = [], np.zeros((F, C, na), np.float32)
+    s_all = time.perf_counter()
+    for t in range(F):
+        s0 = time.perf_counter()
+        if t:
+            b.advance()
+            b.set(pos_targets=tg[t])
+        b.solve(iters, params, from_seed=True)
+        qs[t] = b.get()["q"]
+        lat.append((time.perf_counter() - s0) * 1e3)
+    span = time.perf_counter() - s_all
+    # tracking on every 16th frame: the solution's link positions against the frame's noisy targets
+    # and against the noise-free motion
+    fr = np.arange(0, F, 16)
+    pos = forward_kinematics(spec, qs[fr])[..., links, :3]  # [f, C, 12, 3]
+    clean = np.stack([forward_kinematics(spec, c.q_traj[fr])[:, links, :3] for c in clips], 1)
+    e_tgt = np.linalg.norm(pos - tg[fr][..., :3], axis=-1)
+    e_clean = np.linalg.norm(pos - clean, axis=-1)
+    within = (e_tgt.max(-1) < 0.01)
+    return {"workload": f"{C} clips x {F} frames of the 29-DOF humanoid, 12 link-position targets (5 mm noise) + "
+                        f"SmoothnessTask (w {serving.RT_SMOOTH_W}) + PositionLimit; {iters} LM iterations per frame, "
+                        "each frame warm-started on the device from the previous one",
+            "frames_per_s": round(C * F / span, 1), "span_s": round(span, 3),
+            "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                                 "p99": round(float(np.percentile(lat, 99)), 4),
+                                 "what": "host clock per frame step of all clips: upload the frame's targets, the "
+                                         "device-resident warm start, one LM launch, the solutions back"},
+            "link_error_to_targets_m": {"median": round(float(np.median(e_tgt)), 5),
+                                        "p99": round(float(np.percentile(e_tgt, 99)), 5)},
+            "link_error_to_clean_motion_m": {"median": round(float(np.median(e_clean)), 5),
+                                             "p99": round(float(np.percentile(e_clean, 99)), 5)},
+            "frames_within_1cm": int(within.sum()), "frames_sampled": int(within.size),
+            "what_tracking": "every 16th frame of each clip: all 12 links within 1 cm of their noisy targets"}
 
 
 if __name__ == "__main__":

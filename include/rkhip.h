@@ -269,6 +269,10 @@ int rk_traj_metrics(rk_batch* b, float* pos_err, float* ang_err, float* min_sd);
  * LM path of opt/optimizer.py:216-240 concurrently.  A worker thread packs every queued request of
  * one term structure and iteration count into one launch whenever one of `streams` launch streams
  * is free and the `mem_budget`-byte device arena has room; launches on different streams overlap.
+ * Once trajectory terms are registered (and streams > 1), stream 0 carries trajectory launches and
+ * the others IK launches, and a trajectory launch is enqueued a few iterations at a time, so that an
+ * IK launch never waits behind a whole trajectory solve; `streams` is best kept within the
+ * process's hardware queues (4 by default).
  * Robots and term structures stay resident; a session keeps its last solution resident as the
  * next request's seeds and smoothness reference.  success uses pos_thr / ang_thr
  * (helpers/ik/ik.py:59-82) and needs pose targets.

@@ -740,12 +740,14 @@ __global__ __launch_bounds__(64) void k_fused(Problems P, LMParams lp, int it0, 
 //   V_T*:      one lane per problem, register-resident, serial arms with NA dofs
 //   V_W*:      one wavefront per problem, register-resident rows (trees, <= 64 joints, <= 32 dofs)
 //   V_H*:      two problems per wavefront, 32 lanes each, otherwise as V_W* (trees, <= 32 joints)
+//   V_P*:      as V_T*, with a position + rotation target pair instead of a pose target
 enum Variant {
-  V_S4 = 0, V_S8, V_S16, V_L16, V_L32, V_L64, V_T5, V_T6, V_T7, V_T8, V_W40, V_W012, V_H40, V_H012, NVARIANT
+  V_S4 = 0, V_S8, V_S16, V_L16, V_L32, V_L64, V_T5, V_T6, V_T7, V_T8, V_W40, V_W012, V_H40, V_H012,
+  V_P5, V_P6, V_P7, V_P8, NVARIANT
 };
 __host__ __device__ constexpr bool variant_wave(int v) { return v == V_W40 || v == V_W012; }
 __host__ __device__ constexpr bool variant_half(int v) { return v == V_H40 || v == V_H012; }
-__host__ __device__ constexpr bool variant_thread(int v) { return v >= V_T5 && v <= V_T8; }
+__host__ __device__ constexpr bool variant_thread(int v) { return (v >= V_T5 && v <= V_T8) || (v >= V_P5 && v <= V_P8); }
 __host__ __device__ constexpr int variant_G(int v) {
   return variant_thread(v) ? 1
          : variant_wave(v) ? 64
@@ -794,6 +796,10 @@ __device__ __forceinline__ void run_tile(const Bucket* bk, int local, const LMPa
   else if constexpr (V == V_T6) thread_solve<6, 8>(P, local * WAVE + threadIdx.x, lp, it0, iters);
   else if constexpr (V == V_T7) thread_solve<7, 9>(P, local * WAVE + threadIdx.x, lp, it0, iters);
   else if constexpr (V == V_T8) thread_solve<8, 10>(P, local * WAVE + threadIdx.x, lp, it0, iters);
+  else if constexpr (V == V_P5) thread_solve<5, 7, true>(P, local * WAVE + threadIdx.x, lp, it0, iters);
+  else if constexpr (V == V_P6) thread_solve<6, 8, true>(P, local * WAVE + threadIdx.x, lp, it0, iters);
+  else if constexpr (V == V_P7) thread_solve<7, 9, true>(P, local * WAVE + threadIdx.x, lp, it0, iters);
+  else if constexpr (V == V_P8) thread_solve<8, 10, true>(P, local * WAVE + threadIdx.x, lp, it0, iters);
   else if constexpr (V == V_W40) wave_solve<4, 0, 64>(P, local, lp, it0, iters, lds);
   else if constexpr (V == V_W012) wave_solve<0, 12, 64>(P, local, lp, it0, iters, lds);
   else if constexpr (V == V_H40) wave_solve<4, 0, 32>(P, local, lp, it0, iters, lds);
@@ -846,7 +852,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(variant_wave
 // kernel per variant on forked streams (the fork, the join and the streams' start skew cost
 // ~50 us per solve on the uneven benchmark).
 __host__ __device__ constexpr bool variant_mixable(int v) {
-  return (variant_thread(v) && v != V_T8) || variant_wave(v) || variant_half(v);
+  return (v >= V_T5 && v <= V_T7) || variant_wave(v) || variant_half(v);
 }
 constexpr size_t mix_lds() {
   size_t m = 16;
@@ -1613,7 +1619,7 @@ static int pick_variant(const rk_context* ctx, const Terms& th, int G, int* v) {
                                            : 64;
   }
   if (G == 1 && fits_thread(th, r)) {
-    *v = V_T5 + (r.na - 5);
+    *v = (th.n_pose == 0 && (th.n_pos || th.n_rot) ? V_P5 : V_T5) + (r.na - 5);
     return RK_OK;
   }
   if ((G == RK_TPP_WAVE || G == RK_TPP_HALF) && wave_kind(th, r, -G) >= 0) {
@@ -1674,6 +1680,10 @@ static int launch_variant(hipStream_t st, int v, const Problems& P, const LMPara
     case V_T6: hipLaunchKernelGGL((k_thread<6, 8>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_T7: hipLaunchKernelGGL((k_thread<7, 9>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_T8: hipLaunchKernelGGL((k_thread<8, 10>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_P5: hipLaunchKernelGGL((k_thread<5, 7, true>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_P6: hipLaunchKernelGGL((k_thread<6, 8, true>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_P7: hipLaunchKernelGGL((k_thread<7, 9, true>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_P8: hipLaunchKernelGGL((k_thread<8, 10, true>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_W40: hipLaunchKernelGGL((k_wave<4, 0, 64>), dim3(P.n), dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_W012: hipLaunchKernelGGL((k_wave<0, 12, 64>), dim3(P.n), dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_H40:
@@ -2762,6 +2772,10 @@ static int variant_grid_any(rk_context* ctx, int v, int wg_per_cu, size_t lds, i
     case V_W40: return variant_grid<V_W40>(ctx, wg_per_cu, lds, tiles, grid);
     case V_W012: return variant_grid<V_W012>(ctx, wg_per_cu, lds, tiles, grid);
     case V_H40: return variant_grid<V_H40>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_P5: return variant_grid<V_P5>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_P6: return variant_grid<V_P6>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_P7: return variant_grid<V_P7>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_P8: return variant_grid<V_P8>(ctx, wg_per_cu, lds, tiles, grid);
     default: return variant_grid<V_H012>(ctx, wg_per_cu, lds, tiles, grid);
   }
 }
@@ -2785,6 +2799,10 @@ static void launch_persistent(int v, dim3 grid, size_t lds, hipStream_t st, cons
     RK_LAUNCH_P(V_W012)
     RK_LAUNCH_P(V_H40)
     RK_LAUNCH_P(V_H012)
+    RK_LAUNCH_P(V_P5)
+    RK_LAUNCH_P(V_P6)
+    RK_LAUNCH_P(V_P7)
+    RK_LAUNCH_P(V_P8)
 #undef RK_LAUNCH_P
     default: break;
   }

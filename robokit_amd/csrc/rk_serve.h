@@ -10,8 +10,10 @@
 // converged/failed metrics, results copied back; the launch's arena block is released when its
 // event completes.  Requests arriving while launches are on the device therefore join the next
 // launch as soon as a stream and memory are free, and launches on different streams overlap on
-// the device.  A problem's result does not depend on what it was packed with: every problem is
-// solved independently by the same kernel variant (chosen per term structure).
+// the device.  With trajectory term structures registered, stream 0 carries the trajectory
+// launches, enqueued a chunk of iterations at a time, and the other streams the IK launches.  A
+// problem's result does not depend on what it was packed with: every problem is solved
+// independently by the same kernel variant (chosen per term structure).
 //
 // Warm start: a request may name a session.  The session keeps the last solution of its problems
 // resident in the arena; a later request of the session with no seeds starts from it and uses it
@@ -116,7 +118,17 @@ struct SrvLaunch {
   float *q = nullptr, *qs = nullptr, *pose = nullptr, *pos = nullptr, *qp = nullptr, *cost = nullptr,
         *lam = nullptr, *pe = nullptr, *ae = nullptr;
   int *nacc = nullptr, *ok = nullptr;
+  // a trajectory launch runs its iterations in chunks, each enqueued when the previous one's event
+  // completes: it_done iterations are on the device, iters in all; tail: the metrics and the
+  // download are enqueued (the next completion finishes the launch)
+  int it_done = 0, iters = 0;
+  bool tail = false;
 };
+
+// iterations per enqueued trajectory chunk: an IK launch whose stream shares a hardware queue with
+// the trajectory stream waits behind at most one chunk (~0.7 ms for 64-frame trajectories), not a
+// whole 50-iteration solve
+constexpr int SRV_TRAJ_CHUNK = 5;
 
 struct rk_server {
   rk_context* ctx = nullptr;
@@ -203,11 +215,78 @@ static bool srv_fits_stage(const rk_server* sv, const SrvTerms& t, int n) {
   return ib <= sizeof(float) * sv->stage_floats && ob <= sizeof(float) * sv->stage_floats;
 }
 
-// stage, launch and enqueue the copies of launch slot s (worker thread)
+// the problem view of launch slot s (SoA slices of its arena block, stride n)
+static Problems srv_problems(rk_server* sv, const SrvLaunch& L) {
+  const SrvTerms& T = sv->terms[L.term];
+  Problems P;
+  memset(&P, 0, sizeof(P));
+  P.robot = sv->ctx->robots_dev + T.th.robot;
+  P.terms = T.td;
+  P.q = L.q;
+  P.q_seed = L.qs;
+  P.pose_tgt = L.pose;
+  P.pos_tgt = L.pos;
+  P.q_prev = L.qp;
+  P.cost = L.cost;
+  P.lam = L.lam;
+  P.nacc = L.nacc;
+  P.n = L.n;
+  P.stride = L.n;
+  return P;
+}
+
+// the launch's results: converged / failed flags, the sessions' resident state, one download and
+// the completion event (worker thread)
+static int srv_enqueue_tail(rk_server* sv, int s) {
+  SrvLaunch& L = sv->launches[s];
+  const SrvTerms& T = sv->terms[L.term];
+  const int n = L.n, na = T.na, np = T.th.n_pose;
+  hipStream_t st = sv->streams[s];
+  const Problems P = srv_problems(sv, L);
+  if (T.traj_na) {
+    // lambda is not read back: its slice holds the clearance for the success flags
+    launch_traj_metrics(st, T.traj_na, T.tt, P, L.pe, L.ae, L.lam);
+    hipLaunchKernelGGL(k_srv_traj_ok, dim3((n + 127) / 128), dim3(128), 0, st, L.pe, L.ae, L.lam, L.ok, n,
+                       sv->pos_thr, sv->ang_thr);
+  } else if (np) {
+    hipLaunchKernelGGL(k_ik_metrics, dim3((n + 127) / 128), dim3(128), 0, st, P, sv->pos_thr, sv->ang_thr, L.pe,
+                       L.ae, L.ok);
+  }
+  // session state <- this solution
+  for (SrvReq* r : L.reqs) {
+    if (!r->session) continue;
+    const SrvSession& ss = sv->sessions[r->session];
+    float* dst = reinterpret_cast<float*>(sv->arena_base + ss.off);
+    const int tot = r->n * na, blk = 256;
+    hipLaunchKernelGGL(k_srv_scatter, dim3((tot + blk - 1) / blk), dim3(blk), 0, st, dst, L.q, n, r->first, r->n, na);
+  }
+  size_t in_bytes = 0, out_bytes = 0;
+  srv_layout(T, n, nullptr, nullptr, &in_bytes, &out_bytes);
+  HIP_TRY(hipMemcpyAsync(sv->h_out[s], L.q, out_bytes, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipEventRecord(sv->events[s], st));
+  HIP_TRY(hipGetLastError());
+  L.tail = true;
+  return RK_OK;
+}
+
+// the next chunk of a trajectory launch and its event, or, after the last one, the tail
+static int srv_traj_next(rk_server* sv, int s) {
+  SrvLaunch& L = sv->launches[s];
+  const SrvTerms& T = sv->terms[L.term];
+  if (L.it_done >= L.iters) return srv_enqueue_tail(sv, s);
+  const int n_it = std::min(SRV_TRAJ_CHUNK, L.iters - L.it_done);
+  int rc = launch_traj(sv->streams[s], T.traj_na, T.tt, srv_problems(sv, L), L.reqs[0]->lp, L.it_done, n_it);
+  if (rc) return rc;
+  L.it_done += n_it;
+  HIP_TRY(hipEventRecord(sv->events[s], sv->streams[s]));
+  return RK_OK;
+}
+
+// stage, launch and enqueue the copies of launch slot s (worker thread): an IK launch enqueues
+// every step at once; a trajectory launch its first chunk (srv_traj_next enqueues the rest)
 static int srv_start(rk_server* sv, int s) {
   SrvLaunch& L = sv->launches[s];
   const SrvTerms& T = sv->terms[L.term];
-  rk_context* ctx = sv->ctx;
   const int n = L.n, na = T.na, np = T.th.n_pose, nq = T.th.n_pos;
   hipStream_t st = sv->streams[s];
   char* base = sv->arena_base + L.off;
@@ -244,47 +323,14 @@ static int srv_start(rk_server* sv, int s) {
       hipLaunchKernelGGL(k_srv_gather, dim3((tot + blk - 1) / blk), dim3(blk), 0, st, L.qp, n, r->first, src, r->n,
                          na);
   }
-  Problems P;
-  memset(&P, 0, sizeof(P));
-  P.robot = ctx->robots_dev + T.th.robot;
-  P.terms = T.td;
-  P.q = L.q;
-  P.q_seed = L.qs;
-  P.pose_tgt = L.pose;
-  P.pos_tgt = L.pos;
-  P.q_prev = L.qp;
-  P.cost = L.cost;
-  P.lam = L.lam;
-  P.nacc = L.nacc;
-  P.n = n;
-  P.stride = n;
-  const LMParams lp = L.reqs[0]->lp;
-  if (T.traj_na) {
-    int rc = launch_traj(st, T.traj_na, T.tt, P, lp, 0, L.reqs[0]->iters);
-    if (rc) return rc;
-    // lambda is not read back: its slice holds the clearance for the success flags
-    launch_traj_metrics(st, T.traj_na, T.tt, P, L.pe, L.ae, L.lam);
-    hipLaunchKernelGGL(k_srv_traj_ok, dim3((n + 127) / 128), dim3(128), 0, st, L.pe, L.ae, L.lam, L.ok, n,
-                       sv->pos_thr, sv->ang_thr);
-  } else {
-    int rc = launch_variant(st, T.variant, P, lp, 0, L.reqs[0]->iters);
-    if (rc) return rc;
-    if (np)
-      hipLaunchKernelGGL(k_ik_metrics, dim3((n + 127) / 128), dim3(128), 0, st, P, sv->pos_thr, sv->ang_thr, L.pe,
-                         L.ae, L.ok);
-  }
-  // session state <- this solution
-  for (SrvReq* r : L.reqs) {
-    if (!r->session) continue;
-    const SrvSession& ss = sv->sessions[r->session];
-    float* dst = reinterpret_cast<float*>(sv->arena_base + ss.off);
-    const int tot = r->n * na, blk = 256;
-    hipLaunchKernelGGL(k_srv_scatter, dim3((tot + blk - 1) / blk), dim3(blk), 0, st, dst, L.q, n, r->first, r->n, na);
-  }
-  HIP_TRY(hipMemcpyAsync(sv->h_out[s], L.q, out_bytes, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipEventRecord(sv->events[s], st));
-  HIP_TRY(hipGetLastError());
-  return RK_OK;
+  L.iters = L.reqs[0]->iters;
+  L.it_done = 0;
+  L.tail = false;
+  if (T.traj_na) return srv_traj_next(sv, s);
+  int rc = launch_variant(st, T.variant, srv_problems(sv, L), L.reqs[0]->lp, 0, L.iters);
+  if (rc) return rc;
+  L.it_done = L.iters;
+  return srv_enqueue_tail(sv, s);
 }
 
 // results of a completed launch back into its requests (worker thread, server lock held)
@@ -374,24 +420,28 @@ static void srv_fail(rk_server* sv, SrvReq* r, const std::string& msg) {
   }
 }
 
-// form one launch on free stream slot s from the queue; false if nothing could start.
-// The head is the oldest ready request, except that at most a quarter of the streams (at least
-// one) carry trajectory launches at a time: a trajectory launch runs for tens of milliseconds
-// (one wavefront per trajectory through ~T * dofs serial band columns), an IK launch for well
-// under one, and with every stream holding a trajectory launch the IK requests would queue
-// behind them.
+// does the server hold trajectory term structures?  Then launch stream 0 carries trajectory
+// launches only and the other streams IK launches only: a trajectory solve runs for milliseconds
+// (one wavefront pair per trajectory through ~T * dofs serial band columns), an IK launch for well
+// under one, and IK requests must not queue behind it.  With one stream everything shares it.
+static bool srv_has_traj(const rk_server* sv) {
+  for (const auto& T : sv->terms)
+    if (T.traj_na) return true;
+  return false;
+}
+
+// form one launch on free stream slot s from the queue; false if nothing could start.  The head is
+// the oldest ready request of the kinds stream s carries.
 static bool srv_form(rk_server* sv, int s) {
-  int traj_active = 0;
-  for (const auto& L : sv->launches)
-    if (L.active && sv->terms[L.term].traj_na) ++traj_active;
-  const bool traj_ok = traj_active < std::max(1, sv->nstreams / 4);
+  const bool split = sv->nstreams > 1 && srv_has_traj(sv);
+  auto carries = [&](const SrvReq* r) { return !split || ((sv->terms[r->term].traj_na != 0) == (s == 0)); };
   // Requests of one session run in submission order: once the scan passes over a request of a
   // session (not ready, other key, no room), every later request of that session waits too.
   std::set<int64_t> passed;
   SrvReq* head = nullptr;
   for (SrvReq* r : sv->queue) {
     if (r->session && passed.count(r->session)) continue;
-    if ((traj_ok || !sv->terms[r->term].traj_na) && srv_ready(sv, r)) {
+    if (carries(r) && srv_ready(sv, r)) {
       head = r;
       break;
     }
@@ -479,7 +529,16 @@ static void srv_loop(rk_server* sv) {
       SrvLaunch& L = sv->launches[s];
       if (!L.active) continue;
       const hipError_t e = hipEventQuery(sv->events[s]);
-      if (e == hipSuccess) {
+      if (e == hipSuccess && !L.tail) {  // a trajectory chunk ended: the next one (or the tail)
+        if (srv_traj_next(sv, s)) {
+          hipStreamSynchronize(sv->streams[s]);
+          for (SrvReq* r : L.reqs) srv_fail(sv, r, g_err);
+          L.active = false;
+          L.reqs.clear();
+          sv->arena.release(L.off, L.bytes);
+        }
+        progress = true;
+      } else if (e == hipSuccess) {
         srv_finish(sv, s);
         progress = true;
       } else if (e != hipErrorNotReady) {

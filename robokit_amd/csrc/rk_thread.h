@@ -68,15 +68,17 @@ __device__ __forceinline__ Pose chain_fk(RobotK* rb, int nj, const float* q, int
 // p_link) and / or a RotationTask target (rows 3..5, r = w axis_angle(q_link q_tgt^-1),
 // rotation_task.py:323-349) on links of one parent joint -- the term pair of the IK presets.  In
 // that form tgt.t holds the position target and tgt.q the target rotation.
-template <int NA, bool JAC>
+template <int NA, bool JAC, bool PR>
 __device__ __forceinline__ void chain_rows(RobotK* rb, TermsK* tm, const Pose& Tf, const Pose& tgt, const float* q,
                                            const float* qprev, float (&r_pose)[6], float (&r_lim)[NA],
                                            float (&r_sm)[NA], float (&tg)[36], float (&jl)[NA], float (&js)[NA]) {
-  if (tm->n_pose) {
-    float w[6];
+  if constexpr (!PR) {
+    if (tm->n_pose) {
+      float w[6];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) w[i] = tm->pose_w[0][i];
-    pose_target(Tf, tgt, w, r_pose, JAC ? tg : nullptr);
+      for (int i = 0; i < 6; ++i) w[i] = tm->pose_w[0][i];
+      pose_target(Tf, tgt, w, r_pose, JAC ? tg : nullptr);
+    }
   } else {
 #pragma unroll
     for (int i = 0; i < 6; ++i) r_pose[i] = 0.f;
@@ -111,14 +113,16 @@ __device__ __forceinline__ void chain_rows(RobotK* rb, TermsK* tm, const Pose& T
 }
 
 // cost = sum over terms (pose, limit, smoothness) of 0.5 * sum of squares of the term's rows
-template <int NA>
+template <int NA, bool PR>
 __device__ __forceinline__ float chain_cost(TermsK* tm, const float* r_pose, const float* r_lim, const float* r_sm) {
   float cost = 0.f;
-  if (tm->n_pose) {
-    float s = 0.f;
+  if constexpr (!PR) {
+    if (tm->n_pose) {
+      float s = 0.f;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) s += r_pose[k] * r_pose[k];
-    cost = cost + 0.5f * s;
+      for (int k = 0; k < 6; ++k) s += r_pose[k] * r_pose[k];
+      cost = cost + 0.5f * s;
+    }
   } else {
     if (tm->n_pos) {
       float s = 0.f;
@@ -148,9 +152,10 @@ __device__ __forceinline__ float chain_cost(TermsK* tm, const float* r_pose, con
   return cost;
 }
 
-// One serial-arm problem per lane: <= 1 pose target, or <= 1 position and <= 1 rotation target on
-// links of one parent joint; limit and smoothness rows.
-template <int NA, int NJ>
+// One serial-arm problem per lane: <= 1 pose target (PR false), or <= 1 position and <= 1 rotation
+// target on links of one parent joint (PR true; the two forms are separate kernels so that each
+// keeps its own register allocation); limit and smoothness rows.
+template <int NA, int NJ, bool PR = false>
 __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, const LMParams& lp, int it0, int iters) {
   RobotK* rb = as_const(Pg.robot);
   TermsK* tm = as_const(Pg.terms);
@@ -158,7 +163,7 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
   const bool valid = s_idx < Pg.n;
   const int s = valid ? s_idx : Pg.n - 1;
   const int nj = rb->nj;
-  const int n_pose = tm->n_pose, n_pos = tm->n_pos, n_rot = tm->n_rot;
+  const int n_pose = PR ? 0 : tm->n_pose, n_pos = PR ? tm->n_pos : 0, n_rot = PR ? tm->n_rot : 0;
   const int jt = n_pose ? rb->link_joint[tm->pose_link[0]]
                  : n_pos ? rb->link_joint[tm->pos_link[0]]
                  : n_rot ? rb->link_joint[tm->rot_link[0]]
@@ -171,11 +176,13 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
     qprev[d] = tm->smooth ? Pg.q_prev[d * S + s] : 0.f;
   }
   Pose tgt = identity_pose();
-  if (n_pose) {
-    float v[7];
+  if constexpr (!PR) {
+    if (n_pose) {
+      float v[7];
 #pragma unroll
-    for (int c = 0; c < 7; ++c) v[c] = Pg.pose_tgt[c * S + s];
-    load_pose(v, tgt);
+      for (int c = 0; c < 7; ++c) v[c] = Pg.pose_tgt[c * S + s];
+      load_pose(v, tgt);
+    }
   } else {
     if (n_pos) {
 #pragma unroll
@@ -201,7 +208,7 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
     float S6[NA][6];
     const Pose Tf = chain_fk<NA, NJ, true>(rb, nj, q, jt, S6);
     float r_pose[6], r_lim[NA], r_sm[NA], jl[NA], js[NA], tg[36];
-    chain_rows<NA, true>(rb, tm, Tf, tgt, q, qprev, r_pose, r_lim, r_sm, tg, jl, js);
+    chain_rows<NA, true, PR>(rb, tm, Tf, tgt, q, qprev, r_pose, r_lim, r_sm, tg, jl, js);
     // ---- target-row Jacobian columns (zero for joints after the target link's joint): FrameTask
     // -w jlog Ad(T_f^-1) S; PositionTask -w (v + w x p_link) (position_task.py:253-262); RotationTask
     // w * (the column's angular part) (rotation_task.py:352-401)
@@ -210,8 +217,8 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
     for (int a = 0; a < NA; ++a) {
       float c6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       if (a <= jt) {
-        if (n_pose) {
-          pose_jac_column(tg, S6[a], c6);
+        if constexpr (!PR) {
+          if (n_pose) pose_jac_column(tg, S6[a], c6);
         } else {
           if (n_pos) {
             const float w3[3] = {tm->pos_w[0][0], tm->pos_w[0][1], tm->pos_w[0][2]};
@@ -233,12 +240,12 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
 #pragma unroll
       for (int j = 0; j <= i; ++j) {
         float h = 0.f;
-        if (n_pose && i <= jt) {
+        if (!PR && n_pose && i <= jt) {
           float sp = 0.f;
 #pragma unroll
           for (int r = 0; r < 6; ++r) sp += J[r][i] * J[r][j];
           h = sp;
-        } else if (i <= jt) {  // position term, then rotation term
+        } else if (PR && i <= jt) {  // position term, then rotation term
           if (n_pos) {
             float sp = 0.f;
 #pragma unroll
@@ -259,12 +266,12 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
         H[i][j] = h;
       }
       float gv = 0.f;
-      if (n_pose && i <= jt) {
+      if (!PR && n_pose && i <= jt) {
         float sp = 0.f;
 #pragma unroll
         for (int r = 0; r < 6; ++r) sp += J[r][i] * r_pose[r];
         gv = sp;
-      } else if (i <= jt) {
+      } else if (PR && i <= jt) {
         if (n_pos) {
           float sp = 0.f;
 #pragma unroll
@@ -282,7 +289,7 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
       if (tm->smooth) gv = gv + js[i] * r_sm[i];
       g[i] = gv;
     }
-    cost = chain_cost<NA>(tm, r_pose, r_lim, r_sm);
+    cost = chain_cost<NA, PR>(tm, r_pose, r_lim, r_sm);
     // ---- damped Cholesky (left-looking) and solves.  Each pivot's square root and reciprocal
     // are the hardware's (v_sqrt_f32, v_rcp_f32, 1 ulp) and the column scales by the reciprocal:
     // a correctly rounded division is a dependent chain of ~10 instructions, and this kernel is
@@ -328,8 +335,8 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
 #pragma unroll
     for (int d = 0; d < NA; ++d) qp[d] = q[d] + dq[d];
     const Pose Tfp = chain_fk<NA, NJ, false>(rb, nj, qp, jt, S6);
-    chain_rows<NA, false>(rb, tm, Tfp, tgt, qp, qprev, r_pose, r_lim, r_sm, tg, jl, js);
-    const float cp = chain_cost<NA>(tm, r_pose, r_lim, r_sm);
+    chain_rows<NA, false, PR>(rb, tm, Tfp, tgt, qp, qprev, r_pose, r_lim, r_sm, tg, jl, js);
+    const float cp = chain_cost<NA, PR>(tm, r_pose, r_lim, r_sm);
     if (valid && Pg.trace && gi == 0) Pg.trace[s] = cost;
     const bool acc = lm_accept(lp, cost, cp, pred, lam);
     if (acc) {
@@ -351,7 +358,7 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
   }
 }
 
-template <int NA, int NJ>
+template <int NA, int NJ, bool PR = false>
 __global__ __launch_bounds__(64) void k_thread(Problems P, LMParams lp, int it0, int iters) {
-  thread_solve<NA, NJ>(P, blockIdx.x * 64 + threadIdx.x, lp, it0, iters);
+  thread_solve<NA, NJ, PR>(P, blockIdx.x * 64 + threadIdx.x, lp, it0, iters);
 }
