@@ -234,6 +234,7 @@ int rk_group_kernel_times(rk_group* g, int32_t cap, int32_t* tpp, float* ms, int
  *   accel.     TrajectorySmoothnessTask order 2, weight / dt^2 (first window: no history, weight 0)
  *   collision  TrajectoryCollisionTask, primitive scene, penalty "plain", no sweep
  *              (terms/sparse/trajectory_collision_task.py:176-293): w * max(0, margin - sd)
+ *   bounds     TrajectoryLimitTask, forward stencils of orders 0..2 (optional, see below)
  * Banded normal equations (half-bandwidth 2 * dofs) solved per problem
  * (opt/lm_optimizer.py:81-136).  The batch then takes rk_batch_set (q [B][T][na], pose_targets =
  * goal [B][7]), rk_batch_solve (RK_MODE_FUSED), rk_batch_get, rk_batch_advance.
@@ -255,6 +256,15 @@ typedef struct {
     const float* obs_spheres;     /* [n][4] centre, radius */
     int32_t n_obs_boxes;
     const float* obs_boxes;       /* [n][6] centre, half extents */
+    /* TrajectoryLimitTask, forward stencils (terms/sparse/trajectory_limit_task.py:29-537): bit o of
+     * limit_orders adds one bound row per window of o + 1 frames and dof, order 0 position,
+     * 1 velocity (q[t+1] - q[t]) / dt, 2 acceleration (q[t] - 2 q[t+1] + q[t+2]) / dt^2;
+     * r = w max(lower - v, v - upper, 0).  limit_weights / limit_lower / limit_upper [3][na] (rows of
+     * absent orders unused; NULL when limit_orders = 0).  Rows after the collision rows. */
+    int32_t limit_orders;
+    const float* limit_weights;
+    const float* limit_lower;
+    const float* limit_upper;
 } rk_traj_desc;
 
 int rk_traj_batch_create(rk_context* ctx, const rk_traj_desc* desc, int32_t capacity, int32_t trace_iters,

@@ -322,7 +322,8 @@ class _Traj(C.Structure):
                 ("wv", C.c_float), ("wa", C.c_float), ("dt", C.c_float), ("nsph", C.c_int),
                 ("sph_link", C.c_void_p), ("sph_c", C.c_void_p), ("sph_r", C.c_void_p), ("coll_w", C.c_float),
                 ("margin", C.c_float), ("nobs_s", C.c_int), ("obs_s", C.c_void_p), ("nobs_b", C.c_int),
-                ("obs_b", C.c_void_p)]
+                ("obs_b", C.c_void_p), ("lim_orders", C.c_int), ("lim_w", C.c_void_p), ("lim_lo", C.c_void_p),
+                ("lim_hi", C.c_void_p)]
 
 
 class OracleTraj:
@@ -339,21 +340,29 @@ class OracleTraj:
             np.ascontiguousarray(sc.obstacle_spheres, np.float32).reshape(-1, 4),
             np.ascontiguousarray(sc.obstacle_boxes, np.float32).reshape(-1, 6),
         ]
+        na = tb.spec.num_actuated_joints
+        orders = int(getattr(tb, "lim_orders", 0) or 0)
+        lim = [np.ascontiguousarray(np.broadcast_to(np.asarray(getattr(tb, n), np.float32), (3, na)))
+               for n in ("lim_w", "lim_lo", "lim_hi")] if orders else [None, None, None]
+        self.keep += lim
         k = self.keep
         self.struct = _Traj(tb.T, tb.goal_link, _ptr(k[0]), _ptr(k[1]), tb.wv if wv is None else wv,
                             tb.wa if wa is None else wa, tb.dt, len(k[2]), _ptr(k[2]), _ptr(k[3]), _ptr(k[4]),
                             tb.coll_w if coll_w is None else coll_w, tb.margin if margin is None else margin,
-                            len(k[5]), _ptr(k[5]), len(k[6]), _ptr(k[6]))
+                            len(k[5]), _ptr(k[5]), len(k[6]), _ptr(k[6]), orders, _ptr(lim[0]), _ptr(lim[1]),
+                            _ptr(lim[2]))
         self.tb = tb
 
     def rows(self):
         tb, na = self.tb, self.tb.spec.num_actuated_joints
         s = self.struct
-        return 6 + (tb.T - 1) * na * (s.wv != 0) + (tb.T - 1) * na * (s.wa != 0) + tb.T * s.nsph
+        lim = sum((tb.T - o) * na for o in range(3) if (s.lim_orders >> o) & 1)
+        return 6 + (tb.T - 1) * na * (s.wv != 0) + (tb.T - 1) * na * (s.wa != 0) + tb.T * s.nsph + lim
 
 
 def traj_residual_jacobian(ot: OracleTraj, b: int, q: np.ndarray, dtype: str = "f32", jac: bool = True):
-    """r [R], dense J [R, T*na] (or None), H, g, cost_terms [4] of problem b at the trajectory q [T, na]."""
+    """r [R], dense J [R, T*na] (or None), H, g, cost_terms [7] (goal, velocity, acceleration, collision,
+    bounds of orders 0, 1, 2) of problem b at the trajectory q [T, na]."""
     lib, rt = _lib(dtype), _real(dtype)
     rb = OracleRobot(ot.tb.spec)
     D = ot.tb.T * ot.tb.spec.num_actuated_joints
@@ -363,7 +372,7 @@ def traj_residual_jacobian(ot: OracleTraj, b: int, q: np.ndarray, dtype: str = "
     J = np.zeros((R, D), rt) if jac else None
     H = np.zeros((D, D), rt)
     g = np.zeros(D, rt)
-    ct = np.zeros(4, rt)
+    ct = np.zeros(7, rt)
     lib.ork_traj_residual_jacobian(C.byref(rb.struct), C.byref(ot.struct), int(b), _ptr(q), _ptr(r), _ptr(J), _ptr(H),
                                    _ptr(g), _ptr(ct))
     return r, J, H, g, ct

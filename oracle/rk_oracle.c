@@ -978,14 +978,19 @@ typedef struct {
     const float *obs_s;        /* [n][4] centre, radius */
     int nobs_b;
     const float *obs_b;        /* [n][6] centre, half extents */
+    int lim_orders;            /* bit o: TrajectoryLimitTask rows of order o, forward stencil */
+    const float *lim_w, *lim_lo, *lim_hi; /* [3][na] */
 } ork_traj;
 
-static int traj_rows(const ork_robot *rb, const ork_traj *tj, int *Rg, int *Rv, int *Ra, int *Rc) {
+static int traj_rows(const ork_robot *rb, const ork_traj *tj, int *Rg, int *Rv, int *Ra, int *Rc, int *Rl) {
     *Rg = 6;
     *Rv = tj->wv != 0.f ? (tj->T - 1) * rb->na : 0;
     *Ra = tj->wa != 0.f ? (tj->T - 1) * rb->na : 0;
     *Rc = tj->nsph > 0 ? tj->T * tj->nsph : 0;
-    return *Rg + *Rv + *Ra + *Rc;
+    *Rl = 0;
+    for (int o = 0; o < 3; ++o)
+        if ((tj->lim_orders >> o) & 1) *Rl += (tj->T - o) * rb->na;
+    return *Rg + *Rv + *Ra + *Rc + *Rl;
 }
 
 /* scene SDF at p: signed distance and gradient direction */
@@ -1044,13 +1049,20 @@ typedef struct { int n; int col[8]; real val[8]; } traj_row;
  * All rows of problem b at the trajectory q[T*na]: r[R], the rows' sparse Jacobian entries
  * (optionally dense J[R][T*na]) and, accumulated row by row like the sparse path
  * (terms/task.py:373-400, _sparse_normal_equations_kernel :469-488), H and g.
- * cost_terms[4] = 1/2 sum of squares of the goal, velocity, acceleration, collision rows.
+ * cost_terms[7] = 1/2 sum of squares of the goal, velocity, acceleration, collision rows and of the
+ * bound rows of orders 0, 1, 2.
+ *
+ * Bound rows: TrajectoryLimitTask with the forward stencil (terms/sparse/trajectory_limit_task.py:
+ * 81-118 stencils, _limit_kernel :436-498): for order o, window f = 0..T-1-o and dof d,
+ * v = (sum_k c_k q[f+k][d]) / dt^o with c = (1), (-1, 1), (1, -2, 1) summed from k = 0,
+ * r = w max(max(lo - v, v - hi), 0), Jacobian w sign c_k / dt^o on an active bound (sign +1 above,
+ * -1 below), nothing otherwise.
  */
 EXPORT int ork_traj_residual_jacobian(const ork_robot *rb, const ork_traj *tj, int b, const real *q, real *r,
                                       real *J, real *H, real *g, real *cost_terms) {
     const int na = rb->na, T = tj->T, D = T * na;
-    int Rg, Rv, Ra, Rc;
-    const int R = traj_rows(rb, tj, &Rg, &Rv, &Ra, &Rc);
+    int Rg, Rv, Ra, Rc, Rl;
+    const int R = traj_rows(rb, tj, &Rg, &Rv, &Ra, &Rc, &Rl);
     real *Xl = (real *)calloc(7 * (rb->nj + 1), sizeof(real));
     real *Tj = (real *)calloc(7 * (rb->nj + 1), sizeof(real));
     real *Tl = (real *)calloc(7 * (rb->nl + 1), sizeof(real));
@@ -1059,7 +1071,7 @@ EXPORT int ork_traj_residual_jacobian(const ork_robot *rb, const ork_traj *tj, i
     if (J) memset(J, 0, sizeof(real) * (size_t)R * D);
     if (H) memset(H, 0, sizeof(real) * (size_t)D * D);
     if (g) memset(g, 0, sizeof(real) * D);
-    real ct[4] = {0, 0, 0, 0};
+    real ct[7] = {0, 0, 0, 0, 0, 0, 0};
     int row = 0;
     traj_row rw;
 #define EMIT(term, rv)                                                                        \
@@ -1155,17 +1167,50 @@ EXPORT int ork_traj_residual_jacobian(const ork_robot *rb, const ork_traj *tj, i
             }
         }
     }
+    /* bound rows */
+    for (int o = 0; o < 3 && Rl; ++o) {
+        if (!((tj->lim_orders >> o) & 1)) continue;
+        static const real cf[3][3] = {{1, 0, 0}, {-1, 1, 0}, {1, -2, 1}};
+        real sc = 1;
+        for (int k = 0; k < o; ++k) sc = sc * (real)tj->dt;
+        sc = (real)1 / sc;
+        for (int f = 0; f + o < T; ++f)
+            for (int d = 0; d < na; ++d) {
+                real v = 0;
+                for (int k = 0; k <= o; ++k) v = v + cf[o][k] * q[(size_t)(f + k) * na + d];
+                v = v * sc;
+                const real w = (real)tj->lim_w[o * na + d];
+                const real below = (real)tj->lim_lo[o * na + d] - v, above = v - (real)tj->lim_hi[o * na + d];
+                const real m = below > above ? below : above;
+                real res = 0;
+                rw.n = 0;
+                if (below > 0 || above > 0) {
+                    res = w * (m > 0 ? m : 0);
+                    const real sgn = above > 0 ? (real)1 : (real)-1;
+                    rw.n = o + 1;
+                    for (int k = 0; k <= o; ++k) {
+                        rw.col[k] = (f + k) * na + d;
+                        rw.val[k] = w * sgn * cf[o][k] * sc;
+                    }
+                }
+                EMIT(4 + o, res);
+            }
+    }
 #undef EMIT
-    for (int i = 0; i < 4; ++i) ct[i] = (real)0.5 * ct[i];
+    for (int i = 0; i < 7; ++i) ct[i] = (real)0.5 * ct[i];
     if (cost_terms) memcpy(cost_terms, ct, sizeof(ct));
     free(Xl); free(Tj); free(Tl); free(S); free(Jf);
     return R;
 }
 
+static real traj_cost_sum(const real *ct) {
+    return (((((ct[0] + ct[1]) + ct[2]) + ct[3]) + ct[4]) + ct[5]) + ct[6];
+}
+
 static real traj_cost(const ork_robot *rb, const ork_traj *tj, int b, const real *q, real *r) {
-    real ct[4];
+    real ct[7];
     ork_traj_residual_jacobian(rb, tj, b, q, r, NULL, NULL, NULL, ct);
-    return ((ct[0] + ct[1]) + ct[2]) + ct[3];
+    return traj_cost_sum(ct);
 }
 
 /* one LM step of trajectory problem b from (qb, lam); as the body of ork_lm_step */
@@ -1175,9 +1220,9 @@ static void traj_step_one(const ork_robot *rb, const ork_traj *tj, const ork_lm_
                           real lam, traj_ws *w, real *cost_cur, real *cost_prop, real *pred_out, int32_t *acc_out,
                           real *q_new, real *lam_new) {
     const int D = tj->T * rb->na;
-    real ct[4];
+    real ct[7];
     ork_traj_residual_jacobian(rb, tj, b, qb, w->r, NULL, w->H, w->g, ct);
-    real cost = ((ct[0] + ct[1]) + ct[2]) + ct[3];
+    real cost = traj_cost_sum(ct);
     *cost_cur = cost;
     real l = lam, pred;
     damped_cholesky(D, w->H, w->g, l, w->L, w->dl, &pred);
@@ -1194,8 +1239,8 @@ static void traj_step_one(const ork_robot *rb, const ork_traj *tj, const ork_lm_
 
 static void traj_ws_alloc(traj_ws *w, const ork_robot *rb, const ork_traj *tj) {
     const int D = tj->T * rb->na;
-    int Rg, Rv, Ra, Rc;
-    const int R = traj_rows(rb, tj, &Rg, &Rv, &Ra, &Rc);
+    int Rg, Rv, Ra, Rc, Rl;
+    const int R = traj_rows(rb, tj, &Rg, &Rv, &Ra, &Rc, &Rl);
     w->r = (real *)malloc(sizeof(real) * R);
     w->H = (real *)malloc(sizeof(real) * (size_t)D * D);
     w->L = (real *)malloc(sizeof(real) * (size_t)D * D);

@@ -1505,6 +1505,7 @@ struct rk_batch {
   int it_done = 0;  // iterations run since the last rk_batch_set / seeded solve (RK_CONTINUE offset)
   int traj_T = 0;   // > 0: a trajectory batch (na = traj_T * robot dofs), solved by k_traj
   int traj_na = 0;  // robot dofs of a trajectory batch
+  bool traj_lim = false;  // its problems carry bound rows (TrajectoryLimitTask)
   TrajTerms* tt = nullptr;
   float *q = nullptr, *q_seed = nullptr, *pose_tgt = nullptr, *pos_tgt = nullptr, *q_prev = nullptr;
   float* rot_tgt = nullptr;
@@ -1697,18 +1698,23 @@ static int launch_variant(hipStream_t st, int v, const Problems& P, const LMPara
   return RK_OK;
 }
 
-// trajectory kernel of an arm with `na` dofs (one wavefront per problem)
+// trajectory kernel of an arm with `na` dofs (two wavefronts per problem); lim: the problems carry
+// bound rows (the kernel instantiation with them)
 static int launch_traj(hipStream_t st, int na, const TrajTerms* tt, const Problems& P, const LMParams& lp, int it0,
-                       int iters) {
-  if (na == 6)
+                       int iters, bool lim) {
+  if (na == 6 && lim)
+    hipLaunchKernelGGL((k_traj<6, 8, true>), dim3(P.n), dim3(2 * WAVE), 0, st, P, tt, lp, it0, iters);
+  else if (na == 6)
     hipLaunchKernelGGL((k_traj<6, 8>), dim3(P.n), dim3(2 * WAVE), 0, st, P, tt, lp, it0, iters);
+  else if (lim)
+    hipLaunchKernelGGL((k_traj<7, 9, true>), dim3(P.n), dim3(2 * WAVE), 0, st, P, tt, lp, it0, iters);
   else
     hipLaunchKernelGGL((k_traj<7, 9>), dim3(P.n), dim3(2 * WAVE), 0, st, P, tt, lp, it0, iters);
   HIP_TRY(hipGetLastError());
   return RK_OK;
 }
 static int launch_traj(rk_batch* b, const Problems& P, const LMParams& lp, int it0, int iters) {
-  return launch_traj(b->ctx->stream, b->traj_na, b->tt, P, lp, it0, iters);
+  return launch_traj(b->ctx->stream, b->traj_na, b->tt, P, lp, it0, iters, b->traj_lim);
 }
 static void launch_traj_metrics(hipStream_t st, int na, const TrajTerms* tt, const Problems& P, float* pe, float* ae,
                                 float* sd) {
@@ -2319,6 +2325,22 @@ static int make_traj_terms(rk_context* ctx, const rk_traj_desc* d, TrajTerms* ou
     for (int c = 0; c < 4; ++c) tt.obs_s[i][c] = i < tt.ns ? d->obs_spheres[4 * i + c] : (c < 3 ? RK_SDF_FAR : 0.f);
   for (int i = 0; i < RK_SCENE_MAXB; ++i)
     for (int c = 0; c < 6; ++c) tt.obs_b[i][c] = i < tt.nb ? d->obs_boxes[6 * i + c] : (c < 3 ? RK_SDF_FAR : 0.f);
+  // bound rows (TrajectoryLimitTask, forward stencils of orders 0..2): weights and bounds per order and dof
+  if (d->limit_orders & ~7) return fail(RK_EINVAL, "limit_orders: bits 0 (position), 1 (velocity), 2 (acceleration)");
+  tt.lim_orders = d->limit_orders;
+  for (int o = 0; o < 3; ++o) {
+    tt.lim_scale[o] = 1.f / powf(d->dt, (float)o);
+    if (!((tt.lim_orders >> o) & 1)) continue;
+    if (!d->limit_weights || !d->limit_lower || !d->limit_upper)
+      return fail(RK_EINVAL, "limit_orders needs limit_weights, limit_lower and limit_upper");
+    if (d->frames <= o) return fail(RK_EINVAL, "a forward bound of order o needs more than o frames");
+    for (int k = 0; k < r.na; ++k) {
+      tt.lim_w[o][k] = d->limit_weights[o * r.na + k];
+      tt.lim_lo[o][k] = d->limit_lower[o * r.na + k];
+      tt.lim_hi[o][k] = d->limit_upper[o * r.na + k];
+      if (!(tt.lim_lo[o][k] <= tt.lim_hi[o][k])) return fail(RK_EINVAL, "limit bounds must be ordered, non-NaN");
+    }
+  }
   return RK_OK;
 }
 
@@ -2358,6 +2380,7 @@ int rk_traj_batch_create(rk_context* ctx, const rk_traj_desc* d, int32_t capacit
   b->na = D;
   b->traj_T = d->frames;
   b->traj_na = r.na;
+  b->traj_lim = tt.lim_orders != 0;
   const size_t S = capacity;
   auto A = [&](void** p, size_t bytes) { return dmalloc(ctx, p, bytes, &b->bytes); };
   if ((rc = A((void**)&b->tt, sizeof(TrajTerms))) || (rc = A((void**)&b->td, sizeof(Terms))) ||

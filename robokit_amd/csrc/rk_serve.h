@@ -78,6 +78,7 @@ struct SrvTerms {
   Terms* td = nullptr;
   int variant = 0, na = 0;  // na: variables per problem (frames * dofs for trajectories)
   int traj_na = 0;          // > 0: trajectory problems of an arm with this many dofs
+  bool traj_lim = false;    // they carry bound rows
   TrajTerms* tt = nullptr;
 };
 
@@ -275,7 +276,8 @@ static int srv_traj_next(rk_server* sv, int s) {
   const SrvTerms& T = sv->terms[L.term];
   if (L.it_done >= L.iters) return srv_enqueue_tail(sv, s);
   const int n_it = std::min(SRV_TRAJ_CHUNK, L.iters - L.it_done);
-  int rc = launch_traj(sv->streams[s], T.traj_na, T.tt, srv_problems(sv, L), L.reqs[0]->lp, L.it_done, n_it);
+  int rc = launch_traj(sv->streams[s], T.traj_na, T.tt, srv_problems(sv, L), L.reqs[0]->lp, L.it_done, n_it,
+                       T.traj_lim);
   if (rc) return rc;
   L.it_done += n_it;
   HIP_TRY(hipEventRecord(sv->events[s], sv->streams[s]));
@@ -647,6 +649,7 @@ int rk_server_add_traj(rk_server* sv, const rk_traj_desc* d, int32_t* term_id) {
     if ((rc = make_traj_terms(ctx, d, &tt))) return rc;
     T.th = traj_goal_terms(d, tt);
     T.traj_na = ctx->robots[d->robot].na;
+    T.traj_lim = tt.lim_orders != 0;
     T.na = d->frames * T.traj_na;
     if ((rc = dmalloc(ctx, (void**)&T.td, sizeof(Terms), nullptr))) return rc;
     if ((rc = dmalloc(ctx, (void**)&T.tt, sizeof(TrajTerms), nullptr))) return rc;

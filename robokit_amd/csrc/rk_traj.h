@@ -11,7 +11,9 @@
 // right-looking band Cholesky whose W+1 active rows live in registers of the top W+1 lanes,
 // shifted one lane per column; the forward substitution is fused into it and the backward one
 // runs the same way.  Rows and term costs in this order: goal, velocity, acceleration,
-// collision (the reference sums term costs in list order).
+// collision, then the bound rows of orders 0, 1, 2 (the reference sums term costs in list order).
+// The bound rows (TrajectoryLimitTask, forward stencil) are in a separate kernel instantiation
+// (LIM), so that trajectories without them keep their register allocation.
 
 #define RK_TRAJ_MAXT 64
 #define RK_TRAJ_MAXS 64  // robot collision spheres
@@ -31,6 +33,12 @@ struct TrajTerms {
   int ns, nb;
   float obs_s[RK_SCENE_MAXS][4];
   float obs_b[RK_SCENE_MAXB][6];
+  // TrajectoryLimitTask rows, forward stencil (terms/sparse/trajectory_limit_task.py:436-498):
+  // for order o (bit o of lim_orders) and window f = 0..T-1-o, per dof, v = lim_scale[o] *
+  // sum_k c_k q[f + k] with c = (1), (-1, 1), (1, -2, 1), r = w max(lo - v, v - hi, 0)
+  int lim_orders;
+  float lim_scale[3];  // 1 / dt^o
+  float lim_w[3][8], lim_lo[3][8], lim_hi[3][8];
 };
 typedef const __attribute__((address_space(4))) TrajTerms TrajK;
 
@@ -598,7 +606,59 @@ __device__ __forceinline__ void traj_smooth(TrajK* tt, int t, int T, const float
   }
 }
 
-template <int NA, int NJ>
+// bound rows of lane t: its own window f = t of each present order (q of frames t+1, t+2 from the
+// neighbouring lanes), the sums of squares cl[o]; with JAC the frame's J^T r and band entries of
+// every window that reaches frame t (windows t, t-1, t-2, read from the lanes that own them): the
+// diagonal hl0, the coupling to frame t-1 (hl1) and to frame t-2 (hl2), per dof
+template <int NA, bool JAC>
+__device__ __forceinline__ void traj_limits(TrajK* tt, int t, int T, const float (&q)[NA], float (&gf)[NA],
+                                            float (&hl0)[NA], float (&hl1)[NA], float (&hl2)[NA], float (&cl)[3]) {
+  const int l1 = t + 1 < 64 ? t + 1 : 63, l2 = t + 2 < 64 ? t + 2 : 63;
+  const int m1 = t >= 1 ? t - 1 : 0, m2 = t >= 2 ? t - 2 : 0;
+  const int orders = tt->lim_orders;
+#pragma unroll
+  for (int d = 0; d < NA; ++d) {
+    const float q1 = __shfl(q[d], l1), q2 = __shfl(q[d], l2);
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      if (!((orders >> o) & 1)) continue;  // wave-uniform
+      const float sc = tt->lim_scale[o], w = tt->lim_w[o][d];
+      const bool has = t + o < T;
+      float v = o == 0 ? q[d] : o == 1 ? -q[d] + q1 : (q[d] + -2.f * q1) + q2;
+      v = v * sc;
+      const float below = tt->lim_lo[o][d] - v, above = v - tt->lim_hi[o][d];
+      const bool act = has && (below > 0.f || above > 0.f);
+      const float rr = act ? w * fmaxf(fmaxf(below, above), 0.f) : 0.f;
+      cl[o] += rr * rr;
+      if (JAC) {
+        const float ws = act ? (above > 0.f ? w : -w) * sc : 0.f;  // w * sign * scale
+        // the window's Jacobian entries a_k = ws * c_k
+        const float a0 = o == 0 ? ws : o == 1 ? -ws : ws, a1 = o == 1 ? ws : -2.f * ws, a2 = ws;
+        gf[d] += a0 * rr;
+        hl0[d] += a0 * a0;
+        if (o >= 1) {  // window t-1: frame t is its entry 1
+          const float pa0 = __shfl(a0, m1), pa1 = __shfl(a1, m1), pr = __shfl(rr, m1);
+          if (t >= 1) {
+            gf[d] += pa1 * pr;
+            hl0[d] += pa1 * pa1;
+            hl1[d] += pa1 * pa0;
+          }
+        }
+        if (o == 2) {  // window t-2: frame t is its entry 2
+          const float qa0 = __shfl(a0, m2), qa1 = __shfl(a1, m2), qa2 = __shfl(a2, m2), qr = __shfl(rr, m2);
+          if (t >= 2) {
+            gf[d] += qa2 * qr;
+            hl0[d] += qa2 * qa2;
+            hl1[d] += qa2 * qa1;
+            hl2[d] += qa2 * qa0;
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int NA, int NJ, bool LIM = false>
 __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s, const LMParams& lp, int it0, int iters,
                                            TrajLDS<NA>* sh) {
   constexpr int W = TrajLDS<NA>::W;
@@ -630,16 +690,30 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
   const float hs0 = wv2 * (float)(in_v(t) + in_v(t - 1)) + wa2 * (1.f * in_a(t) + 4.f * in_a(t - 1) + 1.f * in_a(t - 2));
   const float hs1m = wv2 * (in_v(t - 1) ? -1.f : 0.f) + wa2 * (-2.f * in_a(t - 1) - 2.f * in_a(t - 2));  // (t, t-1)
   const float hs2m = wa2 * (float)in_a(t - 2);                                                            // (t, t-2)
+  // bound-row terms after the four others, order by order
+  auto limit_cost = [&](float c, const float (&cl)[3]) {
+    if constexpr (LIM) {
+#pragma unroll
+      for (int o = 0; o < 3; ++o)
+        if ((tt->lim_orders >> o) & 1) c = c + 0.5f * wv_sum(own ? cl[o] : 0.f);
+    }
+    return c;
+  };
   auto eval_cost = [&](const float (&qq)[NA]) {
     float hb[NA][NA], gf[NA];
     float cg = 0.f, cc = 0.f, cv = 0.f, ca = 0.f;
     traj_frame<NA, NJ, false, 0>(rb, tt, qq, last, goal, hb, gf, cg, cc);
     traj_smooth<NA, false>(tt, t, T, qq, gf, cv, ca);
+    float cl[3] = {0.f, 0.f, 0.f};
+    if constexpr (LIM) {
+      float h0[NA], h1[NA], h2[NA];
+      traj_limits<NA, false>(tt, t, T, qq, gf, h0, h1, h2, cl);
+    }
     __syncthreads();  // the second wavefront's collision cost (odd spheres)
     cc += sh->pcc[lane];
     if (!own) cg = cc = cv = ca = 0.f;
     const float c0 = 0.5f * wv_sum(cg), c1 = 0.5f * wv_sum(cv), c2 = 0.5f * wv_sum(ca), c3 = 0.5f * wv_sum(cc);
-    return ((c0 + c1) + c2) + c3;
+    return limit_cost(((c0 + c1) + c2) + c3, cl);
   };
   // rows D..D+W of the band and the right-hand side are zero: they enter the Cholesky window past the end
   for (int e = lane; e < (W + 2) * TrajLDS<NA>::BS; e += 64) sh->band[D + e / TrajLDS<NA>::BS][e % TrajLDS<NA>::BS] = 0.f;
@@ -650,6 +724,9 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
   // iterate, hence these, unchanged, so they are rebuilt only after an accepted step (both
   // wavefronts skip the rows together: the decision is published with the iterate)
   float hb[NA][NA], gf[NA];
+  float hl0[NA], hl1[NA], hl2[NA];  // bound rows' band entries (LIM)
+#pragma unroll
+  for (int d = 0; d < NA; ++d) hl0[d] = hl1[d] = hl2[d] = 0.f;
   bool moved = true;
   for (int it = 0; it < iters; ++it) {
     const int gi = it0 + it;
@@ -670,6 +747,12 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
       float cg = 0.f, cc = 0.f, cv = 0.f, ca = 0.f;
       traj_frame<NA, NJ, true, 0>(rb, tt, q, last, goal, hb, gf, cg, cc);
       traj_smooth<NA, true>(tt, t, T, q, gf, cv, ca);
+      float cl[3] = {0.f, 0.f, 0.f};
+      if constexpr (LIM) {
+#pragma unroll
+        for (int d = 0; d < NA; ++d) hl0[d] = hl1[d] = hl2[d] = 0.f;
+        traj_limits<NA, true>(tt, t, T, q, gf, hl0, hl1, hl2, cl);
+      }
       __syncthreads();  // the second wavefront's rows (odd spheres)
       {
         int e = 0;
@@ -682,7 +765,7 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
         cc += sh->part[e][lane];
       }
       if (!own) cg = cc = cv = ca = 0.f;
-      cost = ((0.5f * wv_sum(cg) + 0.5f * wv_sum(cv)) + 0.5f * wv_sum(ca)) + 0.5f * wv_sum(cc);
+      cost = limit_cost(((0.5f * wv_sum(cg) + 0.5f * wv_sum(cv)) + 0.5f * wv_sum(ca)) + 0.5f * wv_sum(cc), cl);
     }
     // ---- band of J^T J + lam I (rows of frame t), right-hand side J^T r
     if (own) {
@@ -693,9 +776,9 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
         for (int kk = 0; kk <= W; ++kk) {
           float v = 0.f;
           if (kk <= i) v = hb[i][i - kk];
-          if (kk == 0) v = v + hs0 + lam;
-          if (kk == NA) v = hs1m;
-          if (kk == 2 * NA) v = hs2m;
+          if (kk == 0) v = LIM ? v + hs0 + hl0[i] + lam : v + hs0 + lam;
+          if (kk == NA) v = LIM ? hs1m + hl1[i] : hs1m;
+          if (kk == 2 * NA) v = LIM ? hs2m + hl2[i] : hs2m;
           sh->band[row][kk] = v;
         }
         sh->rhs[row] = -gf[i];
@@ -807,12 +890,12 @@ __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s
   }
 }
 
-template <int NA, int NJ>
+template <int NA, int NJ, bool LIM = false>
 __global__ __launch_bounds__(128) void k_traj(Problems P, const TrajTerms* tt, LMParams lp, int it0, int iters) {
   __shared__ __align__(16) TrajLDS<NA> sh;
   if ((int)blockIdx.x >= P.n) return;
   if (threadIdx.x < 64)
-    traj_solve<NA, NJ>(P, as_const(tt), blockIdx.x, lp, it0, iters, &sh);
+    traj_solve<NA, NJ, LIM>(P, as_const(tt), blockIdx.x, lp, it0, iters, &sh);
   else
     traj_worker<NA, NJ>(P, as_const(tt), blockIdx.x, it0, iters, &sh);
 }

@@ -62,7 +62,29 @@ class TrajDescC(C.Structure):
                 ("vel_weight", C.c_float), ("acc_weight", C.c_float), ("dt", C.c_float), ("n_spheres", C.c_int32),
                 ("sphere_links", C.c_void_p), ("sphere_centers", C.c_void_p), ("sphere_radii", C.c_void_p),
                 ("coll_weight", C.c_float), ("coll_margin", C.c_float), ("n_obs_spheres", C.c_int32),
-                ("obs_spheres", C.c_void_p), ("n_obs_boxes", C.c_int32), ("obs_boxes", C.c_void_p)]
+                ("obs_spheres", C.c_void_p), ("n_obs_boxes", C.c_int32), ("obs_boxes", C.c_void_p),
+                ("limit_orders", C.c_int32), ("limit_weights", C.c_void_p), ("limit_lower", C.c_void_p),
+                ("limit_upper", C.c_void_p)]
+
+
+def traj_desc(robot_id: int, tb, goal_weights=None):
+    """(rk_traj_desc, arrays it points at) of a workloads.TrajBatch-shaped problem set: robot,
+    frames, scene, weights, and the optional bound rows (lim_orders bit o, lim_w / lim_lo / lim_hi
+    [3, na])"""
+    sc = tb.scene
+    na = tb.spec.num_actuated_joints
+    arrs = [_f32(np.ones(6) if goal_weights is None else goal_weights), _i32(sc.sphere_links),
+            _f32(sc.sphere_centers), _f32(sc.sphere_radii), _f32(np.asarray(sc.obstacle_spheres).reshape(-1, 4)),
+            _f32(np.asarray(sc.obstacle_boxes).reshape(-1, 6))]
+    orders = int(getattr(tb, "lim_orders", 0) or 0)
+    lim = [None, None, None]
+    if orders:
+        lim = [_f32(np.broadcast_to(np.asarray(getattr(tb, k), np.float32), (3, na))) for k in ("lim_w", "lim_lo", "lim_hi")]
+    arrs += lim
+    d = TrajDescC(robot_id, tb.T, tb.goal_link, _p(arrs[0]), tb.wv, tb.wa, tb.dt, len(arrs[1]), _p(arrs[1]),
+                  _p(arrs[2]), _p(arrs[3]), tb.coll_w, tb.margin, len(arrs[4]), _p(arrs[4]), len(arrs[5]), _p(arrs[5]),
+                  orders, _p(lim[0]), _p(lim[1]), _p(lim[2]))
+    return d, arrs
 
 
 class RequestC(C.Structure):
@@ -503,13 +525,7 @@ class Server:
 
     def add_traj(self, tb, goal_weights=None) -> int:
         """register a trajectory problem set (workloads.TrajBatch shape: robot, frames, scene)"""
-        sc = tb.scene
-        arrs = [_f32(np.ones(6) if goal_weights is None else goal_weights), _i32(sc.sphere_links),
-                _f32(sc.sphere_centers), _f32(sc.sphere_radii), _f32(np.asarray(sc.obstacle_spheres).reshape(-1, 4)),
-                _f32(np.asarray(sc.obstacle_boxes).reshape(-1, 6))]
-        d = TrajDescC(self.ctx.robot(tb.spec), tb.T, tb.goal_link, _p(arrs[0]), tb.wv, tb.wa, tb.dt, len(arrs[1]),
-                      _p(arrs[1]), _p(arrs[2]), _p(arrs[3]), tb.coll_w, tb.margin, len(arrs[4]), _p(arrs[4]),
-                      len(arrs[5]), _p(arrs[5]))
+        d, arrs = traj_desc(self.ctx.robot(tb.spec), tb, goal_weights)
         tid = C.c_int32()
         _check(lib().rk_server_add_traj(self._h, C.byref(d), C.byref(tid)))
         self._terms.append((tb.T * tb.spec.num_actuated_joints, 1, 0))
@@ -572,15 +588,7 @@ class TrajBatch(Batch):
         self._na = self.T * self.robot_na
         self.terms = TermSpec(pose_links=[tb.goal_link], pose_weights=np.ones((1, 6), np.float32))
         self.capacity, self.trace_iters = int(capacity), int(trace_iters)
-        sc = tb.scene
-        self._arrs = [
-            _f32(np.ones(6) if goal_weights is None else goal_weights),
-            _i32(sc.sphere_links), _f32(sc.sphere_centers), _f32(sc.sphere_radii),
-            _f32(np.asarray(sc.obstacle_spheres).reshape(-1, 4)), _f32(np.asarray(sc.obstacle_boxes).reshape(-1, 6)),
-        ]
-        a = self._arrs
-        d = TrajDescC(ctx.robot(tb.spec), tb.T, tb.goal_link, _p(a[0]), tb.wv, tb.wa, tb.dt, len(a[1]), _p(a[1]),
-                      _p(a[2]), _p(a[3]), tb.coll_w, tb.margin, len(a[4]), _p(a[4]), len(a[5]), _p(a[5]))
+        d, self._arrs = traj_desc(ctx.robot(tb.spec), tb, goal_weights)
         self._h = C.c_void_p()
         _check(lib().rk_traj_batch_create(ctx.handle, C.byref(d), self.capacity, self.trace_iters, C.byref(self._h)))
         _LIVE["batch"].add(self)

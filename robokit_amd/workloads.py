@@ -205,15 +205,16 @@ class RetargetClip:
 
 
 def config2(seed: int = 0, frames: int = 4096, humanoid_seed: int = 0) -> RetargetClip:
-    """q_j(t) = sum_{k=1..5} 0.5 sin(2 pi f_k t + phi_k), f ~ U(0.1, 2) Hz, phi ~ U(0, 2 pi)
-    (amplitude 0.5 rad per sinusoid), 30 fps; targets = FK positions + N(0, (5 mm)^2)."""
+    """q_j(t) = sum_{k=1..5} 0.1 sin(2 pi f_k t + phi_k), f ~ U(0.1, 2) Hz, phi ~ U(0, 2 pi): a total
+    amplitude of 0.5 rad shared by the five sinusoids (at most ~0.16 rad per frame at 30 fps; 0.5 rad
+    each would move joints up to ~1 rad per frame), 30 fps; targets = FK positions + N(0, (5 mm)^2)."""
     spec = humanoid29(humanoid_seed)
     r_f, r_p, r_n = _streams(seed, 3)
     na = spec.num_actuated_joints
     f = r_f.uniform(0.1, 2.0, size=(na, 5))
     ph = r_p.uniform(0.0, 2 * PI, size=(na, 5))
     t = np.arange(frames) / 30.0
-    q = (0.5 * np.sin(2 * PI * f[None] * t[:, None, None] + ph[None])).sum(-1)
+    q = (0.1 * np.sin(2 * PI * f[None] * t[:, None, None] + ph[None])).sum(-1)
     q = np.clip(q, -LIMIT, LIMIT)
     links = humanoid_position_links(spec)
     P = forward_kinematics(spec, q)[:, links, :3]
@@ -298,6 +299,12 @@ class TrajBatch:
     dt: float = 1.0
     coll_w: float = 20.0
     margin: float = 0.02
+    # TrajectoryLimitTask bound rows (forward stencils): bit o of lim_orders for order o (0 position,
+    # 1 velocity, 2 acceleration); weight / lower / upper per order and dof [3, na]
+    lim_orders: int = 0
+    lim_w: Optional[np.ndarray] = None
+    lim_lo: Optional[np.ndarray] = None
+    lim_hi: Optional[np.ndarray] = None
 
     @property
     def size(self) -> int:

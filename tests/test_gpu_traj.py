@@ -84,3 +84,71 @@ def test_traj_lm_improves_clearance(ctx):
     print(f"\nmin clearance median {np.median(sd0):.3f} -> {np.median(sd):.3f} m, goal error median "
           f"{np.median(pe0):.3f} -> {np.median(pe):.4f} m")
     assert np.median(sd) > np.median(sd0) and np.median(pe) < np.median(pe0)
+
+
+def _bounded(tb, orders=0b111, w=(3.0, 2.0, 1.5), half=(1.2, 0.15, 0.05)):
+    """TrajectoryLimitTask rows (forward stencils) of every order, tight enough to be active"""
+    import dataclasses
+
+    na = tb.spec.num_actuated_joints
+    lo = np.stack([np.full(na, -half[0]), np.full(na, -half[1]), np.full(na, -half[2])]).astype(np.float32)
+    wts = np.stack([np.full(na, wi) for wi in w]).astype(np.float32)
+    return dataclasses.replace(tb, lim_orders=orders, lim_w=wts, lim_lo=lo, lim_hi=-lo)
+
+
+# bound rows of orders 0, 1, 2 together and alone; the 6-dof arm; a 64-frame problem whose band
+# solve is split over both wavefronts
+@pytest.mark.parametrize("T,B,iters,arm,orders,half", [(16, 16, 15, "panda", 0b111, (1.2, 0.15, 0.05)),
+                                                       (16, 12, 12, "panda", 0b001, (1.2, 0.15, 0.05)),
+                                                       (7, 12, 12, "panda", 0b100, (1.2, 0.15, 0.001)),
+                                                       (16, 12, 12, "ur10", 0b110, (1.2, 0.02, 0.001)),
+                                                       (64, 6, 10, "panda", 0b111, (1.2, 0.15, 0.05))])
+def test_traj_bound_rows_parity(ctx, T, B, iters, arm, orders, half):
+    """TrajectoryLimitTask bound rows in the trajectory kernel (its LIM instantiation): every GPU step
+    shadow-checked against the trajectory oracle's rows (trajectory_limit_task.py:436-498), the seed
+    cost to 1e-5 relative of FP64; continued halves equal one launch bitwise"""
+    tb = _bounded(W.config3(seed=9, batch=B, T=T, arm=arm), orders=orders, half=half)
+    b = N.TrajBatch(ctx, tb, B, trace_iters=iters)
+    b.set(q=tb.q_seed, goals=tb.goals)
+    b.solve(iters, P)
+    gpu = b.get(traces=True)
+    q_it, lam_it = b.iterates()
+    ot = O.OracleTraj(tb)
+    rows = np.repeat(np.arange(B), iters)
+    qi = q_it.reshape(B * iters, T, -1)
+    p = O.LMParams(**P.__dict__)
+    rep = shadow_check(gpu["cost_trace"], gpu["accept"], O.traj_lm_step(ot, qi, lam_it.reshape(-1), p, rows=rows),
+                       O.traj_lm_step(ot, qi, lam_it.reshape(-1), p, dtype="f64", rows=rows))
+    ct = np.array([O.traj_residual_jacobian(ot, i, tb.q_seed[i], dtype="f64", jac=False)[4] for i in range(B)])
+    cf = np.array([O.traj_residual_jacobian(ot, i, gpu["q"][i], dtype="f64", jac=False)[4] for i in range(B)])
+    share = max(ct[:, 4:].sum() / ct.sum(), cf[:, 4:].sum() / cf.sum())
+    print(f"\nbounds {arm} T={T} orders {orders:03b}: {rep}; bound cost share at the seeds "
+          f"{ct[:, 4:].sum() / ct.sum():.3f}, at the solutions {cf[:, 4:].sum() / cf.sum():.3f}")
+    assert rep.ok(), rep
+    assert share > 0.005  # the bound rows carry weight on the path: the check is not vacuous
+    np.testing.assert_allclose(gpu["cost_trace"][:, 0], ct.sum(1), rtol=1e-5)
+    half = iters // 2
+    b.set(q=tb.q_seed, goals=tb.goals)
+    b.solve(half, P)
+    b.solve(iters - half, P, cont=True)
+    assert np.array_equal(b.get()["q"], gpu["q"])
+
+
+def test_traj_bounds_keep_trajectories_inside(ctx):
+    """position bounds (a hinge of weight 50) on the benchmark's trajectory shape: the unbounded
+    solve leaves |q| <= 1 far behind while every frame of every bounded solution stays within the
+    bound up to the soft hinge's balance against the goal rows (5 %)"""
+    tb = W.config3(seed=10, batch=32, T=64)
+    na = tb.spec.num_actuated_joints
+    bnd = _bounded(tb, orders=0b001, w=(50.0, 0.0, 0.0), half=(1.0, 0.0, 0.0))
+    q0 = np.clip(tb.q_seed, -1.0, 1.0)
+    free = N.TrajBatch(ctx, tb, 32)
+    free.set(q=q0, goals=tb.goals)
+    free.solve(50, P)
+    bb = N.TrajBatch(ctx, bnd, 32)
+    bb.set(q=q0, goals=tb.goals)
+    bb.solve(50, P)
+    qf, qb = free.get()["q"], bb.get()["q"]
+    print(f"\nmax |q| unbounded {np.abs(qf).max():.3f}, bounded {np.abs(qb).max():.3f} (bound 1.0, {na} dofs)")
+    assert np.abs(qf).max() > 1.5
+    assert np.abs(qb).max() < 1.05

@@ -10,6 +10,7 @@ sphere-SDF collision hinge), before it checks the GPU kernel.
   LM step        a step from (q, lam) equals the first iteration of the solve
 """
 import copy
+import dataclasses
 
 import numpy as np
 import pytest
@@ -68,8 +69,9 @@ def test_traj_rows_closed_forms():
     e = O.se3("log", O.se3("compose", O.se3("inverse", Tl[0, tb.goal_link][None], dtype="f64"),
                            tb.goals[0][None].astype(np.float64), dtype="f64"), dtype="f64")[0]
     np.testing.assert_allclose(r[:6], e, atol=1e-12)
-    np.testing.assert_allclose(ct, 0.5 * np.array([r[:6] @ r[:6], rv.ravel() @ rv.ravel(), ra.ravel() @ ra.ravel(),
-                                                   rc.ravel() @ rc.ravel()]), rtol=1e-12)
+    np.testing.assert_allclose(ct[:4], 0.5 * np.array([r[:6] @ r[:6], rv.ravel() @ rv.ravel(), ra.ravel() @ ra.ravel(),
+                                                       rc.ravel() @ rc.ravel()]), rtol=1e-12)
+    assert np.all(ct[4:] == 0)  # no bound rows
     np.testing.assert_allclose(H, J.T @ J, rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(g, J.T @ r, rtol=1e-12, atol=1e-12)
 
@@ -125,3 +127,77 @@ def test_traj_lm_reduces_cost_and_reaches_goal_without_obstacles():
     assert np.all(np.diff(sol["cost_trace"], axis=1) <= 1e-12)
     pe, ae, sd = O.traj_metrics(ot, sol["q"], dtype="f64")
     assert np.mean((pe < 1e-3) & (ae < 1e-2)) >= 0.8, (pe, ae)
+
+
+def _with_bounds(tb, orders=0b111, w=(3.0, 2.0, 1.5), half=(1.2, 0.15, 0.05)):
+    """bound rows of every order: position within +-half[0] of the mid-range, velocity and
+    acceleration within +-half[1], +-half[2] (per dt, dt^2), weights per order"""
+    na = tb.spec.num_actuated_joints
+    lo = np.stack([np.full(na, -half[0]), np.full(na, -half[1]), np.full(na, -half[2])]).astype(np.float32)
+    wts = np.stack([np.full(na, wi) for wi in w]).astype(np.float32)
+    return dataclasses.replace(tb, lim_orders=orders, lim_w=wts, lim_lo=lo, lim_hi=-lo)
+
+
+def test_traj_bound_rows_closed_forms():
+    """TrajectoryLimitTask, forward stencil (trajectory_limit_task.py:436-498): row (o, f, d) is
+    w max(lo - v, v - hi, 0) with v = (q[f], q[f+1] - q[f], q[f] - 2 q[f+1] + q[f+2]) / dt^o; the
+    rows follow the collision rows, order by order, frame-major; their cost terms are separate"""
+    tb = _with_bounds(dataclasses.replace(_small(T=6, seed=5), dt=0.5))
+    ot = O.OracleTraj(tb)
+    na, T = tb.spec.num_actuated_joints, tb.T
+    q = (tb.q_seed[0] + np.random.default_rng(3).normal(0, 0.6, tb.q_seed[0].shape)).astype(np.float64)
+    r, J, H, g, ct = O.traj_residual_jacobian(ot, 0, q, dtype="f64")
+    nb = T * na + (T - 1) * na + (T - 2) * na
+    rb = r[-nb:]
+    vals = [q, (q[1:] - q[:-1]) / tb.dt, (q[:-2] - 2 * q[1:-1] + q[2:]) / tb.dt ** 2]
+    off = 0
+    for o, v in enumerate(vals):
+        lo, hi, w = tb.lim_lo[o], tb.lim_hi[o], tb.lim_w[o]
+        expect = w * np.maximum(np.maximum(lo - v, v - hi), 0)
+        got = rb[off:off + v.size].reshape(v.shape)
+        np.testing.assert_allclose(got, expect, atol=1e-12)
+        assert ct[4 + o] == pytest.approx(0.5 * np.sum(expect ** 2), rel=1e-12)
+        off += v.size
+    assert np.count_nonzero(rb) > 10  # the bounds are active somewhere: the check is not vacuous
+    np.testing.assert_allclose(H, J.T @ J, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(g, J.T @ r, rtol=1e-12, atol=1e-12)
+    # inside every bound the rows vanish
+    wide = _with_bounds(tb, half=(10.0, 100.0, 1000.0))
+    r2 = O.traj_residual_jacobian(O.OracleTraj(wide), 0, q, dtype="f64")[0]
+    assert np.all(r2[-nb:] == 0)
+
+
+def test_traj_bound_rows_jacobian_finite_differences():
+    tb = _with_bounds(_small(T=5, seed=6))
+    ot = O.OracleTraj(tb)
+    na, T = tb.spec.num_actuated_joints, tb.T
+    nb = T * na + (T - 1) * na + (T - 2) * na
+    q = (tb.q_seed[1] + np.random.default_rng(4).normal(0, 0.6, tb.q_seed[1].shape)).astype(np.float64).reshape(-1)
+    r, J, *_ = O.traj_residual_jacobian(ot, 1, q, dtype="f64")
+    h = 1e-7
+    Jfd = np.zeros((nb, q.size))
+    for a in range(q.size):
+        qp, qm = q.copy(), q.copy()
+        qp[a] += h
+        qm[a] -= h
+        Jfd[:, a] = (O.traj_residual_jacobian(ot, 1, qp, dtype="f64", jac=False)[0][-nb:]
+                     - O.traj_residual_jacobian(ot, 1, qm, dtype="f64", jac=False)[0][-nb:]) / (2 * h)
+    active = np.abs(r[-nb:]) > 1e-4  # away from the hinge kinks
+    assert active.sum() > 10
+    np.testing.assert_allclose(J[-nb:][active], Jfd[active], atol=1e-6)
+
+
+def test_traj_bounds_pull_a_trajectory_inside():
+    """seeds far outside a position bound: the LM with the bound rows brings the frames inside
+    (the FP64 oracle, no obstacles)"""
+    tb = copy.copy(_small(T=8, seed=7, batch=3))
+    tb.scene = copy.copy(tb.scene)
+    tb.scene.obstacle_spheres = np.zeros((0, 4), np.float32)
+    tb.scene.obstacle_boxes = np.zeros((0, 6), np.float32)
+    tb = _with_bounds(tb, orders=0b001, w=(50.0, 0.0, 0.0), half=(0.6, 0.0, 0.0))
+    ot = O.OracleTraj(tb)
+    p = O.LMParams(lm_lambda=1e-2, lambda_factor=10.0)
+    q0 = np.clip(tb.q_seed * 3.0, -2.8, 2.8).astype(np.float32)
+    assert np.abs(q0).max() > 1.0
+    sol = O.traj_lm_solve(ot, q0, 30, p, dtype="f64")
+    assert np.abs(sol["q"]).max() < 0.6 + 5e-3
