@@ -101,6 +101,7 @@ int rk_se3_op(rk_context* ctx, int32_t op, int32_t n, const float* a, const floa
  *   RotationTask         (terms/dense/rotation_task.py:323-401)   3 rows per rotation target
  *   PositionLimit "abs"  (terms/dense/position_limit.py:508-573)  na rows
  *   SmoothnessTask       (terms/dense/smoothness_task.py:350-422) na rows
+ *   SceneCollisionTask   (terms/dense/scene_collision_task.py:276-908) n_coll rows
  * Limits: n_pose <= 4, n_pos <= 12, n_pose + n_rot <= 4.  Batches with rotation targets run the
  * LDS-staged fused kernels (threads_per_problem 4..64), or, for a serial arm with one position and
  * one rotation target on links of one parent joint (the IK presets' term pair), the one-lane-per-
@@ -121,6 +122,22 @@ typedef struct {
     int32_t n_rot;               /* RotationTask (terms/dense/rotation_task.py:323-401), rows after the position rows */
     const int32_t* rot_links;
     const float* rot_weights;    /* [n_rot][3] */
+    /* SceneCollisionTask (terms/dense/scene_collision_task.py:276-908), rows after the smoothness
+     * rows: one per robot sphere, r = w pen(sd) with sd = the primitive scene's signed distance at
+     * the sphere centre minus its radius (geom/sdf_kernels.py:256-295) and the penalty of
+     * terms/sphere_collision_kernels.py:19-42: 0 smooth (the reference default), 1 plain, 2 surface
+     * distance.  Runs the one-lane-per-problem register kernel of 6- and 7-dof serial arms. */
+    int32_t n_coll;              /* robot spheres, <= 64 (0: no collision term) */
+    const int32_t* coll_links;   /* [n_coll] */
+    const float* coll_centers;   /* [n_coll][3] link frame */
+    const float* coll_radii;     /* [n_coll] */
+    const float* coll_weights;   /* [n_coll] (NULL: ones) */
+    float coll_margin;
+    int32_t coll_penalty;
+    int32_t n_obs_spheres;       /* <= 64 */
+    const float* obs_spheres;    /* [n][4] centre, radius */
+    int32_t n_obs_boxes;         /* <= 16 */
+    const float* obs_boxes;      /* [n][6] centre, half extents (axis-aligned) */
 } rk_terms_desc;
 
 /* LMOptimizerConfig fields (opt/lm_optimizer.py:502-523) */

@@ -20,7 +20,8 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 _LIBS = {}
 
-FRAME, POSITION, LIMIT, SMOOTH, ROTATION = 0, 1, 2, 3, 4
+FRAME, POSITION, LIMIT, SMOOTH, ROTATION, COLLISION = 0, 1, 2, 3, 4, 5
+PENALTY = {"smooth": 0, "plain": 1, "surface_distance": 2}
 
 
 def build() -> None:
@@ -49,8 +50,31 @@ class _Robot(C.Structure):
 class _Term(C.Structure):
     _fields_ = [
         ("kind", C.c_int), ("nframes", C.c_int), ("links", C.c_void_p), ("targets", C.c_void_p),
-        ("weights", C.c_void_p), ("q_prev", C.c_void_p),
+        ("weights", C.c_void_p), ("q_prev", C.c_void_p), ("scene", C.c_void_p),
     ]
+
+
+class _Scene(C.Structure):
+    _fields_ = [
+        ("nsph", C.c_int), ("sph_link", C.c_void_p), ("sph_c", C.c_void_p), ("sph_r", C.c_void_p),
+        ("sph_w", C.c_void_p), ("margin", C.c_float), ("penalty", C.c_int), ("nobs_s", C.c_int),
+        ("obs_s", C.c_void_p), ("nobs_b", C.c_int), ("obs_b", C.c_void_p),
+    ]
+
+
+@dataclass
+class CollisionScene:
+    """SceneCollisionTask data of an oracle COLLISION term: robot spheres (link, centre in the link
+    frame, radius, row weight) against obstacle spheres [n, 4] and axis-aligned boxes [n, 6]."""
+
+    sphere_links: np.ndarray
+    sphere_centers: np.ndarray
+    sphere_radii: np.ndarray
+    weights: np.ndarray
+    margin: float = 0.02
+    penalty: str = "smooth"
+    obstacle_spheres: np.ndarray = None
+    obstacle_boxes: np.ndarray = None
 
 
 class _LMParams(C.Structure):
@@ -92,6 +116,7 @@ class Term:
     targets: Optional[np.ndarray] = None  # FRAME [B, k, 7] poses; POSITION [B, k, 3] points (or poses)
     weights: Optional[np.ndarray] = None
     q_prev: Optional[np.ndarray] = None  # [B, na]
+    scene: Optional[CollisionScene] = None  # COLLISION
 
 
 def _terms(terms: Sequence[Term], keep: list):
@@ -105,10 +130,24 @@ def _terms(terms: Sequence[Term], keep: list):
             tg = np.ascontiguousarray(np.concatenate([tg, pad], -1))
         if tg is not None and tg.shape[-1] != 7:
             raise ValueError(f"targets must end in 3 or 7 floats, got {tg.shape}")
-        w = np.ascontiguousarray(t.weights, np.float32)
+        w = None if t.weights is None else np.ascontiguousarray(t.weights, np.float32)
         qp = None if t.q_prev is None else np.ascontiguousarray(t.q_prev, np.float32)
         keep.extend([links, tg, w, qp])
-        arr[i] = _Term(t.kind, len(links), _ptr(links), _ptr(tg), _ptr(w), _ptr(qp))
+        sc_ptr = None
+        if t.kind == COLLISION:
+            sc = t.scene
+            a = [np.ascontiguousarray(sc.sphere_links, np.int32), np.ascontiguousarray(sc.sphere_centers, np.float32),
+                 np.ascontiguousarray(sc.sphere_radii, np.float32),
+                 np.ascontiguousarray(np.broadcast_to(np.asarray(sc.weights, np.float32), (len(sc.sphere_links),))),
+                 np.ascontiguousarray(np.zeros((0, 4)) if sc.obstacle_spheres is None else sc.obstacle_spheres,
+                                      np.float32).reshape(-1, 4),
+                 np.ascontiguousarray(np.zeros((0, 6)) if sc.obstacle_boxes is None else sc.obstacle_boxes,
+                                      np.float32).reshape(-1, 6)]
+            st = _Scene(len(a[0]), _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(a[3]), float(sc.margin),
+                        PENALTY[sc.penalty], len(a[4]), _ptr(a[4]), len(a[5]), _ptr(a[5]))
+            keep.extend(a + [st])
+            sc_ptr = C.cast(C.pointer(st), C.c_void_p)
+        arr[i] = _Term(t.kind, len(links), _ptr(links), _ptr(tg), _ptr(w), _ptr(qp), sc_ptr)
     return arr
 
 
@@ -162,8 +201,8 @@ def residual_jacobian(spec, terms: Sequence[Term], b: int, q: np.ndarray, dtype:
     keep: list = []
     ta = _terms(terms, keep)
     na = spec.num_actuated_joints
-    R = sum(6 * len(t.links) if t.kind == FRAME else 3 * len(t.links) if t.kind in (POSITION, ROTATION) else na
-            for t in terms)
+    R = sum(6 * len(t.links) if t.kind == FRAME else 3 * len(t.links) if t.kind in (POSITION, ROTATION)
+            else len(t.scene.sphere_links) if t.kind == COLLISION else na for t in terms)
     q = np.ascontiguousarray(q, rt)
     r = np.zeros(R, rt)
     J = np.zeros((R, na), rt)
@@ -218,7 +257,7 @@ def accept(cost, cost_prop, pred, lam, params: LMParams, dtype: str = "f32"):
 def _slice_terms(terms: Sequence[Term], lo: int, hi: int) -> List[Term]:
     """the terms of problems lo..hi-1 (per-problem targets and smoothness references sliced)"""
     return [Term(t.kind, t.links, None if t.targets is None else np.asarray(t.targets)[lo:hi], t.weights,
-                 None if t.q_prev is None else np.asarray(t.q_prev)[lo:hi]) for t in terms]
+                 None if t.q_prev is None else np.asarray(t.q_prev)[lo:hi], t.scene) for t in terms]
 
 
 def _chunks(B: int, threads: int):
@@ -274,7 +313,7 @@ def lm_step(spec, terms: Sequence[Term], q: np.ndarray, lam: np.ndarray, params:
     B = q.shape[0]
     if rows is not None:
         terms = [Term(t.kind, t.links, None if t.targets is None else np.asarray(t.targets)[rows], t.weights,
-                      None if t.q_prev is None else np.asarray(t.q_prev)[rows]) for t in terms]
+                      None if t.q_prev is None else np.asarray(t.q_prev)[rows], t.scene) for t in terms]
     if threads > 1:
         lam_a = np.asarray(lam)
         return _parallel(B, threads, lambda lo, hi: lm_step(spec, _slice_terms(terms, lo, hi), q[lo:hi], lam_a[lo:hi],

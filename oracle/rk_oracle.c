@@ -524,10 +524,75 @@ EXPORT void ork_link_jacobian(const ork_robot *rb, int B, const real *q, const r
     free(S);
 }
 
+/* primitive scene SDF at p (geom/sdf_kernels.py:223-295): the minimum over the obstacle spheres,
+ * then the axis-aligned boxes, the first strict minimum kept; n = its gradient direction */
+static real prims_sdf(int nobs_s, const float *obs_s, int nobs_b, const float *obs_b, const real *p, real *n) {
+    real best = (real)1e6;
+    n[0] = 1; n[1] = 0; n[2] = 0;
+    for (int i = 0; i < nobs_s; ++i) {
+        const float *o = obs_s + 4 * i;
+        real d[3] = {p[0] - (real)o[0], p[1] - (real)o[1], p[2] - (real)o[2]};
+        real len = RSQRT(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        real dist = len - (real)o[3];
+        if (dist < best) {
+            best = dist;
+            if (len > (real)1e-10) { n[0] = d[0] / len; n[1] = d[1] / len; n[2] = d[2] / len; }
+            else { n[0] = 1; n[1] = 0; n[2] = 0; }
+        }
+    }
+    for (int i = 0; i < nobs_b; ++i) {
+        const float *o = obs_b + 6 * i;
+        real ql[3], qv[3], h[3];
+        for (int k = 0; k < 3; ++k) {
+            ql[k] = p[k] - (real)o[k];
+            h[k] = (real)o[3 + k];
+            qv[k] = RFABS(ql[k]) - h[k];
+        }
+        real qmax = qv[0] > qv[1] ? qv[0] : qv[1];
+        if (qv[2] > qmax) qmax = qv[2];
+        real qp[3] = {qv[0] > 0 ? qv[0] : 0, qv[1] > 0 ? qv[1] : 0, qv[2] > 0 ? qv[2] : 0};
+        real dist = RSQRT(qp[0] * qp[0] + qp[1] * qp[1] + qp[2] * qp[2]) + (qmax < 0 ? qmax : 0);
+        if (dist < best) {
+            best = dist;
+            real nn[3] = {0, 0, 1};
+            if (qmax < 0) {
+                if (qv[0] >= qv[1] && qv[0] >= qv[2]) { nn[0] = ql[0] < 0 ? -1 : 1; nn[2] = 0; }
+                else if (qv[1] >= qv[2]) { nn[1] = ql[1] < 0 ? -1 : 1; nn[2] = 0; }
+                else { nn[2] = ql[2] < 0 ? -1 : 1; }
+            } else {
+                real df[3];
+                for (int k = 0; k < 3; ++k) {
+                    real c = ql[k] < -h[k] ? -h[k] : (ql[k] > h[k] ? h[k] : ql[k]);
+                    df[k] = ql[k] - c;
+                }
+                real dl = RSQRT(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
+                if (dl > (real)1e-10) { nn[0] = df[0] / dl; nn[1] = df[1] / dl; nn[2] = df[2] / dl; }
+            }
+            n[0] = nn[0]; n[1] = nn[1]; n[2] = nn[2];
+        }
+    }
+    return best;
+}
+
 /* ------------------------------------------------------------------------- */
 /* residual terms                                                            */
 /* ------------------------------------------------------------------------- */
-enum { ORK_FRAME = 0, ORK_POSITION = 1, ORK_LIMIT = 2, ORK_SMOOTH = 3, ORK_ROTATION = 4 };
+enum { ORK_FRAME = 0, ORK_POSITION = 1, ORK_LIMIT = 2, ORK_SMOOTH = 3, ORK_ROTATION = 4, ORK_COLLISION = 5 };
+
+/* SceneCollisionTask data: the robot's spheres and a primitive scene */
+typedef struct {
+    int nsph;
+    const int32_t *sph_link;  /* [nsph] */
+    const float *sph_c;       /* [nsph][3] link frame */
+    const float *sph_r;       /* [nsph] */
+    const float *sph_w;       /* [nsph] row weights */
+    float margin;
+    int penalty;              /* 0 smooth, 1 plain, 2 surface distance */
+    int nobs_s;
+    const float *obs_s;       /* [n][4] */
+    int nobs_b;
+    const float *obs_b;       /* [n][6] */
+} ork_scene;
 
 typedef struct {
     int kind;
@@ -536,6 +601,7 @@ typedef struct {
     const float *targets;   /* [B][nframes][7] world targets (one per problem) */
     const float *weights;   /* FRAME: 6*nframes; POSITION: 3*nframes; LIMIT/SMOOTH: na */
     const float *q_prev;    /* SMOOTH: [B][na] */
+    const ork_scene *scene; /* COLLISION */
 } ork_term;
 
 static int term_rows(const ork_robot *rb, const ork_term *t) {
@@ -543,7 +609,30 @@ static int term_rows(const ork_robot *rb, const ork_term *t) {
     case ORK_FRAME: return 6 * t->nframes;
     case ORK_POSITION: return 3 * t->nframes;
     case ORK_ROTATION: return 3 * t->nframes;
+    case ORK_COLLISION: return t->scene->nsph;
     default: return rb->na;
+    }
+}
+
+/* _compute_collision_penalty_func (terms/sphere_collision_kernels.py:19-42): the residual of a
+ * signed distance and its derivative; smooth (0), plain (1), surface distance (2) */
+static void coll_penalty(real sd, real margin, int mode, real *pen, real *dpen) {
+    if (mode == 2) {
+        *pen = sd < 0 ? -sd : sd;
+        *dpen = sd < 0 ? (real)-1 : (real)1;
+    } else if (mode == 1) {
+        *pen = sd < margin ? margin - sd : (real)0;
+        *dpen = sd < margin ? (real)-1 : (real)0;
+    } else if (sd < 0) {
+        *pen = (real)0.5 * margin - sd;
+        *dpen = -1;
+    } else if (sd <= margin) {
+        const real diff = sd - margin, inv = (real)1 / (margin + (real)1e-6);
+        *pen = (real)0.5 * inv * diff * diff;
+        *dpen = inv * diff;
+    } else {
+        *pen = 0;
+        *dpen = 0;
     }
 }
 
@@ -569,6 +658,10 @@ static void spatial_column(const ork_robot *rb, const real *S, int link, int a, 
  *   SMOOTH   compute_smoothness_task_weighted_residual/jacobian_kernel smoothness_task.py:350-365, 404-422
  *   ROTATION compute_rotation_task_weighted_residual_kernel rotation_task.py:323-349 (axis-angle of
  *            q_link * q_target^-1) and the geometric Jacobian rotation_task.py:352-401 (w * omega column)
+ *   COLLISION sphere_collision_residual_apply_kernel sphere_collision_kernels.py:68-97 (w pen(sd),
+ *            sd = scene distance of the sphere centre - radius) and _sphere_collision_jacobian_kernel
+ *            scene_collision_task.py:33-81 (w pen'(sd) n . (v + w x c), zero beyond the margin
+ *            except for the surface distance), the point column of scene_distance_task.py:44-70
  */
 static void term_eval(const ork_robot *rb, const ork_term *t, int b, const real *q, const real *Tl, const real *S,
                       real *r, real *J) {
@@ -650,6 +743,30 @@ static void term_eval(const ork_robot *rb, const ork_term *t, int b, const real 
         for (int d = 0; d < na; ++d) {
             r[d] = (real)t->weights[d] * (q[d] - (real)qp[d]);
             if (J) J[d * na + d] = (real)t->weights[d];
+        }
+    } else if (t->kind == ORK_COLLISION) {
+        const ork_scene *sc = t->scene;
+        for (int k = 0; k < sc->nsph; ++k) {
+            const int link = sc->sph_link[k];
+            const real *Tk = Tl + 7 * link;
+            real lc[3] = {(real)sc->sph_c[3 * k], (real)sc->sph_c[3 * k + 1], (real)sc->sph_c[3 * k + 2]}, c[3], n[3];
+            quat_apply(Tk + 3, lc, c);
+            for (int i = 0; i < 3; ++i) c[i] += Tk[i];
+            const real sd = prims_sdf(sc->nobs_s, sc->obs_s, sc->nobs_b, sc->obs_b, c, n) - (real)sc->sph_r[k];
+            real pen, dpen;
+            coll_penalty(sd, (real)sc->margin, sc->penalty, &pen, &dpen);
+            const real w = (real)sc->sph_w[k];
+            r[k] = w * pen;
+            if (J && (sc->penalty == 2 || sd <= (real)sc->margin)) {
+                real cn[3];
+                cross3(c, n, cn);
+                for (int a = 0; a < na; ++a) {
+                    real s6[6];
+                    spatial_column(rb, S, link, a, s6);
+                    J[k * na + a] = w * dpen * ((n[0] * s6[0] + n[1] * s6[1] + n[2] * s6[2]) +
+                                                (cn[0] * s6[3] + cn[1] * s6[4] + cn[2] * s6[5]));
+                }
+            }
         }
     }
 }
@@ -993,53 +1110,9 @@ static int traj_rows(const ork_robot *rb, const ork_traj *tj, int *Rg, int *Rv, 
     return *Rg + *Rv + *Ra + *Rc + *Rl;
 }
 
-/* scene SDF at p: signed distance and gradient direction */
+/* scene SDF of a trajectory problem set */
 static real scene_sdf(const ork_traj *tj, const real *p, real *n) {
-    real best = (real)1e6;
-    n[0] = 1; n[1] = 0; n[2] = 0;
-    for (int i = 0; i < tj->nobs_s; ++i) {
-        const float *o = tj->obs_s + 4 * i;
-        real d[3] = {p[0] - (real)o[0], p[1] - (real)o[1], p[2] - (real)o[2]};
-        real len = RSQRT(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        real dist = len - (real)o[3];
-        if (dist < best) {
-            best = dist;
-            if (len > (real)1e-10) { n[0] = d[0] / len; n[1] = d[1] / len; n[2] = d[2] / len; }
-            else { n[0] = 1; n[1] = 0; n[2] = 0; }
-        }
-    }
-    for (int i = 0; i < tj->nobs_b; ++i) {
-        const float *o = tj->obs_b + 6 * i;
-        real ql[3], qv[3], h[3];
-        for (int k = 0; k < 3; ++k) {
-            ql[k] = p[k] - (real)o[k];
-            h[k] = (real)o[3 + k];
-            qv[k] = RFABS(ql[k]) - h[k];
-        }
-        real qmax = qv[0] > qv[1] ? qv[0] : qv[1];
-        if (qv[2] > qmax) qmax = qv[2];
-        real qp[3] = {qv[0] > 0 ? qv[0] : 0, qv[1] > 0 ? qv[1] : 0, qv[2] > 0 ? qv[2] : 0};
-        real dist = RSQRT(qp[0] * qp[0] + qp[1] * qp[1] + qp[2] * qp[2]) + (qmax < 0 ? qmax : 0);
-        if (dist < best) {
-            best = dist;
-            real nn[3] = {0, 0, 1};
-            if (qmax < 0) {
-                if (qv[0] >= qv[1] && qv[0] >= qv[2]) { nn[0] = ql[0] < 0 ? -1 : 1; nn[2] = 0; }
-                else if (qv[1] >= qv[2]) { nn[1] = ql[1] < 0 ? -1 : 1; nn[2] = 0; }
-                else { nn[2] = ql[2] < 0 ? -1 : 1; }
-            } else {
-                real df[3];
-                for (int k = 0; k < 3; ++k) {
-                    real c = ql[k] < -h[k] ? -h[k] : (ql[k] > h[k] ? h[k] : ql[k]);
-                    df[k] = ql[k] - c;
-                }
-                real dl = RSQRT(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
-                if (dl > (real)1e-10) { nn[0] = df[0] / dl; nn[1] = df[1] / dl; nn[2] = df[2] / dl; }
-            }
-            n[0] = nn[0]; n[1] = nn[1]; n[2] = nn[2];
-        }
-    }
-    return best;
+    return prims_sdf(tj->nobs_s, tj->obs_s, tj->nobs_b, tj->obs_b, p, n);
 }
 
 /* sparse row: up to 7 columns (one frame) or 3 (smoothness) */

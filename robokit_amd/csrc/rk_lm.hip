@@ -35,6 +35,9 @@
 #define RK_MAXT 4   // pose targets per problem
 #define RK_MAXQ 12  // position targets per problem
 #define RK_MAX_ROBOTS 256
+#define RK_SCENE_MAXS 64  // obstacle spheres of a scene
+#define RK_SCENE_MAXB 16  // obstacle boxes (axis-aligned) of a scene
+#define RK_COLL_MAXS 64   // robot collision spheres of an IK term structure
 
 namespace rk {
 
@@ -59,6 +62,17 @@ struct Terms {
   // columns (bit c): the other chunks' trailing updates add exact zeros and are skipped
   int wperm[RK_NA_MAX], wiperm[RK_NA_MAX];
   unsigned wchol[RK_NA_MAX / 4];
+  // SceneCollisionTask rows (after the smoothness rows), one per robot sphere: r = w pen(sd) with
+  // sd = scene distance of the sphere centre - radius; spheres sorted by the joint whose pose
+  // moves their link (-1: the base link) so that FK meets them in order
+  int nc;
+  int c_joint[RK_COLL_MAXS];
+  float c_c[RK_COLL_MAXS][3], c_r[RK_COLL_MAXS], c_w[RK_COLL_MAXS];
+  float c_margin;
+  int c_penalty;  // 0 smooth, 1 plain, 2 surface distance
+  int ns, nb;     // the scene (padded to whole groups: 8 spheres, 4 boxes)
+  float obs_s[RK_SCENE_MAXS][4];
+  float obs_b[RK_SCENE_MAXB][6];
 };
 
 // Dof elimination order and block fill pattern of the damped normal equations of one term
@@ -731,6 +745,7 @@ __global__ __launch_bounds__(64) void k_fused(Problems P, LMParams lp, int it0, 
   solve_tile<G, C>(P, blockIdx.x * (WAVE / G), lp, it0, iters, lds);
 }
 
+#include "rk_scene.h"
 #include "rk_thread.h"
 #include "rk_wave.h"
 #include "rk_traj.h"
@@ -741,13 +756,16 @@ __global__ __launch_bounds__(64) void k_fused(Problems P, LMParams lp, int it0, 
 //   V_W*:      one wavefront per problem, register-resident rows (trees, <= 64 joints, <= 32 dofs)
 //   V_H*:      two problems per wavefront, 32 lanes each, otherwise as V_W* (trees, <= 32 joints)
 //   V_P*:      as V_T*, with a position + rotation target pair instead of a pose target
+//   V_TC*/V_PC*: as V_T* / V_P* (6 or 7 dofs), with SceneCollisionTask rows
 enum Variant {
   V_S4 = 0, V_S8, V_S16, V_L16, V_L32, V_L64, V_T5, V_T6, V_T7, V_T8, V_W40, V_W012, V_H40, V_H012,
-  V_P5, V_P6, V_P7, V_P8, NVARIANT
+  V_P5, V_P6, V_P7, V_P8, V_TC6, V_TC7, V_PC6, V_PC7, NVARIANT
 };
 __host__ __device__ constexpr bool variant_wave(int v) { return v == V_W40 || v == V_W012; }
 __host__ __device__ constexpr bool variant_half(int v) { return v == V_H40 || v == V_H012; }
-__host__ __device__ constexpr bool variant_thread(int v) { return (v >= V_T5 && v <= V_T8) || (v >= V_P5 && v <= V_P8); }
+__host__ __device__ constexpr bool variant_thread(int v) {
+  return (v >= V_T5 && v <= V_T8) || (v >= V_P5 && v <= V_P8) || (v >= V_TC6 && v <= V_PC7);
+}
 __host__ __device__ constexpr int variant_G(int v) {
   return variant_thread(v) ? 1
          : variant_wave(v) ? 64
@@ -800,6 +818,10 @@ __device__ __forceinline__ void run_tile(const Bucket* bk, int local, const LMPa
   else if constexpr (V == V_P6) thread_solve<6, 8, true>(P, local * WAVE + threadIdx.x, lp, it0, iters);
   else if constexpr (V == V_P7) thread_solve<7, 9, true>(P, local * WAVE + threadIdx.x, lp, it0, iters);
   else if constexpr (V == V_P8) thread_solve<8, 10, true>(P, local * WAVE + threadIdx.x, lp, it0, iters);
+  else if constexpr (V == V_TC6) thread_solve<6, 8, false, true>(P, local * WAVE + threadIdx.x, lp, it0, iters);
+  else if constexpr (V == V_TC7) thread_solve<7, 9, false, true>(P, local * WAVE + threadIdx.x, lp, it0, iters);
+  else if constexpr (V == V_PC6) thread_solve<6, 8, true, true>(P, local * WAVE + threadIdx.x, lp, it0, iters);
+  else if constexpr (V == V_PC7) thread_solve<7, 9, true, true>(P, local * WAVE + threadIdx.x, lp, it0, iters);
   else if constexpr (V == V_W40) wave_solve<4, 0, 64>(P, local, lp, it0, iters, lds);
   else if constexpr (V == V_W012) wave_solve<0, 12, 64>(P, local, lp, it0, iters, lds);
   else if constexpr (V == V_H40) wave_solve<4, 0, 32>(P, local, lp, it0, iters, lds);
@@ -1043,6 +1065,25 @@ __global__ void k_plain_rot_jac(Problems P, const float* Sw, float* J) {
   }
 }
 
+// SceneCollisionTask rows (residuals only: zero-iteration scoring), thread per problem
+__global__ void k_plain_coll_res(Problems P, const float* Tj, float* r) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.n) return;
+  const Terms* tm = P.terms;
+  const int S = P.stride, na = P.robot->na;
+  const int off = tm->Rf + tm->Rq + tm->Rr + (tm->limit ? na : 0) + (tm->smooth ? na : 0);
+  for (int k = 0; k < tm->nc; ++k) {
+    const Pose T = plain_pose(Tj, tm->c_joint[k], S, b);
+    float c[3];
+    quat_apply(T.q, tm->c_c[k], c);
+    for (int i = 0; i < 3; ++i) c[i] += T.t[i];
+    int grp;
+    float pen, dpen;
+    coll_penalty(scene_sdf(tm, c, grp) - tm->c_r[k], tm->c_margin, tm->c_penalty, pen, dpen);
+    r[(off + k) * S + b] = tm->c_w[k] * pen;
+  }
+}
+
 // limit + smoothness rows (and their diagonal Jacobian), thread per (problem, dof)
 __global__ void k_plain_diag_rows(Problems P, const float* qsrc, float* r, float* J) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1126,8 +1167,8 @@ __global__ void k_plain_cost(Problems P, const float* __restrict__ r) {
   const Terms* tm = P.terms;
   float c = 0.f;
   int off = 0;
-  const int sizes[5] = {tm->Rf, tm->Rq, tm->Rr, tm->limit ? na : 0, tm->smooth ? na : 0};
-  for (int t = 0; t < 5; ++t) {
+  const int sizes[6] = {tm->Rf, tm->Rq, tm->Rr, tm->limit ? na : 0, tm->smooth ? na : 0, tm->nc};
+  for (int t = 0; t < 6; ++t) {
     if (!sizes[t]) continue;
     float s = 0.f;
     for (int k = 0; k < sizes[t]; ++k) {
@@ -1576,7 +1617,7 @@ static Problems make_problems(rk_batch* b, bool from_seed = false) {
 }
 
 static bool fits_small(const Terms& th, const Robot& r) {
-  return r.nj <= CfgS::nj && r.na <= CfgS::na && th.n_pose + th.n_rot <= CfgS::np && th.n_pos <= CfgS::nq;
+  return r.nj <= CfgS::nj && r.na <= CfgS::na && th.n_pose + th.n_rot <= CfgS::np && th.n_pos <= CfgS::nq && !th.nc;
 }
 
 // register thread-per-problem kernels: serial arms with 5..8 dofs, <= 2 fixed tail joints; one
@@ -1592,7 +1633,7 @@ static bool fits_thread(const Terms& th, const Robot& r) {
 // targets and no position targets, or <= 12 position targets and no pose targets.  hw = 64: one
 // problem per wavefront, <= 64 joints; hw = 32: two per wavefront, <= 32 joints.
 static int wave_kind(const Terms& th, const Robot& r, int hw) {
-  if (r.nj > hw || r.na > WV_NA || th.n_rot) return -1;
+  if (r.nj > hw || r.na > WV_NA || th.n_rot || th.nc) return -1;
   for (int a = 0; a < r.na; ++a)
     if (r.col_start[a + 1] - r.col_start[a] > 2) return -1;
   if (th.n_pos == 0 && th.n_pose <= 4) return hw == 64 ? V_W40 : V_H40;
@@ -1601,7 +1642,8 @@ static int wave_kind(const Terms& th, const Robot& r, int hw) {
 }
 
 static int signature(const Terms& th) {
-  return (((th.robot * 8 + th.n_pose) * 16 + th.n_pos) * 4 + th.limit * 2 + th.smooth) * 8 + th.n_rot;
+  return ((((th.robot * 8 + th.n_pose) * 16 + th.n_pos) * 4 + th.limit * 2 + th.smooth) * 8 + th.n_rot) * 2 +
+         (th.nc ? 1 : 0);
 }
 
 // threads per problem -> variant.  G = 0: the tuned choice for this term structure if the
@@ -1619,8 +1661,17 @@ static int pick_variant(const rk_context* ctx, const Terms& th, int G, int* v) {
           : small                          ? 8
                                            : 64;
   }
+  const bool pr = th.n_pose == 0 && (th.n_pos || th.n_rot);
+  if (th.nc) {  // SceneCollisionTask rows: the register kernels of 6- and 7-dof serial arms
+    if ((G == 1 || G == 0) && fits_thread(th, r) && (r.na == 6 || r.na == 7)) {
+      *v = (pr ? V_PC6 : V_TC6) + (r.na - 6);
+      return RK_OK;
+    }
+    return fail(RK_EINVAL, "collision rows run the one-lane-per-problem register kernel (threads_per_problem 1) of "
+                           "6- and 7-dof serial arms");
+  }
   if (G == 1 && fits_thread(th, r)) {
-    *v = (th.n_pose == 0 && (th.n_pos || th.n_rot) ? V_P5 : V_T5) + (r.na - 5);
+    *v = (pr ? V_P5 : V_T5) + (r.na - 5);
     return RK_OK;
   }
   if ((G == RK_TPP_WAVE || G == RK_TPP_HALF) && wave_kind(th, r, -G) >= 0) {
@@ -1685,6 +1736,10 @@ static int launch_variant(hipStream_t st, int v, const Problems& P, const LMPara
     case V_P6: hipLaunchKernelGGL((k_thread<6, 8, true>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_P7: hipLaunchKernelGGL((k_thread<7, 9, true>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_P8: hipLaunchKernelGGL((k_thread<8, 10, true>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_TC6: hipLaunchKernelGGL((k_thread<6, 8, false, true>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_TC7: hipLaunchKernelGGL((k_thread<7, 9, false, true>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_PC6: hipLaunchKernelGGL((k_thread<6, 8, true, true>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_PC7: hipLaunchKernelGGL((k_thread<7, 9, true, true>), grid, dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_W40: hipLaunchKernelGGL((k_wave<4, 0, 64>), dim3(P.n), dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_W012: hipLaunchKernelGGL((k_wave<0, 12, 64>), dim3(P.n), dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_H40:
@@ -1793,6 +1848,8 @@ static int eval_cost(rk_batch* b, bool from_seed) {
   if (b->th.limit || b->th.smooth)
     hipLaunchKernelGGL(k_plain_diag_rows, dim3(g1.x, na), blk, 0, ctx->stream, P, (const float*)b->q, b->pb.r,
                        (float*)nullptr);
+  if (b->th.nc)
+    hipLaunchKernelGGL(k_plain_coll_res, g1, blk, 0, ctx->stream, P, (const float*)b->pb.Tj, b->pb.r);
   hipLaunchKernelGGL(k_plain_cost, g1, blk, 0, ctx->stream, P, (const float*)b->pb.r);
   HIP_TRY(hipGetLastError());
   return RK_OK;
@@ -1800,6 +1857,7 @@ static int eval_cost(rk_batch* b, bool from_seed) {
 
 static int solve_plain(rk_batch* b, int it0, int iters, const LMParams& lp, bool from_seed) {
   int rc;
+  if (b->th.nc) return fail(RK_EINVAL, "collision rows run the fused register kernel (not the plain per-term launch)");
   if ((rc = alloc_plain(b))) return rc;
   rk_context* ctx = b->ctx;
   if (from_seed)
@@ -2175,8 +2233,13 @@ static int make_terms(rk_context* ctx, const rk_terms_desc* t, Terms* out) {
     return fail(RK_EINVAL, "at most 4 pose targets and 12 position targets per problem");
   if (t->n_rot < 0 || t->n_pose + t->n_rot > RK_MAXT)
     return fail(RK_EINVAL, "at most 4 pose and rotation targets together per problem");
-  if (t->n_pose + t->n_pos + t->n_rot + (t->limit ? 1 : 0) + (t->smooth ? 1 : 0) == 0)
+  if (t->n_pose + t->n_pos + t->n_rot + (t->limit ? 1 : 0) + (t->smooth ? 1 : 0) + (t->n_coll > 0 ? 1 : 0) == 0)
     return fail(RK_EINVAL, "a batch needs at least one residual term");
+  if (t->n_coll < 0 || t->n_coll > RK_COLL_MAXS || (t->n_coll && (t->n_obs_spheres < 0 ||
+      t->n_obs_spheres > RK_SCENE_MAXS || t->n_obs_boxes < 0 || t->n_obs_boxes > RK_SCENE_MAXB)))
+    return fail(RK_EINVAL, "at most 64 robot spheres, 64 obstacle spheres and 16 obstacle boxes");
+  if (t->n_coll && (t->coll_penalty < 0 || t->coll_penalty > 2))
+    return fail(RK_EINVAL, "coll_penalty: 0 smooth, 1 plain, 2 surface distance");
   if (r.nj > CfgL::nj) return fail(RK_EINVAL, "the LM kernels take robots with nj <= 32");
   Terms th;
   memset(&th, 0, sizeof(th));
@@ -2205,10 +2268,37 @@ static int make_terms(rk_context* ctx, const rk_terms_desc* t, Terms* out) {
     if (th.rot_link[i] < 0 || th.rot_link[i] >= r.nl) return fail(RK_EINVAL, "rotation link out of range");
     for (int k = 0; k < 3; ++k) th.rot_w[i][k] = t->rot_weights[3 * i + k];
   }
+  // SceneCollisionTask: spheres sorted (stably) by the joint whose pose moves their link, the scene
+  // padded with far-away primitives to whole groups (8 spheres, 4 boxes)
+  th.nc = t->n_coll;
+  if (th.nc) {
+    std::vector<int> order(th.nc);
+    for (int k = 0; k < th.nc; ++k) {
+      order[k] = k;
+      if (t->coll_links[k] < 0 || t->coll_links[k] >= r.nl) return fail(RK_EINVAL, "collision sphere link out of range");
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return r.link_joint[t->coll_links[a]] < r.link_joint[t->coll_links[b]]; });
+    for (int i = 0; i < th.nc; ++i) {
+      const int k = order[i];
+      th.c_joint[i] = r.link_joint[t->coll_links[k]];
+      for (int c = 0; c < 3; ++c) th.c_c[i][c] = t->coll_centers[3 * k + c];
+      th.c_r[i] = t->coll_radii[k];
+      th.c_w[i] = t->coll_weights ? t->coll_weights[k] : 1.f;
+    }
+    th.c_margin = t->coll_margin;
+    th.c_penalty = t->coll_penalty;
+    th.ns = t->n_obs_spheres;
+    th.nb = t->n_obs_boxes;
+    for (int i = 0; i < RK_SCENE_MAXS; ++i)
+      for (int c = 0; c < 4; ++c) th.obs_s[i][c] = i < th.ns ? t->obs_spheres[4 * i + c] : (c < 3 ? RK_SDF_FAR : 0.f);
+    for (int i = 0; i < RK_SCENE_MAXB; ++i)
+      for (int c = 0; c < 6; ++c) th.obs_b[i][c] = i < th.nb ? t->obs_boxes[6 * i + c] : (c < 3 ? RK_SDF_FAR : 0.f);
+  }
   th.Rf = 6 * th.n_pose;
   th.Rq = 3 * th.n_pos;
   th.Rr = 3 * th.n_rot;
-  th.R = th.Rf + th.Rq + th.Rr + (th.limit ? r.na : 0) + (th.smooth ? r.na : 0);
+  th.R = th.Rf + th.Rq + th.Rr + (th.limit ? r.na : 0) + (th.smooth ? r.na : 0) + th.nc;
   wave_elimination(r, th);
   *out = th;
   return RK_OK;
@@ -2799,6 +2889,10 @@ static int variant_grid_any(rk_context* ctx, int v, int wg_per_cu, size_t lds, i
     case V_P6: return variant_grid<V_P6>(ctx, wg_per_cu, lds, tiles, grid);
     case V_P7: return variant_grid<V_P7>(ctx, wg_per_cu, lds, tiles, grid);
     case V_P8: return variant_grid<V_P8>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_TC6: return variant_grid<V_TC6>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_TC7: return variant_grid<V_TC7>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_PC6: return variant_grid<V_PC6>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_PC7: return variant_grid<V_PC7>(ctx, wg_per_cu, lds, tiles, grid);
     default: return variant_grid<V_H012>(ctx, wg_per_cu, lds, tiles, grid);
   }
 }
@@ -2826,6 +2920,10 @@ static void launch_persistent(int v, dim3 grid, size_t lds, hipStream_t st, cons
     RK_LAUNCH_P(V_P6)
     RK_LAUNCH_P(V_P7)
     RK_LAUNCH_P(V_P8)
+    RK_LAUNCH_P(V_TC6)
+    RK_LAUNCH_P(V_TC7)
+    RK_LAUNCH_P(V_PC6)
+    RK_LAUNCH_P(V_PC7)
 #undef RK_LAUNCH_P
     default: break;
   }

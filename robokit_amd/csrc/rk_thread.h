@@ -30,8 +30,14 @@ __device__ __forceinline__ Pose identity_pose() {
 // FK of a serial arm streamed joint by joint: T_j = T_{j-1} o (P_j o M(q_j)) (fixed tail joints
 // compose P_j only).  Returns the world pose of joint jt; with WITH_S also the motion-subspace
 // column of every actuated joint, computed as its pose is produced.
-template <int NA, int NJ, bool WITH_S>
-__device__ __forceinline__ Pose chain_fk(RobotK* rb, int nj, const float* q, int jt, float (&S6)[NA][6]) {
+struct NoJointHook {
+  __device__ __forceinline__ void operator()(int, const Pose&) const {}
+};
+
+// on(j, T) runs once joint j's world pose T (and, with WITH_S, its subspace column) is known
+template <int NA, int NJ, bool WITH_S, class Hook = NoJointHook>
+__device__ __forceinline__ Pose chain_fk(RobotK* rb, int nj, const float* q, int jt, float (&S6)[NA][6],
+                                         const Hook& on = Hook()) {
   Pose T = identity_pose(), Tt = identity_pose();
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
@@ -56,6 +62,7 @@ __device__ __forceinline__ Pose chain_fk(RobotK* rb, int nj, const float* q, int
       T = j == 0 ? L : compose(T, L);
       if (WITH_S && j < NA) subspace_column(rb, j, T, S6[j]);
       if (j == jt) Tt = T;
+      on(j, T);
     }
   }
   return Tt;
@@ -152,10 +159,74 @@ __device__ __forceinline__ float chain_cost(TermsK* tm, const float* r_pose, con
   return cost;
 }
 
+// SceneCollisionTask penalty of a signed distance (sphere_collision_kernels.py:19-42): the residual
+// and its derivative.  mode 0 smooth: margin / 2 - sd inside, (sd - margin)^2 / (2 (margin + 1e-6))
+// up to the margin; 1 plain: margin - sd below the margin; 2 surface distance: |sd|.
+__device__ __forceinline__ void coll_penalty(float sd, float margin, int mode, float& pen, float& dpen) {
+  if (mode == 2) {
+    pen = fabsf(sd);
+    dpen = sd < 0.f ? -1.f : 1.f;
+  } else if (mode == 1) {
+    pen = sd < margin ? margin - sd : 0.f;
+    dpen = sd < margin ? -1.f : 0.f;
+  } else if (sd < 0.f) {
+    pen = 0.5f * margin - sd;
+    dpen = -1.f;
+  } else if (sd <= margin) {
+    const float diff = sd - margin, inv = 1.f / (margin + 1e-6f);
+    pen = 0.5f * inv * diff * diff;
+    dpen = inv * diff;
+  } else {
+    pen = dpen = 0.f;
+  }
+}
+
+// One collision row of sphere k at its link's world pose T (scene_collision_task.py:33-81, 550-646):
+// r = w pen(sd); with JAC its Jacobian columns a <= j (the joints above the link) are
+// w pen'(sd) n . (v_a + w_a x c), n the scene gradient at the centre c, zero beyond the margin
+// (surface distance: always), accumulated as the term's J^T J (Hc, lower) and J^T r (gc)
+template <int NA, bool JAC>
+__device__ __forceinline__ void coll_row(TermsK* tm, int k, int j, const Pose& T, const float (&S6)[NA][6],
+                                         float (&Hc)[NA][NA], float (&gc)[NA], float& cc) {
+  const float lc[3] = {tm->c_c[k][0], tm->c_c[k][1], tm->c_c[k][2]};
+  float c[3];
+  quat_apply(T.q, lc, c);
+  c[0] += T.t[0];
+  c[1] += T.t[1];
+  c[2] += T.t[2];
+  int grp;
+  const float sd = scene_sdf(tm, c, grp) - tm->c_r[k];
+  float pen, dpen;
+  coll_penalty(sd, tm->c_margin, tm->c_penalty, pen, dpen);
+  const float w = tm->c_w[k];
+  const float r = w * pen;
+  cc += r * r;
+  if (JAC && (tm->c_penalty == 2 || sd <= tm->c_margin)) {
+    float n[3];
+    scene_normal(tm, c, scene_winner(tm, c, grp), n);
+    const float ws = w * dpen;
+    float Jk[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      float wxc[3];
+      cross3(S6[a] + 3, c, wxc);
+      Jk[a] = a <= j ? ws * (n[0] * (S6[a][0] + wxc[0]) + n[1] * (S6[a][1] + wxc[1]) + n[2] * (S6[a][2] + wxc[2]))
+                     : 0.f;
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+#pragma unroll
+      for (int b = 0; b <= a; ++b) Hc[a][b] += Jk[a] * Jk[b];
+      gc[a] += Jk[a] * r;
+    }
+  }
+}
+
 // One serial-arm problem per lane: <= 1 pose target (PR false), or <= 1 position and <= 1 rotation
-// target on links of one parent joint (PR true; the two forms are separate kernels so that each
-// keeps its own register allocation); limit and smoothness rows.
-template <int NA, int NJ, bool PR = false>
+// target on links of one parent joint (PR true); limit and smoothness rows; with CO the
+// SceneCollisionTask rows of the robot's spheres, evaluated as FK reaches each sphere's link.  The
+// forms are separate kernels so that each keeps its own register allocation.
+template <int NA, int NJ, bool PR = false, bool CO = false>
 __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, const LMParams& lp, int it0, int iters) {
   RobotK* rb = as_const(Pg.robot);
   TermsK* tm = as_const(Pg.terms);
@@ -204,9 +275,27 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
       for (int d = 0; d < NA; ++d) Pg.q_trace[(gi * NA + d) * S + s] = q[d];
       Pg.lam_trace[gi * S + s] = lam;
     }
-    // ---- FK with motion subspace, residual rows, Jacobian blocks at q
+    // ---- FK with motion subspace (and the collision rows as the spheres' links appear), residual
+    // rows, Jacobian blocks at q
     float S6[NA][6];
-    const Pose Tf = chain_fk<NA, NJ, true>(rb, nj, q, jt, S6);
+    float Hc[NA][NA], gc[NA], cc = 0.f;
+    if constexpr (CO) {
+#pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        gc[a] = 0.f;
+#pragma unroll
+        for (int b = 0; b < NA; ++b) Hc[a][b] = 0.f;
+        for (int r = 0; r < 6; ++r) S6[a][r] = 0.f;
+      }
+    }
+    int kc = 0;
+    if constexpr (CO)
+      for (; kc < tm->nc && tm->c_joint[kc] < 0; ++kc) coll_row<NA, false>(tm, kc, -1, identity_pose(), S6, Hc, gc, cc);
+    auto coll_at = [&](int j, const Pose& T) {
+      if constexpr (CO)
+        for (; kc < tm->nc && tm->c_joint[kc] == j; ++kc) coll_row<NA, true>(tm, kc, j, T, S6, Hc, gc, cc);
+    };
+    const Pose Tf = chain_fk<NA, NJ, true>(rb, nj, q, jt, S6, coll_at);
     float r_pose[6], r_lim[NA], r_sm[NA], jl[NA], js[NA], tg[36];
     chain_rows<NA, true, PR>(rb, tm, Tf, tgt, q, qprev, r_pose, r_lim, r_sm, tg, jl, js);
     // ---- target-row Jacobian columns (zero for joints after the target link's joint): FrameTask
@@ -287,9 +376,17 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
       }
       if (tm->limit) gv = gv + jl[i] * r_lim[i];
       if (tm->smooth) gv = gv + js[i] * r_sm[i];
+      if (CO) gv = gv + gc[i];
       g[i] = gv;
     }
+    if constexpr (CO) {  // the collision term after the others
+#pragma unroll
+      for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) H[i][j] = H[i][j] + Hc[i][j];
+    }
     cost = chain_cost<NA, PR>(tm, r_pose, r_lim, r_sm);
+    if (CO) cost = cost + 0.5f * cc;
     // ---- damped Cholesky (left-looking) and solves.  Each pivot's square root and reciprocal
     // are the hardware's (v_sqrt_f32, v_rcp_f32, 1 ulp) and the column scales by the reciprocal:
     // a correctly rounded division is a dependent chain of ~10 instructions, and this kernel is
@@ -334,9 +431,18 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
     float qp[NA];
 #pragma unroll
     for (int d = 0; d < NA; ++d) qp[d] = q[d] + dq[d];
-    const Pose Tfp = chain_fk<NA, NJ, false>(rb, nj, qp, jt, S6);
+    float cpc = 0.f;
+    int kp = 0;
+    if constexpr (CO)
+      for (; kp < tm->nc && tm->c_joint[kp] < 0; ++kp) coll_row<NA, false>(tm, kp, -1, identity_pose(), S6, Hc, gc, cpc);
+    auto coll_prop = [&](int j, const Pose& T) {
+      if constexpr (CO)
+        for (; kp < tm->nc && tm->c_joint[kp] == j; ++kp) coll_row<NA, false>(tm, kp, j, T, S6, Hc, gc, cpc);
+    };
+    const Pose Tfp = chain_fk<NA, NJ, false>(rb, nj, qp, jt, S6, coll_prop);
     chain_rows<NA, false, PR>(rb, tm, Tfp, tgt, qp, qprev, r_pose, r_lim, r_sm, tg, jl, js);
-    const float cp = chain_cost<NA, PR>(tm, r_pose, r_lim, r_sm);
+    float cp = chain_cost<NA, PR>(tm, r_pose, r_lim, r_sm);
+    if (CO) cp = cp + 0.5f * cpc;
     if (valid && Pg.trace && gi == 0) Pg.trace[s] = cost;
     const bool acc = lm_accept(lp, cost, cp, pred, lam);
     if (acc) {
@@ -358,7 +464,7 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
   }
 }
 
-template <int NA, int NJ, bool PR = false>
+template <int NA, int NJ, bool PR = false, bool CO = false>
 __global__ __launch_bounds__(64) void k_thread(Problems P, LMParams lp, int it0, int iters) {
-  thread_solve<NA, NJ, PR>(P, blockIdx.x * 64 + threadIdx.x, lp, it0, iters);
+  thread_solve<NA, NJ, PR, CO>(P, blockIdx.x * 64 + threadIdx.x, lp, it0, iters);
 }

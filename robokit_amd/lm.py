@@ -10,6 +10,8 @@ caller of the reference's Levenberg-Marquardt path can switch without edits:
   RotationTask                 terms/dense/rotation_task.py:25-80 (axis-angle rows, geometric Jacobian)
   PositionLimit                terms/dense/position_limit.py:204-220 (residual_mode "abs")
   SmoothnessTask               terms/dense/smoothness_task.py:41-50
+  SceneCollisionTask           terms/dense/scene_collision_task.py:276-356 (robot spheres against a
+                               PrimitiveScene of obstacle spheres and axis-aligned boxes, geom/scene.py)
   LMOptimizerConfig            opt/lm_optimizer.py:502-530 (+ OptimizerConfig, opt/optimizer.py:159-174)
   LMOptimizer                  opt/optimizer.py:177-333 lifecycle: solve / reset / step, costs
 
@@ -66,11 +68,29 @@ class VarKey(Enum):
 
 
 class Robot:
-    """A kinematic model (RobotSpec) with the state factory of robo/robot.py:105-130."""
+    """A kinematic model (RobotSpec) with the state factory of robo/robot.py:105-130.
+    ``collision_spheres`` = (link indices [n], centres in the link frames [n, 3], radii [n]): the
+    robot's collision spheres (RobotSpec.collision_spheres_link_indices /
+    local_collision_sphere_centers / radii in the reference)."""
 
-    def __init__(self, spec: RobotSpec):
+    def __init__(self, spec: RobotSpec, collision_spheres=None):
         spec.validate()
         self.spec = spec
+        self.collision_spheres = None
+        if collision_spheres is not None:
+            links, centers, radii = collision_spheres
+            links = np.asarray(links, np.int32).reshape(-1)
+            centers = np.asarray(centers, np.float32).reshape(-1, 3)
+            radii = np.asarray(radii, np.float32).reshape(-1)
+            if not (len(links) == len(centers) == len(radii)):
+                raise ValueError("collision spheres need one link, centre and radius each")
+            if np.any(links < 0) or np.any(links >= spec.num_links):
+                raise ValueError("collision sphere link out of range")
+            self.collision_spheres = (links, centers, radii)
+
+    @property
+    def has_collision_spheres(self) -> bool:
+        return self.collision_spheres is not None and len(self.collision_spheres[0]) > 0
 
     @staticmethod
     def load(urdf: str) -> "Robot":
@@ -359,6 +379,77 @@ class SmoothnessTask(Task):
         return self.robot.spec.num_actuated_joints
 
 
+class PrimitiveScene:
+    """A world scene of obstacle spheres and axis-aligned boxes: the primitive form of the
+    reference's WarpScene with SphereGeom / BoxGeom members (geom/scene.py:132-256,
+    geom/geoms.py:372-429), queried by its signed distance (geom/sdf_kernels.py:256-295)."""
+
+    def __init__(self, spheres: Optional[np.ndarray] = None, boxes: Optional[np.ndarray] = None):
+        self.spheres = np.zeros((0, 4), np.float32) if spheres is None else np.asarray(spheres, np.float32).reshape(-1, 4)
+        self.boxes = np.zeros((0, 6), np.float32) if boxes is None else np.asarray(boxes, np.float32).reshape(-1, 6)
+
+    def add_spheres(self, centers, radii) -> "PrimitiveScene":
+        c = np.asarray(centers, np.float32).reshape(-1, 3)
+        r = np.asarray(radii, np.float32).reshape(-1, 1)
+        self.spheres = np.concatenate([self.spheres, np.concatenate([c, np.broadcast_to(r, (len(c), 1))], 1)])
+        return self
+
+    def add_boxes(self, centers, half_extents) -> "PrimitiveScene":
+        c = np.asarray(centers, np.float32).reshape(-1, 3)
+        h = np.broadcast_to(np.asarray(half_extents, np.float32).reshape(-1, 3), c.shape)
+        self.boxes = np.concatenate([self.boxes, np.concatenate([c, h], 1)])
+        return self
+
+    @property
+    def geoms(self) -> int:
+        return len(self.spheres) + len(self.boxes)
+
+
+class SceneCollisionTask(Task):
+    """Collision rows r_k = w_k pen(sd_k) per robot sphere, sd_k the scene distance of the sphere
+    centre minus its radius, pen the smooth / plain / surface-distance penalty
+    (terms/dense/scene_collision_task.py:276-356, terms/sphere_collision_kernels.py:19-42)."""
+
+    def __init__(self, robot: Optional[Robot] = None, scene: Optional[PrimitiveScene] = None, geometry=None,
+                 scene_indices=None, use_link_bounding_sphere_filter: bool = False,
+                 weight: Union[float, Sequence[float]] = 1.0, margin: Optional[float] = 0.02,
+                 sphere_indices: Optional[Sequence[int]] = None, residual_mode: Literal["abs", "sqrt_abs"] = "abs",
+                 residual_eps: float = 1e-6, penalty: Literal["smooth", "plain", "surface_distance"] = "smooth",
+                 skip_trajectory_endpoints: bool = True, movable_only: bool = False, name: str = ""):
+        if penalty not in {"smooth", "plain", "surface_distance"}:
+            raise ValueError(f"Unsupported penalty: {penalty}")
+        if robot is None or scene is None or _has_context(scene):
+            self._as_template(scene=scene, weight=weight, margin=margin, sphere_indices=sphere_indices,
+                              residual_mode=residual_mode, penalty=penalty, movable_only=movable_only, name=name)
+            return
+        if geometry is not None or scene_indices is not None or use_link_bounding_sphere_filter:
+            raise NotImplementedError("geometry filters, per-query scenes and bounding-sphere filters are not on "
+                                      "this path")
+        if residual_mode != "abs":
+            raise NotImplementedError("SceneCollisionTask residual_mode must be 'abs' on this path")
+        if not robot.has_collision_spheres:
+            raise RuntimeError("Robot has no collision spheres. Build it with collision_spheres=(links, centers, radii).")
+        self.robot, self.name, self.weight, self.margin, self.penalty = robot, name, weight, float(margin), penalty
+        links, centers, radii = robot.collision_spheres
+        idx = np.arange(len(links))
+        if sphere_indices is None and movable_only:
+            idx = np.flatnonzero(robot.spec.link_parent_joint_indices[links] >= 0)
+        elif sphere_indices is not None:
+            idx = np.asarray(sphere_indices, np.int32).reshape(-1)
+            if np.any(idx < 0) or np.any(idx >= len(links)):
+                raise ValueError("sphere_indices contains out-of-range indices.")
+        w = np.asarray(weight, np.float32)
+        if w.ndim and w.shape != (len(idx),):
+            raise ValueError(f"Expected weight shape ({len(idx)},), got {w.shape}.")
+        self.sphere_links, self.sphere_centers, self.sphere_radii = links[idx], centers[idx], radii[idx]
+        self.weights = np.broadcast_to(w, (len(idx),)).astype(np.float32)
+        self.scene = scene
+
+    @property
+    def residual_dim(self) -> int:
+        return len(self.sphere_links)
+
+
 # ----------------------------------------------------------------------------- optimizer
 @dataclass(frozen=True)
 class OptimizerConfig:
@@ -429,12 +520,14 @@ class LMOptimizer:
         self._rot = [t for t in self.terms if isinstance(t, RotationTask)]
         lim = [t for t in self.terms if isinstance(t, PositionLimit)]
         sm = [t for t in self.terms if isinstance(t, SmoothnessTask)]
-        other = [t for t in self.terms
-                 if not isinstance(t, (FrameTask, PositionTask, RotationTask, PositionLimit, SmoothnessTask))]
+        col = [t for t in self.terms if isinstance(t, SceneCollisionTask)]
+        other = [t for t in self.terms if not isinstance(t, (FrameTask, PositionTask, RotationTask, PositionLimit,
+                                                             SmoothnessTask, SceneCollisionTask))]
         if other:
             raise NotImplementedError(f"terms not on this path: {[type(t).__name__ for t in other]}")
-        if len(lim) > 1 or len(sm) > 1:
-            raise ValueError("at most one PositionLimit and one SmoothnessTask per optimizer")
+        if len(lim) > 1 or len(sm) > 1 or len(col) > 1:
+            raise ValueError("at most one PositionLimit, one SmoothnessTask and one SceneCollisionTask per optimizer")
+        self._collision = col[0] if col else None
         self._limit = lim[0] if lim else None
         self._smooth = sm[0] if sm else None
         pose_links = [f for t in self._frame for f in t.frame_indices]
@@ -452,6 +545,13 @@ class LMOptimizer:
             rot_links=rot_links,
             rot_weights=np.concatenate([t.weights for t in self._rot]) if self._rot else None,
         )
+        if self._collision is not None:
+            c = self._collision
+            self.termspec.coll_links = list(c.sphere_links)
+            self.termspec.coll_centers, self.termspec.coll_radii = c.sphere_centers, c.sphere_radii
+            self.termspec.coll_weights, self.termspec.coll_margin = c.weights, c.margin
+            self.termspec.coll_penalty = c.penalty
+            self.termspec.obstacle_spheres, self.termspec.obstacle_boxes = c.scene.spheres, c.scene.boxes
 
     @property
     def residual_dim(self) -> int:

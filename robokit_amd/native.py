@@ -49,7 +49,13 @@ class TermsDesc(C.Structure):
         ("limit", C.c_int32), ("limit_weights", C.c_void_p),
         ("smooth", C.c_int32), ("smooth_weights", C.c_void_p),
         ("n_rot", C.c_int32), ("rot_links", C.c_void_p), ("rot_weights", C.c_void_p),
+        ("n_coll", C.c_int32), ("coll_links", C.c_void_p), ("coll_centers", C.c_void_p),
+        ("coll_radii", C.c_void_p), ("coll_weights", C.c_void_p), ("coll_margin", C.c_float),
+        ("coll_penalty", C.c_int32), ("n_obs_spheres", C.c_int32), ("obs_spheres", C.c_void_p),
+        ("n_obs_boxes", C.c_int32), ("obs_boxes", C.c_void_p),
     ]
+
+COLL_PENALTY = {"smooth": 0, "plain": 1, "surface_distance": 2}
 
 
 class LMParamsC(C.Structure):
@@ -293,7 +299,7 @@ class Context:
 @dataclass
 class TermSpec:
     """Uniform residual-term structure of a batch (row order: pose, position, rotation, limit,
-    smoothness)."""
+    smoothness, collision)."""
 
     pose_links: Sequence[int] = ()
     pose_weights: Optional[np.ndarray] = None  # [n_pose, 6]
@@ -303,10 +309,20 @@ class TermSpec:
     smooth_weights: Optional[np.ndarray] = None  # [na] or None (no smoothness term)
     rot_links: Sequence[int] = ()
     rot_weights: Optional[np.ndarray] = None  # [n_rot, 3]
+    # SceneCollisionTask: robot spheres (link, centre in the link frame, radius, weight) against a
+    # primitive scene (obstacle spheres [n, 4], axis-aligned boxes [n, 6]), rows after smoothness
+    coll_links: Sequence[int] = ()
+    coll_centers: Optional[np.ndarray] = None  # [n_coll, 3]
+    coll_radii: Optional[np.ndarray] = None  # [n_coll]
+    coll_weights: Optional[np.ndarray] = None  # [n_coll] (None: ones)
+    coll_margin: float = 0.02
+    coll_penalty: str = "smooth"  # smooth / plain / surface_distance
+    obstacle_spheres: Optional[np.ndarray] = None  # [n, 4] centre, radius
+    obstacle_boxes: Optional[np.ndarray] = None  # [n, 6] centre, half extents
 
     def rows(self, na: int) -> int:
         return 6 * len(self.pose_links) + 3 * len(self.pos_links) + 3 * len(self.rot_links) + \
-            na * (self.limit_weights is not None) + na * (self.smooth_weights is not None)
+            na * (self.limit_weights is not None) + na * (self.smooth_weights is not None) + len(self.coll_links)
 
     def desc(self, robot_id: int, na: int):
         """(rk_terms_desc, the arrays it points at)"""
@@ -320,10 +336,25 @@ class TermSpec:
             _i32(list(self.rot_links)),
             _f32(self.rot_weights if self.rot_weights is not None else np.zeros((0, 3))),
         ]
+        nc = len(self.coll_links)
+        if self.coll_penalty not in COLL_PENALTY:
+            raise ValueError(f"Unsupported penalty: {self.coll_penalty}")
+        arrs += [
+            _i32(list(self.coll_links)),
+            _f32(np.zeros((0, 3)) if self.coll_centers is None else np.asarray(self.coll_centers).reshape(-1, 3)),
+            _f32(np.zeros(0) if self.coll_radii is None else self.coll_radii),
+            None if self.coll_weights is None else _f32(np.broadcast_to(np.asarray(self.coll_weights, np.float32), (nc,))),
+            _f32(np.zeros((0, 4)) if self.obstacle_spheres is None else np.asarray(self.obstacle_spheres).reshape(-1, 4)),
+            _f32(np.zeros((0, 6)) if self.obstacle_boxes is None else np.asarray(self.obstacle_boxes).reshape(-1, 6)),
+        ]
+        if nc and (arrs[9].shape[0] != nc or arrs[10].shape[0] != nc):
+            raise ValueError(f"collision spheres: {nc} links but {arrs[9].shape[0]} centres / {arrs[10].shape[0]} radii")
         a = arrs
         d = TermsDesc(robot_id, len(self.pose_links), _p(a[0]), _p(a[1]), len(self.pos_links), _p(a[2]), _p(a[3]),
                       int(self.limit_weights is not None), _p(a[4]), int(self.smooth_weights is not None), _p(a[5]),
-                      len(self.rot_links), _p(a[6]), _p(a[7]))
+                      len(self.rot_links), _p(a[6]), _p(a[7]),
+                      nc, _p(a[8]), _p(a[9]), _p(a[10]), _p(a[11]), float(self.coll_margin),
+                      COLL_PENALTY[self.coll_penalty], a[12].shape[0], _p(a[12]), a[13].shape[0], _p(a[13]))
         return d, arrs
 
 

@@ -550,3 +550,65 @@ def test_group_tuning_is_all_or_nothing(ctx, mb900):
     g.solve(8, C0_PARAMS, use_graph=False, from_seed=True)
     for r, b in zip(ref, batches):
         assert np.array_equal(b.get()["q"], r)
+
+
+# ----------------------------------------------------------------------------- SceneCollisionTask
+@pytest.mark.parametrize("robot,penalty,form", [("panda_dh", "smooth", "pose"), ("panda_dh", "plain", "pose"),
+                                                ("panda_dh", "surface_distance", "pose"), ("ur10", "smooth", "pose"),
+                                                ("panda_dh", "smooth", "pos_rot")])
+def test_scene_collision_parity(ctx, robot, penalty, form):
+    """SceneCollisionTask rows (terms/dense/scene_collision_task.py:33-81, 550-646; penalties of
+    sphere_collision_kernels.py:19-42) beside a FrameTask target (or the presets' PositionTask +
+    RotationTask pair) and PositionLimit, in the one-lane-per-problem register kernel: every step
+    shadow-checked against the oracle's COLLISION rows; zero iterations score the seeds like the
+    oracle; the plain per-term launch refuses the term"""
+    tb = W.config3(seed=12, batch=4, T=5, arm="panda" if robot == "panda_dh" else "ur10")
+    spec, sc = tb.spec, tb.scene
+    na = spec.num_actuated_joints
+    B, iters, margin = 64, 20, 0.05
+    wts = np.linspace(0.5, 2.0, len(sc.sphere_links)).astype(np.float32)
+    ee = spec.num_links - 1
+    wl = W.ik_batch(spec, [ee], B, seed=13)
+    base = dict(coll_links=list(sc.sphere_links), coll_centers=sc.sphere_centers, coll_radii=sc.sphere_radii,
+                coll_weights=wts, coll_margin=margin, coll_penalty=penalty, obstacle_spheres=sc.obstacle_spheres,
+                obstacle_boxes=sc.obstacle_boxes, limit_weights=np.ones(na, np.float32))
+    scene = O.CollisionScene(sc.sphere_links, sc.sphere_centers, sc.sphere_radii, wts, margin=margin, penalty=penalty,
+                             obstacle_spheres=sc.obstacle_spheres, obstacle_boxes=sc.obstacle_boxes)
+    if form == "pose":
+        ts = N.TermSpec(pose_links=[ee], pose_weights=np.ones((1, 6), np.float32), **base)
+        terms = O.ik_terms([ee], wl.targets, na)
+        tgt = dict(pose_targets=wl.targets)
+    else:
+        ts = N.TermSpec(pos_links=[ee], pos_weights=np.full((1, 3), 20.0, np.float32), rot_links=[ee],
+                        rot_weights=np.full((1, 3), 10.0, np.float32), **base)
+        terms = [O.Term(O.POSITION, [ee], wl.targets, np.full(3, 20.0, np.float32)),
+                 O.Term(O.ROTATION, [ee], wl.targets, np.full(3, 10.0, np.float32)),
+                 O.Term(O.LIMIT, (), None, np.ones(na, np.float32))]
+        tgt = dict(pos_targets=wl.targets, rot_targets=wl.targets)
+    terms = terms + [O.Term(O.COLLISION, (), None, None, None, scene)]
+    b = N.Batch(ctx, spec, ts, B, trace_iters=iters)
+    b.set(q=wl.q_seed, **tgt)
+    b.solve(iters, C0_PARAMS)
+    gpu = b.get(traces=True)
+    q_it, lam_it = b.iterates()
+    rows = np.repeat(np.arange(B), iters)
+    p = O.LMParams(**C0_PARAMS.__dict__)
+    args = (spec, terms, q_it.reshape(-1, na), lam_it.reshape(-1), p)
+    atol = 1e-6 * (20.0 if form == "pos_rot" else 2.0)
+    rep = shadow_check(gpu["cost_trace"], gpu["accept"], O.lm_step(*args, rows=rows, threads=8),
+                       O.lm_step(*args, dtype="f64", rows=rows, threads=8), atol=atol)
+    # the collision rows matter: their share of the oracle's cost at the seeds
+    c_all = O.lm_solve(spec, terms, wl.q_seed, 0, p, dtype="f64", traces=False)["cost"]
+    c_wo = O.lm_solve(spec, terms[:-1], wl.q_seed, 0, p, dtype="f64", traces=False)["cost"]
+    active = int(np.sum(c_all - c_wo > 1e-9))
+    print(f"\ncollision {robot} {penalty} {form}: {rep}; collision rows active at the seeds of {active}/{B} "
+          f"problems, share of the seed cost {(c_all - c_wo).sum() / c_all.sum():.3f}")
+    assert rep.ok(), rep
+    assert active >= B // 8  # the collision rows take part: the check is not vacuous
+    b.set(q=wl.q_seed, **tgt)
+    b.solve(0, C0_PARAMS)
+    np.testing.assert_allclose(b.get()["cost"], c_all, rtol=1e-5, atol=1e-8)
+    with pytest.raises(ValueError):
+        b.solve(iters, C0_PARAMS, mode=N.MODE_PLAIN)
+    with pytest.raises(ValueError):  # the LDS-staged kernels have no collision rows
+        b.solve(iters, C0_PARAMS, tpp=8)

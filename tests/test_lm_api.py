@@ -153,3 +153,57 @@ def test_rotation_position_tasks_solve(ctx):
     var, costs = cfg.build(terms).solve(var)
     pe, ae, ok = L.ik_metrics(robot, var.get(L.VarKey.ROBOT).q, wl.links, T)
     assert ok.mean() > 0.8, (pe.max(), ae.max())
+
+
+def _collision_robot():
+    tb = W.config3(seed=12, batch=2, T=5)
+    sc = tb.scene
+    robot = L.Robot(tb.spec, collision_spheres=(sc.sphere_links, sc.sphere_centers, sc.sphere_radii))
+    scene = L.PrimitiveScene(sc.obstacle_spheres, sc.obstacle_boxes)
+    return robot, scene
+
+
+def test_scene_collision_task_validation():
+    """SceneCollisionTask (terms/dense/scene_collision_task.py:281-356): the penalty is checked, a
+    robot without collision spheres is refused (RuntimeError), a robot-free construction is a
+    template, sphere subsets and per-sphere weights map onto the term's rows"""
+    robot, scene = _collision_robot()
+    with pytest.raises(ValueError):
+        L.SceneCollisionTask(robot, scene, penalty="bogus")
+    with pytest.raises(RuntimeError):
+        L.SceneCollisionTask(L.Robot(W.panda_dh()), scene)
+    assert L.SceneCollisionTask(scene=scene, weight=2.0).is_template
+    t = L.SceneCollisionTask(robot, scene, weight=np.arange(1, 4, dtype=np.float32), sphere_indices=[0, 5, 7],
+                             margin=0.05)
+    assert t.residual_dim == 3 and np.array_equal(t.weights, [1, 2, 3])
+    assert list(t.sphere_links) == list(robot.collision_spheres[0][[0, 5, 7]])
+    with pytest.raises(ValueError):
+        L.SceneCollisionTask(robot, scene, weight=[1.0, 2.0], sphere_indices=[0, 5, 7])
+    with pytest.raises(NotImplementedError):
+        L.SceneCollisionTask(robot, scene, residual_mode="sqrt_abs")
+    assert scene.geoms == len(scene.spheres) + len(scene.boxes)
+
+
+@pytest.mark.gpu
+def test_scene_collision_through_the_optimizer(ctx):
+    """FrameTask + PositionLimit + SceneCollisionTask through LMOptimizer equals the batch API's
+    solve of the same term structure (the register kernel with collision rows), bitwise"""
+    robot, scene = _collision_robot()
+    spec = robot.spec
+    wl = W.ik_batch(spec, [spec.num_links - 1], 48, seed=21)
+    terms = [L.FrameTask(robot, wl.links[0], wl.targets[:, 0]), L.PositionLimit(robot),
+             L.SceneCollisionTask(robot, scene, weight=2.0, margin=0.05)]
+    cfg = L.LMOptimizerConfig(max_iter=20, early_stopping_interval=0, lm_lambda=1e-2, lambda_factor=10.0)
+    var = L.VarValues({L.VarKey.ROBOT: robot.state(wl.q_seed)})
+    var, costs = cfg.build(terms).solve(var)
+    links, centers, radii = robot.collision_spheres
+    ts = N.TermSpec(pose_links=wl.links, pose_weights=np.ones((1, 6), np.float32), limit_weights=np.ones(7, np.float32),
+                    coll_links=list(links), coll_centers=centers, coll_radii=radii,
+                    coll_weights=np.full(len(links), 2.0, np.float32), coll_margin=0.05,
+                    obstacle_spheres=scene.spheres, obstacle_boxes=scene.boxes)
+    b = N.Batch(ctx, spec, ts, 48)
+    b.set(q=wl.q_seed, pose_targets=wl.targets)
+    b.solve(20, cfg.params())
+    ref = b.get()
+    np.testing.assert_array_equal(var.get(L.VarKey.ROBOT).q, ref["q"])
+    np.testing.assert_array_equal(costs, ref["cost"])

@@ -509,3 +509,87 @@ def test_panda_tcp_hand_matches_the_reference_ground_truth():
                 assert min(np.abs(t[3:] - ref[name][3:]).max(), np.abs(t[3:] + ref[name][3:]).max()) < 1e-5, name
                 n += 1
     assert n >= 4
+
+
+# ----------------------------------------------------------------------------- SceneCollisionTask
+def _collision_case(penalty, seed=0):
+    """the Panda-DH arm with spheres along its links and a scene around it (config3's), plus a pose
+    target so that the collision rows sit beside another term"""
+    tb = W.config3(seed=seed, batch=4, T=5)
+    sc = tb.scene
+    scene = O.CollisionScene(sc.sphere_links, sc.sphere_centers, sc.sphere_radii,
+                             np.linspace(0.5, 2.0, len(sc.sphere_links)).astype(np.float32), margin=0.05,
+                             penalty=penalty, obstacle_spheres=sc.obstacle_spheres, obstacle_boxes=sc.obstacle_boxes)
+    return tb.spec, scene
+
+
+def _np_scene_sdf(scene, p):
+    best = 1e6
+    for o in np.asarray(scene.obstacle_spheres).reshape(-1, 4):
+        best = min(best, np.linalg.norm(p - o[:3]) - o[3])
+    for o in np.asarray(scene.obstacle_boxes).reshape(-1, 6):
+        qv = np.abs(p - o[:3]) - o[3:]
+        best = min(best, np.linalg.norm(np.maximum(qv, 0)) + min(qv.max(), 0))
+    return best
+
+
+def _np_penalty(sd, margin, mode):
+    """sphere_collision_kernels.py:19-42, written out"""
+    if mode == "surface_distance":
+        return abs(sd)
+    if mode == "plain":
+        return margin - sd if sd < margin else 0.0
+    if sd < 0:
+        return 0.5 * margin - sd
+    if sd <= margin:
+        return 0.5 / (margin + 1e-6) * (sd - margin) ** 2
+    return 0.0
+
+
+@pytest.mark.parametrize("penalty", ["smooth", "plain", "surface_distance"])
+def test_collision_rows_closed_forms(penalty):
+    """row k = w_k pen(sd_k), sd_k = scene distance of the FK sphere centre - radius; H = J^T J"""
+    spec, scene = _collision_case(penalty)
+    rng = np.random.default_rng(7)
+    checked = 0
+    for trial in range(4):
+        q = W.uniform_q(rng, spec, 1)[0].astype(np.float64)
+        terms = [O.Term(O.COLLISION, (), None, None, None, scene)]
+        r, J, H, g, cost = O.residual_jacobian(spec, terms, 0, q, dtype="f64")
+        Tl, _ = O.fk(spec, q[None], dtype="f64")
+        for k in range(len(scene.sphere_links)):
+            c = O.se3("apply", Tl[0, scene.sphere_links[k]][None], scene.sphere_centers[k][None].astype(np.float64),
+                      dtype="f64")[0]
+            sd = _np_scene_sdf(scene, c) - scene.sphere_radii[k]
+            margin = float(np.float32(scene.margin))  # the term stores it in float32
+            assert r[k] == pytest.approx(scene.weights[k] * _np_penalty(sd, margin, penalty), abs=1e-9)
+            checked += r[k] != 0
+        np.testing.assert_allclose(H, J.T @ J, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(g, J.T @ r, rtol=1e-12, atol=1e-12)
+        assert cost == pytest.approx(0.5 * r @ r, rel=1e-12)
+    assert checked > 5  # some spheres inside the margin: the check is not vacuous
+
+
+@pytest.mark.parametrize("penalty", ["smooth", "plain", "surface_distance"])
+def test_collision_jacobian_finite_differences(penalty):
+    """the analytic rows w pen'(sd) n . (v + w x c) equal central differences of the residual, away
+    from kinks (a sphere crossing the margin or switching its closest primitive)"""
+    spec, scene = _collision_case(penalty, seed=1)
+    terms = [O.Term(O.COLLISION, (), None, None, None, scene)]
+    rng = np.random.default_rng(8)
+    h, checked = 1e-6, 0
+    for trial in range(6):
+        q = W.uniform_q(rng, spec, 1)[0].astype(np.float64)
+        r, J, *_ = O.residual_jacobian(spec, terms, 0, q, dtype="f64")
+        Jfd = np.zeros_like(J)
+        for a in range(q.size):
+            qp, qm = q.copy(), q.copy()
+            qp[a] += h
+            qm[a] -= h
+            Jfd[:, a] = (O.residual_jacobian(spec, terms, 0, qp, dtype="f64")[0]
+                         - O.residual_jacobian(spec, terms, 0, qm, dtype="f64")[0]) / (2 * h)
+        rows = np.nonzero(np.abs(r) > 1e-4)[0]
+        good = np.all(np.abs(J[rows] - Jfd[rows]) < 1e-5, axis=1)
+        assert good.sum() >= 0.9 * rows.size, (rows[~good], np.abs(J - Jfd).max(1)[rows[~good]])
+        checked += int(good.sum())
+    assert checked > 10
