@@ -282,6 +282,12 @@ typedef struct {
     const float* limit_weights;
     const float* limit_lower;
     const float* limit_upper;
+    /* linear solver of each LM step: 0 the band Cholesky; n > 0 n iterations of conjugate
+     * gradients on (J^T J + lam I) dq = -g from dq = 0 (LMOptimizerConfig(linear_solver="cg",
+     * cg_max_iter=n), opt/lm_optimizer.py:781-832), always the full budget: the reference's CG
+     * early exit is a host check of the whole batch, which it also skips under graph capture
+     * and with early_stopping_interval = 0.  Predicted reduction -g.dq - dq^T J^T J dq / 2. */
+    int32_t cg_max_iter;
 } rk_traj_desc;
 
 int rk_traj_batch_create(rk_context* ctx, const rk_traj_desc* desc, int32_t capacity, int32_t trace_iters,

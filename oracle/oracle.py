@@ -362,7 +362,7 @@ class _Traj(C.Structure):
                 ("sph_link", C.c_void_p), ("sph_c", C.c_void_p), ("sph_r", C.c_void_p), ("coll_w", C.c_float),
                 ("margin", C.c_float), ("nobs_s", C.c_int), ("obs_s", C.c_void_p), ("nobs_b", C.c_int),
                 ("obs_b", C.c_void_p), ("lim_orders", C.c_int), ("lim_w", C.c_void_p), ("lim_lo", C.c_void_p),
-                ("lim_hi", C.c_void_p)]
+                ("lim_hi", C.c_void_p), ("cg_iters", C.c_int)]
 
 
 class OracleTraj:
@@ -389,7 +389,7 @@ class OracleTraj:
                             tb.wa if wa is None else wa, tb.dt, len(k[2]), _ptr(k[2]), _ptr(k[3]), _ptr(k[4]),
                             tb.coll_w if coll_w is None else coll_w, tb.margin if margin is None else margin,
                             len(k[5]), _ptr(k[5]), len(k[6]), _ptr(k[6]), orders, _ptr(lim[0]), _ptr(lim[1]),
-                            _ptr(lim[2]))
+                            _ptr(lim[2]), int(getattr(tb, "cg_iters", 0) or 0))
         self.tb = tb
 
     def rows(self):
@@ -440,6 +440,19 @@ def traj_lm_step(ot: OracleTraj, q: np.ndarray, lam: np.ndarray, params: LMParam
                          _ptr(out["cost_prop"]), _ptr(out["pred"]), _ptr(acc), _ptr(qn), _ptr(out["lam_new"]))
     out.update(acc=acc, q_new=qn)
     return out
+
+
+def traj_delta(ot: OracleTraj, b: int, q: np.ndarray, lam: float, dtype: str = "f32"):
+    """(delta [T*na], pred) of problem b's linear step at q with damping lam: conjugate gradients
+    when the problem set has cg_iters > 0, else the damped Cholesky"""
+    lib, rt = _lib(dtype), _real(dtype)
+    rb = OracleRobot(ot.tb.spec)
+    q = np.ascontiguousarray(q, rt).reshape(-1)
+    delta = np.zeros(q.shape[0], rt)
+    pred = (C.c_float if dtype == "f32" else C.c_double)()
+    cast = C.c_float if dtype == "f32" else C.c_double
+    lib.ork_traj_delta(C.byref(rb.struct), C.byref(ot.struct), int(b), _ptr(q), cast(lam), _ptr(delta), C.byref(pred))
+    return delta, float(pred.value)
 
 
 def traj_lm_solve(ot: OracleTraj, q_seed: np.ndarray, iters: int, params: LMParams, dtype: str = "f32"):

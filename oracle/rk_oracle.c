@@ -1097,6 +1097,7 @@ typedef struct {
     const float *obs_b;        /* [n][6] centre, half extents */
     int lim_orders;            /* bit o: TrajectoryLimitTask rows of order o, forward stencil */
     const float *lim_w, *lim_lo, *lim_hi; /* [3][na] */
+    int cg_iters;              /* > 0: conjugate-gradient steps (traj_cg) instead of the Cholesky */
 } ork_traj;
 
 static int traj_rows(const ork_robot *rb, const ork_traj *tj, int *Rg, int *Rv, int *Ra, int *Rc, int *Rl) {
@@ -1131,8 +1132,17 @@ typedef struct { int n; int col[8]; real val[8]; } traj_row;
  * r = w max(max(lo - v, v - hi), 0), Jacobian w sign c_k / dt^o on an active bound (sign +1 above,
  * -1 below), nothing otherwise.
  */
+static int traj_eval(const ork_robot *rb, const ork_traj *tj, int b, const real *q, real *r, real *J, real *H,
+                     real *g, real *cost_terms, traj_row *rows_out);
+
 EXPORT int ork_traj_residual_jacobian(const ork_robot *rb, const ork_traj *tj, int b, const real *q, real *r,
                                       real *J, real *H, real *g, real *cost_terms) {
+    return traj_eval(rb, tj, b, q, r, J, H, g, cost_terms, NULL);
+}
+
+/* as ork_traj_residual_jacobian; rows_out [R] (nullable) receives every row's sparse entries */
+static int traj_eval(const ork_robot *rb, const ork_traj *tj, int b, const real *q, real *r, real *J, real *H,
+                     real *g, real *cost_terms, traj_row *rows_out) {
     const int na = rb->na, T = tj->T, D = T * na;
     int Rg, Rv, Ra, Rc, Rl;
     const int R = traj_rows(rb, tj, &Rg, &Rv, &Ra, &Rc, &Rl);
@@ -1151,6 +1161,7 @@ EXPORT int ork_traj_residual_jacobian(const ork_robot *rb, const ork_traj *tj, i
     do {                                                                                      \
         r[row] = (rv);                                                                        \
         ct[term] += r[row] * r[row];                                                          \
+        if (rows_out) rows_out[row] = rw;                                                     \
         for (int u = 0; u < rw.n; ++u) {                                                      \
             if (J) J[(size_t)row * D + rw.col[u]] = rw.val[u];                                \
             if (g) g[rw.col[u]] = g[rw.col[u]] + rw.val[u] * r[row];                          \
@@ -1286,19 +1297,81 @@ static real traj_cost(const ork_robot *rb, const ork_traj *tj, int b, const real
     return traj_cost_sum(ct);
 }
 
+/*
+ * Matrix-free conjugate gradients on (J^T J + lam I) delta = -g over the sparse rows, the
+ * reference's CG step (_compute_sparse_delta, opt/lm_optimizer.py:781-832, with the kernels of
+ * :370-499): delta = 0, r = p = -g; per iteration Ap = J^T (J p) + lam p (_apply_linear_operator,
+ * add_lambda_parallel_kernel), alpha = r.r / (p.Ap + 1e-8), delta += alpha p, r -= alpha Ap,
+ * beta = r'.r' / (r.r + 1e-8), p = r' + beta p; `iters` iterations (the reference's budget without
+ * its host-side early exit).  Then pred = -g.delta - delta.(J^T J delta) / 2
+ * (compute_predicted_reduction_fixed_kernel).  work: 4 D + R reals.
+ */
+static void traj_cg(int D, int R, const traj_row *rows, const real *g, real lam, int iters, real *x, real *pred,
+                    real *work) {
+    real *res = work, *p = res + D, *Ap = p + D, *jtj = Ap + D, *jx = jtj + D;
+    /* out = J^T (J v) (+ lam v when damped) */
+    #define CG_APPLY(v, out, damped)                                                        \
+        do {                                                                                \
+            for (int i_ = 0; i_ < R; ++i_) {                                                \
+                real a_ = 0;                                                                \
+                for (int u_ = 0; u_ < rows[i_].n; ++u_) a_ += rows[i_].val[u_] * (v)[rows[i_].col[u_]]; \
+                jx[i_] = a_;                                                                \
+            }                                                                               \
+            for (int d_ = 0; d_ < D; ++d_) jtj[d_] = 0;                                     \
+            for (int i_ = 0; i_ < R; ++i_)                                                  \
+                for (int u_ = 0; u_ < rows[i_].n; ++u_)                                     \
+                    jtj[rows[i_].col[u_]] = jtj[rows[i_].col[u_]] + rows[i_].val[u_] * jx[i_]; \
+            for (int d_ = 0; d_ < D; ++d_) (out)[d_] = (damped) ? jtj[d_] + lam * (v)[d_] : jtj[d_]; \
+        } while (0)
+    real rr = 0;
+    for (int d = 0; d < D; ++d) {
+        x[d] = 0;
+        res[d] = -g[d];
+        p[d] = res[d];
+        rr += res[d] * res[d];
+    }
+    for (int k = 0; k < iters; ++k) {
+        CG_APPLY(p, Ap, 1);
+        real pap = 0;
+        for (int d = 0; d < D; ++d) pap += p[d] * Ap[d];
+        const real alpha = rr / (pap + (real)1e-8);
+        real nr = 0;
+        for (int d = 0; d < D; ++d) {
+            x[d] = x[d] + alpha * p[d];
+            res[d] = res[d] - alpha * Ap[d];
+            nr += res[d] * res[d];
+        }
+        const real beta = nr / (rr + (real)1e-8);
+        rr = nr;
+        for (int d = 0; d < D; ++d) p[d] = res[d] + beta * p[d];
+    }
+    CG_APPLY(x, Ap, 0);
+    #undef CG_APPLY
+    real gd = 0, qd = 0;
+    for (int d = 0; d < D; ++d) {
+        gd += x[d] * g[d];
+        qd += x[d] * Ap[d];
+    }
+    *pred = -gd - (real)0.5 * qd;
+}
+
 /* one LM step of trajectory problem b from (qb, lam); as the body of ork_lm_step */
-typedef struct { real *r, *H, *L, *g, *dl, *qp; } traj_ws;
+typedef struct { real *r, *H, *L, *g, *dl, *qp, *cgw; traj_row *rows; } traj_ws;
 
 static void traj_step_one(const ork_robot *rb, const ork_traj *tj, const ork_lm_params *p, int b, const real *qb,
                           real lam, traj_ws *w, real *cost_cur, real *cost_prop, real *pred_out, int32_t *acc_out,
                           real *q_new, real *lam_new) {
     const int D = tj->T * rb->na;
     real ct[7];
-    ork_traj_residual_jacobian(rb, tj, b, qb, w->r, NULL, w->H, w->g, ct);
+    const int cg = tj->cg_iters > 0;
+    const int R = traj_eval(rb, tj, b, qb, w->r, NULL, cg ? NULL : w->H, w->g, ct, cg ? w->rows : NULL);
     real cost = traj_cost_sum(ct);
     *cost_cur = cost;
     real l = lam, pred;
-    damped_cholesky(D, w->H, w->g, l, w->L, w->dl, &pred);
+    if (cg)
+        traj_cg(D, R, w->rows, w->g, l, tj->cg_iters, w->dl, &pred, w->cgw);
+    else
+        damped_cholesky(D, w->H, w->g, l, w->L, w->dl, &pred);
     for (int d = 0; d < D; ++d) w->qp[d] = qb[d] + w->dl[d];
     real cp = traj_cost(rb, tj, b, w->qp, w->r);
     *cost_prop = cp;
@@ -1320,9 +1393,30 @@ static void traj_ws_alloc(traj_ws *w, const ork_robot *rb, const ork_traj *tj) {
     w->g = (real *)malloc(sizeof(real) * D);
     w->dl = (real *)malloc(sizeof(real) * D);
     w->qp = (real *)malloc(sizeof(real) * D);
+    w->cgw = (real *)malloc(sizeof(real) * (4 * (size_t)D + R));
+    w->rows = (traj_row *)malloc(sizeof(traj_row) * R);
 }
 
-static void traj_ws_free(traj_ws *w) { free(w->r); free(w->H); free(w->L); free(w->g); free(w->dl); free(w->qp); }
+static void traj_ws_free(traj_ws *w) {
+    free(w->r); free(w->H); free(w->L); free(w->g); free(w->dl); free(w->qp); free(w->cgw); free(w->rows);
+}
+
+/* the linear step alone of problem b at q with damping lam: delta [D] and pred (the CG or the
+ * Cholesky step, as tj->cg_iters selects) */
+EXPORT void ork_traj_delta(const ork_robot *rb, const ork_traj *tj, int b, const real *q, real lam, real *delta,
+                           real *pred) {
+    const int D = tj->T * rb->na;
+    traj_ws w;
+    traj_ws_alloc(&w, rb, tj);
+    real ct[7];
+    const int cg = tj->cg_iters > 0;
+    const int R = traj_eval(rb, tj, b, q, w.r, NULL, cg ? NULL : w.H, w.g, ct, cg ? w.rows : NULL);
+    if (cg)
+        traj_cg(D, R, w.rows, w.g, lam, tj->cg_iters, delta, pred, w.cgw);
+    else
+        damped_cholesky(D, w.H, w.g, lam, w.L, delta, pred);
+    traj_ws_free(&w);
+}
 
 /* one LM step of each trajectory problem from (q[b], lam[b]); as ork_lm_step */
 EXPORT void ork_traj_lm_step(const ork_robot *rb, const ork_traj *tj, const ork_lm_params *p, int B, const real *q,

@@ -2431,6 +2431,8 @@ static int make_traj_terms(rk_context* ctx, const rk_traj_desc* d, TrajTerms* ou
       if (!(tt.lim_lo[o][k] <= tt.lim_hi[o][k])) return fail(RK_EINVAL, "limit bounds must be ordered, non-NaN");
     }
   }
+  if (d->cg_max_iter < 0 || d->cg_max_iter > 4096) return fail(RK_EINVAL, "cg_max_iter must be 0..4096");
+  tt.cg_iters = d->cg_max_iter;
   return RK_OK;
 }
 

@@ -37,6 +37,9 @@ struct TrajTerms {
   int lim_orders;
   float lim_scale[3];  // 1 / dt^o
   float lim_w[3][8], lim_lo[3][8], lim_hi[3][8];
+  // > 0: each LM step solves by this many conjugate-gradient iterations instead of the band
+  // Cholesky (LMOptimizerConfig(linear_solver="cg"), opt/lm_optimizer.py:781-832)
+  int cg_iters;
 };
 typedef const __attribute__((address_space(4))) TrajTerms TrajK;
 
@@ -522,13 +525,87 @@ __device__ __forceinline__ void traj_limits(TrajK* tt, int t, int T, const float
   }
 }
 
+// Conjugate gradients on (J^T J + lam I) dq = -g with a fixed budget of `iters` iterations, the
+// reference's CG step (_compute_sparse_delta, opt/lm_optimizer.py:781-832; its kernels :370-499;
+// eps 1e-8 in both ratios) run without its early exit, as the reference does under graph capture
+// or with early_stopping_interval = 0.  The matrix is block pentadiagonal over frames: lane t
+// holds its frame's diagonal block (hb lower triangle + dg on the diagonal) and the diagonal
+// coupling blocks o1 (to frame t-1) and o2 (to frame t-2); the couplings of frames t+1 and t+2
+// to t are those lanes' o1 / o2, and every neighbour vector comes by a one-lane DPP shift, so the
+// product never leaves registers.  Lanes past the last frame hold zeros.  Returns dq in x and
+// the reference's predicted reduction -g.dq - dq^T (J^T J) dq / 2.
+template <int NA>
+__device__ __forceinline__ float traj_cg(const float (&hb)[NA][NA], const float (&dg)[NA], const float (&o1)[NA],
+                                         const float (&o2)[NA], const float (&gf)[NA], float lam, bool own, int iters,
+                                         float (&x)[NA]) {
+  float o1n[NA], o2n[NA];
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+    o1n[i] = wv_shl1(o1[i], 0.f);
+    o2n[i] = wv_shl1(wv_shl1(o2[i], 0.f), 0.f);
+  }
+  auto apply = [&](const float (&v)[NA], float l, float (&y)[NA]) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const float m1 = wv_shr1(v[i], 0.f), p1 = wv_shl1(v[i], 0.f);
+      const float m2 = wv_shr1(m1, 0.f), p2 = wv_shl1(p1, 0.f);
+      float acc = (dg[i] + l) * v[i];
+#pragma unroll
+      for (int j = 0; j < NA; ++j)
+        if (j != i) acc = fmaf(j < i ? hb[i][j] : hb[j][i], v[j], acc);
+      acc = fmaf(o1[i], m1, acc);
+      acc = fmaf(o2[i], m2, acc);
+      acc = fmaf(o1n[i], p1, acc);
+      acc = fmaf(o2n[i], p2, acc);
+      y[i] = own ? acc : 0.f;
+    }
+  };
+  float r[NA], p[NA], Ap[NA];
+  float rr = 0.f;
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+    x[i] = 0.f;
+    r[i] = own ? -gf[i] : 0.f;
+    p[i] = r[i];
+    rr = fmaf(r[i], r[i], rr);
+  }
+  rr = wv_sum(rr);
+  for (int k = 0; k < iters; ++k) {
+    apply(p, lam, Ap);
+    float pap = 0.f;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) pap = fmaf(p[i], Ap[i], pap);
+    const float alpha = rr / (wv_sum(pap) + 1e-8f);
+    float nr = 0.f;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      x[i] = fmaf(alpha, p[i], x[i]);
+      r[i] = fmaf(-alpha, Ap[i], r[i]);
+      nr = fmaf(r[i], r[i], nr);
+    }
+    nr = wv_sum(nr);
+    const float beta = nr / (rr + 1e-8f);
+    rr = nr;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) p[i] = fmaf(beta, p[i], r[i]);
+  }
+  apply(x, 0.f, Ap);  // J^T J dq, undamped
+  float gd = 0.f, qd = 0.f;
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+    gd = fmaf(x[i], own ? gf[i] : 0.f, gd);
+    qd = fmaf(x[i], Ap[i], qd);
+  }
+  return -wv_sum(gd) - 0.5f * wv_sum(qd);
+}
+
 template <int NA, int NJ, bool LIM = false>
 __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s, const LMParams& lp, int it0, int iters,
                                            TrajLDS<NA>* sh) {
   constexpr int W = TrajLDS<NA>::W;
   RobotK* rb = as_const(Pg.robot);
   const int lane = threadIdx.x & 63;
-  const int T = tt->T, D = T * NA, S = Pg.stride;
+  const int T = tt->T, D = T * NA, S = Pg.stride, cgi = tt->cg_iters;
   const int t = lane < T ? lane : T - 1;
   const bool own = lane < T;  // lanes >= T mirror frame T-1 and contribute nothing
   const bool last = lane == T - 1;
@@ -631,34 +708,56 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
       if (!own) cg = cc = cv = ca = 0.f;
       cost = limit_cost(((0.5f * wv_sum(cg) + 0.5f * wv_sum(cv)) + 0.5f * wv_sum(ca)) + 0.5f * wv_sum(cc), cl);
     }
-    // ---- band of J^T J + lam I (rows of frame t), right-hand side J^T r
-    if (own) {
+    float dq[NA], qp[NA];
+    float pred;
+    if (cgi > 0) {
+      // ---- conjugate gradients in registers; the step goes to the second wavefront in rhs
+      float dg[NA], o1[NA], o2[NA];
 #pragma unroll
       for (int i = 0; i < NA; ++i) {
-        const int row = t * NA + i;
-#pragma unroll
-        for (int kk = 0; kk <= W; ++kk) {
-          float v = 0.f;
-          if (kk <= i) v = hb[i][i - kk];
-          if (kk == 0) v = LIM ? v + hs0 + hl0[i] + lam : v + hs0 + lam;
-          if (kk == NA) v = LIM ? hs1m + hl1[i] : hs1m;
-          if (kk == 2 * NA) v = LIM ? hs2m + hl2[i] : hs2m;
-          sh->band[row][kk] = v;
-        }
-        sh->rhs[row] = -gf[i];
+        dg[i] = LIM ? hb[i][i] + hs0 + hl0[i] : hb[i][i] + hs0;
+        o1[i] = own ? (LIM ? hs1m + hl1[i] : hs1m) : 0.f;
+        o2[i] = own ? (LIM ? hs2m + hl2[i] : hs2m) : 0.f;
       }
-    }
-    __syncthreads();  // band built
-    band_solve<NA>(sh, T, 0);
-    float dq[NA], qp[NA];
-    float pr = 0.f;
+      __syncthreads();  // (pairs with the second wavefront's "band built")
+      pred = traj_cg<NA>(hb, dg, o1, o2, gf, lam, own, cgi, dq);
+      if (own)
 #pragma unroll
-    for (int d = 0; d < NA; ++d) {
-      dq[d] = sh->rhs[t * NA + d];
-      qp[d] = q[d] + dq[d];
-      pr += dq[d] * (lam * dq[d] - gf[d]);
+        for (int d = 0; d < NA; ++d) sh->rhs[t * NA + d] = dq[d];
+      __syncthreads();  // (the three barriers of the band solve)
+      __syncthreads();
+      __syncthreads();
+#pragma unroll
+      for (int d = 0; d < NA; ++d) qp[d] = q[d] + dq[d];
+    } else {
+      // ---- band of J^T J + lam I (rows of frame t), right-hand side J^T r
+      if (own) {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+          const int row = t * NA + i;
+#pragma unroll
+          for (int kk = 0; kk <= W; ++kk) {
+            float v = 0.f;
+            if (kk <= i) v = hb[i][i - kk];
+            if (kk == 0) v = LIM ? v + hs0 + hl0[i] + lam : v + hs0 + lam;
+            if (kk == NA) v = LIM ? hs1m + hl1[i] : hs1m;
+            if (kk == 2 * NA) v = LIM ? hs2m + hl2[i] : hs2m;
+            sh->band[row][kk] = v;
+          }
+          sh->rhs[row] = -gf[i];
+        }
+      }
+      __syncthreads();  // band built
+      band_solve<NA>(sh, T, 0);
+      float pr = 0.f;
+#pragma unroll
+      for (int d = 0; d < NA; ++d) {
+        dq[d] = sh->rhs[t * NA + d];
+        qp[d] = q[d] + dq[d];
+        pr += dq[d] * (lam * dq[d] - gf[d]);
+      }
+      pred = 0.5f * wv_sum(own ? pr : 0.f);
     }
-    const float pred = 0.5f * wv_sum(own ? pr : 0.f);
     const float cp = eval_cost(qp);
     if (Pg.trace && gi == 0 && lane == 0) Pg.trace[s] = cost;
     const bool acc = lm_accept(lp, cost, cp, pred, lam);
@@ -741,7 +840,13 @@ __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s
       __syncthreads();  // rows ready
     }
     __syncthreads();  // band built
-    band_solve<NA>(sh, T, 1);
+    if (tt->cg_iters > 0) {
+      __syncthreads();  // (the first wavefront runs the conjugate gradients alone)
+      __syncthreads();
+      __syncthreads();
+    } else {
+      band_solve<NA>(sh, T, 1);
+    }
     float qp[NA];
 #pragma unroll
     for (int d = 0; d < NA; ++d) qp[d] = q[d] + sh->rhs[t * NA + d];

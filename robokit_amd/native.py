@@ -70,13 +70,13 @@ class TrajDescC(C.Structure):
                 ("coll_weight", C.c_float), ("coll_margin", C.c_float), ("n_obs_spheres", C.c_int32),
                 ("obs_spheres", C.c_void_p), ("n_obs_boxes", C.c_int32), ("obs_boxes", C.c_void_p),
                 ("limit_orders", C.c_int32), ("limit_weights", C.c_void_p), ("limit_lower", C.c_void_p),
-                ("limit_upper", C.c_void_p)]
+                ("limit_upper", C.c_void_p), ("cg_max_iter", C.c_int32)]
 
 
 def traj_desc(robot_id: int, tb, goal_weights=None):
     """(rk_traj_desc, arrays it points at) of a workloads.TrajBatch-shaped problem set: robot,
     frames, scene, weights, and the optional bound rows (lim_orders bit o, lim_w / lim_lo / lim_hi
-    [3, na])"""
+    [3, na]) and the CG budget (cg_iters: 0 = band Cholesky)"""
     sc = tb.scene
     na = tb.spec.num_actuated_joints
     arrs = [_f32(np.ones(6) if goal_weights is None else goal_weights), _i32(sc.sphere_links),
@@ -89,7 +89,7 @@ def traj_desc(robot_id: int, tb, goal_weights=None):
     arrs += lim
     d = TrajDescC(robot_id, tb.T, tb.goal_link, _p(arrs[0]), tb.wv, tb.wa, tb.dt, len(arrs[1]), _p(arrs[1]),
                   _p(arrs[2]), _p(arrs[3]), tb.coll_w, tb.margin, len(arrs[4]), _p(arrs[4]), len(arrs[5]), _p(arrs[5]),
-                  orders, _p(lim[0]), _p(lim[1]), _p(lim[2]))
+                  orders, _p(lim[0]), _p(lim[1]), _p(lim[2]), int(getattr(tb, "cg_iters", 0) or 0))
     return d, arrs
 
 

@@ -305,6 +305,15 @@ class TrajBatch:
     lim_w: Optional[np.ndarray] = None
     lim_lo: Optional[np.ndarray] = None
     lim_hi: Optional[np.ndarray] = None
+    # linear solver: 0 the band Cholesky, n > 0 n conjugate-gradient iterations per LM step
+    # (LMOptimizerConfig(linear_solver="cg", cg_max_iter=n); the reference's default budget is
+    # max(4, min(64, T * na // 4)), cg_default_iters)
+    cg_iters: int = 0
+
+    @property
+    def cg_default_iters(self) -> int:
+        """opt/lm_optimizer.py:577: max(4, min(64, tangent_dim // 4))"""
+        return max(4, min(64, self.T * self.spec.num_actuated_joints // 4))
 
     @property
     def size(self) -> int:

@@ -152,3 +152,61 @@ def test_traj_bounds_keep_trajectories_inside(ctx):
     print(f"\nmax |q| unbounded {np.abs(qf).max():.3f}, bounded {np.abs(qb).max():.3f} (bound 1.0, {na} dofs)")
     assert np.abs(qf).max() > 1.5
     assert np.abs(qb).max() < 1.05
+
+
+# conjugate-gradient steps (LMOptimizerConfig(linear_solver="cg")): the reference's default budget
+# max(4, min(64, D // 4)) on 16 and 64 frames, a short problem, the 6-dof arm, bound rows.  FP32 CG
+# loses conjugacy within a few iterations, so accept decisions near rho_min flip against FP64 about
+# as often for the oracle's FP32 CG as for the GPU's; the short problem takes 48 trajectories so
+# the flip count is a statistic rather than one or two events
+@pytest.mark.parametrize("T,B,iters,arm,bounds,cg", [(16, 16, 15, "panda", False, None), (64, 8, 10, "panda", False, None),
+                                                    (5, 48, 12, "panda", False, None), (16, 12, 12, "ur10", True, None),
+                                                    (16, 12, 10, "panda", True, 6)])
+def test_traj_cg_parity(ctx, T, B, iters, arm, bounds, cg):
+    """the register-resident CG step of the trajectory kernel: every GPU step shadow-checked against
+    the oracle's matrix-free CG over the sparse rows (opt/lm_optimizer.py:781-832, same budget), the
+    seed cost to 1e-5 relative of FP64; continued halves equal one launch bitwise"""
+    import dataclasses
+
+    tb = W.config3(seed=21, batch=B, T=T, arm=arm)
+    if bounds:
+        tb = _bounded(tb)
+    tb = dataclasses.replace(tb, cg_iters=cg or tb.cg_default_iters)
+    b = N.TrajBatch(ctx, tb, B, trace_iters=iters)
+    b.set(q=tb.q_seed, goals=tb.goals)
+    b.solve(iters, P)
+    gpu = b.get(traces=True)
+    q_it, lam_it = b.iterates()
+    ot = O.OracleTraj(tb)
+    rows = np.repeat(np.arange(B), iters)
+    qi = q_it.reshape(B * iters, T, -1)
+    p = O.LMParams(**P.__dict__)
+    rep = shadow_check(gpu["cost_trace"], gpu["accept"], O.traj_lm_step(ot, qi, lam_it.reshape(-1), p, rows=rows),
+                       O.traj_lm_step(ot, qi, lam_it.reshape(-1), p, dtype="f64", rows=rows))
+    print(f"\nCG {arm} T={T} budget {tb.cg_iters} bounds {bounds}: {rep}")
+    assert rep.ok(), rep
+    r64 = np.array([O.traj_residual_jacobian(ot, i, tb.q_seed[i], dtype="f64", jac=False)[4].sum() for i in range(B)])
+    np.testing.assert_allclose(gpu["cost_trace"][:, 0], r64, rtol=1e-5)
+    assert gpu["accept"].mean() > 0.3  # the steps are real descent steps, not rejected noise
+    half = iters // 2
+    b.set(q=tb.q_seed, goals=tb.goals)
+    b.solve(half, P)
+    b.solve(iters - half, P, cont=True)
+    assert np.array_equal(b.get()["q"], gpu["q"])
+
+
+def test_traj_cg_solves_the_workload(ctx):
+    """the benchmark's trajectory shape (64 frames, 7 dofs, the scene) on CG steps with the default
+    budget: clearance and goal error improve as with the band Cholesky"""
+    import dataclasses
+
+    tb = W.config3(seed=7, batch=64, T=64)
+    tb = dataclasses.replace(tb, cg_iters=tb.cg_default_iters)
+    b = N.TrajBatch(ctx, tb, 64)
+    b.set(q=tb.q_seed, goals=tb.goals)
+    pe0, ae0, sd0 = b.metrics()
+    b.solve(50, P)
+    pe, ae, sd = b.metrics()
+    print(f"\nCG: min clearance median {np.median(sd0):.3f} -> {np.median(sd):.3f} m, goal error median "
+          f"{np.median(pe0):.3f} -> {np.median(pe):.4f} m")
+    assert np.median(sd) > np.median(sd0) and np.median(pe) < np.median(pe0)

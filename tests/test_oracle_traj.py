@@ -201,3 +201,54 @@ def test_traj_bounds_pull_a_trajectory_inside():
     assert np.abs(q0).max() > 1.0
     sol = O.traj_lm_solve(ot, q0, 30, p, dtype="f64")
     assert np.abs(sol["q"]).max() < 0.6 + 5e-3
+
+
+def _cg(tb, iters):
+    return dataclasses.replace(tb, cg_iters=iters)
+
+
+def test_traj_cg_converges_to_the_cholesky_step():
+    """the CG restatement (opt/lm_optimizer.py:781-832) with a budget of 3 D iterations reaches the
+    damped Cholesky step in FP64, bound rows included: the residual of (J^T J + lam I) d = -g to
+    1e-6 of |g| (the 1e-8 in both CG ratios sets a floor near there), the step to 1e-5 of its
+    largest entry; its predicted reduction -g.d - d^T J^T J d / 2 then equals the Cholesky form
+    d^T (lam d - g) / 2"""
+    tb = _with_bounds(_small(T=6, seed=11, batch=2))
+    D = tb.T * tb.spec.num_actuated_joints
+    lam = 10.0
+    for b in range(2):
+        _, _, H, g, _ = O.traj_residual_jacobian(O.OracleTraj(tb), b, tb.q_seed[b], dtype="f64", jac=False)
+        d_ch, p_ch = O.traj_delta(O.OracleTraj(tb), b, tb.q_seed[b], lam, dtype="f64")
+        d_cg, p_cg = O.traj_delta(O.OracleTraj(_cg(tb, 3 * D)), b, tb.q_seed[b], lam, dtype="f64")
+        assert np.linalg.norm((H + lam * np.eye(D)) @ d_cg + g) <= 1e-6 * np.linalg.norm(g)
+        np.testing.assert_allclose(d_cg, d_ch, rtol=0, atol=1e-5 * np.abs(d_ch).max())
+        assert abs(p_cg - p_ch) <= 1e-6 * abs(p_ch)
+
+
+def test_traj_cg_first_iterations_closed_form():
+    """one CG iteration is the steepest-descent step d = -(g.g / (g.A g + 1e-8)) g with
+    A = J^T J + lam I; two iterations minimise the quadratic over span{g, A g} (a 2x2 solve);
+    the predicted reduction uses the undamped J^T J"""
+    tb = _small(T=5, seed=12, batch=1)
+    lam = 0.3
+    ot = O.OracleTraj(tb)
+    _, _, H, g, _ = O.traj_residual_jacobian(ot, 0, tb.q_seed[0], dtype="f64", jac=False)
+    A = H + lam * np.eye(H.shape[0])
+    d1, p1 = O.traj_delta(O.OracleTraj(_cg(tb, 1)), 0, tb.q_seed[0], lam, dtype="f64")
+    ref1 = -(g @ g) / (g @ A @ g + 1e-8) * g
+    np.testing.assert_allclose(d1, ref1, rtol=1e-10, atol=1e-14)
+    assert abs(p1 - (-(g @ ref1) - 0.5 * ref1 @ H @ ref1)) <= 1e-10 * abs(p1)
+    d2, _ = O.traj_delta(O.OracleTraj(_cg(tb, 2)), 0, tb.q_seed[0], lam, dtype="f64")
+    K = np.stack([g, A @ g], 1)
+    c = np.linalg.solve(K.T @ A @ K, -K.T @ g)
+    np.testing.assert_allclose(d2, K @ c, rtol=1e-7, atol=1e-12 * np.abs(d2).max())
+
+
+def test_traj_cg_lm_descends():
+    """an LM solve on CG steps (the reference's default budget) lowers the cost monotonically in
+    FP64 and accepts most steps"""
+    tb = _small(T=8, seed=13, batch=3)
+    tb = _cg(tb, tb.cg_default_iters)
+    sol = O.traj_lm_solve(O.OracleTraj(tb), tb.q_seed, 20, O.LMParams(lm_lambda=1e-2, lambda_factor=10.0), dtype="f64")
+    assert np.all(np.diff(sol["cost_trace"], axis=1) <= 1e-12)
+    assert sol["cost_trace"][:, -1].max() < 0.5 * sol["cost_trace"][:, 0].min()
