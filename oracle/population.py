@@ -8,7 +8,8 @@ product package (robokit_amd) never imports it.
                        NaN as +inf, equal costs in candidate order), restated as a stable sort
   population_ik        opt/population_solver.py:291-395 for the IK schedule of
                        helpers/ik/config.py:37-45: each segment a fresh LM (oracle.lm_solve) on
-                       the kept candidates, then a select; the last select keeps one winner
+                       the kept candidates, then a select at every stage boundary (an equal width
+                       reorders best first, :365-375); the last select keeps one winner
 
 Pins (tests/test_population.py): the R_d closed forms (golden ratio for d = 1, plastic number for
 d = 2), the reference test's coverage property (tests/opt/test_sampler.py:57-61), the ranking
@@ -62,7 +63,7 @@ def population_ik(spec, links: Sequence[int], targets: np.ndarray, seeds: np.nda
     cost = None
     selects = []
     for k, (iters, width, params) in enumerate(stages):
-        if k > 0 and width < w:
+        if k > 0 and (width < w or w > 1):  # every stage boundary selects, equal widths reorder
             rows = select_best_k(cost, w, width).reshape(-1)
             selects.append(rows)
             q, cost, inst, w = q[rows], cost[rows], inst[rows], width

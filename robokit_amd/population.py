@@ -122,6 +122,7 @@ class PopulationIK:
         self.batch_size = int(batch_size)
         self.widths = self.config.num_candidates
         self._batches: Dict[int, N.Batch] = {}
+        self._twins: Dict[int, N.Batch] = {}
         for w in dict.fromkeys(self.widths + [1]):
             self._batches[w] = N.Batch(ctx, spec, terms, self.batch_size * w)
         self.offsets = candidate_offsets(self.widths[0], spec.num_actuated_joints,
@@ -179,8 +180,15 @@ class PopulationIK:
         params = cfg.optimizer
         cur, w = self._batches[widths[0]], widths[0]
         for k, stage in enumerate(cfg.stages):
-            if k > 0 and widths[k] < w:  # a select narrows the population; the next segment restarts LM
+            # every stage boundary selects (population_solver.py:365-375): it narrows the population,
+            # or at an equal width reorders each problem's candidates best first; the next segment
+            # restarts LM
+            if k > 0 and (widths[k] < w or w > 1):
                 nxt = self._batches[widths[k]]
+                if nxt is cur:  # same width: the select reads one batch and writes its twin
+                    nxt = self._twin(w)
+                if nxt is self._batches[widths[0]]:  # its resident seeds are overwritten
+                    self._global_loaded = False
                 cur.select(w, nxt, widths[k])
                 cur, w = nxt, widths[k]
             if stage.iters > 0 or k == 0:  # a zero-iteration first stage scores the seeds
@@ -191,6 +199,12 @@ class PopulationIK:
             cur = win
         return cur
 
+    def _twin(self, w: int) -> N.Batch:
+        """a second batch of width ``w``, for a select that keeps the width"""
+        if w not in self._twins:
+            self._twins[w] = N.Batch(self.ctx, self.spec, self.terms, self.batch_size * w)
+        return self._twins[w]
+
     def close(self) -> None:
-        for b in self._batches.values():
+        for b in list(self._batches.values()) + list(self._twins.values()):
             b.close()

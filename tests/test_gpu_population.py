@@ -122,6 +122,33 @@ def test_population_parity(ctx, panda):
     assert ok.sum() >= one.metrics()[2].sum()
 
 
+def test_population_equal_width_stages(ctx, panda):
+    """a schedule whose middle stages keep the width (64 -> 8 -> 8 -> 8 -> 1): each boundary still
+    selects, reordering every problem's candidates best first through a twin batch, and the winners
+    classify like the oracle schedule's (population_solver.py:365-375)"""
+    spec, wl = panda
+    cfg = P.PopulationSolverConfig(stages=[P.Stage(4, 64, N.LMParams(lm_lambda=10.0)), P.Stage(3, 8, N.LMParams(lm_lambda=1.0)),
+                                           P.Stage(3, 8), P.Stage(3, 8), P.Stage(0, 1)])
+    pop = P.PopulationIK(ctx, spec, _terms(spec, wl.links), wl.size, config=cfg)
+    pop.set_targets(pose_targets=wl.targets)
+    win = pop.solve()
+    _, _, ok = win.metrics()
+    stages = [(s.iters, w, O.LMParams(**p.__dict__)) for s, w, p in
+              zip(cfg.stages, cfg.num_candidates, cfg.optimizer)]
+    q32, c32, sel32 = OP.population_ik(spec, wl.links, wl.targets, pop.seeds, stages)
+    q64, _, _ = OP.population_ik(spec, wl.links, wl.targets, pop.seeds, stages, dtype="f64")
+    assert len(sel32) == 4  # 64 -> 8, 8 -> 8 twice, 8 -> 1
+    ok32 = O.ik_metrics(spec, q32, wl.links, wl.targets)[2]
+    ok64 = O.ik_metrics(spec, q64, wl.links, wl.targets, dtype="f64")[2]
+    mism, n_ref = int(np.sum(ok != ok32)), int(np.sum(ok64 != ok32))
+    print(f"\n[population] equal-width stages: converged gpu {ok.mean():.3f} f32 {ok32.mean():.3f}; "
+          f"classification differs gpu/f32 {mism}, f32/f64 {n_ref}")
+    assert mism <= max(n_ref, 1)
+    # a second solve from the resident seeds gives the same winners (the twin keeps the seeds intact)
+    again = pop.solve()
+    np.testing.assert_array_equal(again.get()["q"], win.get()["q"])
+
+
 def test_select_ranks_nan_costs_last(ctx, panda):
     """candidates seeded with NaN joints end with NaN costs: the full 64 -> 64 ranking puts them
     after every finite cost, in candidate order (NaN as +inf); a 1 -> 1 select is the identity"""

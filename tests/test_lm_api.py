@@ -207,3 +207,146 @@ def test_scene_collision_through_the_optimizer(ctx):
     ref = b.get()
     np.testing.assert_array_equal(var.get(L.VarKey.ROBOT).q, ref["q"])
     np.testing.assert_array_equal(costs, ref["cost"])
+
+
+# ----------------------------------------------------------------------------- trajectory terms, variables
+def _traj_terms(tb, **kw):
+    """the reference-shaped terms of a workloads.TrajBatch problem set"""
+    sc = tb.scene
+    robot = L.Robot(tb.spec, collision_spheres=(sc.sphere_links, sc.sphere_centers, sc.sphere_radii))
+    scene = L.PrimitiveScene(sc.obstacle_spheres, sc.obstacle_boxes)
+    goal = L.TrajectoryTask(L.FrameTask(robot, tb.goal_link, tb.goals[:, None]), tb.T, [-1])
+    terms = [goal, L.TrajectorySmoothnessTask(robot, tb.T, tb.wv, order=1, dt=tb.dt),
+             L.TrajectorySmoothnessTask(robot, tb.T, tb.wa, order=2, dt=tb.dt),
+             L.TrajectoryCollisionTask(robot, scene, tb.T, weight=tb.coll_w, margin=tb.margin)]
+    if tb.lim_orders:
+        orders = [o for o in range(3) if tb.lim_orders >> o & 1]
+        terms.append(L.TrajectoryLimitTask(robot, tb.T, orders, lower=tb.lim_lo[orders], upper=tb.lim_hi[orders],
+                                           weight=tb.lim_w[orders], dt=tb.dt, stencil="forward"))
+    return robot, terms
+
+
+def test_trajectory_terms_build_the_kernel_problem_set():
+    """the optimizer's trajectory description equals the workload's: goal link and weights, the
+    difference weights, the collision model, the CG budget of the reference's default formula"""
+    tb = W.config3(seed=3, batch=4, T=16)
+    robot, terms = _traj_terms(tb)
+    opt = L.LMOptimizerConfig(linear_solver="cg").build(terms)
+    d = opt._traj
+    assert (d.T, d.goal_link, d.wv, d.wa, d.dt, d.coll_w, d.margin) == (16, tb.goal_link, tb.wv, tb.wa, 1.0, 20.0, 0.02)
+    np.testing.assert_array_equal(d.scene.sphere_links, tb.scene.sphere_links)
+    np.testing.assert_array_equal(d.scene.obstacle_boxes, tb.scene.obstacle_boxes)
+    assert d.cg_iters == max(4, min(64, 16 * 7 // 4)) == 28
+    np.testing.assert_array_equal(opt._goal_weights, np.ones(6))
+    assert L.LMOptimizerConfig(linear_solver="cg", cg_max_iter=9).build(terms)._traj.cg_iters == 9
+    assert L.LMOptimizer(terms)._traj.cg_iters == 0
+    assert sum(t.residual_dim for t in terms) == 6 + 15 * 7 + 14 * 7 + 16 * 48
+
+
+def test_trajectory_term_validation():
+    tb = W.config3(seed=3, batch=2, T=8)
+    robot, terms = _traj_terms(tb)
+    with pytest.raises(NotImplementedError):
+        L.TrajectoryTask(L.FrameTask(robot, tb.goal_link, tb.goals[:1]), 8, [3])  # goal on a middle frame
+    with pytest.raises(ValueError):
+        L.TrajectorySmoothnessTask(robot, 8, order=8)
+    with pytest.raises(NotImplementedError):
+        L.TrajectorySmoothnessTask(robot, 8, order=3)
+    with pytest.raises(NotImplementedError):
+        L.TrajectoryCollisionTask(robot, L.PrimitiveScene(), 8, penalty="smooth")
+    with pytest.raises(NotImplementedError):
+        L.TrajectoryLimitTask(robot, 8, order=1, upper=1.0)  # central stencil by default
+    with pytest.raises(ValueError):
+        L.TrajectoryLimitTask(robot, 8, order=2, stencil="forward")  # no default acceleration bound
+    lt = L.TrajectoryLimitTask(robot, 8, order=[0, 1], upper=(robot.spec.actuated_joint_limits[:, 1], 2.0),
+                               stencil="forward")
+    np.testing.assert_array_equal(lt.lower[1], -2.0 * np.ones(7))  # derivatives: -upper
+    np.testing.assert_array_equal(lt.lower[0], robot.spec.actuated_joint_limits[:, 0].astype(np.float32))
+    with pytest.raises(ValueError):  # horizons disagree
+        L.LMOptimizer(terms + [L.TrajectorySmoothnessTask(robot, 9, order=1)])
+    with pytest.raises(NotImplementedError):  # mixed dense and trajectory terms
+        L.LMOptimizer(terms + [L.PositionLimit(robot)])
+    with pytest.raises(NotImplementedError):
+        L.LMOptimizer([L.PositionLimit(robot)], L.LMOptimizerConfig(linear_solver="cg"))
+    with pytest.raises(NotImplementedError):
+        L.LMOptimizer(terms, L.LMOptimizerConfig(linear_solver="torch_cholesky"))
+    tpl = L.TrajectorySmoothnessTask(num_frames=8, order=2)
+    assert tpl.is_template and tpl.resolve(robot, {}).order == 2
+
+
+def test_var_values_column_blocks():
+    """VarValues (opt/var_values.py:88-248): column offsets in insertion order, integrate splits the
+    velocity by block, view shares storage, gather / accept per key; VectorVar's unit and max_step
+    follow the reference doctest (opt/var_values.py:255-266)"""
+    robot = L.Robot(W.panda_dh())
+    st = robot.state(np.zeros((3, 7)))
+    vv = L.VarValues({L.VarKey.ROBOT: st, L.VarKey.TIMING: L.VectorVar(np.ones((3, 2)))})
+    assert (vv.tangent_dim, vv.tangent_offset(L.VarKey.ROBOT), vv.tangent_offset(L.VarKey.TIMING)) == (9, 0, 7)
+    v = np.arange(27, dtype=np.float32).reshape(3, 9)
+    out = vv.integrate(v)
+    np.testing.assert_array_equal(out[L.VarKey.ROBOT].q, v[:, :7])
+    np.testing.assert_array_equal(out[L.VarKey.TIMING].x, 1 + v[:, 7:])
+    np.testing.assert_array_equal(vv.difference(out), v)
+    view = vv.view(2, 1)
+    view[L.VarKey.TIMING].x[0, 0] = 5.0
+    assert vv[L.VarKey.TIMING].x[1, 0] == 5.0
+    g = vv.gather([2, 0])
+    assert g.batch_size == 2 and g[L.VarKey.TIMING].x[1, 0] == 1.0
+    vv.accept(np.array([0, 1, 0]), out)
+    np.testing.assert_array_equal(vv[L.VarKey.ROBOT].q[1], v[1, :7])
+    np.testing.assert_array_equal(vv[L.VarKey.ROBOT].q[0], np.zeros(7))
+    one = L.VarValues.from_var(L.VectorVar(np.array([[1.0, 2.0]])), L.VarKey.ACTION)
+    np.testing.assert_array_equal(one.integrate(np.array([[0.5, -1.0]], np.float32))[L.VarKey.ACTION].x, [[1.5, 1.0]])
+    capped = L.VectorVar(np.zeros((1, 2)), unit=10.0, max_step=1.0)
+    np.testing.assert_array_equal(capped.integrate(np.array([[0.0625, 1.0]], np.float32)).x, [[0.625, 1.0]])
+    with pytest.raises(ValueError):
+        L.VarValues({L.VarKey.ROBOT: st, L.VarKey.ACTION: L.VectorVar(np.ones((2, 2)))})  # batch sizes differ
+    with pytest.raises(ValueError):
+        L.VarValues({})
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver,bounds", [("cholesky", False), ("cg", False), ("cholesky", True)])
+def test_trajectory_optimizer_equals_the_batch(ctx, solver, bounds):
+    """LMOptimizer over the trajectory terms runs the trajectory kernel on the same problem set as
+    the native batch of the workload: results bitwise equal, a second variable carried unchanged"""
+    import dataclasses
+
+    tb = W.config3(seed=5, batch=16, T=32)
+    if bounds:
+        na = 7
+        lo = np.stack([-np.full(na, 1.5), -np.full(na, 0.2), -np.full(na, 0.1)]).astype(np.float32)
+        tb = dataclasses.replace(tb, lim_orders=7, lim_w=np.full((3, na), 5.0, np.float32), lim_lo=lo, lim_hi=-lo)
+    cfg = L.LMOptimizerConfig(max_iter=20, early_stopping_interval=0, lm_lambda=1e-2, lambda_factor=10.0,
+                              linear_solver=solver)
+    if solver == "cg":
+        tb = dataclasses.replace(tb, cg_iters=tb.cg_default_iters)
+    robot, terms = _traj_terms(tb)
+    opt = cfg.build(terms)
+    extra = L.VectorVar(np.full((tb.size, 3), 7.0))
+    var = L.VarValues({L.VarKey.ROBOT: robot.state(tb.q_seed), L.VarKey.TIMING: extra})
+    var, costs = opt.solve(var)
+    b = N.TrajBatch(ctx, tb, tb.size)
+    b.set(q=tb.q_seed, goals=tb.goals)
+    b.solve(20, cfg.params())
+    ref = b.get()
+    np.testing.assert_array_equal(var[L.VarKey.ROBOT].q, ref["q"])
+    np.testing.assert_array_equal(costs, ref["cost"])
+    np.testing.assert_array_equal(var[L.VarKey.TIMING].x, np.full((tb.size, 3), 7.0, np.float32))
+    assert np.all(np.isfinite(costs))
+
+
+@pytest.mark.gpu
+def test_extra_variable_leaves_the_ik_solve_unchanged(ctx):
+    """a VectorVar no term reads has zero Jacobian columns: the solve of the robot key is the
+    ROBOT-only solve, bit for bit, and the vector is unchanged"""
+    robot, wl, q0 = _ik_problem(B=32, seed=4)
+    terms = [L.FrameTask(robot, wl.links[0], wl.targets), L.PositionLimit(robot)]
+    cfg = L.LMOptimizerConfig(max_iter=15, early_stopping_interval=0, lm_lambda=1e-2, lambda_factor=10.0)
+    a = L.VarValues({L.VarKey.ROBOT: robot.state(q0)})
+    b = L.VarValues({L.VarKey.ROBOT: robot.state(q0), L.VarKey.ACTION: L.VectorVar(np.ones((32, 4)))})
+    qa, ca = cfg.build(terms).solve(a)
+    qb, cb = cfg.build(terms).solve(b)
+    np.testing.assert_array_equal(qa[L.VarKey.ROBOT].q, qb[L.VarKey.ROBOT].q)
+    np.testing.assert_array_equal(ca, cb)
+    np.testing.assert_array_equal(qb[L.VarKey.ACTION].x, np.ones((32, 4), np.float32))
