@@ -95,6 +95,51 @@ int rk_copy_probe(rk_context* ctx, int64_t bytes, float* kernel_ms);
 int rk_se3_op(rk_context* ctx, int32_t op, int32_t n, const float* a, const float* b, float* out);
 
 /*
+ * Scene geometry beyond primitives, resident in the context until it is destroyed and referenced
+ * by id from the collision terms (rk_terms_desc.geoms, rk_traj_desc.geoms) and rk_scene_query.
+ * Scene order: obstacle spheres, boxes, then the geometries in the order given; the first strict
+ * minimum decides the distance and the normal (the reference chains one query per attached
+ * geometry through a per-point minimum, geom/scene.py:132-256).
+ *
+ * Volume (VolumeGeom / SdfVolume, geom/geoms.py:604-660, geom/sdf_volume.py:80-155): signed
+ * distances at the grid points origin + (i, j, k) * voxel_size, values [nz][ny][nx] (x fastest),
+ * nx, ny, nz >= 2.  Inside [aabb_min, aabb_max] the distance is the trilinear sample and the
+ * normal the normalised gradient of the interpolant ((0, 0, 1) below 1e-8); outside, the distance
+ * to the box plus `padding` and the normal the direction from the box (_sample_volume_sdf_func,
+ * geom/sdf_kernels.py:346-380).  The box must lie inside the grid.
+ * Mesh (MeshGeom, geom/geoms.py:502-600): triangles faces [nf][3] over vertices [nv][3].  The
+ * distance is the exact distance to the nearest triangle, negative inside a closed, consistently
+ * oriented (counter-clockwise seen from outside) mesh: the sign is that of the angle-weighted
+ * pseudonormal of the nearest feature (face, edge or vertex); the normal is the nearest
+ * triangle's face normal (closest_on_meshes, geom/sdf_kernels.py:151-222).  The mesh's bounding
+ * box skips it wherever something nearer is already known.
+ * Limits: <= 64 geometries per context, <= 2^20 triangles, <= 2^26 grid values.
+ */
+typedef struct {
+    const float* values;
+    int32_t nx, ny, nz;
+    float origin[3];
+    float voxel_size;
+    float aabb_min[3], aabb_max[3];
+    float padding;
+} rk_volume_desc;
+typedef struct {
+    const float* vertices;  /* [n_vertices][3] */
+    int32_t n_vertices;
+    const int32_t* faces;   /* [n_faces][3] vertex indices */
+    int32_t n_faces;
+} rk_mesh_desc;
+int rk_geom_upload_volume(rk_context* ctx, const rk_volume_desc* desc, int32_t* geom_id);
+int rk_geom_upload_mesh(rk_context* ctx, const rk_mesh_desc* desc, int32_t* geom_id);
+/* WarpScene.query_sdf (geom/scene.py:132-256) of the scene of primitives and geometries at
+ * points [n][3]: signed distance [n] and the normal of the deciding element [n][3] (nullable);
+ * 1e6 and (1, 0, 0) where nothing is nearer.  Also bakes a mesh into a volume: query the grid
+ * points (SdfVolume.from_mesh, geom/sdf_volume.py:80-155). */
+int rk_scene_query(rk_context* ctx, int32_t n_obs_spheres, const float* obs_spheres, int32_t n_obs_boxes,
+                   const float* obs_boxes, int32_t n_geoms, const int32_t* geoms, int32_t n, const float* points,
+                   float* dist, float* normal);
+
+/*
  * Residual-term structure of a batch (uniform over its problems), in this row order:
  *   FrameTask log mode   (terms/dense/frame_task.py:505-621)      6 rows per pose target
  *   PositionTask         (terms/dense/position_task.py:188-272)   3 rows per position target
@@ -138,6 +183,8 @@ typedef struct {
     const float* obs_spheres;    /* [n][4] centre, radius */
     int32_t n_obs_boxes;         /* <= 16 */
     const float* obs_boxes;      /* [n][6] centre, half extents (axis-aligned) */
+    int32_t n_geoms;             /* <= 4 volumes / meshes of the context (rk_geom_upload_*), after the boxes */
+    const int32_t* geoms;        /* [n_geoms] geometry ids */
 } rk_terms_desc;
 
 /* LMOptimizerConfig fields (opt/lm_optimizer.py:502-523) */
@@ -288,6 +335,8 @@ typedef struct {
      * early exit is a host check of the whole batch, which it also skips under graph capture
      * and with early_stopping_interval = 0.  Predicted reduction -g.dq - dq^T J^T J dq / 2. */
     int32_t cg_max_iter;
+    int32_t n_geoms;              /* <= 4 volumes / meshes of the context (rk_geom_upload_*), after the boxes */
+    const int32_t* geoms;         /* [n_geoms] geometry ids */
 } rk_traj_desc;
 
 int rk_traj_batch_create(rk_context* ctx, const rk_traj_desc* desc, int32_t capacity, int32_t trace_iters,

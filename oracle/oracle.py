@@ -58,8 +58,53 @@ class _Scene(C.Structure):
     _fields_ = [
         ("nsph", C.c_int), ("sph_link", C.c_void_p), ("sph_c", C.c_void_p), ("sph_r", C.c_void_p),
         ("sph_w", C.c_void_p), ("margin", C.c_float), ("penalty", C.c_int), ("nobs_s", C.c_int),
-        ("obs_s", C.c_void_p), ("nobs_b", C.c_int), ("obs_b", C.c_void_p),
+        ("obs_s", C.c_void_p), ("nobs_b", C.c_int), ("obs_b", C.c_void_p), ("ngeom", C.c_int), ("geoms", C.c_void_p),
     ]
+
+
+class _Geom(C.Structure):
+    _fields_ = [("kind", C.c_int), ("values", C.c_void_p), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("origin", C.c_float * 3), ("voxel", C.c_float), ("lo", C.c_float * 3), ("hi", C.c_float * 3),
+                ("pad", C.c_float), ("verts", C.c_void_p), ("nv", C.c_int), ("faces", C.c_void_p), ("nf", C.c_int)]
+
+
+def _geoms(geoms, keep: list):
+    """(count, pointer) of the C geometry table of robokit_amd.geom volumes / meshes"""
+    geoms = list(geoms or [])
+    if not geoms:
+        return 0, None
+    arr = (_Geom * len(geoms))()
+    for i, g in enumerate(geoms):
+        if hasattr(g, "values"):  # a volume
+            v = np.ascontiguousarray(g.values, np.float32)
+            nz, ny, nx = v.shape
+            keep.append(v)
+            arr[i] = _Geom(0, _ptr(v), nx, ny, nz, (C.c_float * 3)(*g.origin), float(g.voxel_size),
+                           (C.c_float * 3)(*g.aabb_min), (C.c_float * 3)(*g.aabb_max), float(g.padding), None, 0, None, 0)
+        else:  # a mesh
+            v = np.ascontiguousarray(g.vertices, np.float32)
+            f = np.ascontiguousarray(g.faces, np.int32)
+            keep.extend([v, f])
+            z3 = (C.c_float * 3)(0, 0, 0)
+            arr[i] = _Geom(1, None, 0, 0, 0, z3, 0.0, z3, z3, 0.0, _ptr(v), len(v), _ptr(f), len(f))
+    keep.append(arr)
+    return len(geoms), C.cast(arr, C.c_void_p)
+
+
+def scene_query(points: np.ndarray, spheres=None, boxes=None, geoms=(), dtype: str = "f32"):
+    """(signed distance [n], normal [n, 3]) of a scene of obstacle spheres, boxes and volumes /
+    meshes at points [n, 3]"""
+    lib, rt = _lib(dtype), _real(dtype)
+    pts = np.ascontiguousarray(np.asarray(points).reshape(-1, 3), rt)
+    sp = np.ascontiguousarray(np.zeros((0, 4)) if spheres is None else spheres, np.float32).reshape(-1, 4)
+    bx = np.ascontiguousarray(np.zeros((0, 6)) if boxes is None else boxes, np.float32).reshape(-1, 6)
+    keep: list = []
+    ng, gp = _geoms(geoms, keep)
+    n = len(pts)
+    dist = np.zeros(n, rt)
+    nrm = np.zeros((n, 3), rt)
+    lib.ork_scene_query(len(sp), _ptr(sp), len(bx), _ptr(bx), ng, gp, n, _ptr(pts), _ptr(dist), _ptr(nrm))
+    return dist, nrm
 
 
 @dataclass
@@ -75,6 +120,7 @@ class CollisionScene:
     penalty: str = "smooth"
     obstacle_spheres: np.ndarray = None
     obstacle_boxes: np.ndarray = None
+    geoms: Sequence = ()  # volumes / meshes after the boxes
 
 
 class _LMParams(C.Structure):
@@ -143,8 +189,9 @@ def _terms(terms: Sequence[Term], keep: list):
                                       np.float32).reshape(-1, 4),
                  np.ascontiguousarray(np.zeros((0, 6)) if sc.obstacle_boxes is None else sc.obstacle_boxes,
                                       np.float32).reshape(-1, 6)]
+            ng, gp = _geoms(sc.geoms, keep)
             st = _Scene(len(a[0]), _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(a[3]), float(sc.margin),
-                        PENALTY[sc.penalty], len(a[4]), _ptr(a[4]), len(a[5]), _ptr(a[5]))
+                        PENALTY[sc.penalty], len(a[4]), _ptr(a[4]), len(a[5]), _ptr(a[5]), ng, gp)
             keep.extend(a + [st])
             sc_ptr = C.cast(C.pointer(st), C.c_void_p)
         arr[i] = _Term(t.kind, len(links), _ptr(links), _ptr(tg), _ptr(w), _ptr(qp), sc_ptr)
@@ -362,7 +409,7 @@ class _Traj(C.Structure):
                 ("sph_link", C.c_void_p), ("sph_c", C.c_void_p), ("sph_r", C.c_void_p), ("coll_w", C.c_float),
                 ("margin", C.c_float), ("nobs_s", C.c_int), ("obs_s", C.c_void_p), ("nobs_b", C.c_int),
                 ("obs_b", C.c_void_p), ("lim_orders", C.c_int), ("lim_w", C.c_void_p), ("lim_lo", C.c_void_p),
-                ("lim_hi", C.c_void_p), ("cg_iters", C.c_int)]
+                ("lim_hi", C.c_void_p), ("cg_iters", C.c_int), ("ngeom", C.c_int), ("geoms", C.c_void_p)]
 
 
 class OracleTraj:
@@ -389,7 +436,8 @@ class OracleTraj:
                             tb.wa if wa is None else wa, tb.dt, len(k[2]), _ptr(k[2]), _ptr(k[3]), _ptr(k[4]),
                             tb.coll_w if coll_w is None else coll_w, tb.margin if margin is None else margin,
                             len(k[5]), _ptr(k[5]), len(k[6]), _ptr(k[6]), orders, _ptr(lim[0]), _ptr(lim[1]),
-                            _ptr(lim[2]), int(getattr(tb, "cg_iters", 0) or 0))
+                            _ptr(lim[2]), int(getattr(tb, "cg_iters", 0) or 0),
+                            *_geoms(getattr(sc, "geoms", None), self.keep))
         self.tb = tb
 
     def rows(self):

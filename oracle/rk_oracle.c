@@ -574,6 +574,225 @@ static real prims_sdf(int nobs_s, const float *obs_s, int nobs_b, const float *o
     return best;
 }
 
+/* Scene volumes and meshes (geom/geoms.py:502-660, geom/sdf_kernels.py:151-222, 346-380). */
+typedef struct {
+    int kind;                  /* 0 volume, 1 mesh */
+    const float *values;       /* volume: [nz][ny][nx] at origin + (i, j, k) * voxel */
+    int nx, ny, nz;
+    float origin[3], voxel, lo[3], hi[3], pad;
+    const float *verts;        /* mesh: [nv][3] */
+    int nv;
+    const int32_t *faces;      /* [nf][3] */
+    int nf;
+} ork_geom;
+
+/* volume: inside [lo, hi] the trilinear interpolant written as the weighted sum
+ * sum_{ijk} w_i(fx) w_j(fy) w_k(fz) v_ijk (w_0(f) = 1 - f, w_1(f) = f) of the 8 grid values of the
+ * cell holding p (cell index clamped to the grid), grad = its derivative in index units;
+ * outside, |p - clamp(p)| + pad and the direction from the box (_sample_volume_sdf_func,
+ * sdf_kernels.py:346-380) */
+static real vol_sdf(const ork_geom *g, const real *p, real *grad) {
+    int inside = 1;
+    for (int k = 0; k < 3; ++k) inside = inside && p[k] >= (real)g->lo[k] && p[k] <= (real)g->hi[k];
+    if (inside) {
+        const int n[3] = {g->nx, g->ny, g->nz};
+        int i[3];
+        real f[3];
+        for (int k = 0; k < 3; ++k) {
+            real u = (p[k] - (real)g->origin[k]) / (real)g->voxel;
+            int b = (int)floor((double)u);
+            if (b < 0) b = 0;
+            if (b > n[k] - 2) b = n[k] - 2;
+            i[k] = b;
+            f[k] = u - (real)b;
+        }
+        real val = 0, gr[3] = {0, 0, 0};
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const real v = (real)g->values[((size_t)(i[2] + dz) * g->ny + (i[1] + dy)) * g->nx + (i[0] + dx)];
+                    const real wx = dx ? f[0] : 1 - f[0], wy = dy ? f[1] : 1 - f[1], wz = dz ? f[2] : 1 - f[2];
+                    val += wx * wy * wz * v;
+                    gr[0] += (dx ? 1 : -1) * wy * wz * v;
+                    gr[1] += wx * (dy ? 1 : -1) * wz * v;
+                    gr[2] += wx * wy * (dz ? 1 : -1) * v;
+                }
+        if (grad) for (int k = 0; k < 3; ++k) grad[k] = gr[k];
+        return val;
+    }
+    real o[3], l2 = 0;
+    for (int k = 0; k < 3; ++k) {
+        real c = p[k] < (real)g->lo[k] ? (real)g->lo[k] : (p[k] > (real)g->hi[k] ? (real)g->hi[k] : p[k]);
+        o[k] = p[k] - c;
+        l2 += o[k] * o[k];
+    }
+    const real ad = RSQRT(l2);
+    if (grad) for (int k = 0; k < 3; ++k) grad[k] = o[k] / ad;
+    return ad + (real)g->pad;
+}
+
+static void mesh_face_normal(const ork_geom *g, int f, real *n) {
+    const int32_t *F = g->faces + 3 * f;
+    real e1[3], e2[3];
+    for (int k = 0; k < 3; ++k) {
+        e1[k] = (real)g->verts[3 * F[1] + k] - (real)g->verts[3 * F[0] + k];
+        e2[k] = (real)g->verts[3 * F[2] + k] - (real)g->verts[3 * F[0] + k];
+    }
+    n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    const real l = RSQRT(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (l > 0) for (int k = 0; k < 3; ++k) n[k] /= l;
+}
+
+/* angle-weighted pseudonormal (Baerentzen and Aanaes, IEEE TVCG 11(3), 2005) of vertex v (u < 0)
+ * or of the edge (v, u): the incident angles times the face normals, or the sum of the normals of
+ * the faces holding the edge, normalised; computed by a scan of every face */
+static void mesh_pseudonormal(const ork_geom *g, int v, int u, real *out) {
+    out[0] = out[1] = out[2] = 0;
+    for (int f = 0; f < g->nf; ++f) {
+        const int32_t *F = g->faces + 3 * f;
+        int cv = -1, hu = 0;
+        for (int c = 0; c < 3; ++c) {
+            if (F[c] == v) cv = c;
+            if (F[c] == u) hu = 1;
+        }
+        if (cv < 0 || (u >= 0 && !hu)) continue;
+        real n[3];
+        mesh_face_normal(g, f, n);
+        real wgt = 1;
+        if (u < 0) {
+            const int v1 = F[(cv + 1) % 3], v2 = F[(cv + 2) % 3];
+            real a[3], b[3];
+            for (int k = 0; k < 3; ++k) {
+                a[k] = (real)g->verts[3 * v1 + k] - (real)g->verts[3 * v + k];
+                b[k] = (real)g->verts[3 * v2 + k] - (real)g->verts[3 * v + k];
+            }
+            const real la = RSQRT(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+            const real lb = RSQRT(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+            real cs = (la > 0 && lb > 0) ? (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / (la * lb) : 1;
+            if (cs > 1) cs = 1;
+            if (cs < -1) cs = -1;
+            wgt = (real)acos((double)cs);
+        }
+        for (int k = 0; k < 3; ++k) out[k] += wgt * n[k];
+    }
+    const real l = RSQRT(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
+    if (l > 0) for (int k = 0; k < 3; ++k) out[k] /= l;
+}
+
+/* mesh: the nearest point of each triangle by the Voronoi-region tests of Ericson, Real-Time
+ * Collision Detection (2005) section 5.1.5, the first strict minimum over the triangles; signed
+ * by the pseudonormal of the nearest feature (vertex, edge or face); face = its triangle
+ * (closest_on_meshes, sdf_kernels.py:151-222) */
+static real mesh_sdf(const ork_geom *g, const real *p, int *face, real *dir) {
+    real best = 1e300, bq[3] = {0, 0, 0};
+    int bf = -1, bv = -1, bu = -1, bface_feat = 0;
+    for (int f = 0; f < g->nf; ++f) {
+        const int32_t *F = g->faces + 3 * f;
+        real a[3], b[3], c[3], ab[3], ac[3], ap[3], bp[3], cp[3], q[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = (real)g->verts[3 * F[0] + k];
+            b[k] = (real)g->verts[3 * F[1] + k];
+            c[k] = (real)g->verts[3 * F[2] + k];
+            ab[k] = b[k] - a[k]; ac[k] = c[k] - a[k];
+            ap[k] = p[k] - a[k]; bp[k] = p[k] - b[k]; cp[k] = p[k] - c[k];
+        }
+#define ODOT(x, y) ((x)[0] * (y)[0] + (x)[1] * (y)[1] + (x)[2] * (y)[2])
+        int fv = -1, fu = -1, isface = 0;  /* nearest feature: vertex fv, edge (fv, fu), face */
+        const real d1 = ODOT(ab, ap), d2 = ODOT(ac, ap);
+        const real d3 = ODOT(ab, bp), d4 = ODOT(ac, bp);
+        const real d5 = ODOT(ab, cp), d6 = ODOT(ac, cp);
+        const real vc = d1 * d4 - d3 * d2, vb = d5 * d2 - d1 * d6, va = d3 * d6 - d5 * d4;
+        if (d1 <= 0 && d2 <= 0) {
+            for (int k = 0; k < 3; ++k) q[k] = a[k];
+            fv = F[0];
+        } else if (d3 >= 0 && d4 <= d3) {
+            for (int k = 0; k < 3; ++k) q[k] = b[k];
+            fv = F[1];
+        } else if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+            const real v = d1 / (d1 - d3);
+            for (int k = 0; k < 3; ++k) q[k] = a[k] + v * ab[k];
+            fv = F[0]; fu = F[1];
+        } else if (d6 >= 0 && d5 <= d6) {
+            for (int k = 0; k < 3; ++k) q[k] = c[k];
+            fv = F[2];
+        } else if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+            const real w = d2 / (d2 - d6);
+            for (int k = 0; k < 3; ++k) q[k] = a[k] + w * ac[k];
+            fv = F[0]; fu = F[2];
+        } else if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+            const real w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+            for (int k = 0; k < 3; ++k) q[k] = b[k] + w * (c[k] - b[k]);
+            fv = F[1]; fu = F[2];
+        } else {
+            const real den = 1 / (va + vb + vc), v = vb * den, w = vc * den;
+            for (int k = 0; k < 3; ++k) q[k] = a[k] + ab[k] * v + ac[k] * w;
+            isface = 1;
+        }
+        real r[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+        const real d = ODOT(r, r);
+        if (d < best) {
+            best = d;
+            bf = f; bv = fv; bu = fu; bface_feat = isface;
+            for (int k = 0; k < 3; ++k) bq[k] = q[k];
+        }
+    }
+    *face = bf;
+    if (bf < 0) return (real)1e30;
+    real pn[3];
+    if (bface_feat) mesh_face_normal(g, bf, pn);
+    else mesh_pseudonormal(g, bv, bu, pn);
+    real r[3] = {p[0] - bq[0], p[1] - bq[1], p[2] - bq[2]};
+    const real d = RSQRT(best);
+    const real sg = ODOT(r, pn) < 0 ? -1 : 1;
+    if (dir) {  /* sign * (p - nearest) / |p - nearest|; the face normal on the surface */
+        if (d > (real)1e-12) for (int k = 0; k < 3; ++k) dir[k] = sg * r[k] / d;
+        else mesh_face_normal(g, bf, dir);
+    }
+    return sg * d;
+#undef ODOT
+}
+
+/* the scene: primitives (prims_sdf), then each geometry in order, the first strict minimum; n =
+ * its normal: a volume's normalised gradient ((0, 0, 1) below 1e-8, sdf_kernels.py:540-545); a
+ * mesh's outward direction from the nearest point = the face normal of the reference
+ * (sdf_kernels.py:206-213) wherever that point is inside a face, the distance gradient on edges
+ * and vertices, where the reference's face choice follows its BVH search order */
+static real scene_sdf_all(int nobs_s, const float *obs_s, int nobs_b, const float *obs_b, int ngeom,
+                          const ork_geom *geoms, const real *p, real *n) {
+    real best = prims_sdf(nobs_s, obs_s, nobs_b, obs_b, p, n);
+    for (int i = 0; i < ngeom; ++i) {
+        const ork_geom *g = geoms + i;
+        if (g->kind == 0) {
+            real gr[3];
+            const real d = vol_sdf(g, p, gr);
+            if (d < best) {
+                best = d;
+                const real l = RSQRT(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
+                if (l > (real)1e-8) { n[0] = gr[0] / l; n[1] = gr[1] / l; n[2] = gr[2] / l; }
+                else { n[0] = 0; n[1] = 0; n[2] = 1; }
+            }
+        } else {
+            int f;
+            real dir[3];
+            const real d = mesh_sdf(g, p, &f, dir);
+            if (d < best) {
+                best = d;
+                n[0] = dir[0]; n[1] = dir[1]; n[2] = dir[2];
+            }
+        }
+    }
+    return best;
+}
+
+/* WarpScene.query_sdf at points [n][3]: distance [n], normal [n][3] */
+EXPORT void ork_scene_query(int nobs_s, const float *obs_s, int nobs_b, const float *obs_b, int ngeom,
+                            const ork_geom *geoms, int n, const real *pts, real *dist, real *normal) {
+    for (int i = 0; i < n; ++i) dist[i] = scene_sdf_all(nobs_s, obs_s, nobs_b, obs_b, ngeom, geoms, pts + 3 * i,
+                                                        normal + 3 * i);
+}
+
 /* ------------------------------------------------------------------------- */
 /* residual terms                                                            */
 /* ------------------------------------------------------------------------- */
@@ -592,6 +811,8 @@ typedef struct {
     const float *obs_s;       /* [n][4] */
     int nobs_b;
     const float *obs_b;       /* [n][6] */
+    int ngeom;                /* then volumes / meshes */
+    const ork_geom *geoms;
 } ork_scene;
 
 typedef struct {
@@ -752,7 +973,8 @@ static void term_eval(const ork_robot *rb, const ork_term *t, int b, const real 
             real lc[3] = {(real)sc->sph_c[3 * k], (real)sc->sph_c[3 * k + 1], (real)sc->sph_c[3 * k + 2]}, c[3], n[3];
             quat_apply(Tk + 3, lc, c);
             for (int i = 0; i < 3; ++i) c[i] += Tk[i];
-            const real sd = prims_sdf(sc->nobs_s, sc->obs_s, sc->nobs_b, sc->obs_b, c, n) - (real)sc->sph_r[k];
+            const real sd = scene_sdf_all(sc->nobs_s, sc->obs_s, sc->nobs_b, sc->obs_b, sc->ngeom, sc->geoms, c, n) -
+                            (real)sc->sph_r[k];
             real pen, dpen;
             coll_penalty(sd, (real)sc->margin, sc->penalty, &pen, &dpen);
             const real w = (real)sc->sph_w[k];
@@ -1098,6 +1320,8 @@ typedef struct {
     int lim_orders;            /* bit o: TrajectoryLimitTask rows of order o, forward stencil */
     const float *lim_w, *lim_lo, *lim_hi; /* [3][na] */
     int cg_iters;              /* > 0: conjugate-gradient steps (traj_cg) instead of the Cholesky */
+    int ngeom;                 /* scene volumes / meshes after the boxes */
+    const ork_geom *geoms;
 } ork_traj;
 
 static int traj_rows(const ork_robot *rb, const ork_traj *tj, int *Rg, int *Rv, int *Ra, int *Rc, int *Rl) {
@@ -1113,7 +1337,7 @@ static int traj_rows(const ork_robot *rb, const ork_traj *tj, int *Rg, int *Rv, 
 
 /* scene SDF of a trajectory problem set */
 static real scene_sdf(const ork_traj *tj, const real *p, real *n) {
-    return prims_sdf(tj->nobs_s, tj->obs_s, tj->nobs_b, tj->obs_b, p, n);
+    return scene_sdf_all(tj->nobs_s, tj->obs_s, tj->nobs_b, tj->obs_b, tj->ngeom, tj->geoms, p, n);
 }
 
 /* sparse row: up to 7 columns (one frame) or 3 (smoothness) */

@@ -27,6 +27,8 @@
 #include <set>
 #include <string>
 #include <thread>
+#include <array>
+#include <cmath>
 #include <vector>
 
 #include "../../include/rkhip.h"
@@ -38,10 +40,27 @@
 #define RK_SCENE_MAXS 64  // obstacle spheres of a scene
 #define RK_SCENE_MAXB 16  // obstacle boxes (axis-aligned) of a scene
 #define RK_COLL_MAXS 64   // robot collision spheres of an IK term structure
+#define RK_SCENE_MAXG 4   // volumes / meshes of a scene
+#define RK_GEOM_MAX 64    // geometries resident in a context
 
 namespace rk {
 
 constexpr int WAVE = 64;
+
+// A resident scene geometry as the kernels see it (copied by value into the term tables; the
+// data stays in the context).  Volume: grid values [nz][ny][nx] at org + (i, j, k) h, sampled
+// trilinearly inside [lo, hi], box distance + pad outside.  Mesh: nt triangles of RK_TRI_FLOATS
+// floats each (vertices a, b, c; face normal; pseudonormals of the edges ab, bc, ca and of the
+// vertices a, b, c), bounding box [lo, hi].
+#define RK_TRI_FLOATS 30
+struct SceneGeom {
+  int kind;  // 0 volume, 1 mesh
+  const float* data;
+  int nx, ny, nz;  // volume grid; mesh: nx = triangles
+  float org[3], inv_h;
+  float lo[3], hi[3];
+  float pad;
+};
 
 struct Terms {
   int robot, n_pose, n_pos, limit, smooth;
@@ -73,6 +92,8 @@ struct Terms {
   int ns, nb;     // the scene (padded to whole groups: 8 spheres, 4 boxes)
   float obs_s[RK_SCENE_MAXS][4];
   float obs_b[RK_SCENE_MAXB][6];
+  int ng;         // then the scene's volumes / meshes
+  SceneGeom geom[RK_SCENE_MAXG];
 };
 
 // Dof elimination order and block fill pattern of the damped normal equations of one term
@@ -1337,6 +1358,36 @@ __global__ void k_copy_probe(const float4* __restrict__ src, float4* __restrict_
     dst[i] = src[i];
 }
 
+// a scene of primitives and geometries as scene_sdf reads it, for rk_scene_query
+struct SceneTable {
+  int ns, nb;
+  float obs_s[RK_SCENE_MAXS][4];
+  float obs_b[RK_SCENE_MAXB][6];
+  int ng;
+  SceneGeom geom[RK_SCENE_MAXG];
+};
+typedef const __attribute__((address_space(4))) SceneTable SceneK;
+
+// one lane per point: the scene's signed distance and the deciding element's normal
+__global__ void k_scene_query(const SceneTable* sc_, int n, const float* pts, float* dist, float* normal) {
+  SceneK* sc = (SceneK*)sc_;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < n;
+  float p[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) p[k] = live ? pts[3 * i + k] : 0.f;
+  int grp;
+  const float d = scene_sdf(sc, p, grp);
+  float nn[3];
+  scene_normal(sc, p, scene_winner(sc, p, grp), nn);
+  if (live) {
+    dist[i] = d;
+    if (normal)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) normal[3 * i + k] = nn[k];
+  }
+}
+
 __global__ void k_se3_op(int op, int n, const float* a, const float* bb, float* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1536,6 +1587,9 @@ struct rk_context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t marks[8] = {};  // rk_context_mark slots, created on first use
   std::map<int, int> tuned;  // batch signature -> threads per problem chosen by the autotuner
+  // scene geometries (rk_geom_upload_*): the kernels' view and the device bytes behind each
+  std::vector<SceneGeom> geoms;
+  std::vector<std::pair<void*, size_t>> geom_mem;
 };
 
 struct rk_batch {
@@ -1546,7 +1600,7 @@ struct rk_batch {
   int it_done = 0;  // iterations run since the last rk_batch_set / seeded solve (RK_CONTINUE offset)
   int traj_T = 0;   // > 0: a trajectory batch (na = traj_T * robot dofs), solved by k_traj
   int traj_na = 0;  // robot dofs of a trajectory batch
-  bool traj_lim = false;  // its problems carry bound rows (TrajectoryLimitTask)
+  bool traj_lim = false;  // its problems carry bound rows (TrajectoryLimitTask) or scene geometries
   TrajTerms* tt = nullptr;
   float *q = nullptr, *q_seed = nullptr, *pose_tgt = nullptr, *pos_tgt = nullptr, *q_prev = nullptr;
   float* rot_tgt = nullptr;
@@ -1964,6 +2018,7 @@ int rk_context_destroy(rk_context* ctx) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   hipFree(ctx->robots_dev);
+  for (auto& m : ctx->geom_mem) hipFree(m.first);
   hipEventDestroy(ctx->ev0);
   hipEventDestroy(ctx->ev1);
   for (hipEvent_t e : ctx->marks)
@@ -2221,7 +2276,189 @@ int rk_se3_op(rk_context* ctx, int32_t op, int32_t n, const float* a, const floa
   return RK_OK;
 }
 
+static int scene_geoms(rk_context* ctx, int n, const int32_t* ids, int* ng, SceneGeom* out);
+
+static int geom_store(rk_context* ctx, const SceneGeom& g, const float* host, size_t floats, int32_t* id) {
+  if ((int)ctx->geoms.size() >= RK_GEOM_MAX) return fail(RK_EINVAL, "at most 64 geometries per context");
+  void* dev = nullptr;
+  int rc;
+  if ((rc = dmalloc(ctx, &dev, sizeof(float) * floats, nullptr))) return rc;
+  hipError_t e = hipMemcpy(dev, host, sizeof(float) * floats, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    dfree(ctx, dev, sizeof(float) * floats);
+    return fail(RK_EHIP, std::string("geometry upload: ") + hipGetErrorString(e));
+  }
+  SceneGeom gd = g;
+  gd.data = (const float*)dev;
+  ctx->geoms.push_back(gd);
+  ctx->geom_mem.push_back({dev, sizeof(float) * floats});
+  *id = (int32_t)ctx->geoms.size() - 1;
+  return RK_OK;
+}
+
+int rk_geom_upload_volume(rk_context* ctx, const rk_volume_desc* d, int32_t* geom_id) {
+  if (!ctx || !d || !geom_id || !d->values) return fail(RK_EINVAL, "null argument");
+  if (d->nx < 2 || d->ny < 2 || d->nz < 2) return fail(RK_EINVAL, "a volume needs at least 2 grid points per axis");
+  const size_t nv = (size_t)d->nx * d->ny * d->nz;
+  if (nv > ((size_t)1 << 26)) return fail(RK_EINVAL, "at most 2^26 grid values");
+  if (!(d->voxel_size > 0.f) || !std::isfinite(d->padding)) return fail(RK_EINVAL, "voxel_size must be positive");
+  const int n[3] = {d->nx, d->ny, d->nz};
+  SceneGeom g{};
+  g.kind = 0;
+  g.nx = d->nx;
+  g.ny = d->ny;
+  g.nz = d->nz;
+  g.inv_h = 1.f / d->voxel_size;
+  g.pad = d->padding;
+  for (int k = 0; k < 3; ++k) {
+    const float top = d->origin[k] + (float)(n[k] - 1) * d->voxel_size, tol = 1e-4f * d->voxel_size;
+    if (!(d->aabb_min[k] <= d->aabb_max[k]) || d->aabb_min[k] < d->origin[k] - tol || d->aabb_max[k] > top + tol)
+      return fail(RK_EINVAL, "the volume's box must be ordered and lie inside its grid");
+    g.org[k] = d->origin[k];
+    g.lo[k] = d->aabb_min[k];
+    g.hi[k] = d->aabb_max[k];
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  return geom_store(ctx, g, d->values, nv, geom_id);
+}
+
+// the triangle table of a mesh: vertices, face normal, and the angle-weighted pseudonormals of
+// its edges (sum of the two adjacent face normals) and vertices (incident angles times face
+// normals), normalised, computed in double (Baerentzen and Aanaes, "Signed distance computation
+// using the angle weighted pseudonormal", IEEE TVCG 11(3), 2005)
+int rk_geom_upload_mesh(rk_context* ctx, const rk_mesh_desc* d, int32_t* geom_id) {
+  if (!ctx || !d || !geom_id || !d->vertices || !d->faces) return fail(RK_EINVAL, "null argument");
+  const int nv = d->n_vertices, nf = d->n_faces;
+  if (nv < 3 || nf < 1 || nf > (1 << 20)) return fail(RK_EINVAL, "a mesh needs >= 3 vertices and 1..2^20 faces");
+  for (int i = 0; i < 3 * nf; ++i)
+    if (d->faces[i] < 0 || d->faces[i] >= nv) return fail(RK_EINVAL, "face vertex index out of range");
+  auto V = [&](int v, int k) { return (double)d->vertices[3 * v + k]; };
+  auto unit = [](double* x) {
+    const double l = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    if (l > 0) for (int k = 0; k < 3; ++k) x[k] /= l;
+  };
+  std::vector<double> fn(3 * (size_t)nf), vn(3 * (size_t)nv, 0.0);
+  std::map<std::pair<int, int>, std::array<double, 3>> en;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int f = 0; f < nf; ++f) {
+    const int* F = d->faces + 3 * f;
+    double e1[3], e2[3], nn[3];
+    for (int k = 0; k < 3; ++k) {
+      e1[k] = V(F[1], k) - V(F[0], k);
+      e2[k] = V(F[2], k) - V(F[0], k);
+    }
+    nn[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    nn[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    nn[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    unit(nn);
+    for (int k = 0; k < 3; ++k) fn[3 * f + k] = nn[k];
+    for (int c = 0; c < 3; ++c) {
+      const int v0 = F[c], v1 = F[(c + 1) % 3], v2 = F[(c + 2) % 3];
+      double a[3], b[3];
+      for (int k = 0; k < 3; ++k) {
+        a[k] = V(v1, k) - V(v0, k);
+        b[k] = V(v2, k) - V(v0, k);
+        lo[k] = std::min(lo[k], V(v0, k));
+        hi[k] = std::max(hi[k], V(v0, k));
+      }
+      const double la = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+      const double lb = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+      double cs = la > 0 && lb > 0 ? (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / (la * lb) : 1.0;
+      const double ang = std::acos(std::max(-1.0, std::min(1.0, cs)));
+      for (int k = 0; k < 3; ++k) vn[3 * v0 + k] += ang * nn[k];
+      auto& e = en[{std::min(v0, v1), std::max(v0, v1)}];
+      for (int k = 0; k < 3; ++k) e[k] += nn[k];
+    }
+  }
+  std::vector<float> tab((size_t)nf * RK_TRI_FLOATS);
+  for (int f = 0; f < nf; ++f) {
+    const int* F = d->faces + 3 * f;
+    float* T = tab.data() + (size_t)f * RK_TRI_FLOATS;
+    for (int c = 0; c < 3; ++c)
+      for (int k = 0; k < 3; ++k) T[3 * c + k] = d->vertices[3 * F[c] + k];
+    for (int k = 0; k < 3; ++k) T[9 + k] = (float)fn[3 * f + k];
+    for (int c = 0; c < 3; ++c) {
+      std::array<double, 3> e = en[{std::min(F[c], F[(c + 1) % 3]), std::max(F[c], F[(c + 1) % 3])}];
+      unit(e.data());
+      double v[3] = {vn[3 * F[c]], vn[3 * F[c] + 1], vn[3 * F[c] + 2]};
+      unit(v);
+      for (int k = 0; k < 3; ++k) {
+        T[12 + 3 * c + k] = (float)e[k];
+        T[21 + 3 * c + k] = (float)v[k];
+      }
+    }
+  }
+  SceneGeom g{};
+  g.kind = 1;
+  g.nx = nf;
+  for (int k = 0; k < 3; ++k) {
+    g.lo[k] = (float)lo[k];
+    g.hi[k] = (float)hi[k];
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  return geom_store(ctx, g, tab.data(), tab.size(), geom_id);
+}
+
+int rk_scene_query(rk_context* ctx, int32_t n_obs_spheres, const float* obs_spheres, int32_t n_obs_boxes,
+                   const float* obs_boxes, int32_t n_geoms, const int32_t* geoms, int32_t n, const float* points,
+                   float* dist, float* normal) {
+  if (!ctx || n < 0 || (n && (!points || !dist))) return fail(RK_EINVAL, "null argument");
+  if (n_obs_spheres < 0 || n_obs_spheres > RK_SCENE_MAXS || n_obs_boxes < 0 || n_obs_boxes > RK_SCENE_MAXB ||
+      (n_obs_spheres && !obs_spheres) || (n_obs_boxes && !obs_boxes))
+    return fail(RK_EINVAL, "at most 64 obstacle spheres and 16 obstacle boxes");
+  if (n == 0) return RK_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  SceneTable st;
+  memset(&st, 0, sizeof(st));
+  st.ns = n_obs_spheres;
+  st.nb = n_obs_boxes;
+  for (int i = 0; i < RK_SCENE_MAXS; ++i)
+    for (int c = 0; c < 4; ++c) st.obs_s[i][c] = i < st.ns ? obs_spheres[4 * i + c] : (c < 3 ? RK_SDF_FAR : 0.f);
+  for (int i = 0; i < RK_SCENE_MAXB; ++i)
+    for (int c = 0; c < 6; ++c) st.obs_b[i][c] = i < st.nb ? obs_boxes[6 * i + c] : (c < 3 ? RK_SDF_FAR : 0.f);
+  int rc;
+  if ((rc = scene_geoms(ctx, n_geoms, geoms, &st.ng, st.geom))) return rc;
+  SceneTable* dst = nullptr;
+  float *dp = nullptr, *dd = nullptr, *dn = nullptr;
+  const size_t pb = sizeof(float) * 3 * (size_t)n;
+  hipError_t e = hipMalloc(&dst, sizeof(SceneTable));
+  if (e == hipSuccess) e = hipMalloc(&dp, pb);
+  if (e == hipSuccess) e = hipMalloc(&dd, sizeof(float) * (size_t)n);
+  if (e == hipSuccess) e = hipMalloc(&dn, pb);
+  if (e == hipSuccess) e = hipMemcpyAsync(dst, &st, sizeof(SceneTable), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dp, points, pb, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_scene_query, dim3((n + 127) / 128), dim3(128), 0, ctx->stream, (const SceneTable*)dst, n,
+                       (const float*)dp, dd, dn);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(dist, dd, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && normal) e = hipMemcpyAsync(normal, dn, pb, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(dst);
+  hipFree(dp);
+  hipFree(dd);
+  hipFree(dn);
+  if (e != hipSuccess) return fail(RK_EHIP, std::string("rk_scene_query: ") + hipGetErrorString(e));
+  return RK_OK;
+}
+
 }  // extern "C"
+
+// the scene geometries of a descriptor, by id, copied into a term table
+static int scene_geoms(rk_context* ctx, int n, const int32_t* ids, int* ng, SceneGeom* out) {
+  if (n < 0 || n > RK_SCENE_MAXG) return fail(RK_EINVAL, "at most 4 volumes / meshes per scene");
+  if (n && !ids) return fail(RK_EINVAL, "null geometry ids");
+  for (int k = 0; k < n; ++k) {
+    if (ids[k] < 0 || ids[k] >= (int)ctx->geoms.size()) return fail(RK_EINVAL, "unknown geometry id");
+    out[k] = ctx->geoms[ids[k]];
+  }
+  *ng = n;
+  return RK_OK;
+}
 
 // validated term structure of a descriptor (rk_batch_create, rk_server_add_terms)
 static int make_terms(rk_context* ctx, const rk_terms_desc* t, Terms* out) {
@@ -2294,6 +2531,7 @@ static int make_terms(rk_context* ctx, const rk_terms_desc* t, Terms* out) {
       for (int c = 0; c < 4; ++c) th.obs_s[i][c] = i < th.ns ? t->obs_spheres[4 * i + c] : (c < 3 ? RK_SDF_FAR : 0.f);
     for (int i = 0; i < RK_SCENE_MAXB; ++i)
       for (int c = 0; c < 6; ++c) th.obs_b[i][c] = i < th.nb ? t->obs_boxes[6 * i + c] : (c < 3 ? RK_SDF_FAR : 0.f);
+    if ((rc = scene_geoms(ctx, t->n_geoms, t->geoms, &th.ng, th.geom))) return rc;
   }
   th.Rf = 6 * th.n_pose;
   th.Rq = 3 * th.n_pos;
@@ -2415,6 +2653,7 @@ static int make_traj_terms(rk_context* ctx, const rk_traj_desc* d, TrajTerms* ou
     for (int c = 0; c < 4; ++c) tt.obs_s[i][c] = i < tt.ns ? d->obs_spheres[4 * i + c] : (c < 3 ? RK_SDF_FAR : 0.f);
   for (int i = 0; i < RK_SCENE_MAXB; ++i)
     for (int c = 0; c < 6; ++c) tt.obs_b[i][c] = i < tt.nb ? d->obs_boxes[6 * i + c] : (c < 3 ? RK_SDF_FAR : 0.f);
+  if ((rc = scene_geoms(ctx, d->n_geoms, d->geoms, &tt.ng, tt.geom))) return rc;
   // bound rows (TrajectoryLimitTask, forward stencils of orders 0..2): weights and bounds per order and dof
   if (d->limit_orders & ~7) return fail(RK_EINVAL, "limit_orders: bits 0 (position), 1 (velocity), 2 (acceleration)");
   tt.lim_orders = d->limit_orders;
@@ -2472,7 +2711,7 @@ int rk_traj_batch_create(rk_context* ctx, const rk_traj_desc* d, int32_t capacit
   b->na = D;
   b->traj_T = d->frames;
   b->traj_na = r.na;
-  b->traj_lim = tt.lim_orders != 0;
+  b->traj_lim = tt.lim_orders != 0 || tt.ng > 0;
   const size_t S = capacity;
   auto A = [&](void** p, size_t bytes) { return dmalloc(ctx, p, bytes, &b->bytes); };
   if ((rc = A((void**)&b->tt, sizeof(TrajTerms))) || (rc = A((void**)&b->td, sizeof(Terms))) ||

@@ -78,7 +78,7 @@ struct SrvTerms {
   Terms* td = nullptr;
   int variant = 0, na = 0;  // na: variables per problem (frames * dofs for trajectories)
   int traj_na = 0;          // > 0: trajectory problems of an arm with this many dofs
-  bool traj_lim = false;    // they carry bound rows
+  bool traj_lim = false;    // they carry bound rows or scene geometries
   TrajTerms* tt = nullptr;
 };
 
@@ -649,7 +649,7 @@ int rk_server_add_traj(rk_server* sv, const rk_traj_desc* d, int32_t* term_id) {
     if ((rc = make_traj_terms(ctx, d, &tt))) return rc;
     T.th = traj_goal_terms(d, tt);
     T.traj_na = ctx->robots[d->robot].na;
-    T.traj_lim = tt.lim_orders != 0;
+    T.traj_lim = tt.lim_orders != 0 || tt.ng > 0;
     T.na = d->frames * T.traj_na;
     if ((rc = dmalloc(ctx, (void**)&T.td, sizeof(Terms), nullptr))) return rc;
     if ((rc = dmalloc(ctx, (void**)&T.tt, sizeof(TrajTerms), nullptr))) return rc;

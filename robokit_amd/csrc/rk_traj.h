@@ -12,8 +12,9 @@
 // shifted one lane per column; the forward substitution is fused into it and the backward one
 // runs the same way.  Rows and term costs in this order: goal, velocity, acceleration,
 // collision, then the bound rows of orders 0, 1, 2 (the reference sums term costs in list order).
-// The bound rows (TrajectoryLimitTask, forward stencil) are in a separate kernel instantiation
-// (LIM), so that trajectories without them keep their register allocation.
+// The bound rows (TrajectoryLimitTask, forward stencil) and scene volumes / meshes are in a
+// separate kernel instantiation (LIM), so that trajectories without them keep their register
+// allocation; with no bound rows and no geometries it computes the same bits as the other.
 
 #define RK_TRAJ_MAXT 64
 #define RK_TRAJ_MAXS 64  // robot collision spheres
@@ -31,6 +32,8 @@ struct TrajTerms {
   int ns, nb;
   float obs_s[RK_SCENE_MAXS][4];
   float obs_b[RK_SCENE_MAXB][6];
+  int ng;  // the scene's volumes / meshes, after the boxes
+  SceneGeom geom[RK_SCENE_MAXG];
   // TrajectoryLimitTask rows, forward stencil (terms/sparse/trajectory_limit_task.py:436-498):
   // for order o (bit o of lim_orders) and window f = 0..T-1-o, per dof, v = lim_scale[o] *
   // sum_k c_k q[f + k] with c = (1), (-1, 1), (1, -2, 1), r = w max(lo - v, v - hi, 0)
@@ -49,7 +52,7 @@ typedef const __attribute__((address_space(4))) TrajTerms TrajK;
 // rows (lane T-1 only) in cg and of the collision rows in cc.  The two wavefronts of a problem
 // split the robot spheres: PART 0 takes the even-numbered ones and the goal rows, PART 1 the
 // odd-numbered ones.
-template <int NA, int NJ, bool JAC, int PART>
+template <int NA, int NJ, bool JAC, int PART, bool GEO>
 __device__ __forceinline__ void traj_frame(RobotK* rb, TrajK* tt, const float (&q)[NA], bool last, const Pose& goal,
                                            float (&hb)[NA][NA], float (&gf)[NA], float& cg, float& cc) {
   const int nj = rb->nj;
@@ -68,13 +71,13 @@ __device__ __forceinline__ void traj_frame(RobotK* rb, TrajK* tt, const float (&
       c[1] += P.t[1];
       c[2] += P.t[2];
       int grp;
-      const float sd = scene_sdf(tt, c, grp) - tt->sph_r[k];
+      const float sd = scene_sdf<GEO>(tt, c, grp) - tt->sph_r[k];
       if (sd < margin) {
         const float res = w * (margin - sd);
         cc += res * res;
         if (JAC) {
           float n[3];
-          scene_normal(tt, c, scene_winner(tt, c, grp), n);
+          scene_normal<GEO>(tt, c, scene_winner(tt, c, grp), n);
           const float cn0 = c[1] * n[2] - c[2] * n[1], cn1 = c[2] * n[0] - c[0] * n[2], cn2 = c[0] * n[1] - c[1] * n[0];
           float jr[NA];
 #pragma unroll
@@ -643,7 +646,7 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
   auto eval_cost = [&](const float (&qq)[NA]) {
     float hb[NA][NA], gf[NA];
     float cg = 0.f, cc = 0.f, cv = 0.f, ca = 0.f;
-    traj_frame<NA, NJ, false, 0>(rb, tt, qq, last, goal, hb, gf, cg, cc);
+    traj_frame<NA, NJ, false, 0, LIM>(rb, tt, qq, last, goal, hb, gf, cg, cc);
     traj_smooth<NA, false>(tt, t, T, qq, gf, cv, ca);
     float cl[3] = {0.f, 0.f, 0.f};
     if constexpr (LIM) {
@@ -686,7 +689,7 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
         for (int b2 = 0; b2 < NA; ++b2) hb[a][b2] = 0.f;
       }
       float cg = 0.f, cc = 0.f, cv = 0.f, ca = 0.f;
-      traj_frame<NA, NJ, true, 0>(rb, tt, q, last, goal, hb, gf, cg, cc);
+      traj_frame<NA, NJ, true, 0, LIM>(rb, tt, q, last, goal, hb, gf, cg, cc);
       traj_smooth<NA, true>(tt, t, T, q, gf, cv, ca);
       float cl[3] = {0.f, 0.f, 0.f};
       if constexpr (LIM) {
@@ -792,7 +795,7 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
 // its half of the band solve.  Its barriers pair one for one with traj_solve's: rows ready (after
 // an accepted step only), band built, the three of band_solve, proposal cost ready, iterate
 // published.
-template <int NA, int NJ>
+template <int NA, int NJ, bool LIM = false>
 __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s, int it0, int iters,
                                             TrajLDS<NA>* sh) {
   RobotK* rb = as_const(Pg.robot);
@@ -809,7 +812,7 @@ __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s
   auto proposal_cost = [&](const float (&qq)[NA]) {
     float hb[NA][NA], gf[NA];
     float cg = 0.f, cc = 0.f;
-    traj_frame<NA, NJ, false, 1>(rb, tt, qq, false, goal, hb, gf, cg, cc);
+    traj_frame<NA, NJ, false, 1, LIM>(rb, tt, qq, false, goal, hb, gf, cg, cc);
     sh->pcc[lane] = cc;
     __syncthreads();
   };
@@ -828,7 +831,7 @@ __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s
         for (int b2 = 0; b2 < NA; ++b2) hb[a][b2] = 0.f;
       }
       float cg = 0.f, cc = 0.f;
-      traj_frame<NA, NJ, true, 1>(rb, tt, q, false, goal, hb, gf, cg, cc);
+      traj_frame<NA, NJ, true, 1, LIM>(rb, tt, q, false, goal, hb, gf, cg, cc);
       int e = 0;
 #pragma unroll
       for (int a = 0; a < NA; ++a)
@@ -859,6 +862,8 @@ __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s
   }
 }
 
+// LIM: the kernel instantiation with bound rows and scene volumes / meshes (the other one, for
+// primitive scenes without bound rows, compiles neither)
 template <int NA, int NJ, bool LIM = false>
 __global__ __launch_bounds__(128) void k_traj(Problems P, const TrajTerms* tt, LMParams lp, int it0, int iters) {
   __shared__ __align__(16) TrajLDS<NA> sh;
@@ -866,7 +871,7 @@ __global__ __launch_bounds__(128) void k_traj(Problems P, const TrajTerms* tt, L
   if (threadIdx.x < 64)
     traj_solve<NA, NJ, LIM>(P, as_const(tt), blockIdx.x, lp, it0, iters, &sh);
   else
-    traj_worker<NA, NJ>(P, as_const(tt), blockIdx.x, it0, iters, &sh);
+    traj_worker<NA, NJ, LIM>(P, as_const(tt), blockIdx.x, it0, iters, &sh);
 }
 
 // goal errors at the last frame and the smallest sphere-to-scene distance (one wave per problem)

@@ -568,13 +568,32 @@ class SmoothnessTask(Task):
 
 
 class PrimitiveScene:
-    """A world scene of obstacle spheres and axis-aligned boxes: the primitive form of the
-    reference's WarpScene with SphereGeom / BoxGeom members (geom/scene.py:132-256,
-    geom/geoms.py:372-429), queried by its signed distance (geom/sdf_kernels.py:256-295)."""
+    """A world scene of obstacle spheres and axis-aligned boxes, then up to four signed-distance
+    volumes / triangle meshes (robokit_amd.geom.SdfVolume / TriMesh): the reference's WarpScene
+    with SphereGeom / BoxGeom / VolumeGeom / MeshGeom members (geom/scene.py:132-256,
+    geom/geoms.py:372-660), queried by its signed distance (geom/sdf_kernels.py:151-380)."""
 
-    def __init__(self, spheres: Optional[np.ndarray] = None, boxes: Optional[np.ndarray] = None):
+    def __init__(self, spheres: Optional[np.ndarray] = None, boxes: Optional[np.ndarray] = None, geoms=()):
+        from .geom import as_geom
+
         self.spheres = np.zeros((0, 4), np.float32) if spheres is None else np.asarray(spheres, np.float32).reshape(-1, 4)
         self.boxes = np.zeros((0, 6), np.float32) if boxes is None else np.asarray(boxes, np.float32).reshape(-1, 6)
+        self.geoms = [as_geom(g) for g in geoms]
+        if len(self.geoms) > 4:
+            raise ValueError("at most 4 volumes / meshes per scene")
+
+    def add_geometry(self, geom) -> "PrimitiveScene":
+        """append a volume (SdfVolume), a mesh (TriMesh) or a (vertices, faces) pair"""
+        from .geom import as_geom
+
+        if len(self.geoms) >= 4:
+            raise ValueError("at most 4 volumes / meshes per scene")
+        self.geoms.append(as_geom(geom))
+        return self
+
+    def query_sdf(self, points: np.ndarray, device=None):
+        """(signed distance [n], normal [n, 3]) at points [n, 3] on the GPU (WarpScene.query_sdf)"""
+        return get_context(device).scene_query(points, self.spheres, self.boxes, self.geoms)
 
     def add_spheres(self, centers, radii) -> "PrimitiveScene":
         c = np.asarray(centers, np.float32).reshape(-1, 3)
@@ -589,8 +608,8 @@ class PrimitiveScene:
         return self
 
     @property
-    def geoms(self) -> int:
-        return len(self.spheres) + len(self.boxes)
+    def num_geoms(self) -> int:
+        return len(self.spheres) + len(self.boxes) + len(self.geoms)
 
 
 class SceneCollisionTask(Task):
@@ -933,6 +952,7 @@ class LMOptimizer:
             self.termspec.coll_weights, self.termspec.coll_margin = c.weights, c.margin
             self.termspec.coll_penalty = c.penalty
             self.termspec.obstacle_spheres, self.termspec.obstacle_boxes = c.scene.spheres, c.scene.boxes
+            self.termspec.geoms = list(c.scene.geoms)
 
     def _plan_trajectory(self) -> None:
         """the trajectory problem set of the terms: goal FrameTask on the last frame, velocity /
@@ -979,7 +999,7 @@ class LMOptimizer:
         if coll:
             c = coll[0]
             scene = TrajScene(c.sphere_links.astype(np.int32), c.sphere_centers, c.sphere_radii,
-                              c.scene.spheres, c.scene.boxes)
+                              c.scene.spheres, c.scene.boxes, list(c.scene.geoms))
         else:
             scene = TrajScene(np.zeros(0, np.int32), np.zeros((0, 3), np.float32), np.zeros(0, np.float32),
                               np.zeros((0, 4), np.float32), np.zeros((0, 6), np.float32))

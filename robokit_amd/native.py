@@ -52,8 +52,18 @@ class TermsDesc(C.Structure):
         ("n_coll", C.c_int32), ("coll_links", C.c_void_p), ("coll_centers", C.c_void_p),
         ("coll_radii", C.c_void_p), ("coll_weights", C.c_void_p), ("coll_margin", C.c_float),
         ("coll_penalty", C.c_int32), ("n_obs_spheres", C.c_int32), ("obs_spheres", C.c_void_p),
-        ("n_obs_boxes", C.c_int32), ("obs_boxes", C.c_void_p),
+        ("n_obs_boxes", C.c_int32), ("obs_boxes", C.c_void_p), ("n_geoms", C.c_int32), ("geoms", C.c_void_p),
     ]
+
+
+class VolumeDescC(C.Structure):
+    _fields_ = [("values", C.c_void_p), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("origin", C.c_float * 3), ("voxel_size", C.c_float), ("aabb_min", C.c_float * 3),
+                ("aabb_max", C.c_float * 3), ("padding", C.c_float)]
+
+
+class MeshDescC(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("n_vertices", C.c_int32), ("faces", C.c_void_p), ("n_faces", C.c_int32)]
 
 COLL_PENALTY = {"smooth": 0, "plain": 1, "surface_distance": 2}
 
@@ -70,13 +80,14 @@ class TrajDescC(C.Structure):
                 ("coll_weight", C.c_float), ("coll_margin", C.c_float), ("n_obs_spheres", C.c_int32),
                 ("obs_spheres", C.c_void_p), ("n_obs_boxes", C.c_int32), ("obs_boxes", C.c_void_p),
                 ("limit_orders", C.c_int32), ("limit_weights", C.c_void_p), ("limit_lower", C.c_void_p),
-                ("limit_upper", C.c_void_p), ("cg_max_iter", C.c_int32)]
+                ("limit_upper", C.c_void_p), ("cg_max_iter", C.c_int32), ("n_geoms", C.c_int32), ("geoms", C.c_void_p)]
 
 
-def traj_desc(robot_id: int, tb, goal_weights=None):
+def traj_desc(robot_id: int, tb, goal_weights=None, ctx: Optional["Context"] = None):
     """(rk_traj_desc, arrays it points at) of a workloads.TrajBatch-shaped problem set: robot,
-    frames, scene, weights, and the optional bound rows (lim_orders bit o, lim_w / lim_lo / lim_hi
-    [3, na]) and the CG budget (cg_iters: 0 = band Cholesky)"""
+    frames, scene (with its volumes / meshes uploaded to ``ctx``), weights, and the optional bound
+    rows (lim_orders bit o, lim_w / lim_lo / lim_hi [3, na]) and the CG budget (cg_iters: 0 = band
+    Cholesky)"""
     sc = tb.scene
     na = tb.spec.num_actuated_joints
     arrs = [_f32(np.ones(6) if goal_weights is None else goal_weights), _i32(sc.sphere_links),
@@ -90,6 +101,13 @@ def traj_desc(robot_id: int, tb, goal_weights=None):
     d = TrajDescC(robot_id, tb.T, tb.goal_link, _p(arrs[0]), tb.wv, tb.wa, tb.dt, len(arrs[1]), _p(arrs[1]),
                   _p(arrs[2]), _p(arrs[3]), tb.coll_w, tb.margin, len(arrs[4]), _p(arrs[4]), len(arrs[5]), _p(arrs[5]),
                   orders, _p(lim[0]), _p(lim[1]), _p(lim[2]), int(getattr(tb, "cg_iters", 0) or 0))
+    geoms = list(getattr(sc, "geoms", None) or [])
+    if geoms:
+        if ctx is None:
+            raise ValueError("a scene with volumes / meshes needs the context they are uploaded to")
+        ids = _i32([ctx.geom(g) for g in geoms])
+        arrs.append(ids)
+        d.n_geoms, d.geoms = len(geoms), _p(ids)
     return d, arrs
 
 
@@ -133,6 +151,7 @@ EXPORTS = [
     "rk_group_destroy", "rk_group_set_tuning", "rk_group_autotune", "rk_group_solve", "rk_group_kernel_times",
     "rk_server_create", "rk_server_add_terms", "rk_server_add_traj", "rk_server_submit", "rk_server_poll", "rk_server_end_session",
     "rk_server_stats", "rk_server_destroy", "rk_traj_batch_create", "rk_traj_metrics",
+    "rk_geom_upload_volume", "rk_geom_upload_mesh", "rk_scene_query",
 ]
 
 
@@ -196,6 +215,7 @@ class Context:
         self._children = weakref.WeakSet()  # batches / groups / servers on this context
         self.device = device
         self._robots: Dict[int, int] = {}
+        self._geoms: Dict[int, int] = {}
         self._keep: List = []
 
     def close(self) -> None:
@@ -253,6 +273,45 @@ class Context:
         self._robots[key] = rid.value
         self._keep.append(spec)
         return rid.value
+
+    def geom(self, g) -> int:
+        """Upload a scene volume / mesh (robokit_amd.geom) once (resident) and return its id."""
+        from .geom import SdfVolume, TriMesh
+
+        key = id(g)
+        if key in self._geoms:
+            return self._geoms[key]
+        gid = C.c_int32()
+        if isinstance(g, SdfVolume):
+            nx, ny, nz = g.shape_xyz
+            vals = _f32(g.values)
+            d = VolumeDescC(_p(vals), nx, ny, nz, (C.c_float * 3)(*g.origin), float(g.voxel_size),
+                            (C.c_float * 3)(*g.aabb_min), (C.c_float * 3)(*g.aabb_max), float(g.padding))
+            _check(lib().rk_geom_upload_volume(self._h, C.byref(d), C.byref(gid)))
+        elif isinstance(g, TriMesh):
+            v, f = _f32(g.vertices), _i32(g.faces)
+            d = MeshDescC(_p(v), len(v), _p(f), len(f))
+            _check(lib().rk_geom_upload_mesh(self._h, C.byref(d), C.byref(gid)))
+        else:
+            raise TypeError(f"scene geometries are SdfVolume or TriMesh, got {type(g).__name__}")
+        self._geoms[key] = gid.value
+        self._keep.append(g)
+        return gid.value
+
+    def scene_query(self, points: np.ndarray, spheres: Optional[np.ndarray] = None,
+                    boxes: Optional[np.ndarray] = None, geoms: Sequence = ()):
+        """(signed distance [n], normal [n, 3]) of the scene of obstacle spheres [k, 4], axis-aligned
+        boxes [k, 6] and volumes / meshes at points [n, 3] (rk_scene_query)"""
+        pts = _f32(np.asarray(points, np.float32).reshape(-1, 3))
+        sp = _f32(np.zeros((0, 4)) if spheres is None else np.asarray(spheres).reshape(-1, 4))
+        bx = _f32(np.zeros((0, 6)) if boxes is None else np.asarray(boxes).reshape(-1, 6))
+        ids = _i32([self.geom(g) for g in geoms])
+        n = len(pts)
+        dist = np.zeros(n, np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        _check(lib().rk_scene_query(self._h, len(sp), _p(sp), len(bx), _p(bx), len(ids), _p(ids), n, _p(pts), _p(dist),
+                                    _p(nrm)))
+        return dist, nrm
 
     def fk(self, spec, q: np.ndarray, T_world_base: Optional[np.ndarray] = None) -> np.ndarray:
         """T_world_link [B, nl, 7]; T_world_base [B, 7] or [7] (None: the identity)"""
@@ -319,13 +378,14 @@ class TermSpec:
     coll_penalty: str = "smooth"  # smooth / plain / surface_distance
     obstacle_spheres: Optional[np.ndarray] = None  # [n, 4] centre, radius
     obstacle_boxes: Optional[np.ndarray] = None  # [n, 6] centre, half extents
+    geoms: Sequence = ()  # scene volumes / meshes (robokit_amd.geom), after the boxes
 
     def rows(self, na: int) -> int:
         return 6 * len(self.pose_links) + 3 * len(self.pos_links) + 3 * len(self.rot_links) + \
             na * (self.limit_weights is not None) + na * (self.smooth_weights is not None) + len(self.coll_links)
 
-    def desc(self, robot_id: int, na: int):
-        """(rk_terms_desc, the arrays it points at)"""
+    def desc(self, robot_id: int, na: int, ctx: Optional["Context"] = None):
+        """(rk_terms_desc, the arrays it points at); the scene's volumes / meshes are uploaded to ``ctx``"""
         arrs = [
             _i32(list(self.pose_links)),
             _f32(self.pose_weights if self.pose_weights is not None else np.zeros((0, 6))),
@@ -355,6 +415,12 @@ class TermSpec:
                       len(self.rot_links), _p(a[6]), _p(a[7]),
                       nc, _p(a[8]), _p(a[9]), _p(a[10]), _p(a[11]), float(self.coll_margin),
                       COLL_PENALTY[self.coll_penalty], a[12].shape[0], _p(a[12]), a[13].shape[0], _p(a[13]))
+        if self.geoms:
+            if ctx is None:
+                raise ValueError("a scene with volumes / meshes needs the context they are uploaded to")
+            ids = _i32([ctx.geom(g) for g in self.geoms])
+            arrs.append(ids)
+            d.n_geoms, d.geoms = len(ids), _p(ids)
         return d, arrs
 
 
@@ -366,7 +432,7 @@ class Batch:
         self.capacity, self.trace_iters = int(capacity), int(trace_iters)
         na = spec.num_actuated_joints
         self._na = na  # variables per problem
-        d, self._arrs = terms.desc(ctx.robot(spec), na)
+        d, self._arrs = terms.desc(ctx.robot(spec), na, ctx)
         self._h = C.c_void_p()
         _check(lib().rk_batch_create(ctx.handle, C.byref(d), self.capacity, self.trace_iters, C.byref(self._h)))
         _LIVE["batch"].add(self)
@@ -548,7 +614,7 @@ class Server:
 
     def add_terms(self, spec, terms: TermSpec) -> int:
         na = spec.num_actuated_joints
-        d, arrs = terms.desc(self.ctx.robot(spec), na)
+        d, arrs = terms.desc(self.ctx.robot(spec), na, self.ctx)
         tid = C.c_int32()
         _check(lib().rk_server_add_terms(self._h, C.byref(d), C.byref(tid)))
         self._terms.append((na, len(terms.pose_links), len(terms.pos_links)))
@@ -556,7 +622,7 @@ class Server:
 
     def add_traj(self, tb, goal_weights=None) -> int:
         """register a trajectory problem set (workloads.TrajBatch shape: robot, frames, scene)"""
-        d, arrs = traj_desc(self.ctx.robot(tb.spec), tb, goal_weights)
+        d, arrs = traj_desc(self.ctx.robot(tb.spec), tb, goal_weights, self.ctx)
         tid = C.c_int32()
         _check(lib().rk_server_add_traj(self._h, C.byref(d), C.byref(tid)))
         self._terms.append((tb.T * tb.spec.num_actuated_joints, 1, 0))
@@ -619,7 +685,7 @@ class TrajBatch(Batch):
         self._na = self.T * self.robot_na
         self.terms = TermSpec(pose_links=[tb.goal_link], pose_weights=np.ones((1, 6), np.float32))
         self.capacity, self.trace_iters = int(capacity), int(trace_iters)
-        d, self._arrs = traj_desc(ctx.robot(tb.spec), tb, goal_weights)
+        d, self._arrs = traj_desc(ctx.robot(tb.spec), tb, goal_weights, ctx)
         self._h = C.c_void_p()
         _check(lib().rk_traj_batch_create(ctx.handle, C.byref(d), self.capacity, self.trace_iters, C.byref(self._h)))
         _LIVE["batch"].add(self)

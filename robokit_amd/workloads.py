@@ -282,6 +282,7 @@ class TrajScene:
     sphere_radii: np.ndarray  # [m] float32
     obstacle_spheres: np.ndarray  # [ns, 4] float32: centre, radius
     obstacle_boxes: np.ndarray  # [nb, 6] float32: centre, half extents (axis-aligned)
+    geoms: list = field(default_factory=list)  # volumes / meshes (robokit_amd.geom), after the boxes
 
 
 @dataclass

@@ -181,7 +181,7 @@ def test_scene_collision_task_validation():
         L.SceneCollisionTask(robot, scene, weight=[1.0, 2.0], sphere_indices=[0, 5, 7])
     with pytest.raises(NotImplementedError):
         L.SceneCollisionTask(robot, scene, residual_mode="sqrt_abs")
-    assert scene.geoms == len(scene.spheres) + len(scene.boxes)
+    assert scene.num_geoms == len(scene.spheres) + len(scene.boxes)
 
 
 @pytest.mark.gpu
