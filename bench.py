@@ -502,7 +502,63 @@ def main() -> int:
                 "value": round(m / t_c, 3), "unit": "trajectories/s", "cores": 1, "kind": "port",
                 "sample": f"the first {m} trajectories, {ITERS} iterations, {t_c:.1f} s single-threaded oracle "
                           "(dense normal equations, the reference's generic step)"}
+        # the same trajectories stepped by conjugate gradients (the reference's default budget)
+        tcg = dataclasses.replace(tb, cg_iters=tb.cg_default_iters)
+        cgb = N.TrajBatch(ctx, tcg, 512)
+        cgb.set(q=tcg.q_seed, goals=tcg.goals)
+        cgb.solve(ITERS, params, from_seed=True)
+        cg_ms = float(np.mean([cgb.solve(ITERS, params, from_seed=True) for _ in range(3)]))
+        pe2, ae2, sd2 = cgb.metrics()
+        out["trajectory"]["cg"] = {
+            "linear_solver": f"conjugate gradients, {tcg.cg_iters} iterations per LM step (max(4, min(64, D / 4)))",
+            "ms_per_solve": round(cg_ms, 3),
+            "goal_reached_and_collision_free": int(((pe2 < 1e-3) & (ae2 < 1e-2) & (sd2 >= 0)).sum()),
+            "median_goal_error_m": round(float(np.median(pe2)), 5)}
+        cgb.close()
         print(f"[bench] trajectory: {out['trajectory']}", file=sys.stderr, flush=True)
+
+        # ---- uneven horizons in the group: the IK batches plus trajectories of 16, 32 and 64 frames in
+        # one graph (the trajectories' launch overlaps the IK kernel) against the two solved in turn
+        mix = [W.config3(seed=args.seed + 40 + k, batch=n, T=T) for k, (n, T) in enumerate([(256, 16), (128, 32),
+                                                                                               (128, 64)])]
+        mt = []
+        for t in mix:
+            tbb = N.TrajBatch(ctx, t, t.size)
+            tbb.set(q=t.q_seed, goals=t.goals)
+            mt.append(tbb)
+        g2 = N.Group(ctx, batches + mt)
+        g2.set_tuning(tpp + [0] * len(mt))
+        for _ in range(2):
+            g2.solve(ITERS, params, use_graph=True, from_seed=True)
+        ctx.sync()
+        reps = max(3, args.steps // 4)
+        ctx.mark(2)
+        for _ in range(reps):
+            g2.solve(ITERS, params, use_graph=True, from_seed=True, sync=False)
+        ctx.mark(3)
+        packed = ctx.elapsed(2, 3) / reps
+        ctx.mark(4)
+        for _ in range(reps):
+            group.solve(ITERS, params, use_graph=True, from_seed=True, sync=False)
+            for tbb in mt:
+                tbb.solve(ITERS, params, from_seed=True, wait=False)
+        ctx.mark(5)
+        apart = ctx.elapsed(4, 5) / reps
+        g2.solve(ITERS, params, use_graph=False, from_seed=True)
+        out["uneven_with_trajectories"] = {
+            "workload": f"the {args.problems} IK problems above + 256 x 16, 128 x 32 and 128 x 64-frame trajectories "
+                        f"(7-DOF, the trajectory leg's terms), {ITERS} iterations",
+            "ms_per_solve_packed": round(packed, 4),
+            "what_packed": "one group graph: the IK kernel and one trajectory launch of all three horizons, on "
+                           "parallel branches (HIP events on the solve stream, solves enqueued back to back)",
+            "ms_per_solve_apart": round(apart, 4),
+            "what_apart": "the IK group graph, then the three trajectory batches one launch each, enqueued back to back",
+            "kernels_eager": [{"tpp": k[0], "ms": round(k[1], 4), "batches": k[2]} for k in g2.kernel_times()],
+        }
+        g2.close()
+        for tbb in mt:
+            tbb.close()
+        print(f"[bench] uneven with trajectories: {out['uneven_with_trajectories']}", file=sys.stderr, flush=True)
 
     # ---- served: Poisson request stream through the continuous-batching server vs plain per request
     if args.serve_seconds > 0:

@@ -220,6 +220,7 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
  * 4, 8, 16, 32, 64 = that many lanes per problem, LDS-staged (any robot the batch accepts) */
 #define RK_TPP_WAVE (-64)
 #define RK_TPP_HALF (-32)
+#define RK_TPP_TRAJ (-128) /* a trajectory: two wavefronts per problem (rk_group_kernel_times, autotune) */
 #define RK_USE_GRAPH 0x1   /* rk_group_solve flag: capture the launch sequence in a hipGraph and replay it */
 #define RK_ASYNC 0x2       /* rk_group_solve flag: return once the solve is enqueued on the context's stream
                             * (kernel_ms is not written; rk_context_sync or a later synchronous call waits) */
@@ -267,9 +268,13 @@ int rk_batch_metrics_noalloc(rk_batch* b, float pos_thr, float ang_thr, float* p
                              int32_t* success);
 
 /*
- * Uneven batch: several batches (different robots / sizes / terms) solved by ONE persistent
- * kernel that pulls problem tiles from a device work queue; each batch runs with its own
- * threads-per-problem.  No reference counterpart (the reference batches one robot per optimizer).
+ * Uneven batch: several batches (different robots / sizes / terms; at most 32) solved by ONE
+ * persistent kernel that pulls problem tiles from a device work queue; each batch runs with its
+ * own threads-per-problem.  Trajectory batches (rk_traj_batch_create) of any horizons join the
+ * group too: those of one arm class (dofs; bound rows or scene geometries) share one launch of
+ * consecutive workgroup ranges, on a stream forked from the context stream, so they overlap the
+ * IK kernel (parallel branches of the captured graph).  No reference counterpart (the reference
+ * batches one robot per optimizer).
  */
 int rk_group_create(rk_context* ctx, int32_t nbatches, rk_batch* const* batches, rk_group** out);
 int rk_group_destroy(rk_group* g);

@@ -1613,6 +1613,9 @@ struct rk_batch {
   int64_t bytes = 0;
 };
 
+#define RK_GROUP_TRAJ_MAX 4  // trajectory arm classes: 6 / 7 dofs, with or without bound rows / geometries
+#define RK_VAR_TRAJ 1000     // last_var code of trajectory class c: RK_VAR_TRAJ + c
+
 struct rk_group {
   rk_context* ctx = nullptr;
   std::vector<rk_batch*> batches;
@@ -1621,9 +1624,14 @@ struct rk_group {
   Bucket* buckets_dev = nullptr;    // bucket table of eager solves (rewritten by each one)
   Bucket* buckets_graph = nullptr;  // bucket table the captured graph reads (written only at capture)
   int* counter = nullptr;  // per variant: tile queue and exit count (self-resetting)
-  hipStream_t aux[NVARIANT] = {};
-  hipEvent_t ev_fork = nullptr, ev_join[NVARIANT] = {};
-  hipEvent_t ev_k0[NVARIANT] = {}, ev_k1[NVARIANT] = {};  // per-variant kernel timing
+  // trajectory batches (any horizon): one k_traj_multi launch per arm class on its own stream
+  std::vector<int> ik, traj;           // indices of the IK / trajectory batches in `batches`
+  TrajBucket* traj_dev = nullptr;      // trajectory table of eager solves
+  TrajBucket* traj_graph = nullptr;    // trajectory table the captured graph reads
+  // kernels of a solve: the IK variants, then the trajectory classes (RK_GROUP_TRAJ_MAX at most)
+  hipStream_t aux[NVARIANT + RK_GROUP_TRAJ_MAX] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[NVARIANT + RK_GROUP_TRAJ_MAX] = {};
+  hipEvent_t ev_k0[NVARIANT + RK_GROUP_TRAJ_MAX] = {}, ev_k1[NVARIANT + RK_GROUP_TRAJ_MAX] = {};
   std::vector<int> last_var;                               // variants of the last solve, launch order
   std::vector<unsigned> last_mask;                         // batches (bit = index) of each of those kernels
   hipGraphExec_t graph = nullptr;
@@ -3017,31 +3025,36 @@ int rk_group_create(rk_context* ctx, int32_t nb, rk_batch* const* batches, rk_gr
   if (!ctx || nb <= 0 || !batches || !out) return fail(RK_EINVAL, "bad argument to rk_group_create");
   for (int i = 0; i < nb; ++i)
     if (!batches[i] || batches[i]->ctx != ctx) return fail(RK_EINVAL, "batch of another context or null");
-  for (int i = 0; i < nb; ++i)
-    if (batches[i]->traj_T) return fail(RK_EINVAL, "trajectory batches are solved on their own (rk_batch_solve)");
+  if (nb > 32) return fail(RK_EINVAL, "at most 32 batches per group");
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device));
   rk_group* g = new rk_group();
   g->ctx = ctx;
   g->batches.assign(batches, batches + nb);
   g->tpp.assign(nb, 0);
-  int rc;
-  if ((rc = dmalloc(ctx, (void**)&g->buckets_dev, sizeof(Bucket) * nb, &g->bytes)) ||
-      (rc = dmalloc(ctx, (void**)&g->buckets_graph, sizeof(Bucket) * nb, &g->bytes)) ||
-      (rc = dmalloc(ctx, (void**)&g->counter, sizeof(int) * 2 * NVARIANT, &g->bytes))) {
-    dfree(ctx, g->buckets_dev, sizeof(Bucket) * nb);
-    dfree(ctx, g->buckets_graph, sizeof(Bucket) * nb);
+  for (int i = 0; i < nb; ++i) (batches[i]->traj_T ? g->traj : g->ik).push_back(i);
+  const int nik = std::max<int>(1, (int)g->ik.size()), ntj = std::max<int>(1, (int)g->traj.size());
+  auto release = [&]() {
+    dfree(ctx, g->buckets_dev, sizeof(Bucket) * nik);
+    dfree(ctx, g->buckets_graph, sizeof(Bucket) * nik);
+    dfree(ctx, g->traj_dev, sizeof(TrajBucket) * ntj);
+    dfree(ctx, g->traj_graph, sizeof(TrajBucket) * ntj);
     dfree(ctx, g->counter, sizeof(int) * 2 * NVARIANT);
     delete g;
+  };
+  int rc;
+  if ((rc = dmalloc(ctx, (void**)&g->buckets_dev, sizeof(Bucket) * nik, &g->bytes)) ||
+      (rc = dmalloc(ctx, (void**)&g->buckets_graph, sizeof(Bucket) * nik, &g->bytes)) ||
+      (rc = dmalloc(ctx, (void**)&g->traj_dev, sizeof(TrajBucket) * ntj, &g->bytes)) ||
+      (rc = dmalloc(ctx, (void**)&g->traj_graph, sizeof(TrajBucket) * ntj, &g->bytes)) ||
+      (rc = dmalloc(ctx, (void**)&g->counter, sizeof(int) * 2 * NVARIANT, &g->bytes))) {
+    release();
     return rc;
   }
   // (creation is never captured into a graph)
   if (hipMemsetAsync(g->counter, 0, sizeof(int) * 2 * NVARIANT, ctx->stream) != hipSuccess ||
       hipStreamSynchronize(ctx->stream) != hipSuccess) {
-    dfree(ctx, g->buckets_dev, sizeof(Bucket) * nb);
-    dfree(ctx, g->buckets_graph, sizeof(Bucket) * nb);
-    dfree(ctx, g->counter, sizeof(int) * 2 * NVARIANT);
-    delete g;
+    release();
     return fail(RK_EHIP, "could not clear the group's queue counters");
   }
   *out = g;
@@ -3054,7 +3067,7 @@ int rk_group_destroy(rk_group* g) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   if (g->graph) hipGraphExecDestroy(g->graph);
-  for (int k = 0; k < NVARIANT; ++k) {
+  for (int k = 0; k < NVARIANT + RK_GROUP_TRAJ_MAX; ++k) {
     if (g->aux[k]) hipStreamDestroy(g->aux[k]);
     if (g->ev_join[k]) hipEventDestroy(g->ev_join[k]);
     if (g->ev_k0[k]) hipEventDestroy(g->ev_k0[k]);
@@ -3063,6 +3076,8 @@ int rk_group_destroy(rk_group* g) {
   if (g->ev_fork) hipEventDestroy(g->ev_fork);
   hipFree(g->buckets_dev);
   hipFree(g->buckets_graph);
+  hipFree(g->traj_dev);
+  hipFree(g->traj_graph);
   hipFree(g->counter);
   ctx->mem_bytes -= g->bytes;
   delete g;
@@ -3074,6 +3089,10 @@ int rk_group_set_tuning(rk_group* g, const int32_t* tpp, int32_t wg_per_cu, int3
   if (wg_per_cu < 0 || iters_per_launch < 0) return fail(RK_EINVAL, "negative tuning value");
   if (tpp) {  // every entry is checked before any is applied
     for (size_t i = 0; i < g->batches.size(); ++i) {
+      if (g->batches[i]->traj_T) {  // trajectories have one kernel form
+        if (tpp[i] != 0 && tpp[i] != RK_TPP_TRAJ) return fail(RK_EINVAL, "a trajectory batch takes tpp 0");
+        continue;
+      }
       int v;
       int rc = pick_variant(g->batches[i], tpp[i], &v);
       if (rc) return rc;
@@ -3097,6 +3116,10 @@ struct GroupLaunch {
   std::vector<int> var, first, nb, tiles, grid;
   std::vector<unsigned> mask;
   std::vector<size_t> lds;
+  // trajectory classes: dofs, bound-row / geometry kernel, first table entry, entries, workgroups
+  const TrajBucket* ttable = nullptr;
+  std::vector<int> t_na, t_lim, t_first, t_nb, t_blocks;
+  std::vector<unsigned> t_mask;
 };
 
 template <int V>
@@ -3174,26 +3197,29 @@ static void launch_persistent(int v, dim3 grid, size_t lds, hipStream_t st, cons
 // grid.  Eager solves upload into g->buckets_dev; a capture uploads into g->buckets_graph, which
 // only the next capture rewrites, so a later eager solve (other seeds flag, other sizes) cannot
 // change what a replay of the captured graph reads.
-static int group_prepare(rk_group* g, GroupLaunch* gl, bool from_seed, Bucket* table) {
+static int group_prepare(rk_group* g, GroupLaunch* gl, bool from_seed, Bucket* table, TrajBucket* ttable) {
   rk_context* ctx = g->ctx;
-  std::vector<Bucket> bk(g->batches.size());
-  for (size_t i = 0; i < g->batches.size(); ++i) {
+  const size_t nik = g->ik.size();
+  std::vector<Bucket> bk(nik);
+  for (size_t i = 0; i < nik; ++i) {
+    rk_batch* b = g->batches[g->ik[i]];
     int v;
-    int rc = pick_variant(g->batches[i], g->tpp[i], &v);
+    int rc = pick_variant(b, g->tpp[g->ik[i]], &v);
     if (rc) return rc;
     bk[i].variant = v;
-    bk[i].P = make_problems(g->batches[i], from_seed);
+    bk[i].P = make_problems(b, from_seed);
     const int ppw = variant_ppw(v);
-    bk[i].tiles = (g->batches[i]->n + ppw - 1) / ppw;
+    bk[i].tiles = (b->n + ppw - 1) / ppw;
   }
   // group by variant; within a variant, big problems first (longest-processing-time order)
-  std::vector<int> order(bk.size());
+  std::vector<int> order(nik);
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  auto ikb = [&](int i) { return g->batches[g->ik[i]]; };
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     if (bk[a].variant != bk[b].variant) return bk[a].variant < bk[b].variant;
-    return g->batches[a]->na * g->batches[a]->th.R > g->batches[b]->na * g->batches[b]->th.R;
+    return ikb(a)->na * ikb(a)->th.R > ikb(b)->na * ikb(b)->th.R;
   });
-  std::vector<Bucket> sorted(bk.size());
+  std::vector<Bucket> sorted(nik);
   *gl = GroupLaunch();
   int base = 0;
   for (size_t i = 0; i < order.size(); ++i) {
@@ -3207,17 +3233,52 @@ static int group_prepare(rk_group* g, GroupLaunch* gl, bool from_seed, Bucket* t
       gl->mask.push_back(0u);
       base = 0;
     }
-    if (order[i] < 32) gl->mask.back() |= 1u << order[i];
+    gl->mask.back() |= 1u << g->ik[order[i]];
     sorted[i].tile_base = base;
     base += sorted[i].tiles;
     gl->nb.back() += 1;
     gl->tiles.back() = base;
   }
   gl->table = table;
-  HIP_TRY(hipMemcpyAsync(table, sorted.data(), sizeof(Bucket) * sorted.size(), hipMemcpyHostToDevice,
-                         ctx->stream));
+  if (nik)
+    HIP_TRY(hipMemcpyAsync(table, sorted.data(), sizeof(Bucket) * sorted.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+  // trajectory batches by arm class (6 / 7 dofs; bound rows or geometries), longest horizon first
+  // within a class: one table entry per batch, consecutive workgroup ranges
+  std::vector<int> tord(g->traj);
+  auto tclass = [&](int i) { return (g->batches[i]->traj_na == 7 ? 2 : 0) + (g->batches[i]->traj_lim ? 1 : 0); };
+  std::stable_sort(tord.begin(), tord.end(), [&](int a, int b) {
+    if (tclass(a) != tclass(b)) return tclass(a) < tclass(b);
+    return g->batches[a]->traj_T > g->batches[b]->traj_T;
+  });
+  std::vector<TrajBucket> tb(tord.size());
+  int blocks = 0;
+  for (size_t i = 0; i < tord.size(); ++i) {
+    rk_batch* b = g->batches[tord[i]];
+    if (i == 0 || tclass(tord[i]) != tclass(tord[i - 1])) {
+      gl->t_na.push_back(b->traj_na);
+      gl->t_lim.push_back(b->traj_lim ? 1 : 0);
+      gl->t_first.push_back((int)i);
+      gl->t_nb.push_back(0);
+      gl->t_blocks.push_back(0);
+      gl->t_mask.push_back(0u);
+      blocks = 0;
+    }
+    memset(&tb[i], 0, sizeof(TrajBucket));
+    tb[i].P = make_problems(b, from_seed);
+    tb[i].tt = b->tt;
+    tb[i].block0 = blocks;
+    blocks += b->n;
+    gl->t_nb.back() += 1;
+    gl->t_blocks.back() = blocks;
+    gl->t_mask.back() |= 1u << tord[i];
+  }
+  gl->ttable = ttable;
+  if (!tb.empty())
+    HIP_TRY(hipMemcpyAsync(ttable, tb.data(), sizeof(TrajBucket) * tb.size(), hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
-  for (size_t k = 0; k < gl->var.size(); ++k) {
+  const size_t nkern = gl->var.size() + gl->t_na.size();
+  for (size_t k = 0; k < nkern; ++k) {
     if (!g->aux[k]) HIP_TRY(hipStreamCreateWithFlags(&g->aux[k], hipStreamNonBlocking));
     if (!g->ev_join[k]) HIP_TRY(hipEventCreateWithFlags(&g->ev_join[k], hipEventDisableTiming));
     if (!g->ev_k0[k]) HIP_TRY(hipEventCreate(&g->ev_k0[k]));
@@ -3238,10 +3299,45 @@ static int group_prepare(rk_group* g, GroupLaunch* gl, bool from_seed, Bucket* t
 // enqueue the solve on the context stream: fork one stream per variant, each runs its
 // persistent kernel over its queue, join; or, for a group of register-resident variants outside
 // the timed path, one k_persistent_reg launch.  Capturable: no allocation, copy or memset.
+// the trajectory classes of a group solve, on their own streams forked from the context stream
+// (parallel branches of a captured graph), joined back to it
+static int group_launch_traj(rk_group* g, const GroupLaunch& gl, int iters, const LMParams& lp, bool timed) {
+  rk_context* ctx = g->ctx;
+  for (size_t c = 0; c < gl.t_na.size(); ++c) {
+    const size_t k = gl.var.size() + c;
+    hipStream_t st = g->aux[k];
+    HIP_TRY(hipStreamWaitEvent(st, g->ev_fork, 0));
+    if (timed) HIP_TRY(hipEventRecord(g->ev_k0[k], st));
+    const dim3 grid(gl.t_blocks[c]), blk(2 * WAVE);
+    const TrajBucket* tb = gl.ttable + gl.t_first[c];
+    const int nb = gl.t_nb[c];
+    if (gl.t_na[c] == 6 && gl.t_lim[c])
+      hipLaunchKernelGGL((k_traj_multi<6, 8, true>), grid, blk, 0, st, tb, nb, lp, 0, iters);
+    else if (gl.t_na[c] == 6)
+      hipLaunchKernelGGL((k_traj_multi<6, 8>), grid, blk, 0, st, tb, nb, lp, 0, iters);
+    else if (gl.t_lim[c])
+      hipLaunchKernelGGL((k_traj_multi<7, 9, true>), grid, blk, 0, st, tb, nb, lp, 0, iters);
+    else
+      hipLaunchKernelGGL((k_traj_multi<7, 9>), grid, blk, 0, st, tb, nb, lp, 0, iters);
+    HIP_TRY(hipGetLastError());
+    if (timed) HIP_TRY(hipEventRecord(g->ev_k1[k], st));
+    HIP_TRY(hipEventRecord(g->ev_join[k], st));
+  }
+  return RK_OK;
+}
+
+static int group_join_traj(rk_group* g, const GroupLaunch& gl) {
+  for (size_t c = 0; c < gl.t_na.size(); ++c)
+    HIP_TRY(hipStreamWaitEvent(g->ctx->stream, g->ev_join[gl.var.size() + c], 0));
+  return RK_OK;
+}
+
 static int group_launch(rk_group* g, const GroupLaunch& gl, int iters, const LMParams& lp, bool timed) {
   rk_context* ctx = g->ctx;
   const int chunk = g->iters_per_launch > 0 ? g->iters_per_launch : std::max(1, iters);
   HIP_TRY(hipEventRecord(g->ev_fork, ctx->stream));
+  int rc;
+  if ((rc = group_launch_traj(g, gl, iters, lp, timed))) return rc;  // the trajectories start first
   // the variant with the most tiles is enqueued first: its workgroups reach the CUs first, and the
   // few wavefronts of the others fill in around them (about 1 % faster on the uneven benchmark
   // than the thread-per-problem kernels first)
@@ -3274,7 +3370,7 @@ static int group_launch(rk_group* g, const GroupLaunch& gl, int iters, const LMP
       if (iters == 0) break;
     }
     HIP_TRY(hipGetLastError());
-    return RK_OK;
+    return group_join_traj(g, gl);
   }
   for (size_t kk = 0; kk < gl.var.size(); ++kk) {
     const size_t k = order[kk];
@@ -3294,7 +3390,7 @@ static int group_launch(rk_group* g, const GroupLaunch& gl, int iters, const LMP
     }
   }
   HIP_TRY(hipGetLastError());
-  return RK_OK;
+  return group_join_traj(g, gl);
 }
 
 extern "C" {
@@ -3324,7 +3420,7 @@ int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t fl
         g->graph = nullptr;
       }
       GroupLaunch gl;
-      if ((rc = group_prepare(g, &gl, from_seed, g->buckets_graph))) return rc;
+      if ((rc = group_prepare(g, &gl, from_seed, g->buckets_graph, g->traj_graph))) return rc;
       hipGraph_t graph = nullptr;
       HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       rc = group_launch(g, gl, iters, lp, false);
@@ -3346,11 +3442,15 @@ int rk_group_solve(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t fl
     g->last_mask.clear();
   } else {
     GroupLaunch gl;
-    if ((rc = group_prepare(g, &gl, from_seed, g->buckets_dev))) return rc;
+    if ((rc = group_prepare(g, &gl, from_seed, g->buckets_dev, g->traj_dev))) return rc;
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     if ((rc = group_launch(g, gl, iters, lp, true))) return rc;
     g->last_var = gl.var;
     g->last_mask = gl.mask;
+    for (size_t c = 0; c < gl.t_na.size(); ++c) {
+      g->last_var.push_back(RK_VAR_TRAJ + (int)c);
+      g->last_mask.push_back(gl.t_mask[c]);
+    }
   }
   if (flags & RK_ASYNC) return RK_OK;  // enqueued; the caller synchronises
   HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
@@ -3364,10 +3464,12 @@ int rk_group_kernel_times(rk_group* g, int32_t cap, int32_t* tpp, float* ms, int
   const int m = (int)g->last_var.size();
   *n = m;
   for (int k = 0; k < m && k < cap; ++k) {
+    const int v = g->last_var[k];
     if (tpp)
-      tpp[k] = variant_wave(g->last_var[k])   ? RK_TPP_WAVE
-               : variant_half(g->last_var[k]) ? RK_TPP_HALF
-                                              : variant_G(g->last_var[k]);
+      tpp[k] = v >= RK_VAR_TRAJ ? RK_TPP_TRAJ
+               : variant_wave(v) ? RK_TPP_WAVE
+               : variant_half(v) ? RK_TPP_HALF
+                                 : variant_G(v);
     if (ms) HIP_TRY(hipEventElapsedTime(&ms[k], g->ev_k0[k], g->ev_k1[k]));
     if (batch_mask) batch_mask[k] = (int32_t)g->last_mask[k];
   }
@@ -3443,6 +3545,10 @@ int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t
   // each batch alone, each legal G; state is restored after every trial
   for (size_t i = 0; i < g->batches.size(); ++i) {
     rk_batch* b = g->batches[i];
+    if (b->traj_T) {  // trajectories have one kernel form
+      if (chosen) chosen[i] = RK_TPP_TRAJ;
+      continue;
+    }
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
     int best_g = 0;

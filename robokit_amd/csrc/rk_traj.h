@@ -874,6 +874,38 @@ __global__ __launch_bounds__(128) void k_traj(Problems P, const TrajTerms* tt, L
     traj_worker<NA, NJ, LIM>(P, as_const(tt), blockIdx.x, it0, iters, &sh);
 }
 
+// Trajectory batches of one arm class (dofs, bound rows / geometries) in a group: consecutive
+// workgroup ranges serve the batches of a table, so trajectories of uneven horizons (and scenes)
+// share one launch.  Workgroup b solves problem b - block0 of the bucket whose range holds it.
+struct TrajBucket {
+  Problems P;
+  const TrajTerms* tt;
+  int block0;
+  int pad;
+};
+typedef const __attribute__((address_space(4))) TrajBucket TrajBucketK;
+
+template <int NA, int NJ, bool LIM = false>
+__global__ __launch_bounds__(128) void k_traj_multi(const TrajBucket* table_, int nb, LMParams lp, int it0, int iters) {
+  __shared__ __align__(16) TrajLDS<NA> sh;
+  TrajBucketK* table = (TrajBucketK*)table_;
+  int k = 0;
+  while (k + 1 < nb && (int)blockIdx.x >= table[k + 1].block0) ++k;  // wave-uniform
+  const __attribute__((address_space(4))) Problems& Q = table[k].P;  // read field by field (scalar loads)
+  Problems P;
+  P.robot = Q.robot; P.terms = Q.terms; P.q = Q.q; P.q_seed = Q.q_seed; P.pose_tgt = Q.pose_tgt;
+  P.pos_tgt = Q.pos_tgt; P.rot_tgt = Q.rot_tgt; P.q_prev = Q.q_prev; P.cost = Q.cost; P.lam = Q.lam;
+  P.nacc = Q.nacc; P.trace = Q.trace; P.acc_trace = Q.acc_trace; P.q_trace = Q.q_trace; P.lam_trace = Q.lam_trace;
+  P.trace_iters = Q.trace_iters; P.n = Q.n; P.stride = Q.stride;
+  const int s = (int)blockIdx.x - table[k].block0;
+  if (s >= P.n) return;
+  TrajK* tt = as_const(table[k].tt);
+  if (threadIdx.x < 64)
+    traj_solve<NA, NJ, LIM>(P, tt, s, lp, it0, iters, &sh);
+  else
+    traj_worker<NA, NJ, LIM>(P, tt, s, it0, iters, &sh);
+}
+
 // goal errors at the last frame and the smallest sphere-to-scene distance (one wave per problem)
 template <int NA, int NJ>
 __global__ __launch_bounds__(64) void k_traj_metrics(Problems P, const TrajTerms* ttg, float* pos_err, float* ang_err,

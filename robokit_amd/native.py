@@ -21,6 +21,7 @@ RK_OK, RK_EINVAL, RK_EHIP, RK_ECAPACITY, RK_ENOMEM, RK_EBUSY = range(6)
 MODE_FUSED, MODE_PLAIN = 0, 1
 TPP_WAVE = -64  # one wavefront per problem, register rows
 TPP_HALF = -32  # two problems per wavefront (32 lanes each), register rows
+TPP_TRAJ = -128  # a trajectory: two wavefronts per problem (group kernel times)
 FROM_SEED, CONTINUE, USE_GRAPH, ASYNC, NOWAIT = 0x100, 0x200, 0x1, 0x2, 0x400
 SE3_OPS = {"compose": (0, 7), "inverse": (1, 7), "log": (2, 6), "jlog": (3, 36), "adjoint": (4, 36), "apply": (5, 3)}
 
@@ -532,7 +533,8 @@ class Batch:
 
 
 class Group:
-    """Uneven batch: several Batches solved by one persistent kernel (rk_group)."""
+    """Uneven batch: several Batches solved by one persistent kernel (rk_group); TrajBatches of any
+    horizons join it, one launch per arm class beside the IK kernel."""
 
     def __init__(self, ctx: Context, batches: Sequence[Batch]):
         self.ctx, self.batches = ctx, list(batches)
@@ -574,7 +576,8 @@ class Group:
         return float(ms.value) if sync else 0.0
 
     def kernel_times(self) -> List[tuple]:
-        """[(threads-per-problem code, ms, batch indices)] of each persistent kernel of the last solve."""
+        """[(threads-per-problem code, ms, batch indices)] of each kernel of the last eager solve
+        (TPP_TRAJ for a trajectory class)."""
         cap = 16
         tpp = np.zeros(cap, np.int32)
         ms = np.zeros(cap, np.float32)

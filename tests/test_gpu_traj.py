@@ -210,3 +210,53 @@ def test_traj_cg_solves_the_workload(ctx):
     print(f"\nCG: min clearance median {np.median(sd0):.3f} -> {np.median(sd):.3f} m, goal error median "
           f"{np.median(pe0):.3f} -> {np.median(pe):.4f} m")
     assert np.median(sd) > np.median(sd0) and np.median(pe) < np.median(pe0)
+
+
+def test_group_packs_uneven_horizons(ctx):
+    """an uneven group of IK batches (6 / 7 / 29 dofs) and trajectory batches of horizons 16, 32 and
+    64 (6- and 7-dof arms, with and without bound rows): every batch's result equals its solve
+    alone, bitwise, eagerly and from two replays of the captured graph; the kernel times list one
+    trajectory launch per arm class with its batches"""
+    from robokit_amd.native import TPP_TRAJ
+
+    iters = 15
+    mb = W.config1(seed=3, total=300)
+    ik, ik_single = [], []
+    for bk in mb.buckets:
+        ts = N.TermSpec(pose_links=bk.links, pose_weights=np.ones((len(bk.links), 6), np.float32),
+                        limit_weights=np.ones(bk.spec.num_actuated_joints, np.float32))
+        b = N.Batch(ctx, bk.spec, ts, bk.size, trace_iters=iters)
+        b.set(q=bk.q_seed, pose_targets=bk.targets)
+        b.solve(iters, P)
+        ik_single.append(b.get(traces=True))
+        b.set(q=bk.q_seed, pose_targets=bk.targets)
+        ik.append(b)
+    tbs = [W.config3(seed=31, batch=8, T=16), W.config3(seed=32, batch=4, T=64),
+           W.config3(seed=33, batch=6, T=32, arm="ur10"), _bounded(W.config3(seed=34, batch=5, T=16))]
+    tj, tj_single = [], []
+    for tb in tbs:
+        b = N.TrajBatch(ctx, tb, tb.size, trace_iters=iters)
+        b.set(q=tb.q_seed, goals=tb.goals)
+        b.solve(iters, P)
+        tj_single.append(b.get(traces=True))
+        b.set(q=tb.q_seed, goals=tb.goals)
+        tj.append(b)
+    g = N.Group(ctx, ik + tj)
+    print(f"\n[group] {len(ik)} IK batches + trajectories T = {[tb.T for tb in tbs]}", flush=True)
+
+    def check():
+        for b, ref in zip(ik + tj, ik_single + tj_single):
+            out = b.get(traces=True)
+            assert np.array_equal(out["q"], ref["q"])
+            assert np.array_equal(out["cost_trace"], ref["cost_trace"])
+
+    g.solve(iters, P, use_graph=False, from_seed=True)
+    check()
+    kt = g.kernel_times()
+    traj = [k for k in kt if k[0] == TPP_TRAJ]
+    # classes: 6 dofs (the UR10), 7 dofs, 7 dofs with bound rows
+    assert sorted(sorted(k[2]) for k in traj) == [[3, 4], [5], [6]], kt
+    assert all(k[1] > 0 for k in kt)
+    for _ in range(2):
+        g.solve(iters, P, use_graph=True, from_seed=True)
+        check()
