@@ -387,19 +387,16 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
     }
     cost = chain_cost<NA, PR>(tm, r_pose, r_lim, r_sm);
     if (CO) cost = cost + 0.5f * cc;
-    // ---- damped Cholesky (left-looking) and solves.  Each pivot's square root and reciprocal
-    // are the hardware's (v_sqrt_f32, v_rcp_f32, 1 ulp) and the column scales by the reciprocal:
-    // a correctly rounded division is a dependent chain of ~10 instructions, and this kernel is
-    // bound by dependency latency.
+    // ---- damped Cholesky (left-looking) and solves.  Each pivot's reciprocal square root is the
+    // hardware's (v_rsq_f32, 1 ulp) and the column scales by it: a correctly rounded division is a
+    // dependent chain of ~10 instructions, and this kernel is bound by dependency latency.
     float dinv[NA];
 #pragma unroll
     for (int j = 0; j < NA; ++j) {
       float sd = H[j][j] + lam;
 #pragma unroll
       for (int k = 0; k < j; ++k) sd -= H[j][k] * H[j][k];
-      const float d = __builtin_amdgcn_sqrtf(sd);
-      dinv[j] = __builtin_amdgcn_rcpf(d);
-      H[j][j] = d;
+      dinv[j] = __builtin_amdgcn_rsqf(sd);  // the pivot itself is never read: only 1 / L_jj
 #pragma unroll
       for (int i = j + 1; i < NA; ++i) {
         float v = H[i][j];

@@ -508,8 +508,8 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     // (the block form of the column-by-column recurrence: one LDS round trip per four columns).
     // Two buffers: step s + 1 writes the other one, and its wait orders step s's reads before
     // step s + 2 overwrites theirs.  Padding rows k >= na stage zeros with pivot lam: they keep
-    // y = 0, give dq = 0 and change no other row.  Square roots and reciprocals of the pivots are
-    // the hardware's (v_sqrt_f32, v_rcp_f32, 1 ulp).
+    // y = 0, give dq = 0 and change no other row.  The pivots' reciprocal square roots are the
+    // hardware's (v_rsq_f32, 1 ulp).
     float b = -gv, y = 0.f, dinv = 1.f;
     const bool stage = HW == 32 || lane < WV_NA;
     float dsum = lam;  // this row's diagonal terms: limit, smoothness, then lam
@@ -535,14 +535,14 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
                    d3 = sh->blk[buf][k0 + 3];
       const float4 bB = *reinterpret_cast<const float4*>(&sh->rhs[buf][k0]);
       // L_D (lower) and its reciprocal diagonal
-      const float l00 = __builtin_amdgcn_sqrtf(d0.x), i00 = __builtin_amdgcn_rcpf(l00);
+      // (only the reciprocal square roots of the pivots are needed: one v_rsq each)
+      const float i00 = __builtin_amdgcn_rsqf(d0.x);
       const float l10 = d1.x * i00, l20 = d2.x * i00, l30 = d3.x * i00;
-      const float l11 = __builtin_amdgcn_sqrtf(d1.y - l10 * l10), i11 = __builtin_amdgcn_rcpf(l11);
+      const float i11 = __builtin_amdgcn_rsqf(d1.y - l10 * l10);
       const float l21 = (d2.y - l20 * l10) * i11, l31 = (d3.y - l30 * l10) * i11;
-      const float l22 = __builtin_amdgcn_sqrtf(d2.z - l20 * l20 - l21 * l21), i22 = __builtin_amdgcn_rcpf(l22);
+      const float i22 = __builtin_amdgcn_rsqf(d2.z - l20 * l20 - l21 * l21);
       const float l32 = (d3.z - l30 * l20 - l31 * l21) * i22;
-      const float l33 = __builtin_amdgcn_sqrtf(d3.w - l30 * l30 - l31 * l31 - l32 * l32),
-                  i33 = __builtin_amdgcn_rcpf(l33);
+      const float i33 = __builtin_amdgcn_rsqf(d3.w - l30 * l30 - l31 * l31 - l32 * l32);
       // y_B
       const float y0 = bB.x * i00;
       const float y1 = (bB.y - l10 * y0) * i11;
