@@ -14,12 +14,14 @@ Mirrors the reference's IK helper (helpers/ik/ik.py, helpers/ik/config.py, helpe
                       position / rotation error and success at 5 mm / 0.05 rad by default
 
 Term templates (robokit_amd.lm FrameTask / PositionTask / RotationTask / PositionLimit /
-SmoothnessTask built without a robot) take IKContext.TARGET_LINKS / TARGET_LINK as frames,
-IKContext.TARGET as targets and IKContext.PREVIOUS_STATE as the smoothness reference.  A config
-without terms solves FrameTask(log) on every target link (``position_weight`` /
-``orientation_weight``) plus PositionLimit(``limit_weight``).  Every solve, score and select runs on
-the GPU through librkhip.so.  Score terms, rest_state pinning, scenes, the floating base and the
-torch adapter are not implemented (ValueError / NotImplementedError).
+SmoothnessTask / SceneCollisionTask built without a robot) take IKContext.TARGET_LINKS /
+TARGET_LINK as frames, IKContext.TARGET as targets, IKContext.PREVIOUS_STATE or IKContext.REST_Q
+(the rest configuration: zero_q until solve_numpy(rest_q=...)) as the smoothness reference, and
+IKContext.SCENE (IKSolver(scene=...)) as the collision scene.  A config without terms solves
+FrameTask(log) on every target link (``position_weight`` / ``orientation_weight``) plus
+PositionLimit(``limit_weight``).  Every solve, score and select runs on the GPU through
+librkhip.so.  Score terms, the floating base and the torch adapter are not implemented
+(NotImplementedError).
 """
 from __future__ import annotations
 
@@ -119,9 +121,15 @@ class IKSolver:
     solve (buffers are sized then, as the reference's)."""
 
     def __init__(self, config: IKSolverConfig, spec, target_links: Union[int, Sequence[int]],
-                 ctx: Optional[N.Context] = None):
+                 ctx: Optional[N.Context] = None, scene: Optional[L.PrimitiveScene] = None):
+        """``spec``: a RobotSpec, or an lm.Robot (whose collision spheres SceneCollisionTask terms
+        use); ``scene``: the collision scene IKContext.SCENE stands for (IKSolver(scene=...),
+        helpers/ik/ik.py:101-111)"""
         self.config = config
+        self.robot = spec if isinstance(spec, L.Robot) else L.Robot(spec)
+        spec = self.robot.spec
         self.spec = spec
+        self.scene = scene
         self.target_links = [int(target_links)] if np.isscalar(target_links) else [int(x) for x in target_links]
         if not self.target_links:
             raise ValueError("at least one target link")
@@ -145,20 +153,29 @@ class IKSolver:
             L.FrameTask(frame_index=IKContext.TARGET_LINKS, T_world_target=IKContext.TARGET,
                         position_weight=self.config.position_weight, orientation_weight=self.config.orientation_weight),
             L.PositionLimit(weight=self.config.limit_weight)]
-        robot = L.Robot(self.spec)
+        robot = self.robot
         ctxv = {IKContext.TARGET_LINKS: list(self.target_links), IKContext.TARGET_LINK: self.target_links[0],
                 IKContext.TARGET: np.tile(np.array([0, 0, 0, 1, 0, 0, 0], np.float32), (1, k, 1)),
-                IKContext.PREVIOUS_STATE: L.RobotState(robot, self.spec.zero_q)}
+                IKContext.PREVIOUS_STATE: L.RobotState(robot, self.spec.zero_q),
+                IKContext.REST_Q: L.RobotState(robot, self.spec.zero_q), IKContext.SCENE: self.scene}
         self._sel = {"pose": [], "pos": [], "rot": []}  # target-list index of each row block
         w = {"pose": [], "pos": [], "rot": []}
-        limit = smooth = None
+        limit = smooth = coll = None
         self.needs_prev = False
+        self.smooth_ref = None  # the smoothness reference: "prev" (PREVIOUS_STATE) or "rest" (REST_Q)
         for t in templates:
             tmpl = t._template or {}
             if isinstance(t, (L.FrameTask, L.PositionTask, L.RotationTask)) and tmpl.get("T_world_target") is not IKContext.TARGET:
                 raise NotImplementedError("IK term targets must be IKContext.TARGET on this path")
-            if isinstance(t, L.SmoothnessTask) and tmpl.get("prev_var") is not IKContext.PREVIOUS_STATE:
-                raise NotImplementedError("the IK smoothness reference must be IKContext.PREVIOUS_STATE")
+            if isinstance(t, L.SmoothnessTask):
+                ref = tmpl.get("prev_var")
+                if ref not in (IKContext.PREVIOUS_STATE, IKContext.REST_Q):
+                    raise NotImplementedError("the IK smoothness reference must be IKContext.PREVIOUS_STATE or REST_Q")
+                if self.smooth_ref is not None:
+                    raise NotImplementedError("one smoothness reference per IK configuration on this path")
+                self.smooth_ref = "prev" if ref is IKContext.PREVIOUS_STATE else "rest"
+            if isinstance(t, L.SceneCollisionTask) and tmpl.get("scene") is IKContext.SCENE and self.scene is None:
+                raise ValueError("a SceneCollisionTask on IKContext.SCENE needs IKSolver(scene=...)")
             r = t.resolve(robot, ctxv)
             key = {L.FrameTask: "pose", L.PositionTask: "pos", L.RotationTask: "rot"}.get(type(r))
             if key:
@@ -170,7 +187,12 @@ class IKSolver:
             elif isinstance(r, L.PositionLimit):
                 limit = r.weights
             elif isinstance(r, L.SmoothnessTask):
-                smooth, self.needs_prev = r.weights, True
+                smooth = r.weights
+                self.needs_prev = self.smooth_ref == "prev"
+            elif isinstance(r, L.SceneCollisionTask):
+                if coll is not None:
+                    raise NotImplementedError("one SceneCollisionTask per IK configuration on this path")
+                coll = r
             else:
                 raise NotImplementedError(f"IK term {type(r).__name__} is not on this path")
         cat = lambda key: np.concatenate(w[key]) if w[key] else None  # noqa: E731
@@ -178,6 +200,11 @@ class IKSolver:
                                 pos_links=[self.target_links[i] for i in self._sel["pos"]], pos_weights=cat("pos"),
                                 rot_links=[self.target_links[i] for i in self._sel["rot"]], rot_weights=cat("rot"),
                                 limit_weights=limit, smooth_weights=smooth)
+        if coll is not None:  # SceneCollisionTask rows (the robot's spheres against the scene)
+            ts = self.terms
+            ts.coll_links, ts.coll_centers, ts.coll_radii = list(coll.sphere_links), coll.sphere_centers, coll.sphere_radii
+            ts.coll_weights, ts.coll_margin, ts.coll_penalty = coll.weights, coll.margin, coll.penalty
+            ts.obstacle_spheres, ts.obstacle_boxes, ts.geoms = coll.scene.spheres, coll.scene.boxes, list(coll.scene.geoms)
         self._metric_terms = N.TermSpec(pose_links=self.target_links, pose_weights=np.ones((k, 6), np.float32))
 
     def term_targets(self, tg: np.ndarray) -> dict:
@@ -211,8 +238,9 @@ class IKSolver:
             raise ValueError(f"Batch size mismatch: expected {self._pop.batch_size}, got {batch_size}")
 
     def solve_numpy(self, T_world_target, prev_q: Optional[np.ndarray] = None, init_q: Optional[np.ndarray] = None,
-                    init_sample_range: Optional[float] = None) -> np.ndarray:
-        """q [batch, na] of each target's winner"""
+                    init_sample_range: Optional[float] = None, rest_q: Optional[np.ndarray] = None) -> np.ndarray:
+        """q [batch, na] of each target's winner.  rest_q [batch, na] or [na]: the rest configuration
+        IKContext.REST_Q stands for (default: the robot's zero configuration, helpers/ik/ik.py:576)"""
         tg = self._targets(T_world_target)
         B = tg.shape[0]
         self._build(B)
@@ -225,14 +253,19 @@ class IKSolver:
             raise ValueError("this configuration's SmoothnessTask needs prev_q (IKContext.PREVIOUS_STATE)")
         pq = None if prev_q is None else np.asarray(prev_q, np.float32).reshape(B, na)
         scale = init_sample_range if init_sample_range is not None else self.config.sampler.scale
-        self._pop.set_targets(**self.term_targets(tg), q_prev=pq if self.needs_prev else None, init_q=init_q,
-                              init_scale=scale)
+        ref = None  # the smoothness term's reference rows
+        if self.smooth_ref == "prev":
+            ref = pq
+        elif self.smooth_ref == "rest":
+            rq = self.spec.zero_q if rest_q is None else rest_q
+            ref = np.ascontiguousarray(np.broadcast_to(np.asarray(rq, np.float32).reshape(-1, na), (B, na)))
+        self._pop.set_targets(**self.term_targets(tg), q_prev=ref, init_q=init_q, init_scale=scale)
         win = self._pop.solve()
         out = win.get()
         q, cost = out["q"], out["cost"]
         self.last_cost, self.last_prev_cost = cost, None
         if pq is not None:  # score prev with the solver's ranking rule; keep it on a tie
-            self._score.set(q=pq, q_prev=pq if self.needs_prev else None, **self.term_targets(tg))
+            self._score.set(q=pq, q_prev=ref, **self.term_targets(tg))
             self._score.solve(0, self.config.solver.optimizer[-1], from_seed=True)
             pc = self._score.get()["cost"]
             keep = pc <= cost

@@ -184,3 +184,59 @@ def test_presets_smooth_needs_and_uses_previous_state(ctx):
     c_o = O.lm_solve(spec, terms, q, 0, O.LMParams(), traces=False)["cost"]
     best = np.minimum(solver.last_cost, solver.last_prev_cost)
     np.testing.assert_allclose(best, c_o, rtol=1e-3, atol=1e-6)
+
+
+def test_solver_scene_and_rest_state_match_oracle_population(ctx):
+    """IKSolver with the basic preset's terms plus a SceneCollisionTask on IKContext.SCENE (primitive
+    obstacles and a box mesh) and a SmoothnessTask pinning the rest configuration IKContext.REST_Q
+    (solve_numpy(rest_q=...)): the winners classify like the FP32 oracle population with the same
+    terms; the rest reference defaults to the zero configuration"""
+    from robokit_amd import geom as G
+    from robokit_amd import lm as L
+
+    tb = W.config3(seed=12, batch=4, T=5)
+    spec, sc = tb.spec, tb.scene
+    ee = spec.num_links - 1
+    wl = W.ik_batch(spec, [ee], 64, seed=47)
+    mesh = G.box_mesh([0.45, 0.0, 0.45], [0.08, 0.25, 0.05])
+    scene = L.PrimitiveScene(sc.obstacle_spheres, sc.obstacle_boxes, geoms=[mesh])
+    robot = L.Robot(spec, collision_spheres=(sc.sphere_links, sc.sphere_centers, sc.sphere_radii))
+    cfg = (IK.IKSolverConfig(sampler=IK.RobertsSamplerConfig(keep_first=True, scale=1.0))
+           .add(L.PositionTask(frame_index=IK.IKContext.TARGET_LINKS, T_world_target=IK.IKContext.TARGET, weight=20.0))
+           .add(L.RotationTask(frame_index=IK.IKContext.TARGET_LINKS, T_world_target=IK.IKContext.TARGET, weight=10.0))
+           .add(L.PositionLimit(weight=50.0))
+           .add(L.SmoothnessTask(prev_var=IK.IKContext.REST_Q, weight=0.05))
+           .add(L.SceneCollisionTask(scene=IK.IKContext.SCENE, margin=0.05)))
+    with pytest.raises(ValueError):
+        IK.IKSolver(cfg, robot, ee, ctx=ctx)  # IKContext.SCENE without a scene
+    solver = IK.IKSolver(cfg, robot, ee, ctx=ctx, scene=scene)
+    rest = np.clip(wl.q_target + 0.3, spec.actuated_joint_limits_f32[:, 0], spec.actuated_joint_limits_f32[:, 1])
+    print(f"\n[ik] scene + rest state, panda {wl.size} targets", flush=True)
+    q = solver.solve_numpy(wl.targets, rest_q=rest)
+    na = spec.num_actuated_joints
+    coll = O.CollisionScene(sc.sphere_links, sc.sphere_centers, sc.sphere_radii, np.ones(len(sc.sphere_links)),
+                            margin=0.05, penalty="smooth", obstacle_spheres=sc.obstacle_spheres,
+                            obstacle_boxes=sc.obstacle_boxes, geoms=[mesh])
+
+    def fn(inst):
+        return _preset_terms([ee], wl.targets[inst], na) + [
+            O.Term(O.SMOOTH, (), None, np.full(na, 0.05, np.float32), rest[inst]),
+            O.Term(O.COLLISION, (), None, None, None, coll)]
+
+    stages = _stages(solver.config.solver)
+    q32, _, _ = OP.population_ik(spec, [ee], wl.targets, solver._pop.seeds, stages, terms_fn=fn, threads=8)
+    q64, _, _ = OP.population_ik(spec, [ee], wl.targets, solver._pop.seeds, stages, dtype="f64", terms_fn=fn,
+                                 threads=8)
+    thr = dict(pos_thr=0.005, ang_thr=0.05)
+    ok = O.ik_metrics(spec, q, [ee], wl.targets, **thr)[2]
+    ok32 = O.ik_metrics(spec, q32, [ee], wl.targets, **thr)[2]
+    ok64 = O.ik_metrics(spec, q64, [ee], wl.targets, dtype="f64", **thr)[2]
+    mism, n_ref = int(np.sum(ok != ok32)), int(np.sum(ok64 != ok32))
+    print(f"  converged (5 mm) gpu {ok.mean():.3f} f32 {ok32.mean():.3f} f64 {ok64.mean():.3f}; "
+          f"differ gpu/f32 {mism}, f32/f64 {n_ref}")
+    assert mism <= max(n_ref, 2)
+    assert ok32.sum() > 0.3 * wl.size
+    # the default rest reference is the zero configuration
+    q0 = solver.solve_numpy(wl.targets)
+    q0b = solver.solve_numpy(wl.targets, rest_q=np.zeros(na, np.float32))
+    np.testing.assert_array_equal(q0, q0b)
