@@ -245,6 +245,16 @@ int rk_batch_set_rotations(rk_batch* b, int32_t B, const float* rot_targets);
  * its next solve starts afresh (lambda = lm_lambda).  1 <= width_out <= width_in <= 64.
  * src_rows [n_inst * width_out] (nullable, host): the source row of each dst row. */
 int rk_batch_select(rk_batch* src, int32_t width_in, rk_batch* dst, int32_t width_out, int32_t* src_rows);
+/* the same select ranking by the costs of `scores` (a batch of src's size whose cost holds each
+ * candidate's score: a zero-iteration solve of the score terms at src's iterates) instead of src's
+ * own costs; dst's costs receive the kept rows' scores -- IKSolverConfig.score_terms
+ * (helpers/ik/config.py:57-59), ranked at every select (PopulationSolver._score / _select,
+ * opt/population_solver.py:395-449) */
+int rk_batch_select_scored(rk_batch* src, const rk_batch* scores, int32_t width_in, rk_batch* dst, int32_t width_out,
+                           int32_t* src_rows);
+/* copies the live problems' q of src into dst (same robot and dofs) and sets dst's size to src's:
+ * a score batch evaluates another batch's iterates */
+int rk_batch_copy_q(rk_batch* src, rk_batch* dst);
 /* warm start for a sequence of frames (helpers/retarget: the previous frame's solution is the
  * next frame's seed and its SmoothnessTask reference, terms/dense/smoothness_task.py:350-365):
  * on the device, q_prev <- q and seeds <- q; the next rk_batch_set then only brings new targets */

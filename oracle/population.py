@@ -8,8 +8,8 @@ product package (robokit_amd) never imports it.
                        NaN as +inf, equal costs in candidate order), restated as a stable sort
   population_ik        opt/population_solver.py:291-395 for the IK schedule of
                        helpers/ik/config.py:37-45: each segment a fresh LM (oracle.lm_solve) on
-                       the kept candidates, then a select at every stage boundary (an equal width
-                       reorders best first, :365-375); the last select keeps one winner
+                       the kept candidates, then a select where the width narrows (an equal
+                       width keeps the population, :436-443); the last select keeps one winner
 
 Pins (tests/test_population.py): the R_d closed forms (golden ratio for d = 1, plastic number for
 d = 2), the reference test's coverage property (tests/opt/test_sampler.py:57-61), the ranking
@@ -47,12 +47,15 @@ def select_best_k(costs: np.ndarray, width_in: int, width_out: int) -> np.ndarra
 
 
 def population_ik(spec, links: Sequence[int], targets: np.ndarray, seeds: np.ndarray,
-                  stages: List[Tuple[int, int, O.LMParams]], dtype: str = "f32", terms_fn=None, threads: int = 1):
+                  stages: List[Tuple[int, int, O.LMParams]], dtype: str = "f32", terms_fn=None, threads: int = 1,
+                  score_fn=None):
     """targets [n_inst, n_pose, 7]; seeds [width0, na] shared by every problem, or
     [n_inst * width0, na] per problem; stages [(iters, width, params)].  terms_fn(inst) gives the
     oracle terms of the candidate rows whose problems are ``inst`` (default: FrameTask on every
-    link + PositionLimit, oracle.ik_terms).  Returns (q [n_inst, na], cost [n_inst], rows of every
-    select)."""
+    link + PositionLimit, oracle.ik_terms).  score_fn(inst): score terms whose cost at the
+    candidates ranks every select instead of the optimization cost (population_solver.py:415-434);
+    the returned costs are then the winners' scores.  Returns (q [n_inst, na], cost [n_inst], rows of
+    every select)."""
     if terms_fn is None:
         terms_fn = lambda inst: O.ik_terms(links, targets[inst], seeds.shape[1])  # noqa: E731
     n_inst, na = targets.shape[0], seeds.shape[1]
@@ -62,16 +65,26 @@ def population_ik(spec, links: Sequence[int], targets: np.ndarray, seeds: np.nda
     q = (np.tile(seeds, (n_inst, 1)) if seeds.shape[0] == w else np.asarray(seeds)).astype(rt)
     cost = None
     selects = []
+    def rank(q, cost, inst):
+        if score_fn is None:
+            return cost
+        return O.lm_solve(spec, score_fn(inst), q, 0, O.LMParams(), dtype=dtype, traces=False,
+                          threads=threads)["cost"].astype(rt)
+
     for k, (iters, width, params) in enumerate(stages):
-        if k > 0 and (width < w or w > 1):  # every stage boundary selects, equal widths reorder
-            rows = select_best_k(cost, w, width).reshape(-1)
+        if k > 0 and width < w:  # an equal width keeps the population (population_solver.py:442-443)
+            sc = rank(q, cost, inst)
+            rows = select_best_k(sc, w, width).reshape(-1)
             selects.append(rows)
-            q, cost, inst, w = q[rows], cost[rows], inst[rows], width
+            q, cost, inst, w = q[rows], sc[rows], inst[rows], width
         if iters > 0 or k == 0:
             res = O.lm_solve(spec, terms_fn(inst), q, iters, params, dtype=dtype, traces=False, threads=threads)
             q, cost = res["q"].astype(rt), res["cost"].astype(rt)
     if w > 1:
-        rows = select_best_k(cost, w, 1).reshape(-1)
+        sc = rank(q, cost, inst)
+        rows = select_best_k(sc, w, 1).reshape(-1)
         selects.append(rows)
-        q, cost = q[rows], cost[rows]
+        q, cost, inst = q[rows], sc[rows], inst[rows]
+    elif score_fn is not None:
+        cost = rank(q, cost, inst)
     return q, cost, selects

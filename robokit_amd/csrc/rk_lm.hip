@@ -2825,7 +2825,42 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
   return RK_OK;
 }
 
+static int batch_select(rk_batch* src, const rk_batch* scores, int32_t width_in, rk_batch* dst, int32_t width_out,
+                        int32_t* src_rows);
+
 int rk_batch_select(rk_batch* src, int32_t width_in, rk_batch* dst, int32_t width_out, int32_t* src_rows) {
+  return batch_select(src, src, width_in, dst, width_out, src_rows);
+}
+
+int rk_batch_select_scored(rk_batch* src, const rk_batch* scores, int32_t width_in, rk_batch* dst, int32_t width_out,
+                           int32_t* src_rows) {
+  if (!scores) return fail(RK_EINVAL, "null score batch");
+  if (scores == dst) return fail(RK_EINVAL, "the score batch cannot be the target");
+  if (scores->ctx != src->ctx || scores->n != src->n || scores->traj_T)
+    return fail(RK_EINVAL, "the score batch must hold one score per source row (same context and size)");
+  return batch_select(src, scores, width_in, dst, width_out, src_rows);
+}
+
+int rk_batch_copy_q(rk_batch* src, rk_batch* dst) {
+  if (!src || !dst || src == dst) return fail(RK_EINVAL, "two distinct batches needed");
+  if (src->ctx != dst->ctx || src->na != dst->na || src->traj_T != dst->traj_T)
+    return fail(RK_EINVAL, "batches of different contexts or shapes");
+  if (src->n > dst->capacity)
+    return fail(RK_ECAPACITY, "batch size " + std::to_string(src->n) + " exceeds the " + std::to_string(dst->capacity) +
+                                  " this batch was built for");
+  rk_context* ctx = src->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (src->n > 0)  // SoA: na rows of n floats, each batch with its own row pitch (its capacity)
+    HIP_TRY(hipMemcpy2DAsync(dst->q, sizeof(float) * dst->capacity, src->q, sizeof(float) * src->capacity,
+                             sizeof(float) * src->n, src->na, hipMemcpyDeviceToDevice, ctx->stream));
+  dst->n = src->n;
+  dst->it_done = 0;
+  return RK_OK;
+}
+
+static int batch_select(rk_batch* src, const rk_batch* scores, int32_t width_in, rk_batch* dst, int32_t width_out,
+                        int32_t* src_rows) {
   if (!src || !dst) return fail(RK_EINVAL, "null batch");
   if (src == dst) return fail(RK_EINVAL, "select needs two batches (the source is read while the target is written)");
   if (src->ctx != dst->ctx) return fail(RK_EINVAL, "batches of different contexts");
@@ -2847,7 +2882,7 @@ int rk_batch_select(rk_batch* src, int32_t width_in, rk_batch* dst, int32_t widt
   HIP_TRY(hipSetDevice(ctx->device));
   int rc;
   if (!dst->sel && (rc = dmalloc(ctx, (void**)&dst->sel, sizeof(int) * dst->capacity, &dst->bytes))) return rc;
-  SelectArgs sa{src->cost,     src->q,         src->pose_tgt, src->pos_tgt, src->rot_tgt, src->q_prev,
+  SelectArgs sa{scores->cost,  src->q,         src->pose_tgt, src->pos_tgt, src->rot_tgt, src->q_prev,
                 dst->q,        dst->q_seed,    dst->pose_tgt, dst->pos_tgt, dst->rot_tgt, dst->q_prev,
                 dst->cost,     dst->sel,      src->capacity,  dst->capacity, width_in,     width_out,    src->na,
                 a.n_pose * 7,  a.n_pos * 3,    a.n_rot * 4};

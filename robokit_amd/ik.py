@@ -20,8 +20,9 @@ TARGET_LINK as frames, IKContext.TARGET as targets, IKContext.PREVIOUS_STATE or 
 IKContext.SCENE (IKSolver(scene=...)) as the collision scene.  A config without terms solves
 FrameTask(log) on every target link (``position_weight`` / ``orientation_weight``) plus
 PositionLimit(``limit_weight``).  Every solve, score and select runs on the GPU through
-librkhip.so.  Score terms, the floating base and the torch adapter are not implemented
-(NotImplementedError).
+librkhip.so.  Score terms (IKSolverConfig.add_score) rank the candidates at every select and
+the previous state by their own cost.  The floating base and several solutions per target are not
+implemented (NotImplementedError).
 """
 from __future__ import annotations
 
@@ -74,8 +75,6 @@ class IKSolverConfig:
     def __post_init__(self):
         if self.enable_T_world_base:
             raise NotImplementedError("the floating base is not implemented on this path")
-        if self.score_terms:
-            raise NotImplementedError("score terms are not implemented on this path (candidates rank by cost)")
 
     def add(self, term: L.Task, *, name: Optional[str] = None) -> "IKSolverConfig":
         """append a term template (helpers/ik/config.py:73-76)"""
@@ -87,10 +86,17 @@ class IKSolverConfig:
         return self
 
     def add_score(self, term: L.Task, *, name: Optional[str] = None) -> "IKSolverConfig":
-        raise NotImplementedError("score terms are not implemented on this path (candidates rank by cost)")
+        """append a score term template: the score terms' cost ranks the candidates at every select
+        instead of the optimization cost (helpers/ik/config.py:57-59, 78-81)"""
+        if name is not None:
+            term.name = name
+            if term._template is not None:
+                term._template["name"] = name
+        self.score_terms.append(term)
+        return self
 
     def task(self, name: str) -> L.Task:
-        for t in self.terms:
+        for t in self.terms + self.score_terms:
             if t.name == name:
                 return t
         raise KeyError(name)
@@ -146,23 +152,36 @@ class IKSolver:
         self.last_prev_cost: Optional[np.ndarray] = None
 
     def _plan_terms(self) -> None:
-        """resolve the config's term templates against the robot and the target links: each term's
-        frames become indices into the target list, whose poses feed its targets at solve time"""
+        """resolve the config's term (and score term) templates against the robot and the target
+        links: each term's frames become indices into the target list, whose poses feed its
+        targets at solve time"""
         k = len(self.target_links)
         templates = self.config.terms or [
             L.FrameTask(frame_index=IKContext.TARGET_LINKS, T_world_target=IKContext.TARGET,
                         position_weight=self.config.position_weight, orientation_weight=self.config.orientation_weight),
             L.PositionLimit(weight=self.config.limit_weight)]
+        self.terms, self._sel, self.smooth_ref = self._resolve(templates)
+        self.score_terms, self._score_sel, self._score_ref = (self._resolve(self.config.score_terms)
+                                                              if self.config.score_terms else (None, None, None))
+        refs = {r for r in (self.smooth_ref, self._score_ref) if r}
+        if len(refs) > 1:
+            raise NotImplementedError("terms and score terms must share one smoothness reference on this path")
+        self.needs_prev = "prev" in refs
+        self._metric_terms = N.TermSpec(pose_links=self.target_links, pose_weights=np.ones((k, 6), np.float32))
+
+    def _resolve(self, templates):
+        """(TermSpec, target-list index of each row block, smoothness reference "prev" / "rest" /
+        None) of a list of term templates"""
+        k = len(self.target_links)
         robot = self.robot
         ctxv = {IKContext.TARGET_LINKS: list(self.target_links), IKContext.TARGET_LINK: self.target_links[0],
                 IKContext.TARGET: np.tile(np.array([0, 0, 0, 1, 0, 0, 0], np.float32), (1, k, 1)),
                 IKContext.PREVIOUS_STATE: L.RobotState(robot, self.spec.zero_q),
                 IKContext.REST_Q: L.RobotState(robot, self.spec.zero_q), IKContext.SCENE: self.scene}
-        self._sel = {"pose": [], "pos": [], "rot": []}  # target-list index of each row block
+        sel = {"pose": [], "pos": [], "rot": []}
         w = {"pose": [], "pos": [], "rot": []}
         limit = smooth = coll = None
-        self.needs_prev = False
-        self.smooth_ref = None  # the smoothness reference: "prev" (PREVIOUS_STATE) or "rest" (REST_Q)
+        smooth_ref = None
         for t in templates:
             tmpl = t._template or {}
             if isinstance(t, (L.FrameTask, L.PositionTask, L.RotationTask)) and tmpl.get("T_world_target") is not IKContext.TARGET:
@@ -171,9 +190,9 @@ class IKSolver:
                 ref = tmpl.get("prev_var")
                 if ref not in (IKContext.PREVIOUS_STATE, IKContext.REST_Q):
                     raise NotImplementedError("the IK smoothness reference must be IKContext.PREVIOUS_STATE or REST_Q")
-                if self.smooth_ref is not None:
+                if smooth_ref is not None:
                     raise NotImplementedError("one smoothness reference per IK configuration on this path")
-                self.smooth_ref = "prev" if ref is IKContext.PREVIOUS_STATE else "rest"
+                smooth_ref = "prev" if ref is IKContext.PREVIOUS_STATE else "rest"
             if isinstance(t, L.SceneCollisionTask) and tmpl.get("scene") is IKContext.SCENE and self.scene is None:
                 raise ValueError("a SceneCollisionTask on IKContext.SCENE needs IKSolver(scene=...)")
             r = t.resolve(robot, ctxv)
@@ -182,13 +201,12 @@ class IKSolver:
                 for f in r.frame_indices:
                     if f not in self.target_links:
                         raise ValueError(f"frame {f} is not a target link")
-                    self._sel[key].append(self.target_links.index(f))
+                    sel[key].append(self.target_links.index(f))
                 w[key].append(r.weights)
             elif isinstance(r, L.PositionLimit):
                 limit = r.weights
             elif isinstance(r, L.SmoothnessTask):
                 smooth = r.weights
-                self.needs_prev = self.smooth_ref == "prev"
             elif isinstance(r, L.SceneCollisionTask):
                 if coll is not None:
                     raise NotImplementedError("one SceneCollisionTask per IK configuration on this path")
@@ -196,23 +214,23 @@ class IKSolver:
             else:
                 raise NotImplementedError(f"IK term {type(r).__name__} is not on this path")
         cat = lambda key: np.concatenate(w[key]) if w[key] else None  # noqa: E731
-        self.terms = N.TermSpec(pose_links=[self.target_links[i] for i in self._sel["pose"]], pose_weights=cat("pose"),
-                                pos_links=[self.target_links[i] for i in self._sel["pos"]], pos_weights=cat("pos"),
-                                rot_links=[self.target_links[i] for i in self._sel["rot"]], rot_weights=cat("rot"),
-                                limit_weights=limit, smooth_weights=smooth)
+        ts = N.TermSpec(pose_links=[self.target_links[i] for i in sel["pose"]], pose_weights=cat("pose"),
+                        pos_links=[self.target_links[i] for i in sel["pos"]], pos_weights=cat("pos"),
+                        rot_links=[self.target_links[i] for i in sel["rot"]], rot_weights=cat("rot"),
+                        limit_weights=limit, smooth_weights=smooth)
         if coll is not None:  # SceneCollisionTask rows (the robot's spheres against the scene)
-            ts = self.terms
             ts.coll_links, ts.coll_centers, ts.coll_radii = list(coll.sphere_links), coll.sphere_centers, coll.sphere_radii
             ts.coll_weights, ts.coll_margin, ts.coll_penalty = coll.weights, coll.margin, coll.penalty
             ts.obstacle_spheres, ts.obstacle_boxes, ts.geoms = coll.scene.spheres, coll.scene.boxes, list(coll.scene.geoms)
-        self._metric_terms = N.TermSpec(pose_links=self.target_links, pose_weights=np.ones((k, 6), np.float32))
+        return ts, sel, smooth_ref
 
-    def term_targets(self, tg: np.ndarray) -> dict:
+    def term_targets(self, tg: np.ndarray, sel: Optional[dict] = None) -> dict:
         """the batch's target arrays (pose [B, n, 7], position [B, n, 3], rotation [B, n, 4]) from
-        the target poses [B, k, 7]"""
+        the target poses [B, k, 7] (of the optimization terms, or of the row blocks ``sel``)"""
+        sel = self._sel if sel is None else sel
         out = {}
         for key, sl in (("pose_targets", "pose"), ("pos_targets", "pos"), ("rot_targets", "rot")):
-            idx = self._sel[sl]
+            idx = sel[sl]
             if idx:
                 a = tg[:, idx]
                 out[key] = np.ascontiguousarray(a[..., :3] if sl == "pos" else a[..., 3:] if sl == "rot" else a)
@@ -232,8 +250,9 @@ class IKSolver:
         if self._pop is None:
             s = self.config.sampler
             self._pop = PopulationIK(self.ctx, self.spec, self.terms, batch_size, self.config.solver, seed=s.seed,
-                                     keep_first=s.keep_first)
-            self._score = N.Batch(self.ctx, self.spec, self.terms, batch_size)
+                                     keep_first=s.keep_first, score_terms=self.score_terms)
+            # the previous state is ranked like the candidates: by the score terms, else the terms
+            self._score = N.Batch(self.ctx, self.spec, self.score_terms or self.terms, batch_size)
         elif batch_size != self._pop.batch_size:
             raise ValueError(f"Batch size mismatch: expected {self._pop.batch_size}, got {batch_size}")
 
@@ -253,19 +272,32 @@ class IKSolver:
             raise ValueError("this configuration's SmoothnessTask needs prev_q (IKContext.PREVIOUS_STATE)")
         pq = None if prev_q is None else np.asarray(prev_q, np.float32).reshape(B, na)
         scale = init_sample_range if init_sample_range is not None else self.config.sampler.scale
-        ref = None  # the smoothness term's reference rows
-        if self.smooth_ref == "prev":
+        ref = None  # the smoothness terms' reference rows
+        kind = self.smooth_ref or self._score_ref
+        if kind == "prev":
             ref = pq
-        elif self.smooth_ref == "rest":
+        elif kind == "rest":
             rq = self.spec.zero_q if rest_q is None else rest_q
             ref = np.ascontiguousarray(np.broadcast_to(np.asarray(rq, np.float32).reshape(-1, na), (B, na)))
-        self._pop.set_targets(**self.term_targets(tg), q_prev=ref, init_q=init_q, init_scale=scale)
+        score_tg = None
+        if self.score_terms is not None:
+            score_tg = dict(self.term_targets(tg, self._score_sel), q_prev=ref if self._score_ref else None)
+        self._pop.set_targets(**self.term_targets(tg), q_prev=ref if self.smooth_ref else None, init_q=init_q,
+                              init_scale=scale, score_targets=score_tg)
         win = self._pop.solve()
         out = win.get()
         q, cost = out["q"], out["cost"]
+        if self.score_terms is not None:  # the winners' scores (also without a final select)
+            sb = self._pop.score_batch(1)
+            win.copy_q_to(sb)
+            sb.solve(0, N.LMParams())
+            cost = sb.get()["cost"]
         self.last_cost, self.last_prev_cost = cost, None
         if pq is not None:  # score prev with the solver's ranking rule; keep it on a tie
-            self._score.set(q=pq, q_prev=ref, **self.term_targets(tg))
+            if self.score_terms is not None:
+                self._score.set(q=pq, q_prev=ref if self._score_ref else None, **self.term_targets(tg, self._score_sel))
+            else:
+                self._score.set(q=pq, q_prev=ref if self.smooth_ref else None, **self.term_targets(tg))
             self._score.solve(0, self.config.solver.optimizer[-1], from_seed=True)
             pc = self._score.get()["cost"]
             keep = pc <= cost

@@ -147,7 +147,8 @@ EXPORTS = [
     "rk_context_mem", "rk_context_mark", "rk_context_elapsed", "rk_robot_upload", "rk_fk", "rk_fk_base", "rk_link_jacobian",
     "rk_link_jacobian_base", "rk_copy_probe", "rk_se3_op",
     "rk_batch_create",
-    "rk_batch_destroy", "rk_batch_set", "rk_batch_set_rotations", "rk_batch_select", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics",
+    "rk_batch_destroy", "rk_batch_set", "rk_batch_set_rotations", "rk_batch_select", "rk_batch_select_scored",
+    "rk_batch_copy_q", "rk_batch_advance", "rk_batch_solve", "rk_batch_get", "rk_batch_get_iterates", "rk_batch_metrics",
     "rk_batch_metrics_noalloc", "rk_group_create",
     "rk_group_destroy", "rk_group_set_tuning", "rk_group_autotune", "rk_group_solve", "rk_group_kernel_times",
     "rk_server_create", "rk_server_add_terms", "rk_server_add_traj", "rk_server_submit", "rk_server_poll", "rk_server_end_session",
@@ -497,15 +498,26 @@ class Batch:
             out.update(cost_trace=ct, accept=at)
         return out
 
-    def select(self, width_in: int, dst: "Batch", width_out: int, rows: bool = False) -> Optional[np.ndarray]:
+    def select(self, width_in: int, dst: "Batch", width_out: int, rows: bool = False,
+               scores: Optional["Batch"] = None) -> Optional[np.ndarray]:
         """Keep each problem's best ``width_out`` of its ``width_in`` candidates (rows
-        i * width_in + j) in ``dst``, best first: lowest cost at the current iterate, NaN last,
-        equal costs in candidate order.  Returns the source row of each ``dst`` row if ``rows``."""
+        i * width_in + j) in ``dst``, best first: lowest cost at the current iterate (or the cost of
+        ``scores``, a batch of this size holding each row's score), NaN last, equal costs in
+        candidate order.  Returns the source row of each ``dst`` row if ``rows``."""
         n_out = self.n // int(width_in) * int(width_out)
         out = np.zeros(n_out, np.int32) if rows else None
-        _check(lib().rk_batch_select(self._h, int(width_in), dst.handle, int(width_out), _p(out)))
+        if scores is None:
+            _check(lib().rk_batch_select(self._h, int(width_in), dst.handle, int(width_out), _p(out)))
+        else:
+            _check(lib().rk_batch_select_scored(self._h, scores.handle, int(width_in), dst.handle, int(width_out),
+                                                _p(out)))
         dst.n = n_out
         return out
+
+    def copy_q_to(self, dst: "Batch") -> None:
+        """dst's q <- this batch's live q, device to device (dst takes this batch's size)"""
+        _check(lib().rk_batch_copy_q(self._h, dst.handle))
+        dst.n = self.n
 
     def advance(self) -> None:
         """Warm start on the device: q_prev <- q and seeds <- q (next frame of a sequence)."""

@@ -116,13 +116,19 @@ class PopulationIK:
 
     def __init__(self, ctx: N.Context, spec, terms: N.TermSpec, batch_size: int,
                  config: Optional[PopulationSolverConfig] = None, seed: Optional[int] = None,
-                 keep_first: bool = True, scale: float = 1.0):
+                 keep_first: bool = True, scale: float = 1.0, score_terms: Optional[N.TermSpec] = None):
+        """``score_terms``: a term structure whose cost ranks the candidates at every select instead
+        of the optimization cost (PopulationSolver score_terms, opt/population_solver.py:163-186,
+        415-434); the winners' costs are then their scores"""
         self.ctx, self.spec, self.terms = ctx, spec, terms
+        self.score_terms = score_terms
+        self._score_targets: dict = {}
+        self._score_set: set = set()  # widths whose score batch holds the current targets
         self.config = config or default_ik_config()
         self.batch_size = int(batch_size)
         self.widths = self.config.num_candidates
         self._batches: Dict[int, N.Batch] = {}
-        self._twins: Dict[int, N.Batch] = {}
+        self._score: Dict[int, N.Batch] = {}  # per width: the score terms' batch (score_terms only)
         for w in dict.fromkeys(self.widths + [1]):
             self._batches[w] = N.Batch(ctx, spec, terms, self.batch_size * w)
         self.offsets = candidate_offsets(self.widths[0], spec.num_actuated_joints,
@@ -148,10 +154,15 @@ class PopulationIK:
 
     def set_targets(self, pose_targets: Optional[np.ndarray] = None, pos_targets: Optional[np.ndarray] = None,
                     q_prev: Optional[np.ndarray] = None, init_q: Optional[np.ndarray] = None,
-                    init_scale: float = 0.2, rot_targets: Optional[np.ndarray] = None) -> None:
+                    init_scale: float = 0.2, rot_targets: Optional[np.ndarray] = None,
+                    score_targets: Optional[dict] = None) -> None:
         """each problem's targets [batch, ...], broadcast over its candidates (rows i * width + j).
         Seeds: init_q None: the global mid-range candidates (uploaded once, then kept resident);
-        else init_q [batch, na] spread by init_scale x joint range (helpers/ik/ik.py:509-532)."""
+        else init_q [batch, na] spread by init_scale x joint range (helpers/ik/ik.py:509-532).
+        score_targets: the score terms' per-problem arrays (Batch.set keywords), with score_terms"""
+        if self.score_terms is not None:
+            self._score_targets = dict(score_targets or {})
+            self._score_set.clear()
         w0 = self.widths[0]
         rep = lambda a: None if a is None else np.repeat(np.asarray(a, np.float32), w0, axis=0)  # noqa: E731
         b0 = self._batches[w0]
@@ -180,31 +191,44 @@ class PopulationIK:
         params = cfg.optimizer
         cur, w = self._batches[widths[0]], widths[0]
         for k, stage in enumerate(cfg.stages):
-            # every stage boundary selects (population_solver.py:365-375): it narrows the population,
-            # or at an equal width reorders each problem's candidates best first; the next segment
-            # restarts LM
-            if k > 0 and (widths[k] < w or w > 1):
+            # a stage boundary that narrows the population selects; at an equal width the population
+            # is kept as it is (PopulationSolver._select returns it, population_solver.py:436-443);
+            # either way the next segment restarts LM
+            if k > 0 and widths[k] < w:
                 nxt = self._batches[widths[k]]
-                if nxt is cur:  # same width: the select reads one batch and writes its twin
-                    nxt = self._twin(w)
-                if nxt is self._batches[widths[0]]:  # its resident seeds are overwritten
-                    self._global_loaded = False
-                cur.select(w, nxt, widths[k])
+                self._select(cur, w, nxt, widths[k])
                 cur, w = nxt, widths[k]
             if stage.iters > 0 or k == 0:  # a zero-iteration first stage scores the seeds
                 cur.solve(stage.iters, params[k], tpp=tpp, from_seed=(k == 0), wait=False)
         if w > 1:  # one winner per problem
             win = self._batches[1]
-            cur.select(w, win, 1)
+            self._select(cur, w, win, 1)
             cur = win
         return cur
 
-    def _twin(self, w: int) -> N.Batch:
-        """a second batch of width ``w``, for a select that keeps the width"""
-        if w not in self._twins:
-            self._twins[w] = N.Batch(self.ctx, self.spec, self.terms, self.batch_size * w)
-        return self._twins[w]
+    def score_batch(self, w: int) -> N.Batch:
+        """the score terms' batch of width ``w`` (its targets uploaded once per set_targets)"""
+        if w not in self._score:
+            self._score[w] = N.Batch(self.ctx, self.spec, self.score_terms, self.batch_size * w)
+        sb = self._score[w]
+        if w not in self._score_set:
+            rep = {k: np.repeat(np.asarray(v, np.float32), w, axis=0) for k, v in self._score_targets.items()
+                   if v is not None}
+            sb.set(n=self.batch_size * w, **rep)
+            self._score_set.add(w)
+        return sb
+
+    def _select(self, cur: N.Batch, w: int, dst: N.Batch, w_out: int) -> None:
+        """keep each problem's best w_out of its w candidates, ranked by cost or, with score terms,
+        by the score terms' cost at the candidates' iterates"""
+        if self.score_terms is None:
+            cur.select(w, dst, w_out)
+            return
+        sb = self.score_batch(w)
+        cur.copy_q_to(sb)
+        sb.solve(0, N.LMParams(), wait=False)
+        cur.select(w, dst, w_out, scores=sb)
 
     def close(self) -> None:
-        for b in list(self._batches.values()) + list(self._twins.values()):
+        for b in list(self._batches.values()) + list(self._score.values()):
             b.close()

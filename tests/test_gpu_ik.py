@@ -240,3 +240,43 @@ def test_solver_scene_and_rest_state_match_oracle_population(ctx):
     q0 = solver.solve_numpy(wl.targets)
     q0b = solver.solve_numpy(wl.targets, rest_q=np.zeros(na, np.float32))
     np.testing.assert_array_equal(q0, q0b)
+
+
+def test_score_terms_rank_the_candidates(ctx):
+    """IKSolverConfig.add_score: the basic preset optimises, a FrameTask (orientation weighted 0.1)
+    ranks the candidates at every select and the previous state; the winners classify like the
+    oracle population ranked by the same score terms, and the reported costs are the scores"""
+    from robokit_amd import lm as L
+
+    spec = W.panda_tcp()
+    tcp = spec.link_names.index("panda_hand_tcp")
+    wl = W.ik_batch(spec, [tcp], 64, seed=53)
+    cfg = IK.IKSolverConfig(sampler=IK.RobertsSamplerConfig(keep_first=True, scale=1.0))
+    for t in IK.presets.basic.terms:
+        cfg.add(t)
+    cfg.add_score(L.FrameTask(frame_index=IK.IKContext.TARGET_LINKS, T_world_target=IK.IKContext.TARGET,
+                              position_weight=1.0, orientation_weight=0.1), name="score")
+    solver = IK.IKSolver(cfg, spec, tcp, ctx=ctx)
+    q = solver.solve_numpy(wl.targets)
+    stages = _stages(solver.config.solver)
+    fn = lambda inst: _preset_terms([tcp], wl.targets[inst], 7)  # noqa: E731
+    w6 = np.array([1, 1, 1, 0.1, 0.1, 0.1], np.float32)
+    sfn = lambda inst: [O.Term(O.FRAME, [tcp], wl.targets[inst], w6)]  # noqa: E731
+    q32, c32, _ = OP.population_ik(spec, [tcp], wl.targets, solver._pop.seeds, stages, terms_fn=fn, threads=8,
+                                   score_fn=sfn)
+    q64, _, _ = OP.population_ik(spec, [tcp], wl.targets, solver._pop.seeds, stages, dtype="f64", terms_fn=fn,
+                                 threads=8, score_fn=sfn)
+    thr = dict(pos_thr=0.005, ang_thr=0.05)
+    ok = O.ik_metrics(spec, q, [tcp], wl.targets, **thr)[2]
+    ok32 = O.ik_metrics(spec, q32, [tcp], wl.targets, **thr)[2]
+    ok64 = O.ik_metrics(spec, q64, [tcp], wl.targets, dtype="f64", **thr)[2]
+    mism, n_ref = int(np.sum(ok != ok32)), int(np.sum(ok64 != ok32))
+    print(f"\n[ik] score terms: converged gpu {ok.mean():.3f} f32 {ok32.mean():.3f}; differ gpu/f32 {mism}, "
+          f"f32/f64 {n_ref}")
+    assert mism <= max(n_ref, 1)
+    # the winners' reported costs are their scores (the oracle's score at the GPU's winners)
+    s_o = O.lm_solve(spec, sfn(np.arange(wl.size)), q, 0, O.LMParams(), traces=False)["cost"]
+    np.testing.assert_allclose(solver.last_cost, s_o, rtol=1e-4, atol=1e-7)
+    # the previous state is ranked by the score terms too
+    solver.solve_numpy(wl.targets, prev_q=q)
+    np.testing.assert_allclose(solver.last_prev_cost, s_o, rtol=1e-4, atol=1e-7)
