@@ -10,6 +10,8 @@ Mirrors the reference's IK helper (helpers/ik/ik.py, helpers/ik/config.py, helpe
                       mid-range (drawn once, kept resident) or around init_q (init_sample_range
                       x joint range), the population schedule, and prev_q kept when its cost is no
                       worse than the winner's (select_better_score_kernel, :43-57)
+  IKSolver.solve_torch  helpers/ik/ik.py:307-347: 4 x 4 target matrices and torch tensors in and
+                      out (torch imported only there; matrix_to_pose by Shepperd's method)
   compute_metrics_numpy  helpers/ik/ik.py:387-470 (_ik_metrics_kernel, :60-82): per-target
                       position / rotation error and success at 5 mm / 0.05 rad by default
 
@@ -27,7 +29,7 @@ implemented (NotImplementedError).
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence, Tuple, Union
+from typing import List, NamedTuple, Optional, Sequence, Tuple, Union
 
 import numpy as np
 
@@ -120,6 +122,46 @@ class presets:
         .add(L.RotationTask(frame_index=IKContext.TARGET_LINKS, T_world_target=IKContext.TARGET, weight=10.0))
         .add(L.PositionLimit(weight=50.0))
         .add(L.SmoothnessTask(prev_var=IKContext.PREVIOUS_STATE, weight=0.1)))
+
+
+class IKResultTorch(NamedTuple):
+    """helpers/ik/ik.py:93-95"""
+
+    q: "object"
+    T_world_base: Optional["object"] = None
+
+
+def matrix_to_pose(T: np.ndarray) -> np.ndarray:
+    """homogeneous transforms [..., 4, 4] -> poses [..., 7] (x, y, z, qw, qx, qy, qz): the rotation's
+    unit quaternion by Shepperd's method (the largest of 4 qw^2, 4 qx^2, 4 qy^2, 4 qz^2 picks the
+    well-conditioned formula), scalar part made non-negative"""
+    T = np.asarray(T, np.float64)
+    R = T[..., :3, :3]
+    t = T[..., :3, 3]
+    tr = R[..., 0, 0] + R[..., 1, 1] + R[..., 2, 2]
+    cand = np.stack([tr, R[..., 0, 0], R[..., 1, 1], R[..., 2, 2]], -1)
+    k = np.argmax(cand, -1)
+    q = np.zeros(T.shape[:-2] + (4,))
+    m = k == 0
+    s0 = np.sqrt(np.maximum(1.0 + tr[m], 0.0)) * 2
+    q[m] = np.stack([0.25 * s0, (R[m][:, 2, 1] - R[m][:, 1, 2]) / s0, (R[m][:, 0, 2] - R[m][:, 2, 0]) / s0,
+                     (R[m][:, 1, 0] - R[m][:, 0, 1]) / s0], -1)
+    for i in range(3):
+        m = k == i + 1
+        if not np.any(m):
+            continue
+        j, l = (i + 1) % 3, (i + 2) % 3
+        Rm = R[m]
+        si = np.sqrt(np.maximum(1.0 + Rm[:, i, i] - Rm[:, j, j] - Rm[:, l, l], 0.0)) * 2
+        qq = np.zeros((Rm.shape[0], 4))
+        qq[:, 0] = (Rm[:, l, j] - Rm[:, j, l]) / si
+        qq[:, 1 + i] = 0.25 * si
+        qq[:, 1 + j] = (Rm[:, j, i] + Rm[:, i, j]) / si
+        qq[:, 1 + l] = (Rm[:, l, i] + Rm[:, i, l]) / si
+        q[m] = qq
+    q *= np.where(q[..., :1] < 0, -1.0, 1.0)
+    q /= np.linalg.norm(q, axis=-1, keepdims=True)
+    return np.concatenate([t, q], -1).astype(np.float32)
 
 
 class IKSolver:
@@ -304,6 +346,25 @@ class IKSolver:
             q = np.where(keep[:, None], pq, q)
             self.last_prev_cost = pc
         return q
+
+    def solve_torch(self, T_world_target, prev_state=None, init_q=None, init_T_world_base=None, rest_q=None,
+                    rest_T_world_base=None, init_sample_range: Optional[float] = None) -> IKResultTorch:
+        """The torch adapter (helpers/ik/ik.py:307-347): T_world_target [..., 4, 4] homogeneous
+        matrices, optional init_q / rest_q tensors and a previous state (tensor or array of q);
+        returns IKResultTorch(q) on the targets' device.  torch is imported only by this call."""
+        import torch
+
+        if init_T_world_base is not None or rest_T_world_base is not None:
+            raise NotImplementedError("the floating base is not implemented on this path")
+        T = T_world_target
+        if not 2 <= T.ndim <= 4 or tuple(T.shape[-2:]) != (4, 4):
+            raise ValueError(f"T_world_target must have shape (..., 4, 4), got {tuple(T.shape)}")
+        as_np = lambda x: None if x is None else (x.detach().cpu().numpy() if torch.is_tensor(x)  # noqa: E731
+                                                   else np.asarray(getattr(x, "q", x)))
+        poses = matrix_to_pose(as_np(T).reshape(-1, self.num_targets, 4, 4))
+        q = self.solve_numpy(poses, prev_q=as_np(getattr(prev_state, "q", prev_state)), init_q=as_np(init_q),
+                             init_sample_range=init_sample_range, rest_q=as_np(rest_q))
+        return IKResultTorch(q=torch.from_numpy(np.ascontiguousarray(q)).to(T.device))
 
     def compute_metrics_numpy(self, q: np.ndarray, T_world_target, position_threshold_m: float = 0.005,
                               orientation_threshold_rad: float = 0.05) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:

@@ -280,3 +280,25 @@ def test_score_terms_rank_the_candidates(ctx):
     # the previous state is ranked by the score terms too
     solver.solve_numpy(wl.targets, prev_q=q)
     np.testing.assert_allclose(solver.last_prev_cost, s_o, rtol=1e-4, atol=1e-7)
+
+
+def test_solve_torch_adapter(ctx, panda):
+    """solve_torch takes 4 x 4 target matrices as torch tensors and returns torch q equal to
+    solve_numpy's on the same poses"""
+    import torch
+
+    spec, wl = panda
+    solver = IK.IKSolver(IK.IKSolverConfig(), spec, wl.links, ctx=ctx)
+    x, y, z = wl.targets[..., 4], wl.targets[..., 5], wl.targets[..., 6]
+    w = wl.targets[..., 3]
+    T = np.zeros(wl.targets.shape[:-1] + (4, 4))
+    T[..., 0, :3] = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1)
+    T[..., 1, :3] = np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1)
+    T[..., 2, :3] = np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)
+    T[..., :3, 3] = wl.targets[..., :3]
+    T[..., 3, 3] = 1
+    res = solver.solve_torch(torch.from_numpy(T.astype(np.float32)))
+    assert torch.is_tensor(res.q) and res.T_world_base is None
+    poses = IK.matrix_to_pose(T.astype(np.float32))
+    q_np = solver.solve_numpy(poses)
+    np.testing.assert_array_equal(res.q.numpy(), q_np)

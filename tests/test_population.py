@@ -171,3 +171,32 @@ def test_ik_solver_rejects_bad_arguments_before_touching_the_device():
         IK.IKSolver(IK.IKSolverConfig(), spec, [])
     with pytest.raises(NotImplementedError):
         IK.IKSolverConfig(enable_T_world_base=True)
+
+
+def test_matrix_to_pose_round_trips_rotations():
+    """the torch adapter's target conversion: Shepperd's quaternion of a rotation matrix, compared
+    with the matrix rebuilt from it (every branch: rotations near 0 and pi about each axis and
+    random ones), the scalar part non-negative"""
+    from robokit_amd.ik import matrix_to_pose
+
+    rng = np.random.default_rng(0)
+
+    def rot(axis, ang):
+        a = np.asarray(axis, float) / np.linalg.norm(axis)
+        K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+        return np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+
+    Rs = [rot(ax, ang) for ax in np.eye(3) for ang in (0.0, 0.3, np.pi - 1e-3, np.pi)]
+    Rs += [rot(rng.normal(size=3), rng.uniform(0, np.pi)) for _ in range(50)]
+    T = np.zeros((len(Rs), 4, 4))
+    T[:, :3, :3] = Rs
+    T[:, :3, 3] = rng.normal(size=(len(Rs), 3))
+    T[:, 3, 3] = 1
+    p = matrix_to_pose(T).astype(np.float64)
+    w, x, y, z = p[:, 3], p[:, 4], p[:, 5], p[:, 6]
+    Rq = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+                   np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+                   np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+    np.testing.assert_allclose(Rq, np.asarray(Rs), atol=2e-6)
+    np.testing.assert_allclose(p[:, :3], T[:, :3, 3], atol=1e-6)
+    assert np.all(w >= 0)
