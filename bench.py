@@ -69,13 +69,15 @@ def variant_name(tpp: int, spec, n_pos: int) -> str:
         return "V_W012" if n_pos else "V_W40"
     if tpp == -32:
         return "V_H012" if n_pos else "V_H40"
+    if tpp == -33:
+        return "V_HS012" if n_pos else "V_HS40"
     if tpp == 1:
         return f"V_T{spec.num_actuated_joints}"
     return f"G{tpp}"
 
 
 # register-resident variants: a group of them runs as one k_persistent_reg launch
-MIXABLE = {"V_T5", "V_T6", "V_T7", "V_W40", "V_W012", "V_H40", "V_H012"}
+MIXABLE = {"V_T5", "V_T6", "V_T7", "V_W40", "V_W012", "V_H40", "V_H012", "V_HS40", "V_HS012"}
 
 
 def profiled_traffic(kernel: str):

@@ -217,9 +217,14 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
  * RK_TPP_WAVE = one wavefront per problem, register-resident rows (trees, <= 64 joints, <= 32 dofs);
  * RK_TPP_HALF = two problems per wavefront, 32 lanes each, register-resident rows (trees,
  *               <= 32 joints, <= 32 dofs);
+ * RK_TPP_SUPER = as RK_TPP_HALF, with the damped normal equations solved in block-arrow form:
+ *               the chains of the elimination tree (a humanoid's limbs, <= 7 dofs each, <= 8 of
+ *               them) factored side by side on their own lanes, then the <= 3-dof root they share
+ *               (the waist) from its Schur complement;
  * 4, 8, 16, 32, 64 = that many lanes per problem, LDS-staged (any robot the batch accepts) */
 #define RK_TPP_WAVE (-64)
 #define RK_TPP_HALF (-32)
+#define RK_TPP_SUPER (-33)
 #define RK_TPP_TRAJ (-128) /* a trajectory: two wavefronts per problem (rk_group_kernel_times, autotune) */
 #define RK_USE_GRAPH 0x1   /* rk_group_solve flag: capture the launch sequence in a hipGraph and replay it */
 #define RK_ASYNC 0x2       /* rk_group_solve flag: return once the solve is enqueued on the context's stream

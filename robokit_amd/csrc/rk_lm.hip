@@ -42,6 +42,8 @@
 #define RK_COLL_MAXS 64   // robot collision spheres of an IK term structure
 #define RK_SCENE_MAXG 4   // volumes / meshes of a scene
 #define RK_GEOM_MAX 64    // geometries resident in a context
+// supernodal half-wave kernels: dofs per slot, root dofs, slots per problem
+constexpr int SN_NB = 7, SN_NR = 3, SN_MAXSLOT = 8;
 
 namespace rk {
 
@@ -81,6 +83,12 @@ struct Terms {
   // columns (bit c): the other chunks' trailing updates add exact zeros and are skipped
   int wperm[RK_NA_MAX], wiperm[RK_NA_MAX];
   unsigned wchol[RK_NA_MAX / 4];
+  // block-arrow form of the same order (supernodal half-wave kernels), when it exists: the dofs
+  // fall into sn_nslot slots of <= SN_NB consecutive positions (sn_start, sn_size), each coupled
+  // only to itself and to the sn_nr root positions starting at sn_r0 (the last ones); else
+  // sn_nslot = 0
+  int sn_nslot, sn_r0, sn_nr;
+  int sn_start[SN_MAXSLOT], sn_size[SN_MAXSLOT];
   // SceneCollisionTask rows (after the smoothness rows), one per robot sphere: r = w pen(sd) with
   // sd = scene distance of the sphere centre - radius; spheres sorted by the joint whose pose
   // moves their link (-1: the base link) so that FK meets them in order
@@ -104,8 +112,11 @@ struct Terms {
 // the symbolic factorisation in that order gives the structurally nonzero (row chunk, column
 // block) pairs.  The ordering only changes which exact zeros are skipped and the rounding
 // order, not the system solved.
+static void sn_structure(const Robot& r, Terms& th, const std::vector<unsigned>& adj0, std::vector<unsigned>& adj);
 static void wave_elimination(const Robot& r, Terms& th) {
   const int na = r.na;
+  th.sn_nslot = th.sn_r0 = th.sn_nr = 0;
+  for (int s = 0; s < SN_MAXSLOT; ++s) th.sn_start[s] = th.sn_size[s] = 0;
   for (int k = 0; k < RK_NA_MAX; ++k) th.wperm[k] = th.wiperm[k] = k;
   for (int s4 = 0; s4 < RK_NA_MAX / 4; ++s4) th.wchol[s4] = ~0u;
   if (na > RK_NA_MAX) return;
@@ -122,6 +133,7 @@ static void wave_elimination(const Robot& r, Terms& th) {
   for (unsigned m : dep)
     for (int a = 0; a < na; ++a)
       if ((m >> a) & 1u) adj[a] |= m & ~(1u << a);
+  const std::vector<unsigned> adj0 = adj;
   unsigned left = na >= 32 ? ~0u : ((1u << na) - 1u);
   for (int k = 0; k < na; ++k) {
     int best = -1, bdeg = 1 << 30;
@@ -137,6 +149,7 @@ static void wave_elimination(const Robot& r, Terms& th) {
     left &= ~(1u << best);
   }
   for (int k = 0; k < na; ++k) th.wiperm[th.wperm[k]] = k;
+  sn_structure(r, th, adj0, adj);
   // filled pattern in the new order: row i has a nonzero in column k < i iff dofs wperm[i],
   // wperm[k] are adjacent in the filled graph (adj now holds it)
   for (int s4 = 0; s4 < RK_NA_MAX / 4; ++s4) {
@@ -147,6 +160,105 @@ static void wave_elimination(const Robot& r, Terms& th) {
           if ((adj[th.wperm[i]] >> th.wperm[k]) & 1u) m |= 1u << c;
     th.wchol[s4] = m;
   }
+}
+
+// symbolic elimination of the coupling graph adj (bit b of adj[a]: dofs a, b share a row block)
+// in the order perm: returns the filled graph (eliminating a joins its remaining neighbours)
+static std::vector<unsigned> sn_fill(std::vector<unsigned> adj, const int* perm, int na) {
+  unsigned left = na >= 32 ? ~0u : ((1u << na) - 1u);
+  for (int k = 0; k < na; ++k) {
+    const int a = perm[k];
+    const unsigned nb = adj[a] & left & ~(1u << a);
+    for (int b = 0; b < na; ++b)
+      if ((nb >> b) & 1u) adj[b] |= nb & ~(1u << b);
+    left &= ~(1u << a);
+  }
+  return adj;
+}
+
+// Block-arrow form of the normal equations (host, at batch creation), in a leaves-first order:
+// dofs by decreasing depth of their joints in the kinematic tree (ties: lowest dof), which for a
+// tree eliminates every dof before the dofs above it, without fill.  In the filled pattern every
+// nonzero (i, k), i > k, joins position k to an ancestor i in the elimination tree (parent(k) =
+// the first later position adjacent to k).  The root set holds every node with two or more
+// children and all their ancestors; what remains of the tree is chains, each coupled only to
+// itself and to the root.  For a humanoid: the two legs and the two arms (one chain each), the
+// waist as the root.  Chains are packed first-fit by decreasing size into slots of <= SN_NB dofs
+// and the order becomes [slot 0][slot 1]...[root] (chains and root keep their relative order: a
+// topological order of the same tree, so the fill is unchanged).  If that form exists the dofs
+// take this order (wperm) and adj becomes its filled graph; without it (a chain longer than
+// SN_NB, a root larger than SN_NR, more than SN_MAXSLOT slots) nothing changes and sn_nslot
+// stays 0: the supernodal kernels are not offered.
+static void sn_structure(const Robot& r, Terms& th, const std::vector<unsigned>& adj0, std::vector<unsigned>& adj) {
+  th.sn_nslot = th.sn_r0 = th.sn_nr = 0;
+  for (int s = 0; s < SN_MAXSLOT; ++s) th.sn_start[s] = th.sn_size[s] = 0;
+  const int na = r.na;
+  if (na <= 0 || na > RK_NA_MAX) return;
+  std::vector<int> jdepth(r.nj, 0), ddepth(na, 0);
+  for (int j = 0; j < r.nj; ++j) jdepth[j] = r.parent[j] >= 0 ? jdepth[r.parent[j]] + 1 : 0;
+  for (int a = 0; a < na; ++a)
+    for (int c = r.col_start[a]; c < r.col_start[a + 1]; ++c) ddepth[a] = std::max(ddepth[a], jdepth[r.col_joint[c]]);
+  int order[RK_NA_MAX];
+  for (int a = 0; a < na; ++a) order[a] = a;
+  std::stable_sort(order, order + na, [&](int a, int b) { return ddepth[a] > ddepth[b]; });
+  const std::vector<unsigned> filled = sn_fill(adj0, order, na);
+  std::vector<int> parent(na, -1), nchild(na, 0);
+  for (int k = 0; k < na; ++k)
+    for (int i = k + 1; i < na; ++i)
+      if ((filled[order[i]] >> order[k]) & 1u) {
+        parent[k] = i;
+        ++nchild[i];
+        break;
+      }
+  std::vector<char> root(na, 0);
+  for (int k = 0; k < na; ++k)
+    if (nchild[k] >= 2)
+      for (int a = k; a >= 0 && !root[a]; a = parent[a]) root[a] = 1;
+  // chains: each non-root position belongs to the chain of its topmost non-root ancestor
+  std::vector<int> top(na), chains;
+  for (int k = na - 1; k >= 0; --k) {
+    if (root[k]) continue;
+    top[k] = (parent[k] >= 0 && !root[parent[k]]) ? top[parent[k]] : k;
+    if (top[k] == k) chains.push_back(k);
+  }
+  std::vector<std::vector<int>> members(na);
+  for (int k = 0; k < na; ++k)
+    if (!root[k]) members[top[k]].push_back(k);
+  int nr = 0;
+  for (int k = 0; k < na; ++k) nr += root[k];
+  if (nr > SN_NR) return;
+  std::sort(chains.begin(), chains.end(), [&](int a, int b) {
+    return members[a].size() != members[b].size() ? members[a].size() > members[b].size() : a < b;
+  });
+  std::vector<std::vector<int>> slots;
+  for (int c : chains) {
+    if ((int)members[c].size() > SN_NB) return;
+    bool put = false;
+    for (auto& sl : slots)
+      if (sl.size() + members[c].size() <= (size_t)SN_NB) {
+        sl.insert(sl.end(), members[c].begin(), members[c].end());
+        put = true;
+        break;
+      }
+    if (!put) slots.push_back(members[c]);
+  }
+  if ((int)slots.size() > SN_MAXSLOT) return;
+  int perm[RK_NA_MAX], pos = 0;
+  for (size_t s = 0; s < slots.size(); ++s) {
+    th.sn_start[s] = pos;
+    th.sn_size[s] = (int)slots[s].size();
+    for (int k : slots[s]) perm[pos++] = order[k];
+  }
+  th.sn_r0 = pos;
+  for (int k = 0; k < na; ++k)
+    if (root[k]) perm[pos++] = order[k];
+  th.sn_nr = nr;
+  th.sn_nslot = (int)slots.size();
+  for (int k = 0; k < na; ++k) {
+    th.wperm[k] = perm[k];
+    th.wiperm[perm[k]] = k;
+  }
+  adj = sn_fill(adj0, perm, na);
 }
 
 struct LMParams {
@@ -780,10 +892,11 @@ __global__ __launch_bounds__(64) void k_fused(Problems P, LMParams lp, int it0, 
 //   V_TC*/V_PC*: as V_T* / V_P* (6 or 7 dofs), with SceneCollisionTask rows
 enum Variant {
   V_S4 = 0, V_S8, V_S16, V_L16, V_L32, V_L64, V_T5, V_T6, V_T7, V_T8, V_W40, V_W012, V_H40, V_H012,
-  V_P5, V_P6, V_P7, V_P8, V_TC6, V_TC7, V_PC6, V_PC7, NVARIANT
+  V_P5, V_P6, V_P7, V_P8, V_TC6, V_TC7, V_PC6, V_PC7, V_HS40, V_HS012, NVARIANT
 };
 __host__ __device__ constexpr bool variant_wave(int v) { return v == V_W40 || v == V_W012; }
-__host__ __device__ constexpr bool variant_half(int v) { return v == V_H40 || v == V_H012; }
+__host__ __device__ constexpr bool variant_super(int v) { return v == V_HS40 || v == V_HS012; }
+__host__ __device__ constexpr bool variant_half(int v) { return v == V_H40 || v == V_H012 || variant_super(v); }
 __host__ __device__ constexpr bool variant_thread(int v) {
   return (v >= V_T5 && v <= V_T8) || (v >= V_P5 && v <= V_P8) || (v >= V_TC6 && v <= V_PC7);
 }
@@ -806,8 +919,8 @@ __host__ __device__ constexpr size_t variant_lds(int v) {
   return variant_thread(v) ? 0
          : v == V_W40      ? wave_lds_bytes<4, 0, 64>()
          : v == V_W012     ? wave_lds_bytes<0, 12, 64>()
-         : v == V_H40      ? wave_lds_bytes<4, 0, 32>()
-         : v == V_H012     ? wave_lds_bytes<0, 12, 32>()
+         : v == V_H40 || v == V_HS40   ? wave_lds_bytes<4, 0, 32>()
+         : v == V_H012 || v == V_HS012 ? wave_lds_bytes<0, 12, 32>()
          : variant_small(v) ? (WAVE / variant_G(v)) * sizeof(Shared<CfgS>)
                             : (WAVE / variant_G(v)) * sizeof(Shared<CfgL>);
 }
@@ -846,6 +959,8 @@ __device__ __forceinline__ void run_tile(const Bucket* bk, int local, const LMPa
   else if constexpr (V == V_W40) wave_solve<4, 0, 64>(P, local, lp, it0, iters, lds);
   else if constexpr (V == V_W012) wave_solve<0, 12, 64>(P, local, lp, it0, iters, lds);
   else if constexpr (V == V_H40) wave_solve<4, 0, 32>(P, local, lp, it0, iters, lds);
+  else if constexpr (V == V_HS40) wave_solve<4, 0, 32, true>(P, local, lp, it0, iters, lds);
+  else if constexpr (V == V_HS012) wave_solve<0, 12, 32, true>(P, local, lp, it0, iters, lds);
   else wave_solve<0, 12, 32>(P, local, lp, it0, iters, lds);
 }
 
@@ -925,6 +1040,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     case V_W40: persist_loop<V_W40>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
     case V_W012: persist_loop<V_W012>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
     case V_H40: persist_loop<V_H40>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
+    case V_HS40: persist_loop<V_HS40>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
+    case V_HS012: persist_loop<V_HS012>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
     default: persist_loop<V_H012>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
   }
 }
@@ -1703,6 +1820,14 @@ static int wave_kind(const Terms& th, const Robot& r, int hw) {
   return -1;
 }
 
+// supernodal half-wave kernels: the half-wave kinds whose normal equations have the block-arrow
+// form (sn_structure found one)
+static int super_kind(const Terms& th, const Robot& r) {
+  const int k = wave_kind(th, r, 32);
+  if (k < 0 || th.sn_nslot <= 0) return -1;
+  return k == V_H40 ? V_HS40 : V_HS012;
+}
+
 static int signature(const Terms& th) {
   return ((((th.robot * 8 + th.n_pose) * 16 + th.n_pos) * 4 + th.limit * 2 + th.smooth) * 8 + th.n_rot) * 2 +
          (th.nc ? 1 : 0);
@@ -1740,6 +1865,10 @@ static int pick_variant(const rk_context* ctx, const Terms& th, int G, int* v) {
     *v = wave_kind(th, r, -G);
     return RK_OK;
   }
+  if (G == RK_TPP_SUPER && super_kind(th, r) >= 0) {
+    *v = super_kind(th, r);
+    return RK_OK;
+  }
   if (small && (G == 4 || G == 8 || G == 16)) {
     *v = G == 4 ? V_S4 : G == 8 ? V_S8 : V_S16;
     return RK_OK;
@@ -1749,8 +1878,9 @@ static int pick_variant(const rk_context* ctx, const Terms& th, int G, int* v) {
     return RK_OK;
   }
   return fail(RK_EINVAL,
-              "threads_per_problem must be 1 (serial arms), -64 (register wave: trees), 4, 8, 16 (small robots) "
-              "or 16, 32, 64");
+              "threads_per_problem must be 1 (serial arms), -64 / -32 (register wave / half-wave: trees), -33 "
+              "(supernodal half-wave: trees whose normal equations have the block-arrow form), 4, 8, 16 (small "
+              "robots) or 16, 32, 64");
 }
 static int pick_variant(const rk_batch* b, int G, int* v) { return pick_variant(b->ctx, b->th, G, v); }
 
@@ -1760,6 +1890,7 @@ static void legal_tpps(const rk_batch* b, std::vector<int>& out) {
   if (fits_thread(b->th, r)) out.push_back(1);
   if (wave_kind(b->th, r, 64) >= 0) out.push_back(RK_TPP_WAVE);
   if (wave_kind(b->th, r, 32) >= 0) out.push_back(RK_TPP_HALF);
+  if (super_kind(b->th, r) >= 0) out.push_back(RK_TPP_SUPER);
   if (fits_small(b->th, r)) out.insert(out.end(), {4, 8, 16, 32, 64});
   else out.insert(out.end(), {16, 32, 64});
 }
@@ -1806,6 +1937,12 @@ static int launch_variant(hipStream_t st, int v, const Problems& P, const LMPara
     case V_W012: hipLaunchKernelGGL((k_wave<0, 12, 64>), dim3(P.n), dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_H40:
       hipLaunchKernelGGL((k_wave<4, 0, 32>), dim3((P.n + 1) / 2), dim3(WAVE), 0, st, P, lp, it0, iters);
+      break;
+    case V_HS40:
+      hipLaunchKernelGGL((k_wave<4, 0, 32, true>), dim3((P.n + 1) / 2), dim3(WAVE), 0, st, P, lp, it0, iters);
+      break;
+    case V_HS012:
+      hipLaunchKernelGGL((k_wave<0, 12, 32, true>), dim3((P.n + 1) / 2), dim3(WAVE), 0, st, P, lp, it0, iters);
       break;
     default:
       hipLaunchKernelGGL((k_wave<0, 12, 32>), dim3((P.n + 1) / 2), dim3(WAVE), 0, st, P, lp, it0, iters);
@@ -3184,6 +3321,8 @@ static int variant_grid_any(rk_context* ctx, int v, int wg_per_cu, size_t lds, i
     case V_W40: return variant_grid<V_W40>(ctx, wg_per_cu, lds, tiles, grid);
     case V_W012: return variant_grid<V_W012>(ctx, wg_per_cu, lds, tiles, grid);
     case V_H40: return variant_grid<V_H40>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_HS40: return variant_grid<V_HS40>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_HS012: return variant_grid<V_HS012>(ctx, wg_per_cu, lds, tiles, grid);
     case V_P5: return variant_grid<V_P5>(ctx, wg_per_cu, lds, tiles, grid);
     case V_P6: return variant_grid<V_P6>(ctx, wg_per_cu, lds, tiles, grid);
     case V_P7: return variant_grid<V_P7>(ctx, wg_per_cu, lds, tiles, grid);
@@ -3215,6 +3354,8 @@ static void launch_persistent(int v, dim3 grid, size_t lds, hipStream_t st, cons
     RK_LAUNCH_P(V_W012)
     RK_LAUNCH_P(V_H40)
     RK_LAUNCH_P(V_H012)
+    RK_LAUNCH_P(V_HS40)
+    RK_LAUNCH_P(V_HS012)
     RK_LAUNCH_P(V_P5)
     RK_LAUNCH_P(V_P6)
     RK_LAUNCH_P(V_P7)
@@ -3503,6 +3644,7 @@ int rk_group_kernel_times(rk_group* g, int32_t cap, int32_t* tpp, float* ms, int
     if (tpp)
       tpp[k] = v >= RK_VAR_TRAJ ? RK_TPP_TRAJ
                : variant_wave(v) ? RK_TPP_WAVE
+               : variant_super(v) ? RK_TPP_SUPER
                : variant_half(v) ? RK_TPP_HALF
                                  : variant_G(v);
     if (ms) HIP_TRY(hipEventElapsedTime(&ms[k], g->ev_k0[k], g->ev_k1[k]));

@@ -124,6 +124,9 @@ struct WaveLDS {
   union {
     alignas(16) float J[WV_NA][JS];     // Jacobian columns (dof-major)
     alignas(16) float L[WV_NA][36];     // Cholesky factor rows, for the transposed backward solve
+    // supernodal kernels: rows of J^T J (row stride 36); the rows past WV_NA only absorb the
+    // masked reads of a slot's padding rows (a slot of s < SN_NB dofs reads SN_NB rows)
+    alignas(16) float M[WV_NA + SN_NB + 1][36];
   };
 };
 
@@ -145,6 +148,173 @@ struct WaveJoint {
   float ax[3], mul, off;
   int type, parent, act, mimic;
 };
+
+// Supernodal solve of (J^T J + diag) dq = b for the block-arrow form of the elimination order
+// (two problems per wavefront; sn_structure builds the form).  With the slots' diagonal blocks
+// D_k, their coupling blocks E_k (root rows x slot columns) and the root block C:
+//   D_k = L_k L_k^T,  F_k = E_k L_k^-T,  y_k = L_k^-1 b_k                 (lane k, all slots at once)
+//   C' = C - sum_k F_k F_k^T,  b_r' = b_r - sum_k F_k y_k,  C' x_r = b_r'  (one lane)
+//   L_k^T x_k = y_k - F_k^T x_r                                          (lane k)
+// which is the block form of the Cholesky factorisation in the same order.  Each slot lane holds
+// its blocks in registers (a slot of s < SN_NB dofs is padded with identity rows); rows of the
+// system go through LDS once.  Returns dq of the lane's dof (0 on padding lanes).  The pivots'
+// reciprocal square roots are the hardware's (v_rsq_f32, 1 ulp).
+template <int NP, int NQ>
+__device__ __forceinline__ float sn_solve(WaveLDS<NP, NQ>* sh, TermsK* tm, int lane, int c, int ns,
+                                          const float (&A)[WV_NA], float dsum, float b) {
+  constexpr int LS = 36, NB = SN_NB, NR = SN_NR, NS = SN_NR * (SN_NR + 1) / 2, SB = 16;
+  static_assert(NS + NR <= SB, "slot scratch");
+  float* const M = &sh->M[0][0];
+  float* const dg = sh->cst[0];  // diagonal terms + lam per row
+  float* const bv = sh->rhs[0];  // right-hand side per row
+  float* const X = sh->dvs;      // solution per row
+  float* const Sb = reinterpret_cast<float*>(&sh->blk[0][0]);  // per slot: Schur block, then F y
+  const int nsl = tm->sn_nslot, r0 = tm->sn_r0, nr = tm->sn_nr;
+  const int lc = lane & (WV_NA - 1);
+#pragma unroll
+  for (int j = 0; j < WV_NA; j += 4)
+    *reinterpret_cast<float4*>(&M[lc * LS + j]) = make_float4(A[j], A[j + 1], A[j + 2], A[j + 3]);
+  dg[lc] = dsum;
+  bv[lc] = b;
+  __syncthreads();
+  float L[NB][NB], F[NR][NB], y[NB], di[NB];
+  const bool slot = ns > 0;
+  if (slot) {
+    const float* Dp = M + c * (LS + 1);  // D[i][j] = Dp[i * LS + j]
+    const float* Ep = M + r0 * LS + c;   // E[w][j] = Ep[w * LS + j]
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+#pragma unroll
+      for (int j = 0; j < i; ++j) L[i][j] = i < ns ? Dp[i * LS + j] : 0.f;
+      L[i][i] = i < ns ? Dp[i * LS + i] + dg[c + i] : 1.f;
+      y[i] = i < ns ? bv[c + i] : 0.f;
+    }
+#pragma unroll
+    for (int w = 0; w < NR; ++w)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) F[w][j] = (w < nr && j < ns) ? Ep[w * LS + j] : 0.f;
+    // left-looking Cholesky of D, with F = E L^-T and y = L^-1 b column by column
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      float d = L[k][k];
+#pragma unroll
+      for (int m = 0; m < k; ++m) d -= L[k][m] * L[k][m];
+      di[k] = __builtin_amdgcn_rsqf(d);
+#pragma unroll
+      for (int i = k + 1; i < NB; ++i) {
+        float v = L[i][k];
+#pragma unroll
+        for (int m = 0; m < k; ++m) v -= L[i][m] * L[k][m];
+        L[i][k] = v * di[k];
+      }
+#pragma unroll
+      for (int w = 0; w < NR; ++w) {
+        float v = F[w][k];
+#pragma unroll
+        for (int m = 0; m < k; ++m) v -= F[w][m] * L[k][m];
+        F[w][k] = v * di[k];
+      }
+      float v = y[k];
+#pragma unroll
+      for (int m = 0; m < k; ++m) v -= L[k][m] * y[m];
+      y[k] = v * di[k];
+    }
+    // this slot's share of the root's Schur complement and right-hand side
+    float* const so = Sb + lane * SB;
+#pragma unroll
+    for (int w = 0, e = 0; w < NR; ++w)
+#pragma unroll
+      for (int v = 0; v <= w; ++v, ++e) {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) acc += F[w][k] * F[v][k];
+        so[e] = acc;
+      }
+#pragma unroll
+    for (int w = 0; w < NR; ++w) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < NB; ++k) acc += F[w][k] * y[k];
+      so[NS + w] = acc;
+    }
+  }
+  __syncthreads();
+  // root: C' = C - sum of the slots' shares, factor and solve (one lane per problem)
+  if (lane == 0 && nr > 0) {
+    float C[NR][NR], r[NR];
+#pragma unroll
+    for (int w = 0; w < NR; ++w) {
+#pragma unroll
+      for (int v = 0; v < w; ++v) C[w][v] = w < nr ? M[(r0 + w) * LS + r0 + v] : 0.f;
+      C[w][w] = w < nr ? M[(r0 + w) * LS + r0 + w] + dg[r0 + w] : 1.f;
+      r[w] = w < nr ? bv[r0 + w] : 0.f;
+    }
+    for (int k = 0; k < nsl; ++k) {
+      const float* si = Sb + k * SB;
+#pragma unroll
+      for (int w = 0, e = 0; w < NR; ++w)
+#pragma unroll
+        for (int v = 0; v <= w; ++v, ++e) C[w][v] -= si[e];
+#pragma unroll
+      for (int w = 0; w < NR; ++w) r[w] -= si[NS + w];
+    }
+    float ci[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      float d = C[k][k];
+#pragma unroll
+      for (int m = 0; m < k; ++m) d -= C[k][m] * C[k][m];
+      ci[k] = __builtin_amdgcn_rsqf(d);
+#pragma unroll
+      for (int i = k + 1; i < NR; ++i) {
+        float v = C[i][k];
+#pragma unroll
+        for (int m = 0; m < k; ++m) v -= C[i][m] * C[k][m];
+        C[i][k] = v * ci[k];
+      }
+      float v = r[k];
+#pragma unroll
+      for (int m = 0; m < k; ++m) v -= C[k][m] * r[m];
+      r[k] = v * ci[k];
+    }
+#pragma unroll
+    for (int k = NR - 1; k >= 0; --k) {
+      float v = r[k];
+#pragma unroll
+      for (int i = k + 1; i < NR; ++i) v -= C[i][k] * r[i];
+      r[k] = v * ci[k];
+    }
+#pragma unroll
+    for (int w = 0; w < NR; ++w)
+      if (w < nr) X[r0 + w] = r[w];
+  }
+  __syncthreads();
+  if (slot) {
+    float xr[NR];
+#pragma unroll
+    for (int w = 0; w < NR; ++w) xr[w] = w < nr ? X[r0 + w] : 0.f;
+    float x[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      float v = y[k];
+#pragma unroll
+      for (int w = 0; w < NR; ++w) v -= F[w][k] * xr[w];
+      x[k] = v;
+    }
+#pragma unroll
+    for (int k = NB - 1; k >= 0; --k) {
+      float v = x[k];
+#pragma unroll
+      for (int i = k + 1; i < NB; ++i) v -= L[i][k] * x[i];
+      x[k] = v * di[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+      if (k < ns) X[c + k] = x[k];
+  }
+  __syncthreads();
+  return X[lc];
+}
 
 // pointer-jumping FK: lane j ends with T_world_joint(j) (identity base)
 template <int HW>
@@ -177,10 +347,11 @@ __device__ __forceinline__ Pose wave_fk(const WaveJoint& jd, int nj, int nrounds
 }
 
 // tile = index of the wavefront's problem group: problems tile * (64 / HW) + threadIdx.x / HW
-template <int NP, int NQ, int HW>
+template <int NP, int NQ, int HW, bool SN = false>
 __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const LMParams& lp, int it0, int iters,
                                            char* lds_raw) {
   static_assert(HW == 64 || HW == 32, "HW lanes per problem: 64 or 32");
+  static_assert(!SN || HW == 32, "the supernodal solve runs two problems per wavefront");
   constexpr int RP = 6 * NP + 3 * NQ;
   const int slot = threadIdx.x / HW;
   WaveLDS<NP, NQ>* sh =
@@ -287,6 +458,13 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
   int nacc = it0 == 0 ? 0 : Pg.nacc[s];
   float cost = 0.f;
   Pose T = wave_fk<HW>(jd, nj, nrounds, q);
+  // supernodal solve: lane k < sn_nslot factors slot k (its first position and size)
+  int sn_c = 0, sn_n = 0;
+  if constexpr (SN) {
+    const int k = lane < SN_MAXSLOT ? lane : 0;
+    sn_c = Pg.terms->sn_start[k];
+    sn_n = lane < Pg.terms->sn_nslot ? Pg.terms->sn_size[k] : 0;
+  }
 
   // residual rows of the lane's target at world poses Tw (all lanes hold their joint's pose);
   // JAC also writes the pose target's Jacobian blocks / the position link's point to LDS
@@ -515,6 +693,10 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     float dsum = lam;  // this row's diagonal terms: limit, smoothness, then lam
     if (has_sm) dsum = ds + dsum;
     if (has_lim) dsum = dl + dsum;
+    float dq = 0.f;
+    if constexpr (SN) {
+      dq = sn_solve<NP, NQ>(sh, as_const(Pg.terms), lane, sn_c, sn_n, A, dsum, b);
+    } else {
 #pragma unroll
     for (int s4 = 0; s4 < WV_NA / 4; ++s4) {
       const int k0 = 4 * s4, buf = s4 & 1;
@@ -606,7 +788,7 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
       sh->dvs[lc] = dinv;
     }
     __syncthreads();
-    float c = y, dq = 0.f;
+    float c = y;
 #pragma unroll
     for (int s4 = WV_NA / 4 - 1; s4 >= 0; --s4) {
       const int k0 = 4 * s4, buf = s4 & 1;
@@ -632,6 +814,7 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
       dq = m == 3 ? x3 : dq;
       c -= e0 * x0 + e1 * x1 + e2 * x2 + e3 * x3;
     }
+    }  // dense
     if (pad) dq = 0.f;
     // predicted reduction 1/2 d.(lam d - g), summed over the dofs
     const float pr = wv_sum<HW>(pad ? 0.f : dq * (lam * dq - gv));
@@ -678,8 +861,8 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
   __syncthreads();
 }
 
-template <int NP, int NQ, int HW>
+template <int NP, int NQ, int HW, bool SN = false>
 __global__ __launch_bounds__(64) void k_wave(Problems P, LMParams lp, int it0, int iters) {
   __shared__ __align__(16) char lds[wave_lds_bytes<NP, NQ, HW>()];
-  wave_solve<NP, NQ, HW>(P, blockIdx.x, lp, it0, iters, lds);
+  wave_solve<NP, NQ, HW, SN>(P, blockIdx.x, lp, it0, iters, lds);
 }

@@ -21,6 +21,7 @@ RK_OK, RK_EINVAL, RK_EHIP, RK_ECAPACITY, RK_ENOMEM, RK_EBUSY = range(6)
 MODE_FUSED, MODE_PLAIN = 0, 1
 TPP_WAVE = -64  # one wavefront per problem, register rows
 TPP_HALF = -32  # two problems per wavefront (32 lanes each), register rows
+TPP_SUPER = -33  # as TPP_HALF, normal equations solved in block-arrow form (limbs side by side, then the root)
 TPP_TRAJ = -128  # a trajectory: two wavefronts per problem (group kernel times)
 FROM_SEED, CONTINUE, USE_GRAPH, ASYNC, NOWAIT = 0x100, 0x200, 0x1, 0x2, 0x400
 SE3_OPS = {"compose": (0, 7), "inverse": (1, 7), "log": (2, 6), "jlog": (3, 36), "adjoint": (4, 36), "apply": (5, 3)}

@@ -97,14 +97,72 @@ def test_ik_parity_serial(ctx, mode, tpp, robot):
     _check_parity(ctx, wl, mode, robot, tpp)
 
 
-@pytest.mark.parametrize("mode,tpp", [(N.MODE_FUSED, N.TPP_WAVE), (N.MODE_FUSED, N.TPP_HALF), (N.MODE_FUSED, 32),
-                                      (N.MODE_PLAIN, 0)])
+@pytest.mark.parametrize("mode,tpp", [(N.MODE_FUSED, N.TPP_WAVE), (N.MODE_FUSED, N.TPP_HALF), (N.MODE_FUSED, N.TPP_SUPER),
+                                      (N.MODE_FUSED, 32), (N.MODE_PLAIN, 0)])
 def test_ik_parity_humanoid(ctx, mode, tpp):
     spec = W.humanoid29(0)
     wl = W.ik_batch(spec, W.humanoid_ee_links(spec), 64, seed=12)
     # seeds near the target joints so that a share of the problems converges
     wl.q_seed = (wl.q_target + np.random.default_rng(1).normal(0, 0.3, wl.q_target.shape)).astype(np.float32)
     _check_parity(ctx, wl, mode, "humanoid", tpp)
+
+
+def _tripod(seed: int = 5):
+    """a 2-dof torso carrying three 3-dof arms: the arms' normal-equation blocks couple only
+    through the torso, so the supernodal form packs two arms into one slot (3 + 3 dofs) and the
+    third into another, with a 2-dof root"""
+    rng = np.random.default_rng(seed)
+    joints, links = [], ["base"]
+
+    def add(name, parent, child):
+        u = rng.normal(size=3)
+        ax = rng.normal(size=3)
+        joints.append({"name": name, "type": "revolute", "parent": parent, "child": child,
+                       "xyz": rng.uniform(0.1, 0.3) * u / np.linalg.norm(u), "rpy": (0.0, 0.0, 0.0),
+                       "axis": ax / np.linalg.norm(ax), "lower": -2.8, "upper": 2.8})
+        links.append(child)
+
+    add("torso1", "base", "torso_a")
+    add("torso2", "torso_a", "torso_b")
+    for a in range(3):
+        prev = "torso_b"
+        for k in range(3):
+            add(f"arm{a}_j{k}", prev, f"arm{a}_l{k}")
+            prev = f"arm{a}_l{k}"
+    from robokit_amd.robo.robot_spec import RobotSpec
+    spec = RobotSpec.from_joints("tripod", links, joints, "base")
+    return spec, [spec.link_names.index(f"arm{a}_l2") for a in range(3)]
+
+
+def test_supernodal_packed_slots_and_partial_root(ctx):
+    """the supernodal half-wave kernel on a tree whose block-arrow form has a slot holding two
+    chains and a root smaller than the kernel's maximum: shadow-checked against the oracle, same
+    classification"""
+    spec, links = _tripod()
+    wl = W.ik_batch(spec, links, 96, seed=13)
+    wl.q_seed = (wl.q_target + np.random.default_rng(2).normal(0, 0.3, wl.q_target.shape)).astype(np.float32)
+    _check_parity(ctx, wl, N.MODE_FUSED, "tripod", N.TPP_SUPER)
+
+
+def test_supernodal_refused_without_block_arrow_form(ctx):
+    """an 8-dof serial chain has no block-arrow form (one chain, longer than a slot's 7 dofs, and
+    nothing branches): threads_per_problem -33 is refused with ValueError, -32 runs"""
+    rng = np.random.default_rng(0)
+    joints, links = [], ["base"]
+    for k in range(8):
+        ax = rng.normal(size=3)
+        joints.append({"name": f"j{k}", "type": "revolute", "parent": links[-1], "child": f"l{k}",
+                       "xyz": (0.0, 0.0, 0.15), "rpy": (0.0, 0.0, 0.0), "axis": ax / np.linalg.norm(ax),
+                       "lower": -2.8, "upper": 2.8})
+        links.append(f"l{k}")
+    from robokit_amd.robo.robot_spec import RobotSpec
+    spec = RobotSpec.from_joints("chain8", links, joints, "base")
+    wl = W.ik_batch(spec, [spec.num_links - 1], 8, seed=1)
+    b = N.Batch(ctx, spec, _ik_terms(spec, wl.links), wl.size)
+    b.set(q=wl.q_seed, pose_targets=wl.targets)
+    with pytest.raises(ValueError):
+        b.solve(5, C0_PARAMS, tpp=N.TPP_SUPER)
+    b.solve(5, C0_PARAMS, tpp=N.TPP_HALF)
 
 
 def test_fused_and_plain_agree(ctx):
@@ -202,7 +260,8 @@ def mb900():
     return W.config1(seed=2, total=900)
 
 
-@pytest.mark.parametrize("tpp", [(1, 1, N.TPP_HALF), (1, 1, N.TPP_WAVE), (N.TPP_WAVE, N.TPP_HALF, N.TPP_HALF),
+@pytest.mark.parametrize("tpp", [(1, 1, N.TPP_HALF), (1, 1, N.TPP_SUPER), (1, 1, N.TPP_WAVE),
+                                 (N.TPP_WAVE, N.TPP_HALF, N.TPP_HALF), (N.TPP_SUPER, N.TPP_SUPER, N.TPP_HALF),
                                  (1, 1, 32)])
 def test_group_single_launch_matches_per_variant(ctx, mb900, tpp):
     """a graph solve of register-resident variants runs as one launch (block ranges per variant);
@@ -263,7 +322,7 @@ def test_autotune_keeps_results(ctx):
     g = N.Group(ctx, batches)
     chosen = g.autotune(5, C0_PARAMS)
     print(f"\nautotune chose {chosen}")
-    assert len(chosen) == 3 and all(c in (1, N.TPP_WAVE, N.TPP_HALF, 4, 8, 16, 32, 64) for c in chosen)
+    assert len(chosen) == 3 and all(c in (1, N.TPP_WAVE, N.TPP_HALF, N.TPP_SUPER, 4, 8, 16, 32, 64) for c in chosen)
     g.solve(10, C0_PARAMS, use_graph=True)
     tuned = [b.get()["q"] for b in batches]
     for bk, b in zip(mb.buckets, batches):
@@ -295,8 +354,8 @@ def _oracle_retarget_terms(spec, links, targets, q_prev):
             O.Term(O.SMOOTH, (), None, np.full(na, RT_SMOOTH_W, np.float32), q_prev)]
 
 
-@pytest.mark.parametrize("mode,tpp", [(N.MODE_FUSED, N.TPP_WAVE), (N.MODE_FUSED, N.TPP_HALF), (N.MODE_FUSED, 64),
-                                      (N.MODE_PLAIN, 0)])
+@pytest.mark.parametrize("mode,tpp", [(N.MODE_FUSED, N.TPP_WAVE), (N.MODE_FUSED, N.TPP_HALF), (N.MODE_FUSED, N.TPP_SUPER),
+                                      (N.MODE_FUSED, 64), (N.MODE_PLAIN, 0)])
 def test_retarget_parity(ctx, mode, tpp):
     """12 link-position targets + smoothness to the previous frame + limits on the 29-dof tree:
     frames t of a smooth clip, each seeded (and smoothed) at the clip's frame t-1 (+ noise)."""
