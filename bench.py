@@ -73,11 +73,13 @@ def variant_name(tpp: int, spec, n_pos: int) -> str:
         return "V_HS012" if n_pos else "V_HS40"
     if tpp == 1:
         return f"V_T{spec.num_actuated_joints}"
+    if tpp == -4:
+        return f"V_Q{spec.num_actuated_joints}"
     return f"G{tpp}"
 
 
 # register-resident variants: a group of them runs as one k_persistent_reg launch
-MIXABLE = {"V_T5", "V_T6", "V_T7", "V_W40", "V_W012", "V_H40", "V_H012", "V_HS40", "V_HS012"}
+MIXABLE = {"V_T5", "V_T6", "V_T7", "V_W40", "V_W012", "V_H40", "V_H012", "V_HS40", "V_HS012", "V_Q6", "V_Q7"}
 
 
 def profiled_traffic(kernel: str):

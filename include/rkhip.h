@@ -225,6 +225,8 @@ int rk_batch_set(rk_batch* b, int32_t B, const float* q, const float* pose_targe
 #define RK_TPP_WAVE (-64)
 #define RK_TPP_HALF (-32)
 #define RK_TPP_SUPER (-33)
+#define RK_TPP_QUAD (-4)   /* four lanes per problem, register-resident: 6- and 7-dof serial arms with one
+                            * pose target (forward kinematics by a scan over the quad) */
 #define RK_TPP_TRAJ (-128) /* a trajectory: two wavefronts per problem (rk_group_kernel_times, autotune) */
 #define RK_USE_GRAPH 0x1   /* rk_group_solve flag: capture the launch sequence in a hipGraph and replay it */
 #define RK_ASYNC 0x2       /* rk_group_solve flag: return once the solve is enqueued on the context's stream

@@ -881,6 +881,7 @@ __global__ __launch_bounds__(64) void k_fused(Problems P, LMParams lp, int it0, 
 #include "rk_scene.h"
 #include "rk_thread.h"
 #include "rk_wave.h"
+#include "rk_quad.h"
 #include "rk_traj.h"
 
 // kernel variants: (threads per problem, data placement)
@@ -892,8 +893,9 @@ __global__ __launch_bounds__(64) void k_fused(Problems P, LMParams lp, int it0, 
 //   V_TC*/V_PC*: as V_T* / V_P* (6 or 7 dofs), with SceneCollisionTask rows
 enum Variant {
   V_S4 = 0, V_S8, V_S16, V_L16, V_L32, V_L64, V_T5, V_T6, V_T7, V_T8, V_W40, V_W012, V_H40, V_H012,
-  V_P5, V_P6, V_P7, V_P8, V_TC6, V_TC7, V_PC6, V_PC7, V_HS40, V_HS012, NVARIANT
+  V_P5, V_P6, V_P7, V_P8, V_TC6, V_TC7, V_PC6, V_PC7, V_HS40, V_HS012, V_Q6, V_Q7, NVARIANT
 };
+__host__ __device__ constexpr bool variant_quad(int v) { return v == V_Q6 || v == V_Q7; }
 __host__ __device__ constexpr bool variant_wave(int v) { return v == V_W40 || v == V_W012; }
 __host__ __device__ constexpr bool variant_super(int v) { return v == V_HS40 || v == V_HS012; }
 __host__ __device__ constexpr bool variant_half(int v) { return v == V_H40 || v == V_H012 || variant_super(v); }
@@ -902,6 +904,7 @@ __host__ __device__ constexpr bool variant_thread(int v) {
 }
 __host__ __device__ constexpr int variant_G(int v) {
   return variant_thread(v) ? 1
+         : variant_quad(v) ? 4
          : variant_wave(v) ? 64
          : variant_half(v) ? 32
          : v == V_S4       ? 4
@@ -916,7 +919,7 @@ __host__ __device__ constexpr bool variant_small(int v) { return v <= V_S16; }
 __host__ __device__ constexpr int variant_ppw(int v) { return variant_thread(v) ? WAVE : WAVE / variant_G(v); }
 // LDS bytes per workgroup of a variant
 __host__ __device__ constexpr size_t variant_lds(int v) {
-  return variant_thread(v) ? 0
+  return variant_thread(v) || variant_quad(v) ? 0
          : v == V_W40      ? wave_lds_bytes<4, 0, 64>()
          : v == V_W012     ? wave_lds_bytes<0, 12, 64>()
          : v == V_H40 || v == V_HS40   ? wave_lds_bytes<4, 0, 32>()
@@ -959,6 +962,8 @@ __device__ __forceinline__ void run_tile(const Bucket* bk, int local, const LMPa
   else if constexpr (V == V_W40) wave_solve<4, 0, 64>(P, local, lp, it0, iters, lds);
   else if constexpr (V == V_W012) wave_solve<0, 12, 64>(P, local, lp, it0, iters, lds);
   else if constexpr (V == V_H40) wave_solve<4, 0, 32>(P, local, lp, it0, iters, lds);
+  else if constexpr (V == V_Q6) quad_solve<6, 8>(P, local, lp, it0, iters);
+  else if constexpr (V == V_Q7) quad_solve<7, 9>(P, local, lp, it0, iters);
   else if constexpr (V == V_HS40) wave_solve<4, 0, 32, true>(P, local, lp, it0, iters, lds);
   else if constexpr (V == V_HS012) wave_solve<0, 12, 32, true>(P, local, lp, it0, iters, lds);
   else wave_solve<0, 12, 32>(P, local, lp, it0, iters, lds);
@@ -972,7 +977,9 @@ __device__ __forceinline__ void run_tile(const Bucket* bk, int local, const LMPa
 // (every other workgroup has taken its final tile) puts both back to zero for the next launch,
 // so no reset kernel precedes a launch.  Both start at zero when the group is created.
 // register budget of a variant's persistent kernel: waves per SIMD the allocation must allow
-__host__ __device__ constexpr int variant_waves_per_eu(int v) { return variant_wave(v) || variant_half(v) ? 2 : 1; }
+__host__ __device__ constexpr int variant_waves_per_eu(int v) {
+  return variant_wave(v) || variant_half(v) || variant_quad(v) ? 2 : 1;
+}
 
 // the tile loop of one workgroup of variant V (nblocks workgroups serve the variant's queue)
 template <int V>
@@ -1010,7 +1017,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(variant_wave
 // kernel per variant on forked streams (the fork, the join and the streams' start skew cost
 // ~50 us per solve on the uneven benchmark).
 __host__ __device__ constexpr bool variant_mixable(int v) {
-  return (v >= V_T5 && v <= V_T7) || variant_wave(v) || variant_half(v);
+  return (v >= V_T5 && v <= V_T7) || variant_wave(v) || variant_half(v) || variant_quad(v);
 }
 constexpr size_t mix_lds() {
   size_t m = 16;
@@ -1041,6 +1048,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     case V_W012: persist_loop<V_W012>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
     case V_H40: persist_loop<V_H40>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
     case V_HS40: persist_loop<V_HS40>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
+    case V_Q6: persist_loop<V_Q6>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
+    case V_Q7: persist_loop<V_Q7>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
     case V_HS012: persist_loop<V_HS012>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
     default: persist_loop<V_H012>(bk, nb, tiles, ctr, nbl, lp, it0, iters, lds, tile_s); break;
   }
@@ -1820,6 +1829,12 @@ static int wave_kind(const Terms& th, const Robot& r, int hw) {
   return -1;
 }
 
+// quad kernels: 6- and 7-dof serial arms with one pose target (no position / rotation targets, no
+// collision rows)
+static bool fits_quad(const Terms& th, const Robot& r) {
+  return fits_thread(th, r) && (r.na == 6 || r.na == 7) && th.n_pose == 1 && th.nc == 0;
+}
+
 // supernodal half-wave kernels: the half-wave kinds whose normal equations have the block-arrow
 // form (sn_structure found one)
 static int super_kind(const Terms& th, const Robot& r) {
@@ -1869,6 +1884,10 @@ static int pick_variant(const rk_context* ctx, const Terms& th, int G, int* v) {
     *v = super_kind(th, r);
     return RK_OK;
   }
+  if (G == RK_TPP_QUAD && fits_quad(th, r)) {
+    *v = r.na == 6 ? V_Q6 : V_Q7;
+    return RK_OK;
+  }
   if (small && (G == 4 || G == 8 || G == 16)) {
     *v = G == 4 ? V_S4 : G == 8 ? V_S8 : V_S16;
     return RK_OK;
@@ -1878,9 +1897,10 @@ static int pick_variant(const rk_context* ctx, const Terms& th, int G, int* v) {
     return RK_OK;
   }
   return fail(RK_EINVAL,
-              "threads_per_problem must be 1 (serial arms), -64 / -32 (register wave / half-wave: trees), -33 "
-              "(supernodal half-wave: trees whose normal equations have the block-arrow form), 4, 8, 16 (small "
-              "robots) or 16, 32, 64");
+              "threads_per_problem must be 1 (serial arms), -4 (a quad of lanes per problem: 6- and 7-dof serial "
+              "arms with one pose target), -64 / -32 (register wave / half-wave: trees), -33 (supernodal "
+              "half-wave: trees whose normal equations have the block-arrow form), 4, 8, 16 (small robots) or "
+              "16, 32, 64");
 }
 static int pick_variant(const rk_batch* b, int G, int* v) { return pick_variant(b->ctx, b->th, G, v); }
 
@@ -1891,6 +1911,7 @@ static void legal_tpps(const rk_batch* b, std::vector<int>& out) {
   if (wave_kind(b->th, r, 64) >= 0) out.push_back(RK_TPP_WAVE);
   if (wave_kind(b->th, r, 32) >= 0) out.push_back(RK_TPP_HALF);
   if (super_kind(b->th, r) >= 0) out.push_back(RK_TPP_SUPER);
+  if (fits_quad(b->th, r)) out.push_back(RK_TPP_QUAD);
   if (fits_small(b->th, r)) out.insert(out.end(), {4, 8, 16, 32, 64});
   else out.insert(out.end(), {16, 32, 64});
 }
@@ -1941,6 +1962,8 @@ static int launch_variant(hipStream_t st, int v, const Problems& P, const LMPara
     case V_HS40:
       hipLaunchKernelGGL((k_wave<4, 0, 32, true>), dim3((P.n + 1) / 2), dim3(WAVE), 0, st, P, lp, it0, iters);
       break;
+    case V_Q6: hipLaunchKernelGGL((k_quad<6, 8>), dim3((P.n + 15) / 16), dim3(WAVE), 0, st, P, lp, it0, iters); break;
+    case V_Q7: hipLaunchKernelGGL((k_quad<7, 9>), dim3((P.n + 15) / 16), dim3(WAVE), 0, st, P, lp, it0, iters); break;
     case V_HS012:
       hipLaunchKernelGGL((k_wave<0, 12, 32, true>), dim3((P.n + 1) / 2), dim3(WAVE), 0, st, P, lp, it0, iters);
       break;
@@ -3322,6 +3345,8 @@ static int variant_grid_any(rk_context* ctx, int v, int wg_per_cu, size_t lds, i
     case V_W012: return variant_grid<V_W012>(ctx, wg_per_cu, lds, tiles, grid);
     case V_H40: return variant_grid<V_H40>(ctx, wg_per_cu, lds, tiles, grid);
     case V_HS40: return variant_grid<V_HS40>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_Q6: return variant_grid<V_Q6>(ctx, wg_per_cu, lds, tiles, grid);
+    case V_Q7: return variant_grid<V_Q7>(ctx, wg_per_cu, lds, tiles, grid);
     case V_HS012: return variant_grid<V_HS012>(ctx, wg_per_cu, lds, tiles, grid);
     case V_P5: return variant_grid<V_P5>(ctx, wg_per_cu, lds, tiles, grid);
     case V_P6: return variant_grid<V_P6>(ctx, wg_per_cu, lds, tiles, grid);
@@ -3356,6 +3381,8 @@ static void launch_persistent(int v, dim3 grid, size_t lds, hipStream_t st, cons
     RK_LAUNCH_P(V_H012)
     RK_LAUNCH_P(V_HS40)
     RK_LAUNCH_P(V_HS012)
+    RK_LAUNCH_P(V_Q6)
+    RK_LAUNCH_P(V_Q7)
     RK_LAUNCH_P(V_P5)
     RK_LAUNCH_P(V_P6)
     RK_LAUNCH_P(V_P7)
@@ -3645,6 +3672,7 @@ int rk_group_kernel_times(rk_group* g, int32_t cap, int32_t* tpp, float* ms, int
       tpp[k] = v >= RK_VAR_TRAJ ? RK_TPP_TRAJ
                : variant_wave(v) ? RK_TPP_WAVE
                : variant_super(v) ? RK_TPP_SUPER
+               : variant_quad(v) ? RK_TPP_QUAD
                : variant_half(v) ? RK_TPP_HALF
                                  : variant_G(v);
     if (ms) HIP_TRY(hipEventElapsedTime(&ms[k], g->ev_k0[k], g->ev_k1[k]));

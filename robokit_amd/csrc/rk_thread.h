@@ -222,12 +222,235 @@ __device__ __forceinline__ void coll_row(TermsK* tm, int k, int j, const Pose& T
   }
 }
 
+// The linearisation of a serial-arm problem at qv: FK with the motion subspace, the residual rows,
+// the Jacobian (target rows: FrameTask -w jlog Ad(T_f^-1) S; PositionTask -w (v + w x p_link),
+// position_task.py:253-262; RotationTask w * the column's angular part, rotation_task.py:352-401;
+// zero for joints after the target link's joint), J^T J (lower triangle, per-term partial sums,
+// rows in order), J^T r; returns the cost.
+template <int NA, int NJ, bool PR>
+__device__ __forceinline__ float chain_linearize(RobotK* rb, TermsK* tm, int nj, int jt, const Pose& tgt,
+                                                 const float* qv, const float* qprev, float (&H)[NA][NA],
+                                                 float (&g)[NA]) {
+  const int n_pose = PR ? 0 : tm->n_pose, n_pos = PR ? tm->n_pos : 0, n_rot = PR ? tm->n_rot : 0;
+  float S6[NA][6];
+  const Pose Tf = chain_fk<NA, NJ, true>(rb, nj, qv, jt, S6);
+  float r_pose[6], r_lim[NA], r_sm[NA], jl[NA], js[NA], tg[36];
+  chain_rows<NA, true, PR>(rb, tm, Tf, tgt, qv, qprev, r_pose, r_lim, r_sm, tg, jl, js);
+  float J[6][NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    float c6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (a <= jt) {
+      if constexpr (!PR) {
+        if (n_pose) pose_jac_column(tg, S6[a], c6);
+      } else {
+        if (n_pos) {
+          const float w3[3] = {tm->pos_w[0][0], tm->pos_w[0][1], tm->pos_w[0][2]};
+          pos_jac_column(S6[a], tg, w3, c6);
+        }
+        if (n_rot) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) c6[3 + i] = tm->rot_w[0][i] * S6[a][3 + i];
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) J[i][a] = c6[i];
+  }
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      float h = 0.f;
+      if (!PR && n_pose && i <= jt) {
+        float sp = 0.f;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) sp += J[r][i] * J[r][j];
+        h = sp;
+      } else if (PR && i <= jt) {  // position term, then rotation term
+        if (n_pos) {
+          float sp = 0.f;
+#pragma unroll
+          for (int r = 0; r < 3; ++r) sp += J[r][i] * J[r][j];
+          h = sp;
+        }
+        if (n_rot) {
+          float sr = 0.f;
+#pragma unroll
+          for (int r = 3; r < 6; ++r) sr += J[r][i] * J[r][j];
+          h = h + sr;
+        }
+      }
+      if (i == j) {
+        if (tm->limit) h = h + jl[i] * jl[i];
+        if (tm->smooth) h = h + js[i] * js[i];
+      }
+      H[i][j] = h;
+    }
+    float gv = 0.f;
+    if (!PR && n_pose && i <= jt) {
+      float sp = 0.f;
+#pragma unroll
+      for (int r = 0; r < 6; ++r) sp += J[r][i] * r_pose[r];
+      gv = sp;
+    } else if (PR && i <= jt) {
+      if (n_pos) {
+        float sp = 0.f;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) sp += J[r][i] * r_pose[r];
+        gv = sp;
+      }
+      if (n_rot) {
+        float sr = 0.f;
+#pragma unroll
+        for (int r = 3; r < 6; ++r) sr += J[r][i] * r_pose[r];
+        gv = gv + sr;
+      }
+    }
+    if (tm->limit) gv = gv + jl[i] * r_lim[i];
+    if (tm->smooth) gv = gv + js[i] * r_sm[i];
+    g[i] = gv;
+  }
+  return chain_cost<NA, PR>(tm, r_pose, r_lim, r_sm);
+}
+
+// damped Cholesky (left-looking) of H + lam I into L and the solve (H + lam I) dq = -g; returns the
+// predicted reduction 1/2 dq.(lam dq - g).  Each pivot's reciprocal square root is the hardware's
+// (v_rsq_f32, 1 ulp) and the column scales by it: a correctly rounded division is a dependent chain
+// of ~10 instructions, and these kernels are bound by dependency latency.
+template <int NA>
+__device__ __forceinline__ float chain_step(const float (&H)[NA][NA], const float (&g)[NA], float lam, float (&dq)[NA]) {
+  float L[NA][NA], dinv[NA];
+#pragma unroll
+  for (int j = 0; j < NA; ++j) {
+    float sd = H[j][j] + lam;
+#pragma unroll
+    for (int k = 0; k < j; ++k) sd -= L[j][k] * L[j][k];
+    dinv[j] = __builtin_amdgcn_rsqf(sd);  // the pivot itself is never read: only 1 / L_jj
+#pragma unroll
+    for (int i = j + 1; i < NA; ++i) {
+      float v = H[i][j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+      L[i][j] = v * dinv[j];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+    float v = -g[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) v -= L[i][k] * dq[k];
+    dq[i] = v * dinv[i];
+  }
+#pragma unroll
+  for (int i = NA - 1; i >= 0; --i) {
+    float v = dq[i];
+#pragma unroll
+    for (int k = i + 1; k < NA; ++k) v -= L[k][i] * dq[k];
+    dq[i] = v * dinv[i];
+  }
+  float pr = 0.f;
+#pragma unroll
+  for (int i = 0; i < NA; ++i) pr += dq[i] * (lam * dq[i] - g[i]);
+  return 0.5f * pr;
+}
+
+// One serial-arm problem per lane without collision rows.  An accepted proposal's FK is the next
+// iterate's, so the problem is linearised once per iteration, at the proposal: its cost decides
+// the step, and its J^T J, J^T r and cost replace the iterate's when the step is accepted (a
+// rejected step keeps them: the iterate did not move).  One FK per iteration instead of two.
+template <int NA, int NJ, bool PR>
+__device__ __forceinline__ void thread_solve_lin(const Problems& Pg, int s_idx, const LMParams& lp, int it0,
+                                                 int iters) {
+  RobotK* rb = as_const(Pg.robot);
+  TermsK* tm = as_const(Pg.terms);
+  const int S = Pg.stride;
+  const bool valid = s_idx < Pg.n;
+  const int s = valid ? s_idx : Pg.n - 1;
+  const int nj = rb->nj;
+  const int n_pose = PR ? 0 : tm->n_pose, n_pos = PR ? tm->n_pos : 0, n_rot = PR ? tm->n_rot : 0;
+  const int jt = n_pose ? rb->link_joint[tm->pose_link[0]]
+                 : n_pos ? rb->link_joint[tm->pos_link[0]]
+                 : n_rot ? rb->link_joint[tm->rot_link[0]]
+                         : -1;
+  float q[NA], qprev[NA];
+#pragma unroll
+  for (int d = 0; d < NA; ++d) {
+    q[d] = (it0 == 0 && Pg.q_seed) ? Pg.q_seed[d * S + s] : Pg.q[d * S + s];
+    qprev[d] = tm->smooth ? Pg.q_prev[d * S + s] : 0.f;
+  }
+  Pose tgt = identity_pose();
+  if constexpr (!PR) {
+    if (n_pose) {
+      float v[7];
+#pragma unroll
+      for (int c = 0; c < 7; ++c) v[c] = Pg.pose_tgt[c * S + s];
+      load_pose(v, tgt);
+    }
+  } else {
+    if (n_pos) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) tgt.t[c] = Pg.pos_tgt[c * S + s];
+    }
+    if (n_rot) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) tgt.q[c] = Pg.rot_tgt[c * S + s];
+    }
+  }
+  float lam = it0 == 0 ? lp.lm_lambda : Pg.lam[s];
+  int nacc = it0 == 0 ? 0 : Pg.nacc[s];
+  float H[NA][NA], g[NA];
+  float cost = iters > 0 ? chain_linearize<NA, NJ, PR>(rb, tm, nj, jt, tgt, q, qprev, H, g) : 0.f;
+  for (int it = 0; it < iters; ++it) {
+    const int gi = it0 + it;
+    if (valid && Pg.q_trace && gi < Pg.trace_iters) {
+#pragma unroll
+      for (int d = 0; d < NA; ++d) Pg.q_trace[(gi * NA + d) * S + s] = q[d];
+      Pg.lam_trace[gi * S + s] = lam;
+    }
+    float dq[NA];
+    const float pred = chain_step<NA>(H, g, lam, dq);
+    float qp[NA];
+#pragma unroll
+    for (int d = 0; d < NA; ++d) qp[d] = q[d] + dq[d];
+    float Hp[NA][NA], gp[NA];
+    const float cp = chain_linearize<NA, NJ, PR>(rb, tm, nj, jt, tgt, qp, qprev, Hp, gp);
+    if (valid && Pg.trace && gi == 0) Pg.trace[s] = cost;
+    const bool acc = lm_accept(lp, cost, cp, pred, lam);
+    if (acc) {
+      ++nacc;
+#pragma unroll
+      for (int d = 0; d < NA; ++d) {
+        q[d] = qp[d];
+        g[d] = gp[d];
+#pragma unroll
+        for (int e = 0; e <= d; ++e) H[d][e] = Hp[d][e];
+      }
+    }
+    if (valid && Pg.trace && gi < Pg.trace_iters) {
+      Pg.trace[(gi + 1) * S + s] = cost;
+      Pg.acc_trace[gi * S + s] = acc;
+    }
+  }
+  if (valid) {
+#pragma unroll
+    for (int d = 0; d < NA; ++d) Pg.q[d * S + s] = q[d];
+    if (iters > 0) Pg.cost[s] = cost;
+    Pg.lam[s] = lam;
+    Pg.nacc[s] = nacc;
+  }
+}
+
 // One serial-arm problem per lane: <= 1 pose target (PR false), or <= 1 position and <= 1 rotation
 // target on links of one parent joint (PR true); limit and smoothness rows; with CO the
 // SceneCollisionTask rows of the robot's spheres, evaluated as FK reaches each sphere's link.  The
 // forms are separate kernels so that each keeps its own register allocation.
 template <int NA, int NJ, bool PR = false, bool CO = false>
 __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, const LMParams& lp, int it0, int iters) {
+  if constexpr (!CO) {
+    thread_solve_lin<NA, NJ, PR>(Pg, s_idx, lp, it0, iters);
+    return;
+  }
   RobotK* rb = as_const(Pg.robot);
   TermsK* tm = as_const(Pg.terms);
   const int S = Pg.stride;

@@ -152,24 +152,25 @@ struct WaveJoint {
 // Supernodal solve of (J^T J + diag) dq = b for the block-arrow form of the elimination order
 // (two problems per wavefront; sn_structure builds the form).  With the slots' diagonal blocks
 // D_k, their coupling blocks E_k (root rows x slot columns) and the root block C:
-//   D_k = L_k L_k^T,  F_k = E_k L_k^-T,  y_k = L_k^-1 b_k                 (lane k, all slots at once)
-//   C' = C - sum_k F_k F_k^T,  b_r' = b_r - sum_k F_k y_k,  C' x_r = b_r'  (one lane)
-//   L_k^T x_k = y_k - F_k^T x_r                                          (lane k)
+//   D_k = L_k L_k^T,  F_k = E_k L_k^-T,  y_k = L_k^-1 b_k          (lane k < SN_MAXSLOT, all at once)
+//   C' = C - sum_k F_k F_k^T,  b_r' = b_r - sum_k F_k y_k           (sums over the slot lanes by DPP)
+//   C' x_r = b_r'                                                   (every slot lane, identically)
+//   L_k^T x_k = y_k - F_k^T x_r                                     (lane k)
 // which is the block form of the Cholesky factorisation in the same order.  Each slot lane holds
-// its blocks in registers (a slot of s < SN_NB dofs is padded with identity rows); rows of the
-// system go through LDS once.  Returns dq of the lane's dof (0 on padding lanes).  The pivots'
-// reciprocal square roots are the hardware's (v_rsq_f32, 1 ulp).
+// its blocks in registers (a slot of s < SN_NB dofs is padded with identity rows; lanes past the
+// slots hold an empty slot and add zeros); rows of the system go through LDS once, the solution
+// once.  Returns dq of the lane's dof (0 on padding lanes).  The pivots' reciprocal square roots
+// are the hardware's (v_rsq_f32, 1 ulp).
 template <int NP, int NQ>
 __device__ __forceinline__ float sn_solve(WaveLDS<NP, NQ>* sh, TermsK* tm, int lane, int c, int ns,
                                           const float (&A)[WV_NA], float dsum, float b) {
-  constexpr int LS = 36, NB = SN_NB, NR = SN_NR, NS = SN_NR * (SN_NR + 1) / 2, SB = 16;
-  static_assert(NS + NR <= SB, "slot scratch");
+  constexpr int LS = 36, NB = SN_NB, NR = SN_NR, NS = SN_NR * (SN_NR + 1) / 2;
+  static_assert(SN_MAXSLOT == 8, "the slot sums run over the eight lanes of a DPP half-row");
   float* const M = &sh->M[0][0];
   float* const dg = sh->cst[0];  // diagonal terms + lam per row
   float* const bv = sh->rhs[0];  // right-hand side per row
   float* const X = sh->dvs;      // solution per row
-  float* const Sb = reinterpret_cast<float*>(&sh->blk[0][0]);  // per slot: Schur block, then F y
-  const int nsl = tm->sn_nslot, r0 = tm->sn_r0, nr = tm->sn_nr;
+  const int r0 = tm->sn_r0, nr = tm->sn_nr;
   const int lc = lane & (WV_NA - 1);
 #pragma unroll
   for (int j = 0; j < WV_NA; j += 4)
@@ -177,11 +178,12 @@ __device__ __forceinline__ float sn_solve(WaveLDS<NP, NQ>* sh, TermsK* tm, int l
   dg[lc] = dsum;
   bv[lc] = b;
   __syncthreads();
-  float L[NB][NB], F[NR][NB], y[NB], di[NB];
-  const bool slot = ns > 0;
-  if (slot) {
+  // ---- slot blocks (every lane: lanes without a slot have ns = 0) and the root block
+  float L[NB][NB], F[NR][NB], y[NB], di[NB], C[NR][NR], r[NR];
+  {
     const float* Dp = M + c * (LS + 1);  // D[i][j] = Dp[i * LS + j]
     const float* Ep = M + r0 * LS + c;   // E[w][j] = Ep[w * LS + j]
+    const float* Cp = M + r0 * (LS + 1);  // C[w][v] = Cp[w * LS + v]
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
 #pragma unroll
@@ -193,124 +195,112 @@ __device__ __forceinline__ float sn_solve(WaveLDS<NP, NQ>* sh, TermsK* tm, int l
     for (int w = 0; w < NR; ++w)
 #pragma unroll
       for (int j = 0; j < NB; ++j) F[w][j] = (w < nr && j < ns) ? Ep[w * LS + j] : 0.f;
-    // left-looking Cholesky of D, with F = E L^-T and y = L^-1 b column by column
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      float d = L[k][k];
-#pragma unroll
-      for (int m = 0; m < k; ++m) d -= L[k][m] * L[k][m];
-      di[k] = __builtin_amdgcn_rsqf(d);
-#pragma unroll
-      for (int i = k + 1; i < NB; ++i) {
-        float v = L[i][k];
-#pragma unroll
-        for (int m = 0; m < k; ++m) v -= L[i][m] * L[k][m];
-        L[i][k] = v * di[k];
-      }
-#pragma unroll
-      for (int w = 0; w < NR; ++w) {
-        float v = F[w][k];
-#pragma unroll
-        for (int m = 0; m < k; ++m) v -= F[w][m] * L[k][m];
-        F[w][k] = v * di[k];
-      }
-      float v = y[k];
-#pragma unroll
-      for (int m = 0; m < k; ++m) v -= L[k][m] * y[m];
-      y[k] = v * di[k];
-    }
-    // this slot's share of the root's Schur complement and right-hand side
-    float* const so = Sb + lane * SB;
-#pragma unroll
-    for (int w = 0, e = 0; w < NR; ++w)
-#pragma unroll
-      for (int v = 0; v <= w; ++v, ++e) {
-        float acc = 0.f;
-#pragma unroll
-        for (int k = 0; k < NB; ++k) acc += F[w][k] * F[v][k];
-        so[e] = acc;
-      }
-#pragma unroll
-    for (int w = 0; w < NR; ++w) {
-      float acc = 0.f;
-#pragma unroll
-      for (int k = 0; k < NB; ++k) acc += F[w][k] * y[k];
-      so[NS + w] = acc;
-    }
-  }
-  __syncthreads();
-  // root: C' = C - sum of the slots' shares, factor and solve (one lane per problem)
-  if (lane == 0 && nr > 0) {
-    float C[NR][NR], r[NR];
 #pragma unroll
     for (int w = 0; w < NR; ++w) {
 #pragma unroll
-      for (int v = 0; v < w; ++v) C[w][v] = w < nr ? M[(r0 + w) * LS + r0 + v] : 0.f;
-      C[w][w] = w < nr ? M[(r0 + w) * LS + r0 + w] + dg[r0 + w] : 1.f;
+      for (int v = 0; v < w; ++v) C[w][v] = w < nr ? Cp[w * LS + v] : 0.f;
+      C[w][w] = w < nr ? Cp[w * LS + w] + dg[r0 + w] : 1.f;
       r[w] = w < nr ? bv[r0 + w] : 0.f;
     }
-    for (int k = 0; k < nsl; ++k) {
-      const float* si = Sb + k * SB;
+  }
+  // ---- left-looking Cholesky of D, with F = E L^-T and y = L^-1 b column by column
 #pragma unroll
-      for (int w = 0, e = 0; w < NR; ++w)
+  for (int k = 0; k < NB; ++k) {
+    float d = L[k][k];
 #pragma unroll
-        for (int v = 0; v <= w; ++v, ++e) C[w][v] -= si[e];
+    for (int m = 0; m < k; ++m) d -= L[k][m] * L[k][m];
+    di[k] = __builtin_amdgcn_rsqf(d);
 #pragma unroll
-      for (int w = 0; w < NR; ++w) r[w] -= si[NS + w];
-    }
-    float ci[NR];
+    for (int i = k + 1; i < NB; ++i) {
+      float v = L[i][k];
 #pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      float d = C[k][k];
-#pragma unroll
-      for (int m = 0; m < k; ++m) d -= C[k][m] * C[k][m];
-      ci[k] = __builtin_amdgcn_rsqf(d);
-#pragma unroll
-      for (int i = k + 1; i < NR; ++i) {
-        float v = C[i][k];
-#pragma unroll
-        for (int m = 0; m < k; ++m) v -= C[i][m] * C[k][m];
-        C[i][k] = v * ci[k];
-      }
-      float v = r[k];
-#pragma unroll
-      for (int m = 0; m < k; ++m) v -= C[k][m] * r[m];
-      r[k] = v * ci[k];
+      for (int m = 0; m < k; ++m) v -= L[i][m] * L[k][m];
+      L[i][k] = v * di[k];
     }
 #pragma unroll
-    for (int k = NR - 1; k >= 0; --k) {
-      float v = r[k];
+    for (int w = 0; w < NR; ++w) {
+      float v = F[w][k];
 #pragma unroll
-      for (int i = k + 1; i < NR; ++i) v -= C[i][k] * r[i];
-      r[k] = v * ci[k];
+      for (int m = 0; m < k; ++m) v -= F[w][m] * L[k][m];
+      F[w][k] = v * di[k];
     }
+    float v = y[k];
+#pragma unroll
+    for (int m = 0; m < k; ++m) v -= L[k][m] * y[m];
+    y[k] = v * di[k];
+  }
+  // ---- the root's Schur complement: this slot's share, summed over the eight slot lanes by a
+  // butterfly of DPP exchanges (pairs, quads, the 8-lane half row), which leaves the same total
+  // on each of them; every slot lane then factors and solves the root itself
+#pragma unroll
+  for (int w = 0; w < NR; ++w) {
+#pragma unroll
+    for (int v = 0; v <= w; ++v) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < NB; ++k) acc += F[w][k] * F[v][k];
+      acc += wv_dpp(acc, 0);
+      acc += wv_dpp(acc, 1);
+      acc += wv_dpp(acc, 2);
+      C[w][v] -= acc;
+    }
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) acc += F[w][k] * y[k];
+    acc += wv_dpp(acc, 0);
+    acc += wv_dpp(acc, 1);
+    acc += wv_dpp(acc, 2);
+    r[w] -= acc;
+  }
+  float ci[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    float d = C[k][k];
+#pragma unroll
+    for (int m = 0; m < k; ++m) d -= C[k][m] * C[k][m];
+    ci[k] = __builtin_amdgcn_rsqf(d);
+#pragma unroll
+    for (int i = k + 1; i < NR; ++i) {
+      float v = C[i][k];
+#pragma unroll
+      for (int m = 0; m < k; ++m) v -= C[i][m] * C[k][m];
+      C[i][k] = v * ci[k];
+    }
+    float v = r[k];
+#pragma unroll
+    for (int m = 0; m < k; ++m) v -= C[k][m] * r[m];
+    r[k] = v * ci[k];
+  }
+#pragma unroll
+  for (int k = NR - 1; k >= 0; --k) {
+    float v = r[k];
+#pragma unroll
+    for (int i = k + 1; i < NR; ++i) v -= C[i][k] * r[i];
+    r[k] = v * ci[k];
+  }
+  // ---- the slot's unknowns: L^T x = y - F^T x_r
+  float x[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    float v = y[k];
+#pragma unroll
+    for (int w = 0; w < NR; ++w) v -= F[w][k] * r[w];
+    x[k] = v;
+  }
+#pragma unroll
+  for (int k = NB - 1; k >= 0; --k) {
+    float v = x[k];
+#pragma unroll
+    for (int i = k + 1; i < NB; ++i) v -= L[i][k] * x[i];
+    x[k] = v * di[k];
+  }
+#pragma unroll
+  for (int k = 0; k < NB; ++k)
+    if (k < ns) X[c + k] = x[k];
+  if (lc == 0) {
 #pragma unroll
     for (int w = 0; w < NR; ++w)
       if (w < nr) X[r0 + w] = r[w];
-  }
-  __syncthreads();
-  if (slot) {
-    float xr[NR];
-#pragma unroll
-    for (int w = 0; w < NR; ++w) xr[w] = w < nr ? X[r0 + w] : 0.f;
-    float x[NB];
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      float v = y[k];
-#pragma unroll
-      for (int w = 0; w < NR; ++w) v -= F[w][k] * xr[w];
-      x[k] = v;
-    }
-#pragma unroll
-    for (int k = NB - 1; k >= 0; --k) {
-      float v = x[k];
-#pragma unroll
-      for (int i = k + 1; i < NB; ++i) v -= L[i][k] * x[i];
-      x[k] = v * di[k];
-    }
-#pragma unroll
-    for (int k = 0; k < NB; ++k)
-      if (k < ns) X[c + k] = x[k];
   }
   __syncthreads();
   return X[lc];

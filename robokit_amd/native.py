@@ -21,6 +21,7 @@ RK_OK, RK_EINVAL, RK_EHIP, RK_ECAPACITY, RK_ENOMEM, RK_EBUSY = range(6)
 MODE_FUSED, MODE_PLAIN = 0, 1
 TPP_WAVE = -64  # one wavefront per problem, register rows
 TPP_HALF = -32  # two problems per wavefront (32 lanes each), register rows
+TPP_QUAD = -4  # four lanes per problem, register rows (6- and 7-dof serial arms, one pose target)
 TPP_SUPER = -33  # as TPP_HALF, normal equations solved in block-arrow form (limbs side by side, then the root)
 TPP_TRAJ = -128  # a trajectory: two wavefronts per problem (group kernel times)
 FROM_SEED, CONTINUE, USE_GRAPH, ASYNC, NOWAIT = 0x100, 0x200, 0x1, 0x2, 0x400

@@ -89,7 +89,8 @@ def _check_classification(wl, gpu, ora, ok_o):
 
 # every kernel family: thread-per-problem registers (1), LDS-staged G lanes (8 small / 64 large),
 # wave-per-problem registers (-64), and the plain per-term launch
-@pytest.mark.parametrize("mode,tpp", [(N.MODE_FUSED, 1), (N.MODE_FUSED, 8), (N.MODE_FUSED, 64), (N.MODE_PLAIN, 0)])
+@pytest.mark.parametrize("mode,tpp", [(N.MODE_FUSED, 1), (N.MODE_FUSED, N.TPP_QUAD), (N.MODE_FUSED, 8),
+                                      (N.MODE_FUSED, 64), (N.MODE_PLAIN, 0)])
 @pytest.mark.parametrize("robot", ["panda_dh", "ur10"])
 def test_ik_parity_serial(ctx, mode, tpp, robot):
     spec = W.panda_dh() if robot == "panda_dh" else W.ur10()
@@ -178,6 +179,21 @@ def test_fused_and_plain_agree(ctx):
     assert np.sum(f["ok"] != p["ok"]) <= 1
 
 
+@pytest.mark.parametrize("robot", ["panda_dh", "ur10"])
+def test_quad_mate_independence(ctx, robot):
+    """16 problems share a wavefront (a quad of lanes each): a problem's result does not depend on
+    the others -- the same problems in reversed order and a count that leaves the last wavefront
+    partly empty give the same bits"""
+    spec = W.panda_dh() if robot == "panda_dh" else W.ur10()
+    wl = W.ik_batch(spec, [spec.num_links - 1], 99, seed=22)
+    h = _gpu_ik(ctx, wl, N.MODE_FUSED, tpp=N.TPP_QUAD, iters=20)
+    rev = W.ik_batch(spec, [spec.num_links - 1], 99, seed=22)
+    rev.q_seed, rev.targets = wl.q_seed[::-1].copy(), wl.targets[::-1].copy()
+    r = _gpu_ik(ctx, rev, N.MODE_FUSED, tpp=N.TPP_QUAD, iters=20)
+    assert np.array_equal(h["q"], r["q"][::-1]) and np.array_equal(h["cost_trace"], r["cost_trace"][::-1])
+    assert np.array_equal(h["accept"], r["accept"][::-1])
+
+
 def test_half_wave_mate_independence(ctx):
     """two problems share a wavefront (32 lanes each, one matrix-core block each): a problem's
     result does not depend on its wave-mate -- the same problems in reversed order (every pair
@@ -260,7 +276,8 @@ def mb900():
     return W.config1(seed=2, total=900)
 
 
-@pytest.mark.parametrize("tpp", [(1, 1, N.TPP_HALF), (1, 1, N.TPP_SUPER), (1, 1, N.TPP_WAVE),
+@pytest.mark.parametrize("tpp", [(1, 1, N.TPP_HALF), (1, 1, N.TPP_SUPER), (N.TPP_QUAD, N.TPP_QUAD, N.TPP_SUPER),
+                                 (N.TPP_QUAD, 1, N.TPP_HALF), (1, 1, N.TPP_WAVE),
                                  (N.TPP_WAVE, N.TPP_HALF, N.TPP_HALF), (N.TPP_SUPER, N.TPP_SUPER, N.TPP_HALF),
                                  (1, 1, 32)])
 def test_group_single_launch_matches_per_variant(ctx, mb900, tpp):
@@ -322,7 +339,7 @@ def test_autotune_keeps_results(ctx):
     g = N.Group(ctx, batches)
     chosen = g.autotune(5, C0_PARAMS)
     print(f"\nautotune chose {chosen}")
-    assert len(chosen) == 3 and all(c in (1, N.TPP_WAVE, N.TPP_HALF, N.TPP_SUPER, 4, 8, 16, 32, 64) for c in chosen)
+    assert len(chosen) == 3 and all(c in (1, N.TPP_QUAD, N.TPP_WAVE, N.TPP_HALF, N.TPP_SUPER, 4, 8, 16, 32, 64) for c in chosen)
     g.solve(10, C0_PARAMS, use_graph=True)
     tuned = [b.get()["q"] for b in batches]
     for bk, b in zip(mb.buckets, batches):
