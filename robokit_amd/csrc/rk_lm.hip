@@ -1858,6 +1858,7 @@ static int pick_variant(const rk_context* ctx, const Terms& th, int G, int* v) {
     if (it != ctx->tuned.end()) G = it->second;
     else
       G = fits_thread(th, r)               ? 1
+          : super_kind(th, r) >= 0         ? RK_TPP_SUPER
           : wave_kind(th, r, 32) >= 0      ? RK_TPP_HALF
           : wave_kind(th, r, 64) >= 0      ? RK_TPP_WAVE
           : small                          ? 8
@@ -3684,7 +3685,8 @@ int rk_group_kernel_times(rk_group* g, int32_t cap, int32_t* tpp, float* ms, int
 // one batch of a group: time every legal threads-per-problem code on `iters` iterations from the
 // batch's current q (restored after each trial and at the end, traces detached meanwhile); on any
 // exit the batch's q, trace pointers and the scratch copy are restored / freed
-static int autotune_batch(rk_group* g, rk_batch* b, int iters, const LMParams& lp, int* best_out) {
+static int autotune_batch(rk_group* g, rk_batch* b, int iters, const LMParams& lp, int* best_out,
+                          std::vector<std::pair<float, int>>* ranked) {
   rk_context* ctx = g->ctx;
   const size_t S = b->capacity, qbytes = sizeof(float) * b->na * S;
   float* q_save = nullptr;
@@ -3734,13 +3736,120 @@ static int autotune_batch(rk_group* g, rk_batch* b, int iters, const LMParams& l
       if (e != hipSuccess) return restore(fail(RK_EHIP, std::string("rk_group_autotune: ") + hipGetErrorString(e)));
       if (rep == 1) ms_sum += ms;
     }
+    ranked->push_back({ms_sum, G});
     if (ms_sum < best) {
       best = ms_sum;
       best_g = G;
     }
   }
   *best_out = best_g;
+  std::stable_sort(ranked->begin(), ranked->end());
   return restore(RK_OK);
+}
+
+// The whole group with the kernel codes tpp, launched as a solve launches it (one k_persistent_reg
+// launch when every variant is register-resident): HIP-event time of the second of two runs from
+// the batches' current q (restored before each run), traces detached.  A variant that is fastest
+// alone need not be fastest beside the others: the launch shares the chip, and a variant with many
+// short wavefronts can leave the resident slots to the long ones of another.
+static int autotune_group_time(rk_group* g, const std::vector<int>& tpp, int iters, const LMParams& lp,
+                               const std::vector<float*>& q_save, float* ms_out) {
+  rk_context* ctx = g->ctx;
+  for (size_t i = 0; i < g->batches.size(); ++i) g->tpp[i] = tpp[i];
+  GroupLaunch gl;
+  int rc;
+  if ((rc = group_prepare(g, &gl, false, g->buckets_dev, g->traj_dev))) return rc;
+  float ms = 0.f;
+  for (int rep = 0; rep < 2; ++rep) {
+    for (size_t i = 0; i < g->batches.size(); ++i) {
+      rk_batch* b = g->batches[i];
+      HIP_TRY(hipMemcpyAsync(b->q, q_save[i], sizeof(float) * b->na * b->capacity, hipMemcpyDeviceToDevice,
+                             ctx->stream));
+    }
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    if ((rc = group_launch(g, gl, iters, lp, false))) return rc;
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  }
+  *ms_out = ms;
+  return RK_OK;
+}
+
+// Group pass of the autotuner: the combinations of each IK batch's front-runners alone (codes
+// within 1.5x of its best, at most two per batch, the first four batches) timed as a whole group;
+// the fastest combination is kept.  State (q, trace pointers) is restored on every exit.
+static int autotune_group(rk_group* g, int iters, const LMParams& lp,
+                          const std::vector<std::vector<std::pair<float, int>>>& ranked) {
+  rk_context* ctx = g->ctx;
+  const size_t nb = g->batches.size();
+  std::vector<std::vector<int>> opts(nb);
+  size_t combos = 1;
+  for (size_t i = 0; i < nb; ++i) {
+    opts[i].push_back(g->tpp[i]);
+    if (g->batches[i]->traj_T || i >= 4) continue;
+    for (const auto& r : ranked[i])
+      if (r.second != g->tpp[i] && r.first <= 1.5f * ranked[i][0].first && opts[i].size() < 2)
+        opts[i].push_back(r.second);
+    combos *= opts[i].size();
+  }
+  if (combos <= 1) return RK_OK;
+  std::vector<float*> q_save(nb, nullptr);
+  std::vector<float*> tr(nb), qt(nb), lt(nb);
+  std::vector<int*> at(nb);
+  const std::vector<int> tpp0 = g->tpp;
+  auto cleanup = [&](int code) {
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess && code == RK_OK)
+      code = fail(RK_EHIP, "rk_group_autotune: stream error");
+    for (size_t i = 0; i < nb; ++i) {
+      rk_batch* b = g->batches[i];
+      if (q_save[i]) {
+        if (hipMemcpy(b->q, q_save[i], sizeof(float) * b->na * b->capacity, hipMemcpyDeviceToDevice) != hipSuccess &&
+            code == RK_OK)
+          code = fail(RK_EHIP, "rk_group_autotune: could not restore a batch's q");
+        dfree(ctx, q_save[i], sizeof(float) * b->na * b->capacity);
+      }
+      b->trace = tr[i];
+      b->acc_trace = at[i];
+      b->q_trace = qt[i];
+      b->lam_trace = lt[i];
+    }
+    if (code != RK_OK) g->tpp = tpp0;
+    return code;
+  };
+  int rc;
+  for (size_t i = 0; i < nb; ++i) {
+    rk_batch* b = g->batches[i];
+    tr[i] = b->trace;
+    at[i] = b->acc_trace;
+    qt[i] = b->q_trace;
+    lt[i] = b->lam_trace;
+    b->trace = nullptr;
+    b->acc_trace = nullptr;
+    b->q_trace = nullptr;
+    b->lam_trace = nullptr;
+    if ((rc = dmalloc(ctx, (void**)&q_save[i], sizeof(float) * b->na * b->capacity, nullptr))) return cleanup(rc);
+    if (hipMemcpyAsync(q_save[i], b->q, sizeof(float) * b->na * b->capacity, hipMemcpyDeviceToDevice, ctx->stream) !=
+        hipSuccess)
+      return cleanup(fail(RK_EHIP, "rk_group_autotune: could not save a batch's q"));
+  }
+  float best = 1e30f;
+  std::vector<int> best_tpp = tpp0, cur(nb);
+  for (size_t c = 0; c < combos; ++c) {
+    size_t k = c;
+    for (size_t i = 0; i < nb; ++i) {
+      cur[i] = opts[i][k % opts[i].size()];
+      k /= opts[i].size();
+    }
+    float ms = 0.f;
+    if ((rc = autotune_group_time(g, cur, iters, lp, q_save, &ms))) return cleanup(rc);
+    if (ms < best) {
+      best = ms;
+      best_tpp = cur;
+    }
+  }
+  g->tpp = best_tpp;
+  return cleanup(RK_OK);
 }
 
 int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t* chosen) {
@@ -3748,21 +3857,27 @@ int rk_group_autotune(rk_group* g, int32_t iters, const rk_lm_params* p, int32_t
   rk_context* ctx = g->ctx;
   const LMParams lp = to_lp(p);
   // each batch alone, each legal G; state is restored after every trial
+  std::vector<std::vector<std::pair<float, int>>> ranked(g->batches.size());
   for (size_t i = 0; i < g->batches.size(); ++i) {
     rk_batch* b = g->batches[i];
-    if (b->traj_T) {  // trajectories have one kernel form
-      if (chosen) chosen[i] = RK_TPP_TRAJ;
-      continue;
-    }
+    if (b->traj_T) continue;  // trajectories have one kernel form
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
     int best_g = 0;
-    const int rc = autotune_batch(g, b, iters, lp, &best_g);
+    const int rc = autotune_batch(g, b, iters, lp, &best_g, &ranked[i]);
     if (rc) return rc;
     g->tpp[i] = best_g;
     ctx->tuned[signature(b->th)] = best_g;
-    if (chosen) chosen[i] = best_g;
   }
+  // then the front-runners' combinations as a whole group
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int rc = autotune_group(g, iters, lp, ranked);
+    if (rc) return rc;
+  }
+  for (size_t i = 0; i < g->batches.size(); ++i)
+    if (chosen) chosen[i] = g->batches[i]->traj_T ? RK_TPP_TRAJ : g->tpp[i];
   if (g->graph) {
     hipGraphExecDestroy(g->graph);
     g->graph = nullptr;
