@@ -880,8 +880,8 @@ __global__ __launch_bounds__(64) void k_fused(Problems P, LMParams lp, int it0, 
 
 #include "rk_scene.h"
 #include "rk_thread.h"
-#include "rk_wave.h"
 #include "rk_quad.h"
+#include "rk_wave.h"
 #include "rk_traj.h"
 
 // kernel variants: (threads per problem, data placement)

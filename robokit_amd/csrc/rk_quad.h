@@ -1,5 +1,5 @@
 // rk_quad.h -- four lanes per problem LM for serial arms (included by rk_lm.hip inside namespace rk,
-// after rk_thread.h).
+// after rk_thread.h; its quad DPP helpers serve rk_wave.h too).
 //
 // The one-lane-per-problem kernel (rk_thread.h) does the least arithmetic per problem, but one
 // problem's iteration is a serial chain of ~3.7k instructions and 6554 problems fill only 103

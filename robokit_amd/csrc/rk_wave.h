@@ -149,52 +149,57 @@ struct WaveJoint {
   int type, parent, act, mimic;
 };
 
+// xor-butterfly exchange inside 32-lane groups (ds_swizzle, bit-mask mode): lane i reads lane i ^ X
+template <int X>
+__device__ __forceinline__ float sn_swz(float v) {
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1F | (X << 10)));
+}
+
 // Supernodal solve of (J^T J + diag) dq = b for the block-arrow form of the elimination order
 // (two problems per wavefront; sn_structure builds the form).  With the slots' diagonal blocks
 // D_k, their coupling blocks E_k (root rows x slot columns) and the root block C:
-//   D_k = L_k L_k^T,  F_k = E_k L_k^-T,  y_k = L_k^-1 b_k          (lane k < SN_MAXSLOT, all at once)
-//   C' = C - sum_k F_k F_k^T,  b_r' = b_r - sum_k F_k y_k           (sums over the slot lanes by DPP)
-//   C' x_r = b_r'                                                   (every slot lane, identically)
-//   L_k^T x_k = y_k - F_k^T x_r                                     (lane k)
-// which is the block form of the Cholesky factorisation in the same order.  Each slot lane holds
-// its blocks in registers (a slot of s < SN_NB dofs is padded with identity rows; lanes past the
-// slots hold an empty slot and add zeros); rows of the system go through LDS once, the solution
-// once.  Returns dq of the lane's dof (0 on padding lanes).  The pivots' reciprocal square roots
-// are the hardware's (v_rsq_f32, 1 ulp).
+//   X_k = D_k^-1 [E_k^T | b_k]                        (quad of lanes 4k..4k+3: column u each)
+//   C' = C - sum_k E_k X_k[:, 0:3],  b_r' = b_r - sum_k E_k X_k[:, 3]
+//   C' x_r = b_r'                                      (every lane, identically)
+//   x_k = X_k[:, 3] - X_k[:, 0:3] x_r                  (summed over the quad)
+// which is block elimination in the same order.  Each lane of a slot's quad factors D_k = L L^T
+// itself and solves its own right-hand side, so the quad's work runs side by side; the slots'
+// shares of the root system are summed by an xor butterfly over the eight quads (the same total
+// on every lane), the quad exchanges are DPP moves.  A slot of s < SN_NB dofs is padded with
+// identity rows; quads past the slots hold an empty slot and add zeros.  Rows of the system go
+// through LDS once, the solution once.  Returns dq of the lane's dof (0 on padding lanes).  The
+// pivots' reciprocal square roots are the hardware's (v_rsq_f32, 1 ulp).
 template <int NP, int NQ>
 __device__ __forceinline__ float sn_solve(WaveLDS<NP, NQ>* sh, TermsK* tm, int lane, int c, int ns,
                                           const float (&A)[WV_NA], float dsum, float b) {
-  constexpr int LS = 36, NB = SN_NB, NR = SN_NR, NS = SN_NR * (SN_NR + 1) / 2;
-  static_assert(SN_MAXSLOT == 8, "the slot sums run over the eight lanes of a DPP half-row");
+  constexpr int LS = 36, NB = SN_NB, NR = SN_NR;
+  static_assert(SN_MAXSLOT == 8 && SN_NR == 3, "eight quads of four lanes: three root columns and b");
   float* const M = &sh->M[0][0];
   float* const dg = sh->cst[0];  // diagonal terms + lam per row
   float* const bv = sh->rhs[0];  // right-hand side per row
   float* const X = sh->dvs;      // solution per row
   const int r0 = tm->sn_r0, nr = tm->sn_nr;
-  const int lc = lane & (WV_NA - 1);
+  const int lc = lane & (WV_NA - 1), u = lane & 3;
 #pragma unroll
   for (int j = 0; j < WV_NA; j += 4)
     *reinterpret_cast<float4*>(&M[lc * LS + j]) = make_float4(A[j], A[j + 1], A[j + 2], A[j + 3]);
   dg[lc] = dsum;
   bv[lc] = b;
   __syncthreads();
-  // ---- slot blocks (every lane: lanes without a slot have ns = 0) and the root block
-  float L[NB][NB], F[NR][NB], y[NB], di[NB], C[NR][NR], r[NR];
+  // ---- the slot's block, this lane's right-hand side (root row u of E, or b), the root block
+  float L[NB][NB], z[NB], C[NR][NR], r[NR];
   {
     const float* Dp = M + c * (LS + 1);  // D[i][j] = Dp[i * LS + j]
-    const float* Ep = M + r0 * LS + c;   // E[w][j] = Ep[w * LS + j]
+    const bool ru = u < NR && u < nr;
+    const float* Zp = u < NR ? M + (r0 + (u < NR ? u : 0)) * LS + c : bv + c;
     const float* Cp = M + r0 * (LS + 1);  // C[w][v] = Cp[w * LS + v]
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
 #pragma unroll
       for (int j = 0; j < i; ++j) L[i][j] = i < ns ? Dp[i * LS + j] : 0.f;
       L[i][i] = i < ns ? Dp[i * LS + i] + dg[c + i] : 1.f;
-      y[i] = i < ns ? bv[c + i] : 0.f;
+      z[i] = i < ns && (u == NR || ru) ? Zp[i] : 0.f;
     }
-#pragma unroll
-    for (int w = 0; w < NR; ++w)
-#pragma unroll
-      for (int j = 0; j < NB; ++j) F[w][j] = (w < nr && j < ns) ? Ep[w * LS + j] : 0.f;
 #pragma unroll
     for (int w = 0; w < NR; ++w) {
 #pragma unroll
@@ -203,7 +208,16 @@ __device__ __forceinline__ float sn_solve(WaveLDS<NP, NQ>* sh, TermsK* tm, int l
       r[w] = w < nr ? bv[r0 + w] : 0.f;
     }
   }
-  // ---- left-looking Cholesky of D, with F = E L^-T and y = L^-1 b column by column
+  // the root rows of E from the quad's lanes 0..2 (before the solve overwrites z)
+  float E[NR][NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    E[0][i] = qbcast<0>(z[i]);
+    E[1][i] = qbcast<1>(z[i]);
+    E[2][i] = qbcast<2>(z[i]);
+  }
+  // ---- left-looking Cholesky of D (every lane of the quad), then D x = z
+  float di[NB];
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
     float d = L[k][k];
@@ -217,40 +231,37 @@ __device__ __forceinline__ float sn_solve(WaveLDS<NP, NQ>* sh, TermsK* tm, int l
       for (int m = 0; m < k; ++m) v -= L[i][m] * L[k][m];
       L[i][k] = v * di[k];
     }
+    float v = z[k];
 #pragma unroll
-    for (int w = 0; w < NR; ++w) {
-      float v = F[w][k];
-#pragma unroll
-      for (int m = 0; m < k; ++m) v -= F[w][m] * L[k][m];
-      F[w][k] = v * di[k];
-    }
-    float v = y[k];
-#pragma unroll
-    for (int m = 0; m < k; ++m) v -= L[k][m] * y[m];
-    y[k] = v * di[k];
+    for (int m = 0; m < k; ++m) v -= L[k][m] * z[m];
+    z[k] = v * di[k];
   }
-  // ---- the root's Schur complement: this slot's share, summed over the eight slot lanes by a
-  // butterfly of DPP exchanges (pairs, quads, the 8-lane half row), which leaves the same total
-  // on each of them; every slot lane then factors and solves the root itself
+#pragma unroll
+  for (int k = NB - 1; k >= 0; --k) {
+    float v = z[k];
+#pragma unroll
+    for (int i = k + 1; i < NB; ++i) v -= L[i][k] * z[i];
+    z[k] = v * di[k];
+  }
+  // ---- this lane's column of the slot's share [E D^-1 E^T | E D^-1 b], summed over the slots
+  float col[NR];
 #pragma unroll
   for (int w = 0; w < NR; ++w) {
-#pragma unroll
-    for (int v = 0; v <= w; ++v) {
-      float acc = 0.f;
-#pragma unroll
-      for (int k = 0; k < NB; ++k) acc += F[w][k] * F[v][k];
-      acc += wv_dpp(acc, 0);
-      acc += wv_dpp(acc, 1);
-      acc += wv_dpp(acc, 2);
-      C[w][v] -= acc;
-    }
     float acc = 0.f;
 #pragma unroll
-    for (int k = 0; k < NB; ++k) acc += F[w][k] * y[k];
-    acc += wv_dpp(acc, 0);
-    acc += wv_dpp(acc, 1);
-    acc += wv_dpp(acc, 2);
-    r[w] -= acc;
+    for (int i = 0; i < NB; ++i) acc += E[w][i] * z[i];
+    acc += sn_swz<4>(acc);
+    acc += sn_swz<8>(acc);
+    acc += sn_swz<16>(acc);
+    col[w] = acc;
+  }
+  // root system from the quad's columns: C' = C - S (S[w][v] on lane v), b_r' = b_r - t (lane 3)
+#pragma unroll
+  for (int w = 0; w < NR; ++w) {
+    C[w][0] -= qbcast<0>(col[w]);
+    if (w >= 1) C[w][1] -= qbcast<1>(col[w]);
+    if (w >= 2) C[w][2] -= qbcast<2>(col[w]);
+    r[w] -= qbcast<3>(col[w]);
   }
   float ci[NR];
 #pragma unroll
@@ -278,25 +289,15 @@ __device__ __forceinline__ float sn_solve(WaveLDS<NP, NQ>* sh, TermsK* tm, int l
     for (int i = k + 1; i < NR; ++i) v -= C[i][k] * r[i];
     r[k] = v * ci[k];
   }
-  // ---- the slot's unknowns: L^T x = y - F^T x_r
-  float x[NB];
+  // ---- the slot's unknowns x = X[:, 3] - X[:, 0:3] x_r, summed over the quad
+  const float xr_u = u == 0 ? r[0] : u == 1 ? r[1] : r[2];
 #pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    float v = y[k];
-#pragma unroll
-    for (int w = 0; w < NR; ++w) v -= F[w][k] * r[w];
-    x[k] = v;
+  for (int i = 0; i < NB; ++i) {
+    float v = u == NR ? z[i] : -(xr_u * z[i]);
+    v += qperm<qp_ctrl(1, 0, 3, 2)>(v);
+    v += qperm<qp_ctrl(2, 3, 0, 1)>(v);
+    if (i < ns && (i & 3) == u) X[c + i] = v;
   }
-#pragma unroll
-  for (int k = NB - 1; k >= 0; --k) {
-    float v = x[k];
-#pragma unroll
-    for (int i = k + 1; i < NB; ++i) v -= L[i][k] * x[i];
-    x[k] = v * di[k];
-  }
-#pragma unroll
-  for (int k = 0; k < NB; ++k)
-    if (k < ns) X[c + k] = x[k];
   if (lc == 0) {
 #pragma unroll
     for (int w = 0; w < NR; ++w)
@@ -448,12 +449,12 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
   int nacc = it0 == 0 ? 0 : Pg.nacc[s];
   float cost = 0.f;
   Pose T = wave_fk<HW>(jd, nj, nrounds, q);
-  // supernodal solve: lane k < sn_nslot factors slot k (its first position and size)
+  // supernodal solve: the quad of lanes 4k..4k+3 factors slot k (its first position and size)
   int sn_c = 0, sn_n = 0;
-  if constexpr (SN) {
-    const int k = lane < SN_MAXSLOT ? lane : 0;
+  if constexpr (SN) {  // slot k = lane / 4 (a quad of lanes per slot)
+    const int k = (lane >> 2) & (SN_MAXSLOT - 1);
     sn_c = Pg.terms->sn_start[k];
-    sn_n = lane < Pg.terms->sn_nslot ? Pg.terms->sn_size[k] : 0;
+    sn_n = k < Pg.terms->sn_nslot ? Pg.terms->sn_size[k] : 0;
   }
 
   // residual rows of the lane's target at world poses Tw (all lanes hold their joint's pose);
