@@ -89,6 +89,17 @@ struct Terms {
   // sn_nslot = 0
   int sn_nslot, sn_r0, sn_nr;
   int sn_start[SN_MAXSLOT], sn_size[SN_MAXSLOT];
+  // (target, dof) pair layout of the pose-target Jacobian (supernodal kernels with pose targets):
+  // lane p of a problem forms the column of dof position pr_pos[p] for pose target pr_t[p] (-1:
+  // idle), the pr_h[p]-th target of its 16-lane block; a target's pairs are its slot's dofs, then
+  // all root dofs if it moves with any.  J^T J of each block's targets is one block-diagonal
+  // 16 x 16 matrix-core product.  Per slot: the lane of its first pair (-1: no target moves it)
+  // and of its target's first root pair (-1: none); per target moving the root: its first root
+  // pair.  pr_ok = 0: no such layout (each slot moved by at most one target, the pairs of each
+  // target in one block of 16 lanes).
+  int pr_ok, pr_kt, pr_nrc;
+  int pr_t[RK_NA_MAX], pr_pos[RK_NA_MAX], pr_h[RK_NA_MAX];
+  int sn_poff[SN_MAXSLOT], sn_proot[SN_MAXSLOT], pr_rc[RK_MAXT];
   // SceneCollisionTask rows (after the smoothness rows), one per robot sphere: r = w pen(sd) with
   // sd = scene distance of the sphere centre - radius; spheres sorted by the joint whose pose
   // moves their link (-1: the base link) so that FK meets them in order
@@ -112,10 +123,12 @@ struct Terms {
 // the symbolic factorisation in that order gives the structurally nonzero (row chunk, column
 // block) pairs.  The ordering only changes which exact zeros are skipped and the rounding
 // order, not the system solved.
-static void sn_structure(const Robot& r, Terms& th, const std::vector<unsigned>& adj0, std::vector<unsigned>& adj);
+static void sn_structure(const Robot& r, Terms& th, const std::vector<unsigned>& adj0, std::vector<unsigned>& adj,
+                         const std::vector<unsigned>& dep);
 static void wave_elimination(const Robot& r, Terms& th) {
   const int na = r.na;
   th.sn_nslot = th.sn_r0 = th.sn_nr = 0;
+  th.pr_ok = th.pr_kt = th.pr_nrc = 0;
   for (int s = 0; s < SN_MAXSLOT; ++s) th.sn_start[s] = th.sn_size[s] = 0;
   for (int k = 0; k < RK_NA_MAX; ++k) th.wperm[k] = th.wiperm[k] = k;
   for (int s4 = 0; s4 < RK_NA_MAX / 4; ++s4) th.wchol[s4] = ~0u;
@@ -149,7 +162,7 @@ static void wave_elimination(const Robot& r, Terms& th) {
     left &= ~(1u << best);
   }
   for (int k = 0; k < na; ++k) th.wiperm[th.wperm[k]] = k;
-  sn_structure(r, th, adj0, adj);
+  sn_structure(r, th, adj0, adj, dep);
   // filled pattern in the new order: row i has a nonzero in column k < i iff dofs wperm[i],
   // wperm[k] are adjacent in the filled graph (adj now holds it)
   for (int s4 = 0; s4 < RK_NA_MAX / 4; ++s4) {
@@ -189,7 +202,9 @@ static std::vector<unsigned> sn_fill(std::vector<unsigned> adj, const int* perm,
 // take this order (wperm) and adj becomes its filled graph; without it (a chain longer than
 // SN_NB, a root larger than SN_NR, more than SN_MAXSLOT slots) nothing changes and sn_nslot
 // stays 0: the supernodal kernels are not offered.
-static void sn_structure(const Robot& r, Terms& th, const std::vector<unsigned>& adj0, std::vector<unsigned>& adj) {
+static void sn_pairs(Terms& th, const std::vector<unsigned>& dep);
+static void sn_structure(const Robot& r, Terms& th, const std::vector<unsigned>& adj0, std::vector<unsigned>& adj,
+                         const std::vector<unsigned>& dep) {
   th.sn_nslot = th.sn_r0 = th.sn_nr = 0;
   for (int s = 0; s < SN_MAXSLOT; ++s) th.sn_start[s] = th.sn_size[s] = 0;
   const int na = r.na;
@@ -231,10 +246,12 @@ static void sn_structure(const Robot& r, Terms& th, const std::vector<unsigned>&
     return members[a].size() != members[b].size() ? members[a].size() > members[b].size() : a < b;
   });
   std::vector<std::vector<int>> slots;
+  const bool merge = (int)chains.size() > SN_MAXSLOT;  // one chain per slot while they fit
   for (int c : chains) {
     if ((int)members[c].size() > SN_NB) return;
     bool put = false;
-    for (auto& sl : slots)
+    if (merge)
+      for (auto& sl : slots)
       if (sl.size() + members[c].size() <= (size_t)SN_NB) {
         sl.insert(sl.end(), members[c].begin(), members[c].end());
         put = true;
@@ -259,6 +276,72 @@ static void sn_structure(const Robot& r, Terms& th, const std::vector<unsigned>&
     th.wiperm[perm[k]] = k;
   }
   adj = sn_fill(adj0, perm, na);
+  sn_pairs(th, dep);
+}
+
+// The (target, dof) pair layout of the pose targets over the block-arrow order (host, at batch
+// creation; dep[t]: bit a = target t's link moves with dof a).  Targets are packed first-fit by
+// decreasing pair count into the two 16-lane blocks of a problem.
+static void sn_pairs(Terms& th, const std::vector<unsigned>& dep) {
+  th.pr_ok = th.pr_kt = th.pr_nrc = 0;
+  for (int p = 0; p < RK_NA_MAX; ++p) th.pr_t[p] = -1, th.pr_pos[p] = 0, th.pr_h[p] = 0;
+  for (int k = 0; k < SN_MAXSLOT; ++k) th.sn_poff[k] = th.sn_proot[k] = -1;
+  for (int t = 0; t < RK_MAXT; ++t) th.pr_rc[t] = -1;
+  if (th.sn_nslot <= 0 || th.n_pose <= 0 || th.n_pos || th.n_rot || th.nc) return;
+  const int nt = th.n_pose;
+  std::vector<int> tslot(nt, -1), troot(nt, 0), tn(nt, 0), owner(th.sn_nslot, -1);
+  for (int t = 0; t < nt; ++t) {
+    for (int k = 0; k < th.sn_nslot; ++k)
+      for (int i = 0; i < th.sn_size[k]; ++i)
+        if ((dep[t] >> th.wperm[th.sn_start[k] + i]) & 1u) {
+          if (tslot[t] >= 0 && tslot[t] != k) return;  // a target moved by two slots
+          tslot[t] = k;
+        }
+    for (int w = 0; w < th.sn_nr; ++w)
+      if ((dep[t] >> th.wperm[th.sn_r0 + w]) & 1u) troot[t] = 1;
+    if (tslot[t] >= 0) {
+      if (owner[tslot[t]] >= 0) return;  // a slot moved by two targets
+      owner[tslot[t]] = t;
+    }
+    tn[t] = (tslot[t] >= 0 ? th.sn_size[tslot[t]] : 0) + (troot[t] ? th.sn_nr : 0);
+  }
+  std::vector<int> order(nt);
+  for (int t = 0; t < nt; ++t) order[t] = t;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tn[a] > tn[b]; });
+  int fill[2] = {0, 0}, kt[2] = {0, 0};
+  std::vector<int> tlane(nt, -1), th_(nt, 0);
+  for (int t : order) {
+    if (tn[t] == 0) continue;
+    int b = fill[0] + tn[t] <= 16 ? 0 : fill[1] + tn[t] <= 16 ? 1 : -1;
+    if (b < 0) return;
+    tlane[t] = 16 * b + fill[b];
+    th_[t] = kt[b]++;
+    fill[b] += tn[t];
+  }
+  for (int t = 0; t < nt; ++t) {
+    if (tlane[t] < 0) continue;
+    int p = tlane[t];
+    if (tslot[t] >= 0) {
+      const int k = tslot[t];
+      th.sn_poff[k] = p;
+      for (int i = 0; i < th.sn_size[k]; ++i, ++p) {
+        th.pr_t[p] = t;
+        th.pr_pos[p] = th.sn_start[k] + i;
+        th.pr_h[p] = th_[t];
+      }
+    }
+    if (troot[t]) {
+      if (tslot[t] >= 0) th.sn_proot[tslot[t]] = p;
+      th.pr_rc[th.pr_nrc++] = p;
+      for (int w = 0; w < th.sn_nr; ++w, ++p) {
+        th.pr_t[p] = t;
+        th.pr_pos[p] = th.sn_r0 + w;
+        th.pr_h[p] = th_[t];
+      }
+    }
+  }
+  th.pr_kt = std::max(kt[0], kt[1]);
+  th.pr_ok = 1;
 }
 
 struct LMParams {
@@ -1836,10 +1919,11 @@ static bool fits_quad(const Terms& th, const Robot& r) {
 }
 
 // supernodal half-wave kernels: the half-wave kinds whose normal equations have the block-arrow
-// form (sn_structure found one)
+// form (sn_structure found one); with pose targets also the (target, dof) pair layout (sn_pairs)
 static int super_kind(const Terms& th, const Robot& r) {
   const int k = wave_kind(th, r, 32);
   if (k < 0 || th.sn_nslot <= 0) return -1;
+  if (k == V_H40 && !th.pr_ok) return -1;  // pose targets: the pair layout is required
   return k == V_H40 ? V_HS40 : V_HS012;
 }
 

@@ -127,6 +127,12 @@ struct WaveLDS {
     // supernodal kernels: rows of J^T J (row stride 36); the rows past WV_NA only absorb the
     // masked reads of a slot's padding rows (a slot of s < SN_NB dofs reads SN_NB rows)
     alignas(16) float M[WV_NA + SN_NB + 1][36];
+    // supernodal kernels with pose targets: J^T J of the problem's two 16-lane pair blocks (row
+    // major, 16 x 16 each) and each pair's J^T r share; kept across rejected steps
+    struct {
+      alignas(16) float blk[2 * 16 * 16];
+      float gp[WV_NA];
+    } pr;
   };
 };
 
@@ -166,48 +172,14 @@ __device__ __forceinline__ float sn_swz(float v) {
 // itself and solves its own right-hand side, so the quad's work runs side by side; the slots'
 // shares of the root system are summed by an xor butterfly over the eight quads (the same total
 // on every lane), the quad exchanges are DPP moves.  A slot of s < SN_NB dofs is padded with
-// identity rows; quads past the slots hold an empty slot and add zeros.  Rows of the system go
-// through LDS once, the solution once.  Returns dq of the lane's dof (0 on padding lanes).  The
-// pivots' reciprocal square roots are the hardware's (v_rsq_f32, 1 ulp).
-template <int NP, int NQ>
-__device__ __forceinline__ float sn_solve(WaveLDS<NP, NQ>* sh, TermsK* tm, int lane, int c, int ns,
-                                          const float (&A)[WV_NA], float dsum, float b) {
-  constexpr int LS = 36, NB = SN_NB, NR = SN_NR;
+// identity rows; quads past the slots hold an empty slot and add zeros.  The solution goes to
+// X[position]; the pivots' reciprocal square roots are the hardware's (v_rsq_f32, 1 ulp).
+// In: L (the slot's block, lower), z (this lane's right-hand side: root row u of E, or b for
+// u = 3), C / r (the root block and right-hand side).
+__device__ __forceinline__ void sn_core(float (&L)[SN_NB][SN_NB], float (&z)[SN_NB], float (&C)[SN_NR][SN_NR],
+                                        float (&r)[SN_NR], int c, int ns, int lc, int u, int r0, int nr, float* X) {
+  constexpr int NB = SN_NB, NR = SN_NR;
   static_assert(SN_MAXSLOT == 8 && SN_NR == 3, "eight quads of four lanes: three root columns and b");
-  float* const M = &sh->M[0][0];
-  float* const dg = sh->cst[0];  // diagonal terms + lam per row
-  float* const bv = sh->rhs[0];  // right-hand side per row
-  float* const X = sh->dvs;      // solution per row
-  const int r0 = tm->sn_r0, nr = tm->sn_nr;
-  const int lc = lane & (WV_NA - 1), u = lane & 3;
-#pragma unroll
-  for (int j = 0; j < WV_NA; j += 4)
-    *reinterpret_cast<float4*>(&M[lc * LS + j]) = make_float4(A[j], A[j + 1], A[j + 2], A[j + 3]);
-  dg[lc] = dsum;
-  bv[lc] = b;
-  __syncthreads();
-  // ---- the slot's block, this lane's right-hand side (root row u of E, or b), the root block
-  float L[NB][NB], z[NB], C[NR][NR], r[NR];
-  {
-    const float* Dp = M + c * (LS + 1);  // D[i][j] = Dp[i * LS + j]
-    const bool ru = u < NR && u < nr;
-    const float* Zp = u < NR ? M + (r0 + (u < NR ? u : 0)) * LS + c : bv + c;
-    const float* Cp = M + r0 * (LS + 1);  // C[w][v] = Cp[w * LS + v]
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-#pragma unroll
-      for (int j = 0; j < i; ++j) L[i][j] = i < ns ? Dp[i * LS + j] : 0.f;
-      L[i][i] = i < ns ? Dp[i * LS + i] + dg[c + i] : 1.f;
-      z[i] = i < ns && (u == NR || ru) ? Zp[i] : 0.f;
-    }
-#pragma unroll
-    for (int w = 0; w < NR; ++w) {
-#pragma unroll
-      for (int v = 0; v < w; ++v) C[w][v] = w < nr ? Cp[w * LS + v] : 0.f;
-      C[w][w] = w < nr ? Cp[w * LS + w] + dg[r0 + w] : 1.f;
-      r[w] = w < nr ? bv[r0 + w] : 0.f;
-    }
-  }
   // the root rows of E from the quad's lanes 0..2 (before the solve overwrites z)
   float E[NR][NB];
 #pragma unroll
@@ -303,8 +275,126 @@ __device__ __forceinline__ float sn_solve(WaveLDS<NP, NQ>* sh, TermsK* tm, int l
     for (int w = 0; w < NR; ++w)
       if (w < nr) X[r0 + w] = r[w];
   }
+}
+
+// the supernodal solve from rows of J^T J staged in LDS (row `lane` of each lane: A), the lane's
+// diagonal terms + lam (dsum) and right-hand side b; returns dq of the lane's dof
+template <int NP, int NQ>
+__device__ __forceinline__ float sn_solve(WaveLDS<NP, NQ>* sh, TermsK* tm, int lane, int c, int ns,
+                                          const float (&A)[WV_NA], float dsum, float b) {
+  constexpr int LS = 36, NB = SN_NB, NR = SN_NR;
+  float* const M = &sh->M[0][0];
+  float* const dg = sh->cst[0];  // diagonal terms + lam per row
+  float* const bv = sh->rhs[0];  // right-hand side per row
+  const int r0 = tm->sn_r0, nr = tm->sn_nr;
+  const int lc = lane & (WV_NA - 1), u = lane & 3;
+#pragma unroll
+  for (int j = 0; j < WV_NA; j += 4)
+    *reinterpret_cast<float4*>(&M[lc * LS + j]) = make_float4(A[j], A[j + 1], A[j + 2], A[j + 3]);
+  dg[lc] = dsum;
+  bv[lc] = b;
   __syncthreads();
-  return X[lc];
+  float L[NB][NB], z[NB], C[NR][NR], r[NR];
+  {
+    const float* Dp = M + c * (LS + 1);  // D[i][j] = Dp[i * LS + j]
+    const bool ru = u < NR && u < nr;
+    const float* Zp = u < NR ? M + (r0 + (u < NR ? u : 0)) * LS + c : bv + c;
+    const float* Cp = M + r0 * (LS + 1);  // C[w][v] = Cp[w * LS + v]
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+#pragma unroll
+      for (int j = 0; j < i; ++j) L[i][j] = i < ns ? Dp[i * LS + j] : 0.f;
+      L[i][i] = i < ns ? Dp[i * LS + i] + dg[c + i] : 1.f;
+      z[i] = i < ns && (u == NR || ru) ? Zp[i] : 0.f;
+    }
+#pragma unroll
+    for (int w = 0; w < NR; ++w) {
+#pragma unroll
+      for (int v = 0; v < w; ++v) C[w][v] = w < nr ? Cp[w * LS + v] : 0.f;
+      C[w][w] = w < nr ? Cp[w * LS + w] + dg[r0 + w] : 1.f;
+      r[w] = w < nr ? bv[r0 + w] : 0.f;
+    }
+  }
+  sn_core(L, z, C, r, c, ns, lc, u, r0, nr, sh->dvs);
+  __syncthreads();
+  return sh->dvs[lc];
+}
+
+// The same solve from the pair layout: J^T J in the two 16 x 16 pair blocks (slot k's block at
+// its first pair p = sn_poff[k], its root coupling at its target's first root pair; the root
+// block summed over the targets that move the root), the pairs' J^T r shares, and per position
+// the diagonal terms + lam (dsum) and the limit / smoothness part of the right-hand side (b).
+// Returns dq of the lane's dof; g receives the position's full J^T r (for the predicted
+// reduction).
+template <int NP, int NQ>
+__device__ __forceinline__ float sn_solve_pairs(WaveLDS<NP, NQ>* sh, TermsK* tm, int lane, int c, int ns,
+                                                float dsum, float b, float& g) {
+  constexpr int NB = SN_NB, NR = SN_NR;
+  const float* const Bk = sh->pr.blk;
+  const float* const Gp = sh->pr.gp;
+  float* const dg = sh->cst[0];  // diagonal terms + lam per position
+  float* const bv = sh->rhs[0];  // limit / smoothness right-hand side per position
+  float* const Gf = sh->cst[1];  // full J^T r per position
+  const int r0 = tm->sn_r0, nr = tm->sn_nr;
+  const int lc = lane & (WV_NA - 1), u = lane & 3, k = (lc >> 2) & (SN_MAXSLOT - 1);
+  dg[lc] = dsum;
+  bv[lc] = b;
+  __syncthreads();
+  const int po = ns > 0 ? tm->sn_poff[k] : -1, pq = ns > 0 ? tm->sn_proot[k] : -1;
+  float L[NB][NB], z[NB], C[NR][NR], r[NR], bs[NB];
+  {
+    const int pb = po < 0 ? 0 : po;
+    const float* Dp = Bk + (pb >> 4) * 256 + (pb & 15) * 17;  // D[i][j] = Dp[i * 16 + j]
+    const int qb = pq < 0 ? 0 : pq;
+    const float* Ep = Bk + (qb >> 4) * 256 + (qb & 15) * 16 + (pb & 15);  // E[w][i] = Ep[w * 16 + i]
+    const bool ru = u < NR && u < nr && pq >= 0;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const bool live = i < ns && po >= 0;
+#pragma unroll
+      for (int j = 0; j < i; ++j) L[i][j] = live ? Dp[i * 16 + j] : 0.f;
+      L[i][i] = i < ns ? (live ? Dp[i * 16 + i] : 0.f) + dg[c + i] : 1.f;
+      bs[i] = i < ns ? bv[c + i] - (live ? Gp[pb + i] : 0.f) : 0.f;
+      const int w = u < NR ? u : 0;
+      z[i] = u == NR ? bs[i] : (i < ns && ru ? Ep[w * 16 + i] : 0.f);
+    }
+#pragma unroll
+    for (int w = 0; w < NR; ++w) {
+#pragma unroll
+      for (int v = 0; v <= w; ++v) C[w][v] = 0.f;
+      r[w] = 0.f;
+    }
+    for (int t = 0; t < tm->pr_nrc; ++t) {
+      const int rc = tm->pr_rc[t];
+      const float* Cp = Bk + (rc >> 4) * 256 + (rc & 15) * 17;
+#pragma unroll
+      for (int w = 0; w < NR; ++w) {
+#pragma unroll
+        for (int v = 0; v <= w; ++v) C[w][v] += w < nr ? Cp[w * 16 + v] : 0.f;
+        r[w] += w < nr ? Gp[rc + w] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < NR; ++w) {
+      C[w][w] = w < nr ? C[w][w] + dg[r0 + w] : 1.f;
+      r[w] = w < nr ? bv[r0 + w] - r[w] : 0.f;
+    }
+  }
+  // the full right-hand side per position (-J^T r): the quad's lane 3 for the slot, lane 0 the root
+  if (u == NR) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+      if (i < ns) Gf[c + i] = -bs[i];
+  }
+  if (lc == 0) {
+#pragma unroll
+    for (int w = 0; w < NR; ++w)
+      if (w < nr) Gf[r0 + w] = -r[w];
+  }
+  sn_core(L, z, C, r, c, ns, lc, u, r0, nr, sh->dvs);
+  __syncthreads();
+  g = Gf[lc];
+  return sh->dvs[lc];
 }
 
 // pointer-jumping FK: lane j ends with T_world_joint(j) (identity base)
@@ -343,6 +433,8 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
                                            char* lds_raw) {
   static_assert(HW == 64 || HW == 32, "HW lanes per problem: 64 or 32");
   static_assert(!SN || HW == 32, "the supernodal solve runs two problems per wavefront");
+  // supernodal with pose targets: the Jacobian in the (target, dof) pair layout
+  constexpr bool PM = SN && NQ == 0 && NP > 0;
   constexpr int RP = 6 * NP + 3 * NQ;
   const int slot = threadIdx.x / HW;
   WaveLDS<NP, NQ>* sh =
@@ -413,6 +505,31 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     in1 |= (unsigned)b1 << t;
     cmask[t] = __ballot(b0 || b1);
   }
+  // pair layout: this lane's (pose target pt, dof) pair -- the dof's joints that move the
+  // target's link (pj0 / pj1, -1: none) and their weights -- and its target's index in the block
+  int pt = -1, ph = 0, pj0 = -1, pj1 = -1;
+  float pw0 = 0.f, pw1 = 0.f;
+  if constexpr (PM) {
+    pt = Pg.terms->pr_t[lane];
+    if (pt >= 0) {
+      const Robot* rg = Pg.robot;
+      const int pdof = Pg.terms->wperm[Pg.terms->pr_pos[lane]];
+      ph = Pg.terms->pr_h[lane];
+      const unsigned long long anc = rg->anc[Pg.terms->pose_link[pt]];
+      const int c0 = rg->col_start[pdof], c1 = rg->col_start[pdof + 1];
+      if (c1 > c0 && ((anc >> rg->col_joint[c0]) & 1ull)) { pj0 = rg->col_joint[c0]; pw0 = rg->col_w[c0]; }
+      if (c1 > c0 + 1 && ((anc >> rg->col_joint[c0 + 1]) & 1ull)) {
+        pj1 = rg->col_joint[c0 + 1];
+        pw1 = rg->col_w[c0 + 1];
+      }
+    }
+  }
+  // both problems' LDS (the pair blocks' matrix-core results land on every lane of the wavefront)
+  WaveLDS<NP, NQ>* const sh0 = reinterpret_cast<WaveLDS<NP, NQ>*>(lds_raw);
+  WaveLDS<NP, NQ>* const sh1 =
+      reinterpret_cast<WaveLDS<NP, NQ>*>(reinterpret_cast<float*>(lds_raw) + wave_lds_stride<NP, NQ>());
+  (void)sh0;
+  (void)sh1;
 
   // ---- problem data
   float q = 0.f, qprev = 0.f;
@@ -526,10 +643,13 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
       if (lane >= nj) S6[0] = S6[1] = S6[2] = S6[3] = S6[4] = S6[5] = 0.f;
     }
     float Sc0[6], Sc1[6];
+    {
+      const int j0 = PM ? pj0 : cj0, j1 = PM ? pj1 : cj1;
 #pragma unroll
-    for (int r = 0; r < 6; ++r) {
-      Sc0[r] = __shfl(S6[r], cj0 < 0 ? 0 : cj0, HW);
-      Sc1[r] = __shfl(S6[r], cj1 < 0 ? 0 : cj1, HW);
+      for (int r = 0; r < 6; ++r) {
+        Sc0[r] = __shfl(S6[r], j0 < 0 ? 0 : j0, HW);
+        Sc1[r] = __shfl(S6[r], j1 < 0 ? 0 : j1, HW);
+      }
     }
     // ---- residual rows and target blocks at q
     float sp = 0.f, sq = 0.f;
@@ -549,6 +669,43 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     }
     __syncthreads();
     cost = term_cost(sp, sq, rl * rl, rs * rs);
+    if constexpr (PM) {
+      // ---- this pair's Jacobian column (its target's six rows, -w jlog Ad(T_f^-1) S) and its
+      // share of J^T r; J^T J of each block's targets on the matrix cores: per target slot h of
+      // the block and row i one v_mfma_f32_16x16x1_4b_f32 (lanes of other targets give zeros),
+      // so block b's 16 x 16 result is block-diagonal over its targets
+      float c6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      float gp = 0.f;
+      if (pt >= 0) {
+        float sc[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) sc[r] = (pj0 >= 0 ? Sc0[r] * pw0 : 0.f) + (pj1 >= 0 ? Sc1[r] * pw1 : 0.f);
+        pose_jac_column(sh->tg[pt < NP ? pt : 0], sc, c6);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) gp += c6[r] * sh->rows[6 * pt + r];
+      }
+      typedef float v16f __attribute__((ext_vector_type(16)));
+      v16f acc = {};
+      const int kt = tm->pr_kt;
+      for (int h = 0; h < kt; ++h) {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          const float x = ph == h ? c6[r] : 0.f;
+          acc = __builtin_amdgcn_mfma_f32_16x16x1f32(x, x, acc, 0, 0, 0);
+        }
+      }
+      // block b (of problem b / 2) row j = lane % 16, columns 4 (lane / 16) .. +3 (symmetric)
+      {
+        const int wl = threadIdx.x & 63, j = wl & 15, g4 = 4 * (wl >> 4);
+        *reinterpret_cast<float4*>(&sh0->pr.blk[j * 16 + g4]) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+        *reinterpret_cast<float4*>(&sh0->pr.blk[256 + j * 16 + g4]) = make_float4(acc[4], acc[5], acc[6], acc[7]);
+        *reinterpret_cast<float4*>(&sh1->pr.blk[j * 16 + g4]) = make_float4(acc[8], acc[9], acc[10], acc[11]);
+        *reinterpret_cast<float4*>(&sh1->pr.blk[256 + j * 16 + g4]) =
+            make_float4(acc[12], acc[13], acc[14], acc[15]);
+      }
+      sh->pr.gp[lane & (WV_NA - 1)] = gp;
+      gv = 0.f;
+    } else {
     // ---- this dof's Jacobian column (pose rows, then position rows), also to LDS
     float Jc[RP > 0 ? RP : 1];
 #pragma unroll
@@ -655,6 +812,7 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
         gv = gv + g2;
       }
     }
+    }  // dof-lane Jacobian
     if (has_lim) gv = gv + jl * rl;
     if (has_sm) gv = gv + js * rs;
     dl = jl * jl;
@@ -662,7 +820,7 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     }  // moved
     float A[WV_NA];
 #pragma unroll
-    for (int j = 0; j < WV_NA; ++j) A[j] = Aj[j];
+    for (int j = 0; j < WV_NA; ++j) A[j] = PM ? 0.f : Aj[j];
     // diagonal terms (limit, smoothness, then lam) enter row k's pivot when column k is staged.
     // Rows na..31 (padding) and their columns are exact zeros: no Jacobian column, diagonal term
     // or right-hand side reaches them.
@@ -685,7 +843,10 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     if (has_sm) dsum = ds + dsum;
     if (has_lim) dsum = dl + dsum;
     float dq = 0.f;
-    if constexpr (SN) {
+    float gfull = gv;  // this position's full J^T r (the pair layout assembles it in the solve)
+    if constexpr (PM) {
+      dq = sn_solve_pairs<NP, NQ>(sh, as_const(Pg.terms), lane, sn_c, sn_n, dsum, b, gfull);
+    } else if constexpr (SN) {
       dq = sn_solve<NP, NQ>(sh, as_const(Pg.terms), lane, sn_c, sn_n, A, dsum, b);
     } else {
 #pragma unroll
@@ -808,7 +969,7 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
     }  // dense
     if (pad) dq = 0.f;
     // predicted reduction 1/2 d.(lam d - g), summed over the dofs
-    const float pr = wv_sum<HW>(pad ? 0.f : dq * (lam * dq - gv));
+    const float pr = wv_sum<HW>(pad ? 0.f : dq * (lam * dq - gfull));
     const float pred = 0.5f * pr;
     // ---- proposal: FK, rows and cost in registers (no LDS)
     const float qp = q + dq;
