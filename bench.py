@@ -82,12 +82,15 @@ def variant_name(tpp: int, spec, n_pos: int) -> str:
 MIXABLE = {"V_T5", "V_T6", "V_T7", "V_W40", "V_W012", "V_H40", "V_H012", "V_HS40", "V_HS012", "V_Q6", "V_Q7"}
 
 
-def profiled_traffic(kernel: str):
-    """HBM bytes per launch of ``kernel`` (k_persistent<V_..> or k_persistent_reg) from the
-    committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by the
+TRAJ_PROFILE = os.path.join(ROOT, "profiles", "k_traj_512x64_counters.md")
+
+
+def profiled_traffic(kernel: str, path: str = PROFILE):
+    """HBM bytes per launch of ``kernel`` (k_persistent<V_..>, k_persistent_reg or k_traj<..>) from
+    the committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by the
     copy-probe calibration); None if absent"""
     try:
-        with open(PROFILE) as f:
+        with open(path) as f:
             in_traffic = False
             for line in f:
                 in_traffic = in_traffic or line.startswith("## HBM traffic")
@@ -226,6 +229,9 @@ def main() -> int:
     ap.add_argument("--clips", type=int, default=16, help="retargeting clips solved side by side (0 skips the leg)")
     ap.add_argument("--clip-frames", type=int, default=4096)
     ap.add_argument("--cpu-threads", type=int, default=16, help="host threads of the all-core CPU baseline")
+    ap.add_argument("--tpp", type=str, default="",
+                    help="comma-separated threads-per-problem codes of the three buckets instead of the autotuner's "
+                         "choice (profiled runs reuse the unprofiled run's choice: a profiler perturbs the tuner's timing)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a streaming copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -256,8 +262,13 @@ def main() -> int:
         b.set(q=bk.q_seed, pose_targets=bk.targets)
         batches.append(b)
     group = N.Group(ctx, batches)
-    tpp = group.autotune(10, params)
-    print(f"[bench] autotuned threads/problem {tpp}", file=sys.stderr, flush=True)
+    if args.tpp:
+        tpp = [int(x) for x in args.tpp.split(",")]
+        group.set_tuning(tpp)
+        print(f"[bench] threads/problem {tpp} (given)", file=sys.stderr, flush=True)
+    else:
+        tpp = group.autotune(10, params)
+        print(f"[bench] autotuned threads/problem {tpp}", file=sys.stderr, flush=True)
 
     # ---- batched: persistent uneven-batch kernel replayed from a hipGraph
     for _ in range(args.warmup):
@@ -491,6 +502,7 @@ def main() -> int:
             "roofline": {"bound": "valu-fp32", "kernel": "k_traj<7, 9>", "kernel_ms": round(tm, 4),
                          "flops_per_launch": 512 * ITERS * flm, "achieved": round(ach, 4), "peak": FP32_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 5),
+                         "traffic": profiled_traffic("k_traj<7, 9>", TRAJ_PROFILE),
                          "achieved_with_model_work": round(512 * ITERS * (flm + fmod) / (tm * 1e-3) / 1e12, 4),
                          "per_problem_iteration": {"f_lm": flm, "f_model": fmod}},
         }
