@@ -300,19 +300,29 @@ __device__ __forceinline__ float sn_solve(WaveLDS<NP, NQ>* sh, TermsK* tm, int l
     const bool ru = u < NR && u < nr;
     const float* Zp = u < NR ? M + (r0 + (u < NR ? u : 0)) * LS + c : bv + c;
     const float* Cp = M + r0 * (LS + 1);  // C[w][v] = Cp[w * LS + v]
+    // every load reads an address inside the LDS block (rows past the slot land in M's spare rows)
+    // and is selected afterwards: conditional loads would become a branch each
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
 #pragma unroll
-      for (int j = 0; j < i; ++j) L[i][j] = i < ns ? Dp[i * LS + j] : 0.f;
-      L[i][i] = i < ns ? Dp[i * LS + i] + dg[c + i] : 1.f;
-      z[i] = i < ns && (u == NR || ru) ? Zp[i] : 0.f;
+      for (int j = 0; j < i; ++j) {
+        const float v = Dp[i * LS + j];
+        L[i][j] = i < ns ? v : 0.f;
+      }
+      const float d = Dp[i * LS + i] + dg[c + i], zi = Zp[i];
+      L[i][i] = i < ns ? d : 1.f;
+      z[i] = i < ns && (u == NR || ru) ? zi : 0.f;
     }
 #pragma unroll
     for (int w = 0; w < NR; ++w) {
 #pragma unroll
-      for (int v = 0; v < w; ++v) C[w][v] = w < nr ? Cp[w * LS + v] : 0.f;
-      C[w][w] = w < nr ? Cp[w * LS + w] + dg[r0 + w] : 1.f;
-      r[w] = w < nr ? bv[r0 + w] : 0.f;
+      for (int v = 0; v < w; ++v) {
+        const float cv = Cp[w * LS + v];
+        C[w][v] = w < nr ? cv : 0.f;
+      }
+      const float d = Cp[w * LS + w] + dg[r0 + w], rv = bv[r0 + w];
+      C[w][w] = w < nr ? d : 1.f;
+      r[w] = w < nr ? rv : 0.f;
     }
   }
   sn_core(L, z, C, r, c, ns, lc, u, r0, nr, sh->dvs);
@@ -348,36 +358,48 @@ __device__ __forceinline__ float sn_solve_pairs(WaveLDS<NP, NQ>* sh, TermsK* tm,
     const int qb = pq < 0 ? 0 : pq;
     const float* Ep = Bk + (qb >> 4) * 256 + (qb & 15) * 16 + (pb & 15);  // E[w][i] = Ep[w * 16 + i]
     const bool ru = u < NR && u < nr && pq >= 0;
+    // every load reads an address inside the WaveLDS block (past the pair blocks it lands in the
+    // union's larger members) and is selected afterwards: conditional loads would become a branch
+    // each
+    const int w = u < NR ? u : 0;
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       const bool live = i < ns && po >= 0;
 #pragma unroll
-      for (int j = 0; j < i; ++j) L[i][j] = live ? Dp[i * 16 + j] : 0.f;
-      L[i][i] = i < ns ? (live ? Dp[i * 16 + i] : 0.f) + dg[c + i] : 1.f;
-      bs[i] = i < ns ? bv[c + i] - (live ? Gp[pb + i] : 0.f) : 0.f;
-      const int w = u < NR ? u : 0;
-      z[i] = u == NR ? bs[i] : (i < ns && ru ? Ep[w * 16 + i] : 0.f);
+      for (int j = 0; j < i; ++j) {
+        const float v = Dp[i * 16 + j];
+        L[i][j] = live ? v : 0.f;
+      }
+      const float dd = Dp[i * 16 + i], dgi = dg[c + i], bvi = bv[c + i], gpi = Gp[pb + i], ei = Ep[w * 16 + i];
+      L[i][i] = i < ns ? (live ? dd : 0.f) + dgi : 1.f;
+      bs[i] = i < ns ? bvi - (live ? gpi : 0.f) : 0.f;
+      z[i] = u == NR ? bs[i] : (i < ns && ru ? ei : 0.f);
     }
 #pragma unroll
-    for (int w = 0; w < NR; ++w) {
+    for (int w2 = 0; w2 < NR; ++w2) {
 #pragma unroll
-      for (int v = 0; v <= w; ++v) C[w][v] = 0.f;
-      r[w] = 0.f;
+      for (int v = 0; v <= w2; ++v) C[w2][v] = 0.f;
+      r[w2] = 0.f;
     }
     for (int t = 0; t < tm->pr_nrc; ++t) {
       const int rc = tm->pr_rc[t];
       const float* Cp = Bk + (rc >> 4) * 256 + (rc & 15) * 17;
 #pragma unroll
-      for (int w = 0; w < NR; ++w) {
+      for (int w2 = 0; w2 < NR; ++w2) {
 #pragma unroll
-        for (int v = 0; v <= w; ++v) C[w][v] += w < nr ? Cp[w * 16 + v] : 0.f;
-        r[w] += w < nr ? Gp[rc + w] : 0.f;
+        for (int v = 0; v <= w2; ++v) {
+          const float cv = Cp[w2 * 16 + v];
+          C[w2][v] += w2 < nr ? cv : 0.f;
+        }
+        const float gv = Gp[rc + w2];
+        r[w2] += w2 < nr ? gv : 0.f;
       }
     }
 #pragma unroll
-    for (int w = 0; w < NR; ++w) {
-      C[w][w] = w < nr ? C[w][w] + dg[r0 + w] : 1.f;
-      r[w] = w < nr ? bv[r0 + w] - r[w] : 0.f;
+    for (int w2 = 0; w2 < NR; ++w2) {
+      const float dgw = dg[r0 + w2], bvw = bv[r0 + w2];
+      C[w2][w2] = w2 < nr ? C[w2][w2] + dgw : 1.f;
+      r[w2] = w2 < nr ? bvw - r[w2] : 0.f;
     }
   }
   // the full right-hand side per position (-J^T r): the quad's lane 3 for the slot, lane 0 the root
