@@ -227,6 +227,31 @@ __device__ __forceinline__ void se3_jlog_blocks(const Pose& T, Mat3& A, Mat3& B)
       B.m[i * 3 + j] = Cm[i * 3 + 0] * A.m[0 * 3 + j] + Cm[i * 3 + 1] * A.m[1 * 3 + j] + Cm[i * 3 + 2] * A.m[2 * 3 + j];
 }
 
+// sin and cos of a joint's half angle.  Cody-Waite reduction by pi/2 in three parts (exact
+// products for |k| < 2^12) and the single-precision minimax polynomials of the Cephes library
+// (S. L. Moshier, "Methods and Programs for Mathematical Functions", 1989: sinf.c / cosf.c) on
+// [-pi/4, pi/4], then the quadrant's swap and signs; about 20 instructions against ~40 for the
+// library's, whose large-argument path (Payne-Hanek) serves |x| > 4096.
+__device__ __forceinline__ void rk_sincos(float x, float* sp, float* cp) {
+  if (fabsf(x) > 4096.f) {
+    sincosf(x, sp, cp);
+    return;
+  }
+  const float k = __builtin_rintf(x * 0.636619772367581343f);  // x / (pi / 2), nearest
+  float r = fmaf(-k, 1.5703125f, x);
+  r = fmaf(-k, 4.837512969970703125e-4f, r);
+  r = fmaf(-k, 7.54978995489188216e-8f, r);
+  const float z = r * r;
+  const float sr = fmaf(fmaf(fmaf(-1.9515295891e-4f, z, 8.3321608736e-3f), z, -1.6666654611e-1f) * z, r, r);
+  const float cr =
+      fmaf(fmaf(fmaf(2.443315711809948e-5f, z, -1.388731625493765e-3f), z, 4.166664568298827e-2f), z * z,
+           fmaf(-0.5f, z, 1.f));
+  const int q = (int)k;
+  const float s1 = (q & 1) ? cr : sr, c1 = (q & 1) ? sr : cr;
+  *sp = (q & 2) ? -s1 : s1;
+  *cp = ((q + 1) & 2) ? -c1 : c1;
+}
+
 // joint local transform P_j * M(q_j)   (_compute_joint_local_transform_func, robot_kernels.py:264-297)
 template <class RB>
 __device__ __forceinline__ Pose joint_local(RB rb, int j, const float* q) {
@@ -245,7 +270,7 @@ __device__ __forceinline__ Pose joint_local(RB rb, int j, const float* q) {
   int jt = rb->joint_type[j];
   if (jt == 2) {
     float s, c;
-    sincosf(pos * 0.5f, &s, &c);
+    rk_sincos(pos * 0.5f, &s, &c);
     X.q[0] = c; X.q[1] = rb->axis[j][0] * s; X.q[2] = rb->axis[j][1] * s; X.q[3] = rb->axis[j][2] * s;
   } else if (jt == 1) {
     X.t[0] = rb->axis[j][0] * pos; X.t[1] = rb->axis[j][1] * pos; X.t[2] = rb->axis[j][2] * pos;

@@ -64,7 +64,7 @@ __device__ __forceinline__ Pose quad_local(const QuadJoint& J, float qj) {
   Pose X = identity_pose();
   if (J.type == 2) {
     float s, c;
-    sincosf(qj * 0.5f, &s, &c);
+    rk_sincos(qj * 0.5f, &s, &c);
     X.q[0] = c; X.q[1] = J.ax[0] * s; X.q[2] = J.ax[1] * s; X.q[3] = J.ax[2] * s;
   } else if (J.type == 1) {
     X.t[0] = J.ax[0] * qj; X.t[1] = J.ax[1] * qj; X.t[2] = J.ax[2] * qj;

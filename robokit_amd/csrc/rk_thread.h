@@ -52,7 +52,7 @@ __device__ __forceinline__ Pose chain_fk(RobotK* rb, int nj, const float* q, int
         const float ax0 = rb->axis[j][0], ax1 = rb->axis[j][1], ax2 = rb->axis[j][2];
         if (rb->joint_type[j] == 2) {
           float s, c;
-          sincosf(q[j] * 0.5f, &s, &c);
+          rk_sincos(q[j] * 0.5f, &s, &c);
           X.q[0] = c; X.q[1] = ax0 * s; X.q[2] = ax1 * s; X.q[3] = ax2 * s;
         } else {
           X.t[0] = ax0 * q[j]; X.t[1] = ax1 * q[j]; X.t[2] = ax2 * q[j];

@@ -430,7 +430,7 @@ __device__ __forceinline__ Pose wave_fk(const WaveJoint& jd, int nj, int nrounds
   Pose X = identity_pose();
   if (jd.type == 2) {
     float s, c;
-    sincosf(pos * 0.5f, &s, &c);
+    rk_sincos(pos * 0.5f, &s, &c);
     X.q[0] = c; X.q[1] = jd.ax[0] * s; X.q[2] = jd.ax[1] * s; X.q[3] = jd.ax[2] * s;
   } else if (jd.type == 1) {
     X.t[0] = jd.ax[0] * pos; X.t[1] = jd.ax[1] * pos; X.t[2] = jd.ax[2] * pos;

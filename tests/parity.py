@@ -68,12 +68,15 @@ class ShadowReport:
     worst: tuple = ()  # (step index, n_gpu, n_f32, n_f64) of the largest norm excess
 
     def ok(self) -> bool:
+        # accept flips against FP64 happen at decisions inside the rounding floor (converged
+        # problems, rho at its threshold); two FP32 evaluations each flip a random subset of
+        # them, so the counts differ by counting noise as well: 3 sigma (sqrt of the FP32 count)
         return (self.norm_bad == 0 and self.prop_bad == 0
-                and self.flips_gpu <= 1.25 * self.flips_f32 + 0.005 * self.steps)
+                and self.flips_gpu <= 1.25 * self.flips_f32 + 3.0 * self.flips_f32 ** 0.5 + 0.005 * self.steps)
 
 
 def shadow_check(gpu_trace, gpu_acc, step32, step64, rtol: float = 1e-4, atol: float = 1e-6,
-                 k_f32: float = 4.0) -> ShadowReport:
+                 k_f32: float = 4.0, step64_perturbed=None) -> ShadowReport:
     """Check every GPU step from the GPU's own iterate (q_i, lam_i).
 
     ``step32`` / ``step64`` are the FP32 oracle's and its FP64 twin's one-step results from the
@@ -85,7 +88,14 @@ def shadow_check(gpu_trace, gpu_acc, step32, step64, rtol: float = 1e-4, atol: f
     error where the formula itself is ill-conditioned (the log map's branch cut at theta = pi,
     costs at the round-off floor).  The same test applies to the proposal's norm, which also
     carries the damped solve's conditioning.  Accept decisions: the GPU may not flip against the
-    FP64 twin more often than the FP32 oracle does (plus 0.5 % of steps).
+    FP64 twin more often than the FP32 oracle does (x 1.25, plus three standard deviations of that
+    count and 0.5 % of steps).
+
+    ``step64_perturbed`` (optional): the FP64 step from each iterate moved at FP32 rounding scale
+    (q (1 + u 2^-22), u = +-1).  Where the step is discontinuous in q -- a residual with a kink,
+    such as the surface-distance collision penalty |sd| whose minimisers sit on sd = 0 -- an FP32
+    evaluation lands on either side, and the FP32 oracle alone may not show it; the proposal
+    yardstick then takes the larger of the FP32 oracle's and the perturbed step's deviation.
     """
     B, T1 = gpu_trace.shape
     iters = T1 - 1
@@ -104,7 +114,11 @@ def shadow_check(gpu_trace, gpu_acc, step32, step64, rtol: float = 1e-4, atol: f
     # compared with the FP32 oracle's: max and 99th percentile within k_f32 times.
     both = (a_g == 1) & (a32 == 1) & (a64 == 1)
     e_g = (np.abs(p_g - p64) / (p64 + atol))[both]
-    e_32 = (np.abs(p32 - p64) / (p64 + atol))[both]
+    e_32 = np.abs(p32 - p64) / (p64 + atol)
+    if step64_perturbed is not None:
+        pp = nrm(step64_perturbed["cost_prop"])
+        e_32 = np.maximum(e_32, np.abs(pp - p64) / (p64 + atol))
+    e_32 = e_32[both]
     prop_bad, prop_excess = 0, 0.0
     if e_g.size:
         for q in (100, 99):

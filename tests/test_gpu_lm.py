@@ -636,8 +636,9 @@ def test_scene_collision_parity(ctx, robot, penalty, form):
     """SceneCollisionTask rows (terms/dense/scene_collision_task.py:33-81, 550-646; penalties of
     sphere_collision_kernels.py:19-42) beside a FrameTask target (or the presets' PositionTask +
     RotationTask pair) and PositionLimit, in the one-lane-per-problem register kernel: every step
-    shadow-checked against the oracle's COLLISION rows; zero iterations score the seeds like the
-    oracle; the plain per-term launch refuses the term"""
+    shadow-checked against the oracle's COLLISION rows (the proposal yardstick includes the FP64
+    steps from FP32-rounding-perturbed iterates: the scene distance has kinks); zero iterations
+    score the seeds like the oracle; the plain per-term launch refuses the term"""
     tb = W.config3(seed=12, batch=4, T=5, arm="panda" if robot == "panda_dh" else "ur10")
     spec, sc = tb.spec, tb.scene
     na = spec.num_actuated_joints
@@ -671,8 +672,23 @@ def test_scene_collision_parity(ctx, robot, penalty, form):
     p = O.LMParams(**C0_PARAMS.__dict__)
     args = (spec, terms, q_it.reshape(-1, na), lam_it.reshape(-1), p)
     atol = 1e-6 * (20.0 if form == "pos_rot" else 2.0)
-    rep = shadow_check(gpu["cost_trace"], gpu["accept"], O.lm_step(*args, rows=rows, threads=8),
-                       O.lm_step(*args, dtype="f64", rows=rows, threads=8), atol=atol)
+    # the scene distance is not differentiable everywhere: the surface-distance penalty |sd| draws
+    # spheres onto obstacle surfaces (sd = 0), and inside a box the gradient switches faces on the
+    # box's medial axis.  An FP32 FK lands on either side of such a point, so the proposal
+    # yardstick also takes the FP64 steps from iterates moved at the scale of FP32 FK's rounding
+    # through the chain (2^-20 relative, three sign patterns)
+    s64 = O.lm_step(*args, dtype="f64", rows=rows, threads=8)
+    pert = None
+    for k in range(3):  # per step, the perturbed FP64 proposal farthest from the unperturbed one
+        q_pert = q_it.reshape(-1, na).astype(np.float64)
+        q_pert *= 1.0 + np.random.default_rng(7 + k).choice([-1.0, 1.0], q_pert.shape) * 2.0 ** -20
+        c = O.lm_step(spec, terms, q_pert, lam_it.reshape(-1), p, dtype="f64", rows=rows, threads=8)["cost_prop"]
+        if pert is None or k == 0:
+            pert = c
+        else:
+            pert = np.where(np.abs(c - s64["cost_prop"]) > np.abs(pert - s64["cost_prop"]), c, pert)
+    rep = shadow_check(gpu["cost_trace"], gpu["accept"], O.lm_step(*args, rows=rows, threads=8), s64, atol=atol,
+                       step64_perturbed={"cost_prop": pert})
     # the collision rows matter: their share of the oracle's cost at the seeds
     c_all = O.lm_solve(spec, terms, wl.q_seed, 0, p, dtype="f64", traces=False)["cost"]
     c_wo = O.lm_solve(spec, terms[:-1], wl.q_seed, 0, p, dtype="f64", traces=False)["cost"]
