@@ -1933,15 +1933,17 @@ static int signature(const Terms& th) {
 }
 
 // threads per problem -> variant.  G = 0: the tuned choice for this term structure if the
-// autotuner made one on this context, else the default (registers where they fit).
-static int pick_variant(const rk_context* ctx, const Terms& th, int G, int* v) {
+// autotuner made one on this context, else the default (registers where they fit; a serial arm
+// takes a quad of lanes per problem while one lane per problem would leave SIMDs idle: fewer
+// than 64 problems per SIMD, n > 0 the problem count, 0 unknown).
+static int pick_variant(const rk_context* ctx, const Terms& th, int G, int* v, int n = 0) {
   const Robot& r = ctx->robots[th.robot];
   const bool small = fits_small(th, r);
   if (G == 0) {
     auto it = ctx->tuned.find(signature(th));
     if (it != ctx->tuned.end()) G = it->second;
     else
-      G = fits_thread(th, r)               ? 1
+      G = fits_thread(th, r)               ? (fits_quad(th, r) && n > 0 && n < WAVE * 4 * ctx->cu_count ? RK_TPP_QUAD : 1)
           : super_kind(th, r) >= 0         ? RK_TPP_SUPER
           : wave_kind(th, r, 32) >= 0      ? RK_TPP_HALF
           : wave_kind(th, r, 64) >= 0      ? RK_TPP_WAVE
@@ -1987,7 +1989,7 @@ static int pick_variant(const rk_context* ctx, const Terms& th, int G, int* v) {
               "half-wave: trees whose normal equations have the block-arrow form), 4, 8, 16 (small robots) or "
               "16, 32, 64");
 }
-static int pick_variant(const rk_batch* b, int G, int* v) { return pick_variant(b->ctx, b->th, G, v); }
+static int pick_variant(const rk_batch* b, int G, int* v) { return pick_variant(b->ctx, b->th, G, v, b->n); }
 
 static void legal_tpps(const rk_batch* b, std::vector<int>& out) {
   const Robot& r = b->ctx->robots[b->th.robot];
@@ -3832,8 +3834,10 @@ static int autotune_batch(rk_group* g, rk_batch* b, int iters, const LMParams& l
 }
 
 // The whole group with the kernel codes tpp, launched as a solve launches it (one k_persistent_reg
-// launch when every variant is register-resident): HIP-event time of the second of two runs from
-// the batches' current q (restored before each run), traces detached.  A variant that is fastest
+// launch when every variant is register-resident): the median HIP-event time of three runs after
+// a warm-up, each from the batches' current q (restored before each run), traces detached.  (A
+// single short run measured too noisily to separate combinations 15 % apart, and the shortest of
+// several favours a combination whose time varies with the workgroups' placement.)  A variant that is fastest
 // alone need not be fastest beside the others: the launch shares the chip, and a variant with many
 // short wavefronts can leave the resident slots to the long ones of another.
 static int autotune_group_time(rk_group* g, const std::vector<int>& tpp, int iters, const LMParams& lp,
@@ -3843,8 +3847,14 @@ static int autotune_group_time(rk_group* g, const std::vector<int>& tpp, int ite
   GroupLaunch gl;
   int rc;
   if ((rc = group_prepare(g, &gl, false, g->buckets_dev, g->traj_dev))) return rc;
-  float ms = 0.f;
-  for (int rep = 0; rep < 2; ++rep) {
+  if (iters <= 0) {  // only the IK workgroups of the launch (for the resident-slot test)
+    int blocks = 0;
+    for (size_t k = 0; k < gl.var.size(); ++k) blocks += gl.grid[k];
+    *ms_out = (float)blocks;
+    return RK_OK;
+  }
+  float t[3] = {0.f, 0.f, 0.f};
+  for (int rep = 0; rep < 4; ++rep) {
     for (size_t i = 0; i < g->batches.size(); ++i) {
       rk_batch* b = g->batches[i];
       HIP_TRY(hipMemcpyAsync(b->q, q_save[i], sizeof(float) * b->na * b->capacity, hipMemcpyDeviceToDevice,
@@ -3854,14 +3864,17 @@ static int autotune_group_time(rk_group* g, const std::vector<int>& tpp, int ite
     if ((rc = group_launch(g, gl, iters, lp, false))) return rc;
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (rep > 0) t[rep - 1] = ms;
   }
-  *ms_out = ms;
+  std::sort(t, t + 3);
+  *ms_out = t[1];
   return RK_OK;
 }
 
 // Group pass of the autotuner: the combinations of each IK batch's front-runners alone (codes
-// within 1.5x of its best, at most two per batch, the first four batches) timed as a whole group;
+// within 2x of its best, at most three per batch, the first four batches) timed as a whole group;
 // the fastest combination is kept.  State (q, trace pointers) is restored on every exit.
 static int autotune_group(rk_group* g, int iters, const LMParams& lp,
                           const std::vector<std::vector<std::pair<float, int>>>& ranked) {
@@ -3873,7 +3886,7 @@ static int autotune_group(rk_group* g, int iters, const LMParams& lp,
     opts[i].push_back(g->tpp[i]);
     if (g->batches[i]->traj_T || i >= 4) continue;
     for (const auto& r : ranked[i])
-      if (r.second != g->tpp[i] && r.first <= 1.5f * ranked[i][0].first && opts[i].size() < 2)
+      if (r.second != g->tpp[i] && r.first <= 2.f * ranked[i][0].first && opts[i].size() < 3)
         opts[i].push_back(r.second);
     combos *= opts[i].size();
   }
@@ -3917,16 +3930,34 @@ static int autotune_group(rk_group* g, int iters, const LMParams& lp,
         hipSuccess)
       return cleanup(fail(RK_EHIP, "rk_group_autotune: could not save a batch's q"));
   }
-  float best = 1e30f;
-  std::vector<int> best_tpp = tpp0, cur(nb);
-  for (size_t c = 0; c < combos; ++c) {
+  // a single launch whose workgroups exceed the resident slots (2 wavefronts per SIMD) runs its
+  // last workgroups only as others finish: its time then hangs on the workgroups' placement.  When
+  // some combination fits, the ones that do not are not considered.
+  auto combo = [&](size_t c, std::vector<int>& cur) {
     size_t k = c;
     for (size_t i = 0; i < nb; ++i) {
       cur[i] = opts[i][k % opts[i].size()];
       k /= opts[i].size();
     }
+  };
+  const float slots = 2.f * 4.f * (float)ctx->cu_count;
+  std::vector<char> fits(combos, 1);
+  bool any_fits = false;
+  std::vector<int> cur(nb);
+  for (size_t c = 0; c < combos; ++c) {
+    combo(c, cur);
+    float blocks = 0.f;
+    if ((rc = autotune_group_time(g, cur, 0, lp, q_save, &blocks))) return cleanup(rc);
+    fits[c] = blocks <= slots;
+    any_fits = any_fits || fits[c];
+  }
+  float best = 1e30f;
+  std::vector<int> best_tpp = tpp0;
+  for (size_t c = 0; c < combos; ++c) {
+    if (any_fits && !fits[c]) continue;
+    combo(c, cur);
     float ms = 0.f;
-    if ((rc = autotune_group_time(g, cur, iters, lp, q_save, &ms))) return cleanup(rc);
+    if ((rc = autotune_group_time(g, cur, std::max(iters, 20), lp, q_save, &ms))) return cleanup(rc);
     if (ms < best) {
       best = ms;
       best_tpp = cur;

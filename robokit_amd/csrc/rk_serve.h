@@ -625,7 +625,8 @@ int rk_server_add_terms(rk_server* sv, const rk_terms_desc* t, int32_t* term_id)
     HIP_TRY(hipSetDevice(ctx->device));
     if ((rc = make_terms(ctx, t, &T.th))) return rc;
     if (T.th.n_rot) return fail(RK_EINVAL, "server requests carry pose and position targets (no rotation targets)");
-    if ((rc = pick_variant(ctx, T.th, 0, &T.variant))) return rc;
+    // launches carry a few hundred problems at most: the default for a small batch
+    if ((rc = pick_variant(ctx, T.th, 0, &T.variant, 1))) return rc;
     T.na = ctx->robots[T.th.robot].na;
     if ((rc = dmalloc(ctx, (void**)&T.td, sizeof(Terms), nullptr))) return rc;
     HIP_TRY(hipMemcpyAsync(T.td, &T.th, sizeof(Terms), hipMemcpyHostToDevice, ctx->stream));

@@ -181,9 +181,10 @@ class PopulationIK:
                n=self.batch_size * w0, rot_targets=rep(rot_targets))
         self._ready = True
 
-    def solve(self, tpp: int = 0) -> N.Batch:
+    def solve(self, tpp=0) -> N.Batch:
         """Run the schedule from the seeds; returns the width-1 batch of winners (get / metrics).
-        The launches and selects are only enqueued on the context's stream: reading the winners
+        ``tpp``: the threads-per-problem code of every stage, or a sequence of one per stage.  The
+        launches and selects are only enqueued on the context's stream: reading the winners
         (get / metrics) waits for them."""
         if not self._ready:
             raise RuntimeError("set_targets first")
@@ -199,7 +200,8 @@ class PopulationIK:
                 self._select(cur, w, nxt, widths[k])
                 cur, w = nxt, widths[k]
             if stage.iters > 0 or k == 0:  # a zero-iteration first stage scores the seeds
-                cur.solve(stage.iters, params[k], tpp=tpp, from_seed=(k == 0), wait=False)
+                tk = tpp[k] if isinstance(tpp, (list, tuple)) else tpp
+                cur.solve(stage.iters, params[k], tpp=tk, from_seed=(k == 0), wait=False)
         if w > 1:  # one winner per problem
             win = self._batches[1]
             self._select(cur, w, win, 1)
