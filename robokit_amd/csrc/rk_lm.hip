@@ -957,6 +957,9 @@ __device__ __forceinline__ void solve_tile(const Problems& P, int first, const L
 
 template <int G, class C>
 __global__ __launch_bounds__(64) void k_fused(Problems P, LMParams lp, int it0, int iters) {
+  // wave issue priority 1 (trajectory waves stay at 0): where IK and trajectory launches share a
+  // SIMD, as in the request server, the short IK solves issue first
+  __builtin_amdgcn_s_setprio(1);
   extern __shared__ __align__(16) char lds[];
   solve_tile<G, C>(P, blockIdx.x * (WAVE / G), lp, it0, iters, lds);
 }
@@ -2063,17 +2066,27 @@ static int launch_variant(hipStream_t st, int v, const Problems& P, const LMPara
 }
 
 // trajectory kernel of an arm with `na` dofs (two wavefronts per problem); lim: the problems carry
-// bound rows (the kernel instantiation with them)
+// bound rows (the kernel instantiation with them); shared: the launch runs beside latency-bound IK
+// launches (the request server), which take the form with v_readlane band updates
 static int launch_traj(hipStream_t st, int na, const TrajTerms* tt, const Problems& P, const LMParams& lp, int it0,
-                       int iters, bool lim) {
-  if (na == 6 && lim)
-    hipLaunchKernelGGL((k_traj<6, 8, true>), dim3(P.n), dim3(2 * WAVE), 0, st, P, tt, lp, it0, iters);
+                       int iters, bool lim, bool shared = false) {
+  const dim3 g(P.n), b(2 * WAVE);
+  if (na == 6 && lim && shared)
+    hipLaunchKernelGGL((k_traj<6, 8, true, false>), g, b, 0, st, P, tt, lp, it0, iters);
+  else if (na == 6 && lim)
+    hipLaunchKernelGGL((k_traj<6, 8, true>), g, b, 0, st, P, tt, lp, it0, iters);
+  else if (na == 6 && shared)
+    hipLaunchKernelGGL((k_traj<6, 8, false, false>), g, b, 0, st, P, tt, lp, it0, iters);
   else if (na == 6)
-    hipLaunchKernelGGL((k_traj<6, 8>), dim3(P.n), dim3(2 * WAVE), 0, st, P, tt, lp, it0, iters);
+    hipLaunchKernelGGL((k_traj<6, 8>), g, b, 0, st, P, tt, lp, it0, iters);
+  else if (lim && shared)
+    hipLaunchKernelGGL((k_traj<7, 9, true, false>), g, b, 0, st, P, tt, lp, it0, iters);
   else if (lim)
-    hipLaunchKernelGGL((k_traj<7, 9, true>), dim3(P.n), dim3(2 * WAVE), 0, st, P, tt, lp, it0, iters);
+    hipLaunchKernelGGL((k_traj<7, 9, true>), g, b, 0, st, P, tt, lp, it0, iters);
+  else if (shared)
+    hipLaunchKernelGGL((k_traj<7, 9, false, false>), g, b, 0, st, P, tt, lp, it0, iters);
   else
-    hipLaunchKernelGGL((k_traj<7, 9>), dim3(P.n), dim3(2 * WAVE), 0, st, P, tt, lp, it0, iters);
+    hipLaunchKernelGGL((k_traj<7, 9>), g, b, 0, st, P, tt, lp, it0, iters);
   HIP_TRY(hipGetLastError());
   return RK_OK;
 }

@@ -277,7 +277,7 @@ static int srv_traj_next(rk_server* sv, int s) {
   if (L.it_done >= L.iters) return srv_enqueue_tail(sv, s);
   const int n_it = std::min(SRV_TRAJ_CHUNK, L.iters - L.it_done);
   int rc = launch_traj(sv->streams[s], T.traj_na, T.tt, srv_problems(sv, L), L.reqs[0]->lp, L.it_done, n_it,
-                       T.traj_lim);
+                       T.traj_lim, true);
   if (rc) return rc;
   L.it_done += n_it;
   HIP_TRY(hipEventRecord(sv->events[s], sv->streams[s]));

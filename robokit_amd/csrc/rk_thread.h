@@ -686,5 +686,8 @@ __device__ __forceinline__ void thread_solve(const Problems& Pg, int s_idx, cons
 
 template <int NA, int NJ, bool PR = false, bool CO = false>
 __global__ __launch_bounds__(64) void k_thread(Problems P, LMParams lp, int it0, int iters) {
+  // wave issue priority 1 (trajectory waves stay at 0): where IK and trajectory launches share a
+  // SIMD, as in the request server, the short IK solves issue first
+  __builtin_amdgcn_s_setprio(1);
   thread_solve<NA, NJ, PR, CO>(P, blockIdx.x * 64 + threadIdx.x, lp, it0, iters);
 }

@@ -191,6 +191,38 @@ __device__ __forceinline__ float wv_shr1(float src, float fill) {
   return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(src), 0x138, 0xF, 0xF, false));
 }
 
+// The band's rank-1 update X[c] += (L of lane WL+c) * nl, c = 1..W, with the window on lanes
+// WL..63 = lanes WL-48..15 of the wavefront's last 16-lane row: each broadcast is a DPP
+// row_newbcast operand of one v_fmac_f32 (lane WL+c of the row to the whole row), so an update
+// is W instructions with no scalar round trip (a v_readlane and an FMA each, otherwise).  The
+// other rows broadcast their own lanes, outside the window.  DPP reads of a VGPR written by the
+// VALU need two wait states, and the compiler does not see the writes inside an asm block: the
+// leading s_nop covers l, written just before, the trailing one the shifts of X that follow.
+// fma(L_c, -l, x) rounds as fma(-l, L_c, x).
+#define RK_BC(i, n) "v_fmac_f32_dpp %" #i ", %14, %15 row_newbcast:" #n " row_mask:0xf bank_mask:0xf\n\t"
+#define RK_BC12(i, n) "v_fmac_f32_dpp %" #i ", %12, %13 row_newbcast:" #n " row_mask:0xf bank_mask:0xf\n\t"
+template <int W>
+__device__ __forceinline__ void band_rank1(float (&X)[W + 1], float l, float nl) {
+  static_assert(W == 14 || W == 12, "window within one 16-lane row: 6- or 7-dof frames");
+  if constexpr (W == 14) {  // WL = 49: lanes 50..63 are row lanes 2..15
+    asm volatile("s_nop 1\n\t" RK_BC(0, 2) RK_BC(1, 3) RK_BC(2, 4) RK_BC(3, 5) RK_BC(4, 6) RK_BC(5, 7)
+                 RK_BC(6, 8) RK_BC(7, 9) RK_BC(8, 10) RK_BC(9, 11) RK_BC(10, 12) RK_BC(11, 13) RK_BC(12, 14)
+                 RK_BC(13, 15) "s_nop 1"
+                 : "+v"(X[1]), "+v"(X[2]), "+v"(X[3]), "+v"(X[4]), "+v"(X[5]), "+v"(X[6]), "+v"(X[7]),
+                   "+v"(X[8]), "+v"(X[9]), "+v"(X[10]), "+v"(X[11]), "+v"(X[12]), "+v"(X[13]), "+v"(X[14])
+                 : "v"(l), "v"(nl));
+  } else {  // WL = 51: lanes 52..63 are row lanes 4..15
+    asm volatile("s_nop 1\n\t" RK_BC12(0, 4) RK_BC12(1, 5) RK_BC12(2, 6) RK_BC12(3, 7) RK_BC12(4, 8)
+                 RK_BC12(5, 9) RK_BC12(6, 10) RK_BC12(7, 11) RK_BC12(8, 12) RK_BC12(9, 13) RK_BC12(10, 14)
+                 RK_BC12(11, 15) "s_nop 1"
+                 : "+v"(X[1]), "+v"(X[2]), "+v"(X[3]), "+v"(X[4]), "+v"(X[5]), "+v"(X[6]), "+v"(X[7]),
+                   "+v"(X[8]), "+v"(X[9]), "+v"(X[10]), "+v"(X[11]), "+v"(X[12])
+                 : "v"(l), "v"(nl));
+  }
+}
+#undef RK_BC
+#undef RK_BC12
+
 // ---- the damped band solve (J^T J + lam I) dq = -g, both wavefronts taking part.
 //
 // Register window (one wavefront): the active rows j..j+W sit on lanes WL..63 (row j+a on lane
@@ -202,7 +234,7 @@ __device__ __forceinline__ float wv_shr1(float src, float fill) {
 // entry (r, c), c <= r, of the reversed matrix sits at band[D-1-c][r-c].  ZERO: the entering
 // rows are zero (nothing follows the columns factorised).  Factorises columns [j0, j0 + n) from
 // the window in (C, rc) and leaves the next window there; L is stored for rows below lim.
-template <int NA, bool REV, bool ZERO>
+template <int NA, bool REV, bool ZERO, bool BC>
 __device__ __forceinline__ void band_factor(TrajLDS<NA>* sh, int D, int j0, int n, int lim, float (&C)[2 * NA + 1],
                                             float& rc) {
   constexpr int W = 2 * NA, WL = 63 - W, BS = TrajLDS<NA>::BS;
@@ -240,8 +272,12 @@ __device__ __forceinline__ void band_factor(TrajLDS<NA>* sh, int D, int j0, int 
     *di = inv;
     *dy = y;
     xr = fmaf(-Lv, y, xr);
+    if constexpr (BC) {
+      band_rank1<W>(X, Lv, -Lv);
+    } else {
 #pragma unroll
-    for (int c = 1; c <= W; ++c) X[c] = fmaf(-Lv, wv_readlane(Lv, WL + c), X[c]);
+      for (int c = 1; c <= W; ++c) X[c] = fmaf(-Lv, wv_readlane(Lv, WL + c), X[c]);
+    }
     piv = wv_readlane(X[1], WL + 1);  // the next pivot (lane WL+1 moves to lane WL)
 #pragma unroll
     for (int c = 0; c < W; ++c) Y[c] = wv_shl1(X[c + 1], Y[c]);
@@ -368,7 +404,7 @@ __device__ __forceinline__ float band_back_window(TrajLDS<NA>* sh, int D, int jt
 // first then factorises S0 + S1 - A_SS, solves S backwards and publishes x_S; then each runs the
 // backward substitution of its part.  Short trajectories (T < 6) are solved whole by the first.
 // Returns with dq in sh->rhs; takes three barriers in both wavefronts.
-template <int NA>
+template <int NA, bool BC>
 __device__ __forceinline__ void band_solve(TrajLDS<NA>* sh, int T, int wv) {
   constexpr int W = 2 * NA, WL = 63 - W;
   const int lane = threadIdx.x & 63, wa = lane - WL;
@@ -380,9 +416,9 @@ __device__ __forceinline__ void band_solve(TrajLDS<NA>* sh, int T, int wv) {
   if (wv == 0) {
     band_window0<NA, false>(sh, D, C, rc);
     if (split)
-      band_factor<NA, false, false>(sh, D, 0, nA, nA + W, C, rc);
+      band_factor<NA, false, false, BC>(sh, D, 0, nA, nA + W, C, rc);
     else
-      band_factor<NA, false, false>(sh, D, 0, D, D, C, rc);
+      band_factor<NA, false, false, BC>(sh, D, 0, D, D, C, rc);
     __syncthreads();  // the second wavefront's update of S is in sx, rx
     float rb = 0.f;
     if (split) {
@@ -397,7 +433,7 @@ __device__ __forceinline__ void band_solve(TrajLDS<NA>* sh, int T, int wv) {
         for (int c = 0; c <= W; ++c) C[c] = 0.f;
         rc = 0.f;
       }
-      band_factor<NA, false, true>(sh, D, nA, W, nA + W, C, rc);
+      band_factor<NA, false, true, BC>(sh, D, nA, W, nA + W, C, rc);
       rb = band_back_window<NA, false>(sh, D, nA + W - 1);
       band_back<NA, false>(sh, D, nA + W - 1, nA, rb);
     }
@@ -411,7 +447,7 @@ __device__ __forceinline__ void band_solve(TrajLDS<NA>* sh, int T, int wv) {
   } else {
     if (split) {
       band_window0<NA, true>(sh, D, C, rc);
-      band_factor<NA, true, false>(sh, D, 0, nB, D, C, rc);
+      band_factor<NA, true, false, BC>(sh, D, 0, nB, D, C, rc);
       // the window now holds the reversed separator rows: lane WL+a is row W-1-a of S
       if (wa >= 0 && wa < W) {
 #pragma unroll
@@ -602,7 +638,7 @@ __device__ __forceinline__ float traj_cg(const float (&hb)[NA][NA], const float 
   return -wv_sum(gd) - 0.5f * wv_sum(qd);
 }
 
-template <int NA, int NJ, bool LIM = false>
+template <int NA, int NJ, bool LIM, bool BC>
 __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s, const LMParams& lp, int it0, int iters,
                                            TrajLDS<NA>* sh) {
   constexpr int W = TrajLDS<NA>::W;
@@ -751,7 +787,7 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
         }
       }
       __syncthreads();  // band built
-      band_solve<NA>(sh, T, 0);
+      band_solve<NA, BC>(sh, T, 0);
       float pr = 0.f;
 #pragma unroll
       for (int d = 0; d < NA; ++d) {
@@ -795,7 +831,7 @@ __device__ __forceinline__ void traj_solve(const Problems& Pg, TrajK* tt, int s,
 // its half of the band solve.  Its barriers pair one for one with traj_solve's: rows ready (after
 // an accepted step only), band built, the three of band_solve, proposal cost ready, iterate
 // published.
-template <int NA, int NJ, bool LIM = false>
+template <int NA, int NJ, bool LIM, bool BC>
 __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s, int it0, int iters,
                                             TrajLDS<NA>* sh) {
   RobotK* rb = as_const(Pg.robot);
@@ -848,7 +884,7 @@ __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s
       __syncthreads();
       __syncthreads();
     } else {
-      band_solve<NA>(sh, T, 1);
+      band_solve<NA, BC>(sh, T, 1);
     }
     float qp[NA];
 #pragma unroll
@@ -864,14 +900,24 @@ __device__ __forceinline__ void traj_worker(const Problems& Pg, TrajK* tt, int s
 
 // LIM: the kernel instantiation with bound rows and scene volumes / meshes (the other one, for
 // primitive scenes without bound rows, compiles neither)
-template <int NA, int NJ, bool LIM = false>
-__global__ __launch_bounds__(128) void k_traj(Problems P, const TrajTerms* tt, LMParams lp, int it0, int iters) {
+// BC: the band's rank-1 updates by DPP row broadcasts (band_rank1); without it, by v_readlane
+// broadcasts, whose scalar round trips leave VALU issue slots to other kernels' wavefronts on the
+// same SIMDs.  The request server launches that form beside its IK launches: on the mixed served
+// trace its IK p99 was 1.65-1.80 ms, against 2.16-2.19 ms beside the broadcast form, which solves
+// a batch of 512 x 64 trajectories in 6.44 ms instead of 6.88.
+// The broadcast form is compiled for one wavefront per SIMD (a batch of 512 trajectories is 1024
+// wavefronts on 1024 SIMDs), so that register allocation and scheduling aim at the latency of one
+// wavefront's chain: compiled for two per SIMD (held to 256 registers), the 7-dof kernel took
+// 7.8 ms per 512 x 64 solve.  The other form keeps the default target (its code as measured).
+template <int NA, int NJ, bool LIM = false, bool BC = true>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, BC ? 1 : 10))) void k_traj(
+    Problems P, const TrajTerms* tt, LMParams lp, int it0, int iters) {
   __shared__ __align__(16) TrajLDS<NA> sh;
   if ((int)blockIdx.x >= P.n) return;
   if (threadIdx.x < 64)
-    traj_solve<NA, NJ, LIM>(P, as_const(tt), blockIdx.x, lp, it0, iters, &sh);
+    traj_solve<NA, NJ, LIM, BC>(P, as_const(tt), blockIdx.x, lp, it0, iters, &sh);
   else
-    traj_worker<NA, NJ, LIM>(P, as_const(tt), blockIdx.x, it0, iters, &sh);
+    traj_worker<NA, NJ, LIM, BC>(P, as_const(tt), blockIdx.x, it0, iters, &sh);
 }
 
 // Trajectory batches of one arm class (dofs, bound rows / geometries) in a group: consecutive
@@ -886,7 +932,9 @@ struct TrajBucket {
 typedef const __attribute__((address_space(4))) TrajBucket TrajBucketK;
 
 template <int NA, int NJ, bool LIM = false>
-__global__ __launch_bounds__(128) void k_traj_multi(const TrajBucket* table_, int nb, LMParams lp, int it0, int iters) {
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_traj_multi(const TrajBucket* table_,
+                                                                                               int nb, LMParams lp,
+                                                                                               int it0, int iters) {
   __shared__ __align__(16) TrajLDS<NA> sh;
   TrajBucketK* table = (TrajBucketK*)table_;
   int k = 0;
@@ -901,9 +949,9 @@ __global__ __launch_bounds__(128) void k_traj_multi(const TrajBucket* table_, in
   if (s >= P.n) return;
   TrajK* tt = as_const(table[k].tt);
   if (threadIdx.x < 64)
-    traj_solve<NA, NJ, LIM>(P, tt, s, lp, it0, iters, &sh);
+    traj_solve<NA, NJ, LIM, true>(P, tt, s, lp, it0, iters, &sh);
   else
-    traj_worker<NA, NJ, LIM>(P, tt, s, it0, iters, &sh);
+    traj_worker<NA, NJ, LIM, true>(P, tt, s, it0, iters, &sh);
 }
 
 // goal errors at the last frame and the smallest sphere-to-scene distance (one wave per problem)

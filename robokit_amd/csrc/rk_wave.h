@@ -1037,6 +1037,9 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
 
 template <int NP, int NQ, int HW, bool SN = false>
 __global__ __launch_bounds__(64) void k_wave(Problems P, LMParams lp, int it0, int iters) {
+  // wave issue priority 1 (trajectory waves stay at 0): where IK and trajectory launches share a
+  // SIMD, as in the request server, the short IK solves issue first
+  __builtin_amdgcn_s_setprio(1);
   __shared__ __align__(16) char lds[wave_lds_bytes<NP, NQ, HW>()];
   wave_solve<NP, NQ, HW, SN>(P, blockIdx.x, lp, it0, iters, lds);
 }
