@@ -54,6 +54,15 @@ constexpr int RK_XCD = 8;
 // ints per variant in a group's queue counters: the tile queue and the exit count
 constexpr int RK_QSTRIDE = 2;
 
+// Workgroup b of n renumbered so that the workgroups of one XCD (the same b mod RK_XCD) take
+// consecutive numbers, a bijection of [0, n): neighbouring problems, which share cache lines of
+// the problem-major arrays, then meet in one XCD's L2 instead of being fetched by every XCD.
+__device__ __forceinline__ int xcd_order(int b, int n) {
+  int t = b / RK_XCD;  // rank among the workgroups of b's class, after the classes before it
+  for (int c = 0; c < b % RK_XCD; ++c) t += (n - c + RK_XCD - 1) / RK_XCD;
+  return t;
+}
+
 // A resident scene geometry as the kernels see it (copied by value into the term tables; the
 // data stays in the context).  Volume: grid values [nz][ny][nx] at org + (i, j, k) h, sampled
 // trilinearly inside [lo, hi], box distance + pad outside.  Mesh: nt triangles of RK_TRI_FLOATS
@@ -966,7 +975,7 @@ __global__ __launch_bounds__(64) void k_fused(Problems P, LMParams lp, int it0, 
   // SIMD, as in the request server, the short IK solves issue first
   __builtin_amdgcn_s_setprio(1);
   extern __shared__ __align__(16) char lds[];
-  solve_tile<G, C>(P, blockIdx.x * (WAVE / G), lp, it0, iters, lds);
+  solve_tile<G, C>(P, xcd_order(blockIdx.x, gridDim.x) * (WAVE / G), lp, it0, iters, lds);
 }
 
 #include "rk_scene.h"
@@ -1081,8 +1090,7 @@ __host__ __device__ constexpr int variant_waves_per_eu(int v) {
 template <int V>
 __device__ __forceinline__ void persist_loop(const Bucket* buckets, int nb, int total_tiles, int* counter, int nblocks,
                                              int r, const LMParams& lp, int it0, int iters, char* lds, int& tile_s) {
-  int tile = r / RK_XCD;  // the first tile: rank among the workgroups of r's XCD class ...
-  for (int c = 0; c < r % RK_XCD; ++c) tile += (nblocks - c + RK_XCD - 1) / RK_XCD;  // ... after the classes before
+  int tile = xcd_order(r, nblocks);  // the first tile
   for (;;) {
     if (tile >= total_tiles) break;
     int b = 0;

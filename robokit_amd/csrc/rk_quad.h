@@ -309,5 +309,5 @@ __global__ __launch_bounds__(64) void k_quad(Problems P, LMParams lp, int it0, i
   // wave issue priority 1 (trajectory waves stay at 0): where IK and trajectory launches share a
   // SIMD, as in the request server, the short IK solves issue first
   __builtin_amdgcn_s_setprio(1);
-  quad_solve<NA, NJ>(P, blockIdx.x, lp, it0, iters);
+  quad_solve<NA, NJ>(P, xcd_order(blockIdx.x, gridDim.x), lp, it0, iters);
 }

@@ -689,5 +689,5 @@ __global__ __launch_bounds__(64) void k_thread(Problems P, LMParams lp, int it0,
   // wave issue priority 1 (trajectory waves stay at 0): where IK and trajectory launches share a
   // SIMD, as in the request server, the short IK solves issue first
   __builtin_amdgcn_s_setprio(1);
-  thread_solve<NA, NJ, PR, CO>(P, blockIdx.x * 64 + threadIdx.x, lp, it0, iters);
+  thread_solve<NA, NJ, PR, CO>(P, xcd_order(blockIdx.x, gridDim.x) * 64 + threadIdx.x, lp, it0, iters);
 }

@@ -913,11 +913,12 @@ template <int NA, int NJ, bool LIM = false, bool BC = true>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, BC ? 1 : 10))) void k_traj(
     Problems P, const TrajTerms* tt, LMParams lp, int it0, int iters) {
   __shared__ __align__(16) TrajLDS<NA> sh;
-  if ((int)blockIdx.x >= P.n) return;
+  const int s = xcd_order(blockIdx.x, gridDim.x);
+  if (s >= P.n) return;
   if (threadIdx.x < 64)
-    traj_solve<NA, NJ, LIM, BC>(P, as_const(tt), blockIdx.x, lp, it0, iters, &sh);
+    traj_solve<NA, NJ, LIM, BC>(P, as_const(tt), s, lp, it0, iters, &sh);
   else
-    traj_worker<NA, NJ, LIM, BC>(P, as_const(tt), blockIdx.x, it0, iters, &sh);
+    traj_worker<NA, NJ, LIM, BC>(P, as_const(tt), s, it0, iters, &sh);
 }
 
 // Trajectory batches of one arm class (dofs, bound rows / geometries) in a group: consecutive
@@ -937,15 +938,16 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                                                                                                int it0, int iters) {
   __shared__ __align__(16) TrajLDS<NA> sh;
   TrajBucketK* table = (TrajBucketK*)table_;
+  const int bx = xcd_order(blockIdx.x, gridDim.x);
   int k = 0;
-  while (k + 1 < nb && (int)blockIdx.x >= table[k + 1].block0) ++k;  // wave-uniform
+  while (k + 1 < nb && bx >= table[k + 1].block0) ++k;  // wave-uniform
   const __attribute__((address_space(4))) Problems& Q = table[k].P;  // read field by field (scalar loads)
   Problems P;
   P.robot = Q.robot; P.terms = Q.terms; P.q = Q.q; P.q_seed = Q.q_seed; P.pose_tgt = Q.pose_tgt;
   P.pos_tgt = Q.pos_tgt; P.rot_tgt = Q.rot_tgt; P.q_prev = Q.q_prev; P.cost = Q.cost; P.lam = Q.lam;
   P.nacc = Q.nacc; P.trace = Q.trace; P.acc_trace = Q.acc_trace; P.q_trace = Q.q_trace; P.lam_trace = Q.lam_trace;
   P.trace_iters = Q.trace_iters; P.n = Q.n; P.stride = Q.stride;
-  const int s = (int)blockIdx.x - table[k].block0;
+  const int s = bx - table[k].block0;
   if (s >= P.n) return;
   TrajK* tt = as_const(table[k].tt);
   if (threadIdx.x < 64)

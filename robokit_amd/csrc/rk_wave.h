@@ -1041,5 +1041,5 @@ __global__ __launch_bounds__(64) void k_wave(Problems P, LMParams lp, int it0, i
   // SIMD, as in the request server, the short IK solves issue first
   __builtin_amdgcn_s_setprio(1);
   __shared__ __align__(16) char lds[wave_lds_bytes<NP, NQ, HW>()];
-  wave_solve<NP, NQ, HW, SN>(P, blockIdx.x, lp, it0, iters, lds);
+  wave_solve<NP, NQ, HW, SN>(P, xcd_order(blockIdx.x, gridDim.x), lp, it0, iters, lds);
 }
