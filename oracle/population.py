@@ -10,6 +10,7 @@ product package (robokit_amd) never imports it.
                        helpers/ik/config.py:37-45: each segment a fresh LM (oracle.lm_solve) on
                        the kept candidates, then a select where the width narrows (an equal
                        width keeps the population, :436-443); the last select keeps one winner
+                       (keep_final: the final population, best first with rank_final, :384-404)
 
 Pins (tests/test_population.py): the R_d closed forms (golden ratio for d = 1, plastic number for
 d = 2), the reference test's coverage property (tests/opt/test_sampler.py:57-61), the ranking
@@ -48,14 +49,18 @@ def select_best_k(costs: np.ndarray, width_in: int, width_out: int) -> np.ndarra
 
 def population_ik(spec, links: Sequence[int], targets: np.ndarray, seeds: np.ndarray,
                   stages: List[Tuple[int, int, O.LMParams]], dtype: str = "f32", terms_fn=None, threads: int = 1,
-                  score_fn=None):
+                  score_fn=None, keep_final: bool = False, rank_final: bool = False):
     """targets [n_inst, n_pose, 7]; seeds [width0, na] shared by every problem, or
     [n_inst * width0, na] per problem; stages [(iters, width, params)].  terms_fn(inst) gives the
     oracle terms of the candidate rows whose problems are ``inst`` (default: FrameTask on every
     link + PositionLimit, oracle.ik_terms).  score_fn(inst): score terms whose cost at the
     candidates ranks every select instead of the optimization cost (population_solver.py:415-434);
     the returned costs are then the winners' scores.  Returns (q [n_inst, na], cost [n_inst], rows of
-    every select)."""
+    every select).  keep_final: return the final stage's population instead of its winners
+    (PopulationSolver.final_var, population_solver.py:384-385), q [n_inst * width, na] in rows
+    i * width + j; rank_final: rank only that population by score_fn (earlier selects keep the
+    optimization cost, population_solver.py:185-186) and order it best first (ranked_indices,
+    :232-236, 392-393)."""
     if terms_fn is None:
         terms_fn = lambda inst: O.ik_terms(links, targets[inst], seeds.shape[1])  # noqa: E731
     n_inst, na = targets.shape[0], seeds.shape[1]
@@ -65,21 +70,30 @@ def population_ik(spec, links: Sequence[int], targets: np.ndarray, seeds: np.nda
     q = (np.tile(seeds, (n_inst, 1)) if seeds.shape[0] == w else np.asarray(seeds)).astype(rt)
     cost = None
     selects = []
-    def rank(q, cost, inst):
-        if score_fn is None:
+    def rank(q, cost, inst, final=True):
+        if score_fn is None or (rank_final and not final):
             return cost
         return O.lm_solve(spec, score_fn(inst), q, 0, O.LMParams(), dtype=dtype, traces=False,
                           threads=threads)["cost"].astype(rt)
 
     for k, (iters, width, params) in enumerate(stages):
         if k > 0 and width < w:  # an equal width keeps the population (population_solver.py:442-443)
-            sc = rank(q, cost, inst)
+            sc = rank(q, cost, inst, final=False)
             rows = select_best_k(sc, w, width).reshape(-1)
             selects.append(rows)
             q, cost, inst, w = q[rows], sc[rows], inst[rows], width
         if iters > 0 or k == 0:
             res = O.lm_solve(spec, terms_fn(inst), q, iters, params, dtype=dtype, traces=False, threads=threads)
             q, cost = res["q"].astype(rt), res["cost"].astype(rt)
+    if keep_final:
+        if rank_final and w > 1:
+            sc = rank(q, cost, inst)
+            rows = select_best_k(sc, w, w).reshape(-1)
+            selects.append(rows)
+            q, cost, inst = q[rows], sc[rows], inst[rows]
+        elif score_fn is not None:
+            cost = rank(q, cost, inst)
+        return q, cost, selects
     if w > 1:
         sc = rank(q, cost, inst)
         rows = select_best_k(sc, w, 1).reshape(-1)

@@ -23,8 +23,9 @@ IKContext.SCENE (IKSolver(scene=...)) as the collision scene.  A config without 
 FrameTask(log) on every target link (``position_weight`` / ``orientation_weight``) plus
 PositionLimit(``limit_weight``).  Every solve, score and select runs on the GPU through
 librkhip.so.  Score terms (IKSolverConfig.add_score) rank the candidates at every select and
-the previous state by their own cost.  The floating base and several solutions per target are not
-implemented (NotImplementedError).
+the previous state by their own cost.  A final stage of several candidates returns all of them
+per target, instance-major (solver.rank_final: best first).  The floating base is not implemented
+(NotImplementedError).
 """
 from __future__ import annotations
 
@@ -185,8 +186,6 @@ class IKSolver:
             raise ValueError(f"target link out of range (robot has {spec.num_links} links)")
         self.ctx = ctx or N.Context(0)
         self._plan_terms()
-        if config.solver.num_candidates[-1] != 1:
-            raise NotImplementedError("a final stage of more than one solution per target is not implemented")
         self._pop: Optional[PopulationIK] = None
         self._score: Optional[N.Batch] = None
         self._metrics: Optional[N.Batch] = None
@@ -298,14 +297,24 @@ class IKSolver:
         elif batch_size != self._pop.batch_size:
             raise ValueError(f"Batch size mismatch: expected {self._pop.batch_size}, got {batch_size}")
 
+    @property
+    def num_solutions(self) -> int:
+        """solutions returned per target: the final stage's candidates (helpers/ik/ik.py:175-178)"""
+        return self.config.solver.num_candidates[-1]
+
     def solve_numpy(self, T_world_target, prev_q: Optional[np.ndarray] = None, init_q: Optional[np.ndarray] = None,
                     init_sample_range: Optional[float] = None, rest_q: Optional[np.ndarray] = None) -> np.ndarray:
-        """q [batch, na] of each target's winner.  rest_q [batch, na] or [na]: the rest configuration
+        """q [batch * num_solutions, na]: each target's final solutions, instance-major (rows
+        i * num_solutions + j; with solver.rank_final best first), the winner when num_solutions
+        is 1 (helpers/ik/ik.py:233-304).  rest_q [batch, na] or [na]: the rest configuration
         IKContext.REST_Q stands for (default: the robot's zero configuration, helpers/ik/ik.py:576)"""
         tg = self._targets(T_world_target)
         B = tg.shape[0]
         self._build(B)
         na = self.spec.num_actuated_joints
+        ns = self.num_solutions
+        if prev_q is not None and ns != 1:
+            raise ValueError("prev_state requires an IK final stage with num_candidates=1.")
         if init_q is not None and np.asarray(init_q).reshape(-1, na).shape[0] != B:
             raise ValueError(f"init_q batch size mismatch: expected {B}")
         if prev_q is not None and np.asarray(prev_q).reshape(-1, na).shape[0] != B:
@@ -326,11 +335,11 @@ class IKSolver:
             score_tg = dict(self.term_targets(tg, self._score_sel), q_prev=ref if self._score_ref else None)
         self._pop.set_targets(**self.term_targets(tg), q_prev=ref if self.smooth_ref else None, init_q=init_q,
                               init_scale=scale, score_targets=score_tg)
-        win = self._pop.solve()
+        win = self._pop.solve(final_population=ns > 1)
         out = win.get()
         q, cost = out["q"], out["cost"]
-        if self.score_terms is not None:  # the winners' scores (also without a final select)
-            sb = self._pop.score_batch(1)
+        if self.score_terms is not None:  # the solutions' scores (also without a final select)
+            sb = self._pop.score_batch(ns)
             win.copy_q_to(sb)
             sb.solve(0, N.LMParams())
             cost = sb.get()["cost"]

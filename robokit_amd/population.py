@@ -66,6 +66,9 @@ class Stage:
 @dataclass
 class PopulationSolverConfig:
     stages: List[Stage] = field(default_factory=lambda: [Stage(iters=10)])
+    # score only the final population (with score terms; earlier selects keep the optimization
+    # cost) and return it best first (population_solver.py:88-89, 384-404)
+    rank_final: bool = False
 
     def __post_init__(self):
         self.stages = list(self.stages)
@@ -128,6 +131,7 @@ class PopulationIK:
         self.batch_size = int(batch_size)
         self.widths = self.config.num_candidates
         self._batches: Dict[int, N.Batch] = {}
+        self._ranked: Optional[N.Batch] = None  # the final population ranked (rank_final)
         self._score: Dict[int, N.Batch] = {}  # per width: the score terms' batch (score_terms only)
         for w in dict.fromkeys(self.widths + [1]):
             self._batches[w] = N.Batch(ctx, spec, terms, self.batch_size * w)
@@ -181,11 +185,13 @@ class PopulationIK:
                n=self.batch_size * w0, rot_targets=rep(rot_targets))
         self._ready = True
 
-    def solve(self, tpp=0) -> N.Batch:
-        """Run the schedule from the seeds; returns the width-1 batch of winners (get / metrics).
-        ``tpp``: the threads-per-problem code of every stage, or a sequence of one per stage.  The
-        launches and selects are only enqueued on the context's stream: reading the winners
-        (get / metrics) waits for them."""
+    def solve(self, tpp=0, final_population: bool = False) -> N.Batch:
+        """Run the schedule from the seeds; returns the width-1 batch of winners (get / metrics), or
+        with ``final_population`` the final stage's batch (width num_candidates[-1], rows
+        i * width + j; best first with config.rank_final -- PopulationSolver.final_var,
+        population_solver.py:384-404).  ``tpp``: the threads-per-problem code of every stage, or a
+        sequence of one per stage.  The launches and selects are only enqueued on the context's
+        stream: reading the winners (get / metrics) waits for them."""
         if not self._ready:
             raise RuntimeError("set_targets first")
         cfg, widths = self.config, self.widths
@@ -197,11 +203,18 @@ class PopulationIK:
             # either way the next segment restarts LM
             if k > 0 and widths[k] < w:
                 nxt = self._batches[widths[k]]
-                self._select(cur, w, nxt, widths[k])
+                self._select(cur, w, nxt, widths[k], scored=not cfg.rank_final)
                 cur, w = nxt, widths[k]
             if stage.iters > 0 or k == 0:  # a zero-iteration first stage scores the seeds
                 tk = tpp[k] if isinstance(tpp, (list, tuple)) else tpp
                 cur.solve(stage.iters, params[k], tpp=tk, from_seed=(k == 0), wait=False)
+        if final_population:
+            if cfg.rank_final and w > 1:  # the final population ranked best first (stable)
+                if self._ranked is None:
+                    self._ranked = N.Batch(self.ctx, self.spec, self.terms, self.batch_size * w)
+                self._select(cur, w, self._ranked, w)
+                cur = self._ranked
+            return cur
         if w > 1:  # one winner per problem
             win = self._batches[1]
             self._select(cur, w, win, 1)
@@ -220,10 +233,10 @@ class PopulationIK:
             self._score_set.add(w)
         return sb
 
-    def _select(self, cur: N.Batch, w: int, dst: N.Batch, w_out: int) -> None:
-        """keep each problem's best w_out of its w candidates, ranked by cost or, with score terms,
-        by the score terms' cost at the candidates' iterates"""
-        if self.score_terms is None:
+    def _select(self, cur: N.Batch, w: int, dst: N.Batch, w_out: int, scored: bool = True) -> None:
+        """keep each problem's best w_out of its w candidates, ranked by cost or, with score terms
+        (and scored), by the score terms' cost at the candidates' iterates"""
+        if self.score_terms is None or not scored:
             cur.select(w, dst, w_out)
             return
         sb = self.score_batch(w)
@@ -232,5 +245,6 @@ class PopulationIK:
         cur.select(w, dst, w_out, scores=sb)
 
     def close(self) -> None:
-        for b in list(self._batches.values()) + list(self._score.values()):
-            b.close()
+        for b in list(self._batches.values()) + list(self._score.values()) + [self._ranked]:
+            if b is not None:
+                b.close()

@@ -54,6 +54,45 @@ def test_solver_matches_oracle_population(ctx, panda):
         solver.solve_numpy(np.zeros((32, 6), np.float32))
 
 
+@pytest.mark.parametrize("rank_final", [False, True])
+def test_several_solutions_per_target(ctx, panda, rank_final):
+    """A final stage of 4 candidates (helpers/ik/ik.py:175-178, 233-304): solve_numpy returns 4
+    solutions per target, instance-major, the final population of the oracle's flow
+    (PopulationSolver.final_var, population_solver.py:384-404); with rank_final best first, the
+    first being the winner of the default schedule (the same stages and a final select of one).
+    The solutions classify like the FP32 oracle population's (per target, the converged counts
+    differ by no more in total than FP32 vs FP64), and a previous state needs one solution."""
+    spec, wl = panda
+    from robokit_amd import native as N
+    cfg = IK.IKSolverConfig(solver=P.PopulationSolverConfig(
+        stages=[P.Stage(6, 64, N.LMParams(lm_lambda=10.0)), P.Stage(10, 4, N.LMParams(lm_lambda=1.0))],
+        rank_final=rank_final))
+    solver = IK.IKSolver(cfg, spec, wl.links, ctx=ctx)
+    assert solver.num_solutions == 4
+    print(f"\n[ik] panda {wl.size} targets x 4 solutions, rank_final={rank_final}", flush=True)
+    q = solver.solve_numpy(wl.targets)
+    assert q.shape == (wl.size * 4, 7)
+    stages = _stages(cfg.solver)
+    q32, c32, _ = OP.population_ik(spec, wl.links, wl.targets, solver._pop.seeds, stages, keep_final=True,
+                                   rank_final=rank_final)
+    q64, _, _ = OP.population_ik(spec, wl.links, wl.targets, solver._pop.seeds, stages, dtype="f64",
+                                 keep_final=True, rank_final=rank_final)
+    rep = lambda a: np.repeat(a, 4, axis=0)  # noqa: E731
+    ok = O.ik_metrics(spec, q, wl.links, rep(wl.targets))[2].reshape(-1, 4).sum(1)
+    ok32 = O.ik_metrics(spec, q32, wl.links, rep(wl.targets))[2].reshape(-1, 4).sum(1)
+    ok64 = O.ik_metrics(spec, q64, wl.links, rep(wl.targets), dtype="f64")[2].reshape(-1, 4).sum(1)
+    mism, n_ref = int(np.abs(ok - ok32).sum()), int(np.abs(ok64 - ok32).sum())
+    print(f"  converged solutions gpu {ok.sum()} f32 {ok32.sum()} f64 {ok64.sum()}; differ gpu/f32 {mism}, "
+          f"f32/f64 {n_ref}")
+    assert mism <= max(n_ref, 1)
+    if rank_final:
+        assert np.all(np.diff(solver.last_cost.reshape(-1, 4), axis=1) >= 0)
+        win = IK.IKSolver(IK.IKSolverConfig(), spec, wl.links, ctx=ctx).solve_numpy(wl.targets)
+        np.testing.assert_array_equal(q.reshape(-1, 4, 7)[:, 0], win)
+    with pytest.raises(ValueError, match="num_candidates=1"):
+        solver.solve_numpy(wl.targets, prev_q=q[::4])
+
+
 def test_prev_state_kept_on_ties(ctx, panda):
     """select_better_score_kernel (helpers/ik/ik.py:43-57): prev_q wins where its cost <= the
     winner's; the solve itself is reproducible, so the other rows are the plain solve's winners"""

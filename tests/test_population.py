@@ -140,6 +140,25 @@ def test_population_oracle_select_consistency():
     assert np.all(np.isfinite(cost))
 
 
+def test_population_oracle_final_population():
+    """keep_final returns the final stage's population (rows i * width + j of problem i); with
+    rank_final it is ordered best first, its first row is the winner of the plain flow, and the
+    unranked population holds the same rows"""
+    spec = W.panda_dh()
+    wl = W.ik_batch(spec, [spec.num_links - 1], 3, seed=5)
+    seeds = _NoDevice(spec, P.default_ik_config()).candidate_seeds(1.0)[:8]
+    stages = [(3, 8, OP.O.LMParams(lm_lambda=10.0)), (4, 4, OP.O.LMParams(lm_lambda=1.0))]
+    q1, c1, _ = OP.population_ik(spec, wl.links, wl.targets, seeds, stages)
+    qa, ca, _ = OP.population_ik(spec, wl.links, wl.targets, seeds, stages, keep_final=True)
+    qr, cr, sel = OP.population_ik(spec, wl.links, wl.targets, seeds, stages, keep_final=True, rank_final=True)
+    assert qa.shape == (12, 7) and qr.shape == (12, 7) and cr.shape == (12,)
+    assert np.all(np.diff(cr.reshape(3, 4), axis=1) >= 0)
+    np.testing.assert_array_equal(qr.reshape(3, 4, 7)[:, 0], q1)
+    np.testing.assert_array_equal(cr.reshape(3, 4)[:, 0], c1)
+    assert [sorted(r) for r in (sel[-1].reshape(3, 4) // 4).tolist()] == [[0] * 4, [1] * 4, [2] * 4]
+    np.testing.assert_array_equal(qr, qa[sel[-1]])
+
+
 def test_select_abi_declared():
     assert "rk_batch_select" in N.EXPORTS
 
