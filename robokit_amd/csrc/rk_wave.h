@@ -600,13 +600,33 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
   // JAC also writes the pose target's Jacobian blocks / the position link's point to LDS
   // (and the lane's sums of squares sp / sq; with jac false -- the proposal -- nothing goes to
   // LDS.  Both paths sum the same rows in the same order, so a cost at the proposal and the cost
-  // recomputed at the accepted iterate agree bitwise)
-  auto target_rows = [&](const Pose& Tw, bool jac, float& sp, float& sq) {
+  // recomputed at the accepted iterate agree bitwise).  In the pair layout (PM) a pose target's
+  // log map is evaluated at the proposal only (into Lp): the iterate's (Lq) is the accepted
+  // proposal's, kept in registers, so a rebuild at the iterate takes its rows and blocks from Lq
+  // (the same bits as evaluating it again at the same pose).  The other layouts, whose register
+  // budgets are spent, evaluate it again.
+  auto target_frame = [&](const Pose& Tw) {
     const Pose Tl = wv_shfl_pose<HW>(Tw, tj < 0 ? 0 : tj);
-    const Pose Tf = tj < 0 ? identity_pose() : Tl;
+    return tj < 0 ? identity_pose() : Tl;
+  };
+  TargetLog Lq, Lp;
+  if constexpr (PM) {
+    const Pose Tf = target_frame(T);
+    if (lane < n_pose) target_log<true>(Tf, tgt, Lq);
+  }
+  auto target_rows = [&](const Pose& Tw, bool jac, float& sp, float& sq) {
+    const Pose Tf = target_frame(Tw);
     if (lane < n_pose) {
       float r6[6];
-      pose_target(Tf, tgt, pw, r6, jac ? sh->tg[lane < NP ? lane : 0] : nullptr);
+      if constexpr (PM) {
+        if (!jac) target_log<true>(Tf, tgt, Lp);
+        const TargetLog& L = jac ? Lq : Lp;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) r6[i] = pw[i] * L.e[i];
+        if (jac) target_blocks<true>(tgt, Tf, L, pw, sh->tg[lane < NP ? lane : 0]);
+      } else {
+        pose_target(Tf, tgt, pw, r6, jac ? sh->tg[lane < NP ? lane : 0] : nullptr);
+      }
       if (jac) {
 #pragma unroll
         for (int i = 0; i < 6; ++i) sh->rows[6 * lane + i] = r6[i];
@@ -1016,6 +1036,7 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
       ++nacc;
       q = qp;
       T = Tp;
+      if constexpr (PM) Lq = Lp;
     }
     if (valid && Pg.trace && gi < Pg.trace_iters && lane == 0) {
       Pg.trace[(gi + 1) * S + s] = cost;
