@@ -710,7 +710,10 @@ __device__ __forceinline__ void wave_solve(const Problems& Pg, int tile, const L
       }
     }
     __syncthreads();
-    cost = term_cost(sp, sq, rl * rl, rs * rs);
+    // the cost at the iterate: after the launch's first rebuild it is already known -- an accepted
+    // proposal's cost (lm_accept stores it, summed from the same rows in the same order), or the
+    // unchanged cost of a problem that did not move -- so only the first evaluates it
+    if (it == 0) cost = term_cost(sp, sq, rl * rl, rs * rs);
     if constexpr (PM) {
       // ---- this pair's Jacobian column (its target's six rows, -w jlog Ad(T_f^-1) S) and its
       // share of J^T r; J^T J of each block's targets on the matrix cores: per target slot h of
