@@ -600,6 +600,46 @@ __device__ __forceinline__ void pose_jac_column(const float* tg, const float* sc
   }
 }
 
+// The same map as one 6 x 6 matrix for a kernel that forms many columns of one target in one
+// lane (the serial-arm thread kernel, a column per dof): with AR = A R and the rows scaled by -w,
+//   top = [-w A R | -w (A [t]x R + B R)] sc,   bot = [0 | -w A R] sc_w
+// M holds the three nonzero 3 x 3 blocks (Mv, Mw, Mb), built once (about 135 operations), each
+// column then 27 FMAs instead of about 60.  The rounding differs from pose_jac_column's.
+__device__ __forceinline__ void pose_jac_matrix(const float* tg, float (&M)[27]) {
+  const float* R = tg;
+  const float* t = tg + 9;
+  const float* A = tg + 12;
+  const float* B = tg + 21;
+  const float* w = tg + 30;
+  float AR[9], tR[9];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {  // column k of [t]x R: t x (column k of R)
+    const float rk[3] = {R[k], R[3 + k], R[6 + k]};
+    float c[3];
+    cross3(t, rk, c);
+    tR[k] = c[0]; tR[3 + k] = c[1]; tR[6 + k] = c[2];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      AR[i * 3 + k] = A[i * 3 + 0] * R[k] + A[i * 3 + 1] * R[3 + k] + A[i * 3 + 2] * R[6 + k];
+      const float mw = A[i * 3 + 0] * tR[k] + A[i * 3 + 1] * tR[3 + k] + A[i * 3 + 2] * tR[6 + k] +
+                       B[i * 3 + 0] * R[k] + B[i * 3 + 1] * R[3 + k] + B[i * 3 + 2] * R[6 + k];
+      M[i * 3 + k] = -w[i] * AR[i * 3 + k];
+      M[9 + i * 3 + k] = -w[i] * mw;
+      M[18 + i * 3 + k] = -w[3 + i] * AR[i * 3 + k];
+    }
+}
+__device__ __forceinline__ void pose_jac_apply(const float (&M)[27], const float* sc, float* out6) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    out6[i] = M[i * 3 + 0] * sc[0] + M[i * 3 + 1] * sc[1] + M[i * 3 + 2] * sc[2] + M[9 + i * 3 + 0] * sc[3] +
+              M[9 + i * 3 + 1] * sc[4] + M[9 + i * 3 + 2] * sc[5];
+    out6[3 + i] = M[18 + i * 3 + 0] * sc[3] + M[18 + i * 3 + 1] * sc[4] + M[18 + i * 3 + 2] * sc[5];
+  }
+}
+
 // position-row Jacobian column: -w * (v + w x p)  (position_task.py:253-262)
 __device__ __forceinline__ void pos_jac_column(const float* sc, const float* p, const float* w, float* out3) {
   float wxp[3];

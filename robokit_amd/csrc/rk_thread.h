@@ -237,12 +237,14 @@ __device__ __forceinline__ float chain_linearize(RobotK* rb, TermsK* tm, int nj,
   float r_pose[6], r_lim[NA], r_sm[NA], jl[NA], js[NA], tg[36];
   chain_rows<NA, true, PR>(rb, tm, Tf, tgt, qv, qprev, r_pose, r_lim, r_sm, tg, jl, js);
   float J[6][NA];
+  float M[27];  // the target's column map, once for all the dofs' columns
+  if constexpr (!PR) pose_jac_matrix(tg, M);
 #pragma unroll
   for (int a = 0; a < NA; ++a) {
     float c6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (a <= jt) {
       if constexpr (!PR) {
-        if (n_pose) pose_jac_column(tg, S6[a], c6);
+        if (n_pose) pose_jac_apply(M, S6[a], c6);
       } else {
         if (n_pos) {
           const float w3[3] = {tm->pos_w[0][0], tm->pos_w[0][1], tm->pos_w[0][2]};
