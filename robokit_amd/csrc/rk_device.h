@@ -41,6 +41,7 @@ struct Robot {
   float off[RK_NJ_MAX];
   float axis[RK_NJ_MAX][3];
   float pT[RK_NJ_MAX][7];      // parent joint transforms
+  float pqa[RK_NJ_MAX][4];     // pT's rotation times the pure axis quaternion: q_P (0, axis)
   int link_joint[RK_NL_MAX];   // link -> parent joint (-1 = base)
   float lo[RK_NA_MAX], hi[RK_NA_MAX];  // float32-narrowed actuated limits
   int order[RK_NJ_MAX];        // joints sorted by depth (identity: joints are renumbered in depth order)

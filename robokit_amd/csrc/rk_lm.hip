@@ -2436,6 +2436,14 @@ int rk_robot_upload(rk_context* ctx, const rk_robot_desc* d, int32_t* robot_id) 
     r.off[n] = d->mimic_off[j];
     for (int k = 0; k < 3; ++k) r.axis[n][k] = d->joint_axis[3 * j + k];
     for (int k = 0; k < 7; ++k) r.pT[n][k] = d->parent_T[7 * j + k];
+    {  // q_P (0, a), not standardised: it is one half of a linear combination, not a rotation
+      const float* p = &r.pT[n][3];
+      const float* a = r.axis[n];
+      r.pqa[n][0] = -(p[1] * a[0] + p[2] * a[1] + p[3] * a[2]);
+      r.pqa[n][1] = p[0] * a[0] + p[2] * a[2] - p[3] * a[1];
+      r.pqa[n][2] = p[0] * a[1] - p[1] * a[2] + p[3] * a[0];
+      r.pqa[n][3] = p[0] * a[2] + p[1] * a[1] - p[2] * a[0];
+    }
     r.order[n] = n;
   }
   for (int l = 0; l < d->nl; ++l) r.link_joint[l] = d->link_parent_joint[l] < 0 ? -1 : old2new[d->link_parent_joint[l]];

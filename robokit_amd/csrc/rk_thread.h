@@ -48,16 +48,21 @@ __device__ __forceinline__ Pose chain_fk(RobotK* rb, int nj, const float* q, int
 #pragma unroll
       for (int k = 0; k < 4; ++k) L.q[k] = rb->pT[j][3 + k];
       if (j < NA) {
-        Pose X = identity_pose();
-        const float ax0 = rb->axis[j][0], ax1 = rb->axis[j][1], ax2 = rb->axis[j][2];
         if (rb->joint_type[j] == 2) {
+          // P o M(q): the translation is P's, the rotation q_P (c, s axis) = c q_P + s q_P (0, axis),
+          // standardised to w >= 0 as quat_mul's products are
           float s, c;
           rk_sincos(q[j] * 0.5f, &s, &c);
-          X.q[0] = c; X.q[1] = ax0 * s; X.q[2] = ax1 * s; X.q[3] = ax2 * s;
+          float w = fmaf(s, rb->pqa[j][0], c * L.q[0]);
+          const float sg = w < 0.f ? -1.f : 1.f;
+          L.q[0] = sg * w;
+#pragma unroll
+          for (int k = 1; k < 4; ++k) L.q[k] = sg * fmaf(s, rb->pqa[j][k], c * L.q[k]);
         } else {
-          X.t[0] = ax0 * q[j]; X.t[1] = ax1 * q[j]; X.t[2] = ax2 * q[j];
+          Pose X = identity_pose();
+          X.t[0] = rb->axis[j][0] * q[j]; X.t[1] = rb->axis[j][1] * q[j]; X.t[2] = rb->axis[j][2] * q[j];
+          L = compose(L, X);
         }
-        L = compose(L, X);
       }
       T = j == 0 ? L : compose(T, L);
       if (WITH_S && j < NA) subspace_column(rb, j, T, S6[j]);
